@@ -1,0 +1,174 @@
+"""Seeded synthetic workloads for the alignment hot path.
+
+Sequences use the reference's int8 codes (A0 C1 N2 G3 T4,
+impl/SequenceCharToUInt8.cpp:10-30).  Every generator takes an explicit seed
+so that tests, the benchmark and the CPU baseline see identical inputs.
+
+Workloads (sizes as in the benchmark definitions):
+  * ``one_pair``       10 kb x 10 kb, 20 planted fragments (30-200 bp,
+                       15% substitutions, 2% 1-5 bp indels).
+  * ``batch``          4,096 pairs, lengths log-uniform on 1-50 kb, GC
+                       U[0.40, 0.60], 0-30 fragments of 20-300 bp at 10-25%
+                       substitutions + 1-3% indels.
+  * ``long_masked``    500 kb x 500 kb at GC 0.45 with 30% lowercase-n runs,
+                       a 3 kb repeat family (200 copies, 10% divergence) and
+                       300 planted fragments.
+  * ``random_pairs``   independent random fragment pairs (lengths log-uniform
+                       on 38-2,000 bp, GC U[0.30, 0.70]) for the lambda/K
+                       score distribution.
+"""
+import numpy as np
+
+CODE = {"A": 0, "C": 1, "N": 2, "G": 3, "T": 4}
+LETTERS = np.frombuffer(b"ACNGT", dtype=np.uint8)
+BASES = np.array([0, 1, 3, 4], dtype=np.int8)
+SEED_BASE = 20261203
+
+
+def random_seq(rng, n, gc=0.5):
+    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    return BASES[rng.choice(4, size=n, p=p)]
+
+
+def mutate(rng, frag, p_sub, p_indel, gc=0.5):
+    """Substitutions (to a different base) and 1-5 bp indels."""
+    out = []
+    i = 0
+    n = len(frag)
+    while i < n:
+        r = rng.random()
+        if r < p_indel / 2:  # deletion of 1-5 bp
+            i += int(rng.integers(1, 6))
+            continue
+        if r < p_indel:  # insertion of 1-5 bp
+            out.extend(random_seq(rng, int(rng.integers(1, 6)), gc).tolist())
+        b = int(frag[i])
+        if b != 2 and rng.random() < p_sub:
+            choices = [x for x in (0, 1, 3, 4) if x != b]
+            b = choices[int(rng.integers(0, 3))]
+        out.append(b)
+        i += 1
+    return np.array(out, dtype=np.int8)
+
+
+def plant(rng, s1, s2, nfrag, lo, hi, p_sub, p_indel, gc=0.5):
+    """Writes nfrag shared fragments into s1 and (mutated) into s2, in place."""
+    for _ in range(nfrag):
+        ln = int(rng.integers(lo, hi + 1))
+        if ln >= len(s1) or ln >= len(s2):
+            continue
+        frag = random_seq(rng, ln, gc)
+        p1 = int(rng.integers(0, len(s1) - ln + 1))
+        s1[p1:p1 + ln] = frag
+        m = mutate(rng, frag, p_sub, p_indel, gc)[: len(s2)]
+        p2 = int(rng.integers(0, len(s2) - len(m) + 1))
+        s2[p2:p2 + len(m)] = m
+
+
+def one_pair(seed=SEED_BASE + 0, L1=10000, L2=10000, nfrag=20):
+    rng = np.random.default_rng(seed)
+    s1 = random_seq(rng, L1)
+    s2 = random_seq(rng, L2)
+    plant(rng, s1, s2, nfrag, 30, 200, 0.15, 0.02)
+    return s1, s2
+
+
+def log_uniform_int(rng, lo, hi, size=None):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size=size)).astype(np.int64)
+
+
+def batch(seed=SEED_BASE + 1, npairs=4096, lo=1000, hi=50000, max_frag=30):
+    """List of (s1, s2) pairs."""
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for _ in range(npairs):
+        L1 = int(log_uniform_int(rng, lo, hi))
+        L2 = int(log_uniform_int(rng, lo, hi))
+        gc = float(rng.uniform(0.40, 0.60))
+        s1 = random_seq(rng, L1, gc)
+        s2 = random_seq(rng, L2, gc)
+        nfrag = int(rng.integers(0, max_frag + 1))
+        plant(rng, s1, s2, nfrag, 20, 300, float(rng.uniform(0.10, 0.25)), float(rng.uniform(0.01, 0.03)), gc)
+        pairs.append((s1, s2))
+    return pairs
+
+
+def long_masked(seed=SEED_BASE + 2, L=500000, gc=0.45, masked_frac=0.30, te_len=3000, te_copies=200,
+                nfrag=300):
+    """Returns (text1, text2): ASCII sequences with lowercase 'n' runs, as the
+    masked genome regions that cut1Gap strips (controlLayer4.cpp:1385-1390)."""
+    rng = np.random.default_rng(seed)
+    te = random_seq(rng, te_len, gc)
+    out = []
+    for _ in range(2):
+        s = random_seq(rng, L, gc)
+        for _ in range(te_copies):
+            c = mutate(rng, te, 0.10, 0.0, gc)[:te_len]
+            p = int(rng.integers(0, L - len(c)))
+            s[p:p + len(c)] = c
+        out.append(s)
+    s1, s2 = out
+    # planted fragments separated by 25-2000 bp gaps
+    p1 = int(rng.integers(0, 1000))
+    p2 = int(rng.integers(0, 1000))
+    for _ in range(nfrag):
+        ln = int(rng.integers(20, 301))
+        if p1 + ln >= L or p2 + ln >= L:
+            break
+        frag = random_seq(rng, ln, gc)
+        s1[p1:p1 + ln] = frag
+        m = mutate(rng, frag, 0.15, 0.0, gc)
+        s2[p2:p2 + len(m)] = m[: max(0, L - p2)]
+        p1 += ln + int(rng.integers(25, 2001))
+        p2 += len(m) + int(rng.integers(25, 2001))
+    texts = []
+    for s in (s1, s2):
+        t = LETTERS[s.astype(np.int64)].copy()
+        covered = 0
+        target = int(masked_frac * L)
+        while covered < target:
+            ln = int(log_uniform_int(rng, 20, 5000))
+            p = int(rng.integers(0, L - ln))
+            t[p:p + ln] = ord("n")
+            covered += ln
+        texts.append(t.tobytes().decode())
+    return texts[0], texts[1]
+
+
+def random_pairs(seed=SEED_BASE + 3, n=1000000, lo=38, hi=2000, gc_lo=0.30, gc_hi=0.70):
+    """Concatenated random fragment pairs: (seqs, ref_off, ref_len, qry_off, qry_len)."""
+    rng = np.random.default_rng(seed)
+    l1 = log_uniform_int(rng, lo, hi, n).astype(np.int32)
+    l2 = log_uniform_int(rng, lo, hi, n).astype(np.int32)
+    gc = rng.uniform(gc_lo, gc_hi, n)
+    tot = int(l1.sum() + l2.sum())
+    # per-pair GC: draw uniforms once and threshold against the pair's composition
+    u = rng.random(tot)
+    gcs = np.repeat(np.stack([gc, gc], 1).ravel(), np.stack([l1, l2], 1).ravel())
+    seqs = np.where(u < gcs / 2, 1, np.where(u < gcs, 3, np.where(u < gcs + (1 - gcs) / 2, 0, 4))).astype(np.int8)
+    lens = np.stack([l1, l2], 1).ravel().astype(np.int64)
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    return seqs, offs[0::2].copy(), l1, offs[1::2].copy(), l2
+
+
+def scoring_settings(seed=SEED_BASE + 3, n=6):
+    """n (match, mismatch, open, extend) settings from the sweep ranges."""
+    rng = np.random.default_rng(seed + 1000)
+    out = []
+    for _ in range(n):
+        out.append((int(rng.integers(1, 4)), -int(rng.integers(1, 5)), -int(rng.integers(2, 7)),
+                    -int(rng.integers(1, 4))))
+    return out
+
+
+def to_text(codes):
+    return LETTERS[np.asarray(codes, dtype=np.int64)].tobytes().decode()
+
+
+def encode(text):
+    """seq_to_int8 (SequenceCharToUInt8.cpp:46-52): acgtu/ACGTU -> codes, else N."""
+    lut = np.full(256, 2, dtype=np.int8)
+    for ch, v in (("A", 0), ("C", 1), ("G", 3), ("T", 4), ("U", 4)):
+        lut[ord(ch)] = v
+        lut[ord(ch.lower())] = v
+    return lut[np.frombuffer(text.encode(), dtype=np.uint8)]

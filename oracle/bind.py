@@ -1,0 +1,154 @@
+"""ctypes bindings for the checker libraries.
+
+TEST INFRASTRUCTURE ONLY.  ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module; the product package
+(``dcns_amd``) never does.
+
+Two libraries share one calling convention:
+  * ``oracle/liboracle.so``      -- this repo's CPU restatement (``orc_*``)
+  * ``oracle/_ref/libdcnsref.so`` -- the reference's own sources compiled by
+    ``oracle/Makefile`` (``ref_*``); present only where it has been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libdcnsref.so")
+REF_CLI = os.path.join(HERE, "_ref", "dCNS_ref")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_D = ctypes.c_double
+_L = ctypes.c_int64
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class Checker:
+    """One of the two CPU libraries, with numpy-friendly wrappers."""
+
+    def __init__(self, path, prefix):
+        self.path = path
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        f = self._f
+        f("sw").argtypes = [_P, _P, _I, _I, _I, _I, _I, _I, _P]
+        f("sw_textbook").argtypes = [_P, _P, _I, _I, _I, _I, _I, _I, _P]
+        f("xextend").argtypes = [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P]
+        f("xextend_tb").argtypes = [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _I]
+        f("semiglobal").argtypes = [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _I]
+        f("seed_set").argtypes = [_P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _I]
+        if prefix == "orc":
+            f("find_similar").argtypes = [_P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _D, _D, _D,
+                                          _P, _P, _I, _P, _P, _L, _P]
+            f("sw_max_batch").argtypes = [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _I]
+            f("sw_uses_simd_variant").argtypes = [_I, _I, _I, _I, _I]
+        else:
+            f("find_similar").argtypes = [_P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _D, _D, _D,
+                                          _P, _P, _I, _P, _P, _L]
+
+    def _f(self, name):
+        return getattr(self.lib, f"{self.prefix}_{name}")
+
+    @staticmethod
+    def _codes(a):
+        return np.ascontiguousarray(a, dtype=np.int8)
+
+    def sw(self, ref, read, sc=(2, -3, -4, -2), textbook=False):
+        ref, read = self._codes(ref), self._codes(read)
+        out = np.zeros(3, np.int32)
+        fn = self._f("sw_textbook" if textbook else "sw")
+        fn(_ptr(ref), _ptr(read), len(ref), len(read), *sc, out.ctypes.data)
+        return tuple(int(x) for x in out)
+
+    def xextend(self, s1, s2, sc=(2, -3, -4, -2), xdrop=20, w=10):
+        s1, s2 = self._codes(s1), self._codes(s2)
+        out = np.zeros(3, np.int32)
+        self._f("xextend")(_ptr(s1), _ptr(s2), len(s1), len(s2), *sc, xdrop, w, out.ctypes.data)
+        return tuple(int(x) for x in out)
+
+    def xextend_tb(self, s1, s2, sc=(2, -3, -4, -2), xdrop=20, w=10):
+        s1, s2 = self._codes(s1), self._codes(s2)
+        out = np.zeros(5, np.int32)
+        cap = 2 * (len(s1) + len(s2)) + 8
+        cig = np.zeros(cap, np.uint32)
+        n = self._f("xextend_tb")(_ptr(s1), _ptr(s2), len(s1), len(s2), *sc, xdrop, w, out.ctypes.data,
+                                  cig.ctypes.data, cap)
+        assert n >= 0
+        return tuple(int(x) for x in out), [int(c) for c in cig[:n]]
+
+    def semiglobal(self, s1, s2, sc=(2, -3, -4, -2, -45, 0), zdrop=50, w=20000, cigar=True):
+        s1, s2 = self._codes(s1), self._codes(s2)
+        out = np.zeros(3, np.int32)
+        cap = 2 * (len(s1) + len(s2)) + 8
+        cig = np.zeros(cap, np.uint32)
+        n = self._f("semiglobal")(_ptr(s1), _ptr(s2), len(s1), len(s2), *sc, zdrop, w, int(cigar),
+                                  out.ctypes.data, cig.ctypes.data, cap)
+        assert n >= 0
+        return tuple(int(x) for x in out), [int(c) for c in cig[:n]]
+
+    def seed_set(self, s1, s2, sc=(2, -3, -4, -2), window=38, step=8, min_score=40, linked=False):
+        s1, s2 = self._codes(s1), self._codes(s2)
+        cap = 1 << 16
+        out = np.zeros(4 * cap, np.int32)
+        n = self._f("seed_set")(_ptr(s1), len(s1), _ptr(s2), len(s2), *sc, window, step, min_score,
+                                int(linked), out.ctypes.data, cap)
+        assert n >= 0
+        return [tuple(int(v) for v in out[4 * i:4 * i + 4]) for i in range(n)]
+
+    def find_similar(self, s1, s2, sc=(2, -3, -4, -2), window=38, step=8, min_score=40, w=10, xdrop=20,
+                     pvalues=0.1, lam=0.382291, k=0.006662, stats=None):
+        """Returns a list of (start1, end1, start2, end2, score, pvalue, evalue, cigar)."""
+        s1, s2 = self._codes(s1), self._codes(s2)
+        cap = 4096
+        cig_cap = 1 << 20
+        ints = np.zeros(5 * cap, np.int32)
+        dbls = np.zeros(2 * cap, np.float64)
+        cig = np.zeros(cig_cap, np.uint32)
+        off = np.zeros(cap + 1, np.int64)
+        args = [_ptr(s1), len(s1), _ptr(s2), len(s2), *sc, window, step, min_score, w, xdrop, pvalues, lam, k,
+                ints.ctypes.data, dbls.ctypes.data, cap, cig.ctypes.data, off.ctypes.data, cig_cap]
+        if self.prefix == "orc":
+            st = np.zeros(4, np.int64)
+            args.append(st.ctypes.data)
+        n = self._f("find_similar")(*args)
+        assert n >= 0, "capacity"
+        if stats is not None and self.prefix == "orc":
+            stats.update(scan_cells=int(st[0]), xdrop_cells=int(st[1]), seeds=int(st[2]), linked=int(st[3]))
+        res = []
+        for i in range(n):
+            a = [int(v) for v in ints[5 * i:5 * i + 5]]
+            res.append((*a, float(dbls[2 * i]), float(dbls[2 * i + 1]),
+                        [int(c) for c in cig[off[i]:off[i + 1]]]))
+        return res
+
+
+def oracle():
+    """The CPU restatement (always available once ``make -C oracle oracle`` ran)."""
+    return Checker(ORACLE_SO, "orc")
+
+
+def reference():
+    """The compiled reference, or None where oracle/_ref has not been built."""
+    if not os.path.exists(REF_SO):
+        return None
+    return Checker(REF_SO, "ref")
+
+
+def sw_max_batch(seqs, ref_off, ref_len, qry_off, qry_len, sc=(2, -3, -4, -2), nthreads=1):
+    """Oracle max local score for a batch of pairs (the ranSco inner call)."""
+    o = oracle()
+    f = o.lib.orc_sw_max_batch
+    f.argtypes = [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _I]
+    n = len(ref_len)
+    out = np.zeros(n, np.int32)
+    arrs = [np.ascontiguousarray(seqs, np.int8), np.ascontiguousarray(ref_off, np.int64),
+            np.ascontiguousarray(ref_len, np.int32), np.ascontiguousarray(qry_off, np.int64),
+            np.ascontiguousarray(qry_len, np.int32)]
+    f(*[a.ctypes.data for a in arrs], n, *sc, out.ctypes.data, nthreads)
+    return out

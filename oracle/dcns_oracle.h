@@ -1,0 +1,79 @@
+// dcns_oracle.h -- CPU restatement of dCNS's alignment hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+// entry point and the CPU-baseline leg of bench.py may load it; the product
+// library (dcns_amd) never links, loads or calls it.
+//
+// Every function restates one reference routine (file:line in the
+// reference tree) in plain scalar C++, with full-matrix or full-row state so
+// that it is easy to read against the reference.  Sequences are the
+// reference's int8 codes: A0 C1 N2 G3 T4 (SequenceCharToUInt8.cpp:10-30).
+//
+// Pinning (how each function is checked, see tests/test_oracle_*.py):
+//   orc_sw, orc_sw_textbook   golden vectors from the compiled reference,
+//                             live comparison with oracle/_ref, brute-force
+//                             path enumeration on tiny inputs, closed forms
+//                             (identical pair -> 2L), textbook==variant under
+//                             default scoring.
+//   orc_xextend(_tb)          golden vectors + live _ref comparison.
+//   orc_semiglobal            golden vectors + live _ref comparison.
+//   orc_seed_set              golden vectors + live _ref comparison.
+//   orc_find_similar          golden vectors + live _ref comparison, closed
+//                             form of the p-value.
+#pragma once
+#include <cstdint>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// SmithWaterman(...bool& positions) dispatch, sequenceAlignment3.cpp:220-469.
+// out3 = {max, end1, end2} (0-based; -1/-1 when no positive cell).
+int orc_sw(const int8_t *ref, const int8_t *read, int32_t refLen, int32_t readLen, int32_t match,
+           int32_t mismatch, int32_t open, int32_t ext, int32_t *out3);
+// SmithWaterman_non_avx (positions=true), sequenceAlignment3.cpp:143-216.
+int orc_sw_textbook(const int8_t *ref, const int8_t *read, int32_t refLen, int32_t readLen, int32_t match,
+                    int32_t mismatch, int32_t open, int32_t ext, int32_t *out3);
+// Which recurrence SmithWaterman uses: 1 = striped int8/int16 variant, 0 = textbook.
+int orc_sw_uses_simd_variant(int32_t refLen, int32_t readLen, int32_t match, int32_t open, int32_t ext);
+// Number of DP cells the reference evaluates for one SmithWaterman call.
+int64_t orc_sw_cells(int32_t refLen, int32_t readLen);
+
+// SemiGlobal_xextend score only, sequenceAlignment3.cpp:1234-1314.  out3 = {max, end1, end2}.
+int orc_xextend(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, int32_t match, int32_t mismatch,
+                int32_t open, int32_t ext, int32_t xdrop, int32_t w, int32_t *out3);
+// SemiGlobal_xextend + traceback, sequenceAlignment3.cpp:1318-1451.
+// out5 = {max, end1, end2, iii, jjj}; returns cigar length or -1 when cap is too small.
+int orc_xextend_tb(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, int32_t match, int32_t mismatch,
+                   int32_t open, int32_t ext, int32_t xdrop, int32_t w, int32_t *out5, uint32_t *cigar,
+                   int32_t cap);
+// SemiGlobal / SemiGlobal_no_avx (two-piece affine), sequenceAlignment3.cpp:681-941.
+int orc_semiglobal(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, int32_t match, int32_t mismatch,
+                   int32_t open1, int32_t ext1, int32_t open2, int32_t ext2, int32_t zdrop, int32_t w,
+                   int32_t return_cigar, int32_t *out3, uint32_t *cigar, int32_t cap);
+
+// Window loop + getSeeds recursion (+ link_seeds when linked != 0),
+// findSimilarFragmentsForPairedSequence4.cpp:61-201, 288-344, 461-483.
+// out = 4 int32 per seed in set order.  Returns the count (negative when cap too small).
+int orc_seed_set(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match, int32_t mismatch,
+                 int32_t open, int32_t ext, int32_t window, int32_t step, int32_t min_score, int32_t linked,
+                 int32_t *out, int32_t cap);
+
+// findSimilarFragmentsForPairedSequence overload 1,
+// findSimilarFragmentsForPairedSequence4.cpp:433-524.  Same output layout as
+// ref_find_similar in ref_shim.cpp.  stats (may be null) receives
+// {seed scan cells, x-drop cells, seeds before link, seeds after link}.
+int orc_find_similar(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
+                     int32_t mismatch, int32_t open, int32_t ext, int32_t window, int32_t step, int32_t min_score,
+                     int32_t w, int32_t xdrop, double pvalues, double lambda, double kvalue, int32_t *ints,
+                     double *dbls, int32_t cap, uint32_t *cig, int64_t *cig_off, int64_t cig_cap, int64_t *stats);
+
+// Max local score of many pairs (the ranSco inner call,
+// permutationLsqLambdaK.cpp:97).  seqs is one buffer; offsets/lengths per pair.
+int orc_sw_max_batch(const int8_t *seqs, const int64_t *ref_off, const int32_t *ref_len, const int64_t *qry_off,
+                     const int32_t *qry_len, int32_t n, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
+                     int32_t *out_max, int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
