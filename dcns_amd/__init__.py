@@ -1,0 +1,260 @@
+"""dcns_amd -- MI355X (gfx950) implementation of dCNS's alignment hot path.
+
+This module is a thin ctypes binding of the C-ABI declared in include/dcns.h
+(library: dcns_amd/lib/libdcns.so, built by ``python -m dcns_amd.build``).
+All computation runs in the library's HIP kernels; there is no CPU fallback:
+``Context()`` raises when the library is missing or no gfx950 device is
+visible.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libdcns.so")
+CLI_PATH = os.path.join(PKG, "bin", "dCNS")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_L = ctypes.c_int64
+_D = ctypes.c_double
+
+
+class DcnsError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    """dcns_params (defaults: cut1Gap defaults, controlLayer4.cpp:28-52)."""
+    _fields_ = [("match", _I), ("mismatch", _I), ("open", _I), ("ext", _I), ("open2", _I), ("ext2", _I),
+                ("zdrop", _I), ("bandwidth", _I), ("window", _I), ("step", _I), ("min_score", _I), ("w", _I),
+                ("xdrop", _I), ("pad", _I), ("pvalue", _D), ("lambda_", _D), ("k", _D)]
+
+
+class Batch(ctypes.Structure):
+    _fields_ = [("seqs", _P), ("off1", _P), ("len1", _P), ("off2", _P), ("len2", _P), ("n", _I), ("pad", _I)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("scan_cells", _L), ("xdrop_cells", _L), ("seeds", _L), ("linked_seeds", _L), ("fragments", _L),
+                ("gpu_scan_cells", _L), ("levels", _I), ("pad", _I), ("ms_seed", _D), ("ms_replay", _D),
+                ("ms_xdrop", _D), ("ms_total", _D)]
+
+
+_lib = None
+
+
+def load_library():
+    """Loads libdcns.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DcnsError(f"{LIB_PATH} is missing: run `python -m dcns_amd.build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.dcns_version.restype = ctypes.c_char_p
+    lib.dcns_last_error.restype = ctypes.c_char_p
+    lib.dcns_last_error.argtypes = [_P]
+    lib.dcns_default_params.argtypes = [ctypes.POINTER(Params)]
+    lib.dcns_ctx_create.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(_P)]
+    lib.dcns_ctx_destroy.argtypes = [_P]
+    lib.dcns_device_info.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.c_char_p, _I]
+    lib.dcns_last_stats.argtypes = [_P, ctypes.POINTER(Stats)]
+    lib.dcns_kernel_time.argtypes = [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_L)]
+    lib.dcns_kernel_time_reset.argtypes = [_P]
+    lib.dcns_sw_batch.argtypes = [_P, ctypes.POINTER(Batch), _I, _I, _I, _I, _P]
+    lib.dcns_xdrop_extend.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _I, _P, _P, _P, _L]
+    lib.dcns_seed_sets.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _I, _P, _P, _L]
+    lib.dcns_find_similar_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), ctypes.POINTER(_P)]
+    lib.dcns_frag_set_total.restype = _L
+    lib.dcns_frag_set_total.argtypes = [_P]
+    lib.dcns_frag_set_pair.argtypes = [_P, _I, ctypes.POINTER(_L), ctypes.POINTER(_L), ctypes.POINTER(_L)]
+    lib.dcns_frag_set_get.argtypes = [_P, _L, _P, _P, _P, _I, ctypes.POINTER(_I)]
+    lib.dcns_frag_set_free.argtypes = [_P]
+    _lib = lib
+    return lib
+
+
+# Entry points declared in include/dcns.h (the library must export all of them).
+EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_destroy", "dcns_last_error",
+           "dcns_device_info", "dcns_last_stats", "dcns_kernel_time", "dcns_kernel_time_reset", "dcns_sw_batch",
+           "dcns_xdrop_extend", "dcns_seed_sets", "dcns_find_similar_batch", "dcns_frag_set_total",
+           "dcns_frag_set_pair", "dcns_frag_set_get", "dcns_frag_set_free"]
+
+
+def default_params(**kw):
+    p = Params()
+    load_library().dcns_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, "lambda_" if k == "lambda" else k, v)
+    return p
+
+
+class _BatchArrays:
+    """Keeps the numpy arrays of a Batch alive."""
+
+    def __init__(self, pairs):
+        s1s = [np.ascontiguousarray(a, np.int8) for a, _ in pairs]
+        s2s = [np.ascontiguousarray(b, np.int8) for _, b in pairs]
+        l1 = np.array([len(a) for a in s1s], np.int32)
+        l2 = np.array([len(b) for b in s2s], np.int32)
+        chunks, off1, off2, o = [], [], [], 0
+        for a, b in zip(s1s, s2s):
+            off1.append(o)
+            chunks.append(a)
+            o += len(a)
+            off2.append(o)
+            chunks.append(b)
+            o += len(b)
+        self.seqs = np.concatenate(chunks) if chunks else np.zeros(1, np.int8)
+        if self.seqs.size == 0:
+            self.seqs = np.zeros(1, np.int8)
+        self.off1 = np.array(off1, np.int64)
+        self.off2 = np.array(off2, np.int64)
+        self.len1, self.len2 = l1, l2
+        self.b = Batch(self.seqs.ctypes.data, self.off1.ctypes.data if len(pairs) else None, l1.ctypes.data
+                       if len(pairs) else None, self.off2.ctypes.data if len(pairs) else None,
+                       l2.ctypes.data if len(pairs) else None, len(pairs), 0)
+
+    @classmethod
+    def from_flat(cls, seqs, off1, len1, off2, len2):
+        self = cls.__new__(cls)
+        self.seqs = np.ascontiguousarray(seqs, np.int8)
+        self.off1 = np.ascontiguousarray(off1, np.int64)
+        self.len1 = np.ascontiguousarray(len1, np.int32)
+        self.off2 = np.ascontiguousarray(off2, np.int64)
+        self.len2 = np.ascontiguousarray(len2, np.int32)
+        self.b = Batch(self.seqs.ctypes.data, self.off1.ctypes.data, self.len1.ctypes.data, self.off2.ctypes.data,
+                       self.len2.ctypes.data, len(self.len1), 0)
+        return self
+
+
+class Context:
+    """One HIP device + stream (dcns_ctx)."""
+
+    def __init__(self, device=0, timing=False):
+        self.lib = load_library()
+        h = _P()
+        rc = self.lib.dcns_ctx_create(device, 1 if timing else 0, ctypes.byref(h))
+        if rc != 0:
+            raise DcnsError(f"dcns_ctx_create failed ({rc}): {self.lib.dcns_last_error(None).decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dcns_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise DcnsError(f"{what} failed ({rc}): {self.lib.dcns_last_error(self.h).decode()}")
+
+    def device_info(self):
+        cu, clk = _I(), _I()
+        name = ctypes.create_string_buffer(64)
+        self._check(self.lib.dcns_device_info(self.h, ctypes.byref(cu), ctypes.byref(clk), name, 64), "device_info")
+        return {"cu": cu.value, "clock_mhz": clk.value, "arch": name.value.decode()}
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.dcns_last_stats(self.h, ctypes.byref(s)), "last_stats")
+        return {f: getattr(s, f) for f, _ in Stats._fields_ if f != "pad"}
+
+    def kernel_time(self, name):
+        ms, n = _D(), _L()
+        self._check(self.lib.dcns_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)),
+                    "kernel_time")
+        return ms.value, n.value
+
+    def kernel_time_reset(self):
+        self._check(self.lib.dcns_kernel_time_reset(self.h), "kernel_time_reset")
+
+    # -- K1 -----------------------------------------------------------------
+    def sw_batch(self, pairs=None, sc=(2, -3, -4, -2), flat=None):
+        """SmithWaterman (max, end1, end2) per pair; pairs = [(ref, read), ...]
+        or flat = (seqs, off1, len1, off2, len2)."""
+        ba = _BatchArrays.from_flat(*flat) if flat is not None else _BatchArrays(pairs)
+        n = ba.b.n
+        out = np.zeros(3 * max(n, 1), np.int32)
+        self._check(self.lib.dcns_sw_batch(self.h, ctypes.byref(ba.b), *sc, out.ctypes.data), "dcns_sw_batch")
+        return out[:3 * n].reshape(n, 3)
+
+    # -- x-drop -------------------------------------------------------------
+    def xdrop_extend(self, pairs, pair_of_seed, starts, params=None):
+        """x_extend_seed with traceback.  starts[i] = (start1, start2) (seed start + 1).
+        Returns a list of (score, start1, end1, start2, end2, ok, cigar)."""
+        ba = _BatchArrays(pairs)
+        p = params or default_params()
+        pos = np.ascontiguousarray(pair_of_seed, np.int32)
+        st = np.ascontiguousarray(np.asarray(starts, np.int32).reshape(-1, 2))
+        n = len(pos)
+        ints = np.zeros(6 * max(n, 1), np.int32)
+        cap = int(sum(int(ba.len1[q]) + int(ba.len2[q]) + 4 for q in pos)) + 16
+        cig = np.zeros(cap, np.uint32)
+        off = np.zeros(n + 1, np.int64)
+        self._check(self.lib.dcns_xdrop_extend(self.h, ctypes.byref(ba.b), ctypes.byref(p), pos.ctypes.data,
+                                               st.ctypes.data, n, ints.ctypes.data, cig.ctypes.data,
+                                               off.ctypes.data, cap), "dcns_xdrop_extend")
+        return [(*[int(v) for v in ints[6 * i:6 * i + 6]], [int(c) for c in cig[off[i]:off[i + 1]]])
+                for i in range(n)]
+
+    # -- seeds --------------------------------------------------------------
+    def seed_sets(self, pairs, params=None, linked=False):
+        ba = _BatchArrays(pairs)
+        p = params or default_params()
+        n = ba.b.n
+        counts = np.zeros(max(n, 1), np.int32)
+        cap = 1 << 16
+        while True:
+            seeds = np.zeros(4 * cap, np.int32)
+            rc = self.lib.dcns_seed_sets(self.h, ctypes.byref(ba.b), ctypes.byref(p), int(linked),
+                                         counts.ctypes.data, seeds.ctypes.data, cap)
+            if rc == -3:
+                cap = int(counts[:n].sum()) + 16
+                continue
+            self._check(rc, "dcns_seed_sets")
+            break
+        out, o = [], 0
+        for i in range(n):
+            c = int(counts[i])
+            out.append([tuple(int(v) for v in seeds[4 * k:4 * k + 4]) for k in range(o, o + c)])
+            o += c
+        return out
+
+    # -- whole path ---------------------------------------------------------
+    def find_similar(self, pairs=None, params=None, flat=None, with_cigar=True):
+        """findSimilarFragmentsForPairedSequence for every pair.  Returns, per
+        pair, (seeds_size, [(start1, end1, start2, end2, score, p, e, cigar), ...])."""
+        ba = _BatchArrays.from_flat(*flat) if flat is not None else _BatchArrays(pairs)
+        p = params or default_params()
+        fs = _P()
+        self._check(self.lib.dcns_find_similar_batch(self.h, ctypes.byref(ba.b), ctypes.byref(p), ctypes.byref(fs)),
+                    "dcns_find_similar_batch")
+        try:
+            res = []
+            ints = np.zeros(5, np.int32)
+            dbls = np.zeros(2, np.float64)
+            cig = np.zeros(1 << 16, np.uint32)
+            nc = _I()
+            for i in range(ba.b.n):
+                first, count, ss = _L(), _L(), _L()
+                self.lib.dcns_frag_set_pair(fs, i, ctypes.byref(first), ctypes.byref(count), ctypes.byref(ss))
+                frs = []
+                for k in range(first.value, first.value + count.value):
+                    rc = self.lib.dcns_frag_set_get(fs, k, ints.ctypes.data, dbls.ctypes.data,
+                                                    cig.ctypes.data if with_cigar else None, len(cig),
+                                                    ctypes.byref(nc))
+                    self._check(rc, "dcns_frag_set_get")
+                    c = [int(x) for x in cig[:nc.value]] if with_cigar else None
+                    frs.append((*[int(v) for v in ints], float(dbls[0]), float(dbls[1]), c))
+                res.append((ss.value, frs))
+            return res
+        finally:
+            self.lib.dcns_frag_set_free(fs)

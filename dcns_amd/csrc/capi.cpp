@@ -1,0 +1,367 @@
+// capi.cpp -- the extern "C" boundary (include/dcns.h).  Exceptions never
+// cross it: every entry point converts them into a status code and keeps
+// the message for dcns_last_error().
+#include <algorithm>
+#include <cstring>
+
+#include "../../include/dcns.h"
+#include "ctx.h"
+#include "kernels.h"
+#include "pipeline.h"
+
+using namespace dcns;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(dcns_ctx *c, F &&f) {
+    try {
+        f();
+        return DCNS_OK;
+    } catch (const Error &e) {
+        if (c) c->err = e.what();
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (c) c->err = "host allocation failed";
+        g_last_error = "host allocation failed";
+        return DCNS_ERR_INTERNAL;
+    } catch (const std::exception &e) {
+        if (c) c->err = e.what();
+        g_last_error = e.what();
+        return DCNS_ERR_INTERNAL;
+    }
+}
+
+void check_batch(const dcns_batch *b) {
+    if (!b || b->n < 0) throw Error(DCNS_ERR_ARG, "invalid batch");
+    if (b->n > 0 && (!b->seqs || !b->off1 || !b->len1 || !b->off2 || !b->len2))
+        throw Error(DCNS_ERR_ARG, "batch arrays must be non-null");
+}
+}  // namespace
+
+extern "C" {
+
+const char *dcns_version(void) { return "dcns-mi355x 0.1 (gfx950)"; }
+
+void dcns_default_params(dcns_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->match = 2;
+    p->mismatch = -3;
+    p->open = -4;
+    p->ext = -2;
+    p->open2 = -45;
+    p->ext2 = 0;
+    p->zdrop = 50;
+    p->bandwidth = 20000;
+    p->window = 38;
+    p->step = 8;
+    p->min_score = 40;
+    p->w = 10;
+    p->xdrop = 20;
+    p->pvalue = 0.1;
+    p->lambda = 0.382291;
+    p->k = 0.006662;
+}
+
+int dcns_ctx_create(int device, uint32_t flags, dcns_ctx **out) {
+    if (!out) return DCNS_ERR_ARG;
+    *out = nullptr;
+    dcns_ctx *c = nullptr;
+    int rc = guarded(nullptr, [&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw Error(DCNS_ERR_DEVICE, "no HIP device available");
+        if (device < 0 || device >= n) throw Error(DCNS_ERR_ARG, "device index out of range");
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            throw Error(DCNS_ERR_DEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        c = new dcns_ctx();
+        c->device = device;
+        c->timing = (flags & 1u) != 0;
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    });
+    if (rc != DCNS_OK) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return DCNS_OK;
+}
+
+int dcns_ctx_destroy(dcns_ctx *c) {
+    if (!c) return DCNS_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &pe : c->pending) {
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DCNS_OK;
+}
+
+const char *dcns_last_error(const dcns_ctx *c) { return c ? c->err.c_str() : g_last_error.c_str(); }
+
+int dcns_device_info(dcns_ctx *c, int32_t *cu_count, int32_t *clock_mhz, char *name, int32_t name_cap) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
+        if (cu_count) *cu_count = prop.multiProcessorCount;
+        if (clock_mhz) *clock_mhz = prop.clockRate / 1000;
+        if (name && name_cap > 0) {
+            std::strncpy(name, prop.gcnArchName, (size_t)name_cap - 1);
+            name[name_cap - 1] = 0;
+        }
+    });
+}
+
+int dcns_last_stats(const dcns_ctx *c, dcns_stats *out) {
+    if (!c || !out) return DCNS_ERR_ARG;
+    *out = c->stats;
+    return DCNS_OK;
+}
+
+int dcns_kernel_time(const dcns_ctx *c, const char *name, double *ms, int64_t *launches) {
+    if (!c || !name) return DCNS_ERR_ARG;
+    auto it = c->timers.find(name);
+    if (ms) *ms = it == c->timers.end() ? 0.0 : it->second.ms;
+    if (launches) *launches = it == c->timers.end() ? 0 : it->second.launches;
+    return DCNS_OK;
+}
+
+int dcns_kernel_time_reset(dcns_ctx *c) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        c->sync();
+        c->timers.clear();
+    });
+}
+
+int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
+                  int32_t *out3) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        check_batch(b);
+        if (b->n > 0 && !out3) throw Error(DCNS_ERR_ARG, "out3 must be non-null");
+        if (match < 1 || match > 127 || mismatch > 0 || mismatch < -128 || open > 0 || open < -128 || ext > 0 ||
+            ext < -128)
+            throw Error(DCNS_ERR_ARG, "scores must fit int8 with match > 0 and penalties <= 0");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        const int32_t n = b->n;
+        if (n == 0) return;
+        dcns_params dp;
+        dcns_default_params(&dp);
+        dp.match = match;
+        dp.mismatch = mismatch;
+        dp.open = open;
+        dp.ext = ext;
+        Params P = to_params(dp);
+        ScoreTab tab = make_score_tab(match, mismatch);
+        // stage sequences
+        std::vector<int64_t> o1((size_t)n), o2((size_t)n);
+        std::vector<int32_t> l1(b->len1, b->len1 + n), l2(b->len2, b->len2 + n);
+        int64_t total = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            if (l1[(size_t)i] < 0 || l2[(size_t)i] < 0) throw Error(DCNS_ERR_ARG, "negative length");
+            total += l1[(size_t)i] + l2[(size_t)i] + 8;
+        }
+        std::vector<int8_t> host((size_t)total + 64, 2);
+        int64_t o = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            o1[(size_t)i] = o;
+            std::memcpy(&host[(size_t)o], b->seqs + b->off1[i], (size_t)l1[(size_t)i]);
+            o += l1[(size_t)i] + 4;
+            o2[(size_t)i] = o;
+            std::memcpy(&host[(size_t)o], b->seqs + b->off2[i], (size_t)l2[(size_t)i]);
+            o += l2[(size_t)i] + 4;
+        }
+        // cost order: pairs of similar stripe counts and lengths share waves
+        std::vector<int32_t> order((size_t)n);
+        for (int32_t i = 0; i < n; ++i) order[(size_t)i] = i;
+        std::sort(order.begin(), order.end(), [&](int32_t a, int32_t bb) {
+            const int64_t ca = (int64_t)((l2[(size_t)a] + 31) / 32) * l1[(size_t)a];
+            const int64_t cb = (int64_t)((l2[(size_t)bb] + 31) / 32) * l1[(size_t)bb];
+            return ca != cb ? ca > cb : a < bb;
+        });
+        auto up = [&](DBuf &d, const void *src, size_t bytes) {
+            d.ensure(std::max<size_t>(bytes, 64));
+            hip_check(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+        };
+        up(c->seq, host.data(), host.size());
+        up(c->sw_off1, o1.data(), o1.size() * 8);
+        up(c->sw_len1, l1.data(), l1.size() * 4);
+        up(c->sw_off2, o2.data(), o2.size() * 8);
+        up(c->sw_len2, l2.data(), l2.size() * 4);
+        up(c->sw_order, order.data(), order.size() * 4);
+        c->sw_out.ensure((size_t)n * 12 + 64);
+        // boundary scratch: chunks of the ordered list with a per-chunk stride
+        const int64_t budget = (int64_t)1 << 30;
+        int32_t s = 0;
+        while (s < n) {
+            int32_t maxl1 = 1, e = s;
+            while (e < n) {
+                const int32_t l = std::max(l1[(size_t)order[(size_t)e]], 1);
+                const int32_t m = std::max(maxl1, l);
+                if ((int64_t)(e - s + 1) * m * 8 > budget && e > s) break;
+                maxl1 = m;
+                ++e;
+            }
+            const int64_t stride = (int64_t)maxl1 * 2;
+            c->sw_bound.ensure((size_t)(e - s) * (size_t)stride * 4 + 64);
+            c->launch("sw_batch", [&] {
+                return launch_sw_batch(c->stream, c->seq.as<int8_t>(), c->sw_off1.as<int64_t>(), c->sw_len1.as<int32_t>(),
+                                       c->sw_off2.as<int64_t>(), c->sw_len2.as<int32_t>(),
+                                       c->sw_order.as<int32_t>() + s, e - s, P, tab, c->sw_out.as<int32_t>(),
+                                       c->sw_bound.as<int32_t>(), stride);
+            });
+            s = e;
+        }
+        hip_check(hipMemcpyAsync(out3, c->sw_out.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream), "download");
+        c->sync();
+    });
+}
+
+int dcns_xdrop_extend(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, const int32_t *pair_of_seed,
+                      const int32_t *seeds, int32_t n, int32_t *out_ints, uint32_t *cigar, int64_t *cig_off,
+                      int64_t cig_cap) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        check_batch(b);
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        validate_params(*p);
+        if (n < 0 || (n > 0 && (!pair_of_seed || !seeds || !out_ints || !cig_off)))
+            throw Error(DCNS_ERR_ARG, "invalid seed arrays");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        Params P = to_params(*p);
+        Staged S = stage_batch(c, *b, P);
+        std::vector<XTask> tasks((size_t)n);
+        for (int32_t i = 0; i < n; ++i) {
+            const int32_t pr = pair_of_seed[i];
+            if (pr < 0 || pr >= b->n) throw Error(DCNS_ERR_ARG, "pair index out of range");
+            const int32_t s1 = seeds[2 * i], s2 = seeds[2 * i + 1];
+            if (s1 < 0 || s1 > b->len1[pr] || s2 < 0 || s2 > b->len2[pr])
+                throw Error(DCNS_ERR_ARG, "seed start outside its sequences");
+            tasks[(size_t)i] = XTask{pr, s1, s2, 0};
+        }
+        std::vector<XResult> xr;
+        int64_t cells = 0;
+        xdrop_extend(c, S, P, tasks, xr, cells);
+        int64_t o = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const XResult &r = xr[(size_t)i];
+            int32_t *q = out_ints + 6 * i;
+            q[0] = r.score;
+            q[1] = r.start1;
+            q[2] = r.end1;
+            q[3] = r.start2;
+            q[4] = r.end2;
+            q[5] = r.ok ? 1 : 0;
+            cig_off[i] = o;
+            if (o + (int64_t)r.cigar.size() > cig_cap) throw Error(DCNS_ERR_CAPACITY, "cigar capacity");
+            if (!r.cigar.empty()) std::memcpy(cigar + o, r.cigar.data(), r.cigar.size() * 4);
+            o += (int64_t)r.cigar.size();
+        }
+        cig_off[n] = o;
+        c->stats = dcns_stats{};
+        c->stats.xdrop_cells = cells;
+    });
+}
+
+int dcns_seed_sets(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, int32_t linked, int32_t *counts,
+                   int32_t *seeds, int64_t cap) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        check_batch(b);
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        validate_params(*p);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        Params P = to_params(*p);
+        Staged S = stage_batch(c, *b, P);
+        SeedResult sr;
+        seed_sets(c, S, P, !linked, sr);
+        const auto &sets = linked ? sr.linked : sr.unlinked;
+        int64_t o = 0;
+        bool fits = true;
+        for (int32_t i = 0; i < b->n; ++i) {
+            if (counts) counts[i] = (int32_t)sets[(size_t)i].size();
+            for (const Seed &s : sets[(size_t)i]) {
+                if (o < cap && seeds) {
+                    seeds[4 * o] = s.getStart1();
+                    seeds[4 * o + 1] = s.getEnd1();
+                    seeds[4 * o + 2] = s.getStart2();
+                    seeds[4 * o + 3] = s.getEnd2();
+                } else {
+                    fits = false;
+                }
+                ++o;
+            }
+        }
+        c->stats = dcns_stats{};
+        c->stats.scan_cells = sr.scan_cells;
+        c->stats.gpu_scan_cells = sr.gpu_cells;
+        c->stats.levels = sr.levels;
+        if (!fits) throw Error(DCNS_ERR_CAPACITY, "seed capacity");
+    });
+}
+
+int dcns_find_similar_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, dcns_frag_set **out) {
+    if (!c || !out) return DCNS_ERR_ARG;
+    *out = nullptr;
+    dcns_frag_set *fs = new (std::nothrow) dcns_frag_set();
+    if (!fs) return DCNS_ERR_INTERNAL;
+    int rc = guarded(c, [&] {
+        check_batch(b);
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        find_similar(c, *b, *p, fs->fs);
+    });
+    if (rc != DCNS_OK) {
+        delete fs;
+        return rc;
+    }
+    *out = fs;
+    return DCNS_OK;
+}
+
+int64_t dcns_frag_set_total(const dcns_frag_set *fs) { return fs ? (int64_t)fs->fs.frags.size() : -1; }
+
+int dcns_frag_set_pair(const dcns_frag_set *fs, int32_t pair, int64_t *first, int64_t *count, int64_t *seeds_size) {
+    if (!fs || pair < 0 || (size_t)pair >= fs->fs.first.size()) return DCNS_ERR_ARG;
+    if (first) *first = fs->fs.first[(size_t)pair];
+    if (count) *count = fs->fs.count[(size_t)pair];
+    if (seeds_size) *seeds_size = fs->fs.seeds_size[(size_t)pair];
+    return DCNS_OK;
+}
+
+int dcns_frag_set_get(const dcns_frag_set *fs, int64_t idx, int32_t *ints5, double *dbls2, uint32_t *cigar,
+                      int32_t cig_cap, int32_t *ncig) {
+    if (!fs || idx < 0 || (size_t)idx >= fs->fs.frags.size()) return DCNS_ERR_ARG;
+    const PairedSimilarFragment &f = fs->fs.frags[(size_t)idx];
+    if (ints5) {
+        ints5[0] = (int32_t)f.getStart1();
+        ints5[1] = (int32_t)f.getEnd1();
+        ints5[2] = (int32_t)f.getStart2();
+        ints5[3] = (int32_t)f.getEnd2();
+        ints5[4] = (int32_t)f.getScore();
+    }
+    if (dbls2) {
+        dbls2[0] = f.getPValue();
+        dbls2[1] = f.getEValue();
+    }
+    const auto &cg = f.getCigar();
+    if (ncig) *ncig = (int32_t)cg.size();
+    if (cigar && cig_cap > 0)
+        std::memcpy(cigar, cg.data(), std::min<size_t>((size_t)cig_cap, cg.size()) * sizeof(uint32_t));
+    return (int64_t)cg.size() > cig_cap && cigar ? DCNS_ERR_CAPACITY : DCNS_OK;
+}
+
+void dcns_frag_set_free(dcns_frag_set *fs) { delete fs; }
+
+}  // extern "C"

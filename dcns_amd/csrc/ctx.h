@@ -1,0 +1,117 @@
+// ctx.h -- device context, grow-only device buffers and launch timing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dcns.h"
+#include "dcns_types.h"
+
+namespace dcns {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw Error(DCNS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device buffer that only grows.  keep=true preserves the old contents.
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void ensure(size_t bytes, hipStream_t st = nullptr, bool keep = false, size_t used = 0) {
+        if (bytes <= cap) return;
+        size_t n = bytes + bytes / 4 + 4096;
+        void *q = nullptr;
+        hip_check(hipMalloc(&q, n), "hipMalloc");
+        if (keep && p && used) {
+            hip_check(hipMemcpyAsync(q, p, used, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync");
+            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        }
+        if (p) hip_check(hipFree(p), "hipFree");
+        p = q;
+        cap = n;
+    }
+    template <class T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+struct KernelTimer {
+    double ms = 0;
+    int64_t launches = 0;
+};
+
+}  // namespace dcns
+
+struct dcns_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::string err;
+    dcns_stats stats{};
+    std::map<std::string, dcns::KernelTimer> timers;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;  // recorded, not yet read
+    std::vector<hipEvent_t> event_pool;
+    // reusable device buffers
+    dcns::DBuf seq, pairs, wtasks, pow, traj, tcount, hitlist, ctr, nodes, roots, tasks_a, tasks_b;
+    dcns::DBuf xtasks, xout, tb_off, tb, cigar, cig_host_dummy;
+    dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound;
+
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        dcns::hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+    // Wrap a launch: records HIP events on the context stream when timing is on.
+    template <class F>
+    void launch(const char *name, F &&f) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (timing) {
+            a = take_event();
+            b = take_event();
+            dcns::hip_check(hipEventRecord(a, stream), "hipEventRecord");
+        }
+        dcns::hip_check(f(), name);
+        if (timing) {
+            dcns::hip_check(hipEventRecord(b, stream), "hipEventRecord");
+            pending.push_back({name, {a, b}});
+        }
+    }
+    // Reads the events of finished launches (call after a stream sync).
+    void collect() {
+        for (auto &pe : pending) {
+            float ms = 0;
+            dcns::hip_check(hipEventElapsedTime(&ms, pe.second.first, pe.second.second), "hipEventElapsedTime");
+            auto &t = timers[pe.first];
+            t.ms += ms;
+            t.launches += 1;
+            event_pool.push_back(pe.second.first);
+            event_pool.push_back(pe.second.second);
+        }
+        pending.clear();
+    }
+    void sync() {
+        dcns::hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        collect();
+    }
+};

@@ -1,0 +1,89 @@
+// dcns_types.h -- plain data shared by host code and gfx950 kernels.
+//
+// Sequences live in one device buffer of int8 codes (A0 C1 N2 G3 T4, the
+// alphabet of impl/SequenceCharToUInt8.cpp:10-30); every pair addresses its
+// reference (seq1) and query (seq2) by offset.  Reverse complements are never
+// materialised: a kernel reads s[L-1-k] and maps c -> 4-c
+// (SequenceCharToUInt8.cpp:62-68).
+#pragma once
+#include <cstdint>
+
+namespace dcns {
+
+constexpr int32_t kOutOfBand = -536870912;  // sequenceAlignment3.cpp:34
+constexpr int kMaxWindow = 64;             // widest seed window the scan kernels hold in registers
+
+// Scoring and seeding parameters of cut1Gap (controlLayer4.cpp:28-52).
+struct Params {
+    int32_t match, mismatch, open, ext;   // Scorei(match, mismatch); gap open/extend (<= 0)
+    int32_t open2, ext2, zdrop, bandwidth; // two-piece piece 2, z-drop, band (SemiGlobal)
+    int32_t window, step, min_score;       // seed window (38), step (8), minimum seed score (40)
+    int32_t w, xdrop;                      // x-drop band half-width (10) and drop (20)
+    int32_t min_ref;                       // miniReferenceSize = min_score / match
+};
+
+struct PairDesc {
+    int64_t off1, off2;   // offsets of seq1 / seq2 in the sequence buffer
+    int32_t len1, len2;
+    int32_t nwin;         // windows in the schedule of overload 1
+    int32_t win_base;     // global index of the pair's first window
+};
+
+// One forward local-alignment scan: rows [a, b) of seq1 against the window
+// that starts at query position sp = win * step.
+struct ScanTask {
+    int32_t pair, win, a, b;
+    int32_t node;   // node resolved once the scan's trajectory is known
+    int32_t scan;   // trajectory slot
+};
+
+// A node of one window's getSeeds recursion tree
+// (findSimilarFragmentsForPairedSequence4.cpp:61-128).  Seed coordinates
+// are absolute and 0-based; e1 < 0 marks "no seed in [a, b)".
+struct Node {
+    int32_t a, b, scan;
+    int32_t s1, e1, s2, e2;
+    int32_t left, right;
+};
+
+// Trajectory entry: the scan's running maximum rose strictly above every
+// earlier cell (and above min_score - 1) at (row, col) with this score.
+struct TrajEntry {
+    int32_t row;
+    int32_t score_col;  // score << 8 | col
+};
+
+// Device counters used by the level loop.
+struct Counters {
+    int32_t nodes;      // allocated nodes
+    int32_t scans;      // allocated trajectory slots
+    int32_t next_tasks; // scan tasks queued for the next level
+    int32_t hits;       // level-0 windows with at least one trajectory entry
+    int32_t overflow;   // set when a pool capacity was exceeded
+    int32_t entries;    // trajectory entries recorded by the last scan launches
+    int32_t pad[2];
+};
+
+// x-drop extension task for one seed (x_extend_seed,
+// findSimilarFragmentsForPairedSequence4.cpp:245-285); start1/start2 are the
+// seed start + 1 as passed at :492-495.
+struct XTask {
+    int32_t pair;
+    int32_t start1, start2;
+    int32_t pad;
+};
+
+struct XOut {
+    int32_t fmax, fe1, fe2;       // forward pass (score only)
+    int32_t rmax, re1, re2;       // reverse pass
+    int32_t fw, rw;               // band half-width of the final pass of each direction
+    int32_t rrows;                // rows evaluated by the final reverse pass
+    int32_t iii, jjj;             // traceback leftovers
+    int32_t ncig;                 // cigar length
+    int32_t status;               // 0 done, 1 needs a wider band kernel
+    int32_t pad;
+    int64_t cig_off;              // offset into the cigar arena
+    int64_t cells;                // DP cells evaluated (all passes)
+};
+
+}  // namespace dcns

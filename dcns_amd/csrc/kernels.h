@@ -1,0 +1,40 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (internal API).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dcns_types.h"
+#include "dp_common.h"
+
+namespace dcns {
+
+// seedscan.hip
+hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
+                               int32_t nwt, const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount,
+                               int32_t cap, int32_t *hitlist, Counters *ctr);
+hipError_t launch_scan_general(hipStream_t st, int wm, bool textbook, const int8_t *seq, const PairDesc *pairs,
+                               const ScanTask *tasks, const int32_t *ntasks_dev, int32_t ntasks_max, const Params &P,
+                               const ScoreTab &tab, TrajEntry *traj, int32_t *tcount, int32_t cap, Counters *ctr);
+hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
+                          const int32_t *pair_of_window, const Params &P, const ScoreTab &tab, int level0,
+                          const int32_t *hitlist, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
+                          Node *nodes, int32_t node_cap, const TrajEntry *traj, const int32_t *tcount, int32_t cap,
+                          int32_t scan_cap, ScanTask *next, int32_t next_cap, Counters *ctr);
+hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v);
+hipError_t launch_gather_roots(hipStream_t st, const Node *nodes, const int32_t *hitlist, const int32_t *n_dev,
+                               int32_t n_max, Node *out);
+
+// xdrop.hip
+// Pass 1: forward score-only extension + reverse score-only extension.
+hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                              int32_t n, const Params &P, XOut *out);
+// Pass 2: reverse extension with traceback into a band-compressed store, then the cigar.
+hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                              int32_t n, const Params &P, XOut *out, const int64_t *tb_off, uint8_t *tb,
+                              uint32_t *cigar);
+
+// swbatch.hip: SmithWaterman max score and first position for arbitrary pairs.
+hipError_t launch_sw_batch(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
+                           const int64_t *off2, const int32_t *len2, const int32_t *order, int32_t n,
+                           const Params &P, const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride);
+
+}  // namespace dcns

@@ -1,0 +1,538 @@
+// pipeline.cpp -- host orchestration of dCNS's hot path over the gfx950
+// kernels: staging, the recursion level loop, exact replay of the reference's
+// std::set<Seed> insertion order, link_seeds, x-drop extension and fragment
+// dedup (findSimilarFragmentsForPairedSequence4.cpp:433-524).
+#include "pipeline.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <functional>
+
+#include "dp_common.h"
+#include "kernels.h"
+
+namespace dcns {
+
+namespace {
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+void upload(dcns_ctx *c, DBuf &buf, const std::vector<T> &v) {
+    buf.ensure(std::max<size_t>(v.size() * sizeof(T), 64));
+    if (!v.empty())
+        hip_check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream), "upload");
+}
+
+template <class T>
+void download(dcns_ctx *c, std::vector<T> &v, const void *src, size_t n) {
+    v.resize(n);
+    if (n) hip_check(hipMemcpyAsync(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
+}
+
+Counters read_ctr(dcns_ctx *c) {
+    Counters h;
+    hip_check(hipMemcpyAsync(&h, c->ctr.p, sizeof(h), hipMemcpyDeviceToHost, c->stream), "read counters");
+    c->sync();
+    return h;
+}
+
+void write_ctr(dcns_ctx *c, const Counters &h) {
+    hip_check(hipMemcpyAsync(c->ctr.p, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "write counters");
+}
+
+// Window schedule of overload 1 (findSimilarFragmentsForPairedSequence4.cpp:461-480):
+// sp = 0, step, 2*step, ...; stop after the first window that ends past L2,
+// or when sp passes L2.
+int32_t window_count(int32_t L2, int32_t W, int32_t step) {
+    int32_t n = 1, sp = 0;
+    while (true) {
+        sp += step;
+        if (sp > L2) break;
+        ++n;
+        if (sp + W > L2) break;
+    }
+    return n;
+}
+}  // namespace
+
+void validate_params(const dcns_params &p) {
+    auto bad = [](const char *m) { throw Error(DCNS_ERR_ARG, m); };
+    if (p.match < 1 || p.match > 127) bad("match must be in [1, 127]");
+    if (p.mismatch > 0 || p.mismatch < -128) bad("mismatch must be in [-128, 0]");
+    if (p.open > 0 || p.open < -128 || p.ext > 0 || p.ext < -128) bad("gap open/extend must be in [-128, 0]");
+    if (p.window < 1 || p.window > kMaxWindow) bad("window must be in [1, 64]");
+    if (p.step < 1) bad("step must be >= 1");
+    if (p.min_score < 1) bad("min_score must be >= 1");
+    if (p.w < 1 || 2 * p.w + 1 > 161) bad("x-drop band half-width must be in [1, 80]");
+    if (p.xdrop < 0) bad("xdrop must be >= 0");
+    if ((int64_t)p.window * p.match >= 32767) bad("window * match must stay below 32767");
+}
+
+Params to_params(const dcns_params &p) {
+    Params P;
+    P.match = p.match;
+    P.mismatch = p.mismatch;
+    P.open = p.open;
+    P.ext = p.ext;
+    P.open2 = p.open2;
+    P.ext2 = p.ext2;
+    P.zdrop = p.zdrop;
+    P.bandwidth = p.bandwidth;
+    P.window = p.window;
+    P.step = p.step;
+    P.min_score = p.min_score;
+    P.w = p.w;
+    P.xdrop = p.xdrop;
+    P.min_ref = p.min_score / p.match;
+    return P;
+}
+
+Staged stage_batch(dcns_ctx *c, const dcns_batch &b, const Params &P) {
+    Staged S;
+    S.pd.resize((size_t)b.n);
+    int64_t total = 0;
+    auto span = [](int64_t len) { return ((len + 64) + 63) & ~(int64_t)63; };  // 64-byte aligned, >= 64 bytes slack
+    for (int32_t i = 0; i < b.n; ++i) {
+        if (b.len1[i] < 0 || b.len2[i] < 0) throw Error(DCNS_ERR_ARG, "negative sequence length");
+        total += span(b.len1[i]) + span(b.len2[i]);
+    }
+    std::vector<int8_t> host((size_t)total + 64, 2);
+    int64_t o = 0, nw = 0;
+    for (int32_t i = 0; i < b.n; ++i) {
+        PairDesc &d = S.pd[(size_t)i];
+        d.len1 = b.len1[i];
+        d.len2 = b.len2[i];
+        d.off1 = o;
+        if (d.len1) std::memcpy(&host[(size_t)o], b.seqs + b.off1[i], (size_t)d.len1);
+        o += span(d.len1);
+        d.off2 = o;
+        if (d.len2) std::memcpy(&host[(size_t)o], b.seqs + b.off2[i], (size_t)d.len2);
+        o += span(d.len2);
+        d.nwin = window_count(d.len2, P.window, P.step);
+        if (nw + d.nwin > INT32_MAX / 2) throw Error(DCNS_ERR_ARG, "batch has too many windows");
+        d.win_base = (int32_t)nw;
+        nw += d.nwin;
+    }
+    S.nwin = nw;
+    upload(c, c->seq, host);
+    upload(c, c->pairs, S.pd);
+    return S;
+}
+
+// ---------------------------------------------------------------------------
+// Seed scan: device level loop + host replay.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Forest {
+    int32_t NW = 0;
+    std::vector<int32_t> root_index;  // per window: index into roots, or -1
+    std::vector<Node> roots, dyn;
+    const Node &node(int32_t id) const {
+        return id < NW ? roots[(size_t)root_index[(size_t)id]] : dyn[(size_t)(id - NW)];
+    }
+};
+
+void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_t &gpu_cells, int32_t &levels) {
+    const int32_t NW = (int32_t)S.nwin;
+    F.NW = NW;
+    F.root_index.assign((size_t)NW, -1);
+    levels = 0;
+    gpu_cells = 0;
+    if (NW == 0 || (int64_t)P.window * P.match < P.min_score) return;  // no window can reach min_score
+    const int wm = P.window <= 38 ? 38 : 64;
+    const int32_t cap = P.window * P.match - P.min_score + 1;
+    const ScoreTab tab = make_score_tab(P.match, P.mismatch);
+    const bool textbook_general = !sw_simd_variant(1 << 30, P.window, P.match, P.open, P.ext);
+
+    // level-0 wave tasks, longest reference first
+    std::vector<int2> wt;
+    std::vector<int32_t> pow((size_t)NW);
+    for (size_t p = 0; p < S.pd.size(); ++p) {
+        const PairDesc &d = S.pd[p];
+        for (int32_t g = 0; g < d.nwin; g += 128) wt.push_back(make_int2((int)p, g));
+        for (int32_t w = 0; w < d.nwin; ++w) pow[(size_t)(d.win_base + w)] = (int32_t)p;
+        for (int32_t w = 0; w < d.nwin; ++w) gpu_cells += (int64_t)d.len1 * std::min(P.window, d.len2 - w * P.step);
+    }
+    std::stable_sort(wt.begin(), wt.end(), [&](const int2 &a, const int2 &b) {
+        return S.pd[(size_t)a.x].len1 > S.pd[(size_t)b.x].len1;
+    });
+    upload(c, c->wtasks, wt);
+    upload(c, c->pow, pow);
+
+    int64_t scan_cap = (int64_t)NW + std::max<int64_t>(NW / 4, 4096);
+    int64_t node_cap = (int64_t)NW + std::max<int64_t>(NW / 2, 8192);
+    int64_t next_cap = std::max<int64_t>(NW / 4, 4096);
+    c->traj.ensure((size_t)scan_cap * cap * sizeof(TrajEntry));
+    c->tcount.ensure((size_t)scan_cap * sizeof(int32_t));
+    c->hitlist.ensure((size_t)NW * sizeof(int32_t) + 64);
+    c->nodes.ensure((size_t)node_cap * sizeof(Node));
+    c->tasks_a.ensure((size_t)next_cap * sizeof(ScanTask));
+    c->tasks_b.ensure((size_t)next_cap * sizeof(ScanTask));
+    c->ctr.ensure(sizeof(Counters) + 64);
+
+    Counters h{};
+    h.nodes = NW;
+    h.scans = NW;
+    write_ctr(c, h);
+    c->launch("scan_uniform", [&] {
+        return launch_scan_uniform(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->wtasks.as<int2>(),
+                                   (int32_t)wt.size(), P, tab, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
+                                   c->hitlist.as<int32_t>(), c->ctr.as<Counters>());
+    });
+    h = read_ctr(c);
+    const int32_t hits = h.hits;
+
+    // resolve with capacity retry: counters keep counting past a full pool,
+    // so one retry with the requested sizes always fits
+    auto resolve = [&](int level0, const ScanTask *tasks, int32_t n, const int32_t *n_dev) {
+        Counters before = h;
+        before.next_tasks = 0;
+        before.overflow = 0;
+        for (int attempt = 0; attempt < 4; ++attempt) {
+            size_t used_nodes = (size_t)before.nodes * sizeof(Node);
+            size_t used_scans = (size_t)before.scans;
+            c->nodes.ensure((size_t)node_cap * sizeof(Node), c->stream, true, used_nodes);
+            c->traj.ensure((size_t)scan_cap * cap * sizeof(TrajEntry), c->stream, true, used_scans * cap * sizeof(TrajEntry));
+            c->tcount.ensure((size_t)scan_cap * sizeof(int32_t), c->stream, true, used_scans * sizeof(int32_t));
+            c->tasks_b.ensure((size_t)next_cap * sizeof(ScanTask));
+            write_ctr(c, before);
+            c->launch("resolve", [&] {
+                return launch_resolve(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->pow.as<int32_t>(), P,
+                                      tab, level0, c->hitlist.as<int32_t>(), tasks, n_dev, n, c->nodes.as<Node>(),
+                                      (int32_t)node_cap, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
+                                      (int32_t)scan_cap, c->tasks_b.as<ScanTask>(), (int32_t)next_cap,
+                                      c->ctr.as<Counters>());
+            });
+            h = read_ctr(c);
+            if (!h.overflow) return;
+            node_cap = std::max<int64_t>(node_cap, (int64_t)h.nodes + h.nodes / 4 + 1024);
+            scan_cap = std::max<int64_t>(scan_cap, (int64_t)h.scans + h.scans / 4 + 1024);
+            next_cap = std::max<int64_t>(next_cap, (int64_t)h.next_tasks + h.next_tasks / 4 + 1024);
+        }
+        throw Error(DCNS_ERR_INTERNAL, "seed recursion pools could not be sized");
+    };
+
+    // the hit count lives in the counter block; resolve reads it from there
+    const int32_t *hits_dev = &c->ctr.as<Counters>()->hits;
+    resolve(1, nullptr, hits, hits_dev);
+    levels = 1;
+    while (h.next_tasks > 0) {
+        const int32_t n = h.next_tasks;
+        std::swap(c->tasks_a.p, c->tasks_b.p);
+        std::swap(c->tasks_a.cap, c->tasks_b.cap);
+        // the level's task count: next_tasks is reset by resolve, so keep a copy in pad[0]
+        h.pad[0] = n;
+        h.next_tasks = 0;
+        write_ctr(c, h);
+        const int32_t *n_dev = &c->ctr.as<Counters>()->pad[0];
+        c->launch("scan_general", [&] {
+            return launch_scan_general(c->stream, wm, textbook_general, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                       c->tasks_a.as<ScanTask>(), n_dev, n, P, tab, c->traj.as<TrajEntry>(),
+                                       c->tcount.as<int32_t>(), cap, c->ctr.as<Counters>());
+        });
+        h = read_ctr(c);
+        {   // cells of this level's scans
+            std::vector<ScanTask> tl;
+            download(c, tl, c->tasks_a.p, (size_t)n);
+            c->sync();
+            for (const ScanTask &t : tl) {
+                const PairDesc &d = S.pd[(size_t)t.pair];
+                gpu_cells += (int64_t)(t.b - t.a) * std::min(P.window, d.len2 - t.win * P.step);
+            }
+        }
+        resolve(0, c->tasks_a.as<ScanTask>(), n, n_dev);
+        ++levels;
+    }
+
+    // bring the forest home: hit windows, their roots, the dynamic nodes
+    std::vector<int32_t> hitl;
+    download(c, hitl, c->hitlist.p, (size_t)hits);
+    c->roots.ensure((size_t)std::max(hits, 1) * sizeof(Node));
+    c->launch("gather", [&] {
+        return launch_gather_roots(c->stream, c->nodes.as<Node>(), c->hitlist.as<int32_t>(), hits_dev, hits,
+                                   c->roots.as<Node>());
+    });
+    download(c, F.roots, c->roots.p, (size_t)hits);
+    download(c, F.dyn, c->nodes.as<Node>() + NW, (size_t)(h.nodes - NW));
+    c->sync();
+    for (int32_t k = 0; k < hits; ++k) F.root_index[(size_t)hitl[(size_t)k]] = k;
+}
+
+Seed seed_of(const Node &n) { return Seed(n.s1, n.e1, n.s2, n.e2); }
+
+// Reverse-SW lengths of getSeeds (:152-153) for a node, for cell accounting.
+int64_t reverse_cells(const Node &n, int32_t sp, int32_t tw, int32_t W) {
+    const int64_t r1 = std::min(1 + (n.e1 - n.a), tw + W);
+    const int64_t r2 = std::min(1 + (n.e2 - sp), tw + W);
+    return r1 * r2;
+}
+
+// Inner getSeeds (:61-128) replayed from the device's recursion tree: the
+// set of the node's range is {seed}, then the left child's set, then the
+// right child's set, each inserted in its own set order.
+std::set<Seed> inner_set(const Forest &F, int32_t id, int32_t sp, int32_t tw, int32_t W, int64_t &cells) {
+    std::set<Seed> out;
+    const Node &n = F.node(id);
+    cells += (int64_t)(n.b - n.a) * tw;
+    if (n.e1 < 0) return out;
+    cells += reverse_cells(n, sp, tw, W);
+    out.insert(seed_of(n));
+    if (n.left >= 0)
+        for (const Seed &s : inner_set(F, n.left, sp, tw, W, cells)) out.insert(s);
+    if (n.right >= 0)
+        for (const Seed &s : inner_set(F, n.right, sp, tw, W, cells)) out.insert(s);
+    return out;
+}
+
+}  // namespace
+
+void link_seeds(std::set<Seed> &seeds) {
+    std::vector<Seed> cur(seeds.begin(), seeds.end());
+    std::sort(cur.begin(), cur.end(), [](Seed a, Seed b) { return a.getStart1() < b.getStart1(); });
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        std::vector<Seed> next;
+        std::vector<char> used(cur.size(), 0);
+        for (size_t i = 0; i < cur.size(); ++i) {
+            if (used[i]) continue;
+            const Seed s1 = cur[i];
+            bool merged = false;
+            for (size_t j = i + 1; j < cur.size(); ++j) {
+                if (used[j]) continue;
+                const Seed s2 = cur[j];
+                if (s1.getStart2() <= s2.getStart2() && s1.getEnd1() >= s2.getEnd1() && s1.getEnd2() >= s2.getEnd2()) {
+                    used[j] = 1;  // s2 inside s1
+                } else if (s1.getStart2() <= s2.getStart2() && s1.getEnd1() <= s2.getEnd1() &&
+                           s1.getEnd2() <= s2.getEnd2()) {
+                    const int32_t d = s2.getStart1() - s1.getStart1();
+                    if (d == s2.getStart2() - s1.getStart2() && d == s2.getEnd1() - s1.getEnd1() &&
+                        d == s2.getEnd2() - s1.getEnd2()) {  // same diagonal: merge
+                        next.push_back(Seed(s1.getStart1(), s2.getEnd1(), s1.getStart2(), s2.getEnd2()));
+                        used[i] = 1;
+                        used[j] = 1;
+                        merged = true;
+                        break;
+                    }
+                } else if (s2.getStart1() > s1.getEnd1() || s2.getStart2() > s1.getEnd2()) {
+                    break;
+                }
+            }
+            if (merged) changed = true;
+            else next.push_back(s1);
+        }
+        cur.swap(next);
+    }
+    seeds.clear();
+    for (const Seed &s : cur) seeds.insert(s);
+}
+
+void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked, SeedResult &out) {
+    Forest F;
+    run_levels(c, S, P, F, out.gpu_cells, out.levels);
+    const size_t np = S.pd.size();
+    out.linked.assign(np, {});
+    out.before_link.assign(np, 0);
+    if (keep_unlinked) out.unlinked.assign(np, {});
+    out.scan_cells = 0;
+    for (size_t p = 0; p < np; ++p) {
+        const PairDesc &d = S.pd[p];
+        std::set<Seed> seeds;
+        int64_t cells = 0;
+        for (int32_t w = 0; w < d.nwin; ++w) {
+            const int32_t sp = w * P.step;
+            const int32_t tw = std::min(P.window, d.len2 - sp);
+            const int32_t gw = d.win_base + w;
+            cells += (int64_t)d.len1 * tw;
+            if (F.root_index.empty() || F.root_index[(size_t)gw] < 0) continue;
+            const Node &r = F.node(gw);
+            if (r.e1 < 0) continue;
+            cells += reverse_cells(r, sp, tw, P.window);
+            const Seed s0 = seed_of(r);
+            if (seeds.find(s0) != seeds.end()) continue;  // the set gate of :170
+            seeds.insert(s0);
+            if (r.left >= 0)
+                for (const Seed &s : inner_set(F, r.left, sp, tw, P.window, cells)) seeds.insert(s);
+            if (r.right >= 0)
+                for (const Seed &s : inner_set(F, r.right, sp, tw, P.window, cells)) seeds.insert(s);
+        }
+        out.scan_cells += cells;
+        out.before_link[p] = (int64_t)seeds.size();
+        if (keep_unlinked) out.unlinked[p].assign(seeds.begin(), seeds.end());
+        link_seeds(seeds);
+        out.linked[p].assign(seeds.begin(), seeds.end());
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x-drop extension.
+// ---------------------------------------------------------------------------
+void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vector<XTask> &tasks,
+                  std::vector<XResult> &out, int64_t &cells) {
+    const int32_t n = (int32_t)tasks.size();
+    out.assign((size_t)n, XResult{});
+    cells = 0;
+    if (n == 0) return;
+    upload(c, c->xtasks, tasks);
+    c->xout.ensure((size_t)n * sizeof(XOut));
+    hip_check(hipMemsetAsync(c->xout.p, 0, (size_t)n * sizeof(XOut), c->stream), "hipMemsetAsync");
+    static const int bands[] = {21, 41, 81, 161};
+    std::vector<XOut> xo;
+    int first = 0;
+    while (first < 4 && bands[first] < 2 * P.w + 1) ++first;
+    bool pending = true;
+    for (int bi = first; bi < 4 && pending; ++bi) {
+        c->launch("xdrop_pass1", [&] {
+            return launch_xdrop_pass1(c->stream, bands[bi], c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                      c->xtasks.as<XTask>(), n, P, c->xout.as<XOut>());
+        });
+        download(c, xo, c->xout.p, (size_t)n);
+        c->sync();
+        pending = false;
+        for (const XOut &o : xo)
+            if (o.status != 0) pending = true;
+    }
+    if (pending) throw Error(DCNS_ERR_ARG, "x-drop band grew beyond 161 cells");
+    // traceback store and cigar arena sizes from pass 1
+    std::vector<int64_t> tboff((size_t)n, 0);
+    int64_t tbtotal = 0, cigtotal = 0;
+    bool any_band[4] = {false, false, false, false};
+    for (int32_t i = 0; i < n; ++i) {
+        XOut &o = xo[(size_t)i];
+        cells += o.cells;
+        o.cig_off = cigtotal;
+        if (o.fe1 < 0 || o.fe2 < 0) continue;
+        const int32_t B = 2 * o.rw + 1, BP = (B + 3) & ~3;
+        tboff[(size_t)i] = tbtotal;
+        tbtotal += (int64_t)(o.rrows + 1) * BP;
+        cigtotal += std::max(0, o.re1) + std::max(0, o.re2) + 2;
+        for (int k = 0; k < 4; ++k)
+            if (B <= bands[k]) { any_band[k] = true; break; }
+    }
+    upload(c, c->xout, xo);
+    upload(c, c->tb_off, tboff);
+    c->tb.ensure((size_t)std::max<int64_t>(tbtotal, 64));
+    c->cigar.ensure((size_t)std::max<int64_t>(cigtotal, 16) * sizeof(uint32_t));
+    for (int k = 0; k < 4; ++k) {
+        if (!any_band[k]) continue;
+        c->launch("xdrop_pass2", [&] {
+            return launch_xdrop_pass2(c->stream, bands[k], c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                      c->xtasks.as<XTask>(), n, P, c->xout.as<XOut>(), c->tb_off.as<int64_t>(),
+                                      c->tb.as<uint8_t>(), c->cigar.as<uint32_t>());
+        });
+    }
+    std::vector<uint32_t> cig;
+    download(c, xo, c->xout.p, (size_t)n);
+    download(c, cig, c->cigar.p, (size_t)cigtotal);
+    c->sync();
+    for (int32_t i = 0; i < n; ++i) {
+        const XOut &o = xo[(size_t)i];
+        const XTask &t = tasks[(size_t)i];
+        XResult &r = out[(size_t)i];
+        r.start1 = t.start1;
+        r.start2 = t.start2;
+        r.end1 = r.end2 = 0;
+        if (o.fe1 < 0 || o.fe2 < 0) {  // forward pass found no positive cell (:259-266)
+            r.ok = false;
+            r.score = o.fmax;
+            continue;
+        }
+        r.score = o.rmax;
+        r.cigar.assign(cig.begin() + o.cig_off, cig.begin() + o.cig_off + o.ncig);
+        if (o.re1 < 0 || o.re2 < 0) {
+            r.ok = false;
+            continue;
+        }
+        r.ok = true;
+        r.end1 = t.start1 + o.fe1 - o.iii;
+        r.end2 = t.start2 + o.fe2 - o.jjj;
+        r.start1 = t.start1 + o.fe1 - o.re1;
+        r.start2 = t.start2 + o.fe2 - o.re2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// findSimilarFragmentsForPairedSequence, overload 1.
+// ---------------------------------------------------------------------------
+void find_similar(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out) {
+    validate_params(p);
+    const Params P = to_params(p);
+    const double t0 = now_ms();
+    Staged S = stage_batch(c, b, P);
+    SeedResult sr;
+    seed_sets(c, S, P, false, sr);
+    const double t1 = now_ms();
+    std::vector<XTask> tasks;
+    std::vector<int64_t> first_task((size_t)b.n + 1, 0);
+    for (int32_t i = 0; i < b.n; ++i) {
+        first_task[(size_t)i] = (int64_t)tasks.size();
+        for (const Seed &s : sr.linked[(size_t)i]) {
+            XTask t;
+            t.pair = i;
+            t.start1 = s.getStart1() + 1;  // :492-495
+            t.start2 = s.getStart2() + 1;
+            t.pad = 0;
+            tasks.push_back(t);
+        }
+    }
+    first_task[(size_t)b.n] = (int64_t)tasks.size();
+    std::vector<XResult> xr;
+    int64_t xcells = 0;
+    xdrop_extend(c, S, P, tasks, xr, xcells);
+    const double t2 = now_ms();
+
+    out.frags.clear();
+    out.first.assign((size_t)b.n, 0);
+    out.count.assign((size_t)b.n, 0);
+    out.seeds_size = sr.before_link;
+    for (int32_t i = 0; i < b.n; ++i) {
+        const double L1d = (double)b.len1[i], L2d = (double)b.len2[i];
+        std::vector<PairedSimilarFragment> fr;
+        for (int64_t k = first_task[(size_t)i]; k < first_task[(size_t)i + 1]; ++k) {
+            const XResult &r = xr[(size_t)k];
+            if (!r.ok) continue;  // p = 100
+            const double e = p.k * L1d * L2d * std::exp(-1.0 * p.lambda * (double)r.score);  // :282-283
+            const double pv = 1.0 - std::exp(-1.0 * e);
+            if (!(pv < p.pvalue)) continue;
+            // dedup, :500-521: 1-based uint32 stored coordinates against 0-based int32 ones
+            PairedSimilarFragment nf((uint32_t)(r.start1 + 1), (uint32_t)(r.end1 + 1), (uint32_t)(r.start2 + 1),
+                                     (uint32_t)(r.end2 + 1), (uint32_t)r.score, r.cigar, pv, e);
+            bool fresh = true;
+            for (PairedSimilarFragment &g : fr) {
+                if (g.getStart1() == (uint32_t)(r.start1 + 1) && g.getStart2() == (uint32_t)(r.start2 + 1)) {
+                    fresh = false;
+                    if ((uint32_t)r.end1 >= g.getEnd1() && (uint32_t)r.end2 >= g.getEnd2()) g = nf;
+                    break;
+                } else if (g.getEnd1() == (uint32_t)(r.end1 + 1) && g.getEnd2() == (uint32_t)(r.end2 + 1)) {
+                    fresh = false;
+                    if ((uint32_t)r.start1 <= g.getStart1() && (uint32_t)r.start2 <= g.getStart2()) g = nf;
+                    break;
+                }
+            }
+            if (fresh) fr.push_back(nf);
+        }
+        out.first[(size_t)i] = (int64_t)out.frags.size();
+        out.count[(size_t)i] = (int64_t)fr.size();
+        for (auto &f : fr) out.frags.push_back(std::move(f));
+    }
+    const double t3 = now_ms();
+    dcns_stats &st = c->stats;
+    st = dcns_stats{};
+    st.scan_cells = sr.scan_cells;
+    st.gpu_scan_cells = sr.gpu_cells;
+    st.xdrop_cells = xcells;
+    for (int64_t v : sr.before_link) st.seeds += v;
+    for (auto &v : sr.linked) st.linked_seeds += (int64_t)v.size();
+    st.fragments = (int64_t)out.frags.size();
+    st.levels = sr.levels;
+    st.ms_seed = t1 - t0;
+    st.ms_xdrop = t2 - t1;
+    st.ms_replay = t3 - t2;
+    st.ms_total = t3 - t0;
+}
+
+}  // namespace dcns

@@ -1,0 +1,56 @@
+// pipeline.h -- host orchestration of the device kernels (internal API).
+#pragma once
+#include <cstdint>
+#include <set>
+#include <vector>
+
+#include "../../include/dcns.h"
+#include "../../include/dcns/model.h"
+#include "ctx.h"
+#include "dcns_types.h"
+
+namespace dcns {
+
+void validate_params(const dcns_params &p);
+Params to_params(const dcns_params &p);
+
+// Sequences of a batch staged on the device.
+struct Staged {
+    std::vector<PairDesc> pd;
+    int64_t nwin = 0;  // windows over all pairs
+};
+Staged stage_batch(dcns_ctx *c, const dcns_batch &b, const Params &P);
+
+// Seed sets of every pair (set order), before and after link_seeds.
+struct SeedResult {
+    std::vector<std::vector<Seed>> linked;  // per pair, set order after link_seeds
+    std::vector<int64_t> before_link;        // seeds.size() before link_seeds (the "seeds size" line)
+    std::vector<std::vector<Seed>> unlinked; // filled only when asked
+    int64_t scan_cells = 0, gpu_cells = 0;
+    int32_t levels = 0;
+};
+void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked, SeedResult &out);
+
+// link_seeds, findSimilarFragmentsForPairedSequence4.cpp:288-344.
+void link_seeds(std::set<Seed> &seeds);
+
+// x_extend_seed with traceback for a list of tasks (seed start + 1 already applied).
+struct XResult {
+    int32_t score, start1, end1, start2, end2;
+    bool ok;  // false: the reference's p = 100 case
+    std::vector<uint32_t> cigar;
+};
+void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vector<XTask> &tasks,
+                  std::vector<XResult> &out, int64_t &cells);
+
+struct FragSet {
+    std::vector<PairedSimilarFragment> frags;
+    std::vector<int64_t> first, count, seeds_size;
+};
+void find_similar(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out);
+
+}  // namespace dcns
+
+struct dcns_frag_set {
+    dcns::FragSet fs;
+};

@@ -1,0 +1,457 @@
+// seedscan.hip -- windowed seed scan of dCNS on gfx950.
+//
+// Reference: getSeeds (findSimilarFragmentsForPairedSequence4.cpp:61-201)
+// calls the striped SmithWaterman (sequenceAlignment3.cpp:220-469) once per
+// 38-bp query window over the whole reference, then recurses on the
+// reference left and right of every hit.  Here:
+//
+//  * k_scan_uniform  -- level 0.  One wave owns 128 consecutive windows of one
+//    pair: lane l holds windows base+l and base+64+l as the two int16 halves
+//    of each VGPR.  The reference base of the current row is the same for the
+//    whole wave (scalar load); the substitution score of a column is one
+//    v_perm_b32 of a uniform 64-bit score table by a per-lane selector.
+//    H, E and the selectors of all window columns stay in VGPRs for the whole
+//    scan, so the only memory traffic is the reference stream.
+//  * k_scan_general  -- later levels (right sub-ranges of the recursion).
+//    One lane per task, int32 lanes, per-lane reference rows.
+//  * k_resolve       -- walks the left spine of the recursion for every
+//    scanned node: the left child's scan is a prefix of its parent's, so its
+//    maximum is read from the parent's trajectory; the reverse SW (<= 76 x 38)
+//    runs inline; right children are queued as new scans.
+//
+// Every scan records its "trajectory": each (row, col, score) where the
+// running maximum rose strictly above both every earlier cell and
+// min_score - 1.  The first maximum of any row prefix [a, b') of the scan is
+// then the last entry with row < b'.
+#include <hip/hip_runtime.h>
+
+#include "dp_common.h"
+#include "kernels.h"
+
+namespace dcns {
+namespace {
+
+__device__ __forceinline__ void uniform_table(const ScoreTab &t, int c, uint32_t &hi, uint32_t &lo) {
+    switch (c) {
+        case 0: hi = t.hi[0]; lo = t.lo[0]; break;
+        case 1: hi = t.hi[1]; lo = t.lo[1]; break;
+        case 3: hi = t.hi[3]; lo = t.lo[3]; break;
+        case 4: hi = t.hi[4]; lo = t.lo[4]; break;
+        default: hi = 0; lo = 0; break;
+    }
+}
+
+__device__ __forceinline__ void record(TrajEntry *__restrict__ traj, int64_t slot, int32_t cap, int32_t &cnt,
+                                       int32_t row, int32_t score, int32_t col) {
+    if (cnt < cap) {
+        TrajEntry e;
+        e.row = row;
+        e.score_col = (score << 8) | col;
+        traj[slot * cap + cnt] = e;
+    }
+    ++cnt;
+}
+
+template <int WM>
+__global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                     const int2 *__restrict__ wtasks, Params P, ScoreTab tab,
+                                                     TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
+                                                     int32_t cap, int32_t *__restrict__ hitlist,
+                                                     Counters *__restrict__ ctr) {
+    const int lane = threadIdx.x;
+    const int2 task = wtasks[blockIdx.x];
+    const PairDesc pd = pairs[task.x];
+    const int8_t *__restrict__ ref = seq + pd.off1;
+    const int8_t *__restrict__ qry = seq + pd.off2;
+    const int L1 = pd.len1, L2 = pd.len2;
+    const int wl = task.y + lane, wh = task.y + 64 + lane;
+    const bool vl = wl < pd.nwin, vh = wh < pd.nwin;
+    const int spl = wl * P.step, sph = wh * P.step;
+    const int twl = vl ? min(P.window, L2 - spl) : 0;
+    const int twh = vh ? min(P.window, L2 - sph) : 0;
+
+    uint32_t sel[WM], H[WM], E[WM];
+#pragma unroll
+    for (int j = 0; j < WM; ++j) {
+        int ql = j < twl ? qry[spl + j] : 2;
+        int qh = j < twh ? qry[sph + j] : 2;
+        sel[j] = sel16(ql) | (sel16(qh) << 16);
+        H[j] = 0;
+        E[j] = 0;
+    }
+    const uint32_t OPEN = (uint32_t)(-P.open) * 0x10001u;
+    const uint32_t EXT = (uint32_t)(-P.ext) * 0x10001u;
+    const uint32_t thr = (uint32_t)(P.min_score - 1);
+    uint32_t best = thr | (thr << 16);
+    int32_t cntl = 0, cnth = 0;
+
+    uint32_t word = L1 > 0 ? *(const uint32_t *)ref : 0u;
+    uint32_t next = 0;
+    for (int i = 0; i < L1; ++i) {
+        const int sub = i & 3;
+        if (sub == 0 && i + 4 < L1) next = *(const uint32_t *)(ref + i + 4);
+        const int c = (int)(int8_t)(word >> (8 * sub));
+        if (sub == 3) word = next;
+        uint32_t thi, tlo;
+        uniform_table(tab, c, thi, tlo);
+        uint32_t diag = 0, left = 0, fc = 0, rmax = 0;
+#pragma unroll
+        for (int j = 0; j < WM; ++j) {
+            const uint32_t s = perm(thi, tlo, sel[j]);
+            const uint32_t t = pk_add(diag, s);
+            diag = H[j];
+            const uint32_t hpre = pk_max_s(t, E[j]);  // E >= 0, so this is max(t, 0, E)
+            E[j] = pk_max_u(pk_subsat(E[j], EXT), pk_subsat(hpre, OPEN));
+            fc = pk_max_u(pk_subsat(left, OPEN), pk_subsat(fc, EXT));
+            const uint32_t h = pk_max_s(hpre, fc);
+            H[j] = h;
+            left = h;
+            rmax = pk_max_s(rmax, h);
+        }
+        const uint32_t imp = pk_subsat(rmax, best);
+        if (__builtin_expect(imp != 0, 0)) {
+            if (imp & 0xffffu) {
+                const uint32_t v = rmax & 0xffffu;
+                int jj = 0;
+#pragma unroll
+                for (int j = WM - 1; j >= 0; --j)
+                    if ((H[j] & 0xffffu) == v) jj = j;
+                record(traj, pd.win_base + wl, cap, cntl, i, (int32_t)v, jj);
+                best = (best & 0xffff0000u) | v;
+            }
+            if (imp >> 16) {
+                const uint32_t v = rmax >> 16;
+                int jj = 0;
+#pragma unroll
+                for (int j = WM - 1; j >= 0; --j)
+                    if ((H[j] >> 16) == v) jj = j;
+                record(traj, pd.win_base + wh, cap, cnth, i, (int32_t)v, jj);
+                best = (best & 0xffffu) | (v << 16);
+            }
+        }
+    }
+    if (vl) {
+        tcount[pd.win_base + wl] = cntl;
+        if (cntl > 0) {
+            hitlist[atomicAdd(&ctr->hits, 1)] = pd.win_base + wl;
+            atomicAdd(&ctr->entries, min(cntl, cap));
+        }
+    }
+    if (vh) {
+        tcount[pd.win_base + wh] = cnth;
+        if (cnth > 0) {
+            hitlist[atomicAdd(&ctr->hits, 1)] = pd.win_base + wh;
+            atomicAdd(&ctr->entries, min(cnth, cap));
+        }
+    }
+}
+
+// One lane per ScanTask: rows [a, b) of seq1 against one window, int32.
+template <int WM, bool TEXTBOOK>
+__global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                     const ScanTask *__restrict__ tasks, const int32_t *__restrict__ ntasks_p,
+                                                     Params P, ScoreTab tab, TrajEntry *__restrict__ traj,
+                                                     int32_t *__restrict__ tcount, int32_t cap,
+                                                     Counters *__restrict__ ctr) {
+    __shared__ uint32_t shi[8], slo[8];
+    if (threadIdx.x < 8) {
+        shi[threadIdx.x] = tab.hi[threadIdx.x];
+        slo[threadIdx.x] = tab.lo[threadIdx.x];
+    }
+    __syncthreads();
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= *ntasks_p) return;
+    const ScanTask t = tasks[gid];
+    const PairDesc pd = pairs[t.pair];
+    const int8_t *__restrict__ ref = seq + pd.off1;
+    const int8_t *__restrict__ qry = seq + pd.off2;
+    const int sp = t.win * P.step;
+    const int tw = min(P.window, pd.len2 - sp);
+    uint32_t sel[WM];
+    int32_t H[WM], E[WM];
+#pragma unroll
+    for (int j = 0; j < WM; ++j) {
+        sel[j] = sel32(j < tw ? qry[sp + j] : 2);
+        H[j] = 0;
+        E[j] = 0;
+    }
+    const int32_t O = P.open, X = P.ext;
+    int32_t best = P.min_score - 1;
+    int32_t cnt = 0;
+    int c_next = t.a < t.b ? ref[t.a] : 2;
+    for (int i = t.a; i < t.b; ++i) {
+        const int c = c_next;
+        if (i + 1 < t.b) c_next = ref[i + 1];
+        const int ci = (unsigned)c < 8u ? c : 2;
+        const uint32_t thi = shi[ci], tlo = slo[ci];
+        int32_t diag = 0, left = 0, fc = 0, rmax = 0;
+#pragma unroll
+        for (int j = 0; j < WM; ++j) {
+            const int32_t s = (int32_t)perm(thi, tlo, sel[j]);
+            const int32_t hpre = max(diag + s, E[j]);
+            diag = H[j];
+            const int32_t f = max(left + O, fc + X);
+            const int32_t h = max(hpre, f);
+            fc = max(f, 0);
+            if (TEXTBOOK)
+                E[j] = max3i(E[j] + X, h + O, 0);
+            else
+                E[j] = max3i(E[j] + X, hpre + O, 0);
+            H[j] = h;
+            left = h;
+            rmax = max(rmax, h);
+        }
+        if (rmax > best) {
+            int jj = 0;
+#pragma unroll
+            for (int j = WM - 1; j >= 0; --j)
+                if (H[j] == rmax) jj = j;
+            record(traj, t.scan, cap, cnt, i, rmax, jj);
+            best = rmax;
+        }
+    }
+    tcount[t.scan] = cnt;
+    if (cnt > 0) atomicAdd(&ctr->entries, min(cnt, cap));
+}
+
+// Reverse SW of getSeeds (:152-159): reverse complements of seq1[e1-r1+1..e1]
+// and seq2[q0-r2+1..q0], max and first position, int32 in registers.
+template <int WM>
+__device__ void reverse_sw(const int8_t *__restrict__ s1, int32_t e1, int32_t r1, const int8_t *__restrict__ s2,
+                           int32_t q0, int32_t r2, const Params &P, const uint32_t *shi, const uint32_t *slo,
+                           int32_t &best, int32_t &bi, int32_t &bj) {
+    uint32_t sel[WM];
+    int32_t H[WM], E[WM];
+#pragma unroll
+    for (int j = 0; j < WM; ++j) {
+        sel[j] = sel32(j < r2 ? 4 - s2[q0 - j] : 2);
+        H[j] = 0;
+        E[j] = 0;
+    }
+    const bool textbook = !sw_simd_variant(r1, r2, P.match, P.open, P.ext);
+    const int32_t O = P.open, X = P.ext;
+    best = 0;
+    bi = -1;
+    bj = -1;
+    for (int i = 0; i < r1; ++i) {
+        const int c = 4 - s1[e1 - i];
+        const int ci = (unsigned)c < 8u ? c : 2;
+        const uint32_t thi = shi[ci], tlo = slo[ci];
+        int32_t diag = 0, left = 0, fc = 0, rmax = 0;
+#pragma unroll
+        for (int j = 0; j < WM; ++j) {
+            const int32_t s = (int32_t)perm(thi, tlo, sel[j]);
+            const int32_t hpre = max(diag + s, E[j]);
+            diag = H[j];
+            const int32_t f = max(left + O, fc + X);
+            const int32_t h = max(hpre, f);
+            fc = max(f, 0);
+            E[j] = max3i(E[j] + X, (textbook ? h : hpre) + O, 0);
+            H[j] = h;
+            left = h;
+            rmax = max(rmax, h);
+        }
+        if (rmax > best) {
+            int jj = 0;
+#pragma unroll
+            for (int j = WM - 1; j >= 0; --j)
+                if (H[j] == rmax) jj = j;
+            best = rmax;
+            bi = i;
+            bj = jj;
+        }
+    }
+}
+
+// Resolves nodes whose scan trajectory is complete.  level0: the inputs are
+// hit windows (node id = global window id); otherwise ScanTasks.
+template <int WM>
+__global__ __launch_bounds__(64) void k_resolve(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                const int32_t *__restrict__ pair_of_window, Params P, ScoreTab tab,
+                                                int level0, const int32_t *__restrict__ hitlist,
+                                                const ScanTask *__restrict__ tasks, const int32_t *__restrict__ n_p,
+                                                Node *__restrict__ nodes, int32_t node_cap,
+                                                const TrajEntry *__restrict__ traj, const int32_t *__restrict__ tcount,
+                                                int32_t cap, int32_t scan_cap, ScanTask *__restrict__ next,
+                                                int32_t next_cap, Counters *__restrict__ ctr) {
+    __shared__ uint32_t shi[8], slo[8];
+    if (threadIdx.x < 8) {
+        shi[threadIdx.x] = tab.hi[threadIdx.x];
+        slo[threadIdx.x] = tab.lo[threadIdx.x];
+    }
+    __syncthreads();
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= *n_p) return;
+    int32_t node, pair, win;
+    if (level0) {
+        node = hitlist[gid];
+        pair = pair_of_window[node];
+        win = node - pairs[pair].win_base;
+        nodes[node].a = 0;
+        nodes[node].b = pairs[pair].len1;
+        nodes[node].scan = node;
+    } else {
+        const ScanTask t = tasks[gid];
+        node = t.node;
+        pair = t.pair;
+        win = t.win;
+    }
+    const PairDesc pd = pairs[pair];
+    const int8_t *__restrict__ s1 = seq + pd.off1;
+    const int8_t *__restrict__ s2 = seq + pd.off2;
+    const int sp = win * P.step;
+    const int tw = min(P.window, pd.len2 - sp);
+    while (true) {
+        Node nd = nodes[node];
+        const int cnt = min(tcount[nd.scan], cap);
+        const TrajEntry *tr = traj + (int64_t)nd.scan * cap;
+        int k = cnt - 1;
+        while (k >= 0 && tr[k].row >= nd.b) --k;
+        if (k < 0) {
+            nd.e1 = -1;
+            nd.s1 = nd.s2 = nd.e2 = -1;
+            nd.left = nd.right = -1;
+            nodes[node] = nd;
+            break;
+        }
+        const int32_t e1 = tr[k].row;
+        const int32_t col = tr[k].score_col & 0xff;
+        const int32_t r1 = min(1 + (e1 - nd.a), tw + P.window);
+        const int32_t r2 = min(1 + col, tw + P.window);
+        int32_t rmax, q1, q2;
+        reverse_sw<WM>(s1, e1, r1, s2, sp + col, r2, P, shi, slo, rmax, q1, q2);
+        nd.e1 = e1;
+        nd.e2 = sp + col;
+        nd.s1 = e1 - q1;
+        nd.s2 = sp + col - q2;
+        nd.left = nd.right = -1;
+        if (nd.b - e1 - 1 >= P.min_ref) {
+            const int32_t r = atomicAdd(&ctr->nodes, 1);
+            const int32_t sc = atomicAdd(&ctr->scans, 1);
+            const int32_t q = atomicAdd(&ctr->next_tasks, 1);
+            if (r < node_cap && sc < scan_cap && q < next_cap) {
+                Node rn;
+                rn.a = e1 + 1;
+                rn.b = nd.b;
+                rn.scan = sc;
+                rn.s1 = rn.e1 = rn.s2 = rn.e2 = -1;
+                rn.left = rn.right = -1;
+                nodes[r] = rn;
+                ScanTask st;
+                st.pair = pair;
+                st.win = win;
+                st.a = e1 + 1;
+                st.b = nd.b;
+                st.node = r;
+                st.scan = sc;
+                next[q] = st;
+                nd.right = r;
+            } else {
+                atomicOr(&ctr->overflow, 1);
+            }
+        }
+        int32_t lchild = -1;
+        if (nd.s1 - nd.a >= P.min_ref) {
+            const int32_t l = atomicAdd(&ctr->nodes, 1);
+            if (l < node_cap) {
+                Node ln;
+                ln.a = nd.a;
+                ln.b = nd.s1;
+                ln.scan = nd.scan;
+                ln.s1 = ln.e1 = ln.s2 = ln.e2 = -1;
+                ln.left = ln.right = -1;
+                nodes[l] = ln;
+                lchild = l;
+            } else {
+                atomicOr(&ctr->overflow, 1);
+            }
+        }
+        nd.left = lchild;
+        nodes[node] = nd;
+        if (lchild < 0) break;
+        node = lchild;
+    }
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_gather_roots(const Node *__restrict__ nodes, const int32_t *__restrict__ hitlist,
+                               const int32_t *__restrict__ n_p, Node *__restrict__ out) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < *n_p) out[gid] = nodes[hitlist[gid]];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers (host side).
+// ---------------------------------------------------------------------------
+hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
+                               int32_t nwt, const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount,
+                               int32_t cap, int32_t *hitlist, Counters *ctr) {
+    if (nwt <= 0) return hipSuccess;
+    if (wm <= 38)
+        hipLaunchKernelGGL(k_scan_uniform<38>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, tab, traj, tcount,
+                           cap, hitlist, ctr);
+    else
+        hipLaunchKernelGGL(k_scan_uniform<64>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, tab, traj, tcount,
+                           cap, hitlist, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_general(hipStream_t st, int wm, bool textbook, const int8_t *seq, const PairDesc *pairs,
+                               const ScanTask *tasks, const int32_t *ntasks_dev, int32_t ntasks_max, const Params &P,
+                               const ScoreTab &tab, TrajEntry *traj, int32_t *tcount, int32_t cap, Counters *ctr) {
+    if (ntasks_max <= 0) return hipSuccess;
+    dim3 g((ntasks_max + 63) / 64), b(64);
+    if (wm <= 38) {
+        if (textbook)
+            hipLaunchKernelGGL((k_scan_general<38, true>), g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj,
+                               tcount, cap, ctr);
+        else
+            hipLaunchKernelGGL((k_scan_general<38, false>), g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj,
+                               tcount, cap, ctr);
+    } else {
+        if (textbook)
+            hipLaunchKernelGGL((k_scan_general<64, true>), g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj,
+                               tcount, cap, ctr);
+        else
+            hipLaunchKernelGGL((k_scan_general<64, false>), g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj,
+                               tcount, cap, ctr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
+                          const int32_t *pair_of_window, const Params &P, const ScoreTab &tab, int level0,
+                          const int32_t *hitlist, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
+                          Node *nodes, int32_t node_cap, const TrajEntry *traj, const int32_t *tcount, int32_t cap,
+                          int32_t scan_cap, ScanTask *next, int32_t next_cap, Counters *ctr) {
+    if (n_max <= 0) return hipSuccess;
+    dim3 g((n_max + 63) / 64), b(64);
+    if (wm <= 38)
+        hipLaunchKernelGGL(k_resolve<38>, g, b, 0, st, seq, pairs, pair_of_window, P, tab, level0, hitlist, tasks, n_dev,
+                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr);
+    else
+        hipLaunchKernelGGL(k_resolve<64>, g, b, 0, st, seq, pairs, pair_of_window, P, tab, level0, hitlist, tasks, n_dev,
+                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_roots(hipStream_t st, const Node *nodes, const int32_t *hitlist, const int32_t *n_dev,
+                               int32_t n_max, Node *out) {
+    if (n_max <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_roots, dim3((n_max + 255) / 256), dim3(256), 0, st, nodes, hitlist, n_dev, out);
+    return hipGetLastError();
+}
+
+}  // namespace dcns

@@ -1,0 +1,131 @@
+// swbatch.hip -- SmithWaterman max score and first position for many
+// independent pairs of arbitrary length (the inner call of ranSco,
+// permutationLsqLambdaK.cpp:97, and any direct SmithWaterman caller).
+//
+// Semantics follow SmithWaterman(...bool& positions)
+// (sequenceAlignment3.cpp:220-469): the striped int8/int16 recurrence when its
+// bound test passes, the textbook recurrence otherwise; first strict maximum
+// in (ref row, read column) order.
+//
+// One lane per pair, int32.  The read is cut into 32-column stripes held in
+// VGPRs; each stripe sweeps all reference rows and hands its last column
+// (H and the clamped horizontal gap) to the next stripe through a per-pair
+// boundary array (8 bytes per reference row).  Pairs are processed in the
+// host's cost order so that lanes of one wave run similar loop counts.
+#include <hip/hip_runtime.h>
+
+#include "dp_common.h"
+#include "kernels.h"
+
+namespace dcns {
+namespace {
+
+constexpr int S = 32;
+
+template <bool TEXTBOOK>
+__device__ void sw_pair(const int8_t *__restrict__ ref, int32_t L1, const int8_t *__restrict__ qry, int32_t L2,
+                        const Params &P, const uint32_t *shi, const uint32_t *slo, int2 *__restrict__ bnd,
+                        int32_t &best, int32_t &bi, int32_t &bj) {
+    best = 0;
+    bi = -1;
+    bj = -1;
+    const int32_t O = P.open, X = P.ext;
+    const int nstripes = (L2 + S - 1) / S;
+    for (int s = 0; s < nstripes; ++s) {
+        const int j0 = s * S;
+        uint32_t sel[S];
+        int32_t H[S], E[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            sel[j] = sel32(j0 + j < L2 ? qry[j0 + j] : 2);
+            H[j] = 0;
+            E[j] = 0;
+        }
+        int32_t sbest = 0, si = -1, sj = -1;
+        int32_t hb_prev = 0;  // boundary H of the previous row (diagonal of column j0)
+        for (int i = 0; i < L1; ++i) {
+            const int c = ref[i];
+            const int ci = (unsigned)c < 8u ? c : 2;
+            const uint32_t thi = shi[ci], tlo = slo[ci];
+            int32_t left = 0, fc = 0;
+            if (s > 0) {
+                const int2 b = bnd[i];
+                left = b.x;
+                fc = b.y;
+            }
+            int32_t diag = hb_prev;
+            hb_prev = left;
+            int32_t rmax = 0;
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                const int32_t sc = (int32_t)perm(thi, tlo, sel[j]);
+                const int32_t hpre = max(diag + sc, E[j]);
+                diag = H[j];
+                const int32_t f = max(left + O, fc + X);
+                const int32_t h = max(hpre, f);
+                fc = max(f, 0);
+                if (TEXTBOOK)
+                    E[j] = max3i(E[j] + X, h + O, 0);
+                else
+                    E[j] = max3i(E[j] + X, hpre + O, 0);
+                H[j] = h;
+                left = h;
+                rmax = max(rmax, h);
+            }
+            if (j0 + S < L2) bnd[i] = make_int2(left, fc);
+            if (rmax > sbest) {
+                int jj = 0;
+#pragma unroll
+                for (int j = S - 1; j >= 0; --j)
+                    if (H[j] == rmax) jj = j;
+                sbest = rmax;
+                si = i;
+                sj = j0 + jj;
+            }
+        }
+        if (sbest > best || (sbest == best && sbest > 0 && (si < bi || (si == bi && sj < bj)))) {
+            best = sbest;
+            bi = si;
+            bj = sj;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_sw_batch(const int8_t *__restrict__ seq, const int64_t *__restrict__ off1,
+                                                 const int32_t *__restrict__ len1, const int64_t *__restrict__ off2,
+                                                 const int32_t *__restrict__ len2, const int32_t *__restrict__ order,
+                                                 int32_t n, Params P, ScoreTab tab, int32_t *__restrict__ out,
+                                                 int32_t *__restrict__ bound, int64_t bound_stride) {
+    __shared__ uint32_t shi[8], slo[8];
+    if (threadIdx.x < 8) {
+        shi[threadIdx.x] = tab.hi[threadIdx.x];
+        slo[threadIdx.x] = tab.lo[threadIdx.x];
+    }
+    __syncthreads();
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int p = order ? order[gid] : gid;
+    const int32_t L1 = len1[p], L2 = len2[p];
+    int2 *bnd = (int2 *)(bound + (int64_t)gid * bound_stride);
+    int32_t best, bi, bj;
+    if (sw_simd_variant(L1, L2, P.match, P.open, P.ext))
+        sw_pair<false>(seq + off1[p], L1, seq + off2[p], L2, P, shi, slo, bnd, best, bi, bj);
+    else
+        sw_pair<true>(seq + off1[p], L1, seq + off2[p], L2, P, shi, slo, bnd, best, bi, bj);
+    out[3 * p + 0] = best;
+    out[3 * p + 1] = bi;
+    out[3 * p + 2] = bj;
+}
+
+}  // namespace
+
+hipError_t launch_sw_batch(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
+                           const int64_t *off2, const int32_t *len2, const int32_t *order, int32_t n, const Params &P,
+                           const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sw_batch, dim3((n + 63) / 64), dim3(64), 0, st, seq, off1, len1, off2, len2, order, n, P, tab,
+                       out, bound, bound_stride);
+    return hipGetLastError();
+}
+
+}  // namespace dcns

@@ -1,0 +1,306 @@
+// xdrop.hip -- banded x-drop extension of dCNS seeds on gfx950.
+//
+// Reference: x_extend_seed (findSimilarFragmentsForPairedSequence4.cpp:245-285)
+// runs SemiGlobal_xextend forward, score only (sequenceAlignment3.cpp:1234-1314),
+// from the seed start + 1, then backwards on reverse complements with
+// traceback (:1318-1451).  A row's band is j in [max(1,i-w), min(i+w,L2)];
+// a pass restarts with 2w when the row maximum sits on a band edge.
+//
+// One lane per seed.  The band is held band-relative in VGPRs: in band
+// coordinates k = j - (i - w) the diagonal predecessor of (i, k) is (i-1, k)
+// and the vertical one is (i-1, k+1), so rows advance without moving data.
+// Query bases ride in a byte window packed four per VGPR that shifts by one
+// byte per row.  Two passes:
+//   pass 1: forward + reverse, score only; reports the final band of each
+//           direction and the rows the final reverse pass evaluated;
+//   pass 2: the final reverse pass again, storing one traceback byte per
+//           band cell (rows x band, default 255 like Matrix::reset), then the
+//           traceback state machine writes the cigar.
+// A task whose band outgrows the kernel's register band is handed to the
+// next wider instantiation by the host.
+#include <hip/hip_runtime.h>
+
+#include "dp_common.h"
+#include "kernels.h"
+
+namespace dcns {
+namespace {
+
+struct Dir {
+    const int8_t *s1;  // forward: s1[x] = seq1[base1 + x]; reverse: 4 - seq1[base1 - x]
+    const int8_t *s2;
+    int32_t L1, L2;
+    bool rev;
+    __device__ __forceinline__ int c1(int x) const { return rev ? 4 - s1[-x] : s1[x]; }
+    __device__ __forceinline__ int c2(int x) const { return rev ? 4 - s2[-x] : s2[x]; }
+};
+
+__device__ __forceinline__ int32_t score(int a, int b, int32_t match, int32_t mism) {
+    return (a == 2 || b == 2) ? 0 : (a == b ? match : mism);
+}
+
+struct PassOut {
+    int32_t max, e1, e2;  // 1-based end (0 when none)
+    int32_t rows;         // rows evaluated
+    int64_t cells;
+    bool restart;
+};
+
+// One pass at half-width w (2w+1 <= BMAX).  With TB, row i's traceback bytes
+// are written to tb[i * BP + k] (k in band coordinates, 255 outside the band).
+template <int BMAX, bool TB>
+__device__ PassOut xpass(const Dir &D, int32_t w, const Params &P, uint8_t *__restrict__ tb, int32_t BP) {
+    constexpr int NQ = BMAX / 4 + 2;
+    int32_t Mp[BMAX], Fp[BMAX];
+    uint32_t qw[NQ];
+#pragma unroll
+    for (int k = 0; k < BMAX; ++k) {
+        Mp[k] = kOutOfBand;
+        Fp[k] = kOutOfBand;
+    }
+    // query window for row 1: columns j = 1 - w + k, code of column j is c2(j-1)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = 1 - w + 4 * q + b;
+            const int code = (j >= 1 && j <= D.L2) ? D.c2(j - 1) : 2;
+            v |= (uint32_t)(code & 0xff) << (8 * b);
+        }
+        qw[q] = v;
+    }
+    PassOut r{0, 0, 0, 0, 0, false};
+    const int32_t O = P.open, X = P.ext;
+    for (int32_t i = 1; i <= D.L1; ++i) {
+        const int32_t st = max(i - w, 1);
+        const int32_t en = min(i + w, D.L2);
+        const int32_t lo = i - w;  // column of band slot 0
+        const int c = D.c1(i - 1);
+        int32_t e = kOutOfBand;
+        int32_t rowMax = 0, rowJ = -1;
+        uint32_t tw[(BMAX + 3) / 4];
+#pragma unroll
+        for (int q = 0; q < (BMAX + 3) / 4; ++q) tw[q] = 0xffffffffu;
+        if (st <= en) r.cells += en - st + 1;
+#pragma unroll
+        for (int k = 0; k < BMAX; ++k) {
+            const int32_t j = lo + k;
+            const bool valid = (j >= st) && (j <= en);
+            const int qk = (qw[k >> 2] >> (8 * (k & 3))) & 0xff;
+            const int32_t s = score(c, qk, P.match, P.mismatch);
+            const int32_t diag = (j == 1) ? (i == 1 ? 0 : kOutOfBand) : Mp[k];
+            const int32_t fv = (k + 1 < BMAX && j != en) ? Fp[k + 1 < BMAX ? k + 1 : k] : kOutOfBand;
+            int32_t m = s + diag;
+            uint32_t d = 0;
+            if (!(m > fv)) { m = fv; d = 1; }
+            if (!(m > e)) { m = e; d = 2; }
+            const int32_t h = m + O;
+            int32_t f = fv + X;
+            if (f >= h) d |= 8; else f = h;
+            int32_t en2 = e + X;
+            if (en2 >= h) d |= 16; else en2 = h;
+            if (valid) {
+                if (m > rowMax) { rowMax = m; rowJ = j; }
+                e = en2;
+                Mp[k] = m;
+                Fp[k] = f;
+                if (TB) tw[k >> 2] = (tw[k >> 2] & ~(0xffu << (8 * (k & 3)))) | (d << (8 * (k & 3)));
+            }
+        }
+        if (TB) {
+            uint32_t *row = (uint32_t *)(tb + (int64_t)i * BP);
+#pragma unroll
+            for (int q = 0; q < (BMAX + 3) / 4; ++q)
+                if (4 * q < BP) row[q] = tw[q];
+        }
+        r.rows = i;
+        if (rowJ != D.L2 && rowJ != 1 && (rowJ == st || rowJ == en)) {
+            r.restart = true;
+            return r;
+        }
+        if (rowMax > r.max) { r.max = rowMax; r.e1 = i; r.e2 = rowJ; }
+        if (r.max - rowMax > P.xdrop) break;
+        if (rowJ <= 0) break;
+        // shift the query window one column right for row i+1
+        const int jn = lo + 4 * NQ;  // column entering slot 4*NQ-1 of row i+1
+        const int code = (jn >= 1 && jn <= D.L2) ? D.c2(jn - 1) : 2;
+#pragma unroll
+        for (int q = 0; q < NQ - 1; ++q) qw[q] = (qw[q] >> 8) | (qw[q + 1] << 24);
+        qw[NQ - 1] = (qw[NQ - 1] >> 8) | ((uint32_t)(code & 0xff) << 24);
+        // band slot k of row i+1 is column j+1: diagonal = Mp[k], vertical = Fp[k+1]
+    }
+    return r;
+}
+
+template <int BMAX, bool TB>
+__device__ PassOut xrun(const Dir &D, int32_t &w, const Params &P, uint8_t *tb, int32_t BP, int64_t &cells,
+                        bool &too_wide) {
+    too_wide = false;
+    while (true) {
+        if (2 * w + 1 > BMAX) {
+            too_wide = true;
+            return PassOut{0, 0, 0, 0, 0, false};
+        }
+        PassOut r = xpass<BMAX, TB>(D, w, P, tb, BP);
+        cells += r.cells;
+        if (!r.restart) return r;
+        w *= 2;
+    }
+}
+
+__device__ Dir forward_dir(const int8_t *seq, const PairDesc &pd, const XTask &t) {
+    Dir D;
+    D.s1 = seq + pd.off1 + t.start1;
+    D.s2 = seq + pd.off2 + t.start2;
+    D.L1 = pd.len1 - t.start1;
+    D.L2 = pd.len2 - t.start2;
+    D.rev = false;
+    return D;
+}
+
+// Reverse complements ending at (start1 + fe1, start2 + fe2), lengths
+// start+1+fe (:268-271).
+__device__ Dir reverse_dir(const int8_t *seq, const PairDesc &pd, const XTask &t, int32_t fe1, int32_t fe2) {
+    Dir D;
+    D.s1 = seq + pd.off1 + t.start1 + fe1;
+    D.s2 = seq + pd.off2 + t.start2 + fe2;
+    D.L1 = t.start1 + 1 + fe1;
+    D.L2 = t.start2 + 1 + fe2;
+    D.rev = true;
+    return D;
+}
+
+template <int BMAX>
+__global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                    const XTask *__restrict__ tasks, int32_t n, Params P,
+                                                    XOut *__restrict__ out) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const XTask t = tasks[gid];
+    XOut o = out[gid];
+    if (o.status == 0 && o.fw != 0) return;  // already finished by a narrower kernel
+    const PairDesc pd = pairs[t.pair];
+    bool wide;
+    if (o.status != 2) {  // forward pass still to do
+        int32_t w = o.fw > 0 ? o.fw : P.w;
+        const Dir F = forward_dir(seq, pd, t);
+        PassOut f = xrun<BMAX, false>(F, w, P, nullptr, 0, o.cells, wide);
+        if (wide) {
+            o.status = 1;
+            o.fw = w;
+            out[gid] = o;
+            return;
+        }
+        o.fmax = f.max;
+        o.fe1 = f.e1 - 1;
+        o.fe2 = f.e2 - 1;
+        o.fw = w;
+        o.rw = P.w;
+        if (o.fe1 < 0 || o.fe2 < 0) {
+            o.status = 0;
+            o.rmax = 0;
+            o.re1 = o.re2 = -1;
+            o.rrows = 0;
+            out[gid] = o;
+            return;
+        }
+    }
+    int32_t w = o.rw > 0 ? o.rw : P.w;
+    const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
+    PassOut r = xrun<BMAX, false>(R, w, P, nullptr, 0, o.cells, wide);
+    if (wide) {
+        o.status = 2;
+        o.rw = w;
+        out[gid] = o;
+        return;
+    }
+    o.status = 0;
+    o.rmax = r.max;
+    o.re1 = r.e1 - 1;
+    o.re2 = r.e2 - 1;
+    o.rw = w;
+    o.rrows = r.rows;
+    out[gid] = o;
+}
+
+__device__ __forceinline__ void push_op(uint32_t *cig, int32_t &n, uint32_t op) {
+    if (n == 0 || (cig[n - 1] & 0xfu) != op) cig[n++] = (1u << 4) | op;
+    else cig[n - 1] += 1u << 4;
+}
+
+// Pass 2: reverse pass with traceback at the final band from pass 1.
+template <int BMAX>
+__global__ __launch_bounds__(64) void k_xdrop_pass2(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                    const XTask *__restrict__ tasks, int32_t n, Params P,
+                                                    XOut *__restrict__ out, const int64_t *__restrict__ tb_off,
+                                                    uint8_t *__restrict__ tb, uint32_t *__restrict__ cigar) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const XTask t = tasks[gid];
+    XOut o = out[gid];
+    if (o.fe1 < 0 || o.fe2 < 0) return;
+    const int32_t w = o.rw;
+    // each task is traced by the narrowest instantiation that holds its band
+    if (2 * w + 1 > BMAX || (BMAX > 21 && 2 * w + 1 <= (BMAX + 1) / 2)) return;
+    const PairDesc pd = pairs[t.pair];
+    const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
+    const int32_t B = 2 * w + 1;
+    const int32_t BP = (B + 3) & ~3;
+    uint8_t *T = tb + tb_off[gid];
+    PassOut r = xpass<BMAX, true>(R, w, P, T, BP);
+    (void)r;
+    // traceback state machine (:1416-1438); cells outside the stored band read 255
+    int32_t ii = o.re1 + 1, jj = o.re2 + 1;
+    uint32_t *cig = cigar + o.cig_off;
+    int32_t nc = 0;
+    int state = 0;
+    while (ii > 0 && jj > 0) {
+        const int32_t k = jj - (ii - w);
+        int tv = 255;
+        if (k >= 0 && k < B && ii <= o.rrows) tv = T[(int64_t)ii * BP + k];
+        if (state == 0) state = tv & 7;
+        else if (!((tv >> (state + 2)) & 1)) state = 0;
+        if (state == 0) state = tv & 7;
+        if (state == 0) { push_op(cig, nc, 0); --ii; --jj; }
+        else if (state == 1 || state == 3) { push_op(cig, nc, 2); --ii; }
+        else { push_op(cig, nc, 1); --jj; }
+    }
+    o.iii = ii;
+    o.jjj = jj;
+    o.ncig = nc;
+    out[gid] = o;
+}
+
+}  // namespace
+
+hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                              int32_t n, const Params &P, XOut *out) {
+    if (n <= 0) return hipSuccess;
+    dim3 g((n + 63) / 64), b(64);
+    switch (band) {
+        case 21: hipLaunchKernelGGL(k_xdrop_pass1<21>, g, b, 0, st, seq, pairs, tasks, n, P, out); break;
+        case 41: hipLaunchKernelGGL(k_xdrop_pass1<41>, g, b, 0, st, seq, pairs, tasks, n, P, out); break;
+        case 81: hipLaunchKernelGGL(k_xdrop_pass1<81>, g, b, 0, st, seq, pairs, tasks, n, P, out); break;
+        case 161: hipLaunchKernelGGL(k_xdrop_pass1<161>, g, b, 0, st, seq, pairs, tasks, n, P, out); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                              int32_t n, const Params &P, XOut *out, const int64_t *tb_off, uint8_t *tb,
+                              uint32_t *cigar) {
+    if (n <= 0) return hipSuccess;
+    dim3 g((n + 63) / 64), b(64);
+    switch (band) {
+        case 21: hipLaunchKernelGGL(k_xdrop_pass2<21>, g, b, 0, st, seq, pairs, tasks, n, P, out, tb_off, tb, cigar); break;
+        case 41: hipLaunchKernelGGL(k_xdrop_pass2<41>, g, b, 0, st, seq, pairs, tasks, n, P, out, tb_off, tb, cigar); break;
+        case 81: hipLaunchKernelGGL(k_xdrop_pass2<81>, g, b, 0, st, seq, pairs, tasks, n, P, out, tb_off, tb, cigar); break;
+        case 161: hipLaunchKernelGGL(k_xdrop_pass2<161>, g, b, 0, st, seq, pairs, tasks, n, P, out, tb_off, tb, cigar); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace dcns

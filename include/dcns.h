@@ -1,0 +1,143 @@
+/* dcns.h -- C-ABI of the MI355X (gfx950) implementation of dCNS's alignment
+ * hot path.  Plain pointers and sizes only; no C++ or torch types cross this
+ * boundary.  Every entry point returns 0 on success and a negative code on
+ * failure; dcns_last_error() then describes the failure.  Nothing is
+ * silently computed on the CPU: when no usable gfx950 device is present,
+ * dcns_ctx_create fails.
+ *
+ * Sequences are int8 codes A0 C1 N2 G3 T4 (the encoding of the reference's
+ * SequenceCharToUInt8::seq_to_int8, impl/SequenceCharToUInt8.cpp:46-52),
+ * passed in host memory; the library stages them to device memory.
+ *
+ * Reference interfaces each entry point replaces (paths in the reference
+ * tree):
+ *   dcns_sw_batch            SmithWaterman(..., bool& positions)
+ *                            impl/sequenceAlignment3.cpp:220 (decl.
+ *                            impl/sequenceAlignment.h:26-28); caller
+ *                            impl/permutationLsqLambdaK.cpp:97
+ *   dcns_xdrop_extend        x_extend_seed (with traceback)
+ *                            impl/findSimilarFragmentsForPairedSequence4.cpp:245
+ *                            over SemiGlobal_xextend
+ *                            impl/sequenceAlignment3.cpp:1234 and :1318
+ *   dcns_seed_sets           the getSeeds window loop and link_seeds of
+ *                            impl/findSimilarFragmentsForPairedSequence4.cpp:61-201,
+ *                            288-344, 461-483
+ *   dcns_find_similar_batch  findSimilarFragmentsForPairedSequence (overload 1)
+ *                            impl/findSimilarFragmentsForPairedSequence4.cpp:433
+ *                            (decl. impl/findSimilarFragmentsForPairedSequence.h:26-32);
+ *                            callers controlLayer4.cpp:212, 1411
+ */
+#ifndef DCNS_H
+#define DCNS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DCNS_OK 0
+#define DCNS_ERR_ARG -1      /* invalid argument or unsupported parameter */
+#define DCNS_ERR_DEVICE -2   /* HIP runtime / device failure */
+#define DCNS_ERR_CAPACITY -3 /* caller-provided output buffer too small */
+#define DCNS_ERR_INTERNAL -4
+
+typedef struct dcns_ctx dcns_ctx;
+typedef struct dcns_frag_set dcns_frag_set;
+
+/* Scoring, seeding and significance parameters; defaults are the cut1Gap
+ * defaults of controlLayer4.cpp:28-52. */
+typedef struct dcns_params {
+    int32_t match, mismatch;        /* Scorei(match, mismatch)           (2, -3) */
+    int32_t open, ext;              /* gap open / extend, <= 0           (-4, -2) */
+    int32_t open2, ext2;            /* second gap piece (SemiGlobal)     (-45, 0) */
+    int32_t zdrop, bandwidth;       /* SemiGlobal z-drop and band        (50, 20000) */
+    int32_t window, step, min_score;/* seed window / step / min score    (38, 8, 40) */
+    int32_t w, xdrop;               /* x-drop band half-width and drop   (10, 20) */
+    int32_t pad;
+    double pvalue, lambda, k;       /* report p < pvalue; E = K*L1*L2*exp(-lambda*S) (0.1, 0.382291, 0.006662) */
+} dcns_params;
+
+/* A batch of sequence pairs in one host buffer of codes. */
+typedef struct dcns_batch {
+    const int8_t *seqs;
+    const int64_t *off1;  /* reference (seq1) offset per pair */
+    const int32_t *len1;
+    const int64_t *off2;  /* query (seq2) offset per pair */
+    const int32_t *len2;
+    int32_t n;
+    int32_t pad;
+} dcns_batch;
+
+/* Counters of the last pipeline call (reference-equivalent work). */
+typedef struct dcns_stats {
+    int64_t scan_cells;     /* DP cells of every SmithWaterman call the reference makes */
+    int64_t xdrop_cells;    /* band cells of every SemiGlobal_xextend pass the reference makes */
+    int64_t seeds;          /* seeds before link_seeds, summed over pairs */
+    int64_t linked_seeds;   /* seeds after link_seeds */
+    int64_t fragments;      /* reported fragments */
+    int64_t gpu_scan_cells; /* cells the GPU evaluated in the seed scan (incl. speculative work) */
+    int32_t levels;         /* recursion levels run on the device */
+    int32_t pad;
+    double ms_seed, ms_replay, ms_xdrop, ms_total; /* wall-clock phases */
+} dcns_stats;
+
+const char *dcns_version(void);
+void dcns_default_params(dcns_params *p);
+
+/* Context: one HIP device and stream, device buffers reused across calls.
+ * flags: bit 0 = record per-kernel HIP event timings. */
+int dcns_ctx_create(int device, uint32_t flags, dcns_ctx **out);
+int dcns_ctx_destroy(dcns_ctx *ctx);
+const char *dcns_last_error(const dcns_ctx *ctx);
+int dcns_device_info(dcns_ctx *ctx, int32_t *cu_count, int32_t *clock_mhz, char *name, int32_t name_cap);
+int dcns_last_stats(const dcns_ctx *ctx, dcns_stats *out);
+
+/* Per-kernel timing (flags bit 0): total milliseconds and launch count of the
+ * kernel family `name` ("scan_uniform", "scan_general", "resolve",
+ * "xdrop_pass1", "xdrop_pass2", "sw_batch") since the last reset. */
+int dcns_kernel_time(const dcns_ctx *ctx, const char *name, double *ms, int64_t *launches);
+int dcns_kernel_time_reset(dcns_ctx *ctx);
+
+/* SmithWaterman max score and first (ref-major) position for n pairs:
+ * out3[3*i] = max, out3[3*i+1] = end1, out3[3*i+2] = end2 (0-based, -1 when
+ * the max is 0). */
+int dcns_sw_batch(dcns_ctx *ctx, const dcns_batch *b, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
+                  int32_t *out3);
+
+/* x_extend_seed (with traceback) for n seeds.  seeds[2*i] = start1,
+ * seeds[2*i+1] = start2 (the seed start + 1, as the reference passes them);
+ * pair_of_seed[i] indexes b.  out_ints (6 per seed): maxScore, start1, end1,
+ * start2, end2 (0-based, after the traceback trim), ok (1 = both passes
+ * found a positive cell, else the reference's p = 100 case).  Cigars are
+ * written to cigar (capacity cig_cap ops) with cig_off[n+1] offsets. */
+int dcns_xdrop_extend(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, const int32_t *pair_of_seed,
+                      const int32_t *seeds, int32_t n, int32_t *out_ints, uint32_t *cigar, int64_t *cig_off,
+                      int64_t cig_cap);
+
+/* The seed set of every pair (window loop + getSeeds recursion + std::set
+ * semantics), optionally after link_seeds.  counts[i] = seeds of pair i;
+ * seeds receives 4 int32 per seed (start1, end1, start2, end2, 0-based) in
+ * set order, pair after pair.  Returns DCNS_ERR_CAPACITY (with counts
+ * filled) when cap seeds do not fit. */
+int dcns_seed_sets(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, int32_t linked, int32_t *counts,
+                   int32_t *seeds, int64_t cap);
+
+/* findSimilarFragmentsForPairedSequence (overload 1) for every pair. */
+int dcns_find_similar_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, dcns_frag_set **out);
+int64_t dcns_frag_set_total(const dcns_frag_set *fs);
+/* Fragments of one pair occupy [first, first + count) in the set. */
+int dcns_frag_set_pair(const dcns_frag_set *fs, int32_t pair, int64_t *first, int64_t *count, int64_t *seeds_size);
+/* Fragment idx: ints5 = start1, end1, start2, end2 (1-based as stored by
+ * PairedSimilarFragment), score; dbls2 = pValue, eValue. *ncig receives the
+ * cigar length; up to cig_cap ops (len << 4 | op, op 0 M, 1 I, 2 D) are
+ * copied to cigar. */
+int dcns_frag_set_get(const dcns_frag_set *fs, int64_t idx, int32_t *ints5, double *dbls2, uint32_t *cigar,
+                      int32_t cig_cap, int32_t *ncig);
+void dcns_frag_set_free(dcns_frag_set *fs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DCNS_H */

@@ -1,0 +1,117 @@
+"""Seed sets, x-drop extension and the whole findSimilarFragmentsForPairedSequence
+path on the GPU against the oracle (which is pinned to the compiled reference)."""
+import numpy as np
+import pytest
+
+import dcns_amd
+from dcns_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_pairs(seed, n, lo=300, hi=3000, nfrag=(0, 12)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(n):
+        L1 = int(rng.integers(lo, hi))
+        L2 = int(rng.integers(lo // 3, hi))
+        out.append(synth.one_pair(seed=seed * 1000 + t, L1=L1, L2=L2, nfrag=int(rng.integers(*nfrag))))
+    return out
+
+
+@pytest.mark.parametrize("linked", [False, True])
+def test_seed_sets_match_oracle(gpu, oracle_lib, linked):
+    pairs = _small_pairs(11, 24)
+    print("seed_sets linked=%d" % linked, flush=True)
+    got = gpu.seed_sets(pairs, linked=linked)
+    for i, (a, b) in enumerate(pairs):
+        assert got[i] == oracle_lib.seed_set(a, b, linked=linked), i
+
+
+def test_seed_sets_repeats_and_edges(gpu, oracle_lib):
+    """Tandem repeats (deep recursion), short queries, partial windows, N runs."""
+    rng = np.random.default_rng(5)
+    unit = synth.random_seq(rng, 60)
+    rep = np.concatenate([synth.mutate(rng, unit, 0.05, 0.0)[:60] for _ in range(40)])
+    q = synth.mutate(rng, unit, 0.02, 0.0)
+    withn = synth.random_seq(rng, 900)
+    withn[100:160] = 2
+    pairs = [(rep, q), (rep, rep[:500].copy()), (synth.random_seq(rng, 500), synth.random_seq(rng, 7)),
+             (withn, withn[50:400].copy()), (rep[:30].copy(), rep[:45].copy()), (q, q.copy())]
+    print("seed_sets edges", flush=True)
+    for linked in (False, True):
+        got = gpu.seed_sets(pairs, linked=linked)
+        for i, (a, b) in enumerate(pairs):
+            assert got[i] == oracle_lib.seed_set(a, b, linked=linked), (i, linked)
+
+
+def _x_extend_oracle(o, s1, s2, start1, start2, sc=(2, -3, -4, -2), xdrop=20, w=10):
+    """x_extend_seed (tb) composed from the oracle's two SemiGlobal_xextend calls."""
+    f = o.xextend(s1[start1:], s2[start2:], sc, xdrop, w)
+    if f[1] < 0 or f[2] < 0:
+        return (f[0], start1, 0, start2, 0, 0, None)
+    r1 = (4 - s1[:start1 + f[1] + 1][::-1]).astype(np.int8)
+    r2 = (4 - s2[:start2 + f[2] + 1][::-1]).astype(np.int8)
+    (mx, e1, e2, iii, jjj), cig = o.xextend_tb(r1, r2, sc, xdrop, w)
+    if e1 < 0 or e2 < 0:
+        return (mx, start1, 0, start2, 0, 0, cig)
+    end1 = start1 + f[1] - iii
+    end2 = start2 + f[2] - jjj
+    return (mx, start1 + f[1] - e1, end1, start2 + f[2] - e2, end2, 1, cig)
+
+
+def test_xdrop_extend_matches_oracle(gpu, oracle_lib):
+    pairs = _small_pairs(21, 6, 500, 2500, (5, 15))
+    rng = np.random.default_rng(9)
+    pos, starts = [], []
+    for i, (a, b) in enumerate(pairs):
+        for s in oracle_lib.seed_set(a, b, linked=True)[:40]:
+            pos.append(i)
+            starts.append((s[0] + 1, s[2] + 1))
+        for _ in range(20):  # arbitrary starts, including near the ends
+            pos.append(i)
+            starts.append((int(rng.integers(0, len(a) + 1)), int(rng.integers(0, len(b) + 1))))
+    print("xdrop", len(pos), flush=True)
+    got = gpu.xdrop_extend(pairs, pos, starts)
+    for k, (i, (s1, s2)) in enumerate(zip(pos, starts)):
+        a, b = pairs[i]
+        want = _x_extend_oracle(oracle_lib, a, b, s1, s2)
+        g = got[k]
+        if want[5] == 0:
+            assert g[5] == 0, (k, g[:6], want[:6])
+            continue
+        assert g[:6] == want[:6], (k, g[:6], want[:6])
+        assert g[6] == want[6], k
+
+
+def _cmp_fragments(got, want, where):
+    assert len(got) == len(want), (where, len(got), len(want))
+    for g, w in zip(got, want):
+        assert g[:5] == w[:5], (where, g[:5], w[:5])
+        assert g[5] == w[5] and g[6] == w[6], (where, g[5:7], w[5:7])
+        assert g[7] == w[7], where
+
+
+def test_find_similar_matches_oracle(gpu, oracle_lib):
+    pairs = _small_pairs(31, 16, 400, 4000, (0, 20))
+    print("find_similar small", flush=True)
+    got = gpu.find_similar(pairs)
+    for i, (a, b) in enumerate(pairs):
+        st = {}
+        want = oracle_lib.find_similar(a, b, stats=st)
+        assert got[i][0] == st["seeds"], i
+        _cmp_fragments(got[i][1], want, i)
+
+
+def test_find_similar_one_pair_10kb(gpu, oracle_lib):
+    """The plain-method workload: 10 kb x 10 kb, 20 planted fragments."""
+    a, b = synth.one_pair()
+    print("find_similar 10kb", flush=True)
+    got = gpu.find_similar([(a, b)])
+    st = {}
+    want = oracle_lib.find_similar(a, b, stats=st)
+    assert got[0][0] == st["seeds"]
+    _cmp_fragments(got[0][1], want, "10kb")
+    s = gpu.stats()
+    assert s["scan_cells"] == st["scan_cells"]
+    assert s["xdrop_cells"] == st["xdrop_cells"]
