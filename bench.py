@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched dCNS alignment hot path on one MI355X.
+
+Workload: 4,096 synthetic sequence pairs, reference and query lengths drawn
+independently log-uniform on 1-50 kb, GC U[0.40, 0.60], 0-30 planted conserved
+fragments per pair (dcns_amd.synth.batch), cut1Gap default scoring.  A step
+is one findSimilarFragmentsForPairedSequence pass over the whole batch (seed
+scan + recursion + link + x-drop + traceback + p-value + dedup) with the
+sequences already resident in HBM.
+
+Printed (one JSON line): GCUPS over the reference-equivalent DP cells (every
+cell the reference algorithm evaluates, counted exactly), pairs/s, the
+seed-scan kernel's roofline against the packed-int16 VALU peak, the same
+pairs run one pair per call (the plain GPU path), and the compiled reference
+timed over the host's cores on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GCUPS & pairs/s, batched vs plain GPU SW; λ/K surrogate speedup at ≤2% err"
+VALU_OPS_PER_CELL = 10  # add (substitution), 3 max (H), 2 add + 1 max (E), 1 add + 1 max (F), 1 max (running max)
+
+
+def window_cells(len1, len2, W=38, step=8):
+    """DP cells of the level-0 window scans: sum over windows of L1 * window width."""
+    total = 0
+    for L1, L2 in zip(len1, len2):
+        sp, widths = 0, [min(W, L2)]
+        while True:
+            sp += step
+            if sp > L2:
+                break
+            widths.append(min(W, L2 - sp))
+            if sp + W > L2:
+                break
+        total += int(L1) * int(sum(widths))
+    return total
+
+
+def _cpu_worker_init():
+    # the reference prints a "seeds size" line per pair; keep stdout for the JSON line
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+
+
+def _cpu_worker(args):
+    a, b = args
+    from oracle import bind
+    ref = bind.reference()
+    chk = ref if ref is not None else bind.oracle()
+    return chk.find_similar(a, b)
+
+
+def cpu_baseline(ctx, pairs, budget_s, cores):
+    """The compiled reference (oracle/_ref) on a bounded sample of the batch,
+    one single-threaded pair per worker over `cores` workers, as the reference
+    is deployed (one process per region).  Returns (json dict, parity flag)."""
+    import multiprocessing as mp
+    from oracle import bind
+    kind = "reference" if bind.reference() is not None else "port"
+    rng = np.random.default_rng(7)
+    order = rng.permutation(len(pairs))
+    # calibrate the single-core rate on a mid-size pair
+    sizes = np.array([len(pairs[i][0]) * len(pairs[i][1]) for i in order])
+    probe = int(order[int(np.argmin(np.abs(sizes - 4e7)))])
+    ctx.find_similar([pairs[probe]], with_cigar=False)
+    st = ctx.stats()
+    probe_cells = st["scan_cells"] + st["xdrop_cells"]
+    with mp.Pool(1, initializer=_cpu_worker_init) as pool:
+        t = time.perf_counter()
+        pool.map(_cpu_worker, [pairs[probe]])
+        rate = probe_cells / (time.perf_counter() - t)
+    # sample: pairs in a fixed random order until ~budget_s * cores of estimated work
+    sample, est = [], 0.0
+    for i in order:
+        if est >= budget_s * cores:
+            break
+        L1, L2 = len(pairs[i][0]), len(pairs[i][1])
+        est += 5.0 * L1 * L2 / rate
+        sample.append(int(i))
+    sub = [pairs[i] for i in sample]
+    with mp.Pool(cores, initializer=_cpu_worker_init) as pool:
+        t = time.perf_counter()
+        cpu_out = pool.map(_cpu_worker, sub, chunksize=1)
+        wall = time.perf_counter() - t
+    gpu_out = ctx.find_similar(sub)
+    st = ctx.stats()
+    cells = st["scan_cells"] + st["xdrop_cells"]
+    identical = all(g[1] == c for g, c in zip(gpu_out, cpu_out))
+    return {"value": round(cells / wall / 1e9, 4), "unit": "GCUPS", "cores": cores, "kind": kind,
+            "pairs_per_s": round(len(sub) / wall, 3),
+            "sample": f"{len(sub)} of {len(pairs)} pairs (fixed random order), {cells:.3e} cells, "
+                      f"{wall:.1f} s wall, one single-threaded pair per worker"}, identical
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=4096)
+    ap.add_argument("--plain-pairs", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-cores", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (pairs shard across GPUs as "
+              "independent batches; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import dcns_amd
+    from dcns_amd import synth
+
+    pairs = synth.batch(npairs=args.pairs)
+    len1 = np.array([len(a) for a, _ in pairs])
+    len2 = np.array([len(b) for _, b in pairs])
+    ctx = dcns_amd.Context(0, timing=True)
+    dev = ctx.device_info()
+
+    # first call including host->device staging (PCIe-inclusive rate, not the headline)
+    t = time.perf_counter()
+    ctx.stage(pairs)
+    ctx.find_similar_staged(collect=False)
+    pcie_s = time.perf_counter() - t
+    for _ in range(max(args.warmup - 1, 0)):
+        ctx.find_similar_staged(collect=False)
+    ctx.kernel_time_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nfrag = ctx.find_similar_staged(collect=False)
+    t1 = time.perf_counter()
+    st = ctx.stats()
+    dt = (t1 - t0) / args.steps
+    cells = st["scan_cells"] + st["xdrop_cells"]
+    gcups = cells / dt / 1e9
+
+    # roofline of the dominant kernel (level-0 window scan), HIP events on the library's stream
+    ms, launches = ctx.kernel_time("scan_uniform")
+    avg_ms = ms / max(launches, 1)
+    l0 = window_cells(len1, len2)
+    achieved = l0 * VALU_OPS_PER_CELL / (avg_ms * 1e-3) / 1e12
+    peak = dev["cu"] * 4 * 16 * 2 * dev["clock_mhz"] * 1e6 / 1e12  # packed int16 VALU lane-ops
+    kernels = {}
+    for k in ("scan_uniform", "resolve", "scan_general", "xdrop_pass1", "xdrop_pass2", "gather"):
+        kms, kn = ctx.kernel_time(k)
+        if kn:
+            kernels[k] = {"ms_per_step": round(kms / args.steps, 3), "launches_per_step": kn / args.steps}
+
+    line = {
+        "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+        "config": {"workload": f"batched {args.pairs} pairs, log-uniform 1-50 kb, GC 0.40-0.60, 0-30 planted "
+                               "fragments, cut1Gap defaults (window 38, step 8, min seed 40, x-drop 20)",
+                   "pairs": args.pairs, "reference_cells_per_step": cells, "fragments": int(nfrag),
+                   "seeds": st["seeds"], "linked_seeds": st["linked_seeds"], "levels": st["levels"]},
+        "pairs_per_s": round(args.pairs / dt, 1),
+        "phases_ms": {"seed": round(st["ms_seed"], 1), "xdrop": round(st["ms_xdrop"], 1),
+                      "dedup": round(st["ms_replay"], 1)},
+        "kernels": kernels,
+        "pcie_inclusive_gcups": round(cells / pcie_s / 1e9, 2),
+        "roofline": {"bound": "valu", "kernel": "k_scan_uniform", "achieved": round(achieved, 3),
+                     "peak": round(peak, 2), "unit": "TOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "work_per_launch": f"{l0:.4e} cells x {VALU_OPS_PER_CELL} int16 ops",
+                     "avg_launch_ms": round(avg_ms, 3)},
+    }
+
+    if not args.no_plain:  # the plain path: one pair per call, host buffers in, fragments out
+        rng = np.random.default_rng(11)
+        idx = rng.choice(len(pairs), size=min(args.plain_pairs, len(pairs)), replace=False)
+        tot_cells, tot_t = 0, 0.0
+        for i in idx:
+            t = time.perf_counter()
+            ctx.find_similar([pairs[int(i)]], with_cigar=False)
+            tot_t += time.perf_counter() - t
+            s = ctx.stats()
+            tot_cells += s["scan_cells"] + s["xdrop_cells"]
+        plain = tot_cells / tot_t / 1e9
+        line["plain_gpu"] = {"gcups": round(plain, 2), "pairs_per_s": round(len(idx) / tot_t, 2),
+                             "sample": f"{len(idx)} random pairs of the batch, one dcns_find_similar_batch "
+                                       "call each (staging included)"}
+        line["batched_vs_plain"] = round(gcups / plain, 2)
+
+    if not args.no_cpu:
+        cores = args.cpu_cores or min(len(os.sched_getaffinity(0)), 16)
+        cb, same = cpu_baseline(ctx, pairs, args.cpu_seconds, cores)
+        line["cpu_baseline"] = cb
+        line["cpu_sample_bit_exact"] = bool(same)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

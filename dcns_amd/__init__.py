@@ -67,6 +67,8 @@ def load_library():
     lib.dcns_xdrop_extend.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _I, _P, _P, _P, _L]
     lib.dcns_seed_sets.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _I, _P, _P, _L]
     lib.dcns_find_similar_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), ctypes.POINTER(_P)]
+    lib.dcns_stage.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params)]
+    lib.dcns_find_similar_staged.argtypes = [_P, ctypes.POINTER(Params), ctypes.POINTER(_P)]
     lib.dcns_frag_set_total.restype = _L
     lib.dcns_frag_set_total.argtypes = [_P]
     lib.dcns_frag_set_pair.argtypes = [_P, _I, ctypes.POINTER(_L), ctypes.POINTER(_L), ctypes.POINTER(_L)]
@@ -79,7 +81,8 @@ def load_library():
 # Entry points declared in include/dcns.h (the library must export all of them).
 EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_destroy", "dcns_last_error",
            "dcns_device_info", "dcns_last_stats", "dcns_kernel_time", "dcns_kernel_time_reset", "dcns_sw_batch",
-           "dcns_xdrop_extend", "dcns_seed_sets", "dcns_find_similar_batch", "dcns_frag_set_total",
+           "dcns_xdrop_extend", "dcns_seed_sets", "dcns_find_similar_batch", "dcns_stage",
+           "dcns_find_similar_staged", "dcns_frag_set_total",
            "dcns_frag_set_pair", "dcns_frag_set_get", "dcns_frag_set_free"]
 
 
@@ -237,13 +240,34 @@ class Context:
         fs = _P()
         self._check(self.lib.dcns_find_similar_batch(self.h, ctypes.byref(ba.b), ctypes.byref(p), ctypes.byref(fs)),
                     "dcns_find_similar_batch")
+        return self._collect(fs, ba.b.n, with_cigar)
+
+    def stage(self, pairs=None, params=None, flat=None):
+        """Keeps a batch resident in device memory for find_similar_staged()."""
+        ba = _BatchArrays.from_flat(*flat) if flat is not None else _BatchArrays(pairs)
+        p = params or default_params()
+        self._check(self.lib.dcns_stage(self.h, ctypes.byref(ba.b), ctypes.byref(p)), "dcns_stage")
+        self._staged_n = ba.b.n
+
+    def find_similar_staged(self, params=None, collect=True, with_cigar=True):
+        p = params or default_params()
+        fs = _P()
+        self._check(self.lib.dcns_find_similar_staged(self.h, ctypes.byref(p), ctypes.byref(fs)),
+                    "dcns_find_similar_staged")
+        if not collect:
+            n = self.lib.dcns_frag_set_total(fs)
+            self.lib.dcns_frag_set_free(fs)
+            return n
+        return self._collect(fs, self._staged_n, with_cigar)
+
+    def _collect(self, fs, npairs, with_cigar):
         try:
             res = []
             ints = np.zeros(5, np.int32)
             dbls = np.zeros(2, np.float64)
             cig = np.zeros(1 << 16, np.uint32)
             nc = _I()
-            for i in range(ba.b.n):
+            for i in range(npairs):
                 first, count, ss = _L(), _L(), _L()
                 self.lib.dcns_frag_set_pair(fs, i, ctypes.byref(first), ctypes.byref(count), ctypes.byref(ss))
                 frs = []

@@ -194,6 +194,7 @@ int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t misma
             d.ensure(std::max<size_t>(bytes, 64));
             hip_check(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
         };
+        c->staged.reset();
         up(c->seq, host.data(), host.size());
         up(c->sw_off1, o1.data(), o1.size() * 8);
         up(c->sw_len1, l1.data(), l1.size() * 4);
@@ -240,6 +241,7 @@ int dcns_xdrop_extend(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, co
             throw Error(DCNS_ERR_ARG, "invalid seed arrays");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         Params P = to_params(*p);
+        c->staged.reset();
         Staged S = stage_batch(c, *b, P);
         std::vector<XTask> tasks((size_t)n);
         for (int32_t i = 0; i < n; ++i) {
@@ -283,6 +285,7 @@ int dcns_seed_sets(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, int32
         validate_params(*p);
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         Params P = to_params(*p);
+        c->staged.reset();
         Staged S = stage_batch(c, *b, P);
         SeedResult sr;
         seed_sets(c, S, P, !linked, sr);
@@ -320,7 +323,39 @@ int dcns_find_similar_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params 
         check_batch(b);
         if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
+        c->staged.reset();
         find_similar(c, *b, *p, fs->fs);
+    });
+    if (rc != DCNS_OK) {
+        delete fs;
+        return rc;
+    }
+    *out = fs;
+    return DCNS_OK;
+}
+
+int dcns_stage(dcns_ctx *c, const dcns_batch *b, const dcns_params *p) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        check_batch(b);
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        validate_params(*p);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        c->staged = std::make_shared<Staged>(stage_batch(c, *b, to_params(*p)));
+        c->sync();
+    });
+}
+
+int dcns_find_similar_staged(dcns_ctx *c, const dcns_params *p, dcns_frag_set **out) {
+    if (!c || !out) return DCNS_ERR_ARG;
+    *out = nullptr;
+    dcns_frag_set *fs = new (std::nothrow) dcns_frag_set();
+    if (!fs) return DCNS_ERR_INTERNAL;
+    int rc = guarded(c, [&] {
+        if (!c->staged) throw Error(DCNS_ERR_ARG, "no staged batch: call dcns_stage first");
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        find_similar_staged(c, *c->staged, *p, fs->fs);
     });
     if (rc != DCNS_OK) {
         delete fs;

@@ -4,6 +4,7 @@
 
 #include <cstring>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -58,8 +59,13 @@ struct KernelTimer {
 
 }  // namespace dcns
 
+namespace dcns {
+struct Staged;
+}
+
 struct dcns_ctx {
     int device = 0;
+    std::shared_ptr<dcns::Staged> staged;  // sequences kept resident by dcns_stage
     hipStream_t stream = nullptr;
     bool timing = false;
     std::string err;

@@ -117,9 +117,26 @@ Staged stage_batch(dcns_ctx *c, const dcns_batch &b, const Params &P) {
         nw += d.nwin;
     }
     S.nwin = nw;
+    S.window = P.window;
+    S.step = P.step;
     upload(c, c->seq, host);
     upload(c, c->pairs, S.pd);
     return S;
+}
+
+void relayout(dcns_ctx *c, Staged &S, const Params &P) {
+    if (S.window == P.window && S.step == P.step) return;
+    int64_t nw = 0;
+    for (PairDesc &d : S.pd) {
+        d.nwin = window_count(d.len2, P.window, P.step);
+        if (nw + d.nwin > INT32_MAX / 2) throw Error(DCNS_ERR_ARG, "batch has too many windows");
+        d.win_base = (int32_t)nw;
+        nw += d.nwin;
+    }
+    S.nwin = nw;
+    S.window = P.window;
+    S.step = P.step;
+    upload(c, c->pairs, S.pd);
 }
 
 // ---------------------------------------------------------------------------
@@ -460,15 +477,22 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
 // ---------------------------------------------------------------------------
 void find_similar(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out) {
     validate_params(p);
+    Staged S = stage_batch(c, b, to_params(p));
+    find_similar_staged(c, S, p, out);
+}
+
+void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &out) {
+    validate_params(p);
     const Params P = to_params(p);
+    relayout(c, S, P);
+    const int32_t npairs = (int32_t)S.pd.size();
     const double t0 = now_ms();
-    Staged S = stage_batch(c, b, P);
     SeedResult sr;
     seed_sets(c, S, P, false, sr);
     const double t1 = now_ms();
     std::vector<XTask> tasks;
-    std::vector<int64_t> first_task((size_t)b.n + 1, 0);
-    for (int32_t i = 0; i < b.n; ++i) {
+    std::vector<int64_t> first_task((size_t)npairs + 1, 0);
+    for (int32_t i = 0; i < npairs; ++i) {
         first_task[(size_t)i] = (int64_t)tasks.size();
         for (const Seed &s : sr.linked[(size_t)i]) {
             XTask t;
@@ -479,18 +503,18 @@ void find_similar(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSe
             tasks.push_back(t);
         }
     }
-    first_task[(size_t)b.n] = (int64_t)tasks.size();
+    first_task[(size_t)npairs] = (int64_t)tasks.size();
     std::vector<XResult> xr;
     int64_t xcells = 0;
     xdrop_extend(c, S, P, tasks, xr, xcells);
     const double t2 = now_ms();
 
     out.frags.clear();
-    out.first.assign((size_t)b.n, 0);
-    out.count.assign((size_t)b.n, 0);
+    out.first.assign((size_t)npairs, 0);
+    out.count.assign((size_t)npairs, 0);
     out.seeds_size = sr.before_link;
-    for (int32_t i = 0; i < b.n; ++i) {
-        const double L1d = (double)b.len1[i], L2d = (double)b.len2[i];
+    for (int32_t i = 0; i < npairs; ++i) {
+        const double L1d = (double)S.pd[(size_t)i].len1, L2d = (double)S.pd[(size_t)i].len2;
         std::vector<PairedSimilarFragment> fr;
         for (int64_t k = first_task[(size_t)i]; k < first_task[(size_t)i + 1]; ++k) {
             const XResult &r = xr[(size_t)k];

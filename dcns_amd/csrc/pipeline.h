@@ -18,8 +18,11 @@ Params to_params(const dcns_params &p);
 struct Staged {
     std::vector<PairDesc> pd;
     int64_t nwin = 0;  // windows over all pairs
+    int32_t window = 0, step = 0;  // schedule the window layout was made for
 };
 Staged stage_batch(dcns_ctx *c, const dcns_batch &b, const Params &P);
+// Recomputes the window layout of staged pairs for another window/step.
+void relayout(dcns_ctx *c, Staged &S, const Params &P);
 
 // Seed sets of every pair (set order), before and after link_seeds.
 struct SeedResult {
@@ -48,6 +51,8 @@ struct FragSet {
     std::vector<int64_t> first, count, seeds_size;
 };
 void find_similar(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out);
+// Same on sequences already resident on the device (dcns_stage).
+void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &out);
 
 }  // namespace dcns
 

@@ -26,29 +26,40 @@ SEED_BASE = 20261203
 
 
 def random_seq(rng, n, gc=0.5):
-    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
-    return BASES[rng.choice(4, size=n, p=p)]
+    """i.i.d. bases with P(C) = P(G) = gc/2 and P(A) = P(T) = (1-gc)/2."""
+    u = rng.random(n)
+    a, c, g = (1 - gc) / 2, gc / 2, gc / 2
+    idx = (u >= a).astype(np.int64) + (u >= a + c) + (u >= a + c + g)
+    return BASES[idx]
 
 
 def mutate(rng, frag, p_sub, p_indel, gc=0.5):
-    """Substitutions (to a different base) and 1-5 bp indels."""
-    out = []
-    i = 0
+    """Substitutions (to a different base, probability p_sub per base) and
+    indel events (probability p_indel per base, half insertions of 1-5 random
+    bases before the base, half deletions of 1-5 bases from it)."""
+    frag = np.asarray(frag, dtype=np.int8)
     n = len(frag)
-    while i < n:
-        r = rng.random()
-        if r < p_indel / 2:  # deletion of 1-5 bp
-            i += int(rng.integers(1, 6))
-            continue
-        if r < p_indel:  # insertion of 1-5 bp
-            out.extend(random_seq(rng, int(rng.integers(1, 6)), gc).tolist())
-        b = int(frag[i])
-        if b != 2 and rng.random() < p_sub:
-            choices = [x for x in (0, 1, 3, 4) if x != b]
-            b = choices[int(rng.integers(0, 3))]
-        out.append(b)
-        i += 1
-    return np.array(out, dtype=np.int8)
+    if n == 0:
+        return frag.copy()
+    ev = np.flatnonzero(rng.random(n) < p_indel)
+    is_del = rng.random(len(ev)) < 0.5
+    lens = rng.integers(1, 6, len(ev))
+    keep = np.ones(n, dtype=bool)
+    for i, ln in zip(ev[is_del], lens[is_del]):
+        keep[i:i + ln] = False
+    out = frag.copy()
+    cls = np.array([0, 1, 0, 2, 3])[np.clip(out, 0, 4)]
+    sub = (rng.random(n) < p_sub) & (out != 2)
+    shifted = BASES[(cls + rng.integers(1, 4, n)) % 4]
+    out = np.where(sub, shifted, out).astype(np.int8)
+    ins_at = ev[~is_del]
+    ins_len = lens[~is_del]
+    if len(ins_at):
+        pos = np.repeat(ins_at, ins_len)
+        vals = random_seq(rng, int(ins_len.sum()), gc)
+        out = np.insert(out, pos, vals)
+        keep = np.insert(keep, pos, True)
+    return out[keep].astype(np.int8)
 
 
 def plant(rng, s1, s2, nfrag, lo, hi, p_sub, p_indel, gc=0.5):

@@ -125,6 +125,12 @@ int dcns_seed_sets(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, int
 
 /* findSimilarFragmentsForPairedSequence (overload 1) for every pair. */
 int dcns_find_similar_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, dcns_frag_set **out);
+/* Keeps a batch resident in device memory (replacing any earlier one; the
+ * other entry points that take a batch drop it) and runs
+ * dcns_find_similar_batch on it without re-uploading.  No reference
+ * counterpart: the reference reads each region per process. */
+int dcns_stage(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p);
+int dcns_find_similar_staged(dcns_ctx *ctx, const dcns_params *p, dcns_frag_set **out);
 int64_t dcns_frag_set_total(const dcns_frag_set *fs);
 /* Fragments of one pair occupy [first, first + count) in the set. */
 int dcns_frag_set_pair(const dcns_frag_set *fs, int32_t pair, int64_t *first, int64_t *count, int64_t *seeds_size);
