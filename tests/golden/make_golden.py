@@ -1,0 +1,201 @@
+"""Writes the golden fixtures in this folder from the compiled reference.
+
+Run in a container where the reference sources are present:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Every fixture holds inputs (sequences as A/C/N/G/T text, parameters) and the
+outputs of the reference build (oracle/_ref) on them: K1 SmithWaterman
+(striped and textbook paths), SemiGlobal_xextend with and without traceback,
+the two-piece SemiGlobal, the seed set before/after link_seeds, the fragments
+of findSimilarFragmentsForPairedSequence, and the text outputs of the
+cut1Gap and ranSco commands.  Only data is written.
+"""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from dcns_amd import synth  # noqa: E402
+from oracle import bind  # noqa: E402
+
+
+def txt(a):
+    return synth.to_text(a)
+
+
+def main():
+    ref = bind.reference()
+    if ref is None:
+        sys.exit("oracle/_ref is not built")
+    rng = np.random.default_rng(20261203)
+    scorings = [(2, -3, -4, -2), (1, -1, -2, -1), (3, -4, -6, -3), (1, -4, -2, -3), (2, -1, -2, -1)]
+
+    # K1
+    k1 = []
+    for t in range(160):
+        sc = scorings[t % len(scorings)]
+        L1, L2 = int(rng.integers(0, 90)), int(rng.integers(0, 50))
+        a = rng.choice([0, 1, 2, 3, 4], L1, p=[.24, .24, .04, .24, .24]).astype(np.int8)
+        b = rng.choice([0, 1, 2, 3, 4], L2, p=[.24, .24, .04, .24, .24]).astype(np.int8)
+        if L1 > 20 and L2 > 10 and t % 2:
+            k = int(rng.integers(0, L1 - 10))
+            m = synth.mutate(rng, a[k:k + L2], 0.1, 0.05)[:L2]
+            b[:len(m)] = m
+        out = list(ref.sw(a, b, sc))
+        if out[0] == 0:
+            out[1] = out[2] = -1  # positions are not written by the reference when nothing scores
+        k1.append({"ref": txt(a), "read": txt(b), "sc": sc, "textbook": False, "out": out})
+        k1.append({"ref": txt(a), "read": txt(b), "sc": sc, "textbook": True, "out": list(ref.sw(a, b, sc, True))})
+    # long pairs past the int16 bound (textbook path inside SmithWaterman)
+    a = synth.random_seq(rng, 11500)
+    b = synth.mutate(rng, a, 0.05, 0.01)[:11200]
+    k1.append({"ref": txt(a), "read": txt(b), "sc": (1, -2, -5, -3), "textbook": False,
+               "out": list(ref.sw(a, b, (1, -2, -5, -3)))})
+    json.dump(k1, open(os.path.join(HERE, "k1_smithwaterman.json"), "w"))
+
+    # K2 / K3 / K4
+    ext = []
+    for t in range(120):
+        sc = scorings[t % len(scorings)]
+        L1, L2 = int(rng.integers(1, 260)), int(rng.integers(1, 260))
+        a = rng.choice([0, 1, 3, 4], L1).astype(np.int8)
+        b = (synth.mutate(rng, a, 0.15, 0.04)[:L2] if t % 4 else rng.choice([0, 1, 3, 4], L2).astype(np.int8))
+        if len(b) == 0:
+            continue
+        w, xd = int(rng.integers(1, 15)), int(rng.integers(5, 40))
+        sc2 = sc + (-int(rng.integers(10, 50)), -int(rng.integers(0, 2)))
+        z, w2 = int(rng.integers(10, 80)), int(rng.integers(3, 400))
+        tb, cig = ref.xextend_tb(a, b, sc, xd, w)
+        sg, sgc = ref.semiglobal(a, b, sc2, z, w2)
+        ext.append({"s1": txt(a), "s2": txt(b), "sc": sc, "w": w, "xdrop": xd,
+                    "xextend": list(ref.xextend(a, b, sc, xd, w)), "xextend_tb": list(tb), "cigar": cig,
+                    "sc2": sc2, "zdrop": z, "bandwidth": w2, "semiglobal": list(sg), "semiglobal_cigar": sgc})
+    json.dump(ext, open(os.path.join(HERE, "k234_extension.json"), "w"))
+
+    # seeds and fragments
+    pipe = []
+    for t in range(10):
+        L1 = int(rng.integers(300, 3000))
+        L2 = int(rng.integers(100, 3000))
+        a, b = synth.one_pair(seed=7000 + t, L1=L1, L2=L2, nfrag=int(rng.integers(0, 12)))
+        pipe.append({"s1": txt(a), "s2": txt(b), "seeds": ref.seed_set(a, b), "linked": ref.seed_set(a, b, linked=True),
+                     "fragments": ref.find_similar(a, b)})
+    a, b = synth.one_pair()
+    pipe.append({"s1": txt(a), "s2": txt(b), "seeds": ref.seed_set(a, b), "linked": ref.seed_set(a, b, linked=True),
+                 "fragments": ref.find_similar(a, b), "note": "10 kb x 10 kb, 20 planted fragments"})
+    json.dump(pipe, open(os.path.join(HERE, "pipeline_fragments.json"), "w"))
+
+    # cut1Gap command: inputs and the reference's text outputs
+    d = os.path.join(HERE, "cut1gap")
+    shutil.rmtree(d, ignore_errors=True)
+    os.makedirs(d)
+    write_cut1gap_inputs(d, rng)
+    for f in os.listdir(d):
+        if f.startswith("out"):
+            os.remove(os.path.join(d, f))
+    for inp, out in (("in.fa", "out"), ("in_refminus.fa", "outm")):
+        r = subprocess.run([bind.REF_CLI, "cut1Gap", "-i", inp, "-r", "ref", "-ra", "refgenome.fa", "-qa",
+                            "querygenome.fa", "-o", out], cwd=d, capture_output=True, text=True)
+        open(os.path.join(d, f"expected_stdout_{out}.txt"), "w").write(r.stdout)
+    for f in sorted(os.listdir(d)):
+        if f.startswith("out"):
+            os.replace(os.path.join(d, f), os.path.join(d, "expected_" + f))
+
+    # ranSco command
+    d = os.path.join(HERE, "ransco")
+    shutil.rmtree(d, ignore_errors=True)
+    os.makedirs(d)
+    for name, n in (("refgenome.fa", 3), ("q1.fa", 2), ("q2.fa", 2)):
+        with open(os.path.join(d, name), "w") as fh:
+            for k in range(n):
+                s = synth.to_text(synth.random_seq(rng, int(rng.integers(3000, 9000)), float(rng.uniform(.35, .6))))
+                s = list(s)
+                for _ in range(3):  # a few n / N runs that the sampler has to reject or strip
+                    p = int(rng.integers(0, len(s) - 200))
+                    ch = "n" if rng.random() < .5 else "N"
+                    ln = int(rng.integers(5, 150))
+                    s[p:p + ln] = [ch] * ln
+                s = "".join(s)
+                fh.write(f">{name[:-3]}_{k}\n")
+                for i in range(0, len(s), 60):
+                    fh.write(s[i:i + 60] + "\n")
+    for args in (["-l", "300", "-p", "200", "-s", "5"], ["-l", "500", "-p", "100", "-s", "11", "-A", "1", "-B", "-2",
+                                                       "-O", "-3", "-E", "-1"]):
+        r = subprocess.run([bind.REF_CLI, "ranSco", "-r", "refgenome.fa", "-i", "q1.fa", "q2.fa"] + args, cwd=d,
+                           capture_output=True, text=True)
+        tag = "_".join(a.strip("-") for a in args)
+        open(os.path.join(d, f"expected_{tag}.txt"), "w").write(r.stdout)
+    print("golden fixtures written to", HERE)
+
+
+def fasta_with_fai(path, records, width=60):
+    """Writes a FASTA file and its samtools-style .fai index."""
+    with open(path, "w") as fh, open(path + ".fai", "w") as fai:
+        off = 0
+        for name, seq in records:
+            head = f">{name}\n"
+            fh.write(head)
+            off += len(head)
+            fai.write(f"{name}\t{len(seq)}\t{off}\t{width}\t{width + 1}\n")
+            for i in range(0, len(seq), width):
+                line = seq[i:i + width] + "\n"
+                fh.write(line)
+                off += len(line)
+
+
+def write_cut1gap_inputs(d, rng):
+    """Two small genomes with masked (lowercase n) stretches, lowercase bases,
+    and an -i file of one reference region and four query regions on both
+    strands."""
+    def masked(codes, nruns):
+        s = list(synth.to_text(codes))
+        for _ in range(nruns):
+            p = int(rng.integers(0, len(s) - 100))
+            ln = int(rng.integers(5, 90))
+            s[p:p + ln] = "n" * len(s[p:p + ln])
+        for _ in range(nruns):  # soft-masked lowercase bases
+            p = int(rng.integers(0, len(s) - 50))
+            s[p:p + 30] = [c.lower() if c != "n" else c for c in s[p:p + 30]]
+        return "".join(s)
+
+    r_chr = synth.random_seq(rng, 9000)
+    q_chr1 = synth.random_seq(rng, 7000)
+    q_chr2 = synth.random_seq(rng, 6000)
+    # plant conserved fragments between the reference region and query regions
+    for q, off in ((q_chr1, 500), (q_chr2, 300)):
+        for _ in range(10):
+            ln = int(rng.integers(40, 160))
+            p1 = int(rng.integers(1000, 6000 - ln))
+            p2 = int(rng.integers(off, min(len(q), off + 4000) - ln))
+            m = synth.mutate(rng, r_chr[p1:p1 + ln], 0.1, 0.02)[:ln]
+            q[p2:p2 + len(m)] = m
+    # a reverse-strand copy for the strand-0 cases
+    rc = (4 - r_chr[2000:2200][::-1]).astype(np.int8)
+    q_chr2[4500:4700] = rc
+    fasta_with_fai(os.path.join(d, "refgenome.fa"), [("chr1", masked(r_chr, 12))])
+    fasta_with_fai(os.path.join(d, "querygenome.fa"), [("qc1", masked(q_chr1, 10)), ("qc2", masked(q_chr2, 10))])
+    with open(os.path.join(d, "in.fa"), "w") as fh:
+        regions = [("ref", "spA", "chr1", 1, 1001, 6000), ("q1", "spB", "qc1", 1, 401, 4600),
+                   ("q2", "spC", "qc2", 1, 201, 4400), ("q3", "spD", "qc2", 0, 3001, 6000),
+                   ("q4", "spE", "qc1", 0, 101, 3000)]
+        for name, sp, chrom, strand, st, en in regions:
+            fh.write(f">{name}\tspecies:{sp}\tchr:{chrom}\tstrand:{strand}\tstart:{st}\tend:{en}\n")
+            fh.write("ACGT\n")
+    with open(os.path.join(d, "in_refminus.fa"), "w") as fh:
+        regions = [("ref", "spA", "chr1", 0, 1001, 6000), ("q1", "spB", "qc1", 1, 401, 4600),
+                   ("q3", "spD", "qc2", 0, 3001, 6000)]
+        for name, sp, chrom, strand, st, en in regions:
+            fh.write(f">{name}\tspecies:{sp}\tchr:{chrom}\tstrand:{strand}\tstart:{st}\tend:{en}\n")
+            fh.write("ACGT\n")
+
+
+if __name__ == "__main__":
+    main()
