@@ -83,7 +83,8 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_device_info", "dcns_last_stats", "dcns_kernel_time", "dcns_kernel_time_reset", "dcns_sw_batch",
            "dcns_xdrop_extend", "dcns_seed_sets", "dcns_find_similar_batch", "dcns_stage",
            "dcns_find_similar_staged", "dcns_frag_set_total",
-           "dcns_frag_set_pair", "dcns_frag_set_get", "dcns_frag_set_free"]
+           "dcns_frag_set_pair", "dcns_frag_set_get", "dcns_frag_set_free", "dcns_cut1gap_main",
+           "dcns_ransco_main"]
 
 
 def default_params(**kw):

@@ -26,8 +26,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
           "-packed-fp32-ops", "-Wno-unused-command-line-argument", f"-I{os.path.join(ROOT, 'include')}",
           f"-I{CSRC}"]
 
-LIB_SOURCES = ["seedscan.hip", "xdrop.hip", "swbatch.hip", "pipeline.cpp", "capi.cpp", "host/refapi.cpp",
-               "host/cut1gap.cpp"]
+LIB_SOURCES = ["seedscan.hip", "xdrop.hip", "swbatch.hip", "pipeline.cpp", "capi.cpp", "host/cli_io.cpp",
+               "host/commands.cpp"]
 CLI_SOURCES = ["cli/main.cpp"]
 
 

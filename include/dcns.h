@@ -142,6 +142,16 @@ int dcns_frag_set_get(const dcns_frag_set *fs, int64_t idx, int32_t *ints5, doub
                       int32_t cig_cap, int32_t *ncig);
 void dcns_frag_set_free(dcns_frag_set *fs);
 
+/* Commands with the reference's flags, files and output text (argv[0] is the
+ * command name):
+ *   dcns_cut1gap_main  int cut1Gap(int argc, char** argv), controlLayer4.cpp:1441
+ *                      (decl. controlLayer.h:109); all query regions of the
+ *                      input are aligned in one batched device call
+ *   dcns_ransco_main   int smitherWaterManScoreOfRandomFragments(int argc, char** argv),
+ *                      controlLayer4.cpp:2088 (decl. controlLayer.h:113) */
+int dcns_cut1gap_main(int argc, char **argv);
+int dcns_ransco_main(int argc, char **argv);
+
 #ifdef __cplusplus
 }
 #endif
