@@ -47,6 +47,20 @@ def window_cells(len1, len2, W=38, step=8):
     return total
 
 
+def measured_traffic(path=os.path.join(ROOT, "profiles", "pmc_traffic_valu_batch4096.md")):
+    """HBM bytes per launch of the window-scan kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated against a copy
+    kernel), or None when that summary is absent."""
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 4 and cells[1].startswith("`k_scan_uniform"):
+                return int((float(cells[3]) + float(cells[4])) * 1e6)
+    except OSError:
+        pass
+    return None
+
+
 def _cpu_worker_init():
     # the reference prints a "seeds size" line per pair; keep stdout for the JSON line
     devnull = os.open(os.devnull, os.O_WRONLY)
@@ -172,7 +186,8 @@ def main():
         "kernels": kernels,
         "pcie_inclusive_gcups": round(cells / pcie_s / 1e9, 2),
         "roofline": {"bound": "valu", "kernel": "k_scan_uniform", "achieved": round(achieved, 3),
-                     "peak": round(peak, 2), "unit": "TOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "peak": round(peak, 2), "unit": "TOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": measured_traffic(),
                      "work_per_launch": f"{l0:.4e} cells x {VALU_OPS_PER_CELL} int16 ops",
                      "avg_launch_ms": round(avg_ms, 3)},
     }

@@ -9,6 +9,10 @@
 #include <string>
 #include <vector>
 
+#include <algorithm>
+#include <atomic>
+#include <thread>
+
 #include "../../include/dcns.h"
 #include "dcns_types.h"
 
@@ -52,10 +56,56 @@ struct DBuf {
     }
 };
 
+// Pinned host buffer (hipHostMalloc) for device->host result transfers.
+struct HBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    HBuf() = default;
+    HBuf(const HBuf &) = delete;
+    HBuf &operator=(const HBuf &) = delete;
+    ~HBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    void ensure(size_t bytes) {
+        if (bytes <= cap) return;
+        if (p) hip_check(hipHostFree(p), "hipHostFree");
+        p = nullptr;
+        const size_t n = bytes + bytes / 4 + 4096;
+        hip_check(hipHostMalloc(&p, n, hipHostMallocDefault), "hipHostMalloc");
+        cap = n;
+    }
+    template <class T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+// Runs fn(i) for i in [0, n) on up to `threads` host threads (dynamic schedule).
+template <class F>
+void parallel_for(int64_t n, F &&fn, int threads = 0);
+
 struct KernelTimer {
     double ms = 0;
     int64_t launches = 0;
 };
+
+template <class F>
+void parallel_for(int64_t n, F &&fn, int threads) {
+    if (threads <= 0) threads = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+    threads = (int)std::min<int64_t>(threads, n);
+    if (threads <= 1) {
+        for (int64_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<int64_t> next(0);
+    auto work = [&]() {
+        for (int64_t i = next.fetch_add(1); i < n; i = next.fetch_add(1)) fn(i);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+}
 
 }  // namespace dcns
 
@@ -77,6 +127,7 @@ struct dcns_ctx {
     dcns::DBuf seq, pairs, wtasks, pow, traj, tcount, hitlist, ctr, nodes, roots, tasks_a, tasks_b;
     dcns::DBuf xtasks, xout, tb_off, tb, cigar, cig_host_dummy;
     dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound;
+    dcns::HBuf h_hits, h_roots, h_nodes, h_xout, h_cigar;  // pinned download buffers
 
     hipEvent_t take_event() {
         if (!event_pool.empty()) {

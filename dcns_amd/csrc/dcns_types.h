@@ -53,15 +53,17 @@ struct TrajEntry {
     int32_t score_col;  // score << 8 | col
 };
 
-// Device counters used by the level loop.
+// Device counters used by the level loop and the x-drop passes.
 struct Counters {
-    int32_t nodes;      // allocated nodes
-    int32_t scans;      // allocated trajectory slots
-    int32_t next_tasks; // scan tasks queued for the next level
-    int32_t hits;       // level-0 windows with at least one trajectory entry
-    int32_t overflow;   // set when a pool capacity was exceeded
-    int32_t entries;    // trajectory entries recorded by the last scan launches
-    int32_t pad[2];
+    int32_t nodes;        // allocated nodes
+    int32_t scans;        // allocated trajectory slots
+    int32_t next_tasks;   // scan tasks queued for the next level
+    int32_t hits;         // level-0 windows with at least one trajectory entry
+    int32_t overflow;     // set when a pool capacity was exceeded
+    int32_t entries;      // trajectory entries recorded by the last scan launches
+    int32_t level_tasks;  // scan tasks of the current level (read by the level's kernels)
+    int32_t pending;      // x-drop tasks that need a wider band kernel
+    unsigned long long gen_cells;  // DP cells evaluated by the later-level scans
 };
 
 // x-drop extension task for one seed (x_extend_seed,

@@ -77,6 +77,49 @@ __host__ __device__ __forceinline__ uint32_t sel32(int q) {
     return base ? (uint32_t)((2 * k + 1) | ((8 + k) * 0x010101u << 8)) : 0x0C0C0C0Cu;
 }
 
+// Packed-f16 variant for the window scan: every DP value there is an integer
+// of magnitude <= 2048 (window * match <= 2048, penalties >= -128), which f16
+// represents exactly, and gfx950 has a three-input packed max
+// (v_pk_maximum3_f16).  Table bytes (2k, 2k+1) hold the f16 score of class k;
+// selector 0x0C0C still yields +0.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ f16x2 as_h2(uint32_t v) { return __builtin_bit_cast(f16x2, v); }
+__device__ __forceinline__ uint32_t bits(f16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ f16x2 hmax3(f16x2 a, f16x2 b, f16x2 c) {
+    return __builtin_elementwise_maximum(__builtin_elementwise_maximum(a, b), c);
+}
+
+inline uint16_t f16_bits_of_int(int v) {  // exact for |v| <= 2048
+    if (v == 0) return 0;
+    const uint16_t sign = v < 0 ? 0x8000u : 0u;
+    uint32_t a = (uint32_t)(v < 0 ? -v : v);
+    const int e = 31 - __builtin_clz(a);  // a in [2^e, 2^(e+1)), e <= 11
+    const uint32_t mant = (e <= 10 ? (a << (10 - e)) : (a >> (e - 10))) & 0x3ffu;
+    return (uint16_t)(sign | ((uint32_t)(e + 15) << 10) | mant);
+}
+
+inline ScoreTab make_score_tab_f16(int32_t match, int32_t mismatch) {
+    ScoreTab t{};
+    const int cls[4] = {0, 1, 3, 4};
+    for (int c = 0; c < 8; ++c) {
+        uint32_t lo = 0, hi = 0;
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t h = f16_bits_of_int(score_of(match, mismatch, c, cls[k]));
+            if (k < 2) lo |= h << (16 * k);
+            else hi |= h << (16 * (k - 2));
+        }
+        t.lo[c] = lo;
+        t.hi[c] = hi;
+    }
+    return t;
+}
+
+__host__ __device__ __forceinline__ uint32_t sel16_f16(int q) {
+    const int k = q - (q > 2);
+    const bool base = (q >= 0) & (q <= 4) & (q != 2);
+    return base ? (uint32_t)((2 * k) | ((2 * k + 1) << 8)) : 0x0C0Cu;
+}
+
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) {
     return __builtin_amdgcn_perm(hi, lo, sel);
 }

@@ -8,10 +8,12 @@
 namespace dcns {
 
 // seedscan.hip
+// The level-0 scan runs in packed f16 when every DP value is an exact f16 integer.
+inline bool scan_uses_f16(const Params &P) { return (int64_t)P.window * P.match <= 2048; }
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
                                int32_t nwt, const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount,
                                int32_t cap, int32_t *hitlist, Counters *ctr);
-hipError_t launch_scan_general(hipStream_t st, int wm, bool textbook, const int8_t *seq, const PairDesc *pairs,
+hipError_t launch_scan_general(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
                                const ScanTask *tasks, const int32_t *ntasks_dev, int32_t ntasks_max, const Params &P,
                                const ScoreTab &tab, TrajEntry *traj, int32_t *tcount, int32_t cap, Counters *ctr);
 hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
@@ -26,7 +28,7 @@ hipError_t launch_gather_roots(hipStream_t st, const Node *nodes, const int32_t 
 // xdrop.hip
 // Pass 1: forward score-only extension + reverse score-only extension.
 hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
-                              int32_t n, const Params &P, XOut *out);
+                              int32_t n, const Params &P, XOut *out, int32_t *pending);
 // Pass 2: reverse extension with traceback into a band-compressed store, then the cigar.
 hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
                               int32_t n, const Params &P, XOut *out, const int64_t *tb_off, uint8_t *tb,

@@ -120,8 +120,33 @@ Staged stage_batch(dcns_ctx *c, const dcns_batch &b, const Params &P) {
     S.window = P.window;
     S.step = P.step;
     upload(c, c->seq, host);
-    upload(c, c->pairs, S.pd);
+    upload_layout(c, S);
     return S;
+}
+
+// Level-0 wave tasks (128 windows of one pair per wave, longest reference
+// first), window -> pair map and per-pair window-scan cells; uploaded once
+// per staged batch and window layout.
+void upload_layout(dcns_ctx *c, Staged &S) {
+    S.wt.clear();
+    S.pair_of_window.assign((size_t)S.nwin, 0);
+    S.level0_cells.assign(S.pd.size(), 0);
+    for (size_t p = 0; p < S.pd.size(); ++p) {
+        const PairDesc &d = S.pd[p];
+        for (int32_t g = 0; g < d.nwin; g += 128) S.wt.push_back(make_int2((int)p, g));
+        int64_t widths = 0;
+        for (int32_t w = 0; w < d.nwin; ++w) {
+            S.pair_of_window[(size_t)(d.win_base + w)] = (int32_t)p;
+            widths += std::min(S.window, d.len2 - w * S.step);
+        }
+        S.level0_cells[p] = (int64_t)d.len1 * widths;
+    }
+    std::stable_sort(S.wt.begin(), S.wt.end(), [&](const int2 &a, const int2 &b) {
+        return S.pd[(size_t)a.x].len1 > S.pd[(size_t)b.x].len1;
+    });
+    upload(c, c->pairs, S.pd);
+    upload(c, c->wtasks, S.wt);
+    upload(c, c->pow, S.pair_of_window);
 }
 
 void relayout(dcns_ctx *c, Staged &S, const Params &P) {
@@ -136,7 +161,7 @@ void relayout(dcns_ctx *c, Staged &S, const Params &P) {
     S.nwin = nw;
     S.window = P.window;
     S.step = P.step;
-    upload(c, c->pairs, S.pd);
+    upload_layout(c, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -145,40 +170,24 @@ void relayout(dcns_ctx *c, Staged &S, const Params &P) {
 namespace {
 
 struct Forest {
-    int32_t NW = 0;
-    std::vector<int32_t> root_index;  // per window: index into roots, or -1
-    std::vector<Node> roots, dyn;
-    const Node &node(int32_t id) const {
-        return id < NW ? roots[(size_t)root_index[(size_t)id]] : dyn[(size_t)(id - NW)];
-    }
+    int32_t NW = 0, hits = 0;
+    const int32_t *hit_window = nullptr;  // global window id of hit k
+    const Node *roots = nullptr;          // root node of hit k
+    const Node *dyn = nullptr;            // nodes allocated during the recursion (id >= NW)
+    const Node &child(int32_t id) const { return dyn[(size_t)(id - NW)]; }
 };
 
 void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_t &gpu_cells, int32_t &levels) {
     const int32_t NW = (int32_t)S.nwin;
+    F = Forest{};
     F.NW = NW;
-    F.root_index.assign((size_t)NW, -1);
     levels = 0;
     gpu_cells = 0;
+    for (int64_t v : S.level0_cells) gpu_cells += v;
     if (NW == 0 || (int64_t)P.window * P.match < P.min_score) return;  // no window can reach min_score
     const int wm = P.window <= 38 ? 38 : 64;
     const int32_t cap = P.window * P.match - P.min_score + 1;
     const ScoreTab tab = make_score_tab(P.match, P.mismatch);
-    const bool textbook_general = !sw_simd_variant(1 << 30, P.window, P.match, P.open, P.ext);
-
-    // level-0 wave tasks, longest reference first
-    std::vector<int2> wt;
-    std::vector<int32_t> pow((size_t)NW);
-    for (size_t p = 0; p < S.pd.size(); ++p) {
-        const PairDesc &d = S.pd[p];
-        for (int32_t g = 0; g < d.nwin; g += 128) wt.push_back(make_int2((int)p, g));
-        for (int32_t w = 0; w < d.nwin; ++w) pow[(size_t)(d.win_base + w)] = (int32_t)p;
-        for (int32_t w = 0; w < d.nwin; ++w) gpu_cells += (int64_t)d.len1 * std::min(P.window, d.len2 - w * P.step);
-    }
-    std::stable_sort(wt.begin(), wt.end(), [&](const int2 &a, const int2 &b) {
-        return S.pd[(size_t)a.x].len1 > S.pd[(size_t)b.x].len1;
-    });
-    upload(c, c->wtasks, wt);
-    upload(c, c->pow, pow);
 
     int64_t scan_cap = (int64_t)NW + std::max<int64_t>(NW / 4, 4096);
     int64_t node_cap = (int64_t)NW + std::max<int64_t>(NW / 2, 8192);
@@ -197,7 +206,7 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
     write_ctr(c, h);
     c->launch("scan_uniform", [&] {
         return launch_scan_uniform(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->wtasks.as<int2>(),
-                                   (int32_t)wt.size(), P, tab, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
+                                   (int32_t)S.wt.size(), P, tab, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
                                    c->hitlist.as<int32_t>(), c->ctr.as<Counters>());
     });
     h = read_ctr(c);
@@ -241,42 +250,45 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
         const int32_t n = h.next_tasks;
         std::swap(c->tasks_a.p, c->tasks_b.p);
         std::swap(c->tasks_a.cap, c->tasks_b.cap);
-        // the level's task count: next_tasks is reset by resolve, so keep a copy in pad[0]
-        h.pad[0] = n;
+        h.level_tasks = n;  // next_tasks is reset by resolve; the level's kernels read this copy
         h.next_tasks = 0;
         write_ctr(c, h);
-        const int32_t *n_dev = &c->ctr.as<Counters>()->pad[0];
+        const int32_t *n_dev = &c->ctr.as<Counters>()->level_tasks;
         c->launch("scan_general", [&] {
-            return launch_scan_general(c->stream, wm, textbook_general, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+            return launch_scan_general(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
                                        c->tasks_a.as<ScanTask>(), n_dev, n, P, tab, c->traj.as<TrajEntry>(),
                                        c->tcount.as<int32_t>(), cap, c->ctr.as<Counters>());
         });
         h = read_ctr(c);
-        {   // cells of this level's scans
-            std::vector<ScanTask> tl;
-            download(c, tl, c->tasks_a.p, (size_t)n);
-            c->sync();
-            for (const ScanTask &t : tl) {
-                const PairDesc &d = S.pd[(size_t)t.pair];
-                gpu_cells += (int64_t)(t.b - t.a) * std::min(P.window, d.len2 - t.win * P.step);
-            }
-        }
         resolve(0, c->tasks_a.as<ScanTask>(), n, n_dev);
         ++levels;
     }
+    gpu_cells += (int64_t)h.gen_cells;
 
-    // bring the forest home: hit windows, their roots, the dynamic nodes
-    std::vector<int32_t> hitl;
-    download(c, hitl, c->hitlist.p, (size_t)hits);
+    // bring the forest home (pinned buffers): hit windows, their roots, the dynamic nodes
+    const size_t ndyn = (size_t)(h.nodes - NW);
     c->roots.ensure((size_t)std::max(hits, 1) * sizeof(Node));
     c->launch("gather", [&] {
         return launch_gather_roots(c->stream, c->nodes.as<Node>(), c->hitlist.as<int32_t>(), hits_dev, hits,
                                    c->roots.as<Node>());
     });
-    download(c, F.roots, c->roots.p, (size_t)hits);
-    download(c, F.dyn, c->nodes.as<Node>() + NW, (size_t)(h.nodes - NW));
+    c->h_hits.ensure((size_t)std::max(hits, 1) * sizeof(int32_t));
+    c->h_roots.ensure((size_t)std::max(hits, 1) * sizeof(Node));
+    c->h_nodes.ensure(std::max<size_t>(ndyn, 1) * sizeof(Node));
+    if (hits) {
+        hip_check(hipMemcpyAsync(c->h_hits.p, c->hitlist.p, (size_t)hits * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 c->stream), "download hits");
+        hip_check(hipMemcpyAsync(c->h_roots.p, c->roots.p, (size_t)hits * sizeof(Node), hipMemcpyDeviceToHost,
+                                 c->stream), "download roots");
+    }
+    if (ndyn)
+        hip_check(hipMemcpyAsync(c->h_nodes.p, c->nodes.as<Node>() + NW, ndyn * sizeof(Node), hipMemcpyDeviceToHost,
+                                 c->stream), "download nodes");
     c->sync();
-    for (int32_t k = 0; k < hits; ++k) F.root_index[(size_t)hitl[(size_t)k]] = k;
+    F.hits = hits;
+    F.hit_window = c->h_hits.as<int32_t>();
+    F.roots = c->h_roots.as<Node>();
+    F.dyn = c->h_nodes.as<Node>();
 }
 
 Seed seed_of(const Node &n) { return Seed(n.s1, n.e1, n.s2, n.e2); }
@@ -293,7 +305,7 @@ int64_t reverse_cells(const Node &n, int32_t sp, int32_t tw, int32_t W) {
 // right child's set, each inserted in its own set order.
 std::set<Seed> inner_set(const Forest &F, int32_t id, int32_t sp, int32_t tw, int32_t W, int64_t &cells) {
     std::set<Seed> out;
-    const Node &n = F.node(id);
+    const Node &n = F.child(id);
     cells += (int64_t)(n.b - n.a) * tw;
     if (n.e1 < 0) return out;
     cells += reverse_cells(n, sp, tw, W);
@@ -355,34 +367,47 @@ void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked
     out.linked.assign(np, {});
     out.before_link.assign(np, 0);
     if (keep_unlinked) out.unlinked.assign(np, {});
-    out.scan_cells = 0;
-    for (size_t p = 0; p < np; ++p) {
-        const PairDesc &d = S.pd[p];
+    // hits bucketed by pair, then each pair's hits in window order
+    std::vector<int64_t> first(np + 1, 0);
+    for (int32_t k = 0; k < F.hits; ++k) ++first[(size_t)S.pair_of_window[(size_t)F.hit_window[k]] + 1];
+    for (size_t p = 0; p < np; ++p) first[p + 1] += first[p];
+    std::vector<int32_t> by_pair((size_t)F.hits);
+    {
+        std::vector<int64_t> fill(first.begin(), first.end() - 1);
+        for (int32_t k = 0; k < F.hits; ++k)
+            by_pair[(size_t)fill[(size_t)S.pair_of_window[(size_t)F.hit_window[k]]]++] = k;
+    }
+    std::vector<int64_t> cells(np, 0);
+    parallel_for((int64_t)np, [&](int64_t p) {
+        const PairDesc &d = S.pd[(size_t)p];
+        int32_t *hk = by_pair.data() + first[(size_t)p];
+        const int64_t nh = first[(size_t)p + 1] - first[(size_t)p];
+        std::sort(hk, hk + nh, [&](int32_t a, int32_t b) { return F.hit_window[a] < F.hit_window[b]; });
         std::set<Seed> seeds;
-        int64_t cells = 0;
-        for (int32_t w = 0; w < d.nwin; ++w) {
+        int64_t cl = S.level0_cells[(size_t)p];
+        for (int64_t q = 0; q < nh; ++q) {
+            const Node &r = F.roots[hk[q]];
+            const int32_t w = F.hit_window[hk[q]] - d.win_base;
             const int32_t sp = w * P.step;
             const int32_t tw = std::min(P.window, d.len2 - sp);
-            const int32_t gw = d.win_base + w;
-            cells += (int64_t)d.len1 * tw;
-            if (F.root_index.empty() || F.root_index[(size_t)gw] < 0) continue;
-            const Node &r = F.node(gw);
             if (r.e1 < 0) continue;
-            cells += reverse_cells(r, sp, tw, P.window);
+            cl += reverse_cells(r, sp, tw, P.window);
             const Seed s0 = seed_of(r);
             if (seeds.find(s0) != seeds.end()) continue;  // the set gate of :170
             seeds.insert(s0);
             if (r.left >= 0)
-                for (const Seed &s : inner_set(F, r.left, sp, tw, P.window, cells)) seeds.insert(s);
+                for (const Seed &x : inner_set(F, r.left, sp, tw, P.window, cl)) seeds.insert(x);
             if (r.right >= 0)
-                for (const Seed &s : inner_set(F, r.right, sp, tw, P.window, cells)) seeds.insert(s);
+                for (const Seed &x : inner_set(F, r.right, sp, tw, P.window, cl)) seeds.insert(x);
         }
-        out.scan_cells += cells;
-        out.before_link[p] = (int64_t)seeds.size();
-        if (keep_unlinked) out.unlinked[p].assign(seeds.begin(), seeds.end());
+        cells[(size_t)p] = cl;
+        out.before_link[(size_t)p] = (int64_t)seeds.size();
+        if (keep_unlinked) out.unlinked[(size_t)p].assign(seeds.begin(), seeds.end());
         link_seeds(seeds);
-        out.linked[p].assign(seeds.begin(), seeds.end());
-    }
+        out.linked[(size_t)p].assign(seeds.begin(), seeds.end());
+    });
+    out.scan_cells = 0;
+    for (int64_t v : cells) out.scan_cells += v;
 }
 
 // ---------------------------------------------------------------------------
@@ -390,36 +415,42 @@ void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked
 // ---------------------------------------------------------------------------
 void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vector<XTask> &tasks,
                   std::vector<XResult> &out, int64_t &cells) {
+    (void)S;
     const int32_t n = (int32_t)tasks.size();
     out.assign((size_t)n, XResult{});
     cells = 0;
     if (n == 0) return;
     upload(c, c->xtasks, tasks);
     c->xout.ensure((size_t)n * sizeof(XOut));
+    c->ctr.ensure(sizeof(Counters) + 64);
     hip_check(hipMemsetAsync(c->xout.p, 0, (size_t)n * sizeof(XOut), c->stream), "hipMemsetAsync");
     static const int bands[] = {21, 41, 81, 161};
-    std::vector<XOut> xo;
     int first = 0;
     while (first < 4 && bands[first] < 2 * P.w + 1) ++first;
-    bool pending = true;
+    int32_t *pending_dev = &c->ctr.as<Counters>()->pending;
+    int32_t pending = 1;
+    // pass 1 with the narrowest register band; tasks whose band outgrows it
+    // count themselves in `pending` and are resumed by the next instantiation
     for (int bi = first; bi < 4 && pending; ++bi) {
+        hip_check(hipMemsetAsync(pending_dev, 0, sizeof(int32_t), c->stream), "hipMemsetAsync");
         c->launch("xdrop_pass1", [&] {
             return launch_xdrop_pass1(c->stream, bands[bi], c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
-                                      c->xtasks.as<XTask>(), n, P, c->xout.as<XOut>());
+                                      c->xtasks.as<XTask>(), n, P, c->xout.as<XOut>(), pending_dev);
         });
-        download(c, xo, c->xout.p, (size_t)n);
+        hip_check(hipMemcpyAsync(&pending, pending_dev, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream), "pending");
         c->sync();
-        pending = false;
-        for (const XOut &o : xo)
-            if (o.status != 0) pending = true;
     }
     if (pending) throw Error(DCNS_ERR_ARG, "x-drop band grew beyond 161 cells");
     // traceback store and cigar arena sizes from pass 1
+    c->h_xout.ensure((size_t)n * sizeof(XOut));
+    XOut *xo = c->h_xout.as<XOut>();
+    hip_check(hipMemcpyAsync(xo, c->xout.p, (size_t)n * sizeof(XOut), hipMemcpyDeviceToHost, c->stream), "xout");
+    c->sync();
     std::vector<int64_t> tboff((size_t)n, 0);
     int64_t tbtotal = 0, cigtotal = 0;
     bool any_band[4] = {false, false, false, false};
     for (int32_t i = 0; i < n; ++i) {
-        XOut &o = xo[(size_t)i];
+        XOut &o = xo[i];
         cells += o.cells;
         o.cig_off = cigtotal;
         if (o.fe1 < 0 || o.fe2 < 0) continue;
@@ -428,9 +459,12 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
         tbtotal += (int64_t)(o.rrows + 1) * BP;
         cigtotal += std::max(0, o.re1) + std::max(0, o.re2) + 2;
         for (int k = 0; k < 4; ++k)
-            if (B <= bands[k]) { any_band[k] = true; break; }
+            if (B <= bands[k]) {
+                any_band[k] = true;
+                break;
+            }
     }
-    upload(c, c->xout, xo);
+    hip_check(hipMemcpyAsync(c->xout.p, xo, (size_t)n * sizeof(XOut), hipMemcpyHostToDevice, c->stream), "xout up");
     upload(c, c->tb_off, tboff);
     c->tb.ensure((size_t)std::max<int64_t>(tbtotal, 64));
     c->cigar.ensure((size_t)std::max<int64_t>(cigtotal, 16) * sizeof(uint32_t));
@@ -442,12 +476,15 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
                                       c->tb.as<uint8_t>(), c->cigar.as<uint32_t>());
         });
     }
-    std::vector<uint32_t> cig;
-    download(c, xo, c->xout.p, (size_t)n);
-    download(c, cig, c->cigar.p, (size_t)cigtotal);
+    c->h_cigar.ensure((size_t)std::max<int64_t>(cigtotal, 16) * sizeof(uint32_t));
+    const uint32_t *cig = c->h_cigar.as<uint32_t>();
+    hip_check(hipMemcpyAsync(xo, c->xout.p, (size_t)n * sizeof(XOut), hipMemcpyDeviceToHost, c->stream), "xout");
+    if (cigtotal)
+        hip_check(hipMemcpyAsync(c->h_cigar.p, c->cigar.p, (size_t)cigtotal * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 c->stream), "cigar");
     c->sync();
-    for (int32_t i = 0; i < n; ++i) {
-        const XOut &o = xo[(size_t)i];
+    parallel_for(n, [&](int64_t i) {
+        const XOut &o = xo[i];
         const XTask &t = tasks[(size_t)i];
         XResult &r = out[(size_t)i];
         r.start1 = t.start1;
@@ -456,20 +493,20 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
         if (o.fe1 < 0 || o.fe2 < 0) {  // forward pass found no positive cell (:259-266)
             r.ok = false;
             r.score = o.fmax;
-            continue;
+            return;
         }
         r.score = o.rmax;
-        r.cigar.assign(cig.begin() + o.cig_off, cig.begin() + o.cig_off + o.ncig);
+        r.cigar.assign(cig + o.cig_off, cig + o.cig_off + o.ncig);
         if (o.re1 < 0 || o.re2 < 0) {
             r.ok = false;
-            continue;
+            return;
         }
         r.ok = true;
         r.end1 = t.start1 + o.fe1 - o.iii;
         r.end2 = t.start2 + o.fe2 - o.jjj;
         r.start1 = t.start1 + o.fe1 - o.re1;
         r.start2 = t.start2 + o.fe2 - o.re2;
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -513,9 +550,10 @@ void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &
     out.first.assign((size_t)npairs, 0);
     out.count.assign((size_t)npairs, 0);
     out.seeds_size = sr.before_link;
-    for (int32_t i = 0; i < npairs; ++i) {
+    std::vector<std::vector<PairedSimilarFragment>> per((size_t)npairs);
+    parallel_for(npairs, [&](int64_t i) {
         const double L1d = (double)S.pd[(size_t)i].len1, L2d = (double)S.pd[(size_t)i].len2;
-        std::vector<PairedSimilarFragment> fr;
+        std::vector<PairedSimilarFragment> &fr = per[(size_t)i];
         for (int64_t k = first_task[(size_t)i]; k < first_task[(size_t)i + 1]; ++k) {
             const XResult &r = xr[(size_t)k];
             if (!r.ok) continue;  // p = 100
@@ -539,9 +577,11 @@ void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &
             }
             if (fresh) fr.push_back(nf);
         }
+    });
+    for (int32_t i = 0; i < npairs; ++i) {
         out.first[(size_t)i] = (int64_t)out.frags.size();
-        out.count[(size_t)i] = (int64_t)fr.size();
-        for (auto &f : fr) out.frags.push_back(std::move(f));
+        out.count[(size_t)i] = (int64_t)per[(size_t)i].size();
+        for (auto &f : per[(size_t)i]) out.frags.push_back(std::move(f));
     }
     const double t3 = now_ms();
     dcns_stats &st = c->stats;

@@ -19,10 +19,14 @@ struct Staged {
     std::vector<PairDesc> pd;
     int64_t nwin = 0;  // windows over all pairs
     int32_t window = 0, step = 0;  // schedule the window layout was made for
+    std::vector<int2> wt;                 // level-0 wave tasks (pair, first window)
+    std::vector<int32_t> pair_of_window;  // global window -> pair
+    std::vector<int64_t> level0_cells;    // per pair: L1 * sum of window widths
 };
 Staged stage_batch(dcns_ctx *c, const dcns_batch &b, const Params &P);
 // Recomputes the window layout of staged pairs for another window/step.
 void relayout(dcns_ctx *c, Staged &S, const Params &P);
+void upload_layout(dcns_ctx *c, Staged &S);
 
 // Seed sets of every pair (set order), before and after link_seeds.
 struct SeedResult {

@@ -94,18 +94,17 @@ __global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ 
         if (sub == 3) word = next;
         uint32_t thi, tlo;
         uniform_table(tab, c, thi, tlo);
-        uint32_t diag = 0, left = 0, fc = 0, rmax = 0;
+        uint32_t diag = 0, fc = 0, rmax = 0;
 #pragma unroll
         for (int j = 0; j < WM; ++j) {
             const uint32_t s = perm(thi, tlo, sel[j]);
             const uint32_t t = pk_add(diag, s);
             diag = H[j];
-            const uint32_t hpre = pk_max_s(t, E[j]);  // E >= 0, so this is max(t, 0, E)
-            E[j] = pk_max_u(pk_subsat(E[j], EXT), pk_subsat(hpre, OPEN));
-            fc = pk_max_u(pk_subsat(left, OPEN), pk_subsat(fc, EXT));
-            const uint32_t h = pk_max_s(hpre, fc);
+            const uint32_t h = pk_max_s(pk_max_s(t, E[j]), fc);  // E, fc >= 0: includes the 0 floor
+            const uint32_t ho = pk_subsat(h, OPEN);                // both gap kinds open from the final H
+            E[j] = pk_max_u(pk_subsat(E[j], EXT), ho);
+            fc = pk_max_u(ho, pk_subsat(fc, EXT));
             H[j] = h;
-            left = h;
             rmax = pk_max_s(rmax, h);
         }
         const uint32_t imp = pk_subsat(rmax, best);
@@ -146,8 +145,115 @@ __global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ 
     }
 }
 
+// Packed-f16 version of k_scan_uniform (same wave layout).  Valid when
+// window * match <= 2048: every H, E, F and score is then an exact f16
+// integer, and v_pk_maximum3_f16 merges the three-way maxima.  Non-negative
+// f16 values order like their bit patterns, so the improvement test and the
+// first-column search compare bits.
+template <int WM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_uniform_h(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                       const int2 *__restrict__ wtasks, Params P, ScoreTab tab,
+                                                       TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
+                                                       int32_t cap, int32_t *__restrict__ hitlist,
+                                                       Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
+                                                       uint32_t thr_h) {
+    const int lane = threadIdx.x;
+    const int2 task = wtasks[blockIdx.x];
+    const PairDesc pd = pairs[task.x];
+    const int8_t *__restrict__ ref = seq + pd.off1;
+    const int8_t *__restrict__ qry = seq + pd.off2;
+    const int L1 = pd.len1, L2 = pd.len2;
+    const int wl = task.y + lane, wh = task.y + 64 + lane;
+    const bool vl = wl < pd.nwin, vh = wh < pd.nwin;
+    const int spl = wl * P.step, sph = wh * P.step;
+    const int twl = vl ? min(P.window, L2 - spl) : 0;
+    const int twh = vh ? min(P.window, L2 - sph) : 0;
+
+    uint32_t sel[WM];
+    f16x2 H[WM], E[WM];
+    const f16x2 zero = as_h2(0u);
+#pragma unroll
+    for (int j = 0; j < WM; ++j) {
+        const int ql = j < twl ? qry[spl + j] : 2;
+        const int qh = j < twh ? qry[sph + j] : 2;
+        sel[j] = sel16_f16(ql) | (sel16_f16(qh) << 16);
+        H[j] = zero;
+        E[j] = zero;
+    }
+    const f16x2 O = as_h2(open_h * 0x10001u), X = as_h2(ext_h * 0x10001u);
+    uint32_t best = thr_h * 0x10001u;
+    int32_t cntl = 0, cnth = 0;
+
+    uint32_t word = L1 > 0 ? *(const uint32_t *)ref : 0u;
+    uint32_t next = 0;
+    for (int i = 0; i < L1; ++i) {
+        const int sub = i & 3;
+        if (sub == 0 && i + 4 < L1) next = *(const uint32_t *)(ref + i + 4);
+        const int c = (int)(int8_t)(word >> (8 * sub));
+        if (sub == 3) word = next;
+        uint32_t thi, tlo;
+        uniform_table(tab, c, thi, tlo);
+        f16x2 diag = zero, fc = zero, rmax = zero;
+#pragma unroll
+        for (int j = 0; j < WM; ++j) {
+            const f16x2 s = as_h2(perm(thi, tlo, sel[j]));
+            const f16x2 t = diag + s;
+            diag = H[j];
+            const f16x2 h = hmax3(t, E[j], fc);  // E, fc >= 0: includes the 0 floor
+            const f16x2 ho = h + O;              // both gap kinds open from the final H
+            E[j] = hmax3(E[j] + X, ho, zero);
+            fc = hmax3(ho, fc + X, zero);
+            H[j] = h;
+            if (j & 1) rmax = hmax3(rmax, H[j - 1], h);
+            else if (j == WM - 1) rmax = __builtin_elementwise_maximum(rmax, h);
+        }
+        const uint32_t imp = pk_subsat(bits(rmax), best);
+        if (__builtin_expect(imp != 0, 0)) {
+            const uint32_t rb = bits(rmax);
+            if (imp & 0xffffu) {
+                const uint32_t v = rb & 0xffffu;
+                int jj = 0;
+#pragma unroll
+                for (int j = WM - 1; j >= 0; --j)
+                    if ((bits(H[j]) & 0xffffu) == v) jj = j;
+                record(traj, pd.win_base + wl, cap, cntl, i, (int32_t)(float)rmax[0], jj);
+                best = (best & 0xffff0000u) | v;
+            }
+            if (imp >> 16) {
+                const uint32_t v = rb >> 16;
+                int jj = 0;
+#pragma unroll
+                for (int j = WM - 1; j >= 0; --j)
+                    if ((bits(H[j]) >> 16) == v) jj = j;
+                record(traj, pd.win_base + wh, cap, cnth, i, (int32_t)(float)rmax[1], jj);
+                best = (best & 0xffffu) | (v << 16);
+            }
+        }
+    }
+    if (vl) {
+        tcount[pd.win_base + wl] = cntl;
+        if (cntl > 0) {
+            hitlist[atomicAdd(&ctr->hits, 1)] = pd.win_base + wl;
+            atomicAdd(&ctr->entries, min(cntl, cap));
+        }
+    }
+    if (vh) {
+        tcount[pd.win_base + wh] = cnth;
+        if (cnth > 0) {
+            hitlist[atomicAdd(&ctr->hits, 1)] = pd.win_base + wh;
+            atomicAdd(&ctr->entries, min(cnth, cap));
+        }
+    }
+}
+
 // One lane per ScanTask: rows [a, b) of seq1 against one window, int32.
-template <int WM, bool TEXTBOOK>
+// Like every kernel here it applies the textbook recurrence (both gap kinds
+// open from the final H): the striped variant of the reference, where the
+// vertical gap opens from H before the horizontal fix-up, yields the same H
+// in every cell (a horizontal run followed by a vertical one can always be
+// swapped without changing score or end cell), and the shared H + open term
+// saves two operations per cell.
+template <int WM>
 __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                      const ScanTask *__restrict__ tasks, const int32_t *__restrict__ ntasks_p,
                                                      Params P, ScoreTab tab, TrajEntry *__restrict__ traj,
@@ -184,22 +290,19 @@ __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ 
         if (i + 1 < t.b) c_next = ref[i + 1];
         const int ci = (unsigned)c < 8u ? c : 2;
         const uint32_t thi = shi[ci], tlo = slo[ci];
-        int32_t diag = 0, left = 0, fc = 0, rmax = 0;
+        int32_t diag = 0, fc = 0, rmax = 0;
 #pragma unroll
         for (int j = 0; j < WM; ++j) {
             const int32_t s = (int32_t)perm(thi, tlo, sel[j]);
-            const int32_t hpre = max(diag + s, E[j]);
+            const int32_t t = diag + s;
             diag = H[j];
-            const int32_t f = max(left + O, fc + X);
-            const int32_t h = max(hpre, f);
-            fc = max(f, 0);
-            if (TEXTBOOK)
-                E[j] = max3i(E[j] + X, h + O, 0);
-            else
-                E[j] = max3i(E[j] + X, hpre + O, 0);
+            const int32_t h = max3i(t, E[j], fc);  // E, fc >= 0: includes the 0 floor
+            const int32_t ho = h + O;
+            E[j] = max3i(E[j] + X, ho, 0);
+            fc = max3i(ho, fc + X, 0);
             H[j] = h;
-            left = h;
-            rmax = max(rmax, h);
+            if (j & 1) rmax = max3i(rmax, H[j - 1], h);
+            else if (j == WM - 1) rmax = max(rmax, h);
         }
         if (rmax > best) {
             int jj = 0;
@@ -212,6 +315,7 @@ __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ 
     }
     tcount[t.scan] = cnt;
     if (cnt > 0) atomicAdd(&ctr->entries, min(cnt, cap));
+    atomicAdd(&ctr->gen_cells, (unsigned long long)(t.b - t.a) * (unsigned long long)max(tw, 0));
 }
 
 // Reverse SW of getSeeds (:152-159): reverse complements of seq1[e1-r1+1..e1]
@@ -228,7 +332,6 @@ __device__ void reverse_sw(const int8_t *__restrict__ s1, int32_t e1, int32_t r1
         H[j] = 0;
         E[j] = 0;
     }
-    const bool textbook = !sw_simd_variant(r1, r2, P.match, P.open, P.ext);
     const int32_t O = P.open, X = P.ext;
     best = 0;
     bi = -1;
@@ -237,19 +340,19 @@ __device__ void reverse_sw(const int8_t *__restrict__ s1, int32_t e1, int32_t r1
         const int c = 4 - s1[e1 - i];
         const int ci = (unsigned)c < 8u ? c : 2;
         const uint32_t thi = shi[ci], tlo = slo[ci];
-        int32_t diag = 0, left = 0, fc = 0, rmax = 0;
+        int32_t diag = 0, fc = 0, rmax = 0;
 #pragma unroll
         for (int j = 0; j < WM; ++j) {
             const int32_t s = (int32_t)perm(thi, tlo, sel[j]);
-            const int32_t hpre = max(diag + s, E[j]);
+            const int32_t t = diag + s;
             diag = H[j];
-            const int32_t f = max(left + O, fc + X);
-            const int32_t h = max(hpre, f);
-            fc = max(f, 0);
-            E[j] = max3i(E[j] + X, (textbook ? h : hpre) + O, 0);
+            const int32_t h = max3i(t, E[j], fc);  // E, fc >= 0: includes the 0 floor
+            const int32_t ho = h + O;
+            E[j] = max3i(E[j] + X, ho, 0);
+            fc = max3i(ho, fc + X, 0);
             H[j] = h;
-            left = h;
-            rmax = max(rmax, h);
+            if (j & 1) rmax = max3i(rmax, H[j - 1], h);
+            else if (j == WM - 1) rmax = max(rmax, h);
         }
         if (rmax > best) {
             int jj = 0;
@@ -393,35 +496,34 @@ hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const 
                                int32_t nwt, const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount,
                                int32_t cap, int32_t *hitlist, Counters *ctr) {
     if (nwt <= 0) return hipSuccess;
-    if (wm <= 38)
+    if (scan_uses_f16(P)) {
+        const ScoreTab th = make_score_tab_f16(P.match, P.mismatch);
+        const uint32_t oh = f16_bits_of_int(P.open), xh = f16_bits_of_int(P.ext), thr = f16_bits_of_int(P.min_score - 1);
+        if (wm <= 38)
+            hipLaunchKernelGGL(k_scan_uniform_h<38>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, th, traj, tcount,
+                               cap, hitlist, ctr, oh, xh, thr);
+        else
+            hipLaunchKernelGGL(k_scan_uniform_h<64>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, th, traj, tcount,
+                               cap, hitlist, ctr, oh, xh, thr);
+    } else if (wm <= 38) {
         hipLaunchKernelGGL(k_scan_uniform<38>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, tab, traj, tcount,
                            cap, hitlist, ctr);
-    else
+    } else {
         hipLaunchKernelGGL(k_scan_uniform<64>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, tab, traj, tcount,
                            cap, hitlist, ctr);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_scan_general(hipStream_t st, int wm, bool textbook, const int8_t *seq, const PairDesc *pairs,
+hipError_t launch_scan_general(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
                                const ScanTask *tasks, const int32_t *ntasks_dev, int32_t ntasks_max, const Params &P,
                                const ScoreTab &tab, TrajEntry *traj, int32_t *tcount, int32_t cap, Counters *ctr) {
     if (ntasks_max <= 0) return hipSuccess;
     dim3 g((ntasks_max + 63) / 64), b(64);
-    if (wm <= 38) {
-        if (textbook)
-            hipLaunchKernelGGL((k_scan_general<38, true>), g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj,
-                               tcount, cap, ctr);
-        else
-            hipLaunchKernelGGL((k_scan_general<38, false>), g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj,
-                               tcount, cap, ctr);
-    } else {
-        if (textbook)
-            hipLaunchKernelGGL((k_scan_general<64, true>), g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj,
-                               tcount, cap, ctr);
-        else
-            hipLaunchKernelGGL((k_scan_general<64, false>), g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj,
-                               tcount, cap, ctr);
-    }
+    if (wm <= 38)
+        hipLaunchKernelGGL(k_scan_general<38>, g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj, tcount, cap, ctr);
+    else
+        hipLaunchKernelGGL(k_scan_general<64>, g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj, tcount, cap, ctr);
     return hipGetLastError();
 }
 

@@ -3,9 +3,10 @@
 // permutationLsqLambdaK.cpp:97, and any direct SmithWaterman caller).
 //
 // Semantics follow SmithWaterman(...bool& positions)
-// (sequenceAlignment3.cpp:220-469): the striped int8/int16 recurrence when its
-// bound test passes, the textbook recurrence otherwise; first strict maximum
-// in (ref row, read column) order.
+// (sequenceAlignment3.cpp:220-469): maximum local score and its first strict
+// occurrence in (ref row, read column) order.  The reference switches between
+// a striped recurrence and the textbook one by a bound test; both give the
+// same H matrix, so the textbook form (one shared H + open term) is used.
 //
 // One lane per pair, int32.  The read is cut into 32-column stripes held in
 // VGPRs; each stripe sweeps all reference rows and hands its last column
@@ -22,7 +23,6 @@ namespace {
 
 constexpr int S = 32;
 
-template <bool TEXTBOOK>
 __device__ void sw_pair(const int8_t *__restrict__ ref, int32_t L1, const int8_t *__restrict__ qry, int32_t L2,
                         const Params &P, const uint32_t *shi, const uint32_t *slo, int2 *__restrict__ bnd,
                         int32_t &best, int32_t &bi, int32_t &bj) {
@@ -59,18 +59,16 @@ __device__ void sw_pair(const int8_t *__restrict__ ref, int32_t L1, const int8_t
 #pragma unroll
             for (int j = 0; j < S; ++j) {
                 const int32_t sc = (int32_t)perm(thi, tlo, sel[j]);
-                const int32_t hpre = max(diag + sc, E[j]);
+                const int32_t t = diag + sc;
                 diag = H[j];
-                const int32_t f = max(left + O, fc + X);
-                const int32_t h = max(hpre, f);
-                fc = max(f, 0);
-                if (TEXTBOOK)
-                    E[j] = max3i(E[j] + X, h + O, 0);
-                else
-                    E[j] = max3i(E[j] + X, hpre + O, 0);
+                // fc here is the clamped horizontal gap entering column j
+                const int32_t h = max3i(t, E[j], fc);
+                const int32_t ho = h + O;
+                E[j] = max3i(E[j] + X, ho, 0);
+                fc = max3i(ho, fc + X, 0);
                 H[j] = h;
                 left = h;
-                rmax = max(rmax, h);
+                if (j & 1) rmax = max3i(rmax, H[j - 1], h);
             }
             if (j0 + S < L2) bnd[i] = make_int2(left, fc);
             if (rmax > sbest) {
@@ -108,10 +106,7 @@ __global__ __launch_bounds__(64) void k_sw_batch(const int8_t *__restrict__ seq,
     const int32_t L1 = len1[p], L2 = len2[p];
     int2 *bnd = (int2 *)(bound + (int64_t)gid * bound_stride);
     int32_t best, bi, bj;
-    if (sw_simd_variant(L1, L2, P.match, P.open, P.ext))
-        sw_pair<false>(seq + off1[p], L1, seq + off2[p], L2, P, shi, slo, bnd, best, bi, bj);
-    else
-        sw_pair<true>(seq + off1[p], L1, seq + off2[p], L2, P, shi, slo, bnd, best, bi, bj);
+    sw_pair(seq + off1[p], L1, seq + off2[p], L2, P, shi, slo, bnd, best, bi, bj);
     out[3 * p + 0] = best;
     out[3 * p + 1] = bi;
     out[3 * p + 2] = bj;

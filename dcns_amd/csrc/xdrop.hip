@@ -174,7 +174,7 @@ __device__ Dir reverse_dir(const int8_t *seq, const PairDesc &pd, const XTask &t
 template <int BMAX>
 __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                     const XTask *__restrict__ tasks, int32_t n, Params P,
-                                                    XOut *__restrict__ out) {
+                                                    XOut *__restrict__ out, int32_t *__restrict__ pending) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
     const XTask t = tasks[gid];
@@ -190,6 +190,7 @@ __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ s
             o.status = 1;
             o.fw = w;
             out[gid] = o;
+            atomicAdd(pending, 1);
             return;
         }
         o.fmax = f.max;
@@ -213,6 +214,7 @@ __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ s
         o.status = 2;
         o.rw = w;
         out[gid] = o;
+        atomicAdd(pending, 1);
         return;
     }
     o.status = 0;
@@ -275,14 +277,14 @@ __global__ __launch_bounds__(64) void k_xdrop_pass2(const int8_t *__restrict__ s
 }  // namespace
 
 hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
-                              int32_t n, const Params &P, XOut *out) {
+                              int32_t n, const Params &P, XOut *out, int32_t *pending) {
     if (n <= 0) return hipSuccess;
     dim3 g((n + 63) / 64), b(64);
     switch (band) {
-        case 21: hipLaunchKernelGGL(k_xdrop_pass1<21>, g, b, 0, st, seq, pairs, tasks, n, P, out); break;
-        case 41: hipLaunchKernelGGL(k_xdrop_pass1<41>, g, b, 0, st, seq, pairs, tasks, n, P, out); break;
-        case 81: hipLaunchKernelGGL(k_xdrop_pass1<81>, g, b, 0, st, seq, pairs, tasks, n, P, out); break;
-        case 161: hipLaunchKernelGGL(k_xdrop_pass1<161>, g, b, 0, st, seq, pairs, tasks, n, P, out); break;
+        case 21: hipLaunchKernelGGL(k_xdrop_pass1<21>, g, b, 0, st, seq, pairs, tasks, n, P, out, pending); break;
+        case 41: hipLaunchKernelGGL(k_xdrop_pass1<41>, g, b, 0, st, seq, pairs, tasks, n, P, out, pending); break;
+        case 81: hipLaunchKernelGGL(k_xdrop_pass1<81>, g, b, 0, st, seq, pairs, tasks, n, P, out, pending); break;
+        case 161: hipLaunchKernelGGL(k_xdrop_pass1<161>, g, b, 0, st, seq, pairs, tasks, n, P, out, pending); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

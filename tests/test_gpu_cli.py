@@ -30,12 +30,10 @@ def test_cut1gap_matches_reference_files(tmp_path, inp, out):
     stdout = _run(["cut1Gap", "-i", inp, "-r", "ref", "-ra", "refgenome.fa", "-qa", "querygenome.fa", "-o", out],
                   tmp_path)
     assert stdout == open(os.path.join(src, f"expected_stdout_{out}.txt")).read()
-    expected = sorted(f[len("expected_"):] for f in os.listdir(src)
-                      if f.startswith("expected_" + out) and not f.startswith("expected_stdout"))
-    produced = sorted(f for f in os.listdir(tmp_path) if f.startswith(out) and not f.startswith(out + "m"))
-    if out == "outm":
-        produced = sorted(f for f in os.listdir(tmp_path) if f.startswith("outm"))
-    assert produced == expected
+    # species names all start with "sp", so out + "sp" separates the two prefixes
+    expected = sorted(f[len("expected_"):] for f in os.listdir(src) if f.startswith("expected_" + out + "sp"))
+    produced = sorted(f for f in os.listdir(tmp_path) if f.startswith(out + "sp"))
+    assert produced == expected and expected
     for f in expected:
         assert open(os.path.join(tmp_path, f)).read() == open(os.path.join(src, "expected_" + f)).read(), f
 

@@ -115,3 +115,17 @@ def test_find_similar_one_pair_10kb(gpu, oracle_lib):
     s = gpu.stats()
     assert s["scan_cells"] == st["scan_cells"]
     assert s["xdrop_cells"] == st["xdrop_cells"]
+
+
+@pytest.mark.parametrize("sc,min_score", [((3, -4, -6, -3), 50), ((1, -1, -2, -1), 20), ((2, -1, -2, -1), 40),
+                                          ((60, -80, -100, -60), 600), ((2, -3, -1, -3), 40)])
+def test_seed_sets_other_scorings(gpu, oracle_lib, sc, min_score):
+    """Scorings that take the packed-f16 window scan and (window * match >
+    2048) the int16 one, including gap open cheaper than gap extend."""
+    pairs = _small_pairs(51, 10, 300, 2500, (2, 12))
+    p = dcns_amd.default_params(match=sc[0], mismatch=sc[1], open=sc[2], ext=sc[3], min_score=min_score)
+    print("seed_sets", sc, min_score, flush=True)
+    for linked in (False, True):
+        got = gpu.seed_sets(pairs, params=p, linked=linked)
+        for i, (a, b) in enumerate(pairs):
+            assert got[i] == oracle_lib.seed_set(a, b, sc, min_score=min_score, linked=linked), (i, linked)

@@ -28,10 +28,11 @@ def brute(ref, read, sc, variant):
                 best[(i, j)] = score
         if i < L1 and j < L2:
             walk(i + 1, j + 1, score + s(ref[i], read[j]), "D")
+        # a run may continue (extend) or restart (open) after a run of its own kind
         if i < L1 and not (variant and last == "H"):
-            walk(i + 1, j, score + (ext if last == "V" else open_), "V")
+            walk(i + 1, j, score + (max(ext, open_) if last == "V" else open_), "V")
         if j < L2:
-            walk(i, j + 1, score + (ext if last == "H" else open_), "H")
+            walk(i, j + 1, score + (max(ext, open_) if last == "H" else open_), "H")
 
     for i0 in range(L1 + 1):
         for j0 in range(L2 + 1):
@@ -45,7 +46,7 @@ def brute(ref, read, sc, variant):
     return (mx, *pos)
 
 
-SCORINGS = [(2, -3, -4, -2), (1, -4, -2, -1), (3, -1, -5, -1), (2, -2, -1, -1)]
+SCORINGS = [(2, -3, -4, -2), (1, -4, -2, -1), (3, -1, -5, -1), (2, -2, -1, -1), (2, -3, -1, -3)]
 
 
 @pytest.mark.parametrize("sc", SCORINGS)
