@@ -74,6 +74,7 @@ def load_library():
     lib.dcns_frag_set_pair.argtypes = [_P, _I, ctypes.POINTER(_L), ctypes.POINTER(_L), ctypes.POINTER(_L)]
     lib.dcns_frag_set_get.argtypes = [_P, _L, _P, _P, _P, _I, ctypes.POINTER(_I)]
     lib.dcns_frag_set_free.argtypes = [_P]
+    lib.dcns_fit_lambda_k.argtypes = [_P, _L, _P, _D, _D, _D, _I, _P]
     _lib = lib
     return lib
 
@@ -84,7 +85,7 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_xdrop_extend", "dcns_seed_sets", "dcns_find_similar_batch", "dcns_stage",
            "dcns_find_similar_staged", "dcns_frag_set_total",
            "dcns_frag_set_pair", "dcns_frag_set_get", "dcns_frag_set_free", "dcns_cut1gap_main",
-           "dcns_ransco_main"]
+           "dcns_ransco_main", "dcns_fit_lambda_k"]
 
 
 def default_params(**kw):
@@ -93,6 +94,21 @@ def default_params(**kw):
     for k, v in kw.items():
         setattr(p, "lambda_" if k == "lambda" else k, v)
     return p
+
+
+def fit_lambda_k(scores, log_space=None, fixed_space=1e6, k0=0.3, lambda0=0.2, maxiter=550):
+    """FP64 Levenberg-Marquardt fit of (K, lambda) to maximum local scores
+    (host code of the library; see dcns_fit_lambda_k).  Returns
+    (K, lambda, rss, iterations)."""
+    lib = load_library()
+    sc = np.ascontiguousarray(scores, np.int32)
+    ls = None if log_space is None else np.ascontiguousarray(log_space, np.float64)
+    out = np.zeros(4, np.float64)
+    rc = lib.dcns_fit_lambda_k(sc.ctypes.data, len(sc), None if ls is None else ls.ctypes.data, float(fixed_space),
+                               float(k0), float(lambda0), int(maxiter), out.ctypes.data)
+    if rc != 0:
+        raise DcnsError(f"dcns_fit_lambda_k failed ({rc})")
+    return float(out[0]), float(out[1]), float(out[2]), int(out[3])
 
 
 class _BatchArrays:

@@ -142,6 +142,18 @@ int dcns_frag_set_get(const dcns_frag_set *fs, int64_t idx, int32_t *ints5, doub
                       int32_t cig_cap, int32_t *ncig);
 void dcns_frag_set_free(dcns_frag_set *fs);
 
+/* Least-squares fit of lambda and K to maximum local scores of random pairs
+ * (the FP64 step of the permutation method; host code).  Replaces the
+ * reference's R fit, scripts/scriptToestimateKandLambda.r:4-15 (nlsLM of
+ * y ~ 1 - exp(-K*N*exp(-lambda*x)) over the empirical survival y(x) of the
+ * sorted unique scores x, fed by ranSco, permutationLsqLambdaK.cpp:97-100).
+ * N is fixed_space for every pair when log_space is NULL (the reference's
+ * N = 1e6), else pair i contributes ln(m_i*n_i) = log_space[i] and the model
+ * averages the survival over the pairs' search spaces.  out4 = K, lambda,
+ * residual sum of squares, iterations. */
+int dcns_fit_lambda_k(const int32_t *scores, int64_t n, const double *log_space, double fixed_space, double k0,
+                      double lambda0, int32_t maxiter, double *out4);
+
 /* Commands with the reference's flags, files and output text (argv[0] is the
  * command name):
  *   dcns_cut1gap_main  int cut1Gap(int argc, char** argv), controlLayer4.cpp:1441
