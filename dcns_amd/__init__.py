@@ -7,6 +7,7 @@ All computation runs in the library's HIP kernels; there is no CPU fallback:
 visible.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -75,6 +76,16 @@ def load_library():
     lib.dcns_frag_set_get.argtypes = [_P, _L, _P, _P, _P, _I, ctypes.POINTER(_I)]
     lib.dcns_frag_set_free.argtypes = [_P]
     lib.dcns_fit_lambda_k.argtypes = [_P, _L, _P, _D, _D, _D, _I, _P]
+    lib.dcns_mlp_create.argtypes = [_P, ctypes.c_uint64, ctypes.POINTER(_P)]
+    lib.dcns_mlp_destroy.argtypes = [_P]
+    lib.dcns_mlp_num_params.restype = _L
+    lib.dcns_mlp_num_params.argtypes = [_P]
+    lib.dcns_mlp_set_scaling.argtypes = [_P, _P, _P, _P, _P]
+    lib.dcns_mlp_get_params.argtypes = [_P, _P, _L]
+    lib.dcns_mlp_set_params.argtypes = [_P, _P, _L]
+    lib.dcns_mlp_predict.argtypes = [_P, _P, _L, _P]
+    lib.dcns_mlp_loss_grad.argtypes = [_P, _P, _P, _L, _P, _P, _L]
+    lib.dcns_mlp_train.argtypes = [_P, _P, _P, _L, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _P]
     _lib = lib
     return lib
 
@@ -85,7 +96,9 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_xdrop_extend", "dcns_seed_sets", "dcns_find_similar_batch", "dcns_stage",
            "dcns_find_similar_staged", "dcns_frag_set_total",
            "dcns_frag_set_pair", "dcns_frag_set_get", "dcns_frag_set_free", "dcns_cut1gap_main",
-           "dcns_ransco_main", "dcns_fit_lambda_k"]
+           "dcns_ransco_main", "dcns_fit_lambda_k", "dcns_mlp_create", "dcns_mlp_destroy", "dcns_mlp_num_params",
+           "dcns_mlp_set_scaling", "dcns_mlp_get_params", "dcns_mlp_set_params", "dcns_mlp_predict",
+           "dcns_mlp_loss_grad", "dcns_mlp_train"]
 
 
 def default_params(**kw):
@@ -160,9 +173,12 @@ class Context:
         if rc != 0:
             raise DcnsError(f"dcns_ctx_create failed ({rc}): {self.lib.dcns_last_error(None).decode()}")
         self.h = h
+        self._children = weakref.WeakSet()  # networks bound to this context
 
     def close(self):
         if getattr(self, "h", None):
+            for child in list(self._children):
+                child.close()
             self.lib.dcns_ctx_destroy(self.h)
             self.h = None
 
@@ -299,3 +315,74 @@ class Context:
             return res
         finally:
             self.lib.dcns_frag_set_free(fs)
+
+
+class Mlp:
+    """The lambda/K surrogate network on the device (dcns_mlp_*): 8 inputs ->
+    4 x 256 GELU -> (ln lambda, ln K), BF16 matrix-core GEMMs with FP32
+    accumulation and FP32 AdamW master weights."""
+
+    N_IN, N_OUT = 8, 2
+
+    def __init__(self, ctx, seed=0):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = _P()
+        ctx._check(self.lib.dcns_mlp_create(ctx.h, int(seed), ctypes.byref(h)), "dcns_mlp_create")
+        self.h = h
+        self.n_params = int(self.lib.dcns_mlp_num_params(h))
+        ctx._children.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dcns_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _f32(a, cols=None):
+        a = np.ascontiguousarray(a, np.float32)
+        if cols is not None and (a.ndim != 2 or a.shape[1] != cols):
+            raise ValueError(f"expected an [n x {cols}] array, got shape {a.shape}")
+        return a
+
+    def set_scaling(self, in_mean, in_std, out_mean, out_std):
+        arrs = [self._f32(v).ravel() for v in (in_mean, in_std, out_mean, out_std)]
+        if [len(a) for a in arrs] != [8, 8, 2, 2]:
+            raise ValueError("scaling takes 8 input and 2 output means / deviations")
+        self.ctx._check(self.lib.dcns_mlp_set_scaling(self.h, *[a.ctypes.data for a in arrs]), "dcns_mlp_set_scaling")
+
+    def get_params(self):
+        out = np.zeros(self.n_params, np.float32)
+        self.ctx._check(self.lib.dcns_mlp_get_params(self.h, out.ctypes.data, len(out)), "dcns_mlp_get_params")
+        return out
+
+    def set_params(self, flat):
+        flat = self._f32(flat).ravel()
+        self.ctx._check(self.lib.dcns_mlp_set_params(self.h, flat.ctypes.data, len(flat)), "dcns_mlp_set_params")
+
+    def predict(self, X):
+        X = self._f32(X, self.N_IN)
+        Y = np.zeros((len(X), self.N_OUT), np.float32)
+        self.ctx._check(self.lib.dcns_mlp_predict(self.h, X.ctypes.data, len(X), Y.ctypes.data), "dcns_mlp_predict")
+        return Y
+
+    def loss_grad(self, X, Y):
+        X, Y = self._f32(X, self.N_IN), self._f32(Y, self.N_OUT)
+        loss = ctypes.c_float()
+        g = np.zeros(self.n_params, np.float32)
+        self.ctx._check(self.lib.dcns_mlp_loss_grad(self.h, X.ctypes.data, Y.ctypes.data, len(X), ctypes.byref(loss),
+                                                    g.ctypes.data, len(g)), "dcns_mlp_loss_grad")
+        return float(loss.value), g
+
+    def train(self, X, Y, steps, batch=4096, lr=1e-3, weight_decay=1e-4, seed=0):
+        X, Y = self._f32(X, self.N_IN), self._f32(Y, self.N_OUT)
+        losses = np.zeros(max(steps, 1), np.float32)
+        self.ctx._check(self.lib.dcns_mlp_train(self.h, X.ctypes.data, Y.ctypes.data, len(X), int(steps), int(batch),
+                                                float(lr), float(weight_decay), int(seed), losses.ctypes.data),
+                        "dcns_mlp_train")
+        return losses[:steps]

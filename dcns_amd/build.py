@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
           f"-I{CSRC}"]
 
 LIB_SOURCES = ["seedscan.hip", "xdrop.hip", "swbatch.hip", "pipeline.cpp", "capi.cpp", "host/cli_io.cpp",
-               "host/commands.cpp", "host/lambdak.cpp"]
+               "host/commands.cpp", "host/lambdak.cpp", "mlp.hip"]
 CLI_SOURCES = ["cli/main.cpp"]
 
 

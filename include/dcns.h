@@ -164,6 +164,43 @@ int dcns_fit_lambda_k(const int32_t *scores, int64_t n, const double *log_space,
 int dcns_cut1gap_main(int argc, char **argv);
 int dcns_ransco_main(int argc, char **argv);
 
+/* lambda/K surrogate network.  No counterpart in the reference, which obtains
+ * lambda and K only by permutation (ranSco, permutationLsqLambdaK.cpp:32-102,
+ * then scripts/scriptToestimateKandLambda.r).  Inputs per row (8 floats):
+ * match, mismatch, gap open, gap extend, long-gap open, GC fraction,
+ * ln(reference length), ln(query length); outputs (2 floats): ln(lambda),
+ * ln(K).  8 -> 4 x 256 (GELU) -> 2, BF16 operands on the matrix cores with
+ * FP32 accumulation, FP32 master weights trained by AdamW on the device.
+ *
+ * Parameters cross the boundary as FP32 in the logical order: for each of the
+ * five layers, W [out x in] row-major, then b [out] (200,194 values).  Inputs
+ * and outputs are standardised inside the network by the scaling set with
+ * dcns_mlp_set_scaling (identity by default); dcns_mlp_predict returns
+ * unstandardised outputs; dcns_mlp_loss_grad and dcns_mlp_train use the mean
+ * squared error of the standardised outputs. */
+typedef struct dcns_mlp dcns_mlp;
+/* A network lives on its context's device and stream: destroy it before the
+ * context. */
+int dcns_mlp_create(dcns_ctx *ctx, uint64_t seed, dcns_mlp **out);
+void dcns_mlp_destroy(dcns_mlp *m);
+int64_t dcns_mlp_num_params(const dcns_mlp *m);
+int dcns_mlp_set_scaling(dcns_mlp *m, const float *in_mean8, const float *in_std8, const float *out_mean2,
+                         const float *out_std2);
+int dcns_mlp_get_params(const dcns_mlp *m, float *out, int64_t cap);
+int dcns_mlp_set_params(dcns_mlp *m, const float *in, int64_t n);
+/* X [n x 8] -> Y [n x 2] (ln lambda, ln K), batches of 65,536 rows. */
+int dcns_mlp_predict(dcns_mlp *m, const float *X, int64_t n, float *Y);
+/* One forward and backward pass over all n rows, no update: loss and the
+ * gradient in the logical parameter order (grad may be NULL). */
+int dcns_mlp_loss_grad(dcns_mlp *m, const float *X, const float *Y, int64_t n, float *loss, float *grad,
+                       int64_t grad_cap);
+/* AdamW (beta 0.9 / 0.999, eps 1e-8, decoupled decay on weights only) over
+ * `steps` minibatches of `batch` rows drawn with replacement (seeded), or,
+ * when batch >= n, `steps` full-batch steps over the rows in order;
+ * losses[steps] (optional) receives each step's loss. */
+int dcns_mlp_train(dcns_mlp *m, const float *X, const float *Y, int64_t n, int32_t steps, int32_t batch, float lr,
+                   float weight_decay, uint64_t seed, float *losses);
+
 #ifdef __cplusplus
 }
 #endif
