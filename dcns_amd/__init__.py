@@ -40,7 +40,7 @@ class Batch(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("scan_cells", _L), ("xdrop_cells", _L), ("seeds", _L), ("linked_seeds", _L), ("fragments", _L),
                 ("gpu_scan_cells", _L), ("levels", _I), ("pad", _I), ("ms_seed", _D), ("ms_replay", _D),
-                ("ms_xdrop", _D), ("ms_total", _D)]
+                ("ms_xdrop", _D), ("ms_total", _D), ("semiglobal_cells", _L), ("ms_semiglobal", _D)]
 
 
 _lib = None
@@ -76,6 +76,9 @@ def load_library():
     lib.dcns_frag_set_get.argtypes = [_P, _L, _P, _P, _P, _I, ctypes.POINTER(_I)]
     lib.dcns_frag_set_free.argtypes = [_P]
     lib.dcns_fit_lambda_k.argtypes = [_P, _L, _P, _D, _D, _D, _I, _P]
+    lib.dcns_find_similar_2gaps_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), ctypes.POINTER(_P)]
+    lib.dcns_semiglobal_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _P, _I, _I, _P,
+                                          _P, _P, _L]
     lib.dcns_mlp_create.argtypes = [_P, ctypes.c_uint64, ctypes.POINTER(_P)]
     lib.dcns_mlp_destroy.argtypes = [_P]
     lib.dcns_mlp_num_params.restype = _L
@@ -98,7 +101,8 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_frag_set_pair", "dcns_frag_set_get", "dcns_frag_set_free", "dcns_cut1gap_main",
            "dcns_ransco_main", "dcns_fit_lambda_k", "dcns_mlp_create", "dcns_mlp_destroy", "dcns_mlp_num_params",
            "dcns_mlp_set_scaling", "dcns_mlp_get_params", "dcns_mlp_set_params", "dcns_mlp_predict",
-           "dcns_mlp_loss_grad", "dcns_mlp_train"]
+           "dcns_mlp_loss_grad", "dcns_mlp_train", "dcns_find_similar_2gaps_batch", "dcns_semiglobal_batch",
+           "dcns_cut2gaps_main", "dcns_paircns2gaps_main", "dcns_paircnsxextend_main"]
 
 
 def default_params(**kw):
@@ -274,6 +278,38 @@ class Context:
         self._check(self.lib.dcns_find_similar_batch(self.h, ctypes.byref(ba.b), ctypes.byref(p), ctypes.byref(fs)),
                     "dcns_find_similar_batch")
         return self._collect(fs, ba.b.n, with_cigar)
+
+    def find_similar_2gaps(self, pairs=None, params=None, flat=None):
+        """findSimilarFragmentsForPairedSequence overload 4 (the two-piece path)
+        per pair: [(seeds_size, [(start1, end1, start2, end2, score, p, e, cigar), ...]), ...]."""
+        ba = _BatchArrays.from_flat(*flat) if flat is not None else _BatchArrays(pairs)
+        p = params or default_params()
+        fs = _P()
+        self._check(self.lib.dcns_find_similar_2gaps_batch(self.h, ctypes.byref(ba.b), ctypes.byref(p),
+                                                           ctypes.byref(fs)), "dcns_find_similar_2gaps_batch")
+        return self._collect(fs, ba.b.n, True)
+
+    def semiglobal(self, pairs, pair_of_task, anchors, reverse=None, params=None, with_cigar=True):
+        """SemiGlobal for many tasks: (out3 [n x 3], cigars or None)."""
+        ba = _BatchArrays(pairs)
+        p = params or default_params()
+        pt = np.ascontiguousarray(pair_of_task, np.int32)
+        an = np.ascontiguousarray(anchors, np.int32).reshape(-1)
+        n = len(pt)
+        rv = None if reverse is None else np.ascontiguousarray(reverse, np.int32)
+        out = np.zeros(3 * max(n, 1), np.int32)
+        lens = [len(a) + len(b) + 2 for a, b in pairs]
+        cap = max(int(sum(lens[int(i)] for i in pt)), 16) if with_cigar else 16
+        cig = np.zeros(cap, np.uint32)
+        off = np.zeros(n + 1, np.int64)
+        self._check(self.lib.dcns_semiglobal_batch(self.h, ctypes.byref(ba.b), ctypes.byref(p), pt.ctypes.data,
+                                                   an.ctypes.data, None if rv is None else rv.ctypes.data, n,
+                                                   1 if with_cigar else 0, out.ctypes.data, cig.ctypes.data,
+                                                   off.ctypes.data, cap), "dcns_semiglobal_batch")
+        res = out[:3 * n].reshape(n, 3)
+        if not with_cigar:
+            return res, None
+        return res, [[int(c) for c in cig[off[i]:off[i + 1]]] for i in range(n)]
 
     def stage(self, pairs=None, params=None, flat=None):
         """Keeps a batch resident in device memory for find_similar_staged()."""

@@ -27,7 +27,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
           f"-I{CSRC}"]
 
 LIB_SOURCES = ["seedscan.hip", "xdrop.hip", "swbatch.hip", "pipeline.cpp", "capi.cpp", "host/cli_io.cpp",
-               "host/commands.cpp", "host/lambdak.cpp", "mlp.hip"]
+               "host/commands.cpp", "host/lambdak.cpp", "mlp.hip",
+               "semiglobal.hip", "twogaps.cpp"]
 CLI_SOURCES = ["cli/main.cpp"]
 
 

@@ -334,6 +334,86 @@ int dcns_find_similar_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params 
     return DCNS_OK;
 }
 
+int dcns_find_similar_2gaps_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, dcns_frag_set **out) {
+    if (!c || !out) return DCNS_ERR_ARG;
+    *out = nullptr;
+    dcns_frag_set *fs = new (std::nothrow) dcns_frag_set();
+    if (!fs) return DCNS_ERR_INTERNAL;
+    int rc = guarded(c, [&] {
+        check_batch(b);
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        c->staged.reset();
+        find_similar_2gaps(c, *b, *p, fs->fs);
+    });
+    if (rc != DCNS_OK) {
+        delete fs;
+        return rc;
+    }
+    *out = fs;
+    return DCNS_OK;
+}
+
+int dcns_semiglobal_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, const int32_t *pair_of_task,
+                          const int32_t *anchors, const int32_t *reverse, int32_t n, int32_t with_cigar,
+                          int32_t *out3, uint32_t *cigar, int64_t *cig_off, int64_t cig_cap) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        check_batch(b);
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        validate_params(*p);
+        if (p->open2 > 0 || p->ext2 > 0 || p->zdrop < 0 || p->bandwidth < 2)
+            throw Error(DCNS_ERR_ARG, "open2/ext2 must be <= 0, zdrop >= 0, bandwidth >= 2");
+        if (n < 0 || (n > 0 && (!pair_of_task || !anchors || !out3)) || (with_cigar && n > 0 && !cig_off))
+            throw Error(DCNS_ERR_ARG, "invalid task arrays");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        Params P = to_params(*p);
+        c->staged.reset();
+        Staged S = stage_batch(c, *b, P);
+        std::vector<SgTask> tasks((size_t)n);
+        for (int32_t i = 0; i < n; ++i) {
+            const int32_t pr = pair_of_task[i];
+            if (pr < 0 || pr >= b->n) throw Error(DCNS_ERR_ARG, "pair index out of range");
+            const int32_t a1 = anchors[2 * i], a2 = anchors[2 * i + 1], L1 = b->len1[pr], L2 = b->len2[pr];
+            const bool rev = reverse && reverse[i];
+            SgTask t{};
+            t.pair = pr;
+            t.rev = rev ? 1 : 0;
+            t.a1 = a1;
+            t.a2 = a2;
+            if (rev) {
+                if (a1 < 0 || a1 >= L1 || a2 < 0 || a2 >= L2) throw Error(DCNS_ERR_ARG, "reverse anchor outside its sequences");
+                t.R = a1 + 1;
+                t.C = a2 + 1;
+            } else {
+                if (a1 < 0 || a1 > L1 || a2 < 0 || a2 > L2) throw Error(DCNS_ERR_ARG, "anchor outside its sequences");
+                t.R = L1 - a1;
+                t.C = L2 - a2;
+            }
+            tasks[(size_t)i] = t;
+        }
+        std::vector<SgResult> res;
+        int64_t cells = 0;
+        semiglobal_batch(c, P, tasks, with_cigar != 0, res, cells);
+        int64_t o = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const SgResult &r = res[(size_t)i];
+            out3[3 * i] = r.max;
+            out3[3 * i + 1] = r.end1;
+            out3[3 * i + 2] = r.end2;
+            if (with_cigar) {
+                cig_off[i] = o;
+                if (o + (int64_t)r.cigar.size() > cig_cap) throw Error(DCNS_ERR_CAPACITY, "cigar capacity");
+                if (!r.cigar.empty()) std::memcpy(cigar + o, r.cigar.data(), r.cigar.size() * 4);
+                o += (int64_t)r.cigar.size();
+            }
+        }
+        if (with_cigar) cig_off[n] = o;
+        c->stats = dcns_stats{};
+        c->stats.semiglobal_cells = cells;
+    });
+}
+
 int dcns_stage(dcns_ctx *c, const dcns_batch *b, const dcns_params *p) {
     if (!c) return DCNS_ERR_ARG;
     return guarded(c, [&] {

@@ -10,6 +10,9 @@ static void usage() {
               << "Usage: dCNS <command> [options]\n"
               << "Commands:\n"
               << "    cut1Gap             masking genome, sliding window seed, x-drop extension\n"
+              << "    cut2Gaps            masking genome, sliding window seed, x-drop and two-piece gap extension\n"
+              << "    pairCnsXExtend      sliding window seed and x-drop extension of the sequences of one fasta file\n"
+              << "    pairCns2Gaps        the same with two-piece gap extension\n"
               << "    ranSco              smith-waterman scores of random sequence fragments\n"
               << "                        for lambda and k fitting\n";
 }
@@ -26,6 +29,9 @@ int main(int argc, char **argv) {
     }
     if (!std::strcmp(p, "cut1Gap")) return dcns_cut1gap_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "ranSco")) return dcns_ransco_main(argc - 1, argv + 1);
+    if (!std::strcmp(p, "cut2Gaps")) return dcns_cut2gaps_main(argc - 1, argv + 1);
+    if (!std::strcmp(p, "pairCns2Gaps")) return dcns_paircns2gaps_main(argc - 1, argv + 1);
+    if (!std::strcmp(p, "pairCnsXExtend")) return dcns_paircnsxextend_main(argc - 1, argv + 1);
     std::cerr << "dCNS: command " << p << " is not part of this build" << std::endl;
     usage();
     return 1;

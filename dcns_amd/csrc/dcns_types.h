@@ -88,4 +88,23 @@ struct XOut {
     int64_t cells;                // DP cells evaluated (all passes)
 };
 
+// One two-piece semi-global DP (SemiGlobal): rows = seq1 from a1, columns =
+// seq2 from a2, forward, or on the reverse complements of the prefixes that
+// end at a1 / a2 (rev = 1).
+struct SgTask {
+    int32_t pair, rev, a1, a2;
+    int32_t R, C;     // rows (seq1 length) and columns (seq2 length), both >= 1
+    int64_t tb;       // byte offset of the traceback store, -1: score only
+    int64_t scr;      // int32 offset of the boundary / row-result scratch
+};
+
+struct SgOut {
+    int32_t max, e1, e2;   // maximum and its 1-based row/column (-1: none)
+    int32_t brk;           // last row evaluated (the termination row, or R)
+    int64_t cells;         // cells of rows 1..brk, as the reference evaluates them
+    int32_t ncig, stale_set;
+    int64_t cig_off;       // cigar arena offset (runs in traceback order)
+    int32_t stale[2];      // M(C+w-2, C-1), M(C+w-1, C-1): the diagonals rows past C+w read
+};
+
 }  // namespace dcns

@@ -364,5 +364,41 @@ void write_sam_and_o(const std::string &prefix, const std::vector<PairedSimilarF
     }
 }
 
+void write_sam_and_o_unmasked(const std::string &prefix, const std::vector<PairedSimilarFragment> &frags,
+                              int32_t refStrand, int32_t queStrand, int32_t refStart, int32_t queStart,
+                              const std::string &refChr, const std::string &queChr, const std::string &seq1,
+                              const std::string &seq2) {
+    std::ofstream sam(prefix + ".sam", std::ios_base::app);
+    std::ofstream o(prefix + "o", std::ios_base::app);
+    o << "#" << refChr << ":" << refStrand << ":" << refStart << "-" << queChr << ":" << queStrand << ":" << queStart
+      << std::endl;
+    for (const PairedSimilarFragment &f : frags) {
+        if ((refStrand == 1 || refStrand == 0) && (queStrand == 1 || queStrand == 0)) {
+            const bool rplus = refStrand == 1, qplus = queStrand == 1;
+            const int32_t quePos = qplus ? (int32_t)(queStart + f.getStart2() - 1)
+                                         : (int32_t)(queStart + seq2.size() - 1 - f.getEnd2() + 1);
+            const int32_t refPos = rplus ? (int32_t)(refStart + f.getStart1() - 1)
+                                         : (int32_t)(refStart + seq1.size() - 1 - f.getEnd1() + 1);
+            const int flag = (refStrand == queStrand) ? 0 : 16;
+            sam << queChr << "\t" << flag << "\t" << refChr << "\t" << refPos << "\t" << f.getScore() << "\t"
+                << quePos << "H";
+            const std::vector<uint32_t> &cg = f.getCigar();
+            if (rplus) {
+                for (uint32_t c : cg) sam << (c >> 4) << "MID"[c & 0xfu];
+            } else {
+                for (int32_t j = (int32_t)cg.size() - 1; j >= 0; --j) sam << (cg[(size_t)j] >> 4) << "MID"[cg[(size_t)j] & 0xfu];
+            }
+            std::string s2 = seq2;
+            const std::string text = rplus ? sub_sequence(s2, (int)f.getStart2(), (int)f.getEnd2())
+                                           : sub_sequence(s2, (int)f.getStart2(), (int)f.getEnd2(), 0);
+            sam << "\t*\t0\t0\t" << text << "\t*" << std::endl;
+        }
+        o << f.getStart1() << " " << f.getEnd1() << " " << f.getStart2() << " " << f.getEnd2() << " " << f.getScore()
+          << " " << f.getPValue() << std::endl;
+    }
+    sam.close();
+    o << "#done" << std::endl;
+}
+
 }  // namespace cli
 }  // namespace dcns

@@ -9,6 +9,7 @@
 //   FastaWithIndex         impl/FastaWithIndex.cpp:11-71, impl/FastaIndexEntryImpl.cpp:8-28
 //   getSubSequence / getReverseComplementary  impl/fasta.cpp:152-217
 //   outputMasking          controlLayer4.cpp:835-1358
+//   outputNoMasking        controlLayer4.cpp:56-117
 #pragma once
 #include <cstdint>
 #include <fstream>
@@ -68,6 +69,13 @@ std::vector<int8_t> encode(const std::string &s);
 void write_sam_and_o(const std::string &prefix, const std::vector<PairedSimilarFragment> &frags, int32_t refStrand,
                      int32_t queStrand, int32_t refStart, int32_t queStart, const std::string &refChr,
                      const std::string &queChr, const std::string &seq1_0, const std::string &seq2_0);
+
+// outputNoMasking, controlLayer4.cpp:56-117: coordinates without masked
+// runs, SAM cigar ops "MID", header line first and "#done" last in the o file.
+void write_sam_and_o_unmasked(const std::string &prefix, const std::vector<PairedSimilarFragment> &frags,
+                              int32_t refStrand, int32_t queStrand, int32_t refStart, int32_t queStart,
+                              const std::string &refChr, const std::string &queChr, const std::string &seq1,
+                              const std::string &seq2);
 
 }  // namespace cli
 }  // namespace dcns

@@ -2,6 +2,9 @@
 // reference's flags, defaults, files and output text:
 //   cut1Gap  controlLayer4.cpp:1361-1529
 //   ranSco   controlLayer4.cpp:2088-2149, impl/permutationLsqLambdaK.cpp:7-102
+//   cut2Gaps controlLayer4.cpp:1534-1713
+//   pairCns2Gaps controlLayer4.cpp:318-462
+//   pairCnsXExtend controlLayer4.cpp:181-317
 // The alignment work runs on the GPU; the host keeps the reference's order
 // of file operations and stdout lines.  All query regions of one cut1Gap
 // input are aligned in one batched call.
@@ -60,6 +63,55 @@ void cut1gap_usage(const Defaults &d) {
               << " -k DOUBLE  K for p-values (default: " << d.k << ")\n"
               << " -p DOUBLE  p-value threshold (default: " << d.pvalues << ")\n"
               << std::endl;
+}
+
+void two_gaps_usage(const char *cmd, const Defaults &d, bool genomes) {
+    std::cerr << "Usage: dCNS " << cmd << " -i input -r reference -o output"
+              << (genomes ? " -ra refGenome.fa -qa queryGenome.fa" : "") << "\n"
+              << "Options\n"
+              << " -i FILE    input file in fasta format\n"
+              << " -r STRING  reference sequence entry name in input fasta file\n";
+    if (genomes)
+        std::cerr << " -ra STRING masked reference genome sequence in fasta format\n"
+                  << " -qa STRING masked query genome sequence in fasta format\n";
+    std::cerr << " -o FILE    output prefix\n"
+              << " -A INT     matching score (default: " << d.match << ")\n"
+              << " -B INT     mismatching penalty (default: " << d.mismatch << ")\n"
+              << " -O1 INT    open gap penalty (default: " << d.open1 << ")\n"
+              << " -E1 INT    extend gap penalty (default: " << d.ext1 << ")\n"
+              << " -u INT     x-drop extension band width (default: " << d.w << ")\n"
+              << " -O2 INT    second-piece open gap penalty (default: " << d.open2 << ")\n"
+              << " -E2 INT    second-piece extend gap penalty (default: " << d.ext2 << ")\n"
+              << " -u2 INT    z-drop alignment band width (default: " << d.bandwidth << ")\n"
+              << " -w INT     seed window size (default: " << d.window << ")\n"
+              << " -c INT     minimum seed score (default: " << d.min_score << ")\n"
+              << " -s INT     seed window step (default: " << d.step << ")\n"
+              << " -l DOUBLE  lambda for p-values (default: " << d.lambda << ")\n"
+              << " -k DOUBLE  K for p-values (default: " << d.k << ")\n"
+              << " -p DOUBLE  p-value threshold (default: " << d.pvalues << ")\n"
+              << std::endl;
+}
+
+dcns_params to_dcns_params(const Defaults &d) {
+    dcns_params p;
+    dcns_default_params(&p);
+    p.match = d.match;
+    p.mismatch = d.mismatch;
+    p.open = d.open1;
+    p.ext = d.ext1;
+    p.open2 = d.open2;
+    p.ext2 = d.ext2;
+    p.zdrop = d.zdrop;
+    p.bandwidth = d.bandwidth;
+    p.window = d.window;
+    p.step = d.step;
+    p.min_score = d.min_score;
+    p.w = d.w;
+    p.xdrop = d.xdrop;
+    p.pvalue = d.pvalues;
+    p.lambda = d.lambda;
+    p.k = d.k;
+    return p;
 }
 
 }  // namespace
@@ -186,6 +238,199 @@ int cut1gap_main(int argc, char **argv) {
     return 0;
 }
 
+// cut2Gaps (controlLayer4.cpp:1534-1713): whole genomes read without an
+// index, every query region aligned by the two-piece path (overload 4), SAM /
+// o output without "#done" lines.  All query regions run as one batch.
+int cut2gaps_main(int argc, char **argv) {
+    Defaults d;
+    Args a(argc, argv);
+    if (a.has("-A")) d.match = std::stoi(a.get("-A"));
+    if (a.has("-B")) d.mismatch = std::stoi(a.get("-B"));
+    if (a.has("-O1")) d.open1 = std::stoi(a.get("-O1"));
+    if (a.has("-E1")) d.ext1 = std::stoi(a.get("-E1"));
+    if (a.has("-O2")) d.open2 = std::stoi(a.get("-O2"));
+    if (a.has("-E2")) d.ext2 = std::stoi(a.get("-E2"));
+    if (a.has("-u")) d.w = std::stoi(a.get("-u"));
+    if (a.has("-u2")) d.bandwidth = std::stoi(a.get("-u2"));
+    if (a.has("-y")) d.syntenic = str2bool(a.get("-y"), d.syntenic);
+    if (a.has("-w")) d.window = std::stoi(a.get("-w"));
+    if (a.has("-c")) d.min_score = std::stoi(a.get("-c"));
+    if (a.has("-s")) d.step = std::stoi(a.get("-s"));
+    if (a.has("-b")) d.boundary = std::stoi(a.get("-b"));
+    if (a.has("-l")) d.lambda = std::stod(a.get("-l"));
+    if (a.has("-k")) d.k = std::stod(a.get("-k"));
+    if (a.has("-p")) d.pvalues = std::stod(a.get("-p"));
+    if (a.has("-h") || a.has("--help") ||
+        !(a.has("-i") && a.has("-r") && a.has("-o") && a.has("-ra") && a.has("-qa"))) {
+        two_gaps_usage("cut2Gaps", d, true);
+        return 1;
+    }
+    if (d.syntenic) {
+        std::cerr << "dCNS: syntenic filtering (-y) is outside this build's scope" << std::endl;
+        return 1;
+    }
+    const std::string input = a.get("-i"), reference = a.get("-r"), output = a.get("-o");
+    std::map<std::string, std::string> refSeqs, queSeqs, seqs;
+    std::vector<std::string> refNames, queNames, names;
+    read_plain_fasta(a.get("-ra"), refSeqs, refNames);
+    read_plain_fasta(a.get("-qa"), queSeqs, queNames);
+    std::map<std::string, Meta> meta;
+    read_meta_fasta(input, seqs, meta, names);
+    const Meta refMeta = meta[reference];
+    std::string seq1_0 = sub_sequence(refSeqs[refMeta.chr], refMeta.start, refMeta.end, refMeta.strand);
+    std::string seq1 = seq1_0;
+    seq1.erase(std::remove(seq1.begin(), seq1.end(), 'n'), seq1.end());
+    const std::vector<int8_t> c1 = encode(seq1);
+    refSeqs.clear();
+    struct Q {
+        std::string name, seq2_0;
+        std::vector<int8_t> codes;
+    };
+    std::vector<Q> qs;
+    std::vector<int8_t> buf(c1.begin(), c1.end());
+    std::vector<int64_t> off1, off2;
+    std::vector<int32_t> len1, len2;
+    for (const std::string &q : names) {
+        if (q.compare(reference) == 0) continue;
+        Q x;
+        x.name = q;
+        const Meta &m = meta[q];
+        x.seq2_0 = sub_sequence(queSeqs[m.chr], m.start, m.end, m.strand);
+        std::string s2 = x.seq2_0;
+        s2.erase(std::remove(s2.begin(), s2.end(), 'n'), s2.end());
+        x.codes = encode(s2);
+        off1.push_back(0);
+        len1.push_back((int32_t)c1.size());
+        off2.push_back((int64_t)buf.size());
+        len2.push_back((int32_t)x.codes.size());
+        buf.insert(buf.end(), x.codes.begin(), x.codes.end());
+        qs.push_back(std::move(x));
+    }
+    FragSet fs;
+    if (!qs.empty()) {
+        dcns_params p = to_dcns_params(d);
+        dcns_batch b{buf.data(), off1.data(), len1.data(), off2.data(), len2.data(), (int32_t)qs.size(), 0};
+        dcns_ctx *ctx = make_ctx();
+        try {
+            find_similar_2gaps(ctx, b, p, fs);
+        } catch (const std::exception &e) {
+            std::cerr << "dCNS: " << e.what() << std::endl;
+            dcns_ctx_destroy(ctx);
+            return 1;
+        }
+        dcns_ctx_destroy(ctx);
+    }
+    for (size_t i = 0; i < qs.size(); ++i) {
+        const Meta &m = meta[qs[i].name];
+        std::vector<PairedSimilarFragment> fr(fs.frags.begin() + fs.first[i], fs.frags.begin() + fs.first[i] + fs.count[i]);
+        write_sam_and_o(output + m.species, fr, refMeta.strand, m.strand, refMeta.start, m.start, refMeta.chr, m.chr,
+                        seq1_0, qs[i].seq2_0);
+    }
+    return 0;
+}
+
+// The pairCns commands: reference and query sequences inline in one FASTA
+// file with region metadata, outputNoMasking.  pairCnsXExtend
+// (controlLayer4.cpp:181-317) runs overload 1 and prints each sequence;
+// pairCns2Gaps (controlLayer4.cpp:318-462) runs the two-piece overload 4.
+//
+// The reference's pairCns2Gaps clears its sequence map after taking the
+// reference (controlLayer4.cpp:346), so every query it aligns is empty, and
+// getAllExtendSeed then scores a window of negative width (sp = 8 > L2 = 0),
+// whose SmithWaterman allocation throws std::bad_alloc: the reference command
+// aborts on every input that has a query.  Here the queries are the
+// sequences of the input file, as in pairCnsXExtend, whose map is not
+// cleared (controlLayer4.cpp:201).
+namespace {
+int paircns_common(int argc, char **argv, bool two_piece) {
+    Defaults d;
+    Args a(argc, argv);
+    if (a.has("-A")) d.match = std::stoi(a.get("-A"));
+    if (a.has("-B")) d.mismatch = std::stoi(a.get("-B"));
+    if (two_piece) {
+        if (a.has("-O1")) d.open1 = std::stoi(a.get("-O1"));
+        if (a.has("-E1")) d.ext1 = std::stoi(a.get("-E1"));
+        if (a.has("-O2")) d.open2 = std::stoi(a.get("-O2"));
+        if (a.has("-E2")) d.ext2 = std::stoi(a.get("-E2"));
+    } else {
+        if (a.has("-O")) d.open1 = std::stoi(a.get("-O"));
+        if (a.has("-E")) d.ext1 = std::stoi(a.get("-E"));
+    }
+    if (a.has("-u")) d.w = std::stoi(a.get("-u"));
+    if (two_piece && a.has("-u2")) d.bandwidth = std::stoi(a.get("-u2"));
+    if (a.has("-y")) d.syntenic = str2bool(a.get("-y"), d.syntenic);
+    if (a.has("-w")) d.window = std::stoi(a.get("-w"));
+    if (a.has("-c")) d.min_score = std::stoi(a.get("-c"));
+    if (a.has("-s")) d.step = std::stoi(a.get("-s"));
+    if (a.has("-b")) d.boundary = std::stoi(a.get("-b"));
+    if (a.has("-l")) d.lambda = std::stod(a.get("-l"));
+    if (a.has("-k")) d.k = std::stod(a.get("-k"));
+    if (a.has("-p")) d.pvalues = std::stod(a.get("-p"));
+    if (a.has("-h") || a.has("--help") || !(a.has("-i") && a.has("-r") && a.has("-o"))) {
+        two_gaps_usage(two_piece ? "pairCns2Gaps" : "pairCnsXExtend", d, false);
+        return 1;
+    }
+    if (d.syntenic) {
+        std::cerr << "dCNS: syntenic filtering (-y) is outside this build's scope" << std::endl;
+        return 1;
+    }
+    const std::string input = a.get("-i"), reference = a.get("-r"), output = a.get("-o");
+    std::map<std::string, std::string> seqs;
+    std::map<std::string, Meta> meta;
+    std::vector<std::string> names;
+    read_meta_fasta(input, seqs, meta, names);
+    const std::string seq1 = seqs[reference];
+    const std::vector<int8_t> c1 = encode(seq1);
+    std::vector<std::string> qn;
+    std::vector<int8_t> buf(c1.begin(), c1.end());
+    std::vector<int64_t> off1, off2;
+    std::vector<int32_t> len1, len2;
+    for (const std::string &q : names) {
+        if (q.compare(reference) == 0) continue;
+        const std::vector<int8_t> c2 = encode(seqs[q]);
+        qn.push_back(q);
+        off1.push_back(0);
+        len1.push_back((int32_t)c1.size());
+        off2.push_back((int64_t)buf.size());
+        len2.push_back((int32_t)c2.size());
+        buf.insert(buf.end(), c2.begin(), c2.end());
+    }
+    FragSet fs;
+    if (!qn.empty()) {
+        dcns_params p = to_dcns_params(d);
+        dcns_batch b{buf.data(), off1.data(), len1.data(), off2.data(), len2.data(), (int32_t)qn.size(), 0};
+        dcns_ctx *ctx = make_ctx();
+        try {
+            if (two_piece)
+                find_similar_2gaps(ctx, b, p, fs);
+            else
+                find_similar(ctx, b, p, fs);
+        } catch (const std::exception &e) {
+            std::cerr << "dCNS: " << e.what() << std::endl;
+            dcns_ctx_destroy(ctx);
+            return 1;
+        }
+        dcns_ctx_destroy(ctx);
+    }
+    if (!two_piece) std::cout << "reference:" << seq1 << std::endl;
+    const Meta refMeta = meta[reference];
+    for (size_t i = 0; i < qn.size(); ++i) {
+        const Meta &m = meta[qn[i]];
+        if (!two_piece) {
+            std::cout << "_query:" << qn[i] << ": " << seqs[qn[i]] << std::endl;
+            std::cout << "seeds size" << fs.seeds_size[i] << std::endl;
+        }
+        std::vector<PairedSimilarFragment> fr(fs.frags.begin() + fs.first[i], fs.frags.begin() + fs.first[i] + fs.count[i]);
+        write_sam_and_o_unmasked(output + m.species, fr, refMeta.strand, m.strand, refMeta.start, m.start, refMeta.chr,
+                                 m.chr, seq1, seqs[qn[i]]);
+    }
+    return 0;
+}
+}  // namespace
+
+int paircns2gaps_main(int argc, char **argv) { return paircns_common(argc, argv, true); }
+int paircnsxextend_main(int argc, char **argv) { return paircns_common(argc, argv, false); }
+
 // ranSco: sample random reference/query fragments with the C library's
 // rand() stream exactly as permutationLsqLambdaK does, score every pair's
 // maximum local alignment on the GPU in one batch, print the maxima in
@@ -285,3 +530,6 @@ int ransco_main(int argc, char **argv) {
 
 extern "C" int dcns_cut1gap_main(int argc, char **argv) { return dcns::cli::cut1gap_main(argc, argv); }
 extern "C" int dcns_ransco_main(int argc, char **argv) { return dcns::cli::ransco_main(argc, argv); }
+extern "C" int dcns_cut2gaps_main(int argc, char **argv) { return dcns::cli::cut2gaps_main(argc, argv); }
+extern "C" int dcns_paircns2gaps_main(int argc, char **argv) { return dcns::cli::paircns2gaps_main(argc, argv); }
+extern "C" int dcns_paircnsxextend_main(int argc, char **argv) { return dcns::cli::paircnsxextend_main(argc, argv); }

@@ -34,9 +34,25 @@ hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const
                               int32_t n, const Params &P, XOut *out, const int64_t *tb_off, uint8_t *tb,
                               uint32_t *cigar);
 
+// Wide bands (2w+1 > 161): band rows in per-task global scratch (scr_off[t] < 0: task not handled).
+hipError_t launch_xdrop_wide_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                                   int32_t n, const Params &P, XOut *out, int32_t *scr, const int64_t *scr_off,
+                                   const int32_t *wmax);
+hipError_t launch_xdrop_wide_pass2(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                                   int32_t n, const Params &P, XOut *out, const int64_t *tb_off, uint8_t *tb,
+                                   uint32_t *cigar, int32_t *scr, const int64_t *scr_off);
+
 // swbatch.hip: SmithWaterman max score and first position for arbitrary pairs.
 hipError_t launch_sw_batch(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
                            const int64_t *off2, const int32_t *len2, const int32_t *order, int32_t n,
                            const Params &P, const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride);
+
+// semiglobal.hip: two-piece semi-global DP, one wave per task.
+int64_t sg_trace_bytes(int32_t R, int32_t C, int32_t w);  // traceback store of one task
+int64_t sg_scratch_ints(int32_t R);                       // boundary + row-result scratch of one task
+hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
+                        int32_t n, const Params &P, int32_t *scr, uint8_t *tb, SgOut *out);
+hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks, int32_t n,
+                            const Params &P, const int32_t *scr, const uint8_t *tb, SgOut *out, uint32_t *cigar);
 
 }  // namespace dcns

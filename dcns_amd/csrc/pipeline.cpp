@@ -414,7 +414,7 @@ void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked
 // x-drop extension.
 // ---------------------------------------------------------------------------
 void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vector<XTask> &tasks,
-                  std::vector<XResult> &out, int64_t &cells) {
+                  std::vector<XResult> &out, int64_t &cells, bool score_only) {
     (void)S;
     const int32_t n = (int32_t)tasks.size();
     out.assign((size_t)n, XResult{});
@@ -440,12 +440,63 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
         hip_check(hipMemcpyAsync(&pending, pending_dev, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream), "pending");
         c->sync();
     }
-    if (pending) throw Error(DCNS_ERR_ARG, "x-drop band grew beyond 161 cells");
-    // traceback store and cigar arena sizes from pass 1
     c->h_xout.ensure((size_t)n * sizeof(XOut));
     XOut *xo = c->h_xout.as<XOut>();
+    bool wide = false;
+    if (pending) {
+        // bands past 161 cells: per-task scratch rows in global memory
+        hip_check(hipMemcpyAsync(xo, c->xout.p, (size_t)n * sizeof(XOut), hipMemcpyDeviceToHost, c->stream), "xout");
+        c->sync();
+        std::vector<int64_t> off((size_t)n, -1);
+        std::vector<int32_t> wmax((size_t)n, 0);
+        int64_t total = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            if (xo[i].status != 1 && xo[i].status != 2) continue;
+            const PairDesc &pd = S.pd[(size_t)tasks[(size_t)i].pair];
+            int32_t w = P.w;
+            while (w < std::max(pd.len1, pd.len2)) w *= 2;
+            wmax[(size_t)i] = w;
+            off[(size_t)i] = total;
+            total += 2 * (2 * (int64_t)w + 1);
+        }
+        upload(c, c->xw_off, off);
+        upload(c, c->xw_wmax, wmax);
+        c->xw_scr.ensure((size_t)std::max<int64_t>(total, 16) * 4);
+        c->launch("xdrop_pass1", [&] {
+            return launch_xdrop_wide_pass1(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                           c->xtasks.as<XTask>(), n, P, c->xout.as<XOut>(), c->xw_scr.as<int32_t>(),
+                                           c->xw_off.as<int64_t>(), c->xw_wmax.as<int32_t>());
+        });
+        wide = true;
+    }
+    // traceback store and cigar arena sizes from pass 1
     hip_check(hipMemcpyAsync(xo, c->xout.p, (size_t)n * sizeof(XOut), hipMemcpyDeviceToHost, c->stream), "xout");
     c->sync();
+    if (score_only) {
+        // x_extend_seed without traceback (findSimilarFragmentsForPairedSequence4.cpp:205-241)
+        for (int32_t i = 0; i < n; ++i) cells += xo[i].cells;
+        parallel_for(n, [&](int64_t i) {
+            const XOut &o = xo[i];
+            const XTask &t = tasks[(size_t)i];
+            XResult &r = out[(size_t)i];
+            r.start1 = t.start1;
+            r.start2 = t.start2;
+            r.end1 = r.end2 = 0;
+            r.ok = false;
+            if (o.fe1 < 0 || o.fe2 < 0) {
+                r.score = o.fmax;
+                return;
+            }
+            r.score = o.rmax;
+            if (o.re1 < 0 || o.re2 < 0) return;
+            r.ok = true;
+            r.end1 = t.start1 + o.fe1;
+            r.end2 = t.start2 + o.fe2;
+            r.start1 = t.start1 + o.fe1 - o.re1;
+            r.start2 = t.start2 + o.fe2 - o.re2;
+        });
+        return;
+    }
     std::vector<int64_t> tboff((size_t)n, 0);
     int64_t tbtotal = 0, cigtotal = 0;
     bool any_band[4] = {false, false, false, false};
@@ -468,6 +519,14 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
     upload(c, c->tb_off, tboff);
     c->tb.ensure((size_t)std::max<int64_t>(tbtotal, 64));
     c->cigar.ensure((size_t)std::max<int64_t>(cigtotal, 16) * sizeof(uint32_t));
+    if (wide) {
+        c->launch("xdrop_pass2", [&] {
+            return launch_xdrop_wide_pass2(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                           c->xtasks.as<XTask>(), n, P, c->xout.as<XOut>(), c->tb_off.as<int64_t>(),
+                                           c->tb.as<uint8_t>(), c->cigar.as<uint32_t>(), c->xw_scr.as<int32_t>(),
+                                           c->xw_off.as<int64_t>());
+        });
+    }
     for (int k = 0; k < 4; ++k) {
         if (!any_band[k]) continue;
         c->launch("xdrop_pass2", [&] {

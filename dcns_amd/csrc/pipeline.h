@@ -47,8 +47,22 @@ struct XResult {
     bool ok;  // false: the reference's p = 100 case
     std::vector<uint32_t> cigar;
 };
+// score_only: the variant without traceback (:205-241): untrimmed coordinates, no cigar.
 void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vector<XTask> &tasks,
-                  std::vector<XResult> &out, int64_t &cells);
+                  std::vector<XResult> &out, int64_t &cells, bool score_only = false);
+
+// SemiGlobal (impl/sequenceAlignment3.cpp:932 -> :681) for many tasks on staged
+// pairs.  Task fields used: pair, rev, a1, a2, R, C (R or C <= 0: no DP, the
+// empty result).  end1/end2 are 0-based (-2 when no cell is positive), as the
+// reference returns them; cigar only when trace is set.
+struct SgResult {
+    int32_t max, end1, end2;
+    std::vector<uint32_t> cigar;
+};
+void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &tasks, bool trace,
+                      std::vector<SgResult> &out, int64_t &cells);
+
+
 
 struct FragSet {
     std::vector<PairedSimilarFragment> frags;
@@ -57,6 +71,10 @@ struct FragSet {
 void find_similar(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out);
 // Same on sequences already resident on the device (dcns_stage).
 void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &out);
+// findSimilarFragmentsForPairedSequence overload 4 (:752-774): getAllExtendSeed
+// then overload 2 (:532-616).
+void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &out);
+void find_similar_2gaps(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out);
 
 }  // namespace dcns
 

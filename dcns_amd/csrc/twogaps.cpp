@@ -1,0 +1,296 @@
+// twogaps.cpp -- host orchestration of the two-piece-affine path:
+//   findSimilarFragmentsForPairedSequence overload 4
+//   (impl/findSimilarFragmentsForPairedSequence4.cpp:752-774) =
+//   getAllExtendSeed (:348-429) followed by overload 2 (:532-616),
+// with every SemiGlobal call (impl/sequenceAlignment3.cpp:932 -> :681) run as
+// batched device tasks.
+//
+// Order of work per batch: seed sets (device scan + host std::set replay),
+// score-only x-drop of every linked seed (device), the x-seed filter and
+// merge (host, per pair, in set order), the reverse SemiGlobal of every
+// x-seed (device, one batch), the StartAlignment set (host), the forward
+// SemiGlobal with traceback of every new start (device, one batch), and the
+// fragment replace/append pass (host, in x-seed order).  The host passes keep
+// the reference's sequential semantics; only independent DPs are batched.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <set>
+#include <utility>
+
+#include "kernels.h"
+#include "pipeline.h"
+
+namespace dcns {
+namespace {
+
+double ms_now() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr int32_t kNeg = -536870912;
+
+void validate_two_piece(const dcns_params &p) {
+    if (p.open2 > 0 || p.open2 < -1000000 || p.ext2 > 0 || p.ext2 < -10000)
+        throw Error(DCNS_ERR_ARG, "second gap piece: open2 in [-1e6, 0], ext2 in [-1e4, 0]");
+    if (p.zdrop < 0) throw Error(DCNS_ERR_ARG, "zdrop must be >= 0");
+    if (p.bandwidth < 2) throw Error(DCNS_ERR_ARG, "bandwidth must be >= 2");
+}
+
+}  // namespace
+
+void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &tasks, bool trace,
+                      std::vector<SgResult> &out, int64_t &cells) {
+    const int32_t n = (int32_t)tasks.size();
+    out.assign((size_t)n, SgResult{0, -2, -2, {}});
+    cells = 0;
+    std::vector<int32_t> idx;
+    for (int32_t i = 0; i < n; ++i)
+        if (tasks[(size_t)i].R > 0 && tasks[(size_t)i].C > 0) idx.push_back(i);
+    if (idx.empty()) return;
+    // largest first: one wave per task, so the long tasks start early
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+        return (int64_t)tasks[(size_t)a].R * tasks[(size_t)a].C > (int64_t)tasks[(size_t)b].R * tasks[(size_t)b].C;
+    });
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const int64_t budget = std::max<int64_t>(std::min<int64_t>((int64_t)(free_b / 2), (int64_t)96 << 30),
+                                             (int64_t)1 << 30);
+    size_t pos = 0;
+    while (pos < idx.size()) {
+        // one chunk: tasks whose traceback stores and scratch fit the budget
+        std::vector<SgTask> ct;
+        std::vector<int32_t> which;
+        std::vector<SgOut> so;
+        int64_t tb_total = 0, scr_total = 0, cig_total = 0;
+        while (pos < idx.size()) {
+            SgTask t = tasks[(size_t)idx[pos]];
+            const int64_t tbb = trace ? (sg_trace_bytes(t.R, t.C, P.bandwidth) + 255) / 256 * 256 : 0;
+            const int64_t scr = sg_scratch_ints(t.R);
+            if (!ct.empty() && tb_total + tbb + 4 * (scr_total + scr) > budget) break;
+            if (ct.empty() && tbb + 4 * scr > budget)
+                throw Error(DCNS_ERR_CAPACITY, "a SemiGlobal task's traceback store exceeds device memory");
+            t.tb = trace ? tb_total : -1;
+            t.scr = scr_total;
+            tb_total += tbb;
+            scr_total += scr;
+            SgOut o{};
+            o.cig_off = cig_total;
+            o.stale[0] = o.stale[1] = kNeg;
+            if (trace) cig_total += (int64_t)t.R + t.C + 2;
+            ct.push_back(t);
+            so.push_back(o);
+            which.push_back(idx[pos]);
+            ++pos;
+        }
+        const int32_t m = (int32_t)ct.size();
+        c->sg_tasks.ensure((size_t)m * sizeof(SgTask));
+        c->sg_out.ensure((size_t)m * sizeof(SgOut));
+        c->sg_scr.ensure((size_t)std::max<int64_t>(scr_total, 16) * 4);
+        if (trace) {
+            c->sg_tb.ensure((size_t)std::max<int64_t>(tb_total, 256));
+            c->sg_cig.ensure((size_t)std::max<int64_t>(cig_total, 16) * 4);
+        }
+        hip_check(hipMemcpyAsync(c->sg_tasks.p, ct.data(), (size_t)m * sizeof(SgTask), hipMemcpyHostToDevice,
+                                 c->stream), "sg tasks");
+        hip_check(hipMemcpyAsync(c->sg_out.p, so.data(), (size_t)m * sizeof(SgOut), hipMemcpyHostToDevice, c->stream),
+                  "sg out");
+        c->launch("semiglobal", [&] {
+            return launch_sg_dp(c->stream, trace, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
+                                m, P, c->sg_scr.as<int32_t>(), trace ? c->sg_tb.as<uint8_t>() : nullptr,
+                                c->sg_out.as<SgOut>());
+        });
+        c->launch("semiglobal_finish", [&] {
+            return launch_sg_finish(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
+                                    m, P, c->sg_scr.as<int32_t>(), trace ? c->sg_tb.as<uint8_t>() : nullptr,
+                                    c->sg_out.as<SgOut>(), trace ? c->sg_cig.as<uint32_t>() : nullptr);
+        });
+        hip_check(hipMemcpyAsync(so.data(), c->sg_out.p, (size_t)m * sizeof(SgOut), hipMemcpyDeviceToHost, c->stream),
+                  "sg out");
+        std::vector<uint32_t> cig;
+        if (trace && cig_total) {
+            cig.resize((size_t)cig_total);
+            hip_check(hipMemcpyAsync(cig.data(), c->sg_cig.p, (size_t)cig_total * 4, hipMemcpyDeviceToHost, c->stream),
+                      "sg cigar");
+        }
+        c->sync();
+        for (int32_t k = 0; k < m; ++k) {
+            const SgOut &o = so[(size_t)k];
+            SgResult &r = out[(size_t)which[(size_t)k]];
+            r.max = o.max;
+            r.end1 = o.e1 - 1;  // the reference's final decrement (:921-922)
+            r.end2 = o.e2 - 1;
+            cells += o.cells;
+            if (trace && o.ncig > 0) {
+                r.cigar.assign(cig.begin() + o.cig_off, cig.begin() + o.cig_off + o.ncig);
+                std::reverse(r.cigar.begin(), r.cigar.end());
+            }
+        }
+    }
+}
+
+void find_similar_2gaps(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out) {
+    validate_params(p);
+    validate_two_piece(p);
+    Staged S = stage_batch(c, b, to_params(p));
+    find_similar_2gaps_staged(c, S, p, out);
+}
+
+void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &out) {
+    validate_params(p);
+    validate_two_piece(p);
+    const Params P = to_params(p);
+    relayout(c, S, P);
+    const int32_t npairs = (int32_t)S.pd.size();
+    const double t0 = ms_now();
+    // getAllExtendSeed: the same window loop and seed set as overload 1 (the
+    // loop at :377-392 differs only by a window at sp > L2, whose negative
+    // width yields no seed), then link_seeds
+    SeedResult sr;
+    seed_sets(c, S, P, false, sr);
+    const double t1 = ms_now();
+    std::vector<XTask> xt;
+    std::vector<int64_t> first((size_t)npairs + 1, 0);
+    for (int32_t i = 0; i < npairs; ++i) {
+        first[(size_t)i] = (int64_t)xt.size();
+        for (const Seed &s : sr.linked[(size_t)i]) xt.push_back(XTask{i, s.getStart1() + 1, s.getStart2() + 1, 0});
+    }
+    first[(size_t)npairs] = (int64_t)xt.size();
+    std::vector<XResult> xr;
+    int64_t xcells = 0;
+    xdrop_extend(c, S, P, xt, xr, xcells, true);
+    // x-seed filter and merge (:400-427): no break, every matching x-seed is visited
+    std::vector<std::vector<Seed>> xs((size_t)npairs);
+    parallel_for(npairs, [&](int64_t i) {
+        const double L1d = (double)S.pd[(size_t)i].len1, L2d = (double)S.pd[(size_t)i].len2;
+        std::vector<Seed> &v = xs[(size_t)i];
+        for (int64_t k = first[(size_t)i]; k < first[(size_t)i + 1]; ++k) {
+            const XResult &r = xr[(size_t)k];
+            if (!r.ok) continue;  // p = 100
+            const double e = p.k * L1d * L2d * std::exp(-1.0 * p.lambda * (double)r.score);
+            const double pv = 1.0 - std::exp(-1.0 * e);
+            if (!(pv < p.pvalue)) continue;
+            bool never_used = true;
+            for (Seed &x : v) {
+                if (x.getStart1() == r.start1 && x.getStart2() == r.start2) {
+                    never_used = false;
+                    if (r.end1 > x.getEnd1() && r.end2 > x.getEnd2()) {
+                        x.setEnd1(r.end1);
+                        x.setEnd2(r.end2);
+                    }
+                } else if (x.getEnd1() == r.end1 && x.getEnd2() == r.end2) {
+                    if (r.start1 < x.getStart1() && r.start2 < x.getStart2()) {
+                        x.setStart1(r.start1);
+                        x.setStart2(r.start2);
+                    }
+                    never_used = false;
+                }
+            }
+            if (never_used) v.push_back(Seed(r.start1, r.end1, r.start2, r.end2));
+        }
+    });
+    const double t2 = ms_now();
+    // overload 2: reverse SemiGlobal (no traceback) from every x-seed's end
+    std::vector<SgTask> rt;
+    std::vector<int64_t> xfirst((size_t)npairs + 1, 0);
+    for (int32_t i = 0; i < npairs; ++i) {
+        xfirst[(size_t)i] = (int64_t)rt.size();
+        for (const Seed &x : xs[(size_t)i]) {
+            SgTask t{};
+            t.pair = i;
+            t.rev = 1;
+            t.a1 = x.getEnd1();
+            t.a2 = x.getEnd2();
+            t.R = x.getEnd1() + 1;
+            t.C = x.getEnd2() + 1;
+            rt.push_back(t);
+        }
+    }
+    xfirst[(size_t)npairs] = (int64_t)rt.size();
+    std::vector<SgResult> rr;
+    int64_t sgcells = 0, c1 = 0, c2 = 0;
+    semiglobal_batch(c, P, rt, false, rr, c1);
+    // StartAlignment set (:572-574), then the forward SemiGlobal with traceback
+    std::vector<SgTask> ft;
+    std::vector<int64_t> ffirst((size_t)npairs + 1, 0);
+    for (int32_t i = 0; i < npairs; ++i) {
+        ffirst[(size_t)i] = (int64_t)ft.size();
+        std::set<std::pair<int32_t, int32_t>> starts;
+        const int32_t L1 = S.pd[(size_t)i].len1, L2 = S.pd[(size_t)i].len2;
+        for (int64_t k = xfirst[(size_t)i]; k < xfirst[(size_t)i + 1]; ++k) {
+            const Seed &x = xs[(size_t)i][(size_t)(k - xfirst[(size_t)i])];
+            const int32_t start1 = x.getEnd1() - rr[(size_t)k].end1, start2 = x.getEnd2() - rr[(size_t)k].end2;
+            if (!starts.insert({start1, start2}).second) continue;
+            SgTask t{};
+            t.pair = i;
+            t.rev = 0;
+            t.a1 = start1;
+            t.a2 = start2;
+            t.R = L1 - start1;
+            t.C = L2 - start2;
+            ft.push_back(t);
+        }
+    }
+    ffirst[(size_t)npairs] = (int64_t)ft.size();
+    std::vector<SgResult> fr;
+    semiglobal_batch(c, P, ft, true, fr, c2);
+    sgcells = c1 + c2;
+    const double t3 = ms_now();
+    // fragment replace / append (:584-611), p-values without a filter
+    std::vector<std::vector<PairedSimilarFragment>> per((size_t)npairs);
+    parallel_for(npairs, [&](int64_t i) {
+        const double L1d = (double)S.pd[(size_t)i].len1, L2d = (double)S.pd[(size_t)i].len2;
+        std::vector<PairedSimilarFragment> &v = per[(size_t)i];
+        for (int64_t k = ffirst[(size_t)i]; k < ffirst[(size_t)i + 1]; ++k) {
+            const SgResult &r = fr[(size_t)k];
+            const int32_t start1 = ft[(size_t)k].a1, start2 = ft[(size_t)k].a2;
+            if (!(r.cigar.size() > 0 && r.end1 > 0 && r.end2 > 0)) continue;
+            const int32_t end1 = r.end1 + start1, end2 = r.end2 + start2;
+            int32_t to_remove = -1;
+            bool found = false;
+            for (size_t q = 0; q < v.size(); ++q) {
+                if (v[q].getEnd1() == (uint32_t)(end1 + 1) && v[q].getEnd2() == (uint32_t)(end2 + 1)) {
+                    found = true;
+                    if ((uint32_t)(start1 + 1) < v[q].getStart1() && (uint32_t)(start2 + 1) < v[q].getStart2())
+                        to_remove = (int32_t)q;
+                }
+            }
+            if (found && to_remove < 0) continue;
+            const double e = p.k * L1d * L2d * std::exp(-1.0 * p.lambda * (double)r.max);
+            const double pv = 1.0 - std::exp(-1.0 * e);
+            PairedSimilarFragment f((uint32_t)(start1 + 1), (uint32_t)(end1 + 1), (uint32_t)(start2 + 1),
+                                    (uint32_t)(end2 + 1), (uint32_t)r.max, r.cigar, pv, e);
+            if (to_remove >= 0)
+                v[(size_t)to_remove] = f;
+            else
+                v.push_back(f);
+        }
+    });
+    out.frags.clear();
+    out.first.assign((size_t)npairs, 0);
+    out.count.assign((size_t)npairs, 0);
+    out.seeds_size = sr.before_link;
+    for (int32_t i = 0; i < npairs; ++i) {
+        out.first[(size_t)i] = (int64_t)out.frags.size();
+        out.count[(size_t)i] = (int64_t)per[(size_t)i].size();
+        for (auto &f : per[(size_t)i]) out.frags.push_back(std::move(f));
+    }
+    const double t4 = ms_now();
+    dcns_stats &st = c->stats;
+    st = dcns_stats{};
+    st.scan_cells = sr.scan_cells;
+    st.gpu_scan_cells = sr.gpu_cells;
+    st.xdrop_cells = xcells;
+    st.semiglobal_cells = sgcells;
+    for (int64_t v : sr.before_link) st.seeds += v;
+    for (auto &v : sr.linked) st.linked_seeds += (int64_t)v.size();
+    st.fragments = (int64_t)out.frags.size();
+    st.levels = sr.levels;
+    st.ms_seed = t1 - t0;
+    st.ms_xdrop = t2 - t1;
+    st.ms_semiglobal = t3 - t2;
+    st.ms_replay = t4 - t3;
+    st.ms_total = t4 - t0;
+}
+
+}  // namespace dcns

@@ -17,7 +17,8 @@
 //           band cell (rows x band, default 255 like Matrix::reset), then the
 //           traceback state machine writes the cigar.
 // A task whose band outgrows the kernel's register band is handed to the
-// next wider instantiation by the host.
+// next wider instantiation by the host, and past 161 cells to a kernel that
+// keeps the band in global scratch.
 #include <hip/hip_runtime.h>
 
 #include "dp_common.h"
@@ -133,6 +134,66 @@ __device__ PassOut xpass(const Dir &D, int32_t w, const Params &P, uint8_t *__re
     return r;
 }
 
+// The same pass for any band width, with the band-relative rows in
+// per-task global scratch (Mp, Fp of 2w+1 entries each) instead of VGPRs.
+// Used only for the tasks whose band outgrows the widest register kernel.
+template <bool TB>
+__device__ PassOut xpass_mem(const Dir &D, int32_t w, const Params &P, uint8_t *__restrict__ tb, int32_t BP,
+                             int32_t *__restrict__ Mp, int32_t *__restrict__ Fp) {
+    const int32_t B = 2 * w + 1;
+    for (int32_t k = 0; k < B; ++k) {
+        Mp[k] = kOutOfBand;
+        Fp[k] = kOutOfBand;
+    }
+    PassOut r{0, 0, 0, 0, 0, false};
+    const int32_t O = P.open, X = P.ext;
+    for (int32_t i = 1; i <= D.L1; ++i) {
+        const int32_t st = max(i - w, 1);
+        const int32_t en = min(i + w, D.L2);
+        const int32_t lo = i - w;
+        const int c = D.c1(i - 1);
+        int32_t e = kOutOfBand;
+        int32_t rowMax = 0, rowJ = -1;
+        uint8_t *row = TB ? tb + (int64_t)i * BP : nullptr;
+        if (st <= en) r.cells += en - st + 1;
+        for (int32_t k = 0; k < B; ++k) {
+            const int32_t j = lo + k;
+            const bool valid = (j >= st) && (j <= en);
+            const int qk = (j >= 1 && j <= D.L2) ? D.c2(j - 1) : 2;
+            const int32_t s = score(c, qk, P.match, P.mismatch);
+            const int32_t diag = (j == 1) ? (i == 1 ? 0 : kOutOfBand) : Mp[k];
+            const int32_t fv = (k + 1 < B && j != en) ? Fp[k + 1] : kOutOfBand;
+            int32_t m = s + diag;
+            uint32_t d = 0;
+            if (!(m > fv)) { m = fv; d = 1; }
+            if (!(m > e)) { m = e; d = 2; }
+            const int32_t h = m + O;
+            int32_t f = fv + X;
+            if (f >= h) d |= 8; else f = h;
+            int32_t en2 = e + X;
+            if (en2 >= h) d |= 16; else en2 = h;
+            if (valid) {
+                if (m > rowMax) { rowMax = m; rowJ = j; }
+                e = en2;
+                Mp[k] = m;
+                Fp[k] = f;
+            }
+            if (TB) row[k] = valid ? (uint8_t)d : (uint8_t)255;
+        }
+        if (TB)
+            for (int32_t k = B; k < BP; ++k) row[k] = 255;
+        r.rows = i;
+        if (rowJ != D.L2 && rowJ != 1 && (rowJ == st || rowJ == en)) {
+            r.restart = true;
+            return r;
+        }
+        if (rowMax > r.max) { r.max = rowMax; r.e1 = i; r.e2 = rowJ; }
+        if (r.max - rowMax > P.xdrop) break;
+        if (rowJ <= 0) break;
+    }
+    return r;
+}
+
 template <int BMAX, bool TB>
 __device__ PassOut xrun(const Dir &D, int32_t &w, const Params &P, uint8_t *tb, int32_t BP, int64_t &cells,
                         bool &too_wide) {
@@ -226,9 +287,92 @@ __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ s
     out[gid] = o;
 }
 
+// Pass 1 for the tasks the register kernels handed back (status 1: forward
+// pass pending, 2: reverse pass pending).  scr_off[t] is the int32 offset of
+// the task's 2 x (2 wmax + 1) scratch; wmax[t] is the first doubling of w
+// that reaches the pair's longer length, where a pass can no longer restart
+// (the band then spans whole rows).
+__global__ void k_xdrop_wide_pass1(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                   const XTask *__restrict__ tasks, int32_t n, Params P, XOut *__restrict__ out,
+                                   int32_t *__restrict__ scr, const int64_t *__restrict__ scr_off,
+                                   const int32_t *__restrict__ wmax) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    if (scr_off[gid] < 0) return;
+    const XTask t = tasks[gid];
+    XOut o = out[gid];
+    const PairDesc pd = pairs[t.pair];
+    int32_t *Mp = scr + scr_off[gid];
+    int32_t *Fp = Mp + 2 * wmax[gid] + 1;
+    if (o.status == 1) {
+        int32_t w = o.fw;
+        const Dir F = forward_dir(seq, pd, t);
+        PassOut f;
+        while (true) {
+            f = xpass_mem<false>(F, w, P, nullptr, 0, Mp, Fp);
+            o.cells += f.cells;
+            if (!f.restart || 2 * w > wmax[gid]) break;
+            w *= 2;
+        }
+        o.fmax = f.max;
+        o.fe1 = f.e1 - 1;
+        o.fe2 = f.e2 - 1;
+        o.fw = w;
+        o.rw = P.w;
+        if (o.fe1 < 0 || o.fe2 < 0) {
+            o.status = 0;
+            o.rmax = 0;
+            o.re1 = o.re2 = -1;
+            o.rrows = 0;
+            out[gid] = o;
+            return;
+        }
+    }
+    int32_t w = o.rw;
+    const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
+    PassOut r;
+    while (true) {
+        r = xpass_mem<false>(R, w, P, nullptr, 0, Mp, Fp);
+        o.cells += r.cells;
+        if (!r.restart || 2 * w > wmax[gid]) break;
+        w *= 2;
+    }
+    o.status = 0;
+    o.rmax = r.max;
+    o.re1 = r.e1 - 1;
+    o.re2 = r.e2 - 1;
+    o.rw = w;
+    o.rrows = r.rows;
+    out[gid] = o;
+}
+
 __device__ __forceinline__ void push_op(uint32_t *cig, int32_t &n, uint32_t op) {
     if (n == 0 || (cig[n - 1] & 0xfu) != op) cig[n++] = (1u << 4) | op;
     else cig[n - 1] += 1u << 4;
+}
+
+// Traceback state machine (:1416-1438) over the band-compressed store of the
+// final reverse pass; cells outside the stored band read 255.
+__device__ void trace_cigar(XOut &o, int32_t w, const uint8_t *__restrict__ T, int32_t BP, uint32_t *__restrict__ cigar) {
+    const int32_t B = 2 * w + 1;
+    int32_t ii = o.re1 + 1, jj = o.re2 + 1;
+    uint32_t *cig = cigar + o.cig_off;
+    int32_t nc = 0;
+    int state = 0;
+    while (ii > 0 && jj > 0) {
+        const int32_t k = jj - (ii - w);
+        int tv = 255;
+        if (k >= 0 && k < B && ii <= o.rrows) tv = T[(int64_t)ii * BP + k];
+        if (state == 0) state = tv & 7;
+        else if (!((tv >> (state + 2)) & 1)) state = 0;
+        if (state == 0) state = tv & 7;
+        if (state == 0) { push_op(cig, nc, 0); --ii; --jj; }
+        else if (state == 1 || state == 3) { push_op(cig, nc, 2); --ii; }
+        else { push_op(cig, nc, 1); --jj; }
+    }
+    o.iii = ii;
+    o.jjj = jj;
+    o.ncig = nc;
 }
 
 // Pass 2: reverse pass with traceback at the final band from pass 1.
@@ -252,25 +396,32 @@ __global__ __launch_bounds__(64) void k_xdrop_pass2(const int8_t *__restrict__ s
     uint8_t *T = tb + tb_off[gid];
     PassOut r = xpass<BMAX, true>(R, w, P, T, BP);
     (void)r;
-    // traceback state machine (:1416-1438); cells outside the stored band read 255
-    int32_t ii = o.re1 + 1, jj = o.re2 + 1;
-    uint32_t *cig = cigar + o.cig_off;
-    int32_t nc = 0;
-    int state = 0;
-    while (ii > 0 && jj > 0) {
-        const int32_t k = jj - (ii - w);
-        int tv = 255;
-        if (k >= 0 && k < B && ii <= o.rrows) tv = T[(int64_t)ii * BP + k];
-        if (state == 0) state = tv & 7;
-        else if (!((tv >> (state + 2)) & 1)) state = 0;
-        if (state == 0) state = tv & 7;
-        if (state == 0) { push_op(cig, nc, 0); --ii; --jj; }
-        else if (state == 1 || state == 3) { push_op(cig, nc, 2); --ii; }
-        else { push_op(cig, nc, 1); --jj; }
-    }
-    o.iii = ii;
-    o.jjj = jj;
-    o.ncig = nc;
+    trace_cigar(o, w, T, BP, cigar);
+    out[gid] = o;
+}
+
+// Pass 2 for the tasks whose final reverse band is wider than 161 cells.
+__global__ void k_xdrop_wide_pass2(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                   const XTask *__restrict__ tasks, int32_t n, Params P, XOut *__restrict__ out,
+                                   const int64_t *__restrict__ tb_off, uint8_t *__restrict__ tb,
+                                   uint32_t *__restrict__ cigar, int32_t *__restrict__ scr,
+                                   const int64_t *__restrict__ scr_off) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const XTask t = tasks[gid];
+    XOut o = out[gid];
+    if (o.fe1 < 0 || o.fe2 < 0) return;
+    const int32_t w = o.rw;
+    if (2 * w + 1 <= 161 || scr_off[gid] < 0) return;
+    const PairDesc pd = pairs[t.pair];
+    const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
+    const int32_t B = 2 * w + 1;
+    const int32_t BP = (B + 3) & ~3;
+    uint8_t *T = tb + tb_off[gid];
+    int32_t *Mp = scr + scr_off[gid];
+    PassOut r = xpass_mem<true>(R, w, P, T, BP, Mp, Mp + B);
+    (void)r;
+    trace_cigar(o, w, T, BP, cigar);
     out[gid] = o;
 }
 
@@ -287,6 +438,24 @@ hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const
         case 161: hipLaunchKernelGGL(k_xdrop_pass1<161>, g, b, 0, st, seq, pairs, tasks, n, P, out, pending); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_xdrop_wide_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                                   int32_t n, const Params &P, XOut *out, int32_t *scr, const int64_t *scr_off,
+                                   const int32_t *wmax) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xdrop_wide_pass1, dim3((n + 63) / 64), dim3(64), 0, st, seq, pairs, tasks, n, P, out, scr,
+                       scr_off, wmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_xdrop_wide_pass2(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                                   int32_t n, const Params &P, XOut *out, const int64_t *tb_off, uint8_t *tb,
+                                   uint32_t *cigar, int32_t *scr, const int64_t *scr_off) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xdrop_wide_pass2, dim3((n + 63) / 64), dim3(64), 0, st, seq, pairs, tasks, n, P, out, tb_off,
+                       tb, cigar, scr, scr_off);
     return hipGetLastError();
 }
 

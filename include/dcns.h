@@ -80,6 +80,8 @@ typedef struct dcns_stats {
     int32_t levels;         /* recursion levels run on the device */
     int32_t pad;
     double ms_seed, ms_replay, ms_xdrop, ms_total; /* wall-clock phases */
+    int64_t semiglobal_cells; /* cells of every SemiGlobal call the reference makes (two-piece path) */
+    double ms_semiglobal;     /* wall-clock of the two-piece DP phase */
 } dcns_stats;
 
 const char *dcns_version(void);
@@ -125,6 +127,25 @@ int dcns_seed_sets(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, int
 
 /* findSimilarFragmentsForPairedSequence (overload 1) for every pair. */
 int dcns_find_similar_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, dcns_frag_set **out);
+/* findSimilarFragmentsForPairedSequence overload 4 -- the two-piece affine
+ * path of cut2Gaps / pairCns2Gaps
+ * (impl/findSimilarFragmentsForPairedSequence4.cpp:752-774: getAllExtendSeed
+ * :348-429, then overload 2 :532-616; callers controlLayer4.cpp:355, 1584).
+ * Uses open2/ext2 (second gap piece), zdrop and bandwidth of p. */
+int dcns_find_similar_2gaps_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, dcns_frag_set **out);
+
+/* SemiGlobal (two-piece affine, z-drop, banded; impl/sequenceAlignment3.cpp:932
+ * -> SemiGlobal_no_avx :681) for n tasks.  Task i aligns pair
+ * pair_of_task[i] from anchors[2i], anchors[2i+1] (0-based): forward on
+ * seq1[a1..), seq2[a2..) or, when reverse[i] != 0, on the reverse complements
+ * of seq1[..a1] and seq2[..a2] (the reverse alignment of overload 2, :564).
+ * out3 = max, end1, end2 (0-based; -2 when no cell is positive).  With
+ * with_cigar, cigars (len << 4 | op, op 0 M, 1 I, 2 D) go to cigar with
+ * cig_off[n+1] offsets. */
+int dcns_semiglobal_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, const int32_t *pair_of_task,
+                          const int32_t *anchors, const int32_t *reverse, int32_t n, int32_t with_cigar,
+                          int32_t *out3, uint32_t *cigar, int64_t *cig_off, int64_t cig_cap);
+
 /* Keeps a batch resident in device memory (replacing any earlier one; the
  * other entry points that take a batch drop it) and runs
  * dcns_find_similar_batch on it without re-uploading.  No reference
@@ -163,6 +184,15 @@ int dcns_fit_lambda_k(const int32_t *scores, int64_t n, const double *log_space,
  *                      controlLayer4.cpp:2088 (decl. controlLayer.h:113) */
 int dcns_cut1gap_main(int argc, char **argv);
 int dcns_ransco_main(int argc, char **argv);
+/*   dcns_cut2gaps_main      void/int cut2Gaps(...), controlLayer4.cpp:1534 / :1608
+ *                           (decl. controlLayer.h); all query regions in one batch
+ *   dcns_paircns2gaps_main  int pairCns2Gaps(int argc, char** argv), controlLayer4.cpp:378
+ *                           (the queries are the input's sequences; the reference
+ *                           clears them first, :346, and aborts)
+ *   dcns_paircnsxextend_main int pairCnsXExtend(int argc, char** argv), controlLayer4.cpp:238 */
+int dcns_cut2gaps_main(int argc, char **argv);
+int dcns_paircns2gaps_main(int argc, char **argv);
+int dcns_paircnsxextend_main(int argc, char **argv);
 
 /* lambda/K surrogate network.  No counterpart in the reference, which obtains
  * lambda and K only by permutation (ranSco, permutationLsqLambdaK.cpp:32-102,

@@ -48,9 +48,11 @@ class Checker:
                                           _P, _P, _I, _P, _P, _L, _P]
             f("sw_max_batch").argtypes = [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _I]
             f("sw_uses_simd_variant").argtypes = [_I, _I, _I, _I, _I]
+            f("find_similar_2gaps").argtypes = [_P, _I, _P, _I] + [_I] * 13 + [_D, _D, _D, _P, _P, _I, _P, _P, _L, _P]
         else:
             f("find_similar").argtypes = [_P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _D, _D, _D,
                                           _P, _P, _I, _P, _P, _L]
+            f("find_similar_2gaps").argtypes = [_P, _I, _P, _I] + [_I] * 13 + [_D, _D, _D, _P, _P, _I, _P, _P, _L]
 
     def _f(self, name):
         return getattr(self.lib, f"{self.prefix}_{name}")
@@ -126,6 +128,33 @@ class Checker:
             res.append((*a, float(dbls[2 * i]), float(dbls[2 * i + 1]),
                         [int(c) for c in cig[off[i]:off[i + 1]]]))
         return res
+
+
+def _find_similar_2gaps(self, s1, s2, sc=(2, -3, -4, -2, -45, 0), window=38, step=8, min_score=40, w=10, xdrop=20,
+                        zdrop=50, bandwidth=20000, pvalues=0.1, lam=0.382291, k=0.006662, stats=None):
+    """findSimilarFragmentsForPairedSequence overload 4: a list of
+    (start1, end1, start2, end2, score, pvalue, evalue, cigar)."""
+    s1, s2 = self._codes(s1), self._codes(s2)
+    cap, cig_cap = 4096, 1 << 22
+    ints = np.zeros(5 * cap, np.int32)
+    dbls = np.zeros(2 * cap, np.float64)
+    cig = np.zeros(cig_cap, np.uint32)
+    off = np.zeros(cap + 1, np.int64)
+    args = [_ptr(s1), len(s1), _ptr(s2), len(s2), *sc, window, step, min_score, w, xdrop, zdrop, bandwidth, pvalues,
+            lam, k, ints.ctypes.data, dbls.ctypes.data, cap, cig.ctypes.data, off.ctypes.data, cig_cap]
+    if self.prefix == "orc":
+        st = np.zeros(5, np.int64)
+        args.append(st.ctypes.data)
+    n = self._f("find_similar_2gaps")(*args)
+    assert n >= 0, "capacity"
+    if stats is not None and self.prefix == "orc":
+        stats.update(scan_cells=int(st[0]), xdrop_cells=int(st[1]), seeds=int(st[2]), linked=int(st[3]),
+                     semiglobal_cells=int(st[4]))
+    return [(*[int(v) for v in ints[5 * i:5 * i + 5]], float(dbls[2 * i]), float(dbls[2 * i + 1]),
+             [int(c) for c in cig[off[i]:off[i + 1]]]) for i in range(n)]
+
+
+Checker.find_similar_2gaps = _find_similar_2gaps
 
 
 def oracle():

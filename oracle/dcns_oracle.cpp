@@ -217,7 +217,11 @@ struct SgResult {
 };
 
 SgResult semiglobal(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, Scoring sc, int32_t o1, int32_t x1,
-                    int32_t o2, int32_t x2, int32_t zdrop, int32_t w, BandTrace *tb, std::vector<uint32_t> *cig) {
+                    int32_t o2, int32_t x2, int32_t zdrop, int32_t w, BandTrace *tb, std::vector<uint32_t> *cig,
+                    int64_t *cells = nullptr) {
+    // an empty query reads seq2[-1] in the reference (undefined); both empty
+    // cases are taken as "no alignment"
+    if (L1 <= 0 || L2 <= 0) return SgResult{0, -2, -2};
     std::vector<int32_t> M1(L2 + 1, kOutOfBand), M2(L2 + 1, kOutOfBand), F1(L2 + 1, kOutOfBand),
         F2(L2 + 1, kOutOfBand);
     M1[0] = 0; M2[0] = 0;
@@ -231,6 +235,7 @@ SgResult semiglobal(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, 
         F1[end] = kOutOfBand;
         F2[end] = kOutOfBand;
         if (tb) tb->open_row(i, start);
+        if (cells) *cells += (int64_t)std::max(0, end - start) + 1;
         for (int32_t j = start; j < end; ++j) {
             int32_t m = sc(s1[i - 1], s2[j - 1]) + M1[j - 1];
             uint8_t d = 0;
@@ -570,6 +575,118 @@ int orc_find_similar(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t
         if (fresh) frags.push_back(nf);
     }
     if (stats) { stats[0] = c.cells; stats[1] = xcells; stats[2] = nseeds; stats[3] = (int64_t)seeds.size(); }
+    int32_t n = (int32_t)frags.size();
+    int64_t total = 0;
+    for (auto &g : frags) total += (int64_t)g.cigar.size();
+    if (n > cap || total > cig_cap) return -n - 1;
+    int64_t o = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        ints[5 * i] = (int32_t)frags[i].start1; ints[5 * i + 1] = (int32_t)frags[i].end1;
+        ints[5 * i + 2] = (int32_t)frags[i].start2; ints[5 * i + 3] = (int32_t)frags[i].end2;
+        ints[5 * i + 4] = (int32_t)frags[i].score;
+        dbls[2 * i] = frags[i].p; dbls[2 * i + 1] = frags[i].e;
+        cig_off[i] = o;
+        for (uint32_t x : frags[i].cigar) cig[o++] = x;
+    }
+    cig_off[n] = o;
+    return n;
+}
+
+int orc_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
+                           int32_t mismatch, int32_t open, int32_t ext, int32_t open2, int32_t ext2,
+                           int32_t window, int32_t step, int32_t min_score, int32_t w, int32_t xdrop, int32_t zdrop,
+                           int32_t bandwidth, double pvalues, double lambda, double kvalue, int32_t *ints,
+                           double *dbls, int32_t cap, uint32_t *cig, int64_t *cig_off, int64_t cig_cap,
+                           int64_t *stats) {
+    SeedCtx c;
+    c.seq1 = seq1; c.seq2 = seq2; c.L1 = L1; c.L2 = L2; c.W = window; c.minScore = min_score;
+    c.minRef = min_score / match; c.sc = Scoring{match, mismatch}; c.open = open; c.ext = ext;
+    c.rc1.resize((size_t)L1 + 1); c.rc2.resize((size_t)L2 + 1);
+    for (int32_t i = 0; i < L1; ++i) c.rc1[i] = (int8_t)(4 - seq1[L1 - 1 - i]);
+    for (int32_t i = 0; i < L2; ++i) c.rc2[i] = (int8_t)(4 - seq2[L2 - 1 - i]);
+    // getAllExtendSeed window loop (:377-392): no sp > L2 guard; a window of
+    // non-positive width scores 0 (the SIMD SmithWaterman with an empty
+    // profile) and adds no seed
+    std::set<Seed> seeds;
+    {
+        int32_t sp = 0;
+        while (true) {
+            if (std::min(c.W, c.L2 - sp) > 0) outer_seeds(c, sp, seeds);
+            sp += step;
+            if (sp + c.W > c.L2) {
+                if (c.L2 - sp > 0) outer_seeds(c, sp, seeds);
+                break;
+            }
+        }
+    }
+    int64_t nseeds = (int64_t)seeds.size();
+    link_seeds(seeds);
+    const double L1d = (double)L1, L2d = (double)L2;
+    int64_t xcells = 0, sgcells = 0;
+    // score-only x_extend_seed (:205-241), p filter and x-seed merge (:400-427)
+    std::vector<Seed> xs;
+    for (const Seed &sd : seeds) {
+        int32_t s1 = sd.start1 + 1, s2 = sd.start2 + 1;
+        XResult f = xextend(seq1 + s1, seq2 + s2, L1 - s1, L2 - s2, c.sc, open, ext, xdrop, w, nullptr);
+        xcells += f.cells;
+        if (f.end1 < 0 || f.end2 < 0) continue;  // p = 100
+        XResult r = xextend(c.rc1.data() + (L1 - 1 - (s1 + f.end1)), c.rc2.data() + (L2 - 1 - (s2 + f.end2)),
+                            s1 + 1 + f.end1, s2 + 1 + f.end2, c.sc, open, ext, xdrop, w, nullptr);
+        xcells += r.cells;
+        if (r.end1 < 0 || r.end2 < 0) continue;
+        const int32_t end1 = s1 + f.end1, end2 = s2 + f.end2;
+        const int32_t start1 = s1 + f.end1 - r.end1, start2 = s2 + f.end2 - r.end2;
+        const double e = kvalue * L1d * L2d * std::exp(-1.0 * lambda * (double)r.max);
+        const double p = 1.0 - std::exp(-1.0 * e);
+        if (!(p < pvalues)) continue;
+        bool never_used = true;
+        for (Seed &x : xs) {
+            if (x.start1 == start1 && x.start2 == start2) {
+                never_used = false;
+                if (end1 > x.end1 && end2 > x.end2) { x.end1 = end1; x.end2 = end2; }
+            } else if (x.end1 == end1 && x.end2 == end2) {
+                if (start1 < x.start1 && start2 < x.start2) { x.start1 = start1; x.start2 = start2; }
+                never_used = false;
+            }
+        }
+        if (never_used) xs.push_back(Seed{start1, end1, start2, end2});
+    }
+    // overload 2 (:532-616)
+    std::vector<Fragment> frags;
+    std::set<std::pair<int32_t, int32_t>> starts;
+    for (const Seed &x : xs) {
+        SgResult rv = semiglobal(c.rc1.data() + (L1 - 1 - x.end1), c.rc2.data() + (L2 - x.end2 - 1), 1 + x.end1,
+                                 x.end2 + 1, c.sc, open, ext, open2, ext2, zdrop, bandwidth, nullptr, nullptr,
+                                 &sgcells);
+        const int32_t start1 = x.end1 - rv.end1, start2 = x.end2 - rv.end2;
+        if (!starts.insert({start1, start2}).second) continue;
+        std::vector<uint32_t> cigar;
+        BandTrace T;
+        SgResult fw = semiglobal(seq1 + start1, seq2 + start2, L1 - start1, L2 - start2, c.sc, open, ext, open2, ext2,
+                                 zdrop, bandwidth, &T, &cigar, &sgcells);
+        if (!(cigar.size() > 0 && fw.end1 > 0 && fw.end2 > 0)) continue;
+        const int32_t end1 = fw.end1 + start1, end2 = fw.end2 + start2;
+        int32_t to_remove = -1;
+        bool found = false;
+        for (size_t i = 0; i < frags.size(); ++i) {
+            if (frags[i].end1 == (uint32_t)(end1 + 1) && frags[i].end2 == (uint32_t)(end2 + 1)) {
+                found = true;
+                if ((uint32_t)(start1 + 1) < frags[i].start1 && (uint32_t)(start2 + 1) < frags[i].start2)
+                    to_remove = (int32_t)i;
+            }
+        }
+        if (found && to_remove < 0) continue;
+        const double e = kvalue * L1d * L2d * std::exp(-1.0 * lambda * (double)fw.max);
+        const double p = 1.0 - std::exp(-1.0 * e);
+        Fragment nf{(uint32_t)(start1 + 1), (uint32_t)(end1 + 1), (uint32_t)(start2 + 1), (uint32_t)(end2 + 1),
+                    (uint32_t)fw.max, cigar, p, e};
+        if (to_remove >= 0) frags[(size_t)to_remove] = nf;
+        else frags.push_back(nf);
+    }
+    if (stats) {
+        stats[0] = c.cells; stats[1] = xcells; stats[2] = nseeds; stats[3] = (int64_t)seeds.size();
+        stats[4] = sgcells;
+    }
     int32_t n = (int32_t)frags.size();
     int64_t total = 0;
     for (auto &g : frags) total += (int64_t)g.cigar.size();

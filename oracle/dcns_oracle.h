@@ -68,6 +68,18 @@ int orc_find_similar(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t
                      int32_t w, int32_t xdrop, double pvalues, double lambda, double kvalue, int32_t *ints,
                      double *dbls, int32_t cap, uint32_t *cig, int64_t *cig_off, int64_t cig_cap, int64_t *stats);
 
+// findSimilarFragmentsForPairedSequence overload 4 (the two-piece path of
+// cut2Gaps / pairCns2Gaps), findSimilarFragmentsForPairedSequence4.cpp:752-774
+// = getAllExtendSeed :348-429 + overload 2 :532-616.  Output layout as
+// orc_find_similar; stats (may be null) receives {seed scan cells, x-drop
+// cells, seeds before link, seeds after link, SemiGlobal cells}.
+int orc_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
+                           int32_t mismatch, int32_t open, int32_t ext, int32_t open2, int32_t ext2,
+                           int32_t window, int32_t step, int32_t min_score, int32_t w, int32_t xdrop, int32_t zdrop,
+                           int32_t bandwidth, double pvalues, double lambda, double kvalue, int32_t *ints,
+                           double *dbls, int32_t cap, uint32_t *cig, int64_t *cig_off, int64_t cig_cap,
+                           int64_t *stats);
+
 // Max local score of many pairs (the ranSco inner call,
 // permutationLsqLambdaK.cpp:97).  seqs is one buffer; offsets/lengths per pair.
 int orc_sw_max_batch(const int8_t *seqs, const int64_t *ref_off, const int32_t *ref_len, const int64_t *qry_off,

@@ -13,6 +13,7 @@
 //   ref_semiglobal    -> SemiGlobal (two-piece affine)         impl/sequenceAlignment3.cpp:932
 //   ref_seed_set      -> getSeeds window loop + link_seeds     impl/findSimilarFragmentsForPairedSequence4.cpp:131,288,461-483
 //   ref_find_similar  -> findSimilarFragmentsForPairedSequence impl/findSimilarFragmentsForPairedSequence4.cpp:433
+//   ref_find_similar_2gaps -> findSimilarFragmentsForPairedSequence (overload 4) :752
 #include <cstdint>
 #include <cstring>
 #include <set>
@@ -160,6 +161,39 @@ int ref_find_similar(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t
     std::vector<PairedSimilarFragment> f = findSimilarFragmentsForPairedSequence(
         a.data(), ar.data(), b.data(), br.data(), l1, l2, win, min_score, mbd, open, ext, match, mismatch, m, step,
         s1str, s2str, pvalues, lambda, kvalue, w, xdrop);
+    int32_t n = (int32_t)f.size();
+    int64_t total = 0;
+    for (auto &x : f) total += (int64_t)x.getCigar().size();
+    if (n > cap || total > cig_cap) return -n - 1;
+    int64_t o = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        ints[5 * i + 0] = (int32_t)f[i].getStart1(); ints[5 * i + 1] = (int32_t)f[i].getEnd1();
+        ints[5 * i + 2] = (int32_t)f[i].getStart2(); ints[5 * i + 3] = (int32_t)f[i].getEnd2();
+        ints[5 * i + 4] = (int32_t)f[i].getScore();
+        dbls[2 * i + 0] = f[i].getPValue(); dbls[2 * i + 1] = f[i].getEValue();
+        cig_off[i] = o;
+        for (uint32_t c : f[i].getCigar()) cig[o++] = c;
+    }
+    cig_off[n] = o;
+    return n;
+}
+
+// findSimilarFragmentsForPairedSequence overload 4 (two-piece path,
+// impl/findSimilarFragmentsForPairedSequence4.cpp:752).  Layout as
+// ref_find_similar.
+int ref_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
+                           int32_t mismatch, int32_t open, int32_t ext, int32_t open2, int32_t ext2, int32_t window,
+                           int32_t step, int32_t min_score, int32_t w, int32_t xdrop, int32_t zdrop,
+                           int32_t bandwidth, double pvalues, double lambda, double kvalue, int32_t *ints,
+                           double *dbls, int32_t cap, uint32_t *cig, int64_t *cig_off, int64_t cig_cap) {
+    Scorei m((int8_t)match, (int8_t)mismatch);
+    std::vector<int8_t> a = copy_of(seq1, L1), b = copy_of(seq2, L2);
+    std::vector<int8_t> ar = rc_of(seq1, L1), br = rc_of(seq2, L2);
+    std::string s1str, s2str;
+    int32_t l1 = L1, l2 = L2, win = window, mbd = 0;
+    std::vector<PairedSimilarFragment> f = findSimilarFragmentsForPairedSequence(
+        a.data(), ar.data(), b.data(), br.data(), l1, l2, win, min_score, mbd, open, ext, open2, ext2, match,
+        mismatch, m, step, s1str, s2str, pvalues, lambda, kvalue, zdrop, bandwidth, w, xdrop);
     int32_t n = (int32_t)f.size();
     int64_t total = 0;
     for (auto &x : f) total += (int64_t)x.getCigar().size();

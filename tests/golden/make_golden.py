@@ -8,8 +8,9 @@ Every fixture holds inputs (sequences as A/C/N/G/T text, parameters) and the
 outputs of the reference build (oracle/_ref) on them: K1 SmithWaterman
 (striped and textbook paths), SemiGlobal_xextend with and without traceback,
 the two-piece SemiGlobal, the seed set before/after link_seeds, the fragments
-of findSimilarFragmentsForPairedSequence, and the text outputs of the
-cut1Gap and ranSco commands.  Only data is written.
+of findSimilarFragmentsForPairedSequence (overloads 1 and 4), and the text
+outputs of the cut1Gap, cut2Gaps, pairCns2Gaps and ranSco commands.  Only
+data is written.
 """
 import json
 import os
@@ -133,7 +134,49 @@ def main():
                            capture_output=True, text=True)
         tag = "_".join(a.strip("-") for a in args)
         open(os.path.join(d, f"expected_{tag}.txt"), "w").write(r.stdout)
+    write_two_piece(HERE, ref)
     print("golden fixtures written to", HERE)
+
+
+TWO_PIECE_SETTINGS = [
+    dict(sc=(2, -3, -4, -2, -45, 0), zdrop=50, bandwidth=20000),
+    dict(sc=(2, -3, -4, -2, -45, 0), zdrop=50, bandwidth=150),
+    dict(sc=(2, -3, -4, -2, -20, -1), zdrop=30, bandwidth=20000),
+    dict(sc=(1, -2, -3, -1, -15, -1), zdrop=20, bandwidth=64, min_score=20),
+    dict(sc=(2, -3, -4, -2, -45, 0), zdrop=10, bandwidth=3),
+]
+
+
+def write_two_piece(here, ref):
+    """Overload 4 (the two-piece path) on planted pairs under several
+    settings, and the cut2Gaps / pairCns2Gaps commands."""
+    rng = np.random.default_rng(20261204)
+    two = []
+    for t in range(10):
+        kw = TWO_PIECE_SETTINGS[t % len(TWO_PIECE_SETTINGS)]
+        L1, L2 = int(rng.integers(600, 3500)), int(rng.integers(300, 3000))
+        a, b = synth.one_pair(seed=9000 + t, L1=L1, L2=L2, nfrag=int(rng.integers(1, 10)))
+        two.append({"s1": txt(a), "s2": txt(b), "params": kw, "fragments": ref.find_similar_2gaps(a, b, **kw)})
+    json.dump(two, open(os.path.join(here, "two_piece_fragments.json"), "w"))
+    d = os.path.join(here, "cut2gaps")
+    shutil.rmtree(d, ignore_errors=True)
+    os.makedirs(d)
+    write_cut1gap_inputs(d, rng)
+    write_pair_inputs(d, rng)
+    # the reference's pairCns2Gaps aborts on every input with a query (its
+    # query sequences are cleared before use), so it has no fixture here
+    runs = [("cut2Gaps", ["-i", "in.fa", "-r", "ref", "-ra", "refgenome.fa", "-qa", "querygenome.fa", "-o", "c2_"]),
+            ("cut2Gaps", ["-i", "in_refminus.fa", "-r", "ref", "-ra", "refgenome.fa", "-qa", "querygenome.fa",
+                          "-o", "c2m_", "-O2", "-30", "-E2", "-1", "-u2", "500"]),
+            ("pairCnsXExtend", ["-i", "pair.fa", "-r", "ref", "-o", "px_"]),
+            ("pairCnsXExtend", ["-i", "pair_refminus.fa", "-r", "ref", "-o", "pxm_", "-O", "-3", "-E", "-1"])]
+    for cmd, args in runs:
+        r = subprocess.run([bind.REF_CLI, cmd] + args, cwd=d, capture_output=True, text=True)
+        tag = args[args.index("-o") + 1]
+        open(os.path.join(d, f"expected_stdout_{tag}.txt"), "w").write(r.stdout)
+    for f in sorted(os.listdir(d)):
+        if f.startswith(("c2_", "c2m_", "px_", "pxm_")):
+            os.replace(os.path.join(d, f), os.path.join(d, "expected_" + f))
 
 
 def fasta_with_fai(path, records, width=60):
@@ -149,6 +192,29 @@ def fasta_with_fai(path, records, width=60):
                 line = seq[i:i + width] + "\n"
                 fh.write(line)
                 off += len(line)
+
+
+def write_pair_inputs(d, rng):
+    """pairCns inputs: the reference region and three query regions written
+    inline (with region metadata), queries carrying mutated copies of
+    reference stretches; a second file with the reference on strand 0."""
+    a = synth.random_seq(rng, 2400)
+    qs = []
+    for k in range(3):
+        q = synth.random_seq(rng, int(rng.integers(1200, 2600)))
+        for _ in range(4):
+            ln = int(rng.integers(50, 200))
+            p1 = int(rng.integers(0, len(a) - ln))
+            p2 = int(rng.integers(0, len(q) - ln))
+            m = synth.mutate(rng, a[p1:p1 + ln], 0.1, 0.02)[:ln]
+            q[p2:p2 + len(m)] = m
+        qs.append(q)
+    for fname, rstrand in (("pair.fa", 1), ("pair_refminus.fa", 0)):
+        with open(os.path.join(d, fname), "w") as fh:
+            fh.write(f">ref\tspecies:spA\tchr:cr\tstrand:{rstrand}\tstart:101\tend:2500\n{synth.to_text(a)}\n")
+            for k, (q, st) in enumerate(zip(qs, (1, 0, 1))):
+                fh.write(f">q{k}\tspecies:sp{'BCD'[k]}\tchr:cq{k}\tstrand:{st}\tstart:{11 + k}\tend:{10 + k + len(q)}\n")
+                fh.write(synth.to_text(q) + "\n")
 
 
 def write_cut1gap_inputs(d, rng):

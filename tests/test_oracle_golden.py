@@ -48,3 +48,14 @@ def test_seed_and_fragment_fixtures(oracle_lib):
         assert len(got) == len(want)
         for g, w in zip(got, want):
             assert list(g[:5]) == w[:5] and g[5] == w[5] and g[6] == w[6] and g[7] == w[7]
+
+
+def test_two_piece_fixtures(oracle_lib):
+    """findSimilarFragmentsForPairedSequence overload 4 (cut2Gaps path)."""
+    for case in load("two_piece_fragments.json"):
+        kw = dict(case["params"])
+        kw["sc"] = tuple(kw["sc"])
+        got = oracle_lib.find_similar_2gaps(enc(case["s1"]), enc(case["s2"]), **kw)
+        assert len(got) == len(case["fragments"])
+        for g, w in zip(got, case["fragments"]):
+            assert list(g[:5]) == w[:5] and g[5] == w[5] and g[6] == w[6] and list(g[7]) == w[7]

@@ -42,3 +42,17 @@ def test_pipeline_live(oracle_lib, ref_lib):
         assert oracle_lib.seed_set(a, b) == ref_lib.seed_set(a, b)
         assert oracle_lib.seed_set(a, b, linked=True) == ref_lib.seed_set(a, b, linked=True)
         assert oracle_lib.find_similar(a, b) == ref_lib.find_similar(a, b)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(sc=(2, -3, -4, -2, -45, 0), zdrop=50, bandwidth=20000),
+    dict(sc=(2, -3, -4, -2, -45, 0), zdrop=50, bandwidth=120),
+    dict(sc=(2, -3, -4, -2, -20, -1), zdrop=30, bandwidth=20000),
+    dict(sc=(2, -3, -4, -2, -45, 0), zdrop=10, bandwidth=3),
+])
+def test_two_piece_live(oracle_lib, ref_lib, kw):
+    rng = np.random.default_rng(44 + kw["bandwidth"])
+    for t in range(3):
+        a, b = synth.one_pair(seed=950 + t, L1=int(rng.integers(300, 2500)), L2=int(rng.integers(200, 2000)),
+                              nfrag=int(rng.integers(1, 8)))
+        assert oracle_lib.find_similar_2gaps(a, b, **kw) == ref_lib.find_similar_2gaps(a, b, **kw)
