@@ -170,17 +170,17 @@ int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t misma
         int64_t total = 0;
         for (int32_t i = 0; i < n; ++i) {
             if (l1[(size_t)i] < 0 || l2[(size_t)i] < 0) throw Error(DCNS_ERR_ARG, "negative length");
-            total += l1[(size_t)i] + l2[(size_t)i] + 8;
+            total += (l1[(size_t)i] + 31) / 16 * 16 + (l2[(size_t)i] + 47) / 16 * 16;
         }
         std::vector<int8_t> host((size_t)total + 64, 2);
         int64_t o = 0;
         for (int32_t i = 0; i < n; ++i) {
             o1[(size_t)i] = o;
             std::memcpy(&host[(size_t)o], b->seqs + b->off1[i], (size_t)l1[(size_t)i]);
-            o += l1[(size_t)i] + 4;
+            o += (l1[(size_t)i] + 31) / 16 * 16;  // 16-byte aligned, >= 16 bytes of padding (the kernel reads 16 at a time)
             o2[(size_t)i] = o;
             std::memcpy(&host[(size_t)o], b->seqs + b->off2[i], (size_t)l2[(size_t)i]);
-            o += l2[(size_t)i] + 4;
+            o += (l2[(size_t)i] + 47) / 16 * 16;  // query stripes read 32 bytes from a multiple of 32
         }
         // cost order: pairs of similar stripe counts and lengths share waves
         std::vector<int32_t> order((size_t)n);
@@ -210,11 +210,11 @@ int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t misma
             while (e < n) {
                 const int32_t l = std::max(l1[(size_t)order[(size_t)e]], 1);
                 const int32_t m = std::max(maxl1, l);
-                if ((int64_t)(e - s + 1) * m * 8 > budget && e > s) break;
+                if ((int64_t)(e - s + 1) * (m + 16) * 8 > budget && e > s) break;
                 maxl1 = m;
                 ++e;
             }
-            const int64_t stride = (int64_t)maxl1 * 2;
+            const int64_t stride = (int64_t)(maxl1 + 15) / 16 * 16 * 2;  // int2 per row, rows rounded to 16
             c->sw_bound.ensure((size_t)(e - s) * (size_t)stride * 4 + 64);
             c->launch("sw_batch", [&] {
                 return launch_sw_batch(c->stream, c->seq.as<int8_t>(), c->sw_off1.as<int64_t>(), c->sw_len1.as<int32_t>(),
