@@ -117,6 +117,108 @@ def cpu_baseline(ctx, pairs, budget_s, cores):
                       f"{wall:.1f} s wall, one single-threaded pair per worker"}, identical
 
 
+def lambdak_phase(ctx, log):
+    """lambda/K: config 3 (the permutation method at scale) and the surrogate
+    against it (config 4), with config-2 fragments' significance calls."""
+    import dcns_amd
+    from dcns_amd import lambdak as lk, synth
+    out = {}
+    settings = synth.scoring_settings()
+    rng = np.random.default_rng(synth.SEED_BASE + 3)
+    # config 3 as stated: 1e6 pairs per setting, lengths log-uniform 38-2000 and
+    # GC U[0.30, 0.70] per pair, int32 max local scores on the device, then one
+    # FP64 fit per setting with the per-pair search space m*n (1024 log bins)
+    n = 1_000_000
+    l1, l2 = lk.log_uniform(rng, 38, 2000, n), lk.log_uniform(rng, 38, 2000, n)
+    gc = rng.uniform(0.30, 0.70, n)
+    order = np.argsort(-(l1 * l2), kind="stable")
+    l1, l2, gc = l1[order], l2[order], gc[order]
+    cells = float(np.sum(l1.astype(np.float64) * l2)) * len(settings)
+    fits, t_scores, t_fit = [], 0.0, 0.0
+    for k, sc in enumerate(settings):
+        t = time.perf_counter()
+        scores = lk.permutation_scores(ctx, sc, gc, l1, l2, 1, seed=300 + k).ravel()
+        t_scores += time.perf_counter() - t
+        t = time.perf_counter()
+        K, lam, rss, it = dcns_amd.fit_lambda_k(scores, log_space=np.log(l1.astype(np.float64) * l2))
+        t_fit += time.perf_counter() - t
+        fits.append({"setting": list(sc), "lambda": lam, "K": K, "rss": rss, "iterations": it,
+                     "mean_score": float(scores.mean())})
+    out["config3"] = {"pairs_per_setting": n, "settings": len(settings), "cells": cells,
+                      "scores_per_s": round(n * len(settings) / t_scores, 1),
+                      "gcups": round(cells / t_scores / 1e9, 2), "fit_seconds": round(t_fit, 3), "fits": fits}
+    log(f"config 3: {n * len(settings)} scores in {t_scores:.2f} s ({cells / t_scores / 1e9:.1f} GCUPS)")
+    # the surrogate against the judge (4e4-pair permutation fits) at 40 random
+    # points of each config-3 setting, and of 6 more settings from the hull
+    sur = lk.Surrogate.load(ctx)
+    hull = [tuple(x) for x in sur.settings]
+    extra = [hull[int(i)] for i in np.random.default_rng(17).choice(len(hull), 6, replace=False)]
+    ev = {}
+    for tag, sset in (("config3_settings", settings), ("hull_settings", extra)):
+        r = lk.evaluate(ctx, sur, sset, points_per_setting=40, judge_pairs=40000,
+                        cheap_pairs=int(sur.meta.get("pairs_per_point", 2048)), log=log)
+        r.pop("rows")
+        r.pop("inside")
+        r["settings"] = [list(x) for x in sset]
+        r["settings_in_hull"] = [list(x) for x in sset if tuple(x) in set(hull)]
+        ev[tag] = r
+    out["surrogate"] = ev
+    qps, dt = lk.inference_rate(sur)
+    perm_s = ev["hull_settings"]["judge"]["seconds_per_query"]
+    out["surrogate"]["inference"] = {"batch": 65536, "queries_per_s": round(qps, 1), "batch_ms": round(dt * 1e3, 3),
+                                     "flop_per_query": 398336,
+                                     "tflops": round(qps * 398336 / 1e12, 3),
+                                     "permutation_seconds_per_query": perm_s,
+                                     "speedup_vs_permutation": round(qps * perm_s, 1),
+                                     "training_cells": sur.meta.get("training_cells"),
+                                     "training_points": sur.meta.get("points")}
+    log(f"surrogate: {qps:.3e} queries/s at batch 65536; permutation fit {perm_s * 1e3:.2f} ms/query")
+    out["config2"] = config2_significance(ctx, sur, log)
+    return out
+
+
+def config2_significance(ctx, sur, log):
+    """Config-2 fragments (500 kb masked pair, default scoring) reported with
+    the lambda/K of the judge, of the surrogate and of the reference's
+    default constants: p-value differences and changed significance calls."""
+    import dcns_amd
+    from dcns_amd import lambdak as lk, synth
+    t = time.perf_counter()
+    t1, t2 = synth.long_masked()
+    a = synth.encode(t1.replace("n", ""))
+    b = synth.encode(t2.replace("n", ""))
+    s = (2, -3, -4, -2)
+    lj, kj, _ = lk.permutation_fit(ctx, s, [0.45], 1000, 1000, pairs=100000, seed=4242)
+    X = lk.features(s, -45, [0.45], 1000, 1000)
+    ls, ks, inside = sur.lambda_k(X)
+    res = {"lengths": [len(a), len(b)], "judge": {"lambda": float(lj[0]), "K": float(kj[0]), "pairs": 100000},
+           "surrogate": {"lambda": float(ls[0]), "K": float(ks[0]), "in_hull": bool(inside[0])},
+           "default_constants": {"lambda": 0.382291, "K": 0.006662}}
+    # the candidates: every extended seed with p < 1 (a threshold no finite
+    # E-value reaches), so the list does not depend on lambda / K; then the
+    # p-value of each candidate under each lambda / K and the calls at 0.1
+    p = dcns_amd.default_params(pvalue=1.0, lambda_=float(lj[0]), k=float(kj[0]))
+    cand = ctx.find_similar([(a, b)], params=p, with_cigar=False)[0][1]
+    st = ctx.stats()
+    res["pipeline"] = {"reference_cells": st["scan_cells"] + st["xdrop_cells"], "seeds": st["seeds"],
+                       "levels": st["levels"], "seconds": round(time.perf_counter() - t, 2)}
+    sj = np.array([f[4] for f in cand], np.float64)
+    pj, _ = lk.pvalue(lj[0], kj[0], len(a), len(b), sj)
+    ps, _ = lk.pvalue(ls[0], ks[0], len(a), len(b), sj)
+    pd, _ = lk.pvalue(0.382291, 0.006662, len(a), len(b), sj)
+    mid = (pj > 1e-12) & (pj < 0.5)
+    res.update({"candidates": int(len(sj)),
+                "significant_judge": int(np.sum(pj < 0.1)), "significant_surrogate": int(np.sum(ps < 0.1)),
+                "significant_default_constants": int(np.sum(pd < 0.1)),
+                "calls_changed_surrogate": int(np.sum((pj < 0.1) != (ps < 0.1))),
+                "calls_changed_default_constants": int(np.sum((pj < 0.1) != (pd < 0.1))),
+                "pvalue_rel_err_surrogate_median": float(np.median(np.abs(ps[mid] / pj[mid] - 1))) if mid.any() else None,
+                "seconds": round(time.perf_counter() - t, 2)})
+    log(f"config 2: {len(sj)} candidates, calls changed: surrogate {res['calls_changed_surrogate']}, "
+        f"default constants {res['calls_changed_default_constants']}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +230,7 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-lambdak", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs shard across GPUs as "
@@ -207,6 +310,11 @@ def main():
                              "sample": f"{len(idx)} random pairs of the batch, one dcns_find_similar_batch "
                                        "call each (staging included)"}
         line["batched_vs_plain"] = round(gcups / plain, 2)
+
+    if not args.no_lambdak:
+        def log(m):
+            print(m, file=sys.stderr, flush=True)
+        line["lambdak"] = lambdak_phase(ctx, log)
 
     if not args.no_cpu:
         cores = args.cpu_cores or min(len(os.sched_getaffinity(0)), 16)

@@ -79,6 +79,8 @@ def load_library():
     lib.dcns_find_similar_2gaps_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), ctypes.POINTER(_P)]
     lib.dcns_semiglobal_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _P, _I, _I, _P,
                                           _P, _P, _L]
+    lib.dcns_fit_lambda_k_batch.argtypes = [_P, _L, _I, _P, _D, _D, _I, _P]
+    lib.dcns_sw_random_max.argtypes = [_P, _I, _I, _I, _I, _P, _P, _P, _P, _I, _I, _P]
     lib.dcns_mlp_create.argtypes = [_P, ctypes.c_uint64, ctypes.POINTER(_P)]
     lib.dcns_mlp_destroy.argtypes = [_P]
     lib.dcns_mlp_num_params.restype = _L
@@ -102,7 +104,8 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_ransco_main", "dcns_fit_lambda_k", "dcns_mlp_create", "dcns_mlp_destroy", "dcns_mlp_num_params",
            "dcns_mlp_set_scaling", "dcns_mlp_get_params", "dcns_mlp_set_params", "dcns_mlp_predict",
            "dcns_mlp_loss_grad", "dcns_mlp_train", "dcns_find_similar_2gaps_batch", "dcns_semiglobal_batch",
-           "dcns_cut2gaps_main", "dcns_paircns2gaps_main", "dcns_paircnsxextend_main"]
+           "dcns_cut2gaps_main", "dcns_paircns2gaps_main", "dcns_paircnsxextend_main", "dcns_sw_random_max",
+           "dcns_fit_lambda_k_batch"]
 
 
 def default_params(**kw):
@@ -126,6 +129,22 @@ def fit_lambda_k(scores, log_space=None, fixed_space=1e6, k0=0.3, lambda0=0.2, m
     if rc != 0:
         raise DcnsError(f"dcns_fit_lambda_k failed ({rc})")
     return float(out[0]), float(out[1]), float(out[2]), int(out[3])
+
+
+def fit_lambda_k_batch(scores, space, k0=0.3, lambda0=0.2, maxiter=550):
+    """One FP64 fit per row of scores [groups x n] with search space space[g]
+    (dcns_fit_lambda_k_batch).  Returns [groups x 4] (K, lambda, rss, iterations)."""
+    lib = load_library()
+    sc = np.ascontiguousarray(scores, np.int32)
+    sp = np.ascontiguousarray(space, np.float64).ravel()
+    if sc.ndim != 2 or len(sp) != sc.shape[0]:
+        raise ValueError("scores must be [groups x n] with one space per group")
+    out = np.zeros((sc.shape[0], 4), np.float64)
+    rc = lib.dcns_fit_lambda_k_batch(sc.ctypes.data, sc.shape[0], sc.shape[1], sp.ctypes.data, float(k0),
+                                     float(lambda0), int(maxiter), out.ctypes.data)
+    if rc != 0:
+        raise DcnsError(f"dcns_fit_lambda_k_batch failed ({rc})")
+    return out
 
 
 class _BatchArrays:
@@ -225,6 +244,22 @@ class Context:
         out = np.zeros(3 * max(n, 1), np.int32)
         self._check(self.lib.dcns_sw_batch(self.h, ctypes.byref(ba.b), *sc, out.ctypes.data), "dcns_sw_batch")
         return out[:3 * n].reshape(n, 3)
+
+    def sw_random_max(self, sc, gc, len1, len2, seed, pairs_per_point):
+        """Max local scores of random pairs generated on the device
+        (dcns_sw_random_max): returns [npoints x pairs_per_point] int32."""
+        gc = np.ascontiguousarray(gc, np.float64).ravel()
+        l1 = np.ascontiguousarray(len1, np.int32).ravel()
+        l2 = np.ascontiguousarray(len2, np.int32).ravel()
+        sd = np.ascontiguousarray(seed, np.uint64).ravel()
+        n = len(gc)
+        if not (len(l1) == len(l2) == len(sd) == n):
+            raise ValueError("gc, len1, len2 and seed must have one entry per point")
+        out = np.zeros((n, pairs_per_point), np.int32)
+        self._check(self.lib.dcns_sw_random_max(self.h, *sc, gc.ctypes.data, l1.ctypes.data, l2.ctypes.data,
+                                                sd.ctypes.data, n, int(pairs_per_point), out.ctypes.data),
+                    "dcns_sw_random_max")
+        return out
 
     # -- x-drop -------------------------------------------------------------
     def xdrop_extend(self, pairs, pair_of_seed, starts, params=None):

@@ -229,6 +229,65 @@ int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t misma
     });
 }
 
+int dcns_sw_random_max(dcns_ctx *c, int32_t match, int32_t mismatch, int32_t open, int32_t ext, const double *gc,
+                       const int32_t *len1, const int32_t *len2, const uint64_t *seed, int32_t npoints,
+                       int32_t pairs_per_point, int32_t *out) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        if (npoints < 0 || pairs_per_point < 1 || (npoints > 0 && (!gc || !len1 || !len2 || !seed || !out)))
+            throw Error(DCNS_ERR_ARG, "invalid point arrays");
+        if (match < 1 || match > 127 || mismatch > 0 || mismatch < -128 || open > 0 || open < -128 || ext > 0 ||
+            ext < -128)
+            throw Error(DCNS_ERR_ARG, "scores must fit int8 with match > 0 and penalties <= 0");
+        if (npoints == 0) return;
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        std::vector<uint32_t> thr((size_t)npoints * 3);
+        int32_t maxl1 = 1;
+        for (int32_t k = 0; k < npoints; ++k) {
+            if (!(gc[k] >= 0.0 && gc[k] <= 1.0)) throw Error(DCNS_ERR_ARG, "gc must be in [0, 1]");
+            if (len1[k] < 1 || len2[k] < 1 || len1[k] > (1 << 24) || len2[k] > (1 << 24))
+                throw Error(DCNS_ERR_ARG, "lengths must be in [1, 2^24]");
+            const double g = gc[k], two32 = 4294967296.0;
+            auto t = [&](double f) { return (uint32_t)std::min(4294967295.0, std::floor(f * two32)); };
+            thr[3 * (size_t)k] = t(g / 2);
+            thr[3 * (size_t)k + 1] = t(g);
+            thr[3 * (size_t)k + 2] = t(g + (1 - g) / 2);
+            maxl1 = std::max(maxl1, len1[k]);
+        }
+        dcns_params dp;
+        dcns_default_params(&dp);
+        dp.match = match;
+        dp.mismatch = mismatch;
+        dp.open = open;
+        dp.ext = ext;
+        const Params P = to_params(dp);
+        const ScoreTab tab = make_score_tab(match, mismatch);
+        auto up = [&](DBuf &d, const void *src, size_t bytes) {
+            d.ensure(std::max<size_t>(bytes, 64));
+            hip_check(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+        };
+        up(c->rnd_thr, thr.data(), thr.size() * 4);
+        up(c->rnd_len1, len1, (size_t)npoints * 4);
+        up(c->rnd_len2, len2, (size_t)npoints * 4);
+        up(c->rnd_seed, seed, (size_t)npoints * 8);
+        const int64_t total = (int64_t)npoints * pairs_per_point;
+        c->rnd_out.ensure((size_t)total * 4 + 64);
+        const int64_t stride = (int64_t)(maxl1 + 15) / 16 * 16 * 2;
+        const int64_t chunk = std::max<int64_t>(64, (((int64_t)1 << 31) / (stride * 4)) / 64 * 64);
+        c->sw_bound.ensure((size_t)std::min(chunk, total) * (size_t)stride * 4 + 64);
+        for (int64_t first = 0; first < total; first += chunk) {
+            const int64_t n = std::min(chunk, total - first);
+            c->launch("sw_random", [&] {
+                return launch_sw_random(c->stream, c->rnd_thr.as<uint32_t>(), c->rnd_len1.as<int32_t>(),
+                                        c->rnd_len2.as<int32_t>(), c->rnd_seed.as<uint64_t>(), pairs_per_point, first,
+                                        n, P, tab, c->rnd_out.as<int32_t>(), c->sw_bound.as<int32_t>(), stride);
+            });
+        }
+        hip_check(hipMemcpyAsync(out, c->rnd_out.p, (size_t)total * 4, hipMemcpyDeviceToHost, c->stream), "download");
+        c->sync();
+    });
+}
+
 int dcns_xdrop_extend(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, const int32_t *pair_of_seed,
                       const int32_t *seeds, int32_t n, int32_t *out_ints, uint32_t *cigar, int64_t *cig_off,
                       int64_t cig_cap) {

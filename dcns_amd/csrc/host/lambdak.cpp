@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../../include/dcns.h"
+#include "../ctx.h"
 
 namespace {
 
@@ -181,5 +182,19 @@ extern "C" int dcns_fit_lambda_k(const int32_t *scores, int64_t n, const double 
     out4[1] = l;
     out4[2] = cur;
     out4[3] = (double)it;
+    return DCNS_OK;
+}
+
+// Many independent fits with a fixed search space each: group g holds
+// scores[g * group_size .. +group_size) and N = space[g]; out4[4g..4g+3] as
+// dcns_fit_lambda_k.  Groups run on the host's cores.
+extern "C" int dcns_fit_lambda_k_batch(const int32_t *scores, int64_t ngroups, int32_t group_size, const double *space,
+                                       double k0, double lambda0, int32_t maxiter, double *out4) {
+    if (!scores || ngroups < 0 || group_size < 1 || !space || !out4 || maxiter < 1) return DCNS_ERR_ARG;
+    for (int64_t g = 0; g < ngroups; ++g)
+        if (!(space[g] > 0)) return DCNS_ERR_ARG;
+    dcns::parallel_for(ngroups, [&](int64_t g) {
+        dcns_fit_lambda_k(scores + g * group_size, group_size, nullptr, space[g], k0, lambda0, maxiter, out4 + 4 * g);
+    });
     return DCNS_OK;
 }

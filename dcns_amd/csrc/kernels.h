@@ -46,6 +46,10 @@ hipError_t launch_xdrop_wide_pass2(hipStream_t st, const int8_t *seq, const Pair
 hipError_t launch_sw_batch(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
                            const int64_t *off2, const int32_t *len2, const int32_t *order, int32_t n,
                            const Params &P, const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride);
+// Max scores of random fragment pairs generated on the device (pairs [first, first + n)).
+hipError_t launch_sw_random(hipStream_t st, const uint32_t *thr, const int32_t *len1, const int32_t *len2,
+                            const uint64_t *seed, int32_t per_point, int64_t first, int64_t n, const Params &P,
+                            const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride);
 
 // semiglobal.hip: two-piece semi-global DP, one wave per task.
 int64_t sg_trace_bytes(int32_t R, int32_t C, int32_t w);  // traceback store of one task

@@ -107,6 +107,19 @@ int dcns_kernel_time_reset(dcns_ctx *ctx);
 int dcns_sw_batch(dcns_ctx *ctx, const dcns_batch *b, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
                   int32_t *out3);
 
+/* Max local scores of random fragment pairs generated on the device: the
+ * sampling loop of permutationLsqLambdaK (impl/permutationLsqLambdaK.cpp:32-102),
+ * which scores random fragment pairs with SmithWaterman, with i.i.d. fragments
+ * of a given composition in place of fragments cut from genome files.  Point k
+ * has pairs_per_point pairs of lengths len1[k] x len2[k]; bases are C or G
+ * with probability gc[k]/2 each and A or T with (1 - gc[k])/2 each, drawn from
+ * a counter-based hash of seed[k] (base b of pair r: bits of
+ * splitmix64(seed[k] + 2r (+1 for the query)) + b/2, low half for even b).
+ * out[k * pairs_per_point + r] = the pair's SmithWaterman maximum. */
+int dcns_sw_random_max(dcns_ctx *ctx, int32_t match, int32_t mismatch, int32_t open, int32_t ext, const double *gc,
+                       const int32_t *len1, const int32_t *len2, const uint64_t *seed, int32_t npoints,
+                       int32_t pairs_per_point, int32_t *out);
+
 /* x_extend_seed (with traceback) for n seeds.  seeds[2*i] = start1,
  * seeds[2*i+1] = start2 (the seed start + 1, as the reference passes them);
  * pair_of_seed[i] indexes b.  out_ints (6 per seed): maxScore, start1, end1,
@@ -174,6 +187,10 @@ void dcns_frag_set_free(dcns_frag_set *fs);
  * residual sum of squares, iterations. */
 int dcns_fit_lambda_k(const int32_t *scores, int64_t n, const double *log_space, double fixed_space, double k0,
                       double lambda0, int32_t maxiter, double *out4);
+/* ngroups independent fits, fixed search space each: group g = scores[g *
+ * group_size, (g + 1) * group_size), N = space[g]; out4 as above per group. */
+int dcns_fit_lambda_k_batch(const int32_t *scores, int64_t ngroups, int32_t group_size, const double *space,
+                            double k0, double lambda0, int32_t maxiter, double *out4);
 
 /* Commands with the reference's flags, files and output text (argv[0] is the
  * command name):

@@ -57,3 +57,54 @@ def test_identical_pair_closed_form(gpu):
     a = synth.random_seq(rng, 1500)
     got = gpu.sw_batch([(a, a.copy())])
     assert tuple(int(v) for v in got[0]) == (2 * 1500, 1499, 1499)
+
+
+def _mix64(z):
+    z = z + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _random_fragment(key, L, gc):
+    """The device generator restated: base b = thresholds of 32 bits of
+    mix64(key + b // 2) (low half for even b)."""
+    x = _mix64(np.uint64(key) + np.arange((L + 1) // 2, dtype=np.uint64))
+    u = np.empty(2 * len(x), np.uint64)
+    u[0::2] = x & np.uint64(0xFFFFFFFF)
+    u[1::2] = x >> np.uint64(32)
+    u = u[:L]
+    t = [min(4294967295, int(np.floor(f * 4294967296.0))) for f in (gc / 2, gc, gc + (1 - gc) / 2)]
+    return np.where(u < t[0], 1, np.where(u < t[1], 3, np.where(u < t[2], 0, 4))).astype(np.int8)
+
+
+def test_sw_random_pairs_match_oracle(gpu, oracle_lib):
+    """Random fragment pairs generated on the device: scores equal the
+    oracle's SmithWaterman on the same fragments regenerated on the host."""
+    with np.errstate(over="ignore"):
+        pts = [(0.3, 38, 2000), (0.5, 700, 41), (0.62, 333, 333), (0.45, 1999, 1500), (0.7, 64, 65)]
+        sc = (2, -3, -4, -2)
+        per = 64
+        seeds = np.array([12345 + 1000 * k for k in range(len(pts))], np.uint64)
+        print("sw random", pts, flush=True)
+        got = gpu.sw_random_max(sc, [p[0] for p in pts], [p[1] for p in pts], [p[2] for p in pts], seeds, per)
+        seqs, o1, l1, o2, l2 = [], [], [], [], []
+        off = 0
+        for k, (g, L1, L2) in enumerate(pts):
+            for r in range(per):
+                a = _random_fragment(_mix64(seeds[k] + np.uint64(2 * r)), L1, g)
+                b = _random_fragment(_mix64(seeds[k] + np.uint64(2 * r + 1)), L2, g)
+                seqs += [a, b]
+                o1.append(off); l1.append(L1); off += L1
+                o2.append(off); l2.append(L2); off += L2
+        want = bind_sw_max(np.concatenate(seqs), o1, l1, o2, l2, sc)
+        assert np.array_equal(got.ravel(), want)
+        # the generator's composition
+        a = np.concatenate([_random_fragment(_mix64(np.uint64(7 + r)), 5000, 0.62) for r in range(8)])
+        assert abs(np.isin(a, (1, 3)).mean() - 0.62) < 0.01
+
+
+def bind_sw_max(seqs, o1, l1, o2, l2, sc):
+    from oracle import bind
+    return bind.sw_max_batch(seqs, np.array(o1, np.int64), np.array(l1, np.int32), np.array(o2, np.int64),
+                             np.array(l2, np.int32), sc, nthreads=8)
