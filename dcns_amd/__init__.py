@@ -79,6 +79,8 @@ def load_library():
     lib.dcns_find_similar_2gaps_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), ctypes.POINTER(_P)]
     lib.dcns_semiglobal_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _P, _I, _I, _P,
                                           _P, _P, _L]
+    lib.dcns_calculate_lambda.restype = _D
+    lib.dcns_calculate_lambda.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, _I, _I, _D, _D]
     lib.dcns_fit_lambda_k_batch.argtypes = [_P, _L, _I, _P, _D, _D, _I, _P]
     lib.dcns_sw_random_max.argtypes = [_P, _I, _I, _I, _I, _P, _P, _P, _P, _I, _I, _P]
     lib.dcns_mlp_create.argtypes = [_P, ctypes.c_uint64, ctypes.POINTER(_P)]
@@ -105,7 +107,7 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_mlp_set_scaling", "dcns_mlp_get_params", "dcns_mlp_set_params", "dcns_mlp_predict",
            "dcns_mlp_loss_grad", "dcns_mlp_train", "dcns_find_similar_2gaps_batch", "dcns_semiglobal_batch",
            "dcns_cut2gaps_main", "dcns_paircns2gaps_main", "dcns_paircnsxextend_main", "dcns_sw_random_max",
-           "dcns_fit_lambda_k_batch"]
+           "dcns_fit_lambda_k_batch", "dcns_calculate_lambda"]
 
 
 def default_params(**kw):
@@ -129,6 +131,14 @@ def fit_lambda_k(scores, log_space=None, fixed_space=1e6, k0=0.3, lambda0=0.2, m
     if rc != 0:
         raise DcnsError(f"dcns_fit_lambda_k failed ({rc})")
     return float(out[0]), float(out[1]), float(out[2]), int(out[3])
+
+
+def calculate_lambda(seq1, match=2, mismatch=-3, epsilon=1e-6, precision=1e-12):
+    """Ungapped Karlin-Altschul lambda of seq1's composition
+    (dcns_calculate_lambda)."""
+    a = np.ascontiguousarray(seq1, np.int8)
+    return float(load_library().dcns_calculate_lambda(a.ctypes.data, None, len(a), 0, int(match), int(mismatch),
+                                                      float(epsilon), float(precision)))
 
 
 def fit_lambda_k_batch(scores, space, k0=0.3, lambda0=0.2, maxiter=550):

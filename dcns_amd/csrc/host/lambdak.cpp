@@ -198,3 +198,51 @@ extern "C" int dcns_fit_lambda_k_batch(const int32_t *scores, int64_t ngroups, i
     });
     return DCNS_OK;
 }
+
+// calculateLambda (impl/calculateLambda.cpp:64-82): the ungapped
+// Karlin-Altschul lambda, the positive root of
+//   sum_ij p_i p_j exp(lambda s_ij) = 1,  s_ij = match (i == j) or mismatch,
+// over the A/C/G/T frequencies of seq1 (the reference uses seq1's
+// frequencies for both factors), by Newton's method with the reference's
+// forward-difference derivative of step epsilon, starting from 1 and doubling
+// until the derivative is positive, stopping when a step is at most
+// precision.  Differences from the reference, whose behaviour is undefined:
+// the frequency arrays start at zero (uninitialised there, :66-69) and the
+// step size is compared as a double (the reference's unqualified abs truncates
+// it to an integer and stops after one step); at most 10,000 steps.
+namespace {
+double lambda_f(const double *f, int32_t match, int32_t mismatch, double lam) {
+    double v = 0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) v += f[i] * f[j] * std::exp((double)(i == j ? match : mismatch) * lam);
+    return v - 1.0;
+}
+}  // namespace
+
+extern "C" double dcns_calculate_lambda(const int8_t *seq1, const int8_t *seq2, uint32_t len1, uint32_t len2,
+                                        int32_t match, int32_t mismatch, double epsilon, double precision) {
+    (void)seq2;
+    (void)len2;
+    if (!seq1 || !(epsilon > 0) || !(precision > 0)) return NAN;
+    double f[4] = {0, 0, 0, 0}, n = 0;
+    for (uint32_t i = 0; i < len1; ++i) {
+        const int c = seq1[i];
+        const int k = c == 0 ? 0 : c == 1 ? 1 : c == 3 ? 2 : c == 4 ? 3 : -1;
+        if (k >= 0) {
+            f[k] += 1;
+            n += 1;
+        }
+    }
+    if (n == 0) return NAN;
+    for (double &v : f) v /= n;
+    auto fp = [&](double lam) { return (lambda_f(f, match, mismatch, lam + epsilon) - lambda_f(f, match, mismatch, lam)) / epsilon; };
+    double lam = 1.0;
+    for (int k = 0; k < 1100 && fp(lam) < 0; ++k) lam *= 2.0;
+    for (int it = 0; it < 10000; ++it) {
+        const double next = lam - lambda_f(f, match, mismatch, lam) / fp(lam);
+        const double d = std::fabs(next - lam);
+        lam = next;
+        if (!(d > precision)) break;
+    }
+    return lam;
+}

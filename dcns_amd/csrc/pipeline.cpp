@@ -5,6 +5,7 @@
 #include "pipeline.h"
 
 #include <algorithm>
+#include <unordered_map>
 #include <chrono>
 #include <cmath>
 #include <functional>
@@ -571,6 +572,30 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
 // ---------------------------------------------------------------------------
 // findSimilarFragmentsForPairedSequence, overload 1.
 // ---------------------------------------------------------------------------
+
+// Positions of the fragments of one pair by (start1, start2) and by (end1,
+// end2), so that the dedup scan's first fragment with an equal start or an
+// equal end is found without walking the list.
+struct FragIndex {
+    std::unordered_map<uint64_t, std::set<int32_t>> by_start, by_end;
+    static uint64_t key(uint32_t a, uint32_t b) { return ((uint64_t)a << 32) | b; }
+    void add(int32_t i, const PairedSimilarFragment &f) {
+        by_start[key(f.getStart1(), f.getStart2())].insert(i);
+        by_end[key(f.getEnd1(), f.getEnd2())].insert(i);
+    }
+    void remove(int32_t i, const PairedSimilarFragment &f) {
+        by_start[key(f.getStart1(), f.getStart2())].erase(i);
+        by_end[key(f.getEnd1(), f.getEnd2())].erase(i);
+    }
+    int32_t first_match(const PairedSimilarFragment &f) const {
+        int32_t best = -1;
+        auto a = by_start.find(key(f.getStart1(), f.getStart2()));
+        if (a != by_start.end() && !a->second.empty()) best = *a->second.begin();
+        auto b = by_end.find(key(f.getEnd1(), f.getEnd2()));
+        if (b != by_end.end() && !b->second.empty() && (best < 0 || *b->second.begin() < best)) best = *b->second.begin();
+        return best;
+    }
+};
 void find_similar(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out) {
     validate_params(p);
     Staged S = stage_batch(c, b, to_params(p));
@@ -613,6 +638,7 @@ void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &
     parallel_for(npairs, [&](int64_t i) {
         const double L1d = (double)S.pd[(size_t)i].len1, L2d = (double)S.pd[(size_t)i].len2;
         std::vector<PairedSimilarFragment> &fr = per[(size_t)i];
+        FragIndex idx;
         for (int64_t k = first_task[(size_t)i]; k < first_task[(size_t)i + 1]; ++k) {
             const XResult &r = xr[(size_t)k];
             if (!r.ok) continue;  // p = 100
@@ -622,19 +648,23 @@ void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &
             // dedup, :500-521: 1-based uint32 stored coordinates against 0-based int32 ones
             PairedSimilarFragment nf((uint32_t)(r.start1 + 1), (uint32_t)(r.end1 + 1), (uint32_t)(r.start2 + 1),
                                      (uint32_t)(r.end2 + 1), (uint32_t)r.score, r.cigar, pv, e);
-            bool fresh = true;
-            for (PairedSimilarFragment &g : fr) {
-                if (g.getStart1() == (uint32_t)(r.start1 + 1) && g.getStart2() == (uint32_t)(r.start2 + 1)) {
-                    fresh = false;
-                    if ((uint32_t)r.end1 >= g.getEnd1() && (uint32_t)r.end2 >= g.getEnd2()) g = nf;
-                    break;
-                } else if (g.getEnd1() == (uint32_t)(r.end1 + 1) && g.getEnd2() == (uint32_t)(r.end2 + 1)) {
-                    fresh = false;
-                    if ((uint32_t)r.start1 <= g.getStart1() && (uint32_t)r.start2 <= g.getStart2()) g = nf;
-                    break;
-                }
+            // the reference scans the fragments in order and stops at the first
+            // one with the same start or the same end; the index finds that one
+            const int32_t g_at = idx.first_match(nf);
+            if (g_at < 0) {
+                idx.add((int32_t)fr.size(), nf);
+                fr.push_back(nf);
+                continue;
             }
-            if (fresh) fr.push_back(nf);
+            PairedSimilarFragment &g = fr[(size_t)g_at];
+            const bool by_start = g.getStart1() == nf.getStart1() && g.getStart2() == nf.getStart2();
+            const bool take = by_start ? ((uint32_t)r.end1 >= g.getEnd1() && (uint32_t)r.end2 >= g.getEnd2())
+                                       : ((uint32_t)r.start1 <= g.getStart1() && (uint32_t)r.start2 <= g.getStart2());
+            if (take) {
+                idx.remove(g_at, g);
+                g = nf;
+                idx.add(g_at, g);
+            }
         }
     });
     for (int32_t i = 0; i < npairs; ++i) {

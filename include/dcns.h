@@ -187,6 +187,15 @@ void dcns_frag_set_free(dcns_frag_set *fs);
  * residual sum of squares, iterations. */
 int dcns_fit_lambda_k(const int32_t *scores, int64_t n, const double *log_space, double fixed_space, double k0,
                       double lambda0, int32_t maxiter, double *out4);
+/* calculateLambda (impl/calculateLambda.h:11, calculateLambda.cpp:64): the
+ * ungapped Karlin-Altschul lambda for the A/C/G/T composition of seq1 (seq2
+ * is ignored, as in the reference), Newton's method with a forward-difference
+ * derivative of step epsilon until a step is <= precision.  The reference
+ * reads uninitialised frequencies and truncates the step size to an integer;
+ * this is the well-defined computation it describes.  NaN on bad input. */
+double dcns_calculate_lambda(const int8_t *seq1, const int8_t *seq2, uint32_t len1, uint32_t len2, int32_t match,
+                             int32_t mismatch, double epsilon, double precision);
+
 /* ngroups independent fits, fixed search space each: group g = scores[g *
  * group_size, (g + 1) * group_size), N = space[g]; out4 as above per group. */
 int dcns_fit_lambda_k_batch(const int32_t *scores, int64_t ngroups, int32_t group_size, const double *space,

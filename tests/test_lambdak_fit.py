@@ -67,3 +67,21 @@ def test_variable_space_fit_matches_exact_mixture():
 def test_fit_rejects_bad_input():
     with pytest.raises(dcns_amd.DcnsError):
         dcns_amd.fit_lambda_k(np.zeros(0, np.int32))
+
+
+def test_calculate_lambda_closed_forms():
+    """Ungapped lambda: +1/-1 on uniform composition solves 0.25 e^l + 0.75 e^-l = 1,
+    i.e. l = ln 3; other scorings and compositions against a bracketing root
+    finder on the same equation."""
+    import dcns_amd
+    from scipy.optimize import brentq
+    uni = np.array([0, 1, 3, 4] * 250, np.int8)
+    assert abs(dcns_amd.calculate_lambda(uni, 1, -1) - np.log(3)) < 1e-9
+    rng = np.random.default_rng(3)
+    for match, mism, gc in ((2, -3, 0.5), (1, -2, 0.35), (3, -4, 0.62), (2, -1, 0.45)):
+        s = rng.choice([1, 3, 0, 4], 20000, p=[gc / 2, gc / 2, (1 - gc) / 2, (1 - gc) / 2]).astype(np.int8)
+        s = np.concatenate([s, [2, 2, 2]]).astype(np.int8)  # N is not counted
+        f = np.array([np.mean(s[s != 2] == c) for c in (0, 1, 3, 4)])
+        F = lambda x: sum(f[i] * f[j] * np.exp((match if i == j else mism) * x) for i in range(4) for j in range(4)) - 1
+        want = brentq(F, 1e-6, 20)
+        assert abs(dcns_amd.calculate_lambda(s, match, mism) - want) < 1e-7
