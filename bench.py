@@ -61,6 +61,19 @@ def measured_traffic(path=os.path.join(ROOT, "profiles", "pmc_traffic_valu_batch
     return None
 
 
+def measured_valu_instructions(path=os.path.join(ROOT, "profiles", "pmc_traffic_valu_batch4096.md")):
+    """VALU wave-instructions per launch of the window-scan kernel from the
+    committed rocprofv3 SQ_INSTS_VALU pass, or None."""
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 5 and cells[1].startswith("`k_scan_uniform"):
+                return float(cells[5])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def _cpu_worker_init():
     # the reference prints a "seeds size" line per pair; keep stdout for the JSON line
     devnull = os.open(os.devnull, os.O_WRONLY)
@@ -201,7 +214,9 @@ def config2_significance(ctx, sur, log):
     cand = ctx.find_similar([(a, b)], params=p, with_cigar=False)[0][1]
     st = ctx.stats()
     res["pipeline"] = {"reference_cells": st["scan_cells"] + st["xdrop_cells"], "seeds": st["seeds"],
-                       "levels": st["levels"], "seconds": round(time.perf_counter() - t, 2)}
+                       "linked_seeds": st["linked_seeds"], "levels": st["levels"],
+                       "ms_seed": round(st["ms_seed"], 1), "ms_xdrop": round(st["ms_xdrop"], 1),
+                       "ms_dedup": round(st["ms_replay"], 1), "seconds": round(time.perf_counter() - t, 2)}
     sj = np.array([f[4] for f in cand], np.float64)
     pj, _ = lk.pvalue(lj[0], kj[0], len(a), len(b), sj)
     ps, _ = lk.pvalue(ls[0], ks[0], len(a), len(b), sj)
@@ -294,6 +309,13 @@ def main():
                      "work_per_launch": f"{l0:.4e} cells x {VALU_OPS_PER_CELL} int16 ops",
                      "avg_launch_ms": round(avg_ms, 3)},
     }
+
+    # the hardware view: VALU wave-instructions per launch (rocprofv3 pass) over
+    # this run's launch time, against 256 CU x 4 SIMD x clock / 4 cycles
+    vi = measured_valu_instructions()
+    if vi:
+        ceiling = dev["cu"] * 4 * dev["clock_mhz"] * 1e6 / 4
+        line["roofline"]["valu_issue_frac"] = round(vi / (avg_ms * 1e-3) / ceiling, 4)
 
     if not args.no_plain:  # the plain path: one pair per call, host buffers in, fragments out
         rng = np.random.default_rng(11)

@@ -129,6 +129,7 @@ struct dcns_ctx {
     dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound;
     dcns::DBuf sg_tasks, sg_out, sg_scr, sg_tb, sg_cig;  // two-piece semi-global tasks
     dcns::DBuf xw_scr, xw_off, xw_wmax;                  // x-drop tasks with bands past 161 cells
+    dcns::DBuf sort_scr, task_order;                     // scan tasks ordered by row count
     dcns::DBuf rnd_thr, rnd_len1, rnd_len2, rnd_seed, rnd_out;  // random fragment pairs generated on the device
     dcns::HBuf h_hits, h_roots, h_nodes, h_xout, h_cigar;  // pinned download buffers
 

@@ -14,13 +14,18 @@ hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const 
                                int32_t nwt, const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount,
                                int32_t cap, int32_t *hitlist, Counters *ctr);
 hipError_t launch_scan_general(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
-                               const ScanTask *tasks, const int32_t *ntasks_dev, int32_t ntasks_max, const Params &P,
+                               const ScanTask *tasks, const int32_t *order, const int32_t *ntasks_dev,
+                               int32_t ntasks_max, const Params &P,
                                const ScoreTab &tab, TrajEntry *traj, int32_t *tcount, int32_t cap, Counters *ctr);
 hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
                           const int32_t *pair_of_window, const Params &P, const ScoreTab &tab, int level0,
                           const int32_t *hitlist, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
                           Node *nodes, int32_t node_cap, const TrajEntry *traj, const int32_t *tcount, int32_t cap,
                           int32_t scan_cap, ScanTask *next, int32_t next_cap, Counters *ctr);
+// tasksort.hip: index of the scan tasks by row count, longest first.
+size_t sort_tasks_scratch_bytes(int32_t n_max);
+hipError_t launch_sort_tasks(hipStream_t st, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
+                             void *scratch, size_t scratch_bytes, int32_t *order);
 hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v);
 hipError_t launch_gather_roots(hipStream_t st, const Node *nodes, const int32_t *hitlist, const int32_t *n_dev,
                                int32_t n_max, Node *out);

@@ -255,10 +255,18 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
         h.next_tasks = 0;
         write_ctr(c, h);
         const int32_t *n_dev = &c->ctr.as<Counters>()->level_tasks;
+        // longest tasks first: the lanes of a wave then run similar row counts
+        const size_t sort_bytes = sort_tasks_scratch_bytes(n);
+        c->sort_scr.ensure(sort_bytes);
+        c->task_order.ensure((size_t)n * sizeof(int32_t) + 64);
+        c->launch("sort_tasks", [&] {
+            return launch_sort_tasks(c->stream, c->tasks_a.as<ScanTask>(), n_dev, n, c->sort_scr.p, sort_bytes,
+                                     c->task_order.as<int32_t>());
+        });
         c->launch("scan_general", [&] {
             return launch_scan_general(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
-                                       c->tasks_a.as<ScanTask>(), n_dev, n, P, tab, c->traj.as<TrajEntry>(),
-                                       c->tcount.as<int32_t>(), cap, c->ctr.as<Counters>());
+                                       c->tasks_a.as<ScanTask>(), c->task_order.as<int32_t>(), n_dev, n, P, tab,
+                                       c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap, c->ctr.as<Counters>());
         });
         h = read_ctr(c);
         resolve(0, c->tasks_a.as<ScanTask>(), n, n_dev);

@@ -255,7 +255,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 // saves two operations per cell.
 template <int WM>
 __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
-                                                     const ScanTask *__restrict__ tasks, const int32_t *__restrict__ ntasks_p,
+                                                     const ScanTask *__restrict__ tasks, const int32_t *__restrict__ order,
+                                                     const int32_t *__restrict__ ntasks_p,
                                                      Params P, ScoreTab tab, TrajEntry *__restrict__ traj,
                                                      int32_t *__restrict__ tcount, int32_t cap,
                                                      Counters *__restrict__ ctr) {
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ 
     __syncthreads();
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= *ntasks_p) return;
-    const ScanTask t = tasks[gid];
+    const ScanTask t = tasks[order ? order[gid] : gid];
     const PairDesc pd = pairs[t.pair];
     const int8_t *__restrict__ ref = seq + pd.off1;
     const int8_t *__restrict__ qry = seq + pd.off2;
@@ -516,14 +517,17 @@ hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const 
 }
 
 hipError_t launch_scan_general(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
-                               const ScanTask *tasks, const int32_t *ntasks_dev, int32_t ntasks_max, const Params &P,
-                               const ScoreTab &tab, TrajEntry *traj, int32_t *tcount, int32_t cap, Counters *ctr) {
+                               const ScanTask *tasks, const int32_t *order, const int32_t *ntasks_dev,
+                               int32_t ntasks_max, const Params &P, const ScoreTab &tab, TrajEntry *traj,
+                               int32_t *tcount, int32_t cap, Counters *ctr) {
     if (ntasks_max <= 0) return hipSuccess;
     dim3 g((ntasks_max + 63) / 64), b(64);
     if (wm <= 38)
-        hipLaunchKernelGGL(k_scan_general<38>, g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj, tcount, cap, ctr);
+        hipLaunchKernelGGL(k_scan_general<38>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, P, tab, traj, tcount,
+                           cap, ctr);
     else
-        hipLaunchKernelGGL(k_scan_general<64>, g, b, 0, st, seq, pairs, tasks, ntasks_dev, P, tab, traj, tcount, cap, ctr);
+        hipLaunchKernelGGL(k_scan_general<64>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, P, tab, traj, tcount,
+                           cap, ctr);
     return hipGetLastError();
 }
 

@@ -129,3 +129,25 @@ def test_seed_sets_other_scorings(gpu, oracle_lib, sc, min_score):
         got = gpu.seed_sets(pairs, params=p, linked=linked)
         for i, (a, b) in enumerate(pairs):
             assert got[i] == oracle_lib.seed_set(a, b, sc, min_score=min_score, linked=linked), (i, linked)
+
+
+def test_long_masked_repeat_pair_vs_oracle(gpu, oracle_lib):
+    """The shape of the 500 kb masked pair at a size the oracle finishes in
+    seconds: lowercase n runs stripped before alignment, 20 copies of a
+    1.5 kb repeat at 10% divergence (deep getSeeds recursion inside every
+    copy), 30 planted fragments.  Fragments, seed counts and the reference
+    cell count must all match."""
+    t1, t2 = synth.long_masked(seed=77, L=20000, te_copies=20, te_len=1500, nfrag=30)
+    a = synth.encode(t1.replace("n", ""))
+    b = synth.encode(t2.replace("n", ""))
+    print("masked repeat pair", len(a), len(b), flush=True)
+    got = gpu.find_similar([(a, b)])[0]
+    st = gpu.stats()
+    o = {}
+    want = oracle_lib.find_similar(a, b, stats=o)
+    assert got[0] == o["seeds"]
+    assert st["seeds"] == o["seeds"] and st["linked_seeds"] == o["linked"]
+    assert st["scan_cells"] == o["scan_cells"] and st["xdrop_cells"] == o["xdrop_cells"]
+    assert len(got[1]) == len(want) > 50
+    for g, w in zip(got[1], want):
+        assert list(g[:5]) == list(w[:5]) and g[5] == w[5] and g[6] == w[6] and list(g[7]) == list(w[7])
