@@ -72,7 +72,7 @@ struct Counters {
 struct XTask {
     int32_t pair;
     int32_t start1, start2;
-    int32_t pad;
+    int32_t hint;  // the seed's length (informational)
 };
 
 struct XOut {
@@ -96,6 +96,17 @@ struct SgTask {
     int32_t R, C;     // rows (seq1 length) and columns (seq2 length), both >= 1
     int64_t tb;       // byte offset of the traceback store, -1: score only
     int64_t scr;      // int32 offset of the boundary / row-result scratch
+    int64_t sched;    // first SgStrip of the task's strip schedule
+    int32_t nstrips;  // strips the band reaches
+    int32_t phases;   // 64-step phases of the whole schedule
+};
+
+// One 256-column strip of a SemiGlobal task in the multi-wave schedule.
+struct SgStrip {
+    int32_t start;     // phase at which the strip's step 0 runs
+    int32_t rlo, rhi;  // rows the band reaches in this strip
+    int32_t pad;
+    int64_t tb;        // byte offset of the strip's trace store inside the task's
 };
 
 struct SgOut {

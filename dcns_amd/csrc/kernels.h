@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "dcns_types.h"
 #include "dp_common.h"
 
@@ -36,8 +38,8 @@ hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const
                               int32_t n, const Params &P, XOut *out, int32_t *pending);
 // Pass 2: reverse extension with traceback into a band-compressed store, then the cigar.
 hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
-                              int32_t n, const Params &P, XOut *out, const int64_t *tb_off, uint8_t *tb,
-                              uint32_t *cigar);
+                              const int32_t *order, int32_t n, const Params &P, XOut *out, const int64_t *tb_off,
+                              uint8_t *tb, uint32_t *cigar);
 
 // Wide bands (2w+1 > 161): band rows in per-task global scratch (scr_off[t] < 0: task not handled).
 hipError_t launch_xdrop_wide_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
@@ -58,10 +60,14 @@ hipError_t launch_sw_random(hipStream_t st, const uint32_t *thr, const int32_t *
 
 // semiglobal.hip: two-piece semi-global DP, one wave per task.
 int64_t sg_trace_bytes(int32_t R, int32_t C, int32_t w);  // traceback store of one task
-int64_t sg_scratch_ints(int32_t R);                       // boundary + row-result scratch of one task
+int64_t sg_scratch_ints(int32_t R, int32_t nw);           // boundary + row-result scratch of one task
+// Strip schedule of one task for nw waves; returns the phase count.
+int32_t sg_schedule(int32_t R, int32_t C, int32_t w, int32_t nw, std::vector<SgStrip> &out);
 hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
-                        int32_t n, const Params &P, int32_t *scr, uint8_t *tb, SgOut *out);
+                        const SgStrip *strips, int32_t n, int32_t nw, const Params &P, int32_t *scr, uint8_t *tb,
+                        SgOut *out);
 hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks, int32_t n,
-                            const Params &P, const int32_t *scr, const uint8_t *tb, SgOut *out, uint32_t *cigar);
+                            int32_t nw, const Params &P, const int32_t *scr, const uint8_t *tb, SgOut *out,
+                            uint32_t *cigar);
 
 }  // namespace dcns

@@ -526,6 +526,30 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
     }
     hip_check(hipMemcpyAsync(c->xout.p, xo, (size_t)n * sizeof(XOut), hipMemcpyHostToDevice, c->stream), "xout up");
     upload(c, c->tb_off, tboff);
+    // pass 2 order: by band class (each instantiation's tasks contiguous),
+    // then by traced cells, largest first
+    {
+        std::vector<int64_t> key((size_t)n);
+        for (int32_t i = 0; i < n; ++i) {
+            const XOut &o = xo[i];
+            if (o.fe1 < 0 || o.fe2 < 0) {
+                key[(size_t)i] = INT64_MIN;
+                continue;
+            }
+            const int32_t B = 2 * o.rw + 1;
+            int cls = 4;
+            for (int k = 0; k < 4; ++k)
+                if (B <= bands[k]) {
+                    cls = k;
+                    break;
+                }
+            key[(size_t)i] = ((int64_t)(4 - cls) << 48) + (int64_t)(o.rrows + 1) * B;
+        }
+        std::vector<int32_t> order2((size_t)n);
+        for (int32_t i = 0; i < n; ++i) order2[(size_t)i] = i;
+        std::stable_sort(order2.begin(), order2.end(), [&](int32_t a, int32_t b) { return key[(size_t)a] > key[(size_t)b]; });
+        upload(c, c->xorder, order2);
+    }
     c->tb.ensure((size_t)std::max<int64_t>(tbtotal, 64));
     c->cigar.ensure((size_t)std::max<int64_t>(cigtotal, 16) * sizeof(uint32_t));
     if (wide) {
@@ -540,8 +564,8 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
         if (!any_band[k]) continue;
         c->launch("xdrop_pass2", [&] {
             return launch_xdrop_pass2(c->stream, bands[k], c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
-                                      c->xtasks.as<XTask>(), n, P, c->xout.as<XOut>(), c->tb_off.as<int64_t>(),
-                                      c->tb.as<uint8_t>(), c->cigar.as<uint32_t>());
+                                      c->xtasks.as<XTask>(), c->xorder.as<int32_t>(), n, P, c->xout.as<XOut>(),
+                                      c->tb_off.as<int64_t>(), c->tb.as<uint8_t>(), c->cigar.as<uint32_t>());
         });
     }
     c->h_cigar.ensure((size_t)std::max<int64_t>(cigtotal, 16) * sizeof(uint32_t));
@@ -628,7 +652,7 @@ void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &
             t.pair = i;
             t.start1 = s.getStart1() + 1;  // :492-495
             t.start2 = s.getStart2() + 1;
-            t.pad = 0;
+            t.hint = s.getEnd1() - s.getStart1() + 1;
             tasks.push_back(t);
         }
     }

@@ -66,7 +66,7 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
         while (pos < idx.size()) {
             SgTask t = tasks[(size_t)idx[pos]];
             const int64_t tbb = trace ? (sg_trace_bytes(t.R, t.C, P.bandwidth) + 255) / 256 * 256 : 0;
-            const int64_t scr = sg_scratch_ints(t.R);
+            const int64_t scr = sg_scratch_ints(t.R, 16);
             if (!ct.empty() && tb_total + tbb + 4 * (scr_total + scr) > budget) break;
             if (ct.empty() && tbb + 4 * scr > budget)
                 throw Error(DCNS_ERR_CAPACITY, "a SemiGlobal task's traceback store exceeds device memory");
@@ -84,6 +84,19 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
             ++pos;
         }
         const int32_t m = (int32_t)ct.size();
+        // waves per task: one per strip up to 16, the same for the whole launch
+        std::vector<SgStrip> strips, one;
+        int32_t nw = 1;
+        for (const SgTask &t : ct) nw = std::max(nw, std::min<int32_t>(16, (t.C + 255) / 256));
+        for (SgTask &t : ct) {
+            t.phases = sg_schedule(t.R, t.C, P.bandwidth, nw, one);
+            t.nstrips = (int32_t)one.size();
+            t.sched = (int64_t)strips.size();
+            strips.insert(strips.end(), one.begin(), one.end());
+        }
+        c->sg_strips.ensure(std::max<size_t>(strips.size(), 1) * sizeof(SgStrip));
+        hip_check(hipMemcpyAsync(c->sg_strips.p, strips.data(), strips.size() * sizeof(SgStrip), hipMemcpyHostToDevice,
+                                 c->stream), "sg strips");
         c->sg_tasks.ensure((size_t)m * sizeof(SgTask));
         c->sg_out.ensure((size_t)m * sizeof(SgOut));
         c->sg_scr.ensure((size_t)std::max<int64_t>(scr_total, 16) * 4);
@@ -97,12 +110,12 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
                   "sg out");
         c->launch("semiglobal", [&] {
             return launch_sg_dp(c->stream, trace, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
-                                m, P, c->sg_scr.as<int32_t>(), trace ? c->sg_tb.as<uint8_t>() : nullptr,
-                                c->sg_out.as<SgOut>());
+                                c->sg_strips.as<SgStrip>(), m, nw, P, c->sg_scr.as<int32_t>(),
+                                trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>());
         });
         c->launch("semiglobal_finish", [&] {
             return launch_sg_finish(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
-                                    m, P, c->sg_scr.as<int32_t>(), trace ? c->sg_tb.as<uint8_t>() : nullptr,
+                                    m, nw, P, c->sg_scr.as<int32_t>(), trace ? c->sg_tb.as<uint8_t>() : nullptr,
                                     c->sg_out.as<SgOut>(), trace ? c->sg_cig.as<uint32_t>() : nullptr);
         });
         hip_check(hipMemcpyAsync(so.data(), c->sg_out.p, (size_t)m * sizeof(SgOut), hipMemcpyDeviceToHost, c->stream),
@@ -153,7 +166,8 @@ void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, Fra
     std::vector<int64_t> first((size_t)npairs + 1, 0);
     for (int32_t i = 0; i < npairs; ++i) {
         first[(size_t)i] = (int64_t)xt.size();
-        for (const Seed &s : sr.linked[(size_t)i]) xt.push_back(XTask{i, s.getStart1() + 1, s.getStart2() + 1, 0});
+        for (const Seed &s : sr.linked[(size_t)i])
+            xt.push_back(XTask{i, s.getStart1() + 1, s.getStart2() + 1, s.getEnd1() - s.getStart1() + 1});
     }
     first[(size_t)npairs] = (int64_t)xt.size();
     std::vector<XResult> xr;

@@ -378,11 +378,13 @@ __device__ void trace_cigar(XOut &o, int32_t w, const uint8_t *__restrict__ T, i
 // Pass 2: reverse pass with traceback at the final band from pass 1.
 template <int BMAX>
 __global__ __launch_bounds__(64) void k_xdrop_pass2(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
-                                                    const XTask *__restrict__ tasks, int32_t n, Params P,
-                                                    XOut *__restrict__ out, const int64_t *__restrict__ tb_off,
-                                                    uint8_t *__restrict__ tb, uint32_t *__restrict__ cigar) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n) return;
+                                                    const XTask *__restrict__ tasks, const int32_t *__restrict__ order,
+                                                    int32_t n, Params P, XOut *__restrict__ out,
+                                                    const int64_t *__restrict__ tb_off, uint8_t *__restrict__ tb,
+                                                    uint32_t *__restrict__ cigar) {
+    const int lane_gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane_gid >= n) return;
+    const int gid = order ? order[lane_gid] : lane_gid;  // the host orders tasks by band class and traced cells
     const XTask t = tasks[gid];
     XOut o = out[gid];
     if (o.fe1 < 0 || o.fe2 < 0) return;
@@ -460,15 +462,15 @@ hipError_t launch_xdrop_wide_pass2(hipStream_t st, const int8_t *seq, const Pair
 }
 
 hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
-                              int32_t n, const Params &P, XOut *out, const int64_t *tb_off, uint8_t *tb,
-                              uint32_t *cigar) {
+                              const int32_t *order, int32_t n, const Params &P, XOut *out, const int64_t *tb_off,
+                              uint8_t *tb, uint32_t *cigar) {
     if (n <= 0) return hipSuccess;
     dim3 g((n + 63) / 64), b(64);
     switch (band) {
-        case 21: hipLaunchKernelGGL(k_xdrop_pass2<21>, g, b, 0, st, seq, pairs, tasks, n, P, out, tb_off, tb, cigar); break;
-        case 41: hipLaunchKernelGGL(k_xdrop_pass2<41>, g, b, 0, st, seq, pairs, tasks, n, P, out, tb_off, tb, cigar); break;
-        case 81: hipLaunchKernelGGL(k_xdrop_pass2<81>, g, b, 0, st, seq, pairs, tasks, n, P, out, tb_off, tb, cigar); break;
-        case 161: hipLaunchKernelGGL(k_xdrop_pass2<161>, g, b, 0, st, seq, pairs, tasks, n, P, out, tb_off, tb, cigar); break;
+        case 21: hipLaunchKernelGGL(k_xdrop_pass2<21>, g, b, 0, st, seq, pairs, tasks, order, n, P, out, tb_off, tb, cigar); break;
+        case 41: hipLaunchKernelGGL(k_xdrop_pass2<41>, g, b, 0, st, seq, pairs, tasks, order, n, P, out, tb_off, tb, cigar); break;
+        case 81: hipLaunchKernelGGL(k_xdrop_pass2<81>, g, b, 0, st, seq, pairs, tasks, order, n, P, out, tb_off, tb, cigar); break;
+        case 161: hipLaunchKernelGGL(k_xdrop_pass2<161>, g, b, 0, st, seq, pairs, tasks, order, n, P, out, tb_off, tb, cigar); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
