@@ -190,6 +190,38 @@ def lambdak_phase(ctx, log):
     return out
 
 
+def two_piece_phase(ctx, log):
+    """The two-piece path (findSimilarFragmentsForPairedSequence overload 4,
+    cut2Gaps) on the 10 kb x 10 kb planted pair, beside the compiled
+    reference's overload 4 on the same pair (one core), results compared."""
+    from dcns_amd import synth
+    from oracle import bind
+    a, b = synth.one_pair()[:2]
+    ctx.find_similar_2gaps([(a, b)])
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        res = ctx.find_similar_2gaps([(a, b)])
+    dt = (time.perf_counter() - t) / reps
+    st = ctx.stats()
+    cells = st["scan_cells"] + st["xdrop_cells"] + st["semiglobal_cells"]
+    out = {"workload": "10,000 x 10,000 pair, 20 planted fragments, cut2Gaps defaults (open2 -45, ext2 0, "
+                       "z-drop 50, band 20000)",
+           "reference_cells": cells, "semiglobal_cells": st["semiglobal_cells"], "seconds": round(dt, 4),
+           "gcups": round(cells / dt / 1e9, 2), "semiglobal_ms": round(st["ms_semiglobal"], 1),
+           "fragments": len(res[0][1])}
+    ref = bind.reference()
+    if ref is not None:
+        t = time.perf_counter()
+        want = ref.find_similar_2gaps(a, b)
+        tc = time.perf_counter() - t
+        out["cpu_reference"] = {"seconds": round(tc, 3), "gcups": round(cells / tc / 1e9, 4), "cores": 1}
+        out["bit_exact"] = [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in res[0][1]] == \
+                           [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in want]
+    log(f"two-piece: {cells:.3e} cells in {dt * 1e3:.1f} ms ({cells / dt / 1e9:.1f} GCUPS)")
+    return out
+
+
 def config2_significance(ctx, sur, log):
     """Config-2 fragments (500 kb masked pair, default scoring) reported with
     the lambda/K of the judge, of the surrogate and of the reference's
@@ -246,6 +278,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-lambdak", action="store_true")
+    ap.add_argument("--no-two-piece", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs shard across GPUs as "
@@ -333,9 +366,12 @@ def main():
                                        "call each (staging included)"}
         line["batched_vs_plain"] = round(gcups / plain, 2)
 
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    if not args.no_two_piece:
+        line["two_piece"] = two_piece_phase(ctx, log)
     if not args.no_lambdak:
-        def log(m):
-            print(m, file=sys.stderr, flush=True)
         line["lambdak"] = lambdak_phase(ctx, log)
 
     if not args.no_cpu:

@@ -392,6 +392,35 @@ void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked
         int32_t *hk = by_pair.data() + first[(size_t)p];
         const int64_t nh = first[(size_t)p + 1] - first[(size_t)p];
         std::sort(hk, hk + nh, [&](int32_t a, int32_t b) { return F.hit_window[a] < F.hit_window[b]; });
+        // The children's sets depend only on the recursion tree, not on the
+        // pair's set: a pair with few windows in flight (one long pair) builds
+        // them on all host threads first, in their own set order; the
+        // window loop below then replays the gate and the insertions in the
+        // reference's order.
+        struct Kids {
+            std::vector<Seed> left, right;
+            int64_t cells = 0;
+        };
+        std::vector<Kids> kids;
+        const bool prebuild = np < 16 && nh > 64;
+        if (prebuild) {
+            kids.resize((size_t)nh);
+            parallel_for(nh, [&](int64_t q) {
+                const Node &r = F.roots[hk[q]];
+                if (r.e1 < 0) return;
+                const int32_t sp = (F.hit_window[hk[q]] - d.win_base) * P.step;
+                const int32_t tw = std::min(P.window, d.len2 - sp);
+                Kids &k = kids[(size_t)q];
+                if (r.left >= 0) {
+                    std::set<Seed> x = inner_set(F, r.left, sp, tw, P.window, k.cells);
+                    k.left.assign(x.begin(), x.end());
+                }
+                if (r.right >= 0) {
+                    std::set<Seed> x = inner_set(F, r.right, sp, tw, P.window, k.cells);
+                    k.right.assign(x.begin(), x.end());
+                }
+            });
+        }
         std::set<Seed> seeds;
         int64_t cl = S.level0_cells[(size_t)p];
         for (int64_t q = 0; q < nh; ++q) {
@@ -404,6 +433,13 @@ void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked
             const Seed s0 = seed_of(r);
             if (seeds.find(s0) != seeds.end()) continue;  // the set gate of :170
             seeds.insert(s0);
+            if (prebuild) {
+                const Kids &k = kids[(size_t)q];
+                for (const Seed &x : k.left) seeds.insert(x);
+                for (const Seed &x : k.right) seeds.insert(x);
+                cl += k.cells;
+                continue;
+            }
             if (r.left >= 0)
                 for (const Seed &x : inner_set(F, r.left, sp, tw, P.window, cl)) seeds.insert(x);
             if (r.right >= 0)
