@@ -95,18 +95,20 @@ struct SgTask {
     int32_t pair, rev, a1, a2;
     int32_t R, C;     // rows (seq1 length) and columns (seq2 length), both >= 1
     int64_t tb;       // byte offset of the traceback store, -1: score only
-    int64_t scr;      // int32 offset of the boundary / row-result scratch
-    int64_t sched;    // first SgStrip of the task's strip schedule
+    int64_t scr;      // int32 offset of the row results (2 ints per row)
+    int64_t sched;    // first SgStrip of the task in the launch's strip list
     int32_t nstrips;  // strips the band reaches
-    int32_t phases;   // 64-step phases of the whole schedule
+    int32_t pad;
 };
 
-// One 256-column strip of a SemiGlobal task in the multi-wave schedule.
+// One 256-column strip of a SemiGlobal task.  A launch lists the strips of
+// all its tasks, each task's strips consecutive and in column order.
 struct SgStrip {
-    int32_t start;     // phase at which the strip's step 0 runs
+    int32_t task;      // task index within the launch
+    int32_t s;         // strip index within the task (columns s*256+1 ..)
     int32_t rlo, rhi;  // rows the band reaches in this strip
-    int32_t pad;
     int64_t tb;        // byte offset of the strip's trace store inside the task's
+    int64_t bnd;       // int32 offset of the strip's right-boundary records (rows rlo..rhi)
 };
 
 struct SgOut {

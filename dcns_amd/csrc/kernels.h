@@ -58,16 +58,18 @@ hipError_t launch_sw_random(hipStream_t st, const uint32_t *thr, const int32_t *
                             const uint64_t *seed, int32_t per_point, int64_t first, int64_t n, const Params &P,
                             const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride);
 
-// semiglobal.hip: two-piece semi-global DP, one wave per task.
-int64_t sg_trace_bytes(int32_t R, int32_t C, int32_t w);  // traceback store of one task
-int64_t sg_scratch_ints(int32_t R, int32_t nw);           // boundary + row-result scratch of one task
-// Strip schedule of one task for nw waves; returns the phase count.
-int32_t sg_schedule(int32_t R, int32_t C, int32_t w, int32_t nw, std::vector<SgStrip> &out);
+// semiglobal.hip: two-piece semi-global DP, one wave per 256-column strip.
+int64_t sg_trace_bytes(int32_t R, int32_t C, int32_t w);   // traceback store of one task
+int64_t sg_scratch_ints(int32_t R, int32_t C, int32_t w);  // row results + strip boundaries of one task
+// Appends the strips of one task (launch-local index `task`, boundary records
+// from int32 offset bnd on); returns their number.
+int32_t sg_task_strips(int32_t R, int32_t C, int32_t w, int32_t task, int64_t bnd, std::vector<SgStrip> &out);
+// sync: 2 + nstrips zeroed ints (start counter, error word, per-strip progress).
 hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
-                        const SgStrip *strips, int32_t n, int32_t nw, const Params &P, int32_t *scr, uint8_t *tb,
-                        SgOut *out);
-hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks, int32_t n,
-                            int32_t nw, const Params &P, const int32_t *scr, const uint8_t *tb, SgOut *out,
-                            uint32_t *cigar);
+                        const SgStrip *strips, int32_t nstrips, const Params &P, int32_t *scr, uint8_t *tb,
+                        SgOut *out, int32_t *sync);
+hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
+                            const SgStrip *strips, int32_t n, const Params &P, const int32_t *scr, const uint8_t *tb,
+                            SgOut *out, uint32_t *cigar);
 
 }  // namespace dcns

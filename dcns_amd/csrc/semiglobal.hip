@@ -12,23 +12,26 @@
 //   the first strict maximum; the caller's termination test (z-drop, empty
 //   row) runs over the row results afterwards.
 //
-// Device layout: one workgroup of up to 16 waves per task.  Columns are cut
-// into strips of 64 lanes x 4 columns, strip s on wave s % nw; inside a strip
-// lane l works on row r = t - l at step t (an anti-diagonal skew), so the
-// values a row carries from left to right (M, the two horizontal gap states,
-// the running row maximum, the reference base) move one lane per step by a
-// DPP wave shift.  The strip's last lane hands each row's carried values to
-// the next strip through a per-task boundary array; the next strip's first
-// lane reads them 64 rows at a time, a barrier-separated phase later.  Every row in the band is
-// computed (the reference's termination test usually never fires with the
-// default long-gap piece, whose column state keeps every later row within
-// open2 of the best score); the finish kernel cuts the rows at the reference's
-// termination row.  Trace bytes are stored step-major (64 lanes x 4 bytes per
-// step, one coalesced 256-byte store), 255 outside the band as after
-// Matrix::reset.
+// Device layout: columns are cut into strips of 64 lanes x 4 columns and
+// every strip of every task is one wave, so one long task spreads over as
+// many CUs as it has strips.  Inside a strip lane l works on row r = t - l at
+// step t (an anti-diagonal skew), so the values a row carries from left to
+// right (M, the two horizontal gap states, the running row maximum, the
+// reference base) move one lane per step by a DPP wave shift.  The strip's
+// last lane writes each row's carried values to the strip's boundary array;
+// every 64 steps the strip publishes how many boundary rows are complete (a
+// progress word), and the next strip waits for the rows its first lane needs
+// in its next 64 steps.  Every row in
+// the band is computed (the reference's termination test usually never
+// fires with the default long-gap piece, whose column state keeps every later
+// row within open2 of the best score); the finish kernel cuts the rows at the
+// reference's termination row.  Trace bytes are stored step-major per strip
+// (64 lanes x 4 bytes per step, one coalesced 256-byte store), 255 outside the
+// band as after Matrix::reset.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "dp_common.h"
@@ -41,16 +44,62 @@ constexpr int K = 4;                 // columns per lane
 constexpr int SW = 64 * K;           // strip width
 constexpr int32_t NEG = -536870912;  // SCORE_OUT_BANDED_ALIGNMENT_REGION
 constexpr int BND = 5;               // ints per boundary record: M, e1, e2, row max, row max column
-
-__device__ __forceinline__ int32_t shr1(int32_t v) {
-    // wave_shr:1 -- lane l receives lane l-1's value; lane 0 keeps its own
-    return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);
-}
+// Longest wait for a predecessor strip, in wall-clock ticks (100 MHz): far
+// beyond any legitimate wait, which is the time the predecessor needs for 64
+// rows plus its own start.
+constexpr uint64_t kWaitTicks = 500000000ull;
 
 __device__ __forceinline__ int32_t code_at(const int8_t *base, int32_t a, int32_t k, int32_t rev) {
     // k-th base (0-based) of the forward suffix starting at a, or of the
     // reverse complement of the prefix ending at a
     return rev ? 4 - (int32_t)base[a - k] : (int32_t)base[a + k];
+}
+
+// Boundary records and row results cross CUs (and XCD L2s): agent-scope
+// atomic accesses.
+__device__ __forceinline__ int32_t ald(const int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ast(int32_t *p, int32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Strip-to-strip hand-over.  Every value one strip reads from another (the
+// boundary records, the progress words, the row results) is written and read
+// with agent-scope atomic accesses, which the gfx9.5 memory model makes
+// coherent across CUs and XCD L2s by themselves.  An agent-scope release or
+// acquire would add an L2 write-back before each progress store and an L2
+// invalidate after each progress load; both exist for plain (non-atomic)
+// data, of which strips exchange none, and the write-back of every strip's
+// trace bytes every 64 steps costs more than the DP.  So the hand-over is the
+// remaining part of release / acquire: the producer waits for all its vector
+// memory operations to complete (its boundary stores are then visible at
+// agent scope) before it stores the progress word, and the consumer issues
+// its boundary loads only after it has observed the progress word.
+__device__ __forceinline__ void publish_rows(int32_t *prog, int32_t rows) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): earlier loads and stores complete
+    __hip_atomic_store(prog, rows, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Waits until *prog >= need.  False when another strip has failed or the
+// wait exceeds kWaitTicks; the caller then abandons its strip.
+__device__ bool wait_rows(const int32_t *prog, int32_t need, const int32_t *err) {
+    bool ok = ald(prog) >= need;
+    if (!ok) {
+        const uint64_t t0 = wall_clock64();
+        for (;;) {
+            __builtin_amdgcn_s_sleep(2);
+            if (ald(prog) >= need) {
+                ok = true;
+                break;
+            }
+            if (ald(err) != 0 || wall_clock64() - t0 > kWaitTicks) break;
+        }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return ok;
 }
 
 // Scores of one column for reference codes 0..3 (bytes of lo) and 4 (hi).
@@ -63,23 +112,11 @@ __device__ __forceinline__ void column_table(int32_t qc, int32_t match, int32_t 
     hi = (uint32_t)((qc == 2) ? 0 : (qc == 4 ? match : mismatch)) & 0xffu;
 }
 
-struct StripGeom {
-    int32_t jb, je, rlo, rhi;
-};
-__device__ __forceinline__ StripGeom strip_geom(int32_t s, int32_t R, int32_t C, int32_t w) {
-    StripGeom g;
-    g.jb = s * SW + 1;
-    g.je = min(g.jb + SW - 1, C);
-    g.rlo = max(1, g.jb - w);
-    g.rhi = min(min(R, g.je + w), C + w);  // rows past C + w are handled by the finish kernel
-    return g;
-}
-
 // Lane-0 input of row r of a strip: the previous strip's boundary record,
 // or the left edge of the matrix (M(0,0) = 0, M(r,0) = NEG); and the
 // reference code of row r.
 struct StripIn {
-    const int32_t *bin;
+    const int32_t *bin;  // previous strip's records, row plo first
     int32_t first, plo, phi, R, a1, rev;
     const int8_t *s1;
     __device__ __forceinline__ void load(int32_t r, int32_t &m, int32_t &e1, int32_t &e2, int32_t &rm, int32_t &rj,
@@ -91,349 +128,449 @@ struct StripIn {
         if (first) {
             if (r == 0) m = 0;
         } else if (r >= plo && r <= phi) {
-            const int32_t *b = bin + (int64_t)r * BND;
-            m = b[0];
-            e1 = b[1];
-            e2 = b[2];
-            rm = b[3];
-            rj = b[4];
+            const int32_t *b = bin + (int64_t)(r - plo) * BND;
+            m = ald(b);
+            e1 = ald(b + 1);
+            e2 = ald(b + 2);
+            rm = ald(b + 3);
+            rj = ald(b + 4);
         }
         if (r >= 1 && r <= R) rc = code_at(s1, a1, r - 1, rev);
     }
 };
 
-// One workgroup of nw waves per task.  Strip s runs on wave s % nw, starting
-// at the phase the host's schedule gives it; a phase is 64 steps of every
-// active wave followed by a workgroup barrier, which publishes the strip
-// boundaries written in the phase.  The schedule starts each strip late
-// enough that every boundary row its first lane loads at the start of a phase
-// was written in an earlier phase, and a wave takes its next strip only after
-// its current one, so nw + 1 boundary arrays suffice (strip s reads array
-// s % (nw+1) and writes (s+1) % (nw+1)).
+// One step of a lane's K columns on row r: scores from the column tables,
+// the five-way maximum, the row maximum, the trace byte and the gap states.
+// In-band columns are lo <= c < hi, and c == hi is the band's last column.
+// Columns left of the band start never re-enter the band, so their state may
+// hold anything; columns right of the band end keep the NEG vertical state
+// they start with until they become the band's last column, which takes the
+// maximum only.  EDGE = false: every column is inside the band.
+template <bool TB, bool EDGE>
+__device__ __forceinline__ void columns(const uint32_t (&tlo)[K], const uint32_t (&thi)[K], int32_t (&F1)[K],
+                                        int32_t (&F2)[K], int32_t (&Mp)[K], bool hsel, uint32_t sh, int32_t lo,
+                                        int32_t hi, int32_t jfirst, int32_t o1, int32_t x1, int32_t o2, int32_t x2,
+                                        int32_t dg, int32_t &e1, int32_t &e2, int32_t &rm, int32_t &rj,
+                                        uint32_t &tword) {
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const int32_t sc = __builtin_amdgcn_sbfe((int32_t)(hsel ? thi[c] : tlo[c]), sh, 8);
+        const int32_t m0 = sc + dg;
+        dg = Mp[c];
+        const int32_t f1 = F1[c], f2o = F2[c];
+        const int32_t m = max(max(max(m0, f1), max(e1, f2o)), e2);
+        const bool left = EDGE && c < lo, inner = !EDGE || c < hi;
+        const bool up = (EDGE ? (!left & (c <= hi)) : true) & (m > rm);
+        rm = up ? m : rm;
+        rj = up ? jfirst + c : rj;
+        const int32_t h = m + o1, f = f1 + x1, ne1 = e1 + x1;
+        const int32_t h2 = m + o2, f2 = f2o + x2, ne2 = e2 + x2;
+        if (TB) {
+            // source: the last of diag, F1, e1, F2, e2 that reaches the maximum
+            int32_t src = (f1 == m) ? 1 : 0;
+            src = (e1 == m) ? 2 : src;
+            src = (f2o == m) ? 3 : src;
+            src = (e2 == m) ? 4 : src;
+            const int32_t bits =
+                ((f >= h) ? 8 : 0) | ((ne1 >= h) ? 16 : 0) | ((f2 >= h2) ? 32 : 0) | ((ne2 >= h2) ? 64 : 0);
+            const uint32_t byte = (!left & inner) ? (uint32_t)(src | bits) : 0xffu;
+            tword = c == 0 ? byte : (tword | (byte << (8 * c)));
+        }
+        F1[c] = inner ? max(f, h) : f1;
+        F2[c] = inner ? max(f2, h2) : f2o;
+        e1 = left ? NEG : max(ne1, h);
+        e2 = left ? NEG : max(ne2, h2);
+        Mp[c] = m;
+    }
+}
+
+// One wave per strip.  Strips take their list index from a launch-wide
+// counter in the order they start running, so a strip's predecessor (the
+// entry before it) has always started: every wait ends, whatever number of
+// waves the device holds at once.  sync[0] = the counter, sync[1] = the
+// error word, sync[2 + k] = rows of strip k whose boundary records are
+// complete.
 template <bool TB>
-__global__ __launch_bounds__(1024) void k_sg_dp(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
-                                                const SgTask *__restrict__ tasks, const SgStrip *__restrict__ strips,
-                                                Params P, int32_t nw, int32_t *__restrict__ scr,
-                                                uint8_t *__restrict__ tb, SgOut *__restrict__ out) {
-    const SgTask T = tasks[blockIdx.x];
+__global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                 const SgTask *__restrict__ tasks, const SgStrip *__restrict__ strips,
+                                                 int32_t total, Params P, int32_t *__restrict__ scr,
+                                                 uint8_t *__restrict__ tb, SgOut *__restrict__ out,
+                                                 int32_t *__restrict__ sync) {
+    const int lane = threadIdx.x;
+    int32_t k = 0;
+    if (lane == 0) k = atomicAdd(sync, 1);
+    k = __builtin_amdgcn_readfirstlane(k);
+    if (k >= total) return;
+    int32_t *err = sync + 1, *prog = sync + 2;
+    const SgStrip S = strips[k];
+    const SgTask T = tasks[S.task];
     const PairDesc pd = pairs[T.pair];
     const int8_t *s1 = seq + pd.off1, *s2 = seq + pd.off2;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int32_t R = T.R, C = T.C, w = P.bandwidth;
     const int32_t o1 = P.open, x1 = P.ext, o2 = P.open2, x2 = P.ext2;
-    const int64_t bsz = (int64_t)(R + 2) * BND;
-    int32_t *rowres = scr + T.scr + bsz * (nw + 1);
-    const SgStrip *sched = strips + T.sched;
+    int32_t *rowres = scr + T.scr;
+    int32_t *bout = scr + S.bnd;
     // the lane and column that hold column C-1 (for the stale diagonals)
     const int32_t stale_lane = (C >= 2) ? ((C - 2) % SW) / K : -1, stale_c = (C >= 2) ? (C - 2) % K : -1;
     const int32_t stale_strip = (C >= 2) ? (C - 2) / SW : -1;
 
-    int32_t cs = wv;  // this wave's current strip
+    StripIn in;
+    in.first = S.s == 0;
+    in.plo = 1;
+    in.phi = 0;
+    in.bin = nullptr;
+    if (!in.first) {
+        const SgStrip Q = strips[k - 1];
+        in.plo = Q.rlo;
+        in.phi = Q.rhi;
+        in.bin = scr + Q.bnd;
+    }
+    in.R = R;
+    in.a1 = T.a1;
+    in.rev = T.rev;
+    in.s1 = s1;
+
+    const int32_t nsteps = S.rhi - S.rlo + 1 + 63;
+    const int32_t jfirst = S.s * SW + 1 + lane * K;
     uint32_t tlo[K], thi[K];
     int32_t F1[K], F2[K], Mp[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const int32_t j = jfirst + c;
+        column_table(j <= C ? code_at(s2, T.a2, j - 1, T.rev) : 2, P.match, P.mismatch, tlo[c], thi[c]);
+        F1[c] = F2[c] = Mp[c] = NEG;
+    }
     int32_t prev_im = NEG, om = NEG, oe1 = NEG, oe2 = NEG, orm = 0, orj = 0, orc = 2;
-    for (int32_t ph = 0; ph < T.phases; ++ph) {
-        if (cs < T.nstrips && ph >= sched[cs].start) {
-            const SgStrip S = sched[cs];
-            const int32_t t0 = (ph - S.start) * 64, nsteps = S.rhi - S.rlo + 1 + 63;
-            const int32_t jb = cs * SW + 1, jfirst = jb + lane * K;
-            StripIn in;
-            in.bin = scr + T.scr + bsz * (cs % (nw + 1));
-            in.first = cs == 0;
-            in.plo = cs > 0 ? sched[cs - 1].rlo : 1;
-            in.phi = cs > 0 ? sched[cs - 1].rhi : 0;
-            in.R = R;
-            in.a1 = T.a1;
-            in.rev = T.rev;
-            in.s1 = s1;
-            int32_t *bout = scr + T.scr + bsz * ((cs + 1) % (nw + 1));
-            if (t0 == 0) {  // a new strip: column tables, empty state above row rlo
-#pragma unroll
-                for (int c = 0; c < K; ++c) {
-                    const int32_t j = jfirst + c;
-                    column_table(j <= C ? code_at(s2, T.a2, j - 1, T.rev) : 2, P.match, P.mismatch, tlo[c], thi[c]);
-                    F1[c] = F2[c] = Mp[c] = NEG;
-                }
-                int32_t a, b, c2, d, f;
-                in.load(S.rlo - 1, prev_im, a, b, c2, d, f);
-                if (lane != 0) prev_im = NEG;
-                om = oe1 = oe2 = NEG;
-                orm = orj = 0;
-                orc = 2;
+    bool ok = true;
+    for (int32_t t0 = 0; t0 < nsteps; t0 += 64) {
+        if (!in.first) {
+            const int32_t need = min(S.rlo + t0 + 63, in.phi);
+            if (need >= in.plo && !wait_rows(prog + k - 1, need, err)) {
+                ok = false;
+                break;
             }
-            int32_t cm, ce1, ce2, crm, crj, crc;  // rows rlo + t0 + lane
-            in.load(S.rlo + t0 + lane, cm, ce1, ce2, crm, crj, crc);
-            const int32_t t1 = min(t0 + 64, nsteps);
-            for (int32_t t = t0; t < t1; ++t) {
-                const int32_t q = t - t0;
-                int32_t im = shr1(om), ie1 = shr1(oe1), ie2 = shr1(oe2), irm = shr1(orm), irj = shr1(orj),
-                        irc = shr1(orc);
-                {
-                    const int32_t a = __builtin_amdgcn_readlane(cm, q), b = __builtin_amdgcn_readlane(ce1, q),
-                                  c2 = __builtin_amdgcn_readlane(ce2, q), d = __builtin_amdgcn_readlane(crm, q),
-                                  e = __builtin_amdgcn_readlane(crj, q), f = __builtin_amdgcn_readlane(crc, q);
-                    if (lane == 0) {
-                        im = a; ie1 = b; ie2 = c2; irm = d; irj = e; irc = f;
-                    }
-                }
-                const int32_t r = S.rlo + t - lane;
-                const bool valid = lane <= t && r <= S.rhi;
-                const int32_t diag = prev_im;
-                prev_im = im;
-                uint32_t tword = 0xffffffffu;
-                if (valid) {
-                    const int32_t st = max(r - w, 1), en = min(r + w, C);
-                    const int32_t lo = st - jfirst, hi = en - jfirst;  // in-band columns: lo <= c < hi; c == hi: last
-                    const bool hsel = irc >= 4;
-                    const uint32_t sh = (uint32_t)(irc & 3) * 8u;
-                    int32_t e1 = ie1, e2 = ie2, rm = irm, rj = irj, dg = diag;
-#pragma unroll
-                    for (int c = 0; c < K; ++c) {
-                        const int32_t sc = __builtin_amdgcn_sbfe((int32_t)(hsel ? thi[c] : tlo[c]), sh, 8);
-                        const int32_t m0 = sc + dg;
-                        dg = Mp[c];
-                        const int32_t m = max(max(max(m0, F1[c]), max(e1, F2[c])), e2);
-                        const bool inb = c >= lo && c < hi;
-                        if (c >= lo && c <= hi && m > rm) {
-                            rm = m;
-                            rj = jfirst + c;
-                        }
-                        const int32_t h = m + o1, f = F1[c] + x1, ne1 = e1 + x1;
-                        const int32_t h2 = m + o2, f2 = F2[c] + x2, ne2 = e2 + x2;
-                        if (TB) {
-                            // source: the last of diag, F1, e1, F2, e2 that reaches the maximum
-                            int32_t d = (e2 == m) ? 4 : (F2[c] == m) ? 3 : (e1 == m) ? 2 : (F1[c] == m) ? 1 : 0;
-                            d |= (f >= h ? 8 : 0) | (ne1 >= h ? 16 : 0) | (f2 >= h2 ? 32 : 0) | (ne2 >= h2 ? 64 : 0);
-                            if (inb) tword = (tword & ~(0xffu << (8 * c))) | ((uint32_t)d << (8 * c));
-                        }
-                        if (inb) {
-                            F1[c] = max(f, h);
-                            F2[c] = max(f2, h2);
-                            e1 = max(ne1, h);
-                            e2 = max(ne2, h2);
-                        } else {
-                            if (c == hi) F1[c] = F2[c] = NEG;  // the band's last column keeps no vertical state
-                            e1 = e2 = NEG;
-                        }
-                        Mp[c] = m;
-                    }
-                    om = Mp[K - 1];
-                    oe1 = e1;
-                    oe2 = e2;
-                    orm = rm;
-                    orj = rj;
-                    if (cs == stale_strip && lane == stale_lane && (r == C + w - 2 || r == C + w - 1))
-                        out[blockIdx.x].stale[r - (C + w - 2)] = Mp[stale_c];
-                } else {
-                    om = NEG;
-                    oe1 = oe2 = NEG;
-                    orm = orj = 0;
-                }
-                orc = irc;
-                if (TB) *(uint32_t *)(tb + T.tb + S.tb + (int64_t)t * SW + lane * K) = tword;
-                if (lane == 63 && valid) {
-                    int32_t *b = bout + (int64_t)r * BND;
-                    b[0] = om;
-                    b[1] = oe1;
-                    b[2] = oe2;
-                    b[3] = orm;
-                    b[4] = orj;
-                    rowres[2 * (int64_t)r] = orm;
-                    rowres[2 * (int64_t)r + 1] = orj;
-                }
-            }
-            if (t1 >= nsteps) cs += nw;
         }
-        __syncthreads();
+        if (t0 == 0) {  // the diagonal of the strip's first row
+            int32_t a, b, c2, d, f;
+            in.load(S.rlo - 1, prev_im, a, b, c2, d, f);
+            if (lane != 0) prev_im = NEG;
+        }
+        int32_t cm, ce1, ce2, crm, crj, crc;  // rows rlo + t0 + lane
+        in.load(S.rlo + t0 + lane, cm, ce1, ce2, crm, crj, crc);
+        const int32_t t1 = min(t0 + 64, nsteps);
+        for (int32_t t = t0; t < t1; ++t) {
+            const int32_t q = t - t0;
+            // lanes 1..63 take the carried values of lane l-1; lane 0 takes row
+            // rlo + t's input (the DPP shift leaves the old operand in lane 0)
+            const int32_t im = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(cm, q), om, 0x138, 0xf, 0xf, false);
+            const int32_t ie1 = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(ce1, q), oe1, 0x138, 0xf, 0xf,
+                                                            false);
+            const int32_t ie2 = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(ce2, q), oe2, 0x138, 0xf, 0xf,
+                                                            false);
+            const int32_t irm = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(crm, q), orm, 0x138, 0xf, 0xf,
+                                                            false);
+            const int32_t irj = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(crj, q), orj, 0x138, 0xf, 0xf,
+                                                            false);
+            const int32_t irc = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(crc, q), orc, 0x138, 0xf, 0xf,
+                                                            false);
+            const int32_t r = S.rlo + t - lane;
+            const bool valid = lane <= t && r <= S.rhi;
+            const int32_t diag = prev_im;
+            prev_im = im;
+            uint32_t tword = 0xffffffffu;
+            if (valid) {
+                const int32_t st = max(r - w, 1), en = min(r + w, C);
+                const int32_t lo = st - jfirst, hi = en - jfirst;  // in-band columns: lo <= c < hi; c == hi: last
+                const bool hsel = irc >= 4;
+                const uint32_t sh = (uint32_t)(irc & 3) * 8u;
+                int32_t e1 = ie1, e2 = ie2, rm = irm, rj = irj;
+                // every lane's columns inside the band (the usual case): no edge masks
+                if (__ballot(lo > 0 || hi < K) == 0)
+                    columns<TB, false>(tlo, thi, F1, F2, Mp, hsel, sh, lo, hi, jfirst, o1, x1, o2, x2, diag, e1, e2,
+                                       rm, rj, tword);
+                else
+                    columns<TB, true>(tlo, thi, F1, F2, Mp, hsel, sh, lo, hi, jfirst, o1, x1, o2, x2, diag, e1, e2,
+                                      rm, rj, tword);
+                om = Mp[K - 1];
+                oe1 = e1;
+                oe2 = e2;
+                orm = rm;
+                orj = rj;
+                if (S.s == stale_strip && lane == stale_lane && (r == C + w - 2 || r == C + w - 1))
+                    out[S.task].stale[r - (C + w - 2)] = Mp[stale_c];
+            } else {
+                om = NEG;
+                oe1 = oe2 = NEG;
+                orm = orj = 0;
+            }
+            orc = irc;
+            if (TB) *(uint32_t *)(tb + T.tb + S.tb + (int64_t)t * SW + lane * K) = tword;
+            if (lane == 63 && valid) {
+                int32_t *b = bout + (int64_t)(r - S.rlo) * BND;
+                ast(b, om);
+                ast(b + 1, oe1);
+                ast(b + 2, oe2);
+                ast(b + 3, orm);
+                ast(b + 4, orj);
+                if ((min(r + w, C) - 1) / SW == S.s) {  // the row's last strip holds its result
+                    ast(rowres + 2 * (int64_t)r, orm);
+                    ast(rowres + 2 * (int64_t)r + 1, orj);
+                }
+            }
+        }
+        // boundary rows up to rlo + t1 - 64 are written (the last lane trails by 63 steps)
+        publish_rows(prog + k, min(S.rlo + t1 - 64, S.rhi));
+    }
+    if (!ok) {
+        if (lane == 0) ast(err, 1);
+        publish_rows(prog + k, INT_MAX);
     }
 }
 
-// One thread per task: the reference's row loop termination over the row
-// results (z-drop, empty row), the maximum, and the traceback (trace tasks).
-__device__ __forceinline__ uint32_t trace_byte(const uint8_t *tb, int64_t base, int32_t R, int32_t C, int32_t w,
-                                               int32_t ii, int32_t jj, int32_t &cs, int64_t &cbase) {
-    const int32_t s = (jj - 1) / SW;
-    while (cs > s) {  // walk the strip base back (jj only decreases)
-        --cs;
-        const StripGeom g = strip_geom(cs, R, C, w);
-        cbase -= (int64_t)(g.rhi - g.rlo + 1 + 63) * SW;
-    }
-    const StripGeom g = strip_geom(s, R, C, w);
-    if (ii < g.rlo || ii > g.rhi) return 255u;
-    const int32_t within = (jj - 1) % SW, l = within / K, c = within % K;
-    return tb[base + cbase + (int64_t)(ii - g.rlo + l) * SW + l * K + c];
+// Trace word (4 columns j..j+3, j = 1 mod 4) of row r, 0xffffffff outside
+// the band's strips and rows.
+__device__ __forceinline__ uint32_t trace_word(const uint8_t *tb, const SgTask &T, const SgStrip *strips, int32_t r,
+                                               int32_t j) {
+    const int32_t s = (j - 1) / SW;
+    if (s >= T.nstrips) return 0xffffffffu;
+    const SgStrip G = strips[T.sched + s];
+    if (r < G.rlo || r > G.rhi) return 0xffffffffu;
+    const int32_t l = ((j - 1) % SW) / K;
+    return *(const uint32_t *)(tb + T.tb + G.tb + (int64_t)(r - G.rlo + l) * SW + l * K);
 }
 
-__global__ void k_sg_finish(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
-                            const SgTask *__restrict__ tasks, int32_t n, Params P, int32_t nw,
-                            const int32_t *__restrict__ scr, const uint8_t *__restrict__ tb, SgOut *__restrict__ out,
-                            uint32_t *__restrict__ cigar) {
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+// Ordering key of a row result for the first strict maximum: the larger
+// score, then the earlier row.
+__device__ __forceinline__ uint64_t best_key(int32_t rm, int32_t row) {
+    return ((uint64_t)((uint32_t)rm ^ 0x80000000u) << 32) | (uint64_t)(0xffffffffu - (uint32_t)row);
+}
+
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
+    const uint32_t lo = (uint32_t)__shfl_up((int32_t)(uint32_t)v, d), hi = (uint32_t)__shfl_up((int32_t)(v >> 32), d);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const uint32_t lo = (uint32_t)__shfl((int32_t)(uint32_t)v, src), hi = (uint32_t)__shfl((int32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t v) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int32_t)(uint32_t)v, d);
+        const uint32_t hi = (uint32_t)__shfl_xor((int32_t)((uint64_t)v >> 32), d);
+        v += (int64_t)(((uint64_t)hi << 32) | lo);
+    }
+    return v;
+}
+
+// One wave per task: the reference's row loop termination over the row
+// results (z-drop, empty row), 64 rows at a time, the maximum, and the
+// traceback (trace tasks) through 64 x 64 tiles of trace bytes loaded by the
+// whole wave at once.
+__global__ __launch_bounds__(64) void k_sg_finish(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                  const SgTask *__restrict__ tasks,
+                                                  const SgStrip *__restrict__ strips, int32_t n, Params P,
+                                                  const int32_t *__restrict__ scr, const uint8_t *__restrict__ tb,
+                                                  SgOut *__restrict__ out, uint32_t *__restrict__ cigar) {
+    __shared__ uint32_t tile[64 * 16];
+    const int32_t k = blockIdx.x;
     if (k >= n) return;
+    const int lane = threadIdx.x;
     const SgTask T = tasks[k];
     const PairDesc pd = pairs[T.pair];
     SgOut &o = out[k];
     const int32_t R = T.R, C = T.C, w = P.bandwidth;
-    const int32_t *rowres = scr + T.scr + (int64_t)(R + 2) * BND * (nw + 1);
-    int32_t best = 0, e1 = -1, e2 = -1, brk = R;
+    const int32_t *rowres = scr + T.scr;
+    const int32_t stale0 = o.stale[0], stale1 = o.stale[1];
+    // the best so far: key (score, row; row 0 = none) and its column
+    uint64_t ckey = best_key(0, 0);
+    int32_t ce2 = -1, brk = R;
     int64_t cells = 0;
-    for (int32_t i = 1; i <= R; ++i) {
-        const int32_t st = max(i - w, 1), en = min(i + w, C);
-        cells += (int64_t)max(0, en - st) + 1;
-        int32_t rm, rj;
-        if (i <= C + w) {
-            rm = rowres[2 * (int64_t)i];
-            rj = rowres[2 * (int64_t)i + 1];
-        } else {
-            // the band has left the matrix: only column C, whose diagonal is
-            // the value the row buffer of this parity last held at C-1
-            const int32_t rc = code_at(seq + pd.off1, T.a1, i - 1, T.rev);
-            const int32_t qc = code_at(seq + pd.off2, T.a2, C - 1, T.rev);
-            const int32_t sc = (rc == 2 || qc == 2) ? 0 : (rc == qc ? P.match : P.mismatch);
-            const int32_t dg = (C >= 2) ? o.stale[(i - (C + w + 1)) & 1] : NEG;
-            const int32_t m = sc + dg;
-            rm = m > 0 ? m : 0;
-            rj = m > 0 ? C : 0;
+    for (int32_t i0 = 1; i0 <= R; i0 += 64) {
+        const int32_t i = i0 + lane;
+        const bool live = i <= R;
+        int32_t rm = 0, rj = 0;
+        int64_t cl = 0;
+        if (live) {
+            const int32_t st = max(i - w, 1), en = min(i + w, C);
+            cl = (int64_t)max(0, en - st) + 1;
+            if (i <= C + w) {
+                rm = rowres[2 * (int64_t)i];
+                rj = rowres[2 * (int64_t)i + 1];
+            } else {
+                // the band has left the matrix: only column C, whose diagonal is
+                // the value the row buffer of this parity last held at C-1
+                const int32_t rc = code_at(seq + pd.off1, T.a1, i - 1, T.rev);
+                const int32_t qc = code_at(seq + pd.off2, T.a2, C - 1, T.rev);
+                const int32_t sc = (rc == 2 || qc == 2) ? 0 : (rc == qc ? P.match : P.mismatch);
+                const int32_t dg = (C >= 2) ? (((i - (C + w + 1)) & 1) ? stale1 : stale0) : NEG;
+                const int32_t m = sc + dg;
+                rm = m > 0 ? m : 0;
+                rj = m > 0 ? C : 0;
+            }
         }
-        if (rm > best) {
-            best = rm;
-            e1 = i;
-            e2 = rj;
+        // best over rows 1..i (inclusive prefix maximum across the wave)
+        uint64_t key = live ? best_key(rm, i) : 0;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t v = shfl_up64(key, d);
+            if (lane >= d && v > key) key = v;
         }
+        if (ckey > key) key = ckey;
+        const int32_t best = (int32_t)((uint32_t)(key >> 32) ^ 0x80000000u);
+        const int32_t row = (int32_t)(0xffffffffu - (uint32_t)key);
+        const int32_t rj_best = __shfl(rj, row >= i0 ? row - i0 : 0);
+        const int32_t e1 = row > 0 ? row : -1;
+        const int32_t e2 = row >= i0 ? rj_best : (row > 0 ? ce2 : -1);
         int32_t l = (i - e1) - (rj - e2);
         l = l > 0 ? l : -l;
-        if (best - rm > P.zdrop + l * P.ext2 || rj <= 0) {
-            brk = i;
+        const bool stop = live && (best - rm > P.zdrop + l * P.ext2 || rj <= 0);
+        const uint64_t hits = __ballot(stop);
+        const int last = hits ? __ffsll((unsigned long long)hits) - 1 : 63;
+        cells += wave_sum64(lane <= last ? cl : 0);
+        ckey = shfl64(key, last);
+        ce2 = __shfl(e2, last);
+        if (hits) {
+            brk = i0 + last;
             break;
         }
     }
-    o.max = best;
-    o.e1 = e1;
-    o.e2 = e2;
-    o.brk = brk;
-    o.cells = cells;
-    o.ncig = 0;
+    const int32_t brow = (int32_t)(0xffffffffu - (uint32_t)ckey);
+    const int32_t e1 = brow > 0 ? brow : -1, e2 = brow > 0 ? ce2 : -1;
+    if (lane == 0) {
+        o.max = (int32_t)((uint32_t)(ckey >> 32) ^ 0x80000000u);
+        o.e1 = e1;
+        o.e2 = e2;
+        o.brk = brk;
+        o.cells = cells;
+        o.ncig = 0;
+    }
     if (T.tb < 0) return;
     // traceback (impl/sequenceAlignment3.cpp:864-915); runs are written in
-    // traceback order and reversed by the host
+    // traceback order and reversed by the host.  Every lane walks the same
+    // path; lane 0 writes the runs.
     uint32_t *cg = cigar + o.cig_off;
-    int32_t nc = 0;
-    auto push = [&](uint32_t op) {
-        if (nc == 0 || op != (cg[nc - 1] & 0xfu))
-            cg[nc++] = (1u << 4) | op;
-        else
-            cg[nc - 1] += 1u << 4;
-    };
-    const int32_t nstrips = (C + SW - 1) / SW;
-    int32_t cs = nstrips;
-    int64_t cbase = 0;
-    for (int32_t s = 0; s < nstrips; ++s) {
-        const StripGeom g = strip_geom(s, R, C, w);
-        if (g.rlo > g.rhi) {
-            cs = s;
-            break;
+    int32_t nc = 0, run_op = -1, run_len = 0;
+    auto push = [&](int32_t op, int32_t len) {
+        if (op == run_op) {
+            run_len += len;
+            return;
         }
-        cbase += (int64_t)(g.rhi - g.rlo + 1 + 63) * SW;
-    }
+        if (run_op >= 0) {
+            if (lane == 0) cg[nc] = ((uint32_t)run_len << 4) | (uint32_t)run_op;
+            ++nc;
+        }
+        run_op = op;
+        run_len = len;
+    };
     int32_t ii = e1, jj = e2, state = 0;
     while (ii > 0 && jj > 0) {
-        const int32_t tmp = (int32_t)trace_byte(tb, T.tb, R, C, w, ii, jj, cs, cbase);
-        if (state == 0)
-            state = tmp & 7;
-        else if (!((tmp >> (state + 2)) & 1))
-            state = 0;
-        if (state == 0) state = tmp & 7;
-        if (state == 0) {
-            push(0);
-            --ii;
-            --jj;
-        } else if (state == 1 || state == 3) {
-            push(2);
-            --ii;
-        } else {
-            push(1);
-            --jj;
+        // the tile of rows ii-63..ii and the 16 four-column groups ending at jj's
+        const int32_t r0 = ii - 63, c0 = ((jj - 1) & ~3) + 1 - 60;
+        {
+            const int32_t r = r0 + lane;
+            for (int g = 0; g < 16; ++g) {
+                const int32_t j = c0 + 4 * g;
+                tile[lane * 16 + g] = (r >= 1 && j >= 1) ? trace_word(tb, T, strips, r, j) : 0xffffffffu;
+            }
         }
+        __syncthreads();
+        while (ii > 0 && jj > 0 && ii >= r0 && jj >= c0) {
+            const uint32_t wd = tile[(ii - r0) * 16 + ((jj - c0) >> 2)];
+            const int32_t tmp = (int32_t)((wd >> (8 * ((jj - c0) & 3))) & 0xffu);
+            if (state == 0)
+                state = tmp & 7;
+            else if (!((tmp >> (state + 2)) & 1))
+                state = 0;
+            if (state == 0) state = tmp & 7;
+            if (state == 0) {
+                push(0, 1);
+                --ii;
+                --jj;
+            } else if (state == 1 || state == 3) {
+                push(2, 1);
+                --ii;
+            } else {
+                push(1, 1);
+                --jj;
+            }
+        }
+        __syncthreads();
     }
-    while (ii > 0) {
-        push(2);
-        --ii;
+    if (ii > 0) push(2, ii);
+    if (jj > 0) push(1, jj);
+    push(-2, 0);  // flush the last run
+    if (lane == 0) o.ncig = nc;
+}
+
+// Strip geometry of one task: the strips the band reaches, with their trace
+// store offsets (boundary offsets are set by the caller).
+int32_t strip_list(int32_t R, int32_t C, int32_t w, std::vector<SgStrip> *out) {
+    int32_t n = 0;
+    int64_t tbo = 0;
+    for (int32_t s = 0; (int64_t)s * SW < C; ++s) {
+        const int32_t jb = s * SW + 1, je = std::min(jb + SW - 1, C);
+        const int32_t rlo = std::max(1, jb - w), rhi = std::min(std::min(R, je + w), C + w);
+        if (rlo > rhi) break;
+        if (out) {
+            SgStrip g{};
+            g.s = s;
+            g.rlo = rlo;
+            g.rhi = rhi;
+            g.tb = tbo;
+            out->push_back(g);
+        }
+        tbo += (int64_t)(rhi - rlo + 1 + 63) * SW;
+        ++n;
     }
-    while (jj > 0) {
-        push(1);
-        --jj;
-    }
-    o.ncig = nc;
+    return n;
 }
 
 }  // namespace
 
 int64_t sg_trace_bytes(int32_t R, int32_t C, int32_t w) {
-    int64_t total = 0;
-    for (int32_t s = 0; s * SW < C; ++s) {
-        const int32_t jb = s * SW + 1, je = std::min(jb + SW - 1, C);
-        const int32_t rlo = std::max(1, jb - w), rhi = std::min(std::min(R, je + w), C + w);
-        if (rlo > rhi) break;
-        total += (int64_t)(rhi - rlo + 1 + 63) * SW;
-    }
-    return total;
+    std::vector<SgStrip> v;
+    strip_list(R, C, w, &v);
+    return v.empty() ? 0 : v.back().tb + (int64_t)(v.back().rhi - v.back().rlo + 1 + 63) * SW;
 }
 
-int64_t sg_scratch_ints(int32_t R, int32_t nw) { return (int64_t)(R + 2) * ((nw + 1) * BND + 2); }
+int64_t sg_scratch_ints(int32_t R, int32_t C, int32_t w) {
+    std::vector<SgStrip> v;
+    strip_list(R, C, w, &v);
+    int64_t n = 2 * ((int64_t)R + 2);
+    for (const SgStrip &g : v) n += (int64_t)(g.rhi - g.rlo + 1) * BND;
+    return n;
+}
 
-// The strip schedule of one task for nw waves (see k_sg_dp).  A strip may
-// start once (a) its predecessor has run floor((126 + d) / 64) + 1 phases,
-// d = the rise of the first row between the two strips, so that the rows its
-// first lane loads at the start of any phase (64 rows ahead, first-lane rows
-// lagging the producer's last lane by 63 steps) were all written in earlier
-// phases, and (b) its wave has finished the strip nw places before it.
-int32_t sg_schedule(int32_t R, int32_t C, int32_t w, int32_t nw, std::vector<SgStrip> &out) {
-    out.clear();
-    int32_t phases = 0;
-    std::vector<int32_t> len;
-    for (int32_t s = 0; (int64_t)s * SW < C; ++s) {
-        const int32_t jb = s * SW + 1, je = std::min(jb + SW - 1, C);
-        const int32_t rlo = std::max(1, jb - w), rhi = std::min(std::min(R, je + w), C + w);
-        if (rlo > rhi) break;
-        SgStrip g{};
-        g.rlo = rlo;
-        g.rhi = rhi;
-        g.tb = s == 0 ? 0 : out.back().tb + (int64_t)(out.back().rhi - out.back().rlo + 1 + 63) * SW;
-        const int32_t p = (rhi - rlo + 1 + 63 + 63) / 64;
-        int32_t start = 0;
-        if (s > 0) {
-            const int32_t d = std::max(0, rlo - out[(size_t)s - 1].rlo);
-            start = out[(size_t)s - 1].start + (126 + d) / 64 + 1;
-        }
-        if (s >= nw) start = std::max(start, out[(size_t)(s - nw)].start + len[(size_t)(s - nw)]);
-        g.start = start;
-        out.push_back(g);
-        len.push_back(p);
-        phases = std::max(phases, start + p);
+int32_t sg_task_strips(int32_t R, int32_t C, int32_t w, int32_t task, int64_t bnd, std::vector<SgStrip> &out) {
+    const size_t first = out.size();
+    const int32_t n = strip_list(R, C, w, &out);
+    for (size_t i = first; i < out.size(); ++i) {
+        out[i].task = task;
+        out[i].bnd = bnd;
+        bnd += (int64_t)(out[i].rhi - out[i].rlo + 1) * BND;
     }
-    return phases;
+    return n;
 }
 
 hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
-                        const SgStrip *strips, int32_t n, int32_t nw, const Params &P, int32_t *scr, uint8_t *tb,
-                        SgOut *out) {
-    if (n <= 0) return hipSuccess;
-    if (nw < 1 || nw > 16) return hipErrorInvalidValue;
+                        const SgStrip *strips, int32_t nstrips, const Params &P, int32_t *scr, uint8_t *tb,
+                        SgOut *out, int32_t *sync) {
+    if (nstrips <= 0) return hipSuccess;
     if (trace)
-        hipLaunchKernelGGL(k_sg_dp<true>, dim3(n), dim3(64 * nw), 0, st, seq, pairs, tasks, strips, P, nw, scr, tb,
-                           out);
+        hipLaunchKernelGGL(k_sg_strip<true>, dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, nstrips, P,
+                           scr, tb, out, sync);
     else
-        hipLaunchKernelGGL(k_sg_dp<false>, dim3(n), dim3(64 * nw), 0, st, seq, pairs, tasks, strips, P, nw, scr, tb,
-                           out);
+        hipLaunchKernelGGL(k_sg_strip<false>, dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, nstrips, P,
+                           scr, tb, out, sync);
     return hipGetLastError();
 }
 
-hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks, int32_t n,
-                            int32_t nw, const Params &P, const int32_t *scr, const uint8_t *tb, SgOut *out,
-                            uint32_t *cigar) {
+hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
+                            const SgStrip *strips, int32_t n, const Params &P, const int32_t *scr, const uint8_t *tb,
+                            SgOut *out, uint32_t *cigar) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sg_finish, dim3((n + 63) / 64), dim3(64), 0, st, seq, pairs, tasks, n, P, nw, scr, tb, out,
-                       cigar);
+    hipLaunchKernelGGL(k_sg_finish, dim3(n), dim3(64), 0, st, seq, pairs, tasks, strips, n, P, scr, tb, out, cigar);
     return hipGetLastError();
 }
 

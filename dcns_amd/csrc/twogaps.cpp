@@ -48,7 +48,7 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
     for (int32_t i = 0; i < n; ++i)
         if (tasks[(size_t)i].R > 0 && tasks[(size_t)i].C > 0) idx.push_back(i);
     if (idx.empty()) return;
-    // largest first: one wave per task, so the long tasks start early
+    // largest first, so the strips of the long tasks start early
     std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
         return (int64_t)tasks[(size_t)a].R * tasks[(size_t)a].C > (int64_t)tasks[(size_t)b].R * tasks[(size_t)b].C;
     });
@@ -66,7 +66,7 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
         while (pos < idx.size()) {
             SgTask t = tasks[(size_t)idx[pos]];
             const int64_t tbb = trace ? (sg_trace_bytes(t.R, t.C, P.bandwidth) + 255) / 256 * 256 : 0;
-            const int64_t scr = sg_scratch_ints(t.R, 16);
+            const int64_t scr = sg_scratch_ints(t.R, t.C, P.bandwidth);
             if (!ct.empty() && tb_total + tbb + 4 * (scr_total + scr) > budget) break;
             if (ct.empty() && tbb + 4 * scr > budget)
                 throw Error(DCNS_ERR_CAPACITY, "a SemiGlobal task's traceback store exceeds device memory");
@@ -84,19 +84,20 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
             ++pos;
         }
         const int32_t m = (int32_t)ct.size();
-        // waves per task: one per strip up to 16, the same for the whole launch
-        std::vector<SgStrip> strips, one;
-        int32_t nw = 1;
-        for (const SgTask &t : ct) nw = std::max(nw, std::min<int32_t>(16, (t.C + 255) / 256));
-        for (SgTask &t : ct) {
-            t.phases = sg_schedule(t.R, t.C, P.bandwidth, nw, one);
-            t.nstrips = (int32_t)one.size();
+        // the strips of every task, a task's strips consecutive: each strip's
+        // boundary records follow its task's row results in the scratch
+        std::vector<SgStrip> strips;
+        for (int32_t k = 0; k < m; ++k) {
+            SgTask &t = ct[(size_t)k];
             t.sched = (int64_t)strips.size();
-            strips.insert(strips.end(), one.begin(), one.end());
+            t.nstrips = sg_task_strips(t.R, t.C, P.bandwidth, k, t.scr + 2 * ((int64_t)t.R + 2), strips);
         }
+        const int32_t ns = (int32_t)strips.size();
         c->sg_strips.ensure(std::max<size_t>(strips.size(), 1) * sizeof(SgStrip));
         hip_check(hipMemcpyAsync(c->sg_strips.p, strips.data(), strips.size() * sizeof(SgStrip), hipMemcpyHostToDevice,
                                  c->stream), "sg strips");
+        c->sg_sync.ensure((size_t)(ns + 2) * 4);
+        hip_check(hipMemsetAsync(c->sg_sync.p, 0, (size_t)(ns + 2) * 4, c->stream), "sg sync");
         c->sg_tasks.ensure((size_t)m * sizeof(SgTask));
         c->sg_out.ensure((size_t)m * sizeof(SgOut));
         c->sg_scr.ensure((size_t)std::max<int64_t>(scr_total, 16) * 4);
@@ -110,14 +111,19 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
                   "sg out");
         c->launch("semiglobal", [&] {
             return launch_sg_dp(c->stream, trace, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
-                                c->sg_strips.as<SgStrip>(), m, nw, P, c->sg_scr.as<int32_t>(),
-                                trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>());
+                                c->sg_strips.as<SgStrip>(), ns, P, c->sg_scr.as<int32_t>(),
+                                trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>(),
+                                c->sg_sync.as<int32_t>());
         });
         c->launch("semiglobal_finish", [&] {
             return launch_sg_finish(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
-                                    m, nw, P, c->sg_scr.as<int32_t>(), trace ? c->sg_tb.as<uint8_t>() : nullptr,
-                                    c->sg_out.as<SgOut>(), trace ? c->sg_cig.as<uint32_t>() : nullptr);
+                                    c->sg_strips.as<SgStrip>(), m, P, c->sg_scr.as<int32_t>(),
+                                    trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>(),
+                                    trace ? c->sg_cig.as<uint32_t>() : nullptr);
         });
+        int32_t stalled = 0;
+        hip_check(hipMemcpyAsync(&stalled, c->sg_sync.as<int32_t>() + 1, 4, hipMemcpyDeviceToHost, c->stream),
+                  "sg error word");
         hip_check(hipMemcpyAsync(so.data(), c->sg_out.p, (size_t)m * sizeof(SgOut), hipMemcpyDeviceToHost, c->stream),
                   "sg out");
         std::vector<uint32_t> cig;
@@ -127,6 +133,7 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
                       "sg cigar");
         }
         c->sync();
+        if (stalled) throw Error(DCNS_ERR_INTERNAL, "SemiGlobal strip pipeline: a strip waited too long for its predecessor");
         for (int32_t k = 0; k < m; ++k) {
             const SgOut &o = so[(size_t)k];
             SgResult &r = out[(size_t)which[(size_t)k]];
