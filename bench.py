@@ -190,7 +190,7 @@ def lambdak_phase(ctx, log):
     return out
 
 
-def two_piece_phase(ctx, log):
+def two_piece_phase(ctx, log, cpu=True):
     """The two-piece path (findSimilarFragmentsForPairedSequence overload 4,
     cut2Gaps) on the 10 kb x 10 kb planted pair, beside the compiled
     reference's overload 4 on the same pair (one core), results compared."""
@@ -210,7 +210,7 @@ def two_piece_phase(ctx, log):
            "reference_cells": cells, "semiglobal_cells": st["semiglobal_cells"], "seconds": round(dt, 4),
            "gcups": round(cells / dt / 1e9, 2), "semiglobal_ms": round(st["ms_semiglobal"], 1),
            "fragments": len(res[0][1])}
-    ref = bind.reference()
+    ref = bind.reference() if cpu else None
     if ref is not None:
         t = time.perf_counter()
         want = ref.find_similar_2gaps(a, b)
@@ -219,6 +219,24 @@ def two_piece_phase(ctx, log):
         out["bit_exact"] = [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in res[0][1]] == \
                            [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in want]
     log(f"two-piece: {cells:.3e} cells in {dt * 1e3:.1f} ms ({cells / dt / 1e9:.1f} GCUPS)")
+    # throughput: 64 pairs of the batch distribution in one overload-4 call
+    pairs = synth.batch(npairs=64, seed=123)
+    ctx.find_similar_2gaps(pairs)
+    ctx.kernel_time_reset()
+    t = time.perf_counter()
+    ctx.find_similar_2gaps(pairs)
+    dt = time.perf_counter() - t
+    st = ctx.stats()
+    dp_ms, _ = ctx.kernel_time("semiglobal")
+    fin_ms, _ = ctx.kernel_time("semiglobal_finish")
+    cells = st["scan_cells"] + st["xdrop_cells"] + st["semiglobal_cells"]
+    out["batch64"] = {"workload": "64 pairs, lengths log-uniform 1-50 kb, cut2Gaps defaults",
+                      "reference_cells": cells, "semiglobal_cells": st["semiglobal_cells"], "seconds": round(dt, 4),
+                      "gcups": round(cells / dt / 1e9, 2), "k_sg_strip_ms": round(dp_ms, 2),
+                      "k_sg_finish_ms": round(fin_ms, 2),
+                      "k_sg_strip_gcups": round(st["semiglobal_cells"] / dp_ms / 1e6, 1)}
+    log(f"two-piece batch of 64: {cells:.3e} cells in {dt * 1e3:.1f} ms; K4 strips "
+        f"{st['semiglobal_cells'] / dp_ms / 1e6:.1f} GCUPS")
     return out
 
 
@@ -279,6 +297,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-lambdak", action="store_true")
     ap.add_argument("--no-two-piece", action="store_true")
+    ap.add_argument("--two-piece-only", action="store_true",
+                    help="run only the two-piece (cut2Gaps) measurements and print them")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs shard across GPUs as "
@@ -287,6 +307,13 @@ def main():
 
     import dcns_amd
     from dcns_amd import synth
+
+    if args.two_piece_only:
+        ctx = dcns_amd.Context(0, timing=True)
+        out = two_piece_phase(ctx, lambda m: print(m, file=sys.stderr, flush=True), cpu=not args.no_cpu)
+        ctx.close()
+        print(json.dumps({"two_piece": out}), flush=True)
+        return
 
     pairs = synth.batch(npairs=args.pairs)
     len1 = np.array([len(a) for a, _ in pairs])

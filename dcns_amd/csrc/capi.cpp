@@ -325,9 +325,9 @@ int dcns_xdrop_extend(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, co
             q[4] = r.end2;
             q[5] = r.ok ? 1 : 0;
             cig_off[i] = o;
-            if (o + (int64_t)r.cigar.size() > cig_cap) throw Error(DCNS_ERR_CAPACITY, "cigar capacity");
-            if (!r.cigar.empty()) std::memcpy(cigar + o, r.cigar.data(), r.cigar.size() * 4);
-            o += (int64_t)r.cigar.size();
+            if (o + r.ncig > cig_cap) throw Error(DCNS_ERR_CAPACITY, "cigar capacity");
+            if (r.ncig > 0) std::memcpy(cigar + o, r.cig, (size_t)r.ncig * 4);
+            o += r.ncig;
         }
         cig_off[n] = o;
         c->stats = dcns_stats{};

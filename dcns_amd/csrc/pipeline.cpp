@@ -563,27 +563,34 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
     hip_check(hipMemcpyAsync(c->xout.p, xo, (size_t)n * sizeof(XOut), hipMemcpyHostToDevice, c->stream), "xout up");
     upload(c, c->tb_off, tboff);
     // pass 2 order: by band class (each instantiation's tasks contiguous),
-    // then by traced cells, largest first
+    // then by traced cells, largest first, on a quarter-octave scale (a
+    // counting sort; the order only balances the launches)
     {
-        std::vector<int64_t> key((size_t)n);
+        constexpr int NB = 4 * 48;
+        std::vector<int32_t> bucket((size_t)n);
+        std::vector<int64_t> start(5 * NB + 2, 0);
         for (int32_t i = 0; i < n; ++i) {
             const XOut &o = xo[i];
-            if (o.fe1 < 0 || o.fe2 < 0) {
-                key[(size_t)i] = INT64_MIN;
-                continue;
+            int32_t b = 5 * NB;  // no traceback: last
+            if (o.fe1 >= 0 && o.fe2 >= 0) {
+                const int32_t B = 2 * o.rw + 1;
+                int cls = 4;
+                for (int k = 0; k < 4; ++k)
+                    if (B <= bands[k]) {
+                        cls = k;
+                        break;
+                    }
+                const uint64_t cl = (uint64_t)(o.rrows + 1) * (uint64_t)B;
+                const int lg = 63 - __builtin_clzll(cl | 1);
+                const int q = lg >= 2 ? (int)((cl >> (lg - 2)) & 3) : 0;
+                b = cls * NB + (NB - 1 - std::min(NB - 1, 4 * lg + q));
             }
-            const int32_t B = 2 * o.rw + 1;
-            int cls = 4;
-            for (int k = 0; k < 4; ++k)
-                if (B <= bands[k]) {
-                    cls = k;
-                    break;
-                }
-            key[(size_t)i] = ((int64_t)(4 - cls) << 48) + (int64_t)(o.rrows + 1) * B;
+            bucket[(size_t)i] = b;
+            ++start[(size_t)b + 1];
         }
+        for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
         std::vector<int32_t> order2((size_t)n);
-        for (int32_t i = 0; i < n; ++i) order2[(size_t)i] = i;
-        std::stable_sort(order2.begin(), order2.end(), [&](int32_t a, int32_t b) { return key[(size_t)a] > key[(size_t)b]; });
+        for (int32_t i = 0; i < n; ++i) order2[(size_t)start[(size_t)bucket[(size_t)i]]++] = i;
         upload(c, c->xorder, order2);
     }
     c->tb.ensure((size_t)std::max<int64_t>(tbtotal, 64));
@@ -624,7 +631,8 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
             return;
         }
         r.score = o.rmax;
-        r.cigar.assign(cig + o.cig_off, cig + o.cig_off + o.ncig);
+        r.cig = cig + o.cig_off;
+        r.ncig = o.ncig;
         if (o.re1 < 0 || o.re2 < 0) {
             r.ok = false;
             return;
@@ -715,7 +723,8 @@ void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &
             if (!(pv < p.pvalue)) continue;
             // dedup, :500-521: 1-based uint32 stored coordinates against 0-based int32 ones
             PairedSimilarFragment nf((uint32_t)(r.start1 + 1), (uint32_t)(r.end1 + 1), (uint32_t)(r.start2 + 1),
-                                     (uint32_t)(r.end2 + 1), (uint32_t)r.score, r.cigar, pv, e);
+                                     (uint32_t)(r.end2 + 1), (uint32_t)r.score,
+                                     std::vector<uint32_t>(r.cig, r.cig + r.ncig), pv, e);
             // the reference scans the fragments in order and stops at the first
             // one with the same start or the same end; the index finds that one
             const int32_t g_at = idx.first_match(nf);

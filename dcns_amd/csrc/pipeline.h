@@ -45,7 +45,10 @@ void link_seeds(std::set<Seed> &seeds);
 struct XResult {
     int32_t score, start1, end1, start2, end2;
     bool ok;  // false: the reference's p = 100 case
-    std::vector<uint32_t> cigar;
+    // the cigar, a view into the context's host cigar buffer (valid until the
+    // context's next x-drop call)
+    const uint32_t *cig;
+    int32_t ncig;
 };
 // score_only: the variant without traceback (:205-241): untrimmed coordinates, no cigar.
 void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vector<XTask> &tasks,

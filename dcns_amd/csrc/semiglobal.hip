@@ -327,18 +327,6 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
     }
 }
 
-// Trace word (4 columns j..j+3, j = 1 mod 4) of row r, 0xffffffff outside
-// the band's strips and rows.
-__device__ __forceinline__ uint32_t trace_word(const uint8_t *tb, const SgTask &T, const SgStrip *strips, int32_t r,
-                                               int32_t j) {
-    const int32_t s = (j - 1) / SW;
-    if (s >= T.nstrips) return 0xffffffffu;
-    const SgStrip G = strips[T.sched + s];
-    if (r < G.rlo || r > G.rhi) return 0xffffffffu;
-    const int32_t l = ((j - 1) % SW) / K;
-    return *(const uint32_t *)(tb + T.tb + G.tb + (int64_t)(r - G.rlo + l) * SW + l * K);
-}
-
 // Ordering key of a row result for the first strict maximum: the larger
 // score, then the earlier row.
 __device__ __forceinline__ uint64_t best_key(int32_t rm, int32_t row) {
@@ -468,11 +456,25 @@ __global__ __launch_bounds__(64) void k_sg_finish(const int8_t *__restrict__ seq
         // the tile of rows ii-63..ii and the 16 four-column groups ending at jj's
         const int32_t r0 = ii - 63, c0 = ((jj - 1) & ~3) + 1 - 60;
         {
+            // 64 columns span at most two strips: those of max(c0, 1) and of jj
+            const int32_t sb = (jj - 1) / SW, sa = (max(c0, 1) - 1) / SW;
+            const SgStrip A = strips[T.sched + sa], B = strips[T.sched + sb];
             const int32_t r = r0 + lane;
-            for (int g = 0; g < 16; ++g) {
+            uint32_t wv[16];
+#pragma unroll
+            for (int g = 0; g < 16; ++g) {  // 16 independent loads, one round trip
                 const int32_t j = c0 + 4 * g;
-                tile[lane * 16 + g] = (r >= 1 && j >= 1) ? trace_word(tb, T, strips, r, j) : 0xffffffffu;
+                const bool inb = j >= 1 && (j - 1) / SW == sb;
+                const int32_t rlo = inb ? B.rlo : A.rlo, rhi = inb ? B.rhi : A.rhi;
+                const int64_t sto = inb ? B.tb : A.tb;
+                const int32_t l = ((j - 1) & (SW - 1)) / K;
+                const bool ok = j >= 1 && r >= 1 && r >= rlo && r <= rhi;
+                const int64_t off = ok ? T.tb + sto + (int64_t)(r - rlo + l) * SW + l * K : T.tb;
+                wv[g] = *(const uint32_t *)(tb + off);
+                wv[g] = ok ? wv[g] : 0xffffffffu;
             }
+#pragma unroll
+            for (int g = 0; g < 16; ++g) tile[lane * 16 + g] = wv[g];
         }
         __syncthreads();
         while (ii > 0 && jj > 0 && ii >= r0 && jj >= c0) {
