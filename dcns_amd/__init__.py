@@ -79,6 +79,12 @@ def load_library():
     lib.dcns_find_similar_2gaps_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), ctypes.POINTER(_P)]
     lib.dcns_semiglobal_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _P, _I, _I, _P,
                                           _P, _P, _L]
+    lib.dcns_read_score_folder.argtypes = [ctypes.c_char_p, _P, _I, ctypes.POINTER(_I)]
+    lib.dcns_gff_categories.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _L, _L, _L, _P]
+    lib.dcns_find_similar_weighted_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _I, _P,
+                                                     _P, _I, ctypes.POINTER(_P)]
+    lib.dcns_semiglobal_weighted_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _I, _P, _P,
+                                                   _I, _P, _P, _P, _I, _I, _P, _P, _P, _L]
     lib.dcns_calculate_lambda.restype = _D
     lib.dcns_calculate_lambda.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, _I, _I, _D, _D]
     lib.dcns_fit_lambda_k_batch.argtypes = [_P, _L, _I, _P, _D, _D, _I, _P]
@@ -107,7 +113,9 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_mlp_set_scaling", "dcns_mlp_get_params", "dcns_mlp_set_params", "dcns_mlp_predict",
            "dcns_mlp_loss_grad", "dcns_mlp_train", "dcns_find_similar_2gaps_batch", "dcns_semiglobal_batch",
            "dcns_cut2gaps_main", "dcns_paircns2gaps_main", "dcns_paircnsxextend_main", "dcns_sw_random_max",
-           "dcns_fit_lambda_k_batch", "dcns_calculate_lambda"]
+           "dcns_fit_lambda_k_batch", "dcns_calculate_lambda", "dcns_read_score_folder", "dcns_gff_categories",
+           "dcns_find_similar_weighted_batch", "dcns_semiglobal_weighted_batch", "dcns_weighted1gap_main",
+           "dcns_weighted2gaps_main"]
 
 
 def default_params(**kw):
@@ -155,6 +163,54 @@ def fit_lambda_k_batch(scores, space, k0=0.3, lambda0=0.2, maxiter=550):
     if rc != 0:
         raise DcnsError(f"dcns_fit_lambda_k_batch failed ({rc})")
     return out
+
+
+# -- weighted (ZDP) inputs ----------------------------------------------------
+# profiles: {category: (match, mismatch, open1, ext1, open2, ext2, zdrop)}
+
+
+def read_score_folder(folder):
+    """Score::Score(folder): {category: 7 values} in matrixList order
+    (dcns_read_score_folder; no device needed)."""
+    lib = load_library()
+    out = np.zeros(8 * 256, np.int32)
+    n = _I()
+    rc = lib.dcns_read_score_folder(folder.encode(), out.ctypes.data, 256, ctypes.byref(n))
+    if rc != 0:
+        raise DcnsError(f"dcns_read_score_folder failed ({rc}): {lib.dcns_last_error(None).decode()}")
+    rows = out[:8 * n.value].reshape(-1, 8)
+    return {int(r[0]): tuple(int(x) for x in r[1:]) for r in rows}
+
+
+def gff_categories(gff_path, chrom, chrom_size, start, count):
+    """readGffFileWithEveryThing's categories of chrom at [start, start + count)
+    (dcns_gff_categories; no device needed)."""
+    lib = load_library()
+    out = np.zeros(max(int(count), 1), np.int16)
+    rc = lib.dcns_gff_categories(gff_path.encode(), chrom.encode(), int(chrom_size), int(start), int(count),
+                                 out.ctypes.data)
+    if rc != 0:
+        raise DcnsError(f"dcns_gff_categories failed ({rc}): {lib.dcns_last_error(None).decode()}")
+    return out[:int(count)]
+
+
+def _profile_array(profiles):
+    return np.array([[int(c), *[int(x) for x in v]] for c, v in profiles.items()], np.int32).reshape(-1)
+
+
+def _category_arrays(pairs, wcats):
+    """Concatenates per-pair category arrays (len(seq1) + 2 values each)."""
+    off = np.zeros(len(pairs), np.int64)
+    parts, o = [], 0
+    for i, ((a, _), w) in enumerate(zip(pairs, wcats)):
+        w = np.ascontiguousarray(w, np.int16)
+        if w.size != len(a) + 2:
+            raise ValueError("categories must cover reference positions -1 .. len1 (len1 + 2 values)")
+        off[i] = o
+        parts.append(w)
+        o += w.size
+    cat = np.concatenate(parts) if parts else np.zeros(1, np.int16)
+    return cat, off
 
 
 class _BatchArrays:
@@ -351,6 +407,47 @@ class Context:
                                                    an.ctypes.data, None if rv is None else rv.ctypes.data, n,
                                                    1 if with_cigar else 0, out.ctypes.data, cig.ctypes.data,
                                                    off.ctypes.data, cap), "dcns_semiglobal_batch")
+        res = out[:3 * n].reshape(n, 3)
+        if not with_cigar:
+            return res, None
+        return res, [[int(c) for c in cig[off[i]:off[i + 1]]] for i in range(n)]
+
+    def find_similar_weighted(self, pairs, wcats, profiles, two_piece=True, params=None):
+        """findSimilarFragmentsForPairedSequence_wighted (two_piece) / _wighted_1gap
+        per pair: [(seeds_size, [(start1, end1, start2, end2, score, p, e, cigar), ...]), ...].
+        wcats[i]: categories of pair i's reference positions -1 .. len1."""
+        ba = _BatchArrays(pairs)
+        p = params or default_params()
+        pr = _profile_array(profiles)
+        cat, off = _category_arrays(pairs, wcats)
+        fs = _P()
+        self._check(self.lib.dcns_find_similar_weighted_batch(self.h, ctypes.byref(ba.b), ctypes.byref(p),
+                                                              pr.ctypes.data, len(profiles), cat.ctypes.data,
+                                                              off.ctypes.data, int(bool(two_piece)),
+                                                              ctypes.byref(fs)), "dcns_find_similar_weighted_batch")
+        return self._collect(fs, ba.b.n, True)
+
+    def semiglobal_weighted(self, pairs, wcats, profiles, pair_of_task, anchors, reverse=None, two_piece=True,
+                            params=None, with_cigar=True):
+        """The weighted SemiGlobal for many tasks: (out3 [n x 3], cigars or None)."""
+        ba = _BatchArrays(pairs)
+        p = params or default_params()
+        pr = _profile_array(profiles)
+        cat, coff = _category_arrays(pairs, wcats)
+        pt = np.ascontiguousarray(pair_of_task, np.int32)
+        an = np.ascontiguousarray(anchors, np.int32).reshape(-1)
+        n = len(pt)
+        rv = None if reverse is None else np.ascontiguousarray(reverse, np.int32)
+        out = np.zeros(3 * max(n, 1), np.int32)
+        lens = [len(a) + len(b) + 2 for a, b in pairs]
+        cap = max(int(sum(lens[int(i)] for i in pt)), 16) if with_cigar else 16
+        cig = np.zeros(cap, np.uint32)
+        off = np.zeros(n + 1, np.int64)
+        self._check(self.lib.dcns_semiglobal_weighted_batch(
+            self.h, ctypes.byref(ba.b), ctypes.byref(p), pr.ctypes.data, len(profiles), cat.ctypes.data,
+            coff.ctypes.data, int(bool(two_piece)), pt.ctypes.data, an.ctypes.data,
+            None if rv is None else rv.ctypes.data, n, 1 if with_cigar else 0, out.ctypes.data, cig.ctypes.data,
+            off.ctypes.data, cap), "dcns_semiglobal_weighted_batch")
         res = out[:3 * n].reshape(n, 3)
         if not with_cigar:
             return res, None

@@ -6,6 +6,7 @@
 
 #include "../../include/dcns.h"
 #include "ctx.h"
+#include "host/weights.h"
 #include "kernels.h"
 #include "pipeline.h"
 
@@ -413,6 +414,74 @@ int dcns_find_similar_2gaps_batch(dcns_ctx *c, const dcns_batch *b, const dcns_p
     return DCNS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+std::vector<Profile> profiles_of(const dcns_profile *prof, int32_t nprof) {
+    if (!prof || nprof < 1) throw Error(DCNS_ERR_ARG, "score profiles are required");
+    std::vector<Profile> v((size_t)nprof);
+    for (int32_t i = 0; i < nprof; ++i) {
+        const dcns_profile &q = prof[i];
+        if (q.category < INT16_MIN || q.category > INT16_MAX) throw Error(DCNS_ERR_ARG, "category outside int16");
+        v[(size_t)i] = Profile{(int16_t)q.category, q.match, q.mismatch, q.open1, q.ext1, q.open2, q.ext2, q.zdrop};
+    }
+    return v;
+}
+
+// Task-level SemiGlobal (unweighted when W is null) on a staged batch.
+void semiglobal_tasks(dcns_ctx *c, const dcns_batch *b, const Params &P, const SgWeights *W,
+                      const int32_t *pair_of_task, const int32_t *anchors, const int32_t *reverse, int32_t n,
+                      int32_t with_cigar, int32_t *out3, uint32_t *cigar, int64_t *cig_off, int64_t cig_cap) {
+    if (n < 0 || (n > 0 && (!pair_of_task || !anchors || !out3)) || (with_cigar && n > 0 && !cig_off))
+        throw Error(DCNS_ERR_ARG, "invalid task arrays");
+    std::vector<SgTask> tasks((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t pr = pair_of_task[i];
+        if (pr < 0 || pr >= b->n) throw Error(DCNS_ERR_ARG, "pair index out of range");
+        const int32_t a1 = anchors[2 * i], a2 = anchors[2 * i + 1], L1 = b->len1[pr], L2 = b->len2[pr];
+        const bool rev = reverse && reverse[i];
+        SgTask t{};
+        t.pair = pr;
+        t.rev = rev ? 1 : 0;
+        t.a1 = a1;
+        t.a2 = a2;
+        if (rev) {
+            if (a1 < 0 || a1 >= L1 || a2 < 0 || a2 >= L2) throw Error(DCNS_ERR_ARG, "reverse anchor outside its sequences");
+            t.R = a1 + 1;
+            t.C = a2 + 1;
+        } else {
+            if (a1 < 0 || a1 > L1 || a2 < 0 || a2 > L2) throw Error(DCNS_ERR_ARG, "anchor outside its sequences");
+            t.R = L1 - a1;
+            t.C = L2 - a2;
+        }
+        tasks[(size_t)i] = t;
+    }
+    std::vector<SgResult> res;
+    int64_t cells = 0;
+    semiglobal_batch(c, P, tasks, with_cigar != 0, res, cells, W);
+    int64_t o = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const SgResult &r = res[(size_t)i];
+        out3[3 * i] = r.max;
+        out3[3 * i + 1] = r.end1;
+        out3[3 * i + 2] = r.end2;
+        if (with_cigar) {
+            cig_off[i] = o;
+            if (o + (int64_t)r.cigar.size() > cig_cap) throw Error(DCNS_ERR_CAPACITY, "cigar capacity");
+            if (!r.cigar.empty()) std::memcpy(cigar + o, r.cigar.data(), r.cigar.size() * 4);
+            o += (int64_t)r.cigar.size();
+        }
+    }
+    if (with_cigar) cig_off[n] = o;
+    c->stats = dcns_stats{};
+    c->stats.semiglobal_cells = cells;
+}
+
+}  // namespace
+
+extern "C" {
+
 int dcns_semiglobal_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, const int32_t *pair_of_task,
                           const int32_t *anchors, const int32_t *reverse, int32_t n, int32_t with_cigar,
                           int32_t *out3, uint32_t *cigar, int64_t *cig_off, int64_t cig_cap) {
@@ -423,53 +492,78 @@ int dcns_semiglobal_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params *p
         validate_params(*p);
         if (p->open2 > 0 || p->ext2 > 0 || p->zdrop < 0 || p->bandwidth < 2)
             throw Error(DCNS_ERR_ARG, "open2/ext2 must be <= 0, zdrop >= 0, bandwidth >= 2");
-        if (n < 0 || (n > 0 && (!pair_of_task || !anchors || !out3)) || (with_cigar && n > 0 && !cig_off))
-            throw Error(DCNS_ERR_ARG, "invalid task arrays");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         Params P = to_params(*p);
         c->staged.reset();
         Staged S = stage_batch(c, *b, P);
-        std::vector<SgTask> tasks((size_t)n);
-        for (int32_t i = 0; i < n; ++i) {
-            const int32_t pr = pair_of_task[i];
-            if (pr < 0 || pr >= b->n) throw Error(DCNS_ERR_ARG, "pair index out of range");
-            const int32_t a1 = anchors[2 * i], a2 = anchors[2 * i + 1], L1 = b->len1[pr], L2 = b->len2[pr];
-            const bool rev = reverse && reverse[i];
-            SgTask t{};
-            t.pair = pr;
-            t.rev = rev ? 1 : 0;
-            t.a1 = a1;
-            t.a2 = a2;
-            if (rev) {
-                if (a1 < 0 || a1 >= L1 || a2 < 0 || a2 >= L2) throw Error(DCNS_ERR_ARG, "reverse anchor outside its sequences");
-                t.R = a1 + 1;
-                t.C = a2 + 1;
-            } else {
-                if (a1 < 0 || a1 > L1 || a2 < 0 || a2 > L2) throw Error(DCNS_ERR_ARG, "anchor outside its sequences");
-                t.R = L1 - a1;
-                t.C = L2 - a2;
-            }
-            tasks[(size_t)i] = t;
-        }
-        std::vector<SgResult> res;
-        int64_t cells = 0;
-        semiglobal_batch(c, P, tasks, with_cigar != 0, res, cells);
-        int64_t o = 0;
-        for (int32_t i = 0; i < n; ++i) {
-            const SgResult &r = res[(size_t)i];
-            out3[3 * i] = r.max;
-            out3[3 * i + 1] = r.end1;
-            out3[3 * i + 2] = r.end2;
-            if (with_cigar) {
-                cig_off[i] = o;
-                if (o + (int64_t)r.cigar.size() > cig_cap) throw Error(DCNS_ERR_CAPACITY, "cigar capacity");
-                if (!r.cigar.empty()) std::memcpy(cigar + o, r.cigar.data(), r.cigar.size() * 4);
-                o += (int64_t)r.cigar.size();
-            }
-        }
-        if (with_cigar) cig_off[n] = o;
-        c->stats = dcns_stats{};
-        c->stats.semiglobal_cells = cells;
+        semiglobal_tasks(c, b, P, nullptr, pair_of_task, anchors, reverse, n, with_cigar, out3, cigar, cig_off,
+                         cig_cap);
+    });
+}
+
+int dcns_semiglobal_weighted_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, const dcns_profile *prof,
+                                   int32_t nprof, const int16_t *cat, const int64_t *cat_off, int32_t two_piece,
+                                   const int32_t *pair_of_task, const int32_t *anchors, const int32_t *reverse,
+                                   int32_t n, int32_t with_cigar, int32_t *out3, uint32_t *cigar, int64_t *cig_off,
+                                   int64_t cig_cap) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        check_batch(b);
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        validate_params(*p);
+        if (p->bandwidth < 1) throw Error(DCNS_ERR_ARG, "bandwidth must be >= 1");
+        const std::vector<Profile> pv = profiles_of(prof, nprof);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        Params P = to_params(*p);
+        c->staged.reset();
+        Staged S = stage_batch(c, *b, P);
+        const SgWeights W = stage_weights(c, *b, pv, PairCategories{cat, cat_off}, two_piece ? 1 : 2);
+        semiglobal_tasks(c, b, P, &W, pair_of_task, anchors, reverse, n, with_cigar, out3, cigar, cig_off, cig_cap);
+    });
+}
+
+int dcns_find_similar_weighted_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params *p,
+                                     const dcns_profile *prof, int32_t nprof, const int16_t *cat,
+                                     const int64_t *cat_off, int32_t two_piece, dcns_frag_set **out) {
+    if (!c || !out) return DCNS_ERR_ARG;
+    *out = nullptr;
+    dcns_frag_set *fs = new (std::nothrow) dcns_frag_set();
+    if (!fs) return DCNS_ERR_INTERNAL;
+    int rc = guarded(c, [&] {
+        check_batch(b);
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        const std::vector<Profile> pv = profiles_of(prof, nprof);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        c->staged.reset();
+        find_similar_weighted(c, *b, *p, pv, PairCategories{cat, cat_off}, two_piece != 0, fs->fs);
+    });
+    if (rc != DCNS_OK) {
+        delete fs;
+        return rc;
+    }
+    *out = fs;
+    return DCNS_OK;
+}
+
+int dcns_read_score_folder(const char *folder, dcns_profile *out, int32_t cap, int32_t *n) {
+    return guarded(nullptr, [&] {
+        if (!folder || !n || cap < 0 || (cap > 0 && !out)) throw Error(DCNS_ERR_ARG, "invalid arguments");
+        const std::vector<Profile> v = read_score_folder(folder);
+        *n = (int32_t)v.size();
+        if ((int32_t)v.size() > cap) throw Error(DCNS_ERR_CAPACITY, "more profiles than cap");
+        for (size_t i = 0; i < v.size(); ++i)
+            out[i] = dcns_profile{v[i].category, v[i].match, v[i].mismatch, v[i].open1, v[i].ext1, v[i].open2,
+                                  v[i].ext2, v[i].zdrop};
+    });
+}
+
+int dcns_gff_categories(const char *gff_path, const char *chr, int64_t chr_size, int64_t from, int64_t count,
+                        int16_t *out) {
+    return guarded(nullptr, [&] {
+        if (!gff_path || !chr || chr_size < 0 || count < 0 || (count > 0 && !out))
+            throw Error(DCNS_ERR_ARG, "invalid arguments");
+        const GffAnnotation g = read_gff(gff_path);
+        gff_categories(g, chr, chr_size, from, count, out);
     });
 }
 

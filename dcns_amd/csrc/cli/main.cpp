@@ -13,6 +13,8 @@ static void usage() {
               << "    cut2Gaps            masking genome, sliding window seed, x-drop and two-piece gap extension\n"
               << "    pairCnsXExtend      sliding window seed and x-drop extension of the sequences of one fasta file\n"
               << "    pairCns2Gaps        the same with two-piece gap extension\n"
+              << "    weighted1Gap        x-drop seeds extended by the weighted (per-base GFF class) alignment\n"
+              << "    weighted2Gaps       the same with the weighted two-piece gap alignment\n"
               << "    ranSco              smith-waterman scores of random sequence fragments\n"
               << "                        for lambda and k fitting\n";
 }
@@ -32,6 +34,8 @@ int main(int argc, char **argv) {
     if (!std::strcmp(p, "cut2Gaps")) return dcns_cut2gaps_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "pairCns2Gaps")) return dcns_paircns2gaps_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "pairCnsXExtend")) return dcns_paircnsxextend_main(argc - 1, argv + 1);
+    if (!std::strcmp(p, "weighted1Gap")) return dcns_weighted1gap_main(argc - 1, argv + 1);
+    if (!std::strcmp(p, "weighted2Gaps")) return dcns_weighted2gaps_main(argc - 1, argv + 1);
     std::cerr << "dCNS: command " << p << " is not part of this build" << std::endl;
     usage();
     return 1;

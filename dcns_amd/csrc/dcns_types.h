@@ -111,6 +111,19 @@ struct SgStrip {
     int64_t bnd;       // int32 offset of the strip's right-boundary records (rows rlo..rhi)
 };
 
+// Per-row score profiles of the weighted (ZDP) semi-global DP
+// (sequenceAlignment3.cpp:1483-1890).  cat[cat_off[pair] + x + 1] is the
+// profile index of reference position x (x = -1 .. len1); tab holds 8 ints per
+// profile: match, mismatch, open1, ext1, open2, ext2, zdrop, 0.
+struct SgWeights {
+    const uint8_t *cat;
+    const int64_t *cat_off;
+    const int32_t *tab;
+    int32_t ncat;
+    int32_t mode;  // 0: unweighted SemiGlobal_no_avx, 1: weighted two-piece, 2: weighted single gap
+};
+constexpr int kMaxProfiles = 64;
+
 struct SgOut {
     int32_t max, e1, e2;   // maximum and its 1-based row/column (-1: none)
     int32_t brk;           // last row evaluated (the termination row, or R)

@@ -5,12 +5,14 @@
 //   cut2Gaps controlLayer4.cpp:1534-1713
 //   pairCns2Gaps controlLayer4.cpp:318-462
 //   pairCnsXExtend controlLayer4.cpp:181-317
+//   weighted1Gap controlLayer4.cpp:481-652, weighted2Gaps :655-829
 // The alignment work runs on the GPU; the host keeps the reference's order
 // of file operations and stdout lines.  All query regions of one cut1Gap
 // input are aligned in one batched call.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <iostream>
 #include <memory>
 #include <sstream>
@@ -19,6 +21,7 @@
 #include "../ctx.h"
 #include "../pipeline.h"
 #include "cli_io.h"
+#include "weights.h"
 
 namespace dcns {
 namespace cli {
@@ -428,6 +431,153 @@ int paircns_common(int argc, char **argv, bool two_piece) {
 }
 }  // namespace
 
+namespace {
+
+void weighted_usage(const char *cmd, const Defaults &d, bool two_piece) {
+    std::cerr << "Usage: dCNS " << cmd << " -i input -r reference -q query -o output\n"
+              << "Options\n"
+              << " -i FILE    input file in fasta format\n"
+              << " -r STRING  reference sequence entry name in input fasta file\n"
+              << " -f STRING  gff3 format reference genome annotation file\n"
+              << " -ra STRING masked reference genome sequence in fasta format\n"
+              << " -sp STRING the folder where score parameters located\n"
+              << " -o FILE    output file, exists file will be overwrote\n"
+              << " -A INT     Matching score (default: " << d.match << ")\n"
+              << " -B INT     Mismatching penalty (default: " << d.mismatch << ")\n"
+              << " -O1 INT    open gap penalty (default: " << d.open1 << ")\n"
+              << " -E1 INT    extend gap penalty (default: " << d.ext1 << ")\n"
+              << " -u INT     xextend alignment band width (default: " << d.w << ")\n";
+    if (two_piece) std::cerr << " -u2 INT    z drop alignment band width (default: " << d.bandwidth << ")\n";
+    std::cerr << " -w INT     seed window size (default: " << d.window << ")\n"
+              << " -c INT     minimum seed score (default: " << d.min_score << ")\n"
+              << " -s INT     seed window step (default: " << d.step << ")\n"
+              << " -l DOUBLE  lambda for p-values (default: " << d.lambda << ")\n"
+              << " -k DOUBLE  K for p-values (default: " << d.k << ")\n"
+              << " -p DOUBLE  p-value threshold (default: " << d.pvalues << ")\n"
+              << std::endl;
+}
+
+void stamp(const char *what) {
+    const time_t t = time(nullptr);
+    std::cout << ctime(&t) << what << std::endl;
+}
+
+// weighted1Gap / weighted2Gaps: the reference region (-r) against every other
+// region of -i, with per-base score profiles (-sp) chosen by the GFF3 class of
+// the reference base (-f over the chromosomes of -ra).
+int weighted_common(int argc, char **argv, bool two_piece) {
+    Defaults d;
+    Args a(argc, argv);
+    if (a.has("-A")) d.match = std::stoi(a.get("-A"));
+    if (a.has("-B")) d.mismatch = std::stoi(a.get("-B"));
+    if (a.has("-O1")) d.open1 = std::stoi(a.get("-O1"));
+    if (a.has("-E1")) d.ext1 = std::stoi(a.get("-E1"));
+    if (a.has("-u")) d.w = std::stoi(a.get("-u"));
+    if (two_piece && a.has("-u2")) d.bandwidth = std::stoi(a.get("-u2"));
+    if (a.has("-y")) d.syntenic = str2bool(a.get("-y"), d.syntenic);
+    if (a.has("-w")) d.window = std::stoi(a.get("-w"));
+    if (a.has("-c")) d.min_score = std::stoi(a.get("-c"));
+    if (a.has("-s")) d.step = std::stoi(a.get("-s"));
+    if (a.has("-b")) d.boundary = std::stoi(a.get("-b"));
+    if (a.has("-l")) d.lambda = std::stod(a.get("-l"));
+    if (a.has("-k")) d.k = std::stod(a.get("-k"));
+    if (a.has("-p")) d.pvalues = std::stod(a.get("-p"));
+    const char *cmd = two_piece ? "weighted2Gaps" : "weighted1Gap";
+    if (a.has("-h") || a.has("--help") ||
+        !(a.has("-i") && a.has("-r") && a.has("-o") && a.has("-ra") && a.has("-sp") && a.has("-f"))) {
+        weighted_usage(cmd, d, two_piece);
+        return 1;
+    }
+    if (d.syntenic) {
+        std::cerr << "dCNS: syntenic filtering (-y) is outside this build's scope" << std::endl;
+        return 1;
+    }
+    const std::string input = a.get("-i"), reference = a.get("-r"), output = a.get("-o");
+    std::map<std::string, std::string> seqs;
+    std::map<std::string, Meta> meta;
+    std::vector<std::string> names;
+    read_meta_fasta(input, seqs, meta, names);
+    std::vector<Profile> prof;
+    std::map<std::string, std::string> genome;
+    std::vector<std::string> chroms;
+    GffAnnotation gff;
+    try {
+        prof = read_score_folder(a.get("-sp"));
+        read_plain_fasta(a.get("-ra"), genome, chroms);
+        gff = read_gff(a.get("-f"));
+    } catch (const std::exception &e) {
+        std::cerr << e.what() << std::endl;
+        return 1;
+    }
+    // the messages of readGffFileWithEveryThing (impl/gffToCategory.cpp:176-237)
+    for (const std::string &l : gff.unknown_lines)
+        std::cout << "we could not analysis the line in the gff/gtf file: " << l << std::endl;
+    stamp("gff file reading done");
+    stamp("gff file intron remove duplication done");
+    stamp("gff file exons remove duplication done");
+    stamp("gff file cds remove duplication done");
+    stamp("gff file transposons remove duplication done");
+    stamp("update weigth introns done");
+    stamp("update weigth exon done");
+    stamp("update weigth cds done");
+    stamp("update weigth transposons done");
+    const Meta refMeta = meta[reference];
+    const std::string seq1 = seqs[reference];
+    auto gchr = genome.find(refMeta.chr);
+    if (gchr == genome.end()) {
+        std::cerr << "dCNS: chromosome " << refMeta.chr << " of the reference region is not in " << a.get("-ra")
+                  << std::endl;
+        return 1;
+    }
+    // categories of the region's bases -1 .. len (weight[chr] + start - 1, :523)
+    const int64_t L1 = (int64_t)seq1.size();
+    std::vector<int16_t> wcat((size_t)L1 + 2);
+    gff_categories(gff, refMeta.chr, (int64_t)gchr->second.size(), (int64_t)refMeta.start - 2, L1 + 2, wcat.data());
+    genome.clear();
+    const std::vector<int8_t> c1 = encode(seq1);
+    std::vector<std::string> qn;
+    std::vector<int8_t> buf(c1.begin(), c1.end());
+    std::vector<int64_t> off1, off2, cat_off;
+    std::vector<int32_t> len1, len2;
+    for (const std::string &q : names) {
+        if (q.compare(reference) == 0) continue;
+        const std::vector<int8_t> c2 = encode(seqs[q]);
+        qn.push_back(q);
+        off1.push_back(0);
+        len1.push_back((int32_t)c1.size());
+        off2.push_back((int64_t)buf.size());
+        len2.push_back((int32_t)c2.size());
+        cat_off.push_back(0);  // every query pairs with the same reference region
+        buf.insert(buf.end(), c2.begin(), c2.end());
+    }
+    FragSet fs;
+    if (!qn.empty()) {
+        dcns_params p = to_dcns_params(d);
+        dcns_batch b{buf.data(), off1.data(), len1.data(), off2.data(), len2.data(), (int32_t)qn.size(), 0};
+        dcns_ctx *ctx = make_ctx();
+        try {
+            find_similar_weighted(ctx, b, p, prof, PairCategories{wcat.data(), cat_off.data()}, two_piece, fs);
+        } catch (const std::exception &e) {
+            std::cerr << "dCNS: " << e.what() << std::endl;
+            dcns_ctx_destroy(ctx);
+            return 1;
+        }
+        dcns_ctx_destroy(ctx);
+    }
+    for (size_t i = 0; i < qn.size(); ++i) {
+        const Meta &m = meta[qn[i]];
+        std::vector<PairedSimilarFragment> fr(fs.frags.begin() + fs.first[i],
+                                              fs.frags.begin() + fs.first[i] + fs.count[i]);
+        write_sam_and_o_unmasked(output + m.species, fr, refMeta.strand, m.strand, refMeta.start, m.start, refMeta.chr,
+                                 m.chr, seq1, seqs[qn[i]]);
+    }
+    return 0;
+}
+
+}  // namespace
+
+int weighted1gap_main(int argc, char **argv) { return weighted_common(argc, argv, false); }
+int weighted2gaps_main(int argc, char **argv) { return weighted_common(argc, argv, true); }
 int paircns2gaps_main(int argc, char **argv) { return paircns_common(argc, argv, true); }
 int paircnsxextend_main(int argc, char **argv) { return paircns_common(argc, argv, false); }
 
@@ -533,3 +683,5 @@ extern "C" int dcns_ransco_main(int argc, char **argv) { return dcns::cli::ransc
 extern "C" int dcns_cut2gaps_main(int argc, char **argv) { return dcns::cli::cut2gaps_main(argc, argv); }
 extern "C" int dcns_paircns2gaps_main(int argc, char **argv) { return dcns::cli::paircns2gaps_main(argc, argv); }
 extern "C" int dcns_paircnsxextend_main(int argc, char **argv) { return dcns::cli::paircnsxextend_main(argc, argv); }
+extern "C" int dcns_weighted1gap_main(int argc, char **argv) { return dcns::cli::weighted1gap_main(argc, argv); }
+extern "C" int dcns_weighted2gaps_main(int argc, char **argv) { return dcns::cli::weighted2gaps_main(argc, argv); }

@@ -65,11 +65,13 @@ int64_t sg_scratch_ints(int32_t R, int32_t C, int32_t w);  // row results + stri
 // from int32 offset bnd on); returns their number.
 int32_t sg_task_strips(int32_t R, int32_t C, int32_t w, int32_t task, int64_t bnd, std::vector<SgStrip> &out);
 // sync: 2 + nstrips zeroed ints (start counter, error word, per-strip progress).
+// W.mode 0: SemiGlobal_no_avx with P's scores; 1 / 2: weighted two-piece /
+// single gap with W's profiles (P's scores unused).
 hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
                         const SgStrip *strips, int32_t nstrips, const Params &P, int32_t *scr, uint8_t *tb,
-                        SgOut *out, int32_t *sync);
+                        SgOut *out, const SgWeights &W, int32_t *sync);
 hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
                             const SgStrip *strips, int32_t n, const Params &P, const int32_t *scr, const uint8_t *tb,
-                            SgOut *out, uint32_t *cigar);
+                            SgOut *out, uint32_t *cigar, const SgWeights &W);
 
 }  // namespace dcns

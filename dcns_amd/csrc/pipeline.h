@@ -62,8 +62,23 @@ struct SgResult {
     int32_t max, end1, end2;
     std::vector<uint32_t> cigar;
 };
+// W (may be null): the weighted (ZDP) DP with per-row score profiles
+// (sequenceAlignment3.cpp:1483-1890) instead of SemiGlobal_no_avx.
 void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &tasks, bool trace,
-                      std::vector<SgResult> &out, int64_t &cells);
+                      std::vector<SgResult> &out, int64_t &cells, const SgWeights *W = nullptr);
+
+// One score profile of the weighted paths: a file of the score folder
+// (model/Score.cpp:9-71).
+struct Profile {
+    int16_t category;
+    int32_t match, mismatch, open1, ext1, open2, ext2, zdrop;
+};
+// Categories of the pairs' reference positions -1 .. len1: len1 + 2 values
+// from cat + off[pair] (the reference's weight pointer at [-1 .. len1]).
+struct PairCategories {
+    const int16_t *cat;
+    const int64_t *off;
+};
 
 
 
@@ -78,6 +93,15 @@ void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &
 // then overload 2 (:532-616).
 void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &out);
 void find_similar_2gaps(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out);
+// Uploads the profiles and every pair's categories (as profile indices) for
+// the weighted DP; mode 1 (two-piece) or 2 (single gap).
+SgWeights stage_weights(dcns_ctx *c, const dcns_batch &b, const std::vector<Profile> &prof, const PairCategories &pc,
+                        int mode);
+// findSimilarFragmentsForPairedSequence_wighted (two_piece, :830-925) and
+// _wighted_1gap (:779-826): getAllExtendSeed with p's scoring, then the
+// weighted reverse / forward DP per x-seed with the profiles and categories.
+void find_similar_weighted(dcns_ctx *c, const dcns_batch &b, const dcns_params &p,
+                           const std::vector<Profile> &prof, const PairCategories &pc, bool two_piece, FragSet &out);
 
 }  // namespace dcns
 

@@ -119,6 +119,7 @@ struct StripIn {
     const int32_t *bin;  // previous strip's records, row plo first
     int32_t first, plo, phi, R, a1, rev;
     const int8_t *s1;
+    const uint8_t *wc;   // weighted: profile indices of reference positions -1 .. len1 (wc[x + 1])
     __device__ __forceinline__ void load(int32_t r, int32_t &m, int32_t &e1, int32_t &e2, int32_t &rm, int32_t &rj,
                                          int32_t &rc) const {
         m = NEG;
@@ -135,7 +136,15 @@ struct StripIn {
             rm = ald(b + 3);
             rj = ald(b + 4);
         }
-        if (r >= 1 && r <= R) rc = code_at(s1, a1, r - 1, rev);
+        if (r >= 1 && r <= R) {
+            rc = code_at(s1, a1, r - 1, rev);
+            if (wc) {
+                // row r's profile (its reference base) in bits 4-11, the next
+                // row's (the vertical gap penalties) in bits 12-19
+                const int32_t pos = rev ? a1 - (r - 1) : a1 + r - 1, pn = rev ? pos - 1 : pos + 1;
+                rc |= ((int32_t)wc[pos + 1] << 4) | ((int32_t)wc[pn + 1] << 12);
+            }
+        }
     }
 };
 
@@ -184,19 +193,79 @@ __device__ __forceinline__ void columns(const uint32_t (&tlo)[K], const uint32_t
     }
 }
 
+// The weighted counterpart of columns() (SemiGlobal with a Score profile per
+// reference base, sequenceAlignment3.cpp:1483-1706; single gap piece when
+// TWO is false, :1710-1890).  Substitution scores, horizontal gap penalties
+// and the row's profile come from this row's base, the vertical gap
+// penalties (vo*, vx*) from the next row's.  Every band column, the last one
+// included, is a full cell; the last column's vertical states start from NEG
+// on every row (F[end] is reset before the row).
+template <bool TB, bool EDGE, bool TWO>
+__device__ __forceinline__ void columns_w(const int32_t (&qc)[K], int32_t (&F1)[K], int32_t (&F2)[K],
+                                          int32_t (&Mp)[K], int32_t rc, int32_t match, int32_t mis, int32_t lo,
+                                          int32_t hi, int32_t jfirst, int32_t o1, int32_t x1, int32_t o2, int32_t x2,
+                                          int32_t vo1, int32_t vx1, int32_t vo2, int32_t vx2, int32_t dg, int32_t &e1,
+                                          int32_t &e2, int32_t &rm, int32_t &rj, uint32_t &tword) {
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const int32_t sc = (qc[c] == 2) ? 0 : (qc[c] == rc ? match : mis);
+        const int32_t m0 = sc + dg;
+        dg = Mp[c];
+        const bool last = EDGE && c == hi;
+        const int32_t f1 = last ? NEG : F1[c];
+        const int32_t f2o = TWO ? (last ? NEG : F2[c]) : NEG;
+        const int32_t m = TWO ? max(max(max(m0, f1), max(e1, f2o)), e2) : max(max(m0, f1), e1);
+        const bool left = EDGE && c < lo;
+        const bool inb = EDGE ? (!left & (c <= hi)) : true;
+        const bool up = inb & (m > rm);
+        rm = up ? m : rm;
+        rj = up ? jfirst + c : rj;
+        const int32_t hv = m + vo1, f = f1 + vx1, hh = m + o1, ne1 = e1 + x1;
+        const int32_t hv2 = m + vo2, f2 = f2o + vx2, hh2 = m + o2, ne2 = e2 + x2;
+        if (TB) {
+            // source: the last of diag, F1, e1(, F2, e2) that reaches the maximum
+            int32_t src = (f1 == m) ? 1 : 0;
+            src = (e1 == m) ? 2 : src;
+            int32_t bits = ((f >= hv) ? 8 : 0) | ((ne1 >= hh) ? 16 : 0);
+            if (TWO) {
+                src = (f2o == m) ? 3 : src;
+                src = (e2 == m) ? 4 : src;
+                bits |= ((f2 >= hv2) ? 32 : 0) | ((ne2 >= hh2) ? 64 : 0);
+            }
+            const uint32_t byte = inb ? (uint32_t)(src | bits) : 0xffu;
+            tword = c == 0 ? byte : (tword | (byte << (8 * c)));
+        }
+        // columns right of the band keep the NEG vertical states they start with
+        const bool upd = EDGE ? (c <= hi) : true;
+        F1[c] = upd ? max(f, hv) : F1[c];
+        e1 = left ? NEG : max(ne1, hh);
+        if (TWO) {
+            F2[c] = upd ? max(f2, hv2) : F2[c];
+            e2 = left ? NEG : max(ne2, hh2);
+        }
+        Mp[c] = m;
+    }
+}
+
 // One wave per strip.  Strips take their list index from a launch-wide
 // counter in the order they start running, so a strip's predecessor (the
 // entry before it) has always started: every wait ends, whatever number of
 // waves the device holds at once.  sync[0] = the counter, sync[1] = the
 // error word, sync[2 + k] = rows of strip k whose boundary records are
-// complete.
-template <bool TB>
+// complete.  MODE 0: SemiGlobal_no_avx; 1: weighted two-piece; 2: weighted
+// single gap (profiles in W).
+template <bool TB, int MODE>
 __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                  const SgTask *__restrict__ tasks, const SgStrip *__restrict__ strips,
                                                  int32_t total, Params P, int32_t *__restrict__ scr,
-                                                 uint8_t *__restrict__ tb, SgOut *__restrict__ out,
+                                                 uint8_t *__restrict__ tb, SgOut *__restrict__ out, SgWeights W,
                                                  int32_t *__restrict__ sync) {
+    __shared__ int32_t wtab[kMaxProfiles * 8];
     const int lane = threadIdx.x;
+    if (MODE != 0) {
+        for (int i = lane; i < W.ncat * 8; i += 64) wtab[i] = W.tab[i];
+        __syncthreads();
+    }
     int32_t k = 0;
     if (lane == 0) k = atomicAdd(sync, 1);
     k = __builtin_amdgcn_readfirstlane(k);
@@ -229,15 +298,17 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
     in.a1 = T.a1;
     in.rev = T.rev;
     in.s1 = s1;
+    in.wc = MODE != 0 ? W.cat + W.cat_off[T.pair] : nullptr;
 
     const int32_t nsteps = S.rhi - S.rlo + 1 + 63;
     const int32_t jfirst = S.s * SW + 1 + lane * K;
     uint32_t tlo[K], thi[K];
-    int32_t F1[K], F2[K], Mp[K];
+    int32_t qc[K], F1[K], F2[K], Mp[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) {
         const int32_t j = jfirst + c;
-        column_table(j <= C ? code_at(s2, T.a2, j - 1, T.rev) : 2, P.match, P.mismatch, tlo[c], thi[c]);
+        qc[c] = j <= C ? code_at(s2, T.a2, j - 1, T.rev) : 2;
+        if (MODE == 0) column_table(qc[c], P.match, P.mismatch, tlo[c], thi[c]);
         F1[c] = F2[c] = Mp[c] = NEG;
     }
     int32_t prev_im = NEG, om = NEG, oe1 = NEG, oe2 = NEG, orm = 0, orj = 0, orc = 2;
@@ -281,22 +352,38 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
             if (valid) {
                 const int32_t st = max(r - w, 1), en = min(r + w, C);
                 const int32_t lo = st - jfirst, hi = en - jfirst;  // in-band columns: lo <= c < hi; c == hi: last
-                const bool hsel = irc >= 4;
-                const uint32_t sh = (uint32_t)(irc & 3) * 8u;
                 int32_t e1 = ie1, e2 = ie2, rm = irm, rj = irj;
                 // every lane's columns inside the band (the usual case): no edge masks
-                if (__ballot(lo > 0 || hi < K) == 0)
-                    columns<TB, false>(tlo, thi, F1, F2, Mp, hsel, sh, lo, hi, jfirst, o1, x1, o2, x2, diag, e1, e2,
-                                       rm, rj, tword);
-                else
-                    columns<TB, true>(tlo, thi, F1, F2, Mp, hsel, sh, lo, hi, jfirst, o1, x1, o2, x2, diag, e1, e2,
-                                      rm, rj, tword);
+                const bool interior = __ballot(lo > 0 || hi < K) == 0;
+                if (MODE == 0) {
+                    const bool hsel = irc >= 4;
+                    const uint32_t sh = (uint32_t)(irc & 3) * 8u;
+                    if (interior)
+                        columns<TB, false>(tlo, thi, F1, F2, Mp, hsel, sh, lo, hi, jfirst, o1, x1, o2, x2, diag, e1,
+                                           e2, rm, rj, tword);
+                    else
+                        columns<TB, true>(tlo, thi, F1, F2, Mp, hsel, sh, lo, hi, jfirst, o1, x1, o2, x2, diag, e1,
+                                          e2, rm, rj, tword);
+                } else {
+                    // this row's profile and the next row's (vertical gaps)
+                    const int32_t rc = irc & 15;
+                    const int32_t *pc = wtab + ((irc >> 4) & 255) * 8, *pv = wtab + (irc >> 12) * 8;
+                    const int32_t match = rc == 2 ? 0 : pc[0], mis = rc == 2 ? 0 : pc[1];
+                    if (interior)
+                        columns_w<TB, false, MODE == 1>(qc, F1, F2, Mp, rc, match, mis, lo, hi, jfirst, pc[2], pc[3],
+                                                        pc[4], pc[5], pv[2], pv[3], pv[4], pv[5], diag, e1, e2, rm,
+                                                        rj, tword);
+                    else
+                        columns_w<TB, true, MODE == 1>(qc, F1, F2, Mp, rc, match, mis, lo, hi, jfirst, pc[2], pc[3],
+                                                       pc[4], pc[5], pv[2], pv[3], pv[4], pv[5], diag, e1, e2, rm,
+                                                       rj, tword);
+                }
                 om = Mp[K - 1];
                 oe1 = e1;
                 oe2 = e2;
                 orm = rm;
                 orj = rj;
-                if (S.s == stale_strip && lane == stale_lane && (r == C + w - 2 || r == C + w - 1))
+                if (MODE == 0 && S.s == stale_strip && lane == stale_lane && (r == C + w - 2 || r == C + w - 1))
                     out[S.task].stale[r - (C + w - 2)] = Mp[stale_c];
             } else {
                 om = NEG;
@@ -360,7 +447,8 @@ __global__ __launch_bounds__(64) void k_sg_finish(const int8_t *__restrict__ seq
                                                   const SgTask *__restrict__ tasks,
                                                   const SgStrip *__restrict__ strips, int32_t n, Params P,
                                                   const int32_t *__restrict__ scr, const uint8_t *__restrict__ tb,
-                                                  SgOut *__restrict__ out, uint32_t *__restrict__ cigar) {
+                                                  SgOut *__restrict__ out, uint32_t *__restrict__ cigar,
+                                                  SgWeights W) {
     __shared__ uint32_t tile[64 * 16];
     const int32_t k = blockIdx.x;
     if (k >= n) return;
@@ -380,7 +468,22 @@ __global__ __launch_bounds__(64) void k_sg_finish(const int8_t *__restrict__ seq
         const bool live = i <= R;
         int32_t rm = 0, rj = 0;
         int64_t cl = 0;
-        if (live) {
+        int32_t zd = P.zdrop, x2 = P.ext2;
+        if (live && W.mode != 0) {
+            // weighted: every band column is a cell; rows past C + w have an
+            // empty band, no positive cell and end the row loop
+            const int32_t st = max(i - w, 1), en = min(i + w, C);
+            cl = (int64_t)max(0, en - st + 1);
+            rj = -1;
+            if (i <= C + w) {
+                rm = rowres[2 * (int64_t)i];
+                rj = rowres[2 * (int64_t)i + 1];
+            }
+            const int32_t pos = T.rev ? T.a1 - (i - 1) : T.a1 + i - 1;
+            const int32_t *pc = W.tab + 8 * (int32_t)W.cat[W.cat_off[T.pair] + pos + 1];
+            zd = pc[6];
+            x2 = W.mode == 1 ? pc[5] : 0;  // the single-gap z-drop has no l term
+        } else if (live) {
             const int32_t st = max(i - w, 1), en = min(i + w, C);
             cl = (int64_t)max(0, en - st) + 1;
             if (i <= C + w) {
@@ -412,7 +515,9 @@ __global__ __launch_bounds__(64) void k_sg_finish(const int8_t *__restrict__ seq
         const int32_t e2 = row >= i0 ? rj_best : (row > 0 ? ce2 : -1);
         int32_t l = (i - e1) - (rj - e2);
         l = l > 0 ? l : -l;
-        const bool stop = live && (best - rm > P.zdrop + l * P.ext2 || rj <= 0);
+        // zdrop + l * ext2 in 32-bit two's complement, as the reference's int32
+        const int32_t lim = (int32_t)((uint32_t)zd + (uint32_t)l * (uint32_t)x2);
+        const bool stop = live && (best - rm > lim || rj <= 0);
         const uint64_t hits = __ballot(stop);
         const int last = hits ? __ffsll((unsigned long long)hits) - 1 : 63;
         cells += wave_sum64(lane <= last ? cl : 0);
@@ -557,22 +662,31 @@ int32_t sg_task_strips(int32_t R, int32_t C, int32_t w, int32_t task, int64_t bn
 
 hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
                         const SgStrip *strips, int32_t nstrips, const Params &P, int32_t *scr, uint8_t *tb,
-                        SgOut *out, int32_t *sync) {
+                        SgOut *out, const SgWeights &W, int32_t *sync) {
     if (nstrips <= 0) return hipSuccess;
-    if (trace)
-        hipLaunchKernelGGL(k_sg_strip<true>, dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, nstrips, P,
-                           scr, tb, out, sync);
-    else
-        hipLaunchKernelGGL(k_sg_strip<false>, dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, nstrips, P,
-                           scr, tb, out, sync);
+    if (W.mode != 0 && (W.ncat < 1 || W.ncat > kMaxProfiles)) return hipErrorInvalidValue;
+#define DCNS_SG_LAUNCH(TBV, MV)                                                                                      \
+    hipLaunchKernelGGL((k_sg_strip<TBV, MV>), dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, nstrips, P,  \
+                       scr, tb, out, W, sync)
+    switch (W.mode * 2 + (trace ? 1 : 0)) {
+        case 0: DCNS_SG_LAUNCH(false, 0); break;
+        case 1: DCNS_SG_LAUNCH(true, 0); break;
+        case 2: DCNS_SG_LAUNCH(false, 1); break;
+        case 3: DCNS_SG_LAUNCH(true, 1); break;
+        case 4: DCNS_SG_LAUNCH(false, 2); break;
+        case 5: DCNS_SG_LAUNCH(true, 2); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef DCNS_SG_LAUNCH
     return hipGetLastError();
 }
 
 hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
                             const SgStrip *strips, int32_t n, const Params &P, const int32_t *scr, const uint8_t *tb,
-                            SgOut *out, uint32_t *cigar) {
+                            SgOut *out, uint32_t *cigar, const SgWeights &W) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sg_finish, dim3(n), dim3(64), 0, st, seq, pairs, tasks, strips, n, P, scr, tb, out, cigar);
+    hipLaunchKernelGGL(k_sg_finish, dim3(n), dim3(64), 0, st, seq, pairs, tasks, strips, n, P, scr, tb, out, cigar,
+                       W);
     return hipGetLastError();
 }
 

@@ -3,7 +3,9 @@
 //   (impl/findSimilarFragmentsForPairedSequence4.cpp:752-774) =
 //   getAllExtendSeed (:348-429) followed by overload 2 (:532-616),
 // with every SemiGlobal call (impl/sequenceAlignment3.cpp:932 -> :681) run as
-// batched device tasks.
+// batched device tasks; and of the weighted paths _wighted (:830-925) and
+// _wighted_1gap (:779-826), the same flow with the weighted DP
+// (sequenceAlignment3.cpp:1483-1890).
 //
 // Order of work per batch: seed sets (device scan + host std::set replay),
 // score-only x-drop of every linked seed (device), the x-seed filter and
@@ -16,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <set>
+#include <unordered_map>
 #include <utility>
 
 #include "kernels.h"
@@ -40,7 +43,8 @@ void validate_two_piece(const dcns_params &p) {
 }  // namespace
 
 void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &tasks, bool trace,
-                      std::vector<SgResult> &out, int64_t &cells) {
+                      std::vector<SgResult> &out, int64_t &cells, const SgWeights *Wp) {
+    const SgWeights W = Wp ? *Wp : SgWeights{nullptr, nullptr, nullptr, 0, 0};
     const int32_t n = (int32_t)tasks.size();
     out.assign((size_t)n, SgResult{0, -2, -2, {}});
     cells = 0;
@@ -112,14 +116,14 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
         c->launch("semiglobal", [&] {
             return launch_sg_dp(c->stream, trace, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
                                 c->sg_strips.as<SgStrip>(), ns, P, c->sg_scr.as<int32_t>(),
-                                trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>(),
+                                trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>(), W,
                                 c->sg_sync.as<int32_t>());
         });
         c->launch("semiglobal_finish", [&] {
             return launch_sg_finish(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
                                     c->sg_strips.as<SgStrip>(), m, P, c->sg_scr.as<int32_t>(),
                                     trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>(),
-                                    trace ? c->sg_cig.as<uint32_t>() : nullptr);
+                                    trace ? c->sg_cig.as<uint32_t>() : nullptr, W);
         });
         int32_t stalled = 0;
         hip_check(hipMemcpyAsync(&stalled, c->sg_sync.as<int32_t>() + 1, 4, hipMemcpyDeviceToHost, c->stream),
@@ -156,9 +160,11 @@ void find_similar_2gaps(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, 
     find_similar_2gaps_staged(c, S, p, out);
 }
 
-void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &out) {
-    validate_params(p);
-    validate_two_piece(p);
+namespace {
+
+// getAllExtendSeed, then overload 2's loop (W.mode 0) or the weighted loops
+// (W.mode 1: _wighted, 2: _wighted_1gap).
+void two_gaps_flow(dcns_ctx *c, Staged &S, const dcns_params &p, const SgWeights &W, FragSet &out) {
     const Params P = to_params(p);
     relayout(c, S, P);
     const int32_t npairs = (int32_t)S.pd.size();
@@ -230,7 +236,7 @@ void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, Fra
     xfirst[(size_t)npairs] = (int64_t)rt.size();
     std::vector<SgResult> rr;
     int64_t sgcells = 0, c1 = 0, c2 = 0;
-    semiglobal_batch(c, P, rt, false, rr, c1);
+    semiglobal_batch(c, P, rt, false, rr, c1, &W);
     // StartAlignment set (:572-574), then the forward SemiGlobal with traceback
     std::vector<SgTask> ft;
     std::vector<int64_t> ffirst((size_t)npairs + 1, 0);
@@ -254,7 +260,7 @@ void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, Fra
     }
     ffirst[(size_t)npairs] = (int64_t)ft.size();
     std::vector<SgResult> fr;
-    semiglobal_batch(c, P, ft, true, fr, c2);
+    semiglobal_batch(c, P, ft, true, fr, c2, &W);
     sgcells = c1 + c2;
     const double t3 = ms_now();
     // fragment replace / append (:584-611), p-values without a filter
@@ -265,7 +271,8 @@ void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, Fra
         for (int64_t k = ffirst[(size_t)i]; k < ffirst[(size_t)i + 1]; ++k) {
             const SgResult &r = fr[(size_t)k];
             const int32_t start1 = ft[(size_t)k].a1, start2 = ft[(size_t)k].a2;
-            if (!(r.cigar.size() > 0 && r.end1 > 0 && r.end2 > 0)) continue;
+            // _wighted_1gap keeps every forward result (:795-822)
+            if (W.mode != 2 && !(r.cigar.size() > 0 && r.end1 > 0 && r.end2 > 0)) continue;
             const int32_t end1 = r.end1 + start1, end2 = r.end2 + start2;
             int32_t to_remove = -1;
             bool found = false;
@@ -312,6 +319,69 @@ void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, Fra
     st.ms_semiglobal = t3 - t2;
     st.ms_replay = t4 - t3;
     st.ms_total = t4 - t0;
+}
+
+}  // namespace
+
+void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &out) {
+    validate_params(p);
+    validate_two_piece(p);
+    two_gaps_flow(c, S, p, SgWeights{nullptr, nullptr, nullptr, 0, 0}, out);
+}
+
+SgWeights stage_weights(dcns_ctx *c, const dcns_batch &b, const std::vector<Profile> &prof, const PairCategories &pc,
+                        int mode) {
+    if (prof.empty() || (int)prof.size() > kMaxProfiles)
+        throw Error(DCNS_ERR_ARG, "1 to 64 score profiles are supported");
+    if (!pc.cat || !pc.off) throw Error(DCNS_ERR_ARG, "categories are required");
+    // dense profile indices; a category without a profile is an error (the
+    // reference dereferences a null score matrix for it)
+    std::unordered_map<int32_t, uint8_t> index;
+    std::vector<int32_t> tab(prof.size() * 8, 0);
+    for (size_t i = 0; i < prof.size(); ++i) {
+        const Profile &f = prof[i];
+        if (!index.emplace((int32_t)f.category, (uint8_t)i).second)
+            throw Error(DCNS_ERR_ARG, "duplicate score profile category " + std::to_string(f.category));
+        const int32_t v[8] = {f.match, f.mismatch, f.open1, f.ext1, f.open2, f.ext2, f.zdrop, 0};
+        std::copy(v, v + 8, tab.begin() + 8 * (int64_t)i);
+    }
+    const int32_t np = b.n;
+    std::vector<int64_t> off((size_t)std::max(np, 1), 0);
+    int64_t total = 0;
+    for (int32_t i = 0; i < np; ++i) {
+        off[(size_t)i] = total;
+        total += (int64_t)b.len1[i] + 2;
+    }
+    std::vector<uint8_t> dense((size_t)std::max<int64_t>(total, 1));
+    for (int32_t i = 0; i < np; ++i) {
+        const int16_t *src = pc.cat + pc.off[i];
+        for (int64_t x = 0; x < (int64_t)b.len1[i] + 2; ++x) {
+            auto it = index.find((int32_t)src[x]);
+            if (it == index.end())
+                throw Error(DCNS_ERR_ARG, "category " + std::to_string(src[x]) + " has no score profile");
+            dense[(size_t)(off[(size_t)i] + x)] = it->second;
+        }
+    }
+    c->sg_wcat.ensure(dense.size());
+    c->sg_wcat_off.ensure(off.size() * sizeof(int64_t));
+    c->sg_wtab.ensure(tab.size() * sizeof(int32_t));
+    hip_check(hipMemcpyAsync(c->sg_wcat.p, dense.data(), dense.size(), hipMemcpyHostToDevice, c->stream), "categories");
+    hip_check(hipMemcpyAsync(c->sg_wcat_off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                             c->stream), "category offsets");
+    hip_check(hipMemcpyAsync(c->sg_wtab.p, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream), "score profiles");
+    c->sync();  // the host vectors end here
+    return SgWeights{c->sg_wcat.as<uint8_t>(), c->sg_wcat_off.as<int64_t>(), c->sg_wtab.as<int32_t>(),
+                     (int32_t)prof.size(), mode};
+}
+
+void find_similar_weighted(dcns_ctx *c, const dcns_batch &b, const dcns_params &p,
+                           const std::vector<Profile> &prof, const PairCategories &pc, bool two_piece, FragSet &out) {
+    validate_params(p);
+    if (p.bandwidth < 1) throw Error(DCNS_ERR_ARG, "bandwidth must be >= 1");
+    Staged S = stage_batch(c, b, to_params(p));
+    const SgWeights W = stage_weights(c, b, prof, pc, two_piece ? 1 : 2);
+    two_gaps_flow(c, S, p, W, out);
 }
 
 }  // namespace dcns

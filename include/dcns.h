@@ -159,6 +159,55 @@ int dcns_semiglobal_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params 
                           const int32_t *anchors, const int32_t *reverse, int32_t n, int32_t with_cigar,
                           int32_t *out3, uint32_t *cigar, int64_t *cig_off, int64_t cig_cap);
 
+/* ---- Weighted (ZDP) paths: weighted1Gap / weighted2Gaps ------------------ */
+
+/* One score profile: the seven lines of a score-folder file
+ * (scoreMatrix/<category>, model/Score.cpp:9-71). */
+typedef struct dcns_profile {
+    int32_t category;
+    int32_t match, mismatch, open1, ext1, open2, ext2, zdrop;
+} dcns_profile;
+
+/* Score::Score(folder) (model/Score.cpp:9-71): the profiles listed in
+ * folder/matrixList, in list order (a category listed again takes the later
+ * file's values).  n receives the count; DCNS_ERR_CAPACITY when it exceeds
+ * cap.  No device needed. */
+int dcns_read_score_folder(const char *folder, dcns_profile *out, int32_t cap, int32_t *n);
+
+/* readGffFileWithEveryThing (impl/gffToCategory.cpp:81-238): the category of
+ * each position [from, from + count) (0-based) of chromosome chr (chr_size
+ * bases) under the GFF3 annotation: 10 intergenic, 70 gene-like records, 85
+ * merged exons (without each chromosome's last merged interval, as
+ * removeDuplications :11-33 leaves it), 100 CDS, 1 transposable elements.
+ * Positions outside the chromosome, which the reference reads outside its
+ * array, get 10.  Records of other chromosomes are ignored.  No device
+ * needed. */
+int dcns_gff_categories(const char *gff_path, const char *chr, int64_t chr_size, int64_t from, int64_t count,
+                        int16_t *out);
+
+/* findSimilarFragmentsForPairedSequence_wighted (two_piece != 0,
+ * impl/findSimilarFragmentsForPairedSequence4.cpp:830-925) and
+ * findSimilarFragmentsForPairedSequence_wighted_1gap (two_piece == 0,
+ * :779-826), called from controlLayer4.cpp:701 (weighted2Gaps) and :525
+ * (weighted1Gap), for every pair.  p's scoring drives getAllExtendSeed; the
+ * weighted DP takes its scores from the profiles.  cat + cat_off[i] holds the
+ * categories of pair i's reference positions -1 .. len1[i] (len1[i] + 2
+ * values: the reference's weight pointer at [-1 .. len1]); every category
+ * needs a profile.  p->bandwidth is the DP band. */
+int dcns_find_similar_weighted_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p,
+                                     const dcns_profile *prof, int32_t nprof, const int16_t *cat,
+                                     const int64_t *cat_off, int32_t two_piece, dcns_frag_set **out);
+
+/* The weighted SemiGlobal (impl/sequenceAlignment3.cpp:1483, two-piece) or
+ * SemiGlobal_single_gap_penalty (:1710) for n tasks, laid out as
+ * dcns_semiglobal_batch; categories and profiles as
+ * dcns_find_similar_weighted_batch. */
+int dcns_semiglobal_weighted_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p,
+                                   const dcns_profile *prof, int32_t nprof, const int16_t *cat,
+                                   const int64_t *cat_off, int32_t two_piece, const int32_t *pair_of_task,
+                                   const int32_t *anchors, const int32_t *reverse, int32_t n, int32_t with_cigar,
+                                   int32_t *out3, uint32_t *cigar, int64_t *cig_off, int64_t cig_cap);
+
 /* Keeps a batch resident in device memory (replacing any earlier one; the
  * other entry points that take a batch drop it) and runs
  * dcns_find_similar_batch on it without re-uploading.  No reference
@@ -215,10 +264,16 @@ int dcns_ransco_main(int argc, char **argv);
  *   dcns_paircns2gaps_main  int pairCns2Gaps(int argc, char** argv), controlLayer4.cpp:378
  *                           (the queries are the input's sequences; the reference
  *                           clears them first, :346, and aborts)
- *   dcns_paircnsxextend_main int pairCnsXExtend(int argc, char** argv), controlLayer4.cpp:238 */
+ *   dcns_paircnsxextend_main int pairCnsXExtend(int argc, char** argv), controlLayer4.cpp:238
+ *   dcns_weighted1gap_main   int weighted1Gap(int argc, char** argv), controlLayer4.cpp:554
+ *   dcns_weighted2gaps_main  int weighted2Gaps(int argc, char** argv), controlLayer4.cpp:729
+ *                           (all query regions in one batch; the GFF3 categories
+ *                           of the reference region only) */
 int dcns_cut2gaps_main(int argc, char **argv);
 int dcns_paircns2gaps_main(int argc, char **argv);
 int dcns_paircnsxextend_main(int argc, char **argv);
+int dcns_weighted1gap_main(int argc, char **argv);
+int dcns_weighted2gaps_main(int argc, char **argv);
 
 /* lambda/K surrogate network.  No counterpart in the reference, which obtains
  * lambda and K only by permutation (ranSco, permutationLsqLambdaK.cpp:32-102,

@@ -156,6 +156,119 @@ def _find_similar_2gaps(self, s1, s2, sc=(2, -3, -4, -2, -45, 0), window=38, ste
 
 Checker.find_similar_2gaps = _find_similar_2gaps
 
+# -- weighted (ZDP) paths ----------------------------------------------------
+# profiles: {category: (match, mismatch, open1, ext1, open2, ext2, zdrop)},
+# the seven lines of one score-folder file (model/Score.cpp:9-71).
+
+
+def write_score_folder(path, profiles):
+    """Writes a score folder (matrixList + one file per category)."""
+    os.makedirs(path, exist_ok=True)
+    with open(os.path.join(path, "matrixList"), "w") as f:
+        f.write("\n".join(str(c) for c in profiles))
+    for c, v in profiles.items():
+        with open(os.path.join(path, str(c)), "w") as f:
+            f.write("\n".join(str(int(x)) for x in v))
+    return path
+
+
+def _profile_arrays(profiles):
+    cats = np.array(list(profiles), np.int16)
+    vals = np.array([list(profiles[c]) for c in profiles], np.int32).reshape(-1)
+    return cats, vals
+
+
+def _semiglobal_weighted(self, s1, s2, wcat, profiles, w=20000, two_piece=True, cigar=True, folder=None):
+    """Weighted SemiGlobal on one task.  wcat: categories of rows -1 .. L1
+    (L1 + 2 values; the weights pointer is wcat + 1).  Returns (max, end1,
+    end2, cigar, cells) -- cells from the oracle only (None for the reference)."""
+    s1, s2 = self._codes(s1), self._codes(s2)
+    wc = np.ascontiguousarray(wcat, np.int16)
+    assert wc.size == len(s1) + 2
+    out = np.zeros(3, np.int32)
+    cap = len(s1) + len(s2) + 16
+    cig = np.zeros(cap, np.uint32)
+    wptr = wc.ctypes.data + 2
+    if self.prefix == "orc":
+        cats, vals = _profile_arrays(profiles)
+        f = self._f("semiglobal_weighted")
+        f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _P, _I, _I, _I, _P, _P, _I, _P]
+        cells = np.zeros(1, np.int64)
+        n = f(_ptr(s1), _ptr(s2), len(s1), len(s2), wptr, len(cats), cats.ctypes.data, vals.ctypes.data, w,
+              int(two_piece), int(cigar), out.ctypes.data, cig.ctypes.data, cap, cells.ctypes.data)
+        cl = int(cells[0])
+    else:
+        assert folder is not None, "the reference reads its profiles from a score folder"
+        f = self._f("semiglobal_weighted")
+        f.argtypes = [_P, _P, _I, _I, _P, ctypes.c_char_p, _I, _I, _I, _P, _P, _I]
+        n = f(_ptr(s1), _ptr(s2), len(s1), len(s2), wptr, folder.encode(), w, int(two_piece), int(cigar),
+              out.ctypes.data, cig.ctypes.data, cap)
+        cl = None
+    assert n >= 0
+    return int(out[0]), int(out[1]), int(out[2]), [int(c) for c in cig[:n]], cl
+
+
+def _find_similar_weighted(self, s1, s2, wcat, profiles, two_piece=True, sc=(2, -3, -4, -2), window=38, step=8,
+                           min_score=40, w=10, xdrop=20, bandwidth=20000, pvalues=0.1, lam=0.382291, k=0.006662,
+                           folder=None, stats=None):
+    """findSimilarFragmentsForPairedSequence_wighted (two_piece) / _wighted_1gap:
+    a list of (start1, end1, start2, end2, score, pvalue, evalue, cigar)."""
+    s1, s2 = self._codes(s1), self._codes(s2)
+    wc = np.ascontiguousarray(wcat, np.int16)
+    assert wc.size == len(s1) + 2
+    cap, cig_cap = 4096, 1 << 22
+    ints = np.zeros(5 * cap, np.int32)
+    dbls = np.zeros(2 * cap, np.float64)
+    cig = np.zeros(cig_cap, np.uint32)
+    off = np.zeros(cap + 1, np.int64)
+    head = [_ptr(s1), len(s1), _ptr(s2), len(s2), *sc, window, step, min_score, w, xdrop, bandwidth, pvalues, lam, k,
+            wc.ctypes.data]
+    tail = [ints.ctypes.data, dbls.ctypes.data, cap, cig.ctypes.data, off.ctypes.data, cig_cap]
+    f = self._f("find_similar_weighted")
+    if self.prefix == "orc":
+        cats, vals = _profile_arrays(profiles)
+        f.argtypes = [_P, _I, _P, _I] + [_I] * 10 + [_D, _D, _D, _P, _I, _P, _P, _I, _P, _P, _I, _P, _P, _L, _P]
+        st = np.zeros(5, np.int64)
+        n = f(*head, len(cats), cats.ctypes.data, vals.ctypes.data, int(two_piece), *tail, st.ctypes.data)
+        if stats is not None:
+            stats.update(scan_cells=int(st[0]), xdrop_cells=int(st[1]), seeds=int(st[2]), linked=int(st[3]),
+                         semiglobal_cells=int(st[4]))
+    else:
+        assert folder is not None, "the reference reads its profiles from a score folder"
+        f.argtypes = [_P, _I, _P, _I] + [_I] * 10 + [_D, _D, _D, _P, ctypes.c_char_p, _I, _P, _P, _I, _P, _P, _L]
+        n = f(*head, folder.encode(), int(two_piece), *tail)
+    assert n >= 0, "capacity"
+    return [(*[int(v) for v in ints[5 * i:5 * i + 5]], float(dbls[2 * i]), float(dbls[2 * i + 1]),
+             [int(c) for c in cig[off[i]:off[i + 1]]]) for i in range(n)]
+
+
+def _score_profile(self, folder, category):
+    """The reference's Score(folder) getters for one category (reference only)."""
+    f = self._f("score_profile")
+    f.argtypes = [ctypes.c_char_p, ctypes.c_int16, _P]
+    v = np.zeros(7, np.int32)
+    assert f(folder.encode(), category, v.ctypes.data) == 0
+    return tuple(int(x) for x in v)
+
+
+def _gff_categories(self, gff, chroms, which, start, count):
+    """readGffFileWithEveryThing's categories of chromosome `which` at
+    [start, start + count) (reference only); chroms = [(name, size), ...]."""
+    f = self._f("gff_categories")
+    f.argtypes = [ctypes.c_char_p, _P, _P, _I, _I, _L, _L, _P]
+    names = (ctypes.c_char_p * len(chroms))(*[n.encode() for n, _ in chroms])
+    sizes = np.array([s for _, s in chroms], np.int32)
+    out = np.zeros(count, np.int16)
+    assert f(gff.encode(), ctypes.cast(names, _P), sizes.ctypes.data, len(chroms), which, start, count,
+             out.ctypes.data) == 0
+    return out
+
+
+Checker.semiglobal_weighted = _semiglobal_weighted
+Checker.find_similar_weighted = _find_similar_weighted
+Checker.score_profile = _score_profile
+Checker.gff_categories = _gff_categories
+
 
 def oracle():
     """The CPU restatement (always available once ``make -C oracle oracle`` ran)."""

@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <set>
 #include <thread>
 #include <vector>
@@ -268,6 +269,96 @@ SgResult semiglobal(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, 
         int32_t l = (i - bi) - (rowJ - bj);
         if (l < 0) l = -l;
         if (best - rowMax > zdrop + l * x2) break;
+        if (rowJ <= 0) break;
+        std::swap(M1, M2);
+        M1[0] = kOutOfBand;
+        M2[0] = kOutOfBand;
+    }
+    if (tb && cig) {
+        int32_t ii = bi, jj = bj;
+        trace(*tb, ii, jj, *cig);
+        while (ii > 0) { push_op(*cig, 2); --ii; }
+        while (jj > 0) { push_op(*cig, 1); --jj; }
+        std::reverse(cig->begin(), cig->end());
+    }
+    return SgResult{best, bi - 1, bj - 1};
+}
+
+// --------------------------------------------------------------------------
+// Weighted semi-global DP (ZDP): SemiGlobal(..., weights, Score&, ...),
+// sequenceAlignment3.cpp:1483-1706 (two-piece), and
+// SemiGlobal_single_gap_penalty, :1710-1890 (one piece).  Row i takes its
+// substitution scores, horizontal gap penalties and z-drop from the score
+// profile of weights[i-1] and its vertical gap penalties from weights[i];
+// weights[-1 .. L1] are read.
+// --------------------------------------------------------------------------
+struct Profile {
+    int32_t match, mismatch, o1, x1, o2, x2, zdrop;
+};
+
+struct Profiles {
+    std::map<int16_t, Profile> by_cat;
+    const Profile &operator[](int16_t c) const { return by_cat.at(c); }
+    // Score::getM: 5x5, N row and column 0 (model/Score.cpp:50-64)
+    static int32_t sub(const Profile &p, int8_t a, int8_t b) {
+        if (a == 2 || b == 2) return 0;
+        return a == b ? p.match : p.mismatch;
+    }
+};
+
+SgResult semiglobal_weighted(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, const int16_t *weights,
+                             const Profiles &P, int32_t w, bool two_piece, BandTrace *tb, std::vector<uint32_t> *cig,
+                             int64_t *cells = nullptr) {
+    if (L1 <= 0 || L2 <= 0) return SgResult{0, -2, -2};
+    std::vector<int32_t> M1(L2 + 1, kOutOfBand), M2(L2 + 1, kOutOfBand), F1(L2 + 1, kOutOfBand),
+        F2(L2 + 1, kOutOfBand);
+    M1[0] = 0; M2[0] = 0;
+    int32_t best = 0, bi = -1, bj = -1;
+    if (tb) tb->reset(L1 + 1);
+    for (int32_t i = 1; i <= L1; ++i) {
+        int32_t rowMax = 0, rowJ = two_piece ? kOutOfBand : -1;
+        const int32_t start = std::max(i - w, 1), end = std::min(i + w, L2);
+        int32_t e1 = kOutOfBand, e2 = kOutOfBand;
+        F1[end] = kOutOfBand;
+        F2[end] = kOutOfBand;
+        const Profile &cur = P[weights[i - 1]], &nxt = P[weights[i]];
+        if (tb) tb->open_row(i, start);
+        if (cells) *cells += (int64_t)std::max(0, end - start + 1);
+        for (int32_t j = start; j <= end; ++j) {
+            int32_t m = Profiles::sub(cur, s1[i - 1], s2[j - 1]) + M1[j - 1];
+            uint8_t d = 0;
+            if (!(m > F1[j])) { m = F1[j]; d = 1; }
+            if (!(m > e1)) { m = e1; d = 2; }
+            if (two_piece) {
+                if (!(m > F2[j])) { m = F2[j]; d = 3; }
+                if (!(m > e2)) { m = e2; d = 4; }
+            }
+            M2[j] = m;
+            if (m > rowMax) { rowMax = m; rowJ = j; }
+            int32_t h = m + nxt.o1, f = F1[j] + nxt.x1;  // vertical: the next row's profile
+            if (f >= h) d |= 8; else f = h;
+            F1[j] = f;
+            h = m + cur.o1;  // horizontal: this row's profile
+            e1 += cur.x1;
+            if (e1 >= h) d |= 16; else e1 = h;
+            if (two_piece) {
+                int32_t h2 = m + nxt.o2, f2 = F2[j] + nxt.x2;
+                if (f2 >= h2) d |= 32; else f2 = h2;
+                F2[j] = f2;
+                h2 = m + cur.o2;
+                e2 += cur.x2;
+                if (e2 >= h2) d |= 64; else e2 = h2;
+            }
+            if (tb) tb->set(i, j, d);
+        }
+        if (rowMax > best) { best = rowMax; bi = i; bj = rowJ; }
+        if (two_piece) {
+            int32_t l = (i - bi) - (rowJ - bj);
+            if (l < 0) l = -l;
+            if (best - rowMax > cur.zdrop + l * cur.x2) break;
+        } else if (best - rowMax > cur.zdrop) {
+            break;
+        }
         if (rowJ <= 0) break;
         std::swap(M1, M2);
         M1[0] = kOutOfBand;
@@ -592,21 +683,19 @@ int orc_find_similar(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t
     return n;
 }
 
-int orc_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
-                           int32_t mismatch, int32_t open, int32_t ext, int32_t open2, int32_t ext2,
-                           int32_t window, int32_t step, int32_t min_score, int32_t w, int32_t xdrop, int32_t zdrop,
-                           int32_t bandwidth, double pvalues, double lambda, double kvalue, int32_t *ints,
-                           double *dbls, int32_t cap, uint32_t *cig, int64_t *cig_off, int64_t cig_cap,
-                           int64_t *stats) {
-    SeedCtx c;
-    c.seq1 = seq1; c.seq2 = seq2; c.L1 = L1; c.L2 = L2; c.W = window; c.minScore = min_score;
-    c.minRef = min_score / match; c.sc = Scoring{match, mismatch}; c.open = open; c.ext = ext;
-    c.rc1.resize((size_t)L1 + 1); c.rc2.resize((size_t)L2 + 1);
-    for (int32_t i = 0; i < L1; ++i) c.rc1[i] = (int8_t)(4 - seq1[L1 - 1 - i]);
-    for (int32_t i = 0; i < L2; ++i) c.rc2[i] = (int8_t)(4 - seq2[L2 - 1 - i]);
-    // getAllExtendSeed window loop (:377-392): no sp > L2 guard; a window of
-    // non-positive width scores 0 (the SIMD SmithWaterman with an empty
-    // profile) and adds no seed
+namespace {
+
+// getAllExtendSeed (findSimilarFragmentsForPairedSequence4.cpp:348-429): the
+// window loop, link_seeds, the score-only x_extend_seed of every seed, the
+// p-value filter and the x-seed merge.
+std::vector<Seed> extend_seeds(SeedCtx &c, int32_t open, int32_t ext, int32_t step, int32_t w, int32_t xdrop,
+                               double pvalues, double lambda, double kvalue, int64_t &nseeds, int64_t &nlinked,
+                               int64_t &xcells) {
+    const int8_t *seq1 = c.seq1, *seq2 = c.seq2;
+    const int32_t L1 = c.L1, L2 = c.L2;
+    // window loop (:377-392): no sp > L2 guard; a window of non-positive
+    // width scores 0 (the SIMD SmithWaterman with an empty profile) and adds
+    // no seed
     std::set<Seed> seeds;
     {
         int32_t sp = 0;
@@ -619,10 +708,10 @@ int orc_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, i
             }
         }
     }
-    int64_t nseeds = (int64_t)seeds.size();
+    nseeds = (int64_t)seeds.size();
     link_seeds(seeds);
+    nlinked = (int64_t)seeds.size();
     const double L1d = (double)L1, L2d = (double)L2;
-    int64_t xcells = 0, sgcells = 0;
     // score-only x_extend_seed (:205-241), p filter and x-seed merge (:400-427)
     std::vector<Seed> xs;
     for (const Seed &sd : seeds) {
@@ -651,42 +740,42 @@ int orc_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, i
         }
         if (never_used) xs.push_back(Seed{start1, end1, start2, end2});
     }
-    // overload 2 (:532-616)
-    std::vector<Fragment> frags;
-    std::set<std::pair<int32_t, int32_t>> starts;
-    for (const Seed &x : xs) {
-        SgResult rv = semiglobal(c.rc1.data() + (L1 - 1 - x.end1), c.rc2.data() + (L2 - x.end2 - 1), 1 + x.end1,
-                                 x.end2 + 1, c.sc, open, ext, open2, ext2, zdrop, bandwidth, nullptr, nullptr,
-                                 &sgcells);
-        const int32_t start1 = x.end1 - rv.end1, start2 = x.end2 - rv.end2;
-        if (!starts.insert({start1, start2}).second) continue;
-        std::vector<uint32_t> cigar;
-        BandTrace T;
-        SgResult fw = semiglobal(seq1 + start1, seq2 + start2, L1 - start1, L2 - start2, c.sc, open, ext, open2, ext2,
-                                 zdrop, bandwidth, &T, &cigar, &sgcells);
-        if (!(cigar.size() > 0 && fw.end1 > 0 && fw.end2 > 0)) continue;
-        const int32_t end1 = fw.end1 + start1, end2 = fw.end2 + start2;
-        int32_t to_remove = -1;
-        bool found = false;
-        for (size_t i = 0; i < frags.size(); ++i) {
-            if (frags[i].end1 == (uint32_t)(end1 + 1) && frags[i].end2 == (uint32_t)(end2 + 1)) {
-                found = true;
-                if ((uint32_t)(start1 + 1) < frags[i].start1 && (uint32_t)(start2 + 1) < frags[i].start2)
-                    to_remove = (int32_t)i;
-            }
+    return xs;
+}
+
+void init_ctx(SeedCtx &c, const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
+              int32_t mismatch, int32_t open, int32_t ext, int32_t window, int32_t min_score) {
+    c.seq1 = seq1; c.seq2 = seq2; c.L1 = L1; c.L2 = L2; c.W = window; c.minScore = min_score;
+    c.minRef = min_score / match; c.sc = Scoring{match, mismatch}; c.open = open; c.ext = ext;
+    c.rc1.resize((size_t)L1 + 1); c.rc2.resize((size_t)L2 + 1);
+    for (int32_t i = 0; i < L1; ++i) c.rc1[i] = (int8_t)(4 - seq1[L1 - 1 - i]);
+    for (int32_t i = 0; i < L2; ++i) c.rc2[i] = (int8_t)(4 - seq2[L2 - 1 - i]);
+}
+
+// Overload 2's fragment replace / append (:584-611), p-values without a filter.
+void add_fragment(std::vector<Fragment> &frags, int32_t start1, int32_t start2, int32_t end1, int32_t end2,
+                  int32_t score, const std::vector<uint32_t> &cigar, double L1d, double L2d, double lambda,
+                  double kvalue) {
+    int32_t to_remove = -1;
+    bool found = false;
+    for (size_t i = 0; i < frags.size(); ++i) {
+        if (frags[i].end1 == (uint32_t)(end1 + 1) && frags[i].end2 == (uint32_t)(end2 + 1)) {
+            found = true;
+            if ((uint32_t)(start1 + 1) < frags[i].start1 && (uint32_t)(start2 + 1) < frags[i].start2)
+                to_remove = (int32_t)i;
         }
-        if (found && to_remove < 0) continue;
-        const double e = kvalue * L1d * L2d * std::exp(-1.0 * lambda * (double)fw.max);
-        const double p = 1.0 - std::exp(-1.0 * e);
-        Fragment nf{(uint32_t)(start1 + 1), (uint32_t)(end1 + 1), (uint32_t)(start2 + 1), (uint32_t)(end2 + 1),
-                    (uint32_t)fw.max, cigar, p, e};
-        if (to_remove >= 0) frags[(size_t)to_remove] = nf;
-        else frags.push_back(nf);
     }
-    if (stats) {
-        stats[0] = c.cells; stats[1] = xcells; stats[2] = nseeds; stats[3] = (int64_t)seeds.size();
-        stats[4] = sgcells;
-    }
+    if (found && to_remove < 0) return;
+    const double e = kvalue * L1d * L2d * std::exp(-1.0 * lambda * (double)score);
+    const double p = 1.0 - std::exp(-1.0 * e);
+    Fragment nf{(uint32_t)(start1 + 1), (uint32_t)(end1 + 1), (uint32_t)(start2 + 1), (uint32_t)(end2 + 1),
+                (uint32_t)score, cigar, p, e};
+    if (to_remove >= 0) frags[(size_t)to_remove] = nf;
+    else frags.push_back(nf);
+}
+
+int emit(const std::vector<Fragment> &frags, int32_t *ints, double *dbls, int32_t cap, uint32_t *cig,
+         int64_t *cig_off, int64_t cig_cap) {
     int32_t n = (int32_t)frags.size();
     int64_t total = 0;
     for (auto &g : frags) total += (int64_t)g.cigar.size();
@@ -702,6 +791,110 @@ int orc_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, i
     }
     cig_off[n] = o;
     return n;
+}
+
+}  // namespace
+
+int orc_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
+                           int32_t mismatch, int32_t open, int32_t ext, int32_t open2, int32_t ext2,
+                           int32_t window, int32_t step, int32_t min_score, int32_t w, int32_t xdrop, int32_t zdrop,
+                           int32_t bandwidth, double pvalues, double lambda, double kvalue, int32_t *ints,
+                           double *dbls, int32_t cap, uint32_t *cig, int64_t *cig_off, int64_t cig_cap,
+                           int64_t *stats) {
+    SeedCtx c;
+    init_ctx(c, seq1, L1, seq2, L2, match, mismatch, open, ext, window, min_score);
+    int64_t nseeds = 0, nlinked = 0, xcells = 0, sgcells = 0;
+    std::vector<Seed> xs = extend_seeds(c, open, ext, step, w, xdrop, pvalues, lambda, kvalue, nseeds, nlinked, xcells);
+    const double L1d = (double)L1, L2d = (double)L2;
+    // overload 2 (:532-616)
+    std::vector<Fragment> frags;
+    std::set<std::pair<int32_t, int32_t>> starts;
+    for (const Seed &x : xs) {
+        SgResult rv = semiglobal(c.rc1.data() + (L1 - 1 - x.end1), c.rc2.data() + (L2 - x.end2 - 1), 1 + x.end1,
+                                 x.end2 + 1, c.sc, open, ext, open2, ext2, zdrop, bandwidth, nullptr, nullptr,
+                                 &sgcells);
+        const int32_t start1 = x.end1 - rv.end1, start2 = x.end2 - rv.end2;
+        if (!starts.insert({start1, start2}).second) continue;
+        std::vector<uint32_t> cigar;
+        BandTrace T;
+        SgResult fw = semiglobal(seq1 + start1, seq2 + start2, L1 - start1, L2 - start2, c.sc, open, ext, open2, ext2,
+                                 zdrop, bandwidth, &T, &cigar, &sgcells);
+        if (!(cigar.size() > 0 && fw.end1 > 0 && fw.end2 > 0)) continue;
+        add_fragment(frags, start1, start2, fw.end1 + start1, fw.end2 + start2, fw.max, cigar, L1d, L2d, lambda,
+                     kvalue);
+    }
+    if (stats) {
+        stats[0] = c.cells; stats[1] = xcells; stats[2] = nseeds; stats[3] = nlinked;
+        stats[4] = sgcells;
+    }
+    return emit(frags, ints, dbls, cap, cig, cig_off, cig_cap);
+}
+
+int orc_semiglobal_weighted(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, const int16_t *weights,
+                            int32_t nprof, const int16_t *pcat, const int32_t *pval, int32_t w, int32_t two_piece,
+                            int32_t return_cigar, int32_t *out3, uint32_t *cigar, int32_t cap, int64_t *cells) {
+    Profiles P;
+    for (int32_t i = 0; i < nprof; ++i) {
+        const int32_t *v = pval + 7 * i;
+        P.by_cat[pcat[i]] = Profile{v[0], v[1], v[2], v[3], v[4], v[5], v[6]};
+    }
+    BandTrace T;
+    std::vector<uint32_t> cig;
+    int64_t cl = 0;
+    SgResult r = semiglobal_weighted(s1, s2, L1, L2, weights, P, w, two_piece != 0, return_cigar ? &T : nullptr,
+                                     return_cigar ? &cig : nullptr, &cl);
+    out3[0] = r.max; out3[1] = r.end1; out3[2] = r.end2;
+    if (cells) *cells = cl;
+    if (!return_cigar) return 0;
+    if ((int32_t)cig.size() > cap) return -1;
+    for (size_t i = 0; i < cig.size(); ++i) cigar[i] = cig[i];
+    return (int32_t)cig.size();
+}
+
+int orc_find_similar_weighted(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
+                              int32_t mismatch, int32_t open, int32_t ext, int32_t window, int32_t step,
+                              int32_t min_score, int32_t w, int32_t xdrop, int32_t bandwidth, double pvalues,
+                              double lambda, double kvalue, const int16_t *wcat, int32_t nprof, const int16_t *pcat,
+                              const int32_t *pval, int32_t two_piece, int32_t *ints, double *dbls, int32_t cap,
+                              uint32_t *cig, int64_t *cig_off, int64_t cig_cap, int64_t *stats) {
+    Profiles P;
+    for (int32_t i = 0; i < nprof; ++i) {
+        const int32_t *v = pval + 7 * i;
+        P.by_cat[pcat[i]] = Profile{v[0], v[1], v[2], v[3], v[4], v[5], v[6]};
+    }
+    SeedCtx c;
+    init_ctx(c, seq1, L1, seq2, L2, match, mismatch, open, ext, window, min_score);
+    int64_t nseeds = 0, nlinked = 0, xcells = 0, sgcells = 0;
+    std::vector<Seed> xs = extend_seeds(c, open, ext, step, w, xdrop, pvalues, lambda, kvalue, nseeds, nlinked, xcells);
+    // weights of the region and of its reverse complement: W[x] = wcat[x + 1]
+    // for x = -1 .. L1; the reversed array holds W[L1 - 1 - k] at k
+    const int16_t *Wf = wcat + 1;
+    std::vector<int16_t> rev((size_t)L1 + 2);
+    for (int32_t k = -1; k <= L1; ++k) rev[(size_t)(k + 1)] = Wf[L1 - 1 - k];
+    const int16_t *Wr = rev.data() + 1;
+    const double L1d = (double)L1, L2d = (double)L2;
+    // findSimilarFragmentsForPairedSequence_wighted (:830-925) and _wighted_1gap (:779-826)
+    std::vector<Fragment> frags;
+    std::set<std::pair<int32_t, int32_t>> starts;
+    for (const Seed &x : xs) {
+        SgResult rv = semiglobal_weighted(c.rc1.data() + (L1 - 1 - x.end1), c.rc2.data() + (L2 - x.end2 - 1),
+                                          1 + x.end1, x.end2 + 1, Wr + (L1 - 1 - x.end1), P, bandwidth,
+                                          two_piece != 0, nullptr, nullptr, &sgcells);
+        const int32_t start1 = x.end1 - rv.end1, start2 = x.end2 - rv.end2;
+        if (!starts.insert({start1, start2}).second) continue;
+        std::vector<uint32_t> cigar;
+        BandTrace T;
+        SgResult fw = semiglobal_weighted(seq1 + start1, seq2 + start2, L1 - start1, L2 - start2, Wf + start1, P,
+                                          bandwidth, two_piece != 0, &T, &cigar, &sgcells);
+        if (two_piece && !(cigar.size() > 0 && fw.end1 > 0 && fw.end2 > 0)) continue;
+        add_fragment(frags, start1, start2, fw.end1 + start1, fw.end2 + start2, fw.max, cigar, L1d, L2d, lambda,
+                     kvalue);
+    }
+    if (stats) {
+        stats[0] = c.cells; stats[1] = xcells; stats[2] = nseeds; stats[3] = nlinked;
+        stats[4] = sgcells;
+    }
+    return emit(frags, ints, dbls, cap, cig, cig_off, cig_cap);
 }
 
 int orc_sw_max_batch(const int8_t *seqs, const int64_t *ref_off, const int32_t *ref_len, const int64_t *qry_off,

@@ -20,6 +20,10 @@
 //   orc_seed_set              golden vectors + live _ref comparison.
 //   orc_find_similar          golden vectors + live _ref comparison, closed
 //                             form of the p-value.
+//   orc_semiglobal_weighted,  golden vectors + live _ref comparison; with one
+//   orc_find_similar_weighted profile for every category the two-piece form
+//                             equals SemiGlobal's recurrence except at the
+//                             band's last column (tested).
 #pragma once
 #include <cstdint>
 
@@ -79,6 +83,33 @@ int orc_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, i
                            int32_t bandwidth, double pvalues, double lambda, double kvalue, int32_t *ints,
                            double *dbls, int32_t cap, uint32_t *cig, int64_t *cig_off, int64_t cig_cap,
                            int64_t *stats);
+
+// Weighted semi-global DP: SemiGlobal(..., const int16_t *weights, Score &, ...),
+// sequenceAlignment3.cpp:1483-1706 (two_piece = 1), or
+// SemiGlobal_single_gap_penalty, :1710-1890 (two_piece = 0).  weights[-1 ..
+// L1] must be readable (row i uses the score profile of weights[i-1] and the
+// vertical gap penalties of weights[i]).  Profiles: nprof categories pcat[k]
+// with pval[7k..7k+6] = {match, mismatch, open1, ext1, open2, ext2, zdrop}
+// (the score-folder files, model/Score.cpp:9-71).  out3 = {max, end1, end2}
+// (0-based); returns the cigar length (traceback) or 0; cells (may be null)
+// receives the evaluated band cells.
+int orc_semiglobal_weighted(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, const int16_t *weights,
+                            int32_t nprof, const int16_t *pcat, const int32_t *pval, int32_t w, int32_t two_piece,
+                            int32_t return_cigar, int32_t *out3, uint32_t *cigar, int32_t cap, int64_t *cells);
+
+// findSimilarFragmentsForPairedSequence_wighted (two_piece = 1,
+// findSimilarFragmentsForPairedSequence4.cpp:830-925) and
+// findSimilarFragmentsForPairedSequence_wighted_1gap (two_piece = 0, :779-826):
+// getAllExtendSeed with the fixed scoring, then per x-seed the weighted
+// reverse and forward DP.  wcat holds the categories of reference positions
+// -1 .. L1 (L1 + 2 values; the reference reads one base past each end of the
+// region).  Output layout and stats as orc_find_similar_2gaps.
+int orc_find_similar_weighted(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
+                              int32_t mismatch, int32_t open, int32_t ext, int32_t window, int32_t step,
+                              int32_t min_score, int32_t w, int32_t xdrop, int32_t bandwidth, double pvalues,
+                              double lambda, double kvalue, const int16_t *wcat, int32_t nprof, const int16_t *pcat,
+                              const int32_t *pval, int32_t two_piece, int32_t *ints, double *dbls, int32_t cap,
+                              uint32_t *cig, int64_t *cig_off, int64_t cig_cap, int64_t *stats);
 
 // Max local score of many pairs (the ranSco inner call,
 // permutationLsqLambdaK.cpp:97).  seqs is one buffer; offsets/lengths per pair.

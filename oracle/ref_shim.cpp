@@ -14,9 +14,17 @@
 //   ref_seed_set      -> getSeeds window loop + link_seeds     impl/findSimilarFragmentsForPairedSequence4.cpp:131,288,461-483
 //   ref_find_similar  -> findSimilarFragmentsForPairedSequence impl/findSimilarFragmentsForPairedSequence4.cpp:433
 //   ref_find_similar_2gaps -> findSimilarFragmentsForPairedSequence (overload 4) :752
+//   ref_semiglobal_weighted -> SemiGlobal(..., weights, Score&) / SemiGlobal_single_gap_penalty
+//                                                             impl/sequenceAlignment3.cpp:1483,1710
+//   ref_find_similar_weighted -> findSimilarFragmentsForPairedSequence_wighted(_1gap)
+//                                                             impl/findSimilarFragmentsForPairedSequence4.cpp:779,830
+//   ref_score_profile -> Score(folder) getters                model/Score.cpp:9-104
+//   ref_gff_categories -> readGffFileWithEveryThing           impl/gffToCategory.cpp:81-238
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <set>
+#include <string>
 #include <vector>
 #include "impl/impl.h"
 #include "model/model.h"
@@ -209,6 +217,101 @@ int ref_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, i
     }
     cig_off[n] = o;
     return n;
+}
+
+// Score(folder) and its getters for one category: vals7 = {match, mismatch,
+// open1, ext1, open2, ext2, zdrop}.
+int ref_score_profile(const char *folder, int16_t category, int32_t *vals7) {
+    Score score{std::string(folder)};
+    int32_t **m = score.getM(category);
+    if (!m) return -1;
+    vals7[0] = m[0][0];
+    vals7[1] = m[0][1];
+    vals7[2] = score.getOpenGapPenalty1(category);
+    vals7[3] = score.getExtendGapPenalty1(category);
+    vals7[4] = score.getOpenGapPenalty2(category);
+    vals7[5] = score.getExtendGapPenalty2(category);
+    vals7[6] = score.getZdrop(category);
+    return 0;
+}
+
+// Weighted SemiGlobal on one task; weights[-1 .. L1] readable.  The score
+// profiles come from the folder.  out3 = {max, end1, end2}; returns the cigar
+// length (return_cigar) or 0, -1 when cap is too small.
+int ref_semiglobal_weighted(const int8_t *s1, const int8_t *s2, int32_t L1, int32_t L2, const int16_t *weights,
+                            const char *folder, int32_t w, int32_t two_piece, int32_t return_cigar, int32_t *out3,
+                            uint32_t *cigar, int32_t cap) {
+    Score score{std::string(folder)};
+    std::vector<int8_t> a = copy_of(s1, L1), b = copy_of(s2, L2);
+    int32_t l1 = L1, l2 = L2, mx = 0, e1 = -1, e2 = -1, z = 0, ww = w;
+    Matrix T(L1 + 1, L2 + 1);
+    std::vector<uint32_t> cig =
+        two_piece ? SemiGlobal(a.data(), b.data(), l1, l2, weights, score, mx, e1, e2, return_cigar != 0, z, ww, T)
+                  : SemiGlobal_single_gap_penalty(a.data(), b.data(), l1, l2, weights, score, mx, e1, e2,
+                                                  return_cigar != 0, z, ww, T);
+    out3[0] = mx; out3[1] = e1; out3[2] = e2;
+    if (!return_cigar) return 0;
+    if ((int32_t)cig.size() > cap) return -1;
+    for (size_t i = 0; i < cig.size(); ++i) cigar[i] = cig[i];
+    return (int32_t)cig.size();
+}
+
+// findSimilarFragmentsForPairedSequence_wighted (two_piece = 1) or _1gap.
+// wcat: categories of reference positions -1 .. L1 (L1 + 2 values), i.e. the
+// reference's weight pointer at [-1 .. L1]; the reversed array is built as
+// readGffFileWithEveryThing builds categories_rev.  Layout as ref_find_similar.
+int ref_find_similar_weighted(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, int32_t match,
+                              int32_t mismatch, int32_t open, int32_t ext, int32_t window, int32_t step,
+                              int32_t min_score, int32_t w, int32_t xdrop, int32_t bandwidth, double pvalues,
+                              double lambda, double kvalue, const int16_t *wcat, const char *folder,
+                              int32_t two_piece, int32_t *ints, double *dbls, int32_t cap, uint32_t *cig,
+                              int64_t *cig_off, int64_t cig_cap) {
+    Scorei m((int8_t)match, (int8_t)mismatch);
+    Score score{std::string(folder)};
+    std::vector<int8_t> a = copy_of(seq1, L1), b = copy_of(seq2, L2);
+    std::vector<int8_t> ar = rc_of(seq1, L1), br = rc_of(seq2, L2);
+    std::vector<int16_t> fwd(wcat, wcat + L1 + 2), rev((size_t)L1 + 2);
+    for (int32_t k = -1; k <= L1; ++k) rev[(size_t)(k + 1)] = wcat[(L1 - 1 - k) + 1];
+    std::string s1str, s2str;
+    int32_t l1 = L1, l2 = L2, win = window, mbd = 0, ww = w, zd = 0;
+    std::vector<PairedSimilarFragment> f =
+        two_piece ? findSimilarFragmentsForPairedSequence_wighted(
+                        a.data(), ar.data(), b.data(), br.data(), l1, l2, win, min_score, mbd, open, ext, match,
+                        mismatch, m, step, s1str, s2str, pvalues, lambda, kvalue, zd, bandwidth, ww, xdrop, score,
+                        fwd.data() + 1, rev.data() + 1)
+                  : findSimilarFragmentsForPairedSequence_wighted_1gap(
+                        a.data(), ar.data(), b.data(), br.data(), l1, l2, win, min_score, mbd, open, ext, match,
+                        mismatch, m, step, s1str, s2str, pvalues, lambda, kvalue, zd, bandwidth, ww, xdrop, score,
+                        fwd.data() + 1, rev.data() + 1);
+    int32_t n = (int32_t)f.size();
+    int64_t total = 0;
+    for (auto &x : f) total += (int64_t)x.getCigar().size();
+    if (n > cap || total > cig_cap) return -n - 1;
+    int64_t o = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        ints[5 * i + 0] = (int32_t)f[i].getStart1(); ints[5 * i + 1] = (int32_t)f[i].getEnd1();
+        ints[5 * i + 2] = (int32_t)f[i].getStart2(); ints[5 * i + 3] = (int32_t)f[i].getEnd2();
+        ints[5 * i + 4] = (int32_t)f[i].getScore();
+        dbls[2 * i + 0] = f[i].getPValue(); dbls[2 * i + 1] = f[i].getEValue();
+        cig_off[i] = o;
+        for (uint32_t c : f[i].getCigar()) cig[o++] = c;
+    }
+    cig_off[n] = o;
+    return n;
+}
+
+// readGffFileWithEveryThing over chromosomes (names, sizes); copies the
+// categories of chromosome `which` at [from, from + count) into out.
+int ref_gff_categories(const char *gff, const char *const *names, const int32_t *sizes, int32_t nchr, int32_t which,
+                       int64_t from, int64_t count, int16_t *out) {
+    std::map<std::string, int32_t> chrSize;
+    for (int32_t i = 0; i < nchr; ++i) chrSize[names[i]] = sizes[i];
+    std::map<std::string, int16_t *> cats;
+    readGffFileWithEveryThing(std::string(gff), chrSize, cats);
+    const int16_t *c = cats[names[which]];
+    for (int64_t k = 0; k < count; ++k) out[k] = c[from + k];
+    for (auto &kv : cats) delete[] kv.second;
+    return 0;
 }
 
 }  // extern "C"

@@ -8,20 +8,24 @@ Every fixture holds inputs (sequences as A/C/N/G/T text, parameters) and the
 outputs of the reference build (oracle/_ref) on them: K1 SmithWaterman
 (striped and textbook paths), SemiGlobal_xextend with and without traceback,
 the two-piece SemiGlobal, the seed set before/after link_seeds, the fragments
-of findSimilarFragmentsForPairedSequence (overloads 1 and 4), and the text
-outputs of the cut1Gap, cut2Gaps, pairCns2Gaps and ranSco commands.  Only
-data is written.
+of findSimilarFragmentsForPairedSequence (overloads 1 and 4), the weighted
+paths (score profiles, weighted SemiGlobal tasks, the _wighted / _wighted_1gap
+overloads, GFF categories), and the text outputs of the cut1Gap, cut2Gaps,
+pairCnsXExtend, ranSco, weighted1Gap and weighted2Gaps commands.  Only data is
+written.  ``--weighted`` writes the weighted fixtures only.
 """
 import json
 import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
+REF_ROOT = os.environ.get("DCNS_REFERENCE", "/root/reference")
 sys.path.insert(0, ROOT)
 
 from dcns_amd import synth  # noqa: E402
@@ -135,6 +139,7 @@ def main():
         tag = "_".join(a.strip("-") for a in args)
         open(os.path.join(d, f"expected_{tag}.txt"), "w").write(r.stdout)
     write_two_piece(HERE, ref)
+    write_weighted(HERE, ref)
     print("golden fixtures written to", HERE)
 
 
@@ -176,6 +181,158 @@ def write_two_piece(here, ref):
         open(os.path.join(d, f"expected_stdout_{tag}.txt"), "w").write(r.stdout)
     for f in sorted(os.listdir(d)):
         if f.startswith(("c2_", "c2m_", "px_", "pxm_")):
+            os.replace(os.path.join(d, f), os.path.join(d, "expected_" + f))
+
+
+# the reference's scoreMatrix folder (scoreMatrix/0 .. 100), one profile per
+# category: match, mismatch, open1, ext1, open2, ext2, zdrop
+REF_SCORE_DIR = os.path.join(REF_ROOT, "scoreMatrix")
+# a second set: smaller scores, a non-zero second extend (the z-drop's l term)
+ALT_PROFILES = {0: (0, 0, 0, 0, 0, 0, 1073741824), 1: (1, -1, -2, -1, -15, -1, 40), 10: (2, -3, -4, -2, -30, -1, 60),
+                70: (3, -4, -6, -3, -40, -1, 80), 85: (3, -4, -6, -2, -40, 0, 90), 100: (4, -5, -8, -3, -50, -1, 120)}
+
+
+def category_runs(rng, n, cats):
+    """Category array of n values made of runs (genes, exons, repeats)."""
+    wc = np.full(n, 10, np.int16)
+    for _ in range(max(1, n // 300)):
+        s = int(rng.integers(0, n))
+        e = min(n, s + int(rng.integers(20, 900)))
+        wc[s:e] = rng.choice(cats)
+    return wc
+
+
+def write_weighted(here, ref):
+    """The weighted (ZDP) paths: score profiles, single DP tasks, overloads
+    _wighted / _wighted_1gap, GFF categories, and the weighted1Gap /
+    weighted2Gaps commands."""
+    rng = np.random.default_rng(20261205)
+    cats = [0, 1, 10, 70, 85, 100]
+    default = {c: ref.score_profile(REF_SCORE_DIR, c) for c in cats}
+    alt_dir = bind.write_score_folder(tempfile.mkdtemp(), ALT_PROFILES)
+    sets = {"default": (default, REF_SCORE_DIR), "alt": (ALT_PROFILES, alt_dir)}
+    out = {"profiles": {k: {str(c): list(v) for c, v in p.items()} for k, (p, _) in sets.items()}, "tasks": [],
+           "pairs": []}
+    for t in range(96):
+        name = "default" if t % 2 else "alt"
+        prof, folder = sets[name]
+        L1, L2 = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+        a = rng.choice([0, 1, 2, 3, 4], L1, p=[.24, .24, .04, .24, .24]).astype(np.int8)
+        b = (synth.mutate(rng, a, 0.15, 0.04)[:L2] if t % 4 else
+             rng.choice([0, 1, 3, 4], L2).astype(np.int8))
+        if len(b) == 0:
+            continue
+        wc = category_runs(rng, L1 + 2, cats)
+        w = int(rng.choice([2, 3, 7, 40, 300, 20000]))
+        tp = bool(t % 3)
+        mx, e1, e2, cig, _ = ref.semiglobal_weighted(a, b, wc, prof, w=w, two_piece=tp, folder=folder)
+        mx2, f1, f2, _, _ = ref.semiglobal_weighted(a, b, wc, prof, w=w, two_piece=tp, cigar=False, folder=folder)
+        out["tasks"].append({"s1": txt(a), "s2": txt(b), "wcat": [int(x) for x in wc], "profiles": name, "w": w,
+                             "two_piece": tp, "out": [mx, e1, e2], "cigar": cig, "out_score_only": [mx2, f1, f2]})
+    for t in range(8):
+        name = "default" if t % 2 else "alt"
+        prof, folder = sets[name]
+        L1, L2 = int(rng.integers(800, 3500)), int(rng.integers(500, 3000))
+        a, b = synth.one_pair(seed=9500 + t, L1=L1, L2=L2, nfrag=int(rng.integers(2, 12)))
+        wc = category_runs(rng, L1 + 2, cats)
+        for tp in (True, False):
+            kw = dict(bandwidth=int(rng.choice([20000, 300, 40])))
+            out["pairs"].append({"s1": txt(a), "s2": txt(b), "wcat": [int(x) for x in wc], "profiles": name,
+                                 "two_piece": tp, "params": kw,
+                                 "fragments": ref.find_similar_weighted(a, b, wc, prof, two_piece=tp, folder=folder,
+                                                                        **kw)})
+    # GFF categories of a synthetic annotation
+    chroms = [("c1", 30000), ("c2", 12000)]
+    gff = write_gff(os.path.join(here, "weighted_annotation.gff3"), rng, chroms)
+    out["gff"] = {"file": "weighted_annotation.gff3", "chroms": chroms,
+                  "categories": {n: [int(x) for x in ref.gff_categories(gff, chroms, k, 0, size)]
+                                 for k, (n, size) in enumerate(chroms)}}
+    json.dump(out, open(os.path.join(here, "weighted.json"), "w"))
+    write_weighted_commands(here, rng, default)
+
+
+def write_gff(path, rng, chroms):
+    """A GFF3 with every feature class the category painter knows, nested and
+    overlapping records, duplicates, reversed coordinates, comments, an
+    ignored type and one unknown type."""
+    types_gene = ["gene", "mRNA", "transcript", "ncRNA_gene", "lnc_RNA"]
+    types_exon = ["exon", "five_prime_UTR", "three_prime_utr"]
+    types_cds = ["CDS", "start_codon", "stop_codon"]
+    types_te = ["transposable_element", "LTR_retrotransposon", "helitron", "repeat_region"]
+    lines = ["##gff-version 3"]
+    for name, size in chroms:
+        lines.append(f"{name}\tsrc\tchromosome\t1\t{size}\t.\t.\t.\tID={name}")
+        pos = int(rng.integers(100, 900))
+        while pos < size - 3000:
+            glen = int(rng.integers(800, 2800))
+            g0, g1 = pos, pos + glen
+            lines.append(f"{name}\tsrc\t{rng.choice(types_gene)}\t{g0}\t{g1}\t.\t+\t.\tID=g{pos}")
+            lines.append(f"{name}\tsrc\tmRNA\t{g0}\t{g1}\t.\t+\t.\tID=t{pos}")  # duplicate extent
+            e = g0
+            while e < g1 - 100:
+                el = int(rng.integers(60, 400))
+                ee = min(g1, e + el)
+                a_, b_ = (e, ee) if rng.random() < 0.9 else (ee, e)  # some reversed coordinates
+                lines.append(f"{name}\tsrc\t{rng.choice(types_exon)}\t{a_}\t{b_}\t.\t+\t.\tParent=t{pos}")
+                if rng.random() < 0.7:
+                    c0 = e + int(rng.integers(0, 30))
+                    lines.append(f"{name}\tsrc\t{rng.choice(types_cds)}\t{c0}\t{max(c0, ee - 20)}\t.\t+\t0\tParent=t{pos}")
+                if rng.random() < 0.2:  # overlapping exon record
+                    lines.append(f"{name}\tsrc\texon\t{e + 10}\t{ee + 50}\t.\t+\t.\tParent=t{pos}")
+                e = ee + int(rng.integers(50, 300))
+            if rng.random() < 0.5:
+                t0 = g1 + int(rng.integers(50, 500))
+                lines.append(f"{name}\tsrc\t{rng.choice(types_te)}\t{t0}\t{t0 + int(rng.integers(100, 900))}\t.\t.\t.\tID=te{pos}")
+            pos = g1 + int(rng.integers(300, 2500))
+        lines.append(f"# comment line on {name}")
+    lines.append(f"{chroms[0][0]}\tsrc\tcontig\t1\t500\t.\t.\t.\tID=ctg")
+    lines.append(f"{chroms[0][0]}\tsrc\tmystery_feature\t100\t200\t.\t.\t.\tID=x")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return path
+
+
+def write_weighted_commands(here, rng, default):
+    """weighted1Gap / weighted2Gaps on a reference region annotated by a GFF,
+    with the reference's own scoreMatrix folder and the alternative one."""
+    d = os.path.join(here, "weighted_cmd")
+    shutil.rmtree(d, ignore_errors=True)
+    os.makedirs(d)
+    chrom = synth.random_seq(rng, 16000)
+    chroms = [("cr", len(chrom)), ("cz", 5000)]
+    fasta_with_fai(os.path.join(d, "refgenome.fa"), [("cr", synth.to_text(chrom)),
+                                                     ("cz", synth.to_text(synth.random_seq(rng, 5000)))])
+    write_gff(os.path.join(d, "ann.gff3"), rng, chroms)
+    bind.write_score_folder(os.path.join(d, "alt_scores"), ALT_PROFILES)
+    r0, r1 = 3001, 7400  # the reference region (1-based, inclusive)
+    region = chrom[r0 - 1:r1]
+    qs = []
+    for k in range(3):
+        q = synth.random_seq(rng, int(rng.integers(1500, 3200)))
+        for _ in range(5):
+            ln = int(rng.integers(40, 220))
+            p1 = int(rng.integers(0, len(region) - ln))
+            p2 = int(rng.integers(0, len(q) - ln))
+            m = synth.mutate(rng, region[p1:p1 + ln], 0.12, 0.02)[:ln]
+            q[p2:p2 + len(m)] = m
+        qs.append(q)
+    with open(os.path.join(d, "in.fa"), "w") as fh:
+        fh.write(f">ref\tspecies:spA\tchr:cr\tstrand:1\tstart:{r0}\tend:{r1}\n{synth.to_text(region)}\n")
+        for k, (q, st) in enumerate(zip(qs, (1, 0, 1))):
+            fh.write(f">q{k}\tspecies:sp{'BCD'[k]}\tchr:cq{k}\tstrand:{st}\tstart:{21 + k}\tend:{20 + k + len(q)}\n")
+            fh.write(synth.to_text(q) + "\n")
+    bind.write_score_folder(os.path.join(d, "ref_scores"), default)
+    runs = [("weighted1Gap", ["-sp", "ref_scores", "-o", "w1_"]),
+            ("weighted2Gaps", ["-sp", "ref_scores", "-o", "w2_"]),
+            ("weighted1Gap", ["-sp", "alt_scores", "-o", "w1a_", "-O1", "-3", "-E1", "-1"]),
+            ("weighted2Gaps", ["-sp", "alt_scores", "-o", "w2a_", "-u2", "300"])]
+    for cmd, args in runs:
+        full = ["-i", "in.fa", "-r", "ref", "-ra", "refgenome.fa", "-f", "ann.gff3"] + args
+        r = subprocess.run([bind.REF_CLI, cmd] + full, cwd=d, capture_output=True, text=True)
+        tag = args[args.index("-o") + 1]
+        open(os.path.join(d, f"expected_stdout_{tag}.txt"), "w").write(r.stdout)
+    for f in sorted(os.listdir(d)):
+        if f.startswith(("w1_", "w2_", "w1a_", "w2a_")):
             os.replace(os.path.join(d, f), os.path.join(d, "expected_" + f))
 
 
@@ -264,4 +421,10 @@ def write_cut1gap_inputs(d, rng):
 
 
 if __name__ == "__main__":
-    main()
+    if "--weighted" in sys.argv[1:]:
+        r = bind.reference()
+        if r is None:
+            sys.exit("oracle/_ref is not built")
+        write_weighted(HERE, r)
+    else:
+        main()
