@@ -240,6 +240,63 @@ def two_piece_phase(ctx, log, cpu=True):
     return out
 
 
+def weighted_phase(ctx, log, cpu=True):
+    """The weighted paths (findSimilarFragmentsForPairedSequence_wighted,
+    weighted2Gaps, and _wighted_1gap, weighted1Gap) on the 10 kb x 10 kb
+    planted pair with GFF-like base classes and the reference's score
+    profiles, beside the compiled reference on one core; then 64 pairs of the
+    batch distribution in one call."""
+    from dcns_amd import synth
+    from oracle import bind
+    rng = np.random.default_rng(synth.SEED_BASE + 5)
+    prof = synth.WEIGHTED_PROFILES
+    a, b = synth.one_pair()[:2]
+    wc = synth.gene_categories(rng, len(a) + 2)
+    out = {"workload": "10,000 x 10,000 pair, 20 planted fragments, GFF-like base classes, the reference's "
+                       "scoreMatrix profiles, band 20000"}
+    ref = bind.reference() if cpu else None
+    folder = None
+    if ref is not None:
+        import tempfile
+        folder = bind.write_score_folder(tempfile.mkdtemp(), prof)
+    for tp, name in ((True, "weighted2Gaps"), (False, "weighted1Gap")):
+        ctx.find_similar_weighted([(a, b)], [wc], prof, two_piece=tp)
+        reps = 3
+        t = time.perf_counter()
+        for _ in range(reps):
+            res = ctx.find_similar_weighted([(a, b)], [wc], prof, two_piece=tp)
+        dt = (time.perf_counter() - t) / reps
+        st = ctx.stats()
+        cells = st["scan_cells"] + st["xdrop_cells"] + st["semiglobal_cells"]
+        r = {"reference_cells": cells, "weighted_cells": st["semiglobal_cells"], "seconds": round(dt, 4),
+             "gcups": round(cells / dt / 1e9, 2), "fragments": len(res[0][1])}
+        if ref is not None:
+            t = time.perf_counter()
+            want = ref.find_similar_weighted(a, b, wc, prof, two_piece=tp, folder=folder)
+            tc = time.perf_counter() - t
+            r["cpu_reference"] = {"seconds": round(tc, 3), "gcups": round(cells / tc / 1e9, 4), "cores": 1}
+            r["bit_exact"] = [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in res[0][1]] == \
+                             [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in want]
+        out[name] = r
+        log(f"{name}: {cells:.3e} cells in {dt * 1e3:.1f} ms ({cells / dt / 1e9:.1f} GCUPS)")
+    pairs = synth.batch(npairs=64, seed=123)
+    wcs = [synth.gene_categories(rng, len(x) + 2) for x, _ in pairs]
+    ctx.find_similar_weighted(pairs, wcs, prof, two_piece=True)
+    ctx.kernel_time_reset()
+    t = time.perf_counter()
+    ctx.find_similar_weighted(pairs, wcs, prof, two_piece=True)
+    dt = time.perf_counter() - t
+    st = ctx.stats()
+    dp_ms, _ = ctx.kernel_time("semiglobal")
+    cells = st["scan_cells"] + st["xdrop_cells"] + st["semiglobal_cells"]
+    out["batch64_weighted2Gaps"] = {"reference_cells": cells, "weighted_cells": st["semiglobal_cells"],
+                                    "seconds": round(dt, 4), "gcups": round(cells / dt / 1e9, 2),
+                                    "k_sg_strip_ms": round(dp_ms, 2),
+                                    "k_sg_strip_gcups": round(st["semiglobal_cells"] / dp_ms / 1e6, 1)}
+    log(f"weighted2Gaps batch of 64: {cells:.3e} cells in {dt * 1e3:.1f} ms")
+    return out
+
+
 def config2_significance(ctx, sur, log):
     """Config-2 fragments (500 kb masked pair, default scoring) reported with
     the lambda/K of the judge, of the surrogate and of the reference's
@@ -297,8 +354,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-lambdak", action="store_true")
     ap.add_argument("--no-two-piece", action="store_true")
+    ap.add_argument("--no-weighted", action="store_true")
     ap.add_argument("--two-piece-only", action="store_true",
-                    help="run only the two-piece (cut2Gaps) measurements and print them")
+                    help="run only the two-piece (cut2Gaps) and weighted measurements and print them")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs shard across GPUs as "
@@ -310,9 +368,12 @@ def main():
 
     if args.two_piece_only:
         ctx = dcns_amd.Context(0, timing=True)
-        out = two_piece_phase(ctx, lambda m: print(m, file=sys.stderr, flush=True), cpu=not args.no_cpu)
+        lg = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+        out = {"two_piece": two_piece_phase(ctx, lg, cpu=not args.no_cpu)}
+        if not args.no_weighted:
+            out["weighted"] = weighted_phase(ctx, lg, cpu=not args.no_cpu)
         ctx.close()
-        print(json.dumps({"two_piece": out}), flush=True)
+        print(json.dumps(out), flush=True)
         return
 
     pairs = synth.batch(npairs=args.pairs)
@@ -398,6 +459,8 @@ def main():
 
     if not args.no_two_piece:
         line["two_piece"] = two_piece_phase(ctx, log)
+    if not args.no_weighted:
+        line["weighted"] = weighted_phase(ctx, log)
     if not args.no_lambdak:
         line["lambdak"] = lambdak_phase(ctx, log)
 

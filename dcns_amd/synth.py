@@ -183,3 +183,41 @@ def encode(text):
         lut[ord(ch)] = v
         lut[ord(ch.lower())] = v
     return lut[np.frombuffer(text.encode(), dtype=np.uint8)]
+
+
+# The score profiles of the reference's scoreMatrix folder (scoreMatrix/0,
+# /1, /10, /70, /85, /100; one per GFF class: 0 unused, 1 transposable
+# element, 10 intergenic, 70 gene body, 85 exon / UTR, 100 CDS):
+# match, mismatch, open1, ext1, open2, ext2, zdrop.
+WEIGHTED_PROFILES = {
+    0: (0, 0, 0, 0, 0, 0, 1073741824),
+    1: (2, -3, -4, -2, -45, 0, 10000),
+    10: (20, -30, -40, -20, -450, 0, 5000),
+    70: (140, -210, -280, -140, -3150, 0, 5000),
+    85: (170, -255, -340, -170, -3825, 0, 5000),
+    100: (200, -300, -400, -200, -4500, 0, 5000),
+}
+
+
+def gene_categories(rng, n):
+    """GFF-like categories for n positions: intergenic (10) with gene bodies
+    (70) holding exons (85) and CDS (100), and transposable elements (1)."""
+    cat = np.full(n, 10, np.int16)
+    pos = int(rng.integers(0, 1500))
+    while pos < n:
+        if rng.random() < 0.2:  # a transposable element
+            ln = int(rng.integers(300, 3000))
+            cat[pos:pos + ln] = 1
+            pos += ln + int(rng.integers(200, 2000))
+            continue
+        glen = int(rng.integers(1000, 6000))
+        cat[pos:pos + glen] = 70
+        e = pos + int(rng.integers(0, 200))
+        while e < pos + glen:
+            el = int(rng.integers(60, 400))
+            cat[e:min(e + el, pos + glen)] = 85
+            c0 = e + int(rng.integers(0, 30))
+            cat[c0:min(e + el - 10, pos + glen)] = 100
+            e += el + int(rng.integers(80, 800))
+        pos += glen + int(rng.integers(500, 4000))
+    return cat
