@@ -85,6 +85,8 @@ def load_library():
                                                      _P, _I, ctypes.POINTER(_P)]
     lib.dcns_semiglobal_weighted_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _I, _P, _P,
                                                    _I, _P, _P, _P, _I, _I, _P, _P, _P, _L]
+    lib.dcns_find_similar_seeded_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _I,
+                                                   ctypes.POINTER(_P)]
     lib.dcns_calculate_lambda.restype = _D
     lib.dcns_calculate_lambda.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, _I, _I, _D, _D]
     lib.dcns_fit_lambda_k_batch.argtypes = [_P, _L, _I, _P, _D, _D, _I, _P]
@@ -115,7 +117,7 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_cut2gaps_main", "dcns_paircns2gaps_main", "dcns_paircnsxextend_main", "dcns_sw_random_max",
            "dcns_fit_lambda_k_batch", "dcns_calculate_lambda", "dcns_read_score_folder", "dcns_gff_categories",
            "dcns_find_similar_weighted_batch", "dcns_semiglobal_weighted_batch", "dcns_weighted1gap_main",
-           "dcns_weighted2gaps_main"]
+           "dcns_weighted2gaps_main", "dcns_find_similar_seeded_batch", "dcns_cut2gaps2_main"]
 
 
 def default_params(**kw):
@@ -425,6 +427,24 @@ class Context:
                                                               pr.ctypes.data, len(profiles), cat.ctypes.data,
                                                               off.ctypes.data, int(bool(two_piece)),
                                                               ctypes.byref(fs)), "dcns_find_similar_weighted_batch")
+        return self._collect(fs, ba.b.n, True)
+
+    def find_similar_seeded(self, pairs, seeds, max_align_length=140000, params=None):
+        """findSimilarFragmentsForPairedSequence overload 3 (cut2Gaps2) per pair:
+        seeds[i] = [(start1, end1, start2, end2), ...] (0-based).  Returns
+        [(n_seeds, [(start1, end1, start2, end2, score, p, e, cigar), ...]), ...]."""
+        ba = _BatchArrays(pairs)
+        p = params or default_params()
+        first = np.zeros(len(pairs) + 1, np.int64)
+        flat = []
+        for i, s in enumerate(seeds):
+            flat.extend(int(v) for t in s for v in t)
+            first[i + 1] = first[i] + len(s)
+        sd = np.array(flat if flat else [0, 0, 0, 0], np.int32)
+        fs = _P()
+        self._check(self.lib.dcns_find_similar_seeded_batch(self.h, ctypes.byref(ba.b), ctypes.byref(p),
+                                                            sd.ctypes.data, first.ctypes.data, int(max_align_length),
+                                                            ctypes.byref(fs)), "dcns_find_similar_seeded_batch")
         return self._collect(fs, ba.b.n, True)
 
     def semiglobal_weighted(self, pairs, wcats, profiles, pair_of_task, anchors, reverse=None, two_piece=True,

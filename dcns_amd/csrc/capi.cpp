@@ -545,6 +545,35 @@ int dcns_find_similar_weighted_batch(dcns_ctx *c, const dcns_batch *b, const dcn
     return DCNS_OK;
 }
 
+int dcns_find_similar_seeded_batch(dcns_ctx *c, const dcns_batch *b, const dcns_params *p, const int32_t *seeds,
+                                   const int64_t *seed_first, int32_t max_align_length, dcns_frag_set **out) {
+    if (!c || !out) return DCNS_ERR_ARG;
+    *out = nullptr;
+    dcns_frag_set *fs = new (std::nothrow) dcns_frag_set();
+    if (!fs) return DCNS_ERR_INTERNAL;
+    int rc = guarded(c, [&] {
+        check_batch(b);
+        if (!p) throw Error(DCNS_ERR_ARG, "params must be non-null");
+        if (b->n > 0 && !seed_first) throw Error(DCNS_ERR_ARG, "seed_first must be non-null");
+        std::vector<std::vector<Seed>> sd((size_t)b->n);
+        for (int32_t i = 0; i < b->n; ++i) {
+            if (seed_first[i + 1] < seed_first[i] || (seed_first[i + 1] > seed_first[i] && !seeds))
+                throw Error(DCNS_ERR_ARG, "invalid seed ranges");
+            for (int64_t k = seed_first[i]; k < seed_first[i + 1]; ++k)
+                sd[(size_t)i].push_back(Seed(seeds[4 * k], seeds[4 * k + 1], seeds[4 * k + 2], seeds[4 * k + 3]));
+        }
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        c->staged.reset();
+        find_similar_seeded(c, *b, *p, sd, max_align_length, fs->fs);
+    });
+    if (rc != DCNS_OK) {
+        delete fs;
+        return rc;
+    }
+    *out = fs;
+    return DCNS_OK;
+}
+
 int dcns_read_score_folder(const char *folder, dcns_profile *out, int32_t cap, int32_t *n) {
     return guarded(nullptr, [&] {
         if (!folder || !n || cap < 0 || (cap > 0 && !out)) throw Error(DCNS_ERR_ARG, "invalid arguments");

@@ -11,6 +11,7 @@ static void usage() {
               << "Commands:\n"
               << "    cut1Gap             masking genome, sliding window seed, x-drop extension\n"
               << "    cut2Gaps            masking genome, sliding window seed, x-drop and two-piece gap extension\n"
+              << "    cut2Gaps2           two-piece gap extension of the alignments of a cut1Gap sam file\n"
               << "    pairCnsXExtend      sliding window seed and x-drop extension of the sequences of one fasta file\n"
               << "    pairCns2Gaps        the same with two-piece gap extension\n"
               << "    weighted1Gap        x-drop seeds extended by the weighted (per-base GFF class) alignment\n"
@@ -32,6 +33,7 @@ int main(int argc, char **argv) {
     if (!std::strcmp(p, "cut1Gap")) return dcns_cut1gap_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "ranSco")) return dcns_ransco_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "cut2Gaps")) return dcns_cut2gaps_main(argc - 1, argv + 1);
+    if (!std::strcmp(p, "cut2Gaps2")) return dcns_cut2gaps2_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "pairCns2Gaps")) return dcns_paircns2gaps_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "pairCnsXExtend")) return dcns_paircnsxextend_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "weighted1Gap")) return dcns_weighted1gap_main(argc - 1, argv + 1);

@@ -3,6 +3,7 @@
 #include "cli_io.h"
 
 #include <algorithm>
+#include <regex>
 #include <cctype>
 #include <cstdlib>
 #include <iostream>
@@ -195,6 +196,90 @@ std::string IndexedFasta::fetch(const std::string &chr, int32_t start1, int32_t 
 
 // Upper-case IUPAC complement; every other character (lower case included)
 // is kept as it is.
+namespace {
+// numberOfNInAInterval (impl/samFileToSeeds.cpp:8-16), inside the string
+int32_t n_count(const std::string &s, int64_t start, int64_t end) {
+    int32_t n = 0;
+    const int64_t lo = std::max<int64_t>(start, 0), hi = std::min<int64_t>(end, (int64_t)s.size() - 1);
+    for (int64_t i = lo; i <= hi; ++i)
+        if (s[(size_t)i] == 'n') ++n;
+    return n;
+}
+}  // namespace
+
+void sam_to_seeds(const std::string &sam, std::map<std::string, std::string> &ref_genome,
+                  std::map<std::string, std::string> &query_genome, SeedMap &pos, SeedMap &neg) {
+    std::ifstream in(sam);
+    if (!in.good()) {
+        std::cerr << "error in opening sam file " << sam << std::endl;
+        std::exit(1);
+    }
+    // the reference's patterns (a SAM line, a cigar operation, the leading clip)
+    static const std::regex line_re("^(.*?)\\s+(\\d+)\\s+(\\S+)\\s+(\\d+)\\s+(\\d+)\\s+(\\w+)");
+    static const std::regex op_re("(\\d+)([MIDNSH])");
+    static const std::regex clip_re("^(\\d+)([SH])");
+    std::string line;
+    int32_t lineno = 0;
+    while (std::getline(in, line)) {
+        if (lineno % 1000 == 0) std::cout << "reading samfile file line " << lineno << std::endl;
+        ++lineno;
+        if (line.empty() || line[0] == '@' || line.size() < 9) continue;
+        std::smatch m;
+        std::regex_search(line, m, line_re);
+        if (m.empty()) continue;
+        std::string cigar = m[6];
+        std::smatch clip;
+        std::regex_search(cigar, clip, clip_re);
+        if (clip.empty()) continue;
+        const std::string ref_chr = m[3], query_chr = m[1];
+        int32_t ref_start = std::stoi(m[4].str()), query_start = std::stoi(clip[1].str());
+        int32_t ref_end = ref_start, query_end = query_start;
+        std::smatch op;
+        while (std::regex_search(cigar, op, op_re)) {
+            const char c = op[2].str()[0];
+            const int32_t len = std::stoi(op[1].str());
+            if (c == 'M') {
+                ref_end += len;
+                query_end += len;
+            } else if (c == 'D' || c == 'N') {
+                ref_end += len;
+            } else if (c == 'I') {
+                query_end += len;
+            }
+            cigar = op.suffix().str();
+        }
+        const int32_t flag = std::stoi(m[2].str());
+        ref_start -= 1;
+        ref_end -= 2;
+        query_start -= 1;
+        query_end -= 2;
+        const std::string &rg = ref_genome[ref_chr];
+        int32_t n1 = n_count(rg, 0, ref_start - 1), n2 = n_count(rg, ref_start, ref_end);
+        ref_start -= n1;
+        ref_end -= n1;
+        ref_end -= n2;
+        const std::string &qg = query_genome[query_chr];
+        if (flag == 0) {
+            n1 = n_count(qg, 0, query_start - 1);
+            n2 = n_count(qg, query_start, query_end);
+            query_start -= n1;
+            query_end -= n1;
+            query_end -= n2;
+            pos[ref_chr][query_chr].push_back(Seed(ref_start, ref_end, query_start, query_end));
+        } else {
+            n1 = n_count(qg, query_start, query_end);
+            n2 = n_count(qg, query_end + 1, (int64_t)qg.size() - 1);
+            const int32_t qlen = (int32_t)qg.size();
+            query_end = qlen - 1 - query_start;
+            query_start = qlen - 1 - query_end;
+            query_start -= n2;
+            query_end -= n2;
+            query_end -= n1;
+            neg[ref_chr][query_chr].push_back(Seed(ref_start, ref_end, query_start, query_end));
+        }
+    }
+}
+
 std::string reverse_complement(const std::string &s) {
     std::string r;
     r.reserve(s.size());

@@ -61,6 +61,16 @@ class IndexedFasta {
     std::ifstream in_;
 };
 
+// samFileToSeeds (impl/samFileToSeeds.cpp:19-141): the records of a cut1Gap
+// SAM file as seeds in stripped (no 'n') coordinates, keyed by reference then
+// query chromosome, plus-strand records in pos and minus-strand ones in neg
+// (query coordinates on the reverse complement).  Prints the reference's
+// progress line every 1000 lines.  Unknown chromosomes read as empty
+// sequences, as the reference's map lookups make them.
+using SeedMap = std::map<std::string, std::map<std::string, std::vector<Seed>>>;
+void sam_to_seeds(const std::string &sam, std::map<std::string, std::string> &ref_genome,
+                  std::map<std::string, std::string> &query_genome, SeedMap &pos, SeedMap &neg);
+
 std::string reverse_complement(const std::string &s);
 std::string sub_sequence(std::string s, const int &start, const int &end);
 std::string sub_sequence(std::string &s, const int &start, const int &end, const int &strand);

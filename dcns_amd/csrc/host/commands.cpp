@@ -6,6 +6,7 @@
 //   pairCns2Gaps controlLayer4.cpp:318-462
 //   pairCnsXExtend controlLayer4.cpp:181-317
 //   weighted1Gap controlLayer4.cpp:481-652, weighted2Gaps :655-829
+//   cut2Gaps2 controlLayer4.cpp:1815-1994
 // The alignment work runs on the GPU; the host keeps the reference's order
 // of file operations and stdout lines.  All query regions of one cut1Gap
 // input are aligned in one batched call.
@@ -576,6 +577,132 @@ int weighted_common(int argc, char **argv, bool two_piece) {
 
 }  // namespace
 
+namespace {
+
+void cut2gaps2_usage(const Defaults &d, int32_t max_len) {
+    std::cerr << "Usage: dCNS cut2Gaps2 -i input -r reference -q query -o output\n"
+              << "          the default parameters use about 20GB RAM\n"
+              << "Options\n"
+              << " -ra STRING masked reference genome sequence in fasta format\n"
+              << " -qa STRING masked query genome sequence in fasta format\n"
+              << " -sa STRING sam file from the cut1Gap \n"
+              << " -o  FILE   output file, exists file will be overwrote\n"
+              << " -A  INT    Matching score (default: " << d.match << ")\n"
+              << " -B  INT    Mismatching penalty (default: " << d.mismatch << ")\n"
+              << " -O1 INT    open gap penalty (default: " << d.open1 << ")\n"
+              << " -E1 INT    extend gap penalty (default: " << d.ext1 << ")\n"
+              << " -O2 INT    open gap penalty (default: " << d.open2 << ")\n"
+              << " -E2 INT    extend gap penalty (default: " << d.ext2 << ")\n"
+              << " -u2 INT    z drop alignment band width (default: " << d.bandwidth << ")\n"
+              << " -m  INT    maximum aligned length (default: " << max_len << ")\n"
+              << " -w  INT    seed window size (default: " << d.window << ")\n"
+              << std::endl;
+}
+
+}  // namespace
+
+// cut2Gaps2: the alignments of a cut1Gap SAM file re-extended with the
+// two-piece SemiGlobal over whole chromosomes (overload 3), every chromosome
+// pair of the file in one batched call.
+int cut2gaps2_main(int argc, char **argv) {
+    Defaults d;
+    int32_t max_len = 140000;
+    Args a(argc, argv);
+    if (a.has("-A")) d.match = std::stoi(a.get("-A"));
+    if (a.has("-B")) d.mismatch = std::stoi(a.get("-B"));
+    if (a.has("-O1")) d.open1 = std::stoi(a.get("-O1"));
+    if (a.has("-E1")) d.ext1 = std::stoi(a.get("-E1"));
+    if (a.has("-O2")) d.open2 = std::stoi(a.get("-O2"));
+    if (a.has("-E2")) d.ext2 = std::stoi(a.get("-E2"));
+    if (a.has("-m")) max_len = std::stoi(a.get("-m"));
+    if (a.has("-u2")) d.bandwidth = std::stoi(a.get("-u2"));
+    if (a.has("-y")) d.syntenic = str2bool(a.get("-y"), d.syntenic);
+    if (a.has("-w")) d.window = std::stoi(a.get("-w"));
+    if (a.has("-h") || a.has("--help") || !(a.has("-sa") && a.has("-o") && a.has("-ra") && a.has("-qa"))) {
+        cut2gaps2_usage(d, max_len);
+        return 1;
+    }
+    if (d.syntenic) {
+        std::cerr << "dCNS: syntenic filtering (-y) is outside this build's scope" << std::endl;
+        return 1;
+    }
+    const std::string output = a.get("-o");
+    std::map<std::string, std::string> query_genome, ref_genome;
+    std::vector<std::string> qnames, rnames;
+    read_plain_fasta(a.get("-qa"), query_genome, qnames);
+    std::cout << "query genome reading done" << std::endl;
+    read_plain_fasta(a.get("-ra"), ref_genome, rnames);
+    std::cout << "reference genome reading done" << std::endl;
+    SeedMap pos, neg;
+    sam_to_seeds(a.get("-sa"), ref_genome, query_genome, pos, neg);
+    std::cout << "sam file reading done" << std::endl;
+    { std::ofstream(output + ".sam", std::ios_base::app); std::ofstream(output + "o", std::ios_base::app); }
+    // every (reference, query) chromosome pair: plus strand first, then minus
+    // (the query's reverse complement), in map order; 'n' stripped
+    struct Job {
+        std::string ref_chr, query_chr, s1_0, s2_0;
+        const std::vector<Seed> *seeds;
+    };
+    std::vector<Job> jobs;
+    for (int strand = 0; strand < 2; ++strand)
+        for (auto &rc : (strand == 0 ? pos : neg))
+            for (auto &qc : rc.second) {
+                Job j;
+                j.ref_chr = rc.first;
+                j.query_chr = qc.first;
+                j.s1_0 = ref_genome[rc.first];
+                j.s2_0 = strand == 0 ? query_genome[qc.first] : reverse_complement(query_genome[qc.first]);
+                j.seeds = &qc.second;
+                jobs.push_back(std::move(j));
+            }
+    auto strip = [](std::string s) {
+        s.erase(std::remove(s.begin(), s.end(), 'n'), s.end());
+        return s;
+    };
+    std::vector<int8_t> buf;
+    std::vector<int64_t> off1, off2;
+    std::vector<int32_t> len1, len2;
+    std::vector<std::vector<Seed>> seeds;
+    std::map<std::string, std::pair<int64_t, int32_t>> ref_at;  // one copy of each reference chromosome
+    for (const Job &j : jobs) {
+        auto it = ref_at.find(j.ref_chr);
+        if (it == ref_at.end()) {
+            const std::vector<int8_t> c1 = encode(strip(j.s1_0));
+            it = ref_at.emplace(j.ref_chr, std::make_pair((int64_t)buf.size(), (int32_t)c1.size())).first;
+            buf.insert(buf.end(), c1.begin(), c1.end());
+        }
+        const std::vector<int8_t> c2 = encode(strip(j.s2_0));
+        off1.push_back(it->second.first);
+        len1.push_back(it->second.second);
+        off2.push_back((int64_t)buf.size());
+        len2.push_back((int32_t)c2.size());
+        buf.insert(buf.end(), c2.begin(), c2.end());
+        seeds.push_back(*j.seeds);
+    }
+    ref_genome.clear();
+    query_genome.clear();
+    FragSet fs;
+    if (!jobs.empty()) {
+        dcns_params p = to_dcns_params(d);
+        dcns_batch b{buf.data(), off1.data(), len1.data(), off2.data(), len2.data(), (int32_t)jobs.size(), 0};
+        dcns_ctx *ctx = make_ctx();
+        try {
+            find_similar_seeded(ctx, b, p, seeds, max_len, fs);
+        } catch (const std::exception &e) {
+            std::cerr << "dCNS: " << e.what() << std::endl;
+            dcns_ctx_destroy(ctx);
+            return 1;
+        }
+        dcns_ctx_destroy(ctx);
+    }
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        std::vector<PairedSimilarFragment> fr(fs.frags.begin() + fs.first[i], fs.frags.begin() + fs.first[i] + fs.count[i]);
+        write_sam_and_o(output, fr, 1, 1, 1, 1, jobs[i].ref_chr, jobs[i].query_chr, jobs[i].s1_0, jobs[i].s2_0);
+    }
+    std::ofstream(output + "o", std::ios_base::app) << "#done" << std::endl;
+    return 0;
+}
+
 int weighted1gap_main(int argc, char **argv) { return weighted_common(argc, argv, false); }
 int weighted2gaps_main(int argc, char **argv) { return weighted_common(argc, argv, true); }
 int paircns2gaps_main(int argc, char **argv) { return paircns_common(argc, argv, true); }
@@ -685,3 +812,4 @@ extern "C" int dcns_paircns2gaps_main(int argc, char **argv) { return dcns::cli:
 extern "C" int dcns_paircnsxextend_main(int argc, char **argv) { return dcns::cli::paircnsxextend_main(argc, argv); }
 extern "C" int dcns_weighted1gap_main(int argc, char **argv) { return dcns::cli::weighted1gap_main(argc, argv); }
 extern "C" int dcns_weighted2gaps_main(int argc, char **argv) { return dcns::cli::weighted2gaps_main(argc, argv); }
+extern "C" int dcns_cut2gaps2_main(int argc, char **argv) { return dcns::cli::cut2gaps2_main(argc, argv); }

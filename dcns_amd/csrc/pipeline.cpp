@@ -100,8 +100,10 @@ Staged stage_batch(dcns_ctx *c, const dcns_batch &b, const Params &P) {
         if (b.len1[i] < 0 || b.len2[i] < 0) throw Error(DCNS_ERR_ARG, "negative sequence length");
         total += span(b.len1[i]) + span(b.len2[i]);
     }
-    std::vector<int8_t> host((size_t)total + 64, 2);
-    int64_t o = 0, nw = 0;
+    // N (code 2) around every sequence, 64 bytes before the first one too, so
+    // that a DP reading one base past either end of a sequence reads N
+    std::vector<int8_t> host((size_t)total + 128, 2);
+    int64_t o = 64, nw = 0;
     for (int32_t i = 0; i < b.n; ++i) {
         PairDesc &d = S.pd[(size_t)i];
         d.len1 = b.len1[i];

@@ -61,6 +61,7 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
 struct SgResult {
     int32_t max, end1, end2;
     std::vector<uint32_t> cigar;
+    int64_t cells = 0;  // cells the reference evaluates for this task
 };
 // W (may be null): the weighted (ZDP) DP with per-row score profiles
 // (sequenceAlignment3.cpp:1483-1890) instead of SemiGlobal_no_avx.
@@ -93,6 +94,11 @@ void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &
 // then overload 2 (:532-616).
 void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &out);
 void find_similar_2gaps(dcns_ctx *c, const dcns_batch &b, const dcns_params &p, FragSet &out);
+// findSimilarFragmentsForPairedSequence overload 3 (:623-745, the cut2Gaps2
+// path): SemiGlobal from given seeds (seeds[pair], 0-based) with both DPs
+// capped at max_len rows and columns.
+void find_similar_seeded(dcns_ctx *c, const dcns_batch &b, const dcns_params &p,
+                         const std::vector<std::vector<Seed>> &seeds, int32_t max_len, FragSet &out);
 // Uploads the profiles and every pair's categories (as profile indices) for
 // the weighted DP; mode 1 (two-piece) or 2 (single gap).
 SgWeights stage_weights(dcns_ctx *c, const dcns_batch &b, const std::vector<Profile> &prof, const PairCategories &pc,

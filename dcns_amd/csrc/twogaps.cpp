@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <map>
 #include <set>
 #include <unordered_map>
 #include <utility>
@@ -46,7 +47,7 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
                       std::vector<SgResult> &out, int64_t &cells, const SgWeights *Wp) {
     const SgWeights W = Wp ? *Wp : SgWeights{nullptr, nullptr, nullptr, 0, 0};
     const int32_t n = (int32_t)tasks.size();
-    out.assign((size_t)n, SgResult{0, -2, -2, {}});
+    out.assign((size_t)n, SgResult{0, -2, -2, {}, 0});
     cells = 0;
     std::vector<int32_t> idx;
     for (int32_t i = 0; i < n; ++i)
@@ -142,6 +143,7 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
             const SgOut &o = so[(size_t)k];
             SgResult &r = out[(size_t)which[(size_t)k]];
             r.max = o.max;
+            r.cells = o.cells;
             r.end1 = o.e1 - 1;  // the reference's final decrement (:921-922)
             r.end2 = o.e2 - 1;
             cells += o.cells;
@@ -327,6 +329,144 @@ void find_similar_2gaps_staged(dcns_ctx *c, Staged &S, const dcns_params &p, Fra
     validate_params(p);
     validate_two_piece(p);
     two_gaps_flow(c, S, p, SgWeights{nullptr, nullptr, nullptr, 0, 0}, out);
+}
+
+void find_similar_seeded(dcns_ctx *c, const dcns_batch &b, const dcns_params &p,
+                         const std::vector<std::vector<Seed>> &seeds, int32_t max_len, FragSet &out) {
+    validate_params(p);
+    validate_two_piece(p);
+    if (max_len < 1) throw Error(DCNS_ERR_ARG, "maximumAlignLength must be >= 1");
+    if ((int32_t)seeds.size() != b.n) throw Error(DCNS_ERR_ARG, "one seed list per pair");
+    const Params P = to_params(p);
+    const double t0 = ms_now();
+    Staged S = stage_batch(c, b, P);
+    const int32_t npairs = b.n;
+    // every seed's reverse SemiGlobal (:657-670), speculatively: whether a seed
+    // is used depends on the fragments of the seeds before it, the DP does not
+    std::vector<SgTask> rt;
+    std::vector<std::vector<int64_t>> rtask((size_t)npairs);  // -1: seed outside the sequences
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> pend((size_t)npairs);  // patched ends
+    for (int32_t i = 0; i < npairs; ++i) {
+        const int32_t L1 = b.len1[i], L2 = b.len2[i];
+        const int8_t *s1 = b.seqs + b.off1[i], *s2 = b.seqs + b.off2[i];
+        for (const Seed &x : seeds[(size_t)i]) {
+            int32_t e1 = x.getEnd1(), e2 = x.getEnd2();
+            if (e1 < 0 || e1 >= L1 || e2 < 0 || e2 >= L2) {  // undefined in the reference
+                rtask[(size_t)i].push_back(-1);
+                pend[(size_t)i].push_back({0, 0});
+                continue;
+            }
+            SgTask t{};
+            t.pair = i;
+            t.rev = 1;
+            t.R = std::min(1 + e1, max_len);
+            t.C = std::min(1 + e2, max_len);
+            if (s1[e1] != s2[e2]) {  // the reference's patch (:661-665); the DP keeps R x C
+                --e1;
+                --e2;
+            }
+            t.a1 = e1;
+            t.a2 = e2;
+            rtask[(size_t)i].push_back((int64_t)rt.size());
+            pend[(size_t)i].push_back({e1, e2});
+            rt.push_back(t);
+        }
+    }
+    std::vector<SgResult> rr;
+    int64_t c1 = 0, c2 = 0;
+    semiglobal_batch(c, P, rt, false, rr, c1);
+    // the forward SemiGlobal with traceback of every distinct start
+    std::vector<SgTask> ft;
+    std::vector<std::map<std::pair<int32_t, int32_t>, int64_t>> ftask((size_t)npairs);
+    for (int32_t i = 0; i < npairs; ++i) {
+        const int32_t L1 = b.len1[i], L2 = b.len2[i];
+        for (size_t k = 0; k < seeds[(size_t)i].size(); ++k) {
+            const int64_t r = rtask[(size_t)i][k];
+            if (r < 0) continue;
+            const int32_t s1 = pend[(size_t)i][k].first - rr[(size_t)r].end1,
+                          s2 = pend[(size_t)i][k].second - rr[(size_t)r].end2;
+            if (ftask[(size_t)i].count({s1, s2})) continue;
+            SgTask t{};
+            t.pair = i;
+            t.rev = 0;
+            t.a1 = s1;
+            t.a2 = s2;
+            t.R = (s1 < 0 || s2 < 0) ? 0 : std::min(L1 - s1, max_len);  // outside: no DP
+            t.C = (s1 < 0 || s2 < 0) ? 0 : std::min(L2 - s2, max_len);
+            ftask[(size_t)i][{s1, s2}] = (int64_t)ft.size();
+            ft.push_back(t);
+        }
+    }
+    std::vector<SgResult> fr;
+    semiglobal_batch(c, P, ft, true, fr, c2);
+    const double t1 = ms_now();
+    // the reference's loop (:649-742), per pair in seed order
+    std::vector<std::vector<PairedSimilarFragment>> per((size_t)npairs);
+    std::vector<int64_t> cells((size_t)npairs, 0);
+    parallel_for(npairs, [&](int64_t i) {
+        const double L1d = (double)b.len1[i], L2d = (double)b.len2[i];
+        std::vector<PairedSimilarFragment> &v = per[(size_t)i];
+        std::set<std::pair<int32_t, int32_t>> starts;
+        for (size_t k = 0; k < seeds[(size_t)i].size(); ++k) {
+            const Seed &x = seeds[(size_t)i][k];
+            bool use = true;
+            for (const PairedSimilarFragment &f : v)  // 1-based uint32 against 0-based seeds
+                if (f.getStart1() <= (uint32_t)x.getStart1() && f.getEnd1() >= (uint32_t)x.getEnd1() &&
+                    f.getStart2() <= (uint32_t)x.getStart2() && f.getEnd2() >= (uint32_t)x.getEnd2()) {
+                    use = false;
+                    break;
+                }
+            if (!use) continue;
+            const int64_t r = rtask[(size_t)i][k];
+            if (r < 0) continue;
+            cells[(size_t)i] += rr[(size_t)r].cells;
+            const int32_t start1 = pend[(size_t)i][k].first - rr[(size_t)r].end1,
+                          start2 = pend[(size_t)i][k].second - rr[(size_t)r].end2;
+            if (!starts.insert({start1, start2}).second) continue;
+            const SgResult &g = fr[(size_t)ftask[(size_t)i].at({start1, start2})];
+            cells[(size_t)i] += g.cells;
+            if (!(g.cigar.size() > 0 && g.end1 > 0 && g.end2 > 0)) continue;  // the reference reports an error
+            const int32_t end1 = g.end1 + start1, end2 = g.end2 + start2;
+            int32_t to_remove = -1;
+            bool found = false;
+            for (size_t q = 0; q < v.size(); ++q) {
+                if (v[q].getEnd1() == (uint32_t)(end1 + 1) && v[q].getEnd2() == (uint32_t)(end2 + 1)) {
+                    found = true;
+                    if ((uint32_t)(start1 + 1) < v[q].getStart1() && (uint32_t)(start2 + 1) < v[q].getStart2())
+                        to_remove = (int32_t)q;
+                }
+            }
+            if (found && to_remove < 0) continue;
+            const double e = p.k * L1d * L2d * std::exp(-1.0 * p.lambda * (double)g.max);
+            const double pv = 1.0 - std::exp(-1.0 * e);
+            PairedSimilarFragment f((uint32_t)(start1 + 1), (uint32_t)(end1 + 1), (uint32_t)(start2 + 1),
+                                    (uint32_t)(end2 + 1), (uint32_t)g.max, g.cigar, pv, e);
+            if (to_remove >= 0)
+                v[(size_t)to_remove] = f;
+            else
+                v.push_back(f);
+        }
+    });
+    out.frags.clear();
+    out.first.assign((size_t)npairs, 0);
+    out.count.assign((size_t)npairs, 0);
+    out.seeds_size.assign((size_t)npairs, 0);
+    int64_t total = 0;
+    for (int32_t i = 0; i < npairs; ++i) {
+        out.first[(size_t)i] = (int64_t)out.frags.size();
+        out.count[(size_t)i] = (int64_t)per[(size_t)i].size();
+        out.seeds_size[(size_t)i] = (int64_t)seeds[(size_t)i].size();
+        for (auto &f : per[(size_t)i]) out.frags.push_back(std::move(f));
+        total += cells[(size_t)i];
+    }
+    const double t2 = ms_now();
+    dcns_stats &st = c->stats;
+    st = dcns_stats{};
+    st.semiglobal_cells = total;
+    st.fragments = (int64_t)out.frags.size();
+    st.ms_semiglobal = t1 - t0;
+    st.ms_replay = t2 - t1;
+    st.ms_total = t2 - t0;
 }
 
 SgWeights stage_weights(dcns_ctx *c, const dcns_batch &b, const std::vector<Profile> &prof, const PairCategories &pc,

@@ -159,6 +159,19 @@ int dcns_semiglobal_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params 
                           const int32_t *anchors, const int32_t *reverse, int32_t n, int32_t with_cigar,
                           int32_t *out3, uint32_t *cigar, int64_t *cig_off, int64_t cig_cap);
 
+/* findSimilarFragmentsForPairedSequence overload 3 -- the cut2Gaps2 path
+ * (impl/findSimilarFragmentsForPairedSequence4.cpp:623-745, called from
+ * controlLayer4.cpp:1928 and :1967).  Pair i's seeds are seeds[4k .. 4k+3] =
+ * start1, end1, start2, end2 (0-based) for k in [seed_first[i],
+ * seed_first[i+1]), in the reference's order; both SemiGlobal DPs of a seed
+ * are capped at max_align_length rows and columns.  Uses open2/ext2, zdrop,
+ * bandwidth, lambda and k of p.  Seeds whose end lies outside the sequences
+ * are skipped (the reference reads outside its arrays for them); where the
+ * reference's end correction (:661-665) makes the reverse DP reach one base
+ * before a sequence, that base is N. */
+int dcns_find_similar_seeded_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, const int32_t *seeds,
+                                   const int64_t *seed_first, int32_t max_align_length, dcns_frag_set **out);
+
 /* ---- Weighted (ZDP) paths: weighted1Gap / weighted2Gaps ------------------ */
 
 /* One score profile: the seven lines of a score-folder file
@@ -268,12 +281,15 @@ int dcns_ransco_main(int argc, char **argv);
  *   dcns_weighted1gap_main   int weighted1Gap(int argc, char** argv), controlLayer4.cpp:554
  *   dcns_weighted2gaps_main  int weighted2Gaps(int argc, char** argv), controlLayer4.cpp:729
  *                           (all query regions in one batch; the GFF3 categories
- *                           of the reference region only) */
+ *                           of the reference region only)
+ *   dcns_cut2gaps2_main      int cut2Gaps2(int argc, char** argv), controlLayer4.cpp:1815
+ *                           (every chromosome pair of the SAM file in one batch) */
 int dcns_cut2gaps_main(int argc, char **argv);
 int dcns_paircns2gaps_main(int argc, char **argv);
 int dcns_paircnsxextend_main(int argc, char **argv);
 int dcns_weighted1gap_main(int argc, char **argv);
 int dcns_weighted2gaps_main(int argc, char **argv);
+int dcns_cut2gaps2_main(int argc, char **argv);
 
 /* lambda/K surrogate network.  No counterpart in the reference, which obtains
  * lambda and K only by permutation (ranSco, permutationLsqLambdaK.cpp:32-102,

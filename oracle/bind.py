@@ -264,6 +264,37 @@ def _gff_categories(self, gff, chroms, which, start, count):
     return out
 
 
+def _find_similar_seeded(self, s1, s2, seeds, sc=(2, -3, -4, -2, -45, 0), zdrop=50, bandwidth=20000, max_len=140000,
+                         lam=0.382291, k=0.006662, stats=None):
+    """findSimilarFragmentsForPairedSequence overload 3 (cut2Gaps2): seeds =
+    [(start1, end1, start2, end2), ...] (0-based).  A list of (start1, end1,
+    start2, end2, score, pvalue, evalue, cigar)."""
+    s1, s2 = self._codes(s1), self._codes(s2)
+    sd = np.ascontiguousarray(np.asarray(seeds, np.int32).reshape(-1, 4))
+    cap, cig_cap = 4096, 1 << 22
+    ints = np.zeros(5 * cap, np.int32)
+    dbls = np.zeros(2 * cap, np.float64)
+    cig = np.zeros(cig_cap, np.uint32)
+    off = np.zeros(cap + 1, np.int64)
+    args = [_ptr(s1), len(s1), _ptr(s2), len(s2), sd.ctypes.data if sd.size else None, len(sd), *sc, zdrop, bandwidth,
+            max_len, lam, k, ints.ctypes.data, dbls.ctypes.data, cap, cig.ctypes.data, off.ctypes.data, cig_cap]
+    f = self._f("find_similar_seeded")
+    base = [_P, _I, _P, _I, _P, _I] + [_I] * 9 + [_D, _D, _P, _P, _I, _P, _P, _L]
+    if self.prefix == "orc":
+        f.argtypes = base + [_P]
+        cl = np.zeros(1, np.int64)
+        n = f(*args, cl.ctypes.data)
+        if stats is not None:
+            stats["semiglobal_cells"] = int(cl[0])
+    else:
+        f.argtypes = base
+        n = f(*args)
+    assert n >= 0, "capacity"
+    return [(*[int(v) for v in ints[5 * i:5 * i + 5]], float(dbls[2 * i]), float(dbls[2 * i + 1]),
+             [int(c) for c in cig[off[i]:off[i + 1]]]) for i in range(n)]
+
+
+Checker.find_similar_seeded = _find_similar_seeded
 Checker.semiglobal_weighted = _semiglobal_weighted
 Checker.find_similar_weighted = _find_similar_weighted
 Checker.score_profile = _score_profile

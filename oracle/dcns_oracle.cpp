@@ -747,7 +747,8 @@ void init_ctx(SeedCtx &c, const int8_t *seq1, int32_t L1, const int8_t *seq2, in
               int32_t mismatch, int32_t open, int32_t ext, int32_t window, int32_t min_score) {
     c.seq1 = seq1; c.seq2 = seq2; c.L1 = L1; c.L2 = L2; c.W = window; c.minScore = min_score;
     c.minRef = min_score / match; c.sc = Scoring{match, mismatch}; c.open = open; c.ext = ext;
-    c.rc1.resize((size_t)L1 + 1); c.rc2.resize((size_t)L2 + 1);
+    // one N past each reverse complement: the base before the sequence
+    c.rc1.assign((size_t)L1 + 1, 2); c.rc2.assign((size_t)L2 + 1, 2);
     for (int32_t i = 0; i < L1; ++i) c.rc1[i] = (int8_t)(4 - seq1[L1 - 1 - i]);
     for (int32_t i = 0; i < L2; ++i) c.rc2[i] = (int8_t)(4 - seq2[L2 - 1 - i]);
 }
@@ -827,6 +828,58 @@ int orc_find_similar_2gaps(const int8_t *seq1, int32_t L1, const int8_t *seq2, i
         stats[0] = c.cells; stats[1] = xcells; stats[2] = nseeds; stats[3] = nlinked;
         stats[4] = sgcells;
     }
+    return emit(frags, ints, dbls, cap, cig, cig_off, cig_cap);
+}
+
+int orc_find_similar_seeded(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, const int32_t *seeds,
+                            int32_t nseeds, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
+                            int32_t open2, int32_t ext2, int32_t zdrop, int32_t bandwidth, int32_t max_len,
+                            double lambda, double kvalue, int32_t *ints, double *dbls, int32_t cap, uint32_t *cig,
+                            int64_t *cig_off, int64_t cig_cap, int64_t *sgcells) {
+    // findSimilarFragmentsForPairedSequence overload 3
+    // (findSimilarFragmentsForPairedSequence4.cpp:623-745)
+    SeedCtx c;
+    init_ctx(c, seq1, L1, seq2, L2, match, mismatch, open, ext, 38, 40);
+    const Scoring sc{match, mismatch};
+    const double L1d = (double)L1, L2d = (double)L2;
+    std::vector<Fragment> frags;
+    std::set<std::pair<int32_t, int32_t>> starts;
+    int64_t cl = 0;
+    for (int32_t k = 0; k < nseeds; ++k) {
+        const int32_t s1 = seeds[4 * k], e1s = seeds[4 * k + 1], s2 = seeds[4 * k + 2], e2s = seeds[4 * k + 3];
+        bool use = true;
+        for (const Fragment &f : frags)  // 1-based uint32 against 0-based seeds
+            if (f.start1 <= (uint32_t)s1 && f.end1 >= (uint32_t)e1s && f.start2 <= (uint32_t)s2 &&
+                f.end2 >= (uint32_t)e2s) {
+                use = false;
+                break;
+            }
+        if (!use) continue;
+        int32_t e1 = e1s, e2 = e2s;
+        if (e1 < 0 || e1 >= L1 || e2 < 0 || e2 >= L2) continue;  // outside the sequences (undefined in the reference)
+        const int32_t r1 = std::min(1 + e1, max_len), r2 = std::min(1 + e2, max_len);
+        if (seq1[e1] != seq2[e2]) {  // the reference's patch (:661-665)
+            --e1;
+            --e2;
+        }
+        // after the patch the reverse DP keeps r1 x r2 and starts one base
+        // earlier: its last row / column is the base before the sequence (N)
+        const SgResult rv = semiglobal(c.rc1.data() + (L1 - 1 - e1), c.rc2.data() + (L2 - 1 - e2), r1, r2, sc, open,
+                                       ext, open2, ext2, zdrop, bandwidth, nullptr, nullptr, &cl);
+        const int32_t start1 = e1 - rv.end1, start2 = e2 - rv.end2;
+        if (!starts.insert({start1, start2}).second) continue;
+        const int32_t f1 = std::min(L1 - start1, max_len), f2 = std::min(L2 - start2, max_len);
+        std::vector<uint32_t> cigar;
+        BandTrace T;
+        SgResult fw{0, -2, -2};
+        if (start1 >= 0 && start2 >= 0 && f1 > 0 && f2 > 0)
+            fw = semiglobal(seq1 + start1, seq2 + start2, f1, f2, sc, open, ext, open2, ext2, zdrop, bandwidth, &T,
+                            &cigar, &cl);
+        if (!(cigar.size() > 0 && fw.end1 > 0 && fw.end2 > 0)) continue;
+        add_fragment(frags, start1, start2, fw.end1 + start1, fw.end2 + start2, fw.max, cigar, L1d, L2d, lambda,
+                     kvalue);
+    }
+    if (sgcells) *sgcells = cl;
     return emit(frags, ints, dbls, cap, cig, cig_off, cig_cap);
 }
 

@@ -111,6 +111,19 @@ int orc_find_similar_weighted(const int8_t *seq1, int32_t L1, const int8_t *seq2
                               const int32_t *pval, int32_t two_piece, int32_t *ints, double *dbls, int32_t cap,
                               uint32_t *cig, int64_t *cig_off, int64_t cig_cap, int64_t *stats);
 
+// findSimilarFragmentsForPairedSequence overload 3 (the cut2Gaps2 path,
+// findSimilarFragmentsForPairedSequence4.cpp:623-745): SemiGlobal from every
+// seed (4 ints each: start1, end1, start2, end2, 0-based) in order, skipping
+// seeds inside an earlier fragment, with both DPs capped at max_len rows and
+// columns.  Seeds whose end lies outside the sequences are skipped.  Output
+// layout as orc_find_similar; sgcells (may be null) receives the SemiGlobal
+// cells.
+int orc_find_similar_seeded(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, const int32_t *seeds,
+                            int32_t nseeds, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
+                            int32_t open2, int32_t ext2, int32_t zdrop, int32_t bandwidth, int32_t max_len,
+                            double lambda, double kvalue, int32_t *ints, double *dbls, int32_t cap, uint32_t *cig,
+                            int64_t *cig_off, int64_t cig_cap, int64_t *sgcells);
+
 // Max local score of many pairs (the ranSco inner call,
 // permutationLsqLambdaK.cpp:97).  seqs is one buffer; offsets/lengths per pair.
 int orc_sw_max_batch(const int8_t *seqs, const int64_t *ref_off, const int32_t *ref_len, const int64_t *qry_off,

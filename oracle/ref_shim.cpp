@@ -18,6 +18,7 @@
 //                                                             impl/sequenceAlignment3.cpp:1483,1710
 //   ref_find_similar_weighted -> findSimilarFragmentsForPairedSequence_wighted(_1gap)
 //                                                             impl/findSimilarFragmentsForPairedSequence4.cpp:779,830
+//   ref_find_similar_seeded -> findSimilarFragmentsForPairedSequence (overload 3) :623
 //   ref_score_profile -> Score(folder) getters                model/Score.cpp:9-104
 //   ref_gff_categories -> readGffFileWithEveryThing           impl/gffToCategory.cpp:81-238
 #include <cstdint>
@@ -42,7 +43,9 @@ void link_seeds(std::set<Seed> &seeds);
 
 namespace {
 std::vector<int8_t> rc_of(const int8_t *s, int32_t n) {
-    std::vector<int8_t> r((size_t)n + 1);
+    // one N past the end: the base the reference's patched reverse alignment
+    // of overload 3 reads there (outside its array)
+    std::vector<int8_t> r((size_t)n + 1, 2);
     for (int32_t i = 0; i < n; ++i) r[i] = (int8_t)(4 - s[n - 1 - i]);
     return r;
 }
@@ -312,6 +315,46 @@ int ref_gff_categories(const char *gff, const char *const *names, const int32_t 
     for (int64_t k = 0; k < count; ++k) out[k] = c[from + k];
     for (auto &kv : cats) delete[] kv.second;
     return 0;
+}
+
+// findSimilarFragmentsForPairedSequence overload 3 (the cut2Gaps2 path,
+// impl/findSimilarFragmentsForPairedSequence4.cpp:623): seeds as 4 ints
+// (start1, end1, start2, end2); layout as ref_find_similar.
+int ref_find_similar_seeded(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2, const int32_t *seeds,
+                            int32_t nseeds, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
+                            int32_t open2, int32_t ext2, int32_t zdrop, int32_t bandwidth, int32_t max_len,
+                            double lambda, double kvalue, int32_t *ints, double *dbls, int32_t cap, uint32_t *cig,
+                            int64_t *cig_off, int64_t cig_cap) {
+    Scorei m((int8_t)match, (int8_t)mismatch);
+    std::vector<int8_t> a = copy_of(seq1, L1), b = copy_of(seq2, L2);
+    std::vector<int8_t> ar = rc_of(seq1, L1), br = rc_of(seq2, L2);
+    std::string s1str, s2str;
+    std::vector<Seed> xs;
+    for (int32_t k = 0; k < nseeds; ++k) {
+        int32_t s1 = seeds[4 * k], e1 = seeds[4 * k + 1], s2 = seeds[4 * k + 2], e2 = seeds[4 * k + 3];
+        xs.push_back(Seed(s1, e1, s2, e2));
+    }
+    int32_t l1 = L1, l2 = L2, win = 38, ml = max_len;
+    const int32_t tdim = std::min(max_len, std::max(L1, L2) + 2) + 1;
+    Matrix T(tdim, tdim);
+    std::vector<PairedSimilarFragment> f = findSimilarFragmentsForPairedSequence(
+        a.data(), ar.data(), b.data(), br.data(), l1, l2, win, open, ext, open2, ext2, match, mismatch, m, s1str,
+        s2str, 0.1, lambda, kvalue, zdrop, bandwidth, xs, T, ml);
+    int32_t n = (int32_t)f.size();
+    int64_t total = 0;
+    for (auto &x : f) total += (int64_t)x.getCigar().size();
+    if (n > cap || total > cig_cap) return -n - 1;
+    int64_t o = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        ints[5 * i + 0] = (int32_t)f[i].getStart1(); ints[5 * i + 1] = (int32_t)f[i].getEnd1();
+        ints[5 * i + 2] = (int32_t)f[i].getStart2(); ints[5 * i + 3] = (int32_t)f[i].getEnd2();
+        ints[5 * i + 4] = (int32_t)f[i].getScore();
+        dbls[2 * i + 0] = f[i].getPValue(); dbls[2 * i + 1] = f[i].getEValue();
+        cig_off[i] = o;
+        for (uint32_t c : f[i].getCigar()) cig[o++] = c;
+    }
+    cig_off[n] = o;
+    return n;
 }
 
 }  // extern "C"

@@ -10,9 +10,10 @@ outputs of the reference build (oracle/_ref) on them: K1 SmithWaterman
 the two-piece SemiGlobal, the seed set before/after link_seeds, the fragments
 of findSimilarFragmentsForPairedSequence (overloads 1 and 4), the weighted
 paths (score profiles, weighted SemiGlobal tasks, the _wighted / _wighted_1gap
-overloads, GFF categories), and the text outputs of the cut1Gap, cut2Gaps,
-pairCnsXExtend, ranSco, weighted1Gap and weighted2Gaps commands.  Only data is
-written.  ``--weighted`` writes the weighted fixtures only.
+overloads, GFF categories), overload 3 (seeded), and the text outputs of the
+cut1Gap, cut2Gaps, pairCnsXExtend, ranSco, weighted1Gap, weighted2Gaps and
+cut2Gaps2 commands.  Only data is written.  ``--weighted`` / ``--seeded``
+write those fixtures only.
 """
 import json
 import os
@@ -140,6 +141,7 @@ def main():
         open(os.path.join(d, f"expected_{tag}.txt"), "w").write(r.stdout)
     write_two_piece(HERE, ref)
     write_weighted(HERE, ref)
+    write_seeded(HERE, ref)
     print("golden fixtures written to", HERE)
 
 
@@ -336,6 +338,52 @@ def write_weighted_commands(here, rng, default):
             os.replace(os.path.join(d, f), os.path.join(d, "expected_" + f))
 
 
+def write_seeded(here, ref):
+    """Overload 3 (cut2Gaps2) on planted pairs with seeds taken from the
+    reference's own fragments (some ends moved off a match, which triggers the
+    reference's end correction), and the cut2Gaps2 command on the cut1Gap
+    fixture genomes with the cut1Gap SAM outputs as input."""
+    rng = np.random.default_rng(20261206)
+    cases = []
+    for t in range(8):
+        L1, L2 = int(rng.integers(900, 3500)), int(rng.integers(900, 3500))
+        a, b = synth.one_pair(seed=9700 + t, L1=L1, L2=L2, nfrag=int(rng.integers(3, 10)))[:2]
+        seeds = []
+        for f in ref.find_similar(a, b):
+            d = int(rng.integers(-2, 3))
+            seeds.append([f[0] - 1, min(f[1] - 1 + d, L1 - 1), f[2] - 1, min(f[3] - 1 + d, L2 - 1)])
+        for _ in range(3):
+            e1, e2 = int(rng.integers(0, L1)), int(rng.integers(0, L2))
+            seeds.append([max(0, e1 - 40), e1, max(0, e2 - 40), e2])
+        if seeds:
+            seeds.append(list(seeds[0]))
+        kw = [dict(max_len=140000), dict(max_len=400), dict(max_len=140000, bandwidth=60),
+              dict(max_len=140000, sc=(2, -3, -4, -2, -20, -1), zdrop=30)][t % 4]
+        cases.append({"s1": txt(a), "s2": txt(b), "seeds": seeds, "params": kw,
+                      "fragments": ref.find_similar_seeded(a, b, seeds, **kw)})
+    json.dump(cases, open(os.path.join(here, "seeded_fragments.json"), "w"))
+    d = os.path.join(here, "cut2gaps2")
+    shutil.rmtree(d, ignore_errors=True)
+    os.makedirs(d)
+    src = os.path.join(here, "cut1gap")
+    for f in ("refgenome.fa", "querygenome.fa"):
+        shutil.copy(os.path.join(src, f), d)
+    with open(os.path.join(d, "in.sam"), "w") as fh:  # every cut1Gap alignment, both input files
+        for f in sorted(os.listdir(src)):
+            if f.startswith("expected_out") and f.endswith(".sam"):
+                fh.write(open(os.path.join(src, f)).read())
+    runs = [["-o", "g2_", "-m", "12000"],
+            ["-o", "g2s_", "-m", "3000", "-u2", "500", "-O2", "-30", "-E2", "-1"]]
+    for args in runs:
+        full = ["-sa", "in.sam", "-ra", "refgenome.fa", "-qa", "querygenome.fa"] + args
+        r = subprocess.run([bind.REF_CLI, "cut2Gaps2"] + full, cwd=d, capture_output=True, text=True)
+        tag = args[1]
+        open(os.path.join(d, f"expected_stdout_{tag}.txt"), "w").write(r.stdout)
+    for f in sorted(os.listdir(d)):
+        if f.startswith(("g2_", "g2s_")):
+            os.replace(os.path.join(d, f), os.path.join(d, "expected_" + f))
+
+
 def fasta_with_fai(path, records, width=60):
     """Writes a FASTA file and its samtools-style .fai index."""
     with open(path, "w") as fh, open(path + ".fai", "w") as fai:
@@ -421,10 +469,13 @@ def write_cut1gap_inputs(d, rng):
 
 
 if __name__ == "__main__":
-    if "--weighted" in sys.argv[1:]:
+    if "--weighted" in sys.argv[1:] or "--seeded" in sys.argv[1:]:
         r = bind.reference()
         if r is None:
             sys.exit("oracle/_ref is not built")
-        write_weighted(HERE, r)
+        if "--weighted" in sys.argv[1:]:
+            write_weighted(HERE, r)
+        if "--seeded" in sys.argv[1:]:
+            write_seeded(HERE, r)
     else:
         main()
