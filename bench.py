@@ -297,6 +297,39 @@ def weighted_phase(ctx, log, cpu=True):
     return out
 
 
+def seeded_phase(ctx, log, cpu=True):
+    """Overload 3 (cut2Gaps2's re-extension of cut1Gap alignments) on the
+    10 kb x 10 kb planted pair, seeded with that pair's overload-1 fragments
+    (every other end moved by one base), beside the compiled reference on one
+    core."""
+    from dcns_amd import synth
+    from oracle import bind
+    a, b = synth.one_pair()[:2]
+    frags = ctx.find_similar([(a, b)])[0][1]
+    seeds = [(f[0] - 1, f[1] - 1 - (k % 2), f[2] - 1, f[3] - 1) for k, f in enumerate(frags)]
+    ctx.find_similar_seeded([(a, b)], [seeds], max_align_length=140000)
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        res = ctx.find_similar_seeded([(a, b)], [seeds], max_align_length=140000)
+    dt = (time.perf_counter() - t) / reps
+    cells = ctx.stats()["semiglobal_cells"]
+    out = {"workload": f"10,000 x 10,000 pair, {len(seeds)} seeds from its cut1Gap fragments, cut2Gaps2 defaults "
+                       "(open2 -45, ext2 0, z-drop 50, band 20000, maximumAlignLength 140000)",
+           "reference_cells": cells, "seconds": round(dt, 4), "gcups": round(cells / dt / 1e9, 2),
+           "fragments": len(res[0][1])}
+    ref = bind.reference() if cpu else None
+    if ref is not None:
+        t = time.perf_counter()
+        want = ref.find_similar_seeded(a, b, seeds)
+        tc = time.perf_counter() - t
+        out["cpu_reference"] = {"seconds": round(tc, 3), "gcups": round(cells / tc / 1e9, 4), "cores": 1}
+        out["bit_exact"] = [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in res[0][1]] == \
+                           [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in want]
+    log(f"cut2Gaps2 overload 3: {cells:.3e} cells in {dt * 1e3:.1f} ms ({cells / dt / 1e9:.1f} GCUPS)")
+    return out
+
+
 def config2_significance(ctx, sur, log):
     """Config-2 fragments (500 kb masked pair, default scoring) reported with
     the lambda/K of the judge, of the surrogate and of the reference's
@@ -372,6 +405,7 @@ def main():
         out = {"two_piece": two_piece_phase(ctx, lg, cpu=not args.no_cpu)}
         if not args.no_weighted:
             out["weighted"] = weighted_phase(ctx, lg, cpu=not args.no_cpu)
+        out["seeded"] = seeded_phase(ctx, lg, cpu=not args.no_cpu)
         ctx.close()
         print(json.dumps(out), flush=True)
         return
@@ -461,6 +495,8 @@ def main():
         line["two_piece"] = two_piece_phase(ctx, log)
     if not args.no_weighted:
         line["weighted"] = weighted_phase(ctx, log)
+    if not args.no_two_piece:
+        line["seeded"] = seeded_phase(ctx, log)
     if not args.no_lambdak:
         line["lambdak"] = lambdak_phase(ctx, log)
 
