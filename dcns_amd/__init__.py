@@ -91,6 +91,19 @@ def load_library():
     lib.dcns_calculate_lambda.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, _I, _I, _D, _D]
     lib.dcns_fit_lambda_k_batch.argtypes = [_P, _L, _I, _P, _D, _D, _I, _P]
     lib.dcns_sw_random_max.argtypes = [_P, _I, _I, _I, _I, _P, _P, _P, _P, _I, _I, _P]
+    lib.dcns_kmer_set_read.argtypes = [ctypes.c_char_p, _I, ctypes.POINTER(_P)]
+    lib.dcns_kmer_set_info.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_L), ctypes.POINTER(_L)]
+    lib.dcns_kmer_set_get.argtypes = [_P, _P, _L]
+    lib.dcns_kmer_set_free.argtypes = [_P]
+    lib.dcns_cds_mask_create.argtypes = [_P, _P, _I, ctypes.POINTER(_P)]
+    lib.dcns_cds_mask_gff.argtypes = [_P, ctypes.c_char_p, _I]
+    lib.dcns_cds_mask_sam.argtypes = [_P, ctypes.c_char_p, ctypes.c_char_p, _D, ctypes.POINTER(_I)]
+    lib.dcns_cds_mask_get.argtypes = [_P, _I, _P, _L]
+    lib.dcns_cds_mask_free.argtypes = [_P]
+    lib.dcns_mask_stage.argtypes = [_P, _P, _P, _P, _I]
+    lib.dcns_mask_body_size.restype = _L
+    lib.dcns_mask_body_size.argtypes = [_L, _I]
+    lib.dcns_mask_run.argtypes = [_P, _P, ctypes.c_char, _I, _I, _P, _P]
     lib.dcns_mlp_create.argtypes = [_P, ctypes.c_uint64, ctypes.POINTER(_P)]
     lib.dcns_mlp_destroy.argtypes = [_P]
     lib.dcns_mlp_num_params.restype = _L
@@ -117,7 +130,10 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_cut2gaps_main", "dcns_paircns2gaps_main", "dcns_paircnsxextend_main", "dcns_sw_random_max",
            "dcns_fit_lambda_k_batch", "dcns_calculate_lambda", "dcns_read_score_folder", "dcns_gff_categories",
            "dcns_find_similar_weighted_batch", "dcns_semiglobal_weighted_batch", "dcns_weighted1gap_main",
-           "dcns_weighted2gaps_main", "dcns_find_similar_seeded_batch", "dcns_cut2gaps2_main"]
+           "dcns_weighted2gaps_main", "dcns_find_similar_seeded_batch", "dcns_cut2gaps2_main", "dcns_kmer_set_read",
+           "dcns_kmer_set_info", "dcns_kmer_set_get", "dcns_kmer_set_free", "dcns_cds_mask_create",
+           "dcns_cds_mask_gff", "dcns_cds_mask_sam", "dcns_cds_mask_get", "dcns_cds_mask_free", "dcns_mask_stage",
+           "dcns_mask_body_size", "dcns_mask_run", "dcns_mask_genome_main"]
 
 
 def default_params(**kw):
@@ -194,6 +210,105 @@ def gff_categories(gff_path, chrom, chrom_size, start, count):
     if rc != 0:
         raise DcnsError(f"dcns_gff_categories failed ({rc}): {lib.dcns_last_error(None).decode()}")
     return out[:int(count)]
+
+
+# -- maskGenome (controlLayer4.cpp:1999-2084) ---------------------------------
+
+def _last_error(lib):
+    return lib.dcns_last_error(None).decode()
+
+
+class KmerSet:
+    """The k-mer profile of a jellyfish dump (dcns_kmer_set_read,
+    impl/maskGenome.cpp:15-48): records counted more than ``freq`` times,
+    upper-cased; ``k`` is the length of the dump's last record.  ``freq <= 0``
+    reads nothing (k = 10, no members)."""
+
+    def __init__(self, path, freq):
+        self.lib = load_library()
+        h = _P()
+        rc = self.lib.dcns_kmer_set_read(path.encode() if path else None, int(freq), ctypes.byref(h))
+        if rc != 0:
+            raise DcnsError(f"dcns_kmer_set_read failed ({rc}): {_last_error(self.lib)}")
+        self.h = h
+        k, m, r = _I(), _L(), _L()
+        self.lib.dcns_kmer_set_info(h, ctypes.byref(k), ctypes.byref(m), ctypes.byref(r))
+        self.k, self.members, self.records = k.value, m.value, r.value
+
+    def kmers(self):
+        """The members of length k as bytes (a member listed twice appears twice)."""
+        buf = np.zeros(max(1, self.members * self.k), np.uint8)
+        rc = self.lib.dcns_kmer_set_get(self.h, buf.ctypes.data, buf.size)
+        if rc != 0:
+            raise DcnsError(f"dcns_kmer_set_get failed ({rc})")
+        return [buf[i * self.k:(i + 1) * self.k].tobytes() for i in range(self.members)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dcns_kmer_set_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CdsMask:
+    """The reference's ifCds over a genome of uniquely named sequences
+    (dcns_cds_mask_*): gff() is gffToMask / gffToMaskGene, sam() samToMask
+    (impl/getAllTheCdsPositions.cpp:9-173).  Host code, no device needed."""
+
+    def __init__(self, names, sizes):
+        self.lib = load_library()
+        self._names = [ctypes.c_char_p(n.encode()) for n in names]
+        arr = (ctypes.c_char_p * max(1, len(names)))(*self._names)
+        sz = np.ascontiguousarray(sizes, np.int64)
+        h = _P()
+        rc = self.lib.dcns_cds_mask_create(ctypes.cast(arr, _P), sz.ctypes.data if len(sz) else None, len(names),
+                                           ctypes.byref(h))
+        if rc != 0:
+            raise DcnsError(f"dcns_cds_mask_create failed ({rc}): {_last_error(self.lib)}")
+        self.h = h
+        self.sizes = [int(x) for x in sz]
+
+    def gff(self, path, gene=False):
+        rc = self.lib.dcns_cds_mask_gff(self.h, path.encode(), int(bool(gene)))
+        if rc != 0:
+            raise DcnsError(f"dcns_cds_mask_gff failed ({rc}): {_last_error(self.lib)}")
+
+    def sam(self, sam_path, cds_fasta, similarity=0.8):
+        """Returns True when the masks were first rebuilt empty."""
+        r = _I()
+        rc = self.lib.dcns_cds_mask_sam(self.h, sam_path.encode(), cds_fasta.encode(), float(similarity),
+                                        ctypes.byref(r))
+        if rc != 0:
+            raise DcnsError(f"dcns_cds_mask_sam failed ({rc}): {_last_error(self.lib)}")
+        return bool(r.value)
+
+    def mask(self, s):
+        out = np.zeros(max(1, self.sizes[s]), np.uint8)
+        rc = self.lib.dcns_cds_mask_get(self.h, int(s), out.ctypes.data, out.size)
+        if rc != 0:
+            raise DcnsError(f"dcns_cds_mask_get failed ({rc}): {_last_error(self.lib)}")
+        return out[:self.sizes[s]]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dcns_cds_mask_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mask_body_size(length, width):
+    """Bytes of one masked FASTA body (bases, line breaks, closing newline)."""
+    return int(load_library().dcns_mask_body_size(int(length), int(width)))
 
 
 def _profile_array(profiles):
@@ -472,6 +587,37 @@ class Context:
         if not with_cigar:
             return res, None
         return res, [[int(c) for c in cig[off[i]:off[i + 1]]] for i in range(n)]
+
+    # -- maskGenome ----------------------------------------------------------
+    def mask_stage(self, seqs, kmers=None):
+        """Uploads sequences (bytes) and the k-mer set (KmerSet or None)."""
+        buf = np.frombuffer(b"".join(seqs) or b"\0", np.uint8)
+        off = np.zeros(len(seqs) + 1, np.int64)
+        off[1:] = np.cumsum([len(x) for x in seqs])
+        self._mask_off = off
+        self._check(self.lib.dcns_mask_stage(self.h, kmers.h if kmers is not None else None, buf.ctypes.data,
+                                             off.ctypes.data, len(seqs)), "mask_stage")
+
+    def mask_run(self, cds=None, alt="n", soft=False, width=60, want_mask=True, want_bodies=True):
+        """Masks the staged genome.  Returns (masks, bodies): per sequence a
+        0/1 uint8 array and the FASTA body bytes (None when not wanted)."""
+        off = self._mask_off
+        n = len(off) - 1
+        mask = np.zeros(max(1, int(off[-1])), np.uint8) if want_mask else None
+        sizes = [mask_body_size(int(off[i + 1] - off[i]), width) for i in range(n)] if width != 0 else [0] * n
+        boff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        body = np.zeros(max(1, int(boff[-1])), np.uint8) if want_bodies else None
+        a = alt.encode()[:1] if isinstance(alt, str) else bytes([alt])
+        self._check(self.lib.dcns_mask_run(self.h, cds.h if cds is not None else None, a or b"\0", int(bool(soft)),
+                                           int(width), mask.ctypes.data if mask is not None else None,
+                                           body.ctypes.data if body is not None else None), "mask_run")
+        masks = [mask[off[i]:off[i + 1]] for i in range(n)] if want_mask else None
+        bodies = [body[boff[i]:boff[i + 1]].tobytes() for i in range(n)] if want_bodies else None
+        return masks, bodies
+
+    def mask_genome(self, seqs, kmers=None, cds=None, alt="n", soft=False, width=60):
+        self.mask_stage(seqs, kmers)
+        return self.mask_run(cds, alt, soft, width)
 
     def stage(self, pairs=None, params=None, flat=None):
         """Keeps a batch resident in device memory for find_similar_staged()."""

@@ -3,14 +3,26 @@
 // the message for dcns_last_error().
 #include <algorithm>
 #include <cstring>
+#include <fstream>
+#include <map>
 
 #include "../../include/dcns.h"
 #include "ctx.h"
+#include "host/cli_io.h"
+#include "host/mask_io.h"
 #include "host/weights.h"
 #include "kernels.h"
+#include "maskgenome.h"
 #include "pipeline.h"
 
 using namespace dcns;
+
+struct dcns_kmer_set {
+    KmerSet ks;
+};
+struct dcns_cds_mask {
+    CdsMask m;
+};
 
 namespace {
 thread_local std::string g_last_error;
@@ -660,5 +672,112 @@ int dcns_frag_set_get(const dcns_frag_set *fs, int64_t idx, int32_t *ints5, doub
 }
 
 void dcns_frag_set_free(dcns_frag_set *fs) { delete fs; }
+
+int dcns_kmer_set_read(const char *path, int32_t freq, dcns_kmer_set **out) {
+    if (!out) return DCNS_ERR_ARG;
+    *out = nullptr;
+    dcns_kmer_set *s = nullptr;
+    const int rc = guarded(nullptr, [&] {
+        if (!path && freq > 0) throw Error(DCNS_ERR_ARG, "k-mer dump path must be non-null");
+        s = new dcns_kmer_set{read_kmer_dump(path ? path : "", freq)};
+    });
+    if (rc == DCNS_OK) *out = s;
+    return rc;
+}
+
+int dcns_kmer_set_info(const dcns_kmer_set *s, int32_t *k, int64_t *members, int64_t *records) {
+    if (!s) return DCNS_ERR_ARG;
+    if (k) *k = s->ks.k;
+    if (members) *members = (int64_t)s->ks.codes.size() + s->ks.n_general();
+    if (records) *records = s->ks.records;
+    return DCNS_OK;
+}
+
+int dcns_kmer_set_get(const dcns_kmer_set *s, char *out, int64_t cap) {
+    if (!s || cap < 0 || (cap > 0 && !out)) return DCNS_ERR_ARG;
+    const int64_t k = s->ks.k, n = (int64_t)s->ks.codes.size() + s->ks.n_general();
+    if (n * k > cap) return DCNS_ERR_CAPACITY;
+    static const char letter[4] = {'A', 'C', 'T', 'G'};  // inverse of (b >> 1) & 3
+    int64_t o = 0;
+    for (uint64_t code : s->ks.codes) {
+        for (int64_t j = k - 1; j >= 0; --j) out[o + j] = letter[code & 3], code >>= 2;
+        o += k;
+    }
+    if (!s->ks.general.empty()) std::memcpy(out + o, s->ks.general.data(), s->ks.general.size());
+    return DCNS_OK;
+}
+
+void dcns_kmer_set_free(dcns_kmer_set *s) { delete s; }
+
+int dcns_cds_mask_create(const char *const *names, const int64_t *sizes, int32_t n, dcns_cds_mask **out) {
+    if (!out) return DCNS_ERR_ARG;
+    *out = nullptr;
+    dcns_cds_mask *m = nullptr;
+    const int rc = guarded(nullptr, [&] {
+        if (n < 0 || (n > 0 && (!names || !sizes))) throw Error(DCNS_ERR_ARG, "invalid genome arrays");
+        std::vector<std::string> nm;
+        for (int32_t i = 0; i < n; ++i) {
+            if (!names[i]) throw Error(DCNS_ERR_ARG, "null sequence name");
+            nm.emplace_back(names[i]);
+        }
+        m = new dcns_cds_mask{CdsMask(nm, std::vector<int64_t>(sizes, sizes + n))};
+    });
+    if (rc == DCNS_OK) *out = m;
+    return rc;
+}
+
+int dcns_cds_mask_gff(dcns_cds_mask *m, const char *gff_path, int32_t gene) {
+    return guarded(nullptr, [&] {
+        if (!m || !gff_path) throw Error(DCNS_ERR_ARG, "invalid arguments");
+        m->m.gff(gff_path, gene != 0);
+    });
+}
+
+int dcns_cds_mask_sam(dcns_cds_mask *m, const char *sam_path, const char *cds_fasta, double similarity,
+                      int32_t *rebuilt) {
+    return guarded(nullptr, [&] {
+        if (!m || !sam_path || !cds_fasta) throw Error(DCNS_ERR_ARG, "invalid arguments");
+        if (!std::ifstream(cds_fasta).good()) throw Error(DCNS_ERR_ARG, std::string("error in opening fasta file ") + cds_fasta);
+        std::map<std::string, std::string> seqs;
+        std::vector<std::string> names;
+        cli::read_plain_fasta(cds_fasta, seqs, names);
+        std::map<std::string, int64_t> sizes;
+        for (const auto &kv : seqs) sizes[kv.first] = (int64_t)kv.second.size();
+        const bool r = m->m.sam(sam_path, sizes, similarity);
+        if (rebuilt) *rebuilt = r ? 1 : 0;
+    });
+}
+
+int dcns_cds_mask_get(const dcns_cds_mask *m, int32_t s, uint8_t *out, int64_t cap) {
+    return guarded(nullptr, [&] {
+        if (!m || s < 0 || s >= m->m.size()) throw Error(DCNS_ERR_ARG, "invalid arguments");
+        const int64_t n = m->m.seq_size(s);
+        if (cap < n || (n > 0 && !out)) throw Error(DCNS_ERR_CAPACITY, "mask buffer too small");
+        std::memset(out, 0, (size_t)n);
+        for (const auto &r : m->m.ranges(s)) std::memset(out + r.first, 1, (size_t)(r.second - r.first));
+    });
+}
+
+void dcns_cds_mask_free(dcns_cds_mask *m) { delete m; }
+
+int dcns_mask_stage(dcns_ctx *c, const dcns_kmer_set *kmers, const char *seq, const int64_t *offsets, int32_t nseq) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        static const KmerSet none;  // no dump: k = 10, no members
+        mask_stage(c, kmers ? kmers->ks : none, (const uint8_t *)seq, offsets, nseq);
+    });
+}
+
+int64_t dcns_mask_body_size(int64_t len, int32_t width) { return len < 0 ? -1 : mask_body_size(len, width); }
+
+int dcns_mask_run(dcns_ctx *c, const dcns_cds_mask *cds, char alt, int32_t soft, int32_t width, uint8_t *mask_out,
+                  char *body_out) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        mask_run(c, cds ? &cds->m : nullptr, (uint8_t)alt, soft != 0, width, mask_out, (uint8_t *)body_out);
+    });
+}
 
 }  // extern "C"

@@ -111,11 +111,13 @@ void parallel_for(int64_t n, F &&fn, int threads) {
 
 namespace dcns {
 struct Staged;
+struct MaskStaged;
 }
 
 struct dcns_ctx {
     int device = 0;
     std::shared_ptr<dcns::Staged> staged;  // sequences kept resident by dcns_stage
+    std::shared_ptr<dcns::MaskStaged> mstaged;  // genome and k-mer tables kept resident by dcns_mask_stage
     hipStream_t stream = nullptr;
     bool timing = false;
     std::string err;
@@ -133,6 +135,8 @@ struct dcns_ctx {
     dcns::DBuf sort_scr, task_order;                     // scan tasks ordered by row count
     dcns::DBuf xorder;                                   // x-drop pass-2 order
     dcns::DBuf rnd_thr, rnd_len1, rnd_len2, rnd_seed, rnd_out;  // random fragment pairs generated on the device
+    dcns::DBuf mk_seq, mk_off, mk_codes, mk_table, mk_gbytes, mk_ghash, mk_gtab;  // maskGenome: genome, k-mer set
+    dcns::DBuf mk_mask, mk_ranges, mk_boff, mk_body;                          // maskGenome: mask, CDS, FASTA bodies
     dcns::HBuf h_hits, h_roots, h_nodes, h_xout, h_cigar;  // pinned download buffers
 
     hipEvent_t take_event() {

@@ -134,3 +134,21 @@ struct SgOut {
 };
 
 }  // namespace dcns
+
+namespace dcns {
+
+// Members of a maskGenome k-mer set that are not 2-bit codes: any member
+// holding a byte other than upper-case A/C/G/T, or every member when k >= 32.
+// Looked up by the polynomial hash sum(b[j] * base^(k-1-j)) mod 2^64 of the
+// window's bytes, each candidate verified byte by byte.
+struct KmerGeneral {
+    const uint32_t *table;   // member index + 1 per slot, 0 = empty
+    const uint64_t *hashes;  // hash of each member
+    const uint8_t *bytes;    // the members, k bytes each
+    int64_t n;               // members (0: no such member)
+    uint64_t mask;           // slots - 1
+    uint64_t pk1;            // base^(k-1) mod 2^64
+};
+constexpr uint64_t kKmerHashBase = 0x100000001b3ull;
+
+}  // namespace dcns

@@ -17,16 +17,6 @@ int32_t to_int(const std::string &s, const std::string &what) {
     }
 }
 
-// split of model/utils.cpp:9-17: tab-separated, empty fields dropped
-std::vector<std::string> fields(const std::string &line) {
-    std::vector<std::string> out;
-    std::stringstream ss(line);
-    std::string item;
-    while (std::getline(ss, item, '\t'))
-        if (!item.empty()) out.push_back(item);
-    return out;
-}
-
 const std::set<std::string> &gene_like() {
     static const std::set<std::string> s = {
         "transcript", "pseudogenic_transcript", "mRNA", "miRNA", "tRNA", "ncRNA", "mRNA_TE_gene", "rRNA",
@@ -80,6 +70,16 @@ void paint(const std::vector<std::pair<int64_t, int64_t>> &iv, int16_t value, in
 
 }  // namespace
 
+// split of model/utils.cpp:9-17: tab-separated, empty fields dropped
+std::vector<std::string> split_fields(const std::string &line) {
+    std::vector<std::string> out;
+    std::stringstream ss(line);
+    std::string item;
+    while (std::getline(ss, item, '\t'))
+        if (!item.empty()) out.push_back(item);
+    return out;
+}
+
 std::vector<Profile> read_score_folder(const std::string &folder) {
     std::ifstream list(folder + "/matrixList");
     if (!list.good()) throw Error(DCNS_ERR_ARG, "error in opening score matrix list file " + folder + "/matrixList");
@@ -117,7 +117,7 @@ GffAnnotation read_gff(const std::string &path) {
     std::string line;
     while (std::getline(in, line)) {
         if (line.empty() || line[0] == '#' || line.size() < 9) continue;
-        const std::vector<std::string> e = fields(line);
+        const std::vector<std::string> e = split_fields(line);
         if (e.size() != 9) continue;
         int64_t start = to_int(e[3], path), end = to_int(e[4], path);
         if (start > end) std::swap(start, end);

@@ -10,6 +10,9 @@
 
 namespace dcns {
 
+// split, model/utils.cpp:9-17: tab-separated fields, empty fields dropped.
+std::vector<std::string> split_fields(const std::string &line);
+
 // Score::Score(folder), model/Score.cpp:9-71: one profile per line of
 // folder/matrixList, in list order; a category listed again takes the later
 // file's values (the reference's maps keep a value a shorter later file does

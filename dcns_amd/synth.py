@@ -17,6 +17,8 @@ Workloads (sizes as in the benchmark definitions):
                        on 38-2,000 bp, GC U[0.30, 0.70]) for the lambda/K
                        score distribution.
 """
+import os
+
 import numpy as np
 
 CODE = {"A": 0, "C": 1, "N": 2, "G": 3, "T": 4}
@@ -221,3 +223,200 @@ def gene_categories(rng, n):
             e += el + int(rng.integers(80, 800))
         pos += glen + int(rng.integers(500, 4000))
     return cat
+
+
+# -- maskGenome inputs ---------------------------------------------------------
+
+def _wrap(s, width):
+    return "\n".join(s[i:i + width] for i in range(0, len(s), width)) if s else ""
+
+
+def repeat_genome(rng, n, families=40, fam_len=(200, 3000), copies=(3, 60), p_sub=0.02, gc=0.45):
+    """One chromosome of n bases (upper-case ACGT bytes) made of random
+    sequence with copies of `families` repeat elements planted at random
+    places, each copy with substitutions -- the k-mer spectrum a jellyfish
+    count of a repetitive plant genome shows (many unique k-mers, a tail of
+    high-count ones)."""
+    g = LETTERS[[0, 1, 3, 4]][rng.choice(4, n, p=[(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])].copy()
+    for _ in range(families):
+        L = int(rng.integers(*fam_len))
+        fam = LETTERS[[0, 1, 3, 4]][rng.integers(0, 4, L)]
+        for _ in range(int(rng.integers(*copies))):
+            at = int(rng.integers(0, max(1, n - L)))
+            c = fam.copy()
+            sub = rng.random(L) < p_sub
+            c[sub] = LETTERS[[0, 1, 3, 4]][rng.integers(0, 4, int(sub.sum()))]
+            g[at:at + L] = c[:n - at]
+    return g
+
+
+def kmer_counts(genome, k, min_count):
+    """The distinct k-mers of genome (uint8 ACGT bytes) seen more than
+    min_count times, as (kmers [m x k] uint8, counts), counted on the forward
+    strand only, as the masking scan compares windows."""
+    n = len(genome) - k + 1
+    if n <= 0:
+        return np.zeros((0, k), np.uint8), np.zeros(0, np.int64)
+    code = np.zeros(n, np.uint64)
+    b = ((genome.astype(np.uint64) >> np.uint64(1)) & np.uint64(3))
+    for j in range(k):
+        code = (code << np.uint64(2)) | b[j:j + n]
+    u, cnt = np.unique(code, return_counts=True)
+    keep = cnt > min_count
+    u, cnt = u[keep], cnt[keep]
+    out = np.zeros((len(u), k), np.uint8)
+    letters = np.frombuffer(b"ACTG", np.uint8)
+    for j in range(k):
+        out[:, k - 1 - j] = letters[((u >> np.uint64(2 * j)) & np.uint64(3)).astype(np.int64)]
+    return out, cnt
+
+
+def dump_text(kmers, counts):
+    """A jellyfish ``dump`` (fasta: ">count" then the k-mer)."""
+    lines = []
+    for km, c in zip(kmers, counts):
+        lines.append(f">{int(c)}")
+        lines.append(km.tobytes().decode())
+    return "\n".join(lines) + "\n"
+
+
+def mask_case(rng, small=True):
+    """Random maskGenome inputs as file texts: genome, dump, gff, sam, cds
+    fasta, plus command options, exercising the reader and writer rules the
+    reference has (mixed case, N runs and other bytes, header forms, k-mer
+    records of other lengths, CDS ranges past a sequence end, SAM records
+    below the identity threshold)."""
+    nchr = int(rng.integers(1, 5))
+    k = int(rng.choice([1, 2, 3, 5, 8, 12, 20, 31, 32, 37] if small else [20]))
+    chroms = []
+    for c in range(nchr):
+        L = int(rng.integers(max(k, 1), 2500 if small else 200000))
+        s = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, L)].copy()
+        if L > 60 and rng.random() < 0.7:  # a repeat inside the chromosome
+            a, m = int(rng.integers(0, L // 2)), int(rng.integers(5, L // 4))
+            b = int(rng.integers(L // 2, L - m))
+            s[b:b + m] = s[a:a + m]
+        for _ in range(int(rng.integers(0, 4))):  # N runs, lower case runs, other bytes
+            a = int(rng.integers(0, L))
+            m = int(rng.integers(1, 40))
+            kind = rng.random()
+            if kind < 0.4:
+                s[a:a + m] = ord("N")
+            elif kind < 0.8:
+                x = s[a:a + m]
+                s[a:a + m] = np.where((x >= 65) & (x <= 90), x + 32, x)
+            else:
+                s[a:a + m] = np.frombuffer(b"RYKM-nx", np.uint8)[rng.integers(0, 7, len(s[a:a + m]))]
+        name = f"chr{c}"
+        head = [name, f"{name} description text", f"{name}x\tfield", f">{name}"][int(rng.integers(0, 4))]
+        chroms.append((name, head, s.tobytes().decode()))
+    genome = "".join(f">{h}\n{_wrap(s, int(rng.choice([60, 70, 13])))}\n" for _, h, s in chroms)
+    names = []
+    for _, h, _ in chroms:  # the names the reference's reader makes of these headers
+        h = h.split(" ")[0]
+        if h.startswith(">"):
+            t = h.find("\t")
+            names.append(h[1:] if t < 0 else h[1:t])
+        else:
+            t = h.find("\t")
+            names.append(h if t <= 0 else h[:t - 1])
+
+    # dump: windows of the genome (some in lower case, some with N), random k-mers, other lengths
+    freq = int(rng.integers(1, 20))
+    recs = []
+    pool = [s for _, _, s in chroms]
+    for _ in range(int(rng.integers(0, 60))):
+        src = pool[int(rng.integers(0, len(pool)))]
+        if len(src) >= k:
+            a = int(rng.integers(0, len(src) - k + 1))
+            km = src[a:a + k]
+        else:
+            km = "".join(rng.choice(list("ACGT"), k))
+        if rng.random() < 0.3:
+            km = km.lower()
+        if rng.random() < 0.05:
+            km = km[:max(1, k - 1)]  # another length: never matches
+        recs.append((int(rng.integers(0, 3 * freq)), km))
+    for _ in range(int(rng.integers(0, 10))):
+        recs.append((int(rng.integers(0, 3 * freq)), "".join(rng.choice(list("ACGTN"), k))))
+    rng.shuffle(recs)
+    recs.append((int(rng.integers(0, 3 * freq)), "".join(rng.choice(list("ACGT"), k))))  # the last one sets k
+    dump = []
+    for c, km in recs:
+        head = f">{c}" if rng.random() < 0.8 else f">{c} extra"
+        body = km if rng.random() < 0.8 or len(km) < 2 else km[:len(km) // 2] + "\n" + km[len(km) // 2:]
+        dump.append(head + "\n" + body)
+    dump = "\n".join(dump) + "\n"
+
+    # GFF3: CDS-like records (some past the chromosome end), genes with and without biotype
+    gff = ["##gff-version 3"]
+    for name, (_, _, s) in zip(names, chroms):
+        L = len(s)
+        for _ in range(int(rng.integers(0, 8))):
+            a = int(rng.integers(1, L + 1))
+            b = a + int(rng.integers(0, 300))
+            if rng.random() < 0.85:
+                b = min(b, L)
+            if rng.random() < 0.2:
+                a, b = b, a
+            typ = ["CDS", "start_codon", "stop_codon", "exon", "gene", "gene", "mRNA"][int(rng.integers(0, 7))]
+            attr = "ID=g;biotype=protein_coding" if rng.random() < 0.6 else "ID=g;biotype=lncRNA"
+            gff.append("\t".join([name, "src", typ, str(a), str(b), ".", "+", ".", attr]))
+        gff.append(f"{name}\tsrc\tCDS\t5")  # too few fields
+    gff = "\n".join(gff) + "\n"
+
+    # SAM of "CDS" sequences against the genome, and their fasta
+    cds_recs, sam = [], ["@HD\tVN:1.6"]
+    for q in range(int(rng.integers(0, 8))):
+        ci = int(rng.integers(0, len(chroms)))
+        L = len(chroms[ci][2])
+        pos = int(rng.integers(1, L + 1))
+        ops, span = [], 0
+        for _ in range(int(rng.integers(1, 5))):
+            op = str(rng.choice(list("MMMIDNSH")))
+            n = int(rng.integers(1, 60))
+            ops.append(f"{n}{op}")
+        cigar = "".join(ops)
+        qlen = int(rng.integers(20, 200))
+        cds_recs.append((f"tx{q}", "".join(rng.choice(list("ACGT"), qlen))))
+        same = int(rng.integers(0, qlen + 1))
+        cs = f"cs:Z::{same // 2}*ag:{same - same // 2}" if rng.random() < 0.9 else "cs:Z:+ac"
+        flag = int(rng.choice([0, 16, 256]))
+        qname = f"tx{q}" if rng.random() < 0.9 else f"unknown{q}"
+        sam.append("\t".join([qname, str(flag), names[ci], str(pos), "60", cigar, "*", "0", "0", "ACGT", "*",
+                              f"NM:i:{int(rng.integers(0, 9))}", "ms:i:10", cs]))
+    sam = "\n".join(sam) + "\n"
+    cds = "".join(f">{n}\n{s}\n" for n, s in cds_recs)
+
+    opts = {
+        "width": int(rng.choice([60, 60, 7, 1, -3])),
+        "soft": bool(rng.random() < 0.5),
+        "alt": str(rng.choice(["n", "N", "x"])),
+        "gene": bool(rng.random() < 0.3),
+        "similarity": float(rng.choice([0.8, 0.3, 0.0])),
+        "freq": freq,
+        "use_dump": bool(rng.random() < 0.85),
+        "use_gff": bool(rng.random() < 0.5),
+        "use_sam": bool(rng.random() < 0.5),
+    }
+    return {"genome": genome, "dump": dump, "gff": gff, "sam": sam, "cds": cds, "opts": opts, "k": k,
+            "names": names, "seqs": [s for _, _, s in chroms]}
+
+
+def mask_args(case, d, out="out.fa"):
+    """Writes a mask_case's files into folder d; returns the maskGenome flags."""
+    files = {"genome.fa": case["genome"], "kmers.fa": case["dump"], "ann.gff3": case["gff"], "cds.sam": case["sam"],
+             "cds.fa": case["cds"]}
+    for f, t in files.items():
+        with open(os.path.join(d, f), "w") as fh:
+            fh.write(t)
+    o = case["opts"]
+    args = ["-i", "genome.fa", "-o", out, "-w", str(o["width"])]
+    if o["use_dump"]:
+        args += ["-k", "kmers.fa", "-f", str(o["freq"])]
+    if o["use_gff"]:
+        args += ["-g", "ann.gff3"] + (["-z"] if o["gene"] else [])
+    if o["use_sam"]:
+        args += ["-s", "cds.sam", "-c", "cds.fa", "-p", str(o["similarity"])]
+    args += ["-d"] if o["soft"] else ["-l", o["alt"]]
+    return args

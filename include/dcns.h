@@ -291,6 +291,70 @@ int dcns_weighted1gap_main(int argc, char **argv);
 int dcns_weighted2gaps_main(int argc, char **argv);
 int dcns_cut2gaps2_main(int argc, char **argv);
 
+/* ---- maskGenome (controlLayer4.cpp:1999-2084) ------------------------------
+ * The reference masks a genome in one function,
+ *   void maskGenome(std::map<std::string, std::string>& genome, char& alterChar,
+ *                   std::string& kmerProfileFile, int32_t& maskKmerFrequency,
+ *                   int32_t& outputLineWidth, std::map<std::string, std::string>& ifCds,
+ *                   std::string& outputFile, std::vector<std::string>& seqNames,
+ *                   bool& ifSoftmask)                       impl/maskGenome.cpp:8
+ * after gffToMask / gffToMaskGene / samToMask have filled ifCds
+ * (impl/getAllTheCdsPositions.cpp:9, :59, :105).  Here the same steps are
+ * separate calls: the k-mer profile (host), the CDS masks (host), and the
+ * scan plus FASTA writer on the device over a staged genome. */
+typedef struct dcns_kmer_set dcns_kmer_set;
+typedef struct dcns_cds_mask dcns_cds_mask;
+
+/* The k-mer profile, impl/maskGenome.cpp:15-48: the records of a jellyfish
+ * dump ("><count>" line, then sequence lines) whose count is above freq,
+ * upper-cased; k is the length of the dump's last record.  freq <= 0 reads
+ * nothing and gives the empty set with k = 10, as the reference does. */
+int dcns_kmer_set_read(const char *path, int32_t freq, dcns_kmer_set **out);
+/* members: members of length k, the only ones a window can equal (a k-mer
+ * listed twice counts twice); records: all records above the threshold. */
+int dcns_kmer_set_info(const dcns_kmer_set *s, int32_t *k, int64_t *members, int64_t *records);
+/* The members of length k, k bytes each, back to back (cap >= members * k). */
+int dcns_kmer_set_get(const dcns_kmer_set *s, char *out, int64_t cap);
+void dcns_kmer_set_free(dcns_kmer_set *s);
+
+/* The reference's ifCds over a genome of n uniquely named sequences. */
+int dcns_cds_mask_create(const char *const *names, const int64_t *sizes, int32_t n, dcns_cds_mask **out);
+/* gffToMask (gene = 0: CDS, start_codon, stop_codon), getAllTheCdsPositions.cpp:9-56, or
+ * gffToMaskGene (gene = 1: gene records with "biotype=protein_coding"), :59-101.  Starts from
+ * empty masks.  A record starting past the end of its sequence's mask string is an error
+ * (the reference throws std::out_of_range). */
+int dcns_cds_mask_gff(dcns_cds_mask *m, const char *gff_path, int32_t gene);
+/* samToMask, getAllTheCdsPositions.cpp:105-173: M/D runs of the SAM records whose cs-tag
+ * identical bases reach similarity x the CDS length (CDS sequences from cds_fasta, read as
+ * impl/fasta.cpp:7-44 reads them).  *rebuilt = 1 when the masks were first rebuilt empty
+ * (no GFF masks, or one ran past a sequence end): the reference then prints "GFF was not
+ * used for genome masking". */
+int dcns_cds_mask_sam(dcns_cds_mask *m, const char *sam_path, const char *cds_fasta, double similarity,
+                      int32_t *rebuilt);
+/* 0/1 per position of sequence s (cap >= its size). */
+int dcns_cds_mask_get(const dcns_cds_mask *m, int32_t s, uint8_t *out, int64_t cap);
+void dcns_cds_mask_free(dcns_cds_mask *m);
+
+/* Stages a genome on the context's device for maskGenome.cpp:50-61: sequences
+ * seq[offsets[s], offsets[s + 1]) (offsets[0] = 0, n + 1 entries) and the k-mer
+ * set (NULL: no dump, k = 10).  Every sequence must hold at least k bases when
+ * k >= 1: the reference's scan throws std::out_of_range on a shorter one. */
+int dcns_mask_stage(dcns_ctx *ctx, const dcns_kmer_set *kmers, const char *seq, const int64_t *offsets,
+                    int32_t nseq);
+/* Bytes of one FASTA body written by dcns_mask_run: the bases, a newline after every width-th
+ * base (none for width < 0), and the closing newline (maskGenome.cpp:66-121). */
+int64_t dcns_mask_body_size(int64_t len, int32_t width);
+/* maskGenome.cpp:50-121 on the staged genome: a position is masked when a window of k bases
+ * inside its sequence covering it equals a member, or when cds (NULL: none; same sequences,
+ * order and sizes as staged) marks it.  mask_out (N bytes, 0/1) and body_out (the FASTA
+ * bodies back to back: bases upper-cased, masked ones written as alt, or lower-cased when
+ * soft != 0) are filled when non-NULL.  width 0 is an error (the reference divides by it). */
+int dcns_mask_run(dcns_ctx *ctx, const dcns_cds_mask *cds, char alt, int32_t soft, int32_t width,
+                  uint8_t *mask_out, char *body_out);
+/* int maskGenome(int argc, char** argv), controlLayer4.cpp:1999: the command with its flags,
+ * stdout lines and output file. */
+int dcns_mask_genome_main(int argc, char **argv);
+
 /* lambda/K surrogate network.  No counterpart in the reference, which obtains
  * lambda and K only by permutation (ranSco, permutationLsqLambdaK.cpp:32-102,
  * then scripts/scriptToestimateKandLambda.r).  Inputs per row (8 floats):

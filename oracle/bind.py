@@ -325,3 +325,54 @@ def sw_max_batch(seqs, ref_off, ref_len, qry_off, qry_len, sc=(2, -3, -4, -2), n
             np.ascontiguousarray(qry_len, np.int32)]
     f(*[a.ctypes.data for a in arrs], n, *sc, out.ctypes.data, nthreads)
     return out
+
+
+# -- maskGenome --------------------------------------------------------------
+
+def mask_genome(d, opts, out="out.fa"):
+    """The oracle's maskGenome on the files mask_args wrote into folder d
+    (``opts`` as in dcns_amd.synth.mask_case).  Returns (status, stdout text)."""
+    f = oracle().lib.orc_mask_genome
+    f.argtypes = [ctypes.c_char_p] * 3 + [_I, ctypes.c_char_p, _I, ctypes.c_char_p, ctypes.c_char_p, _D,
+                                          ctypes.c_char, _I, _I, ctypes.c_char_p, _I]
+    p = lambda name: os.path.join(d, name).encode()  # noqa: E731
+    log = ctypes.create_string_buffer(1 << 16)
+    rc = f(p("genome.fa"), p(out), p("kmers.fa") if opts["use_dump"] else None, opts["freq"],
+           p("ann.gff3") if opts["use_gff"] else None, int(opts["gene"]),
+           p("cds.sam") if opts["use_sam"] else None, p("cds.fa") if opts["use_sam"] else None,
+           opts["similarity"], opts["alt"].encode()[:1], int(opts["soft"]), opts["width"], log, len(log))
+    return rc, log.value.decode()
+
+
+def reference_mask_genome(d, args, timeout=120):
+    """oracle/_ref/dCNS_ref maskGenome in folder d, or None where it is not built."""
+    import subprocess
+    if not os.path.exists(REF_CLI):
+        return None
+    return subprocess.run([REF_CLI, "maskGenome"] + list(args), cwd=d, capture_output=True, text=True,
+                          timeout=timeout)
+
+
+def kmer_mask(dump, freq, seqs):
+    """The oracle's k-mer scan over byte strings seqs: (k, [0/1 uint8 arrays])."""
+    f = oracle().lib.orc_kmer_mask
+    f.argtypes = [ctypes.c_char_p, _I, _P, _P, _I, _P]
+    buf = np.frombuffer(b"".join(seqs) or b"\0", np.uint8)
+    off = np.zeros(len(seqs) + 1, np.int64)
+    off[1:] = np.cumsum([len(s) for s in seqs])
+    mask = np.zeros(max(1, int(off[-1])), np.uint8)
+    k = f(dump.encode() if dump else None, freq, buf.ctypes.data, off.ctypes.data, len(seqs), mask.ctypes.data)
+    return k, [mask[off[i]:off[i + 1]].copy() for i in range(len(seqs))]
+
+
+def kmer_members(dump, freq):
+    """The oracle's k-mer profile: (k, sorted list of members of length k)."""
+    f = oracle().lib.orc_kmer_members
+    f.argtypes = [ctypes.c_char_p, _I, _P, _L, _P]
+    k = np.zeros(1, np.int32)
+    cap = max(1, os.path.getsize(dump)) if dump and os.path.exists(dump) else 1
+    out = np.zeros(cap, np.uint8)
+    n = f(dump.encode() if dump else None, freq, out.ctypes.data, cap, k.ctypes.data)
+    assert n >= 0, n
+    kk = int(k[0])
+    return kk, [out[i * kk:(i + 1) * kk].tobytes() for i in range(n)]

@@ -24,6 +24,11 @@
 //   orc_find_similar_weighted profile for every category the two-piece form
 //                             equals SemiGlobal's recurrence except at the
 //                             band's last column (tested).
+//   orc_mask_genome,          golden files of the compiled reference's
+//   orc_kmer_mask             maskGenome command + live comparison with
+//                             oracle/_ref/dCNS_ref on random genomes, dumps,
+//                             GFF3 and SAM files; brute force (a window
+//                             compared with every member) for the scan.
 #pragma once
 #include <cstdint>
 
@@ -129,6 +134,23 @@ int orc_find_similar_seeded(const int8_t *seq1, int32_t L1, const int8_t *seq2, 
 int orc_sw_max_batch(const int8_t *seqs, const int64_t *ref_off, const int32_t *ref_len, const int64_t *qry_off,
                      const int32_t *qry_len, int32_t n, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
                      int32_t *out_max, int32_t nthreads);
+
+// maskGenome, controlLayer4.cpp:1999-2084 with impl/maskGenome.cpp:8-123 and
+// impl/getAllTheCdsPositions.cpp:9-173 (mask_oracle.cpp): reads fasta, the
+// jellyfish dump (null: none), the GFF3 (null: none; gene != 0 is -z) and the
+// SAM + CDS fasta (null: none) and writes the masked fasta to out.  log
+// receives the command's stdout lines, or the failure on error (returns -1).
+int orc_mask_genome(const char *fasta, const char *out, const char *dump, int32_t freq, const char *gff,
+                    int32_t gene, const char *sam, const char *cds_fasta, double similarity, char alt, int32_t soft,
+                    int32_t width, char *log, int32_t logcap);
+// The k-mer scan alone (maskGenome.cpp:15-61) over sequences
+// seq[offsets[s], offsets[s + 1]): mask = 0/1 per position.  Returns k, or -1
+// when a sequence is shorter than k or the dump cannot be read.
+int orc_kmer_mask(const char *dump, int32_t freq, const char *seq, const int64_t *offsets, int32_t nseq,
+                  uint8_t *mask);
+// The distinct members of length k of the dump's k-mer profile, k bytes each
+// in sorted order; returns their number (-2: cap too small, -1: error).
+int orc_kmer_members(const char *dump, int32_t freq, char *out, int64_t cap, int32_t *k);
 
 #ifdef __cplusplus
 }

@@ -12,8 +12,9 @@ of findSimilarFragmentsForPairedSequence (overloads 1 and 4), the weighted
 paths (score profiles, weighted SemiGlobal tasks, the _wighted / _wighted_1gap
 overloads, GFF categories), overload 3 (seeded), and the text outputs of the
 cut1Gap, cut2Gaps, pairCnsXExtend, ranSco, weighted1Gap, weighted2Gaps and
-cut2Gaps2 commands.  Only data is written.  ``--weighted`` / ``--seeded``
-write those fixtures only.
+cut2Gaps2 commands, and the maskGenome command's files.  Only data is
+written.  ``--weighted`` / ``--seeded`` / ``--mask`` write those fixtures
+only.
 """
 import json
 import os
@@ -468,8 +469,59 @@ def write_cut1gap_inputs(d, rng):
             fh.write("ACGT\n")
 
 
+def write_mask(here):
+    """maskGenome: inputs and the reference command's output file and stdout,
+    one folder per case under mask/ (case.json: flags, expected status)."""
+    root = os.path.join(here, "mask")
+    if os.path.isdir(root):
+        shutil.rmtree(root)
+    os.makedirs(root)
+    rng = np.random.default_rng(20261204)
+    cases = []
+    while len(cases) < 14:  # random inputs that the reference completes on
+        c = synth.mask_case(rng)
+        with tempfile.TemporaryDirectory() as d:
+            args = synth.mask_args(c, d)
+            r = bind.reference_mask_genome(d, args)
+            if r.returncode == 0:
+                cases.append(c)
+    # a realistic one: a repetitive chromosome pair and its 20-mer counts
+    g1 = synth.repeat_genome(rng, 150000, families=25)
+    g2 = synth.repeat_genome(rng, 60000, families=10)
+    km, cnt = synth.kmer_counts(np.concatenate([g1, g2]), 20, 1)
+    sel = rng.random(len(km)) < 0.5  # part of the low-count tail
+    cases.append({"genome": f">chr1\n{synth._wrap(g1.tobytes().decode(), 60)}\n>chr2\n"
+                            f"{synth._wrap(g2.tobytes().decode(), 60)}\n",
+                  "dump": synth.dump_text(km[sel], cnt[sel]), "gff": "", "sam": "", "cds": "",
+                  "opts": {"width": 60, "soft": False, "alt": "n", "gene": False, "similarity": 0.8, "freq": 3,
+                           "use_dump": True, "use_gff": False, "use_sam": False}, "k": 20, "names": ["chr1", "chr2"]})
+    # failures: a sequence shorter than k, line width 0, a CDS past the end of its sequence
+    short = {"genome": ">a\nACGTACGTACGTAAA\n>b\nACG\n", "dump": ">9\nACGTA\n", "gff": "", "sam": "", "cds": "",
+             "opts": {"width": 60, "soft": False, "alt": "n", "gene": False, "similarity": 0.8, "freq": 2,
+                      "use_dump": True, "use_gff": False, "use_sam": False}, "k": 5, "names": ["a", "b"]}
+    zero = dict(short, genome=">a\nACGTACGTACGTAAA\n", opts=dict(short["opts"], width=0))
+    past = dict(short, genome=">a\nACGTACGTACGTAAA\n", gff="a\ts\tCDS\t40\t50\t.\t+\t.\tID=x\n",
+                opts=dict(short["opts"], use_gff=True))
+    for i, c in enumerate(cases + [short, zero, past]):
+        d = os.path.join(root, f"case{i:02d}")
+        os.makedirs(d)
+        args = synth.mask_args(c, d, "expected.fa")
+        for f in ("kmers.fa", "ann.gff3", "cds.sam", "cds.fa"):  # keep only the inputs the flags name
+            if f not in args:
+                os.remove(os.path.join(d, f))
+        r = bind.reference_mask_genome(d, args)
+        meta = {"args": args, "status": r.returncode, "stdout": r.stdout, "opts": c["opts"]}
+        if r.returncode != 0 and os.path.exists(os.path.join(d, "expected.fa")):
+            os.remove(os.path.join(d, "expected.fa"))
+        with open(os.path.join(d, "case.json"), "w") as f:
+            json.dump(meta, f, indent=1)
+    print("mask cases:", len(cases) + 3)
+
+
 if __name__ == "__main__":
-    if "--weighted" in sys.argv[1:] or "--seeded" in sys.argv[1:]:
+    if "--mask" in sys.argv[1:]:
+        write_mask(HERE)
+    elif "--weighted" in sys.argv[1:] or "--seeded" in sys.argv[1:]:
         r = bind.reference()
         if r is None:
             sys.exit("oracle/_ref is not built")

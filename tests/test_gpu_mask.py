@@ -1,0 +1,238 @@
+"""maskGenome on the GPU: the dCNS maskGenome command against the compiled
+reference's files (output bytes, stdout, failures), the command against the
+oracle on random inputs, the C-ABI scan (dcns_mask_stage / dcns_mask_run)
+against the oracle's scan across k, member alphabets and sequence layouts,
+CDS ranges over the k-mer mask, and, at 64 Mb, the scan against an
+independent numpy computation (window codes, sorted-member lookup, cover by
+prefix sums).  Masks and files are compared exactly."""
+import glob
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import dcns_amd
+from dcns_amd import synth
+from oracle import bind
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mask")
+CASES = sorted(glob.glob(os.path.join(GOLD, "case*")))
+
+
+def _run_cli(d, args):
+    print("dCNS maskGenome", " ".join(args), flush=True)
+    return subprocess.run([dcns_amd.CLI_PATH, "maskGenome"] + args, cwd=d, capture_output=True, text=True,
+                          timeout=120)
+
+
+@pytest.mark.parametrize("case", CASES, ids=os.path.basename)
+def test_cli_matches_golden(case, tmp_path):
+    for f in os.listdir(case):
+        if not f.startswith("expected") and f != "case.json":
+            shutil.copy(os.path.join(case, f), tmp_path)
+    meta = json.load(open(os.path.join(case, "case.json")))
+    r = _run_cli(tmp_path, meta["args"])
+    if meta["status"] != 0:
+        assert r.returncode != 0
+        assert not os.path.exists(tmp_path / "expected.fa")
+        return
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == meta["stdout"]
+    assert open(tmp_path / "expected.fa", "rb").read() == open(os.path.join(case, "expected.fa"), "rb").read()
+
+
+def test_cli_vs_oracle_random(tmp_path):
+    rng = np.random.default_rng(2024)
+    for t in range(16):
+        case = synth.mask_case(rng)
+        d = tmp_path / f"c{t}"
+        d.mkdir()
+        args = synth.mask_args(case, str(d), "gpu.fa")
+        rc, log = bind.mask_genome(str(d), case["opts"], "orc.fa")
+        r = _run_cli(str(d), args)
+        assert (r.returncode == 0) == (rc == 0), (t, r.stderr, log)
+        if rc == 0:
+            assert r.stdout == log, t
+            assert open(d / "gpu.fa", "rb").read() == open(d / "orc.fa", "rb").read(), t
+
+
+def _dump(path, members, k_last="A"):
+    """A dump whose records are the members (count 9) and a last record, counted
+    once, that sets k."""
+    with open(path, "w") as f:
+        for m in members:
+            f.write(f">9\n{m.decode() if isinstance(m, bytes) else m}\n")
+        f.write(f">1\n{k_last}\n")
+    return str(path)
+
+
+def _check_scan(gpu, tmp_path, seqs, members, k, tag, freq=5):
+    dump = _dump(tmp_path / f"{tag}.fa", members, "A" * k)
+    ks = dcns_amd.KmerSet(dump, freq)
+    assert ks.k == k
+    print("kmer scan", tag, "k", k, "members", ks.members, "sequences", len(seqs),
+          "bases", sum(len(s) for s in seqs), flush=True)
+    masks, _ = gpu.mask_genome(seqs, ks, width=60)
+    k_o, want = bind.kmer_mask(dump, freq, seqs)
+    assert k_o == k
+    for i, (m, w) in enumerate(zip(masks, want)):
+        assert np.array_equal(m, w), (tag, i, int(np.sum(m != w)))
+    ks.close()
+    return masks
+
+
+@pytest.mark.parametrize("k", [1, 5, 12, 20, 31, 32, 37])
+def test_scan_vs_oracle_across_k(gpu, tmp_path, k):
+    rng = np.random.default_rng(100 + k)
+    seqs = [synth.repeat_genome(rng, int(rng.integers(k, 120000)), families=8).tobytes() for _ in range(3)]
+    seqs.append(b"ACGT" * k)  # several windows only
+    seqs.append(synth.repeat_genome(rng, k, families=0).tobytes())  # exactly one window
+    pool = b"".join(seqs)
+    members = set()
+    for _ in range(4000):
+        a = int(rng.integers(0, len(pool) - k + 1))
+        members.add(pool[a:a + k])
+    members |= {bytes(rng.choice(list(b"ACGT"), k)) for _ in range(500)}
+    masks = _check_scan(gpu, tmp_path, seqs, sorted(members), k, f"k{k}")
+    assert any(m.any() for m in masks)
+
+
+def test_scan_other_bytes_and_case(gpu, tmp_path):
+    """Members holding N and other bytes (general table) next to ACGT members
+    (code table); lower-case runs, which never match; many small sequences so
+    that lanes span sequence ends."""
+    rng = np.random.default_rng(7)
+    k = 9
+    seqs = []
+    for i in range(700):
+        s = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, int(rng.integers(k, 3 * k + 40)))].copy()
+        for _ in range(int(rng.integers(0, 3))):
+            a = int(rng.integers(0, len(s)))
+            s[a:a + int(rng.integers(1, 12))] = ord(str(rng.choice(list("NnRacgt-"))))
+        seqs.append(s.tobytes())
+    pool = b"".join(seqs)
+    members = set()
+    for _ in range(6000):
+        a = int(rng.integers(0, len(pool) - k + 1))
+        members.add(pool[a:a + k].upper())
+    members = sorted(m for m in members if b"\n" not in m and b">" not in m)
+    masks = _check_scan(gpu, tmp_path, seqs, members, k, "mixed")
+    assert sum(int(m.sum()) for m in masks) > 0
+
+
+def test_scan_edge_sets(gpu, tmp_path):
+    rng = np.random.default_rng(3)
+    seqs = [synth.repeat_genome(rng, 5000, families=3).tobytes(), b"ACGTACGTAC", b"acgtACGTNNACGT"]
+    # no member above the threshold: nothing is masked
+    m = _check_scan(gpu, tmp_path, seqs, [b"ACGTACGTAC"], 10, "none", freq=50)
+    assert not any(x.any() for x in m)
+    # one member equal to a whole sequence
+    _check_scan(gpu, tmp_path, seqs, [b"ACGTACGTAC", b"CGTACGTACG"], 10, "whole")
+    # no dump at all: k = 10 and no members
+    masks, _ = gpu.mask_genome(seqs, None)
+    assert not any(x.any() for x in masks)
+
+
+def test_short_sequence_and_width_errors(gpu, tmp_path):
+    dump = _dump(tmp_path / "d.fa", [b"ACGTA"], "AAAAA")
+    ks = dcns_amd.KmerSet(dump, 5)
+    with pytest.raises(dcns_amd.DcnsError):
+        gpu.mask_stage([b"ACGTACGT", b"ACG"], ks)  # shorter than k = 5
+    with pytest.raises(dcns_amd.DcnsError):
+        gpu.mask_stage([b"ACGTACGT", b"ACGTACGTA"], None)  # no dump: k = 10
+    gpu.mask_stage([b"ACGTACGT", b"ACGTA"], ks)
+    with pytest.raises(dcns_amd.DcnsError):
+        gpu.mask_run(width=0)
+    masks, bodies = gpu.mask_run(width=3)
+    assert masks[1].tolist() == [1] * 5 and bodies[1] == b"nnn\nnn\n"
+    ks.close()
+
+
+def test_bodies_and_cds_vs_oracle(gpu, tmp_path):
+    """C-ABI bodies with CDS masks from a GFF3 and a SAM file against the
+    oracle's command output on the same files."""
+    rng = np.random.default_rng(31)
+    for t in range(6):
+        case = synth.mask_case(rng)
+        case["opts"].update(use_dump=True, use_gff=True, use_sam=t % 2 == 0)
+        d = tmp_path / f"c{t}"
+        d.mkdir()
+        synth.mask_args(case, str(d))
+        rc, log = bind.mask_genome(str(d), case["opts"], "orc.fa")
+        if rc != 0:
+            continue
+        # the genome as the command's reader leaves it: names from the headers,
+        # only the last record upper-cased; the library's sequence order is by name
+        read = {n: (q.upper() if i == len(case["seqs"]) - 1 else q).encode()
+                for i, (n, q) in enumerate(zip(case["names"], case["seqs"]))}
+        order = case["names"]
+        ordered = sorted(read)
+        seqs = [read[n] for n in ordered]
+        cds = dcns_amd.CdsMask(ordered, [len(s) for s in seqs])
+        cds.gff(str(d / "ann.gff3"), case["opts"]["gene"])
+        if case["opts"]["use_sam"]:
+            cds.sam(str(d / "cds.sam"), str(d / "cds.fa"), case["opts"]["similarity"])
+        ks = dcns_amd.KmerSet(str(d / "kmers.fa"), case["opts"]["freq"])
+        o = case["opts"]
+        print("bodies", t, o, flush=True)
+        _, bodies = gpu.mask_genome(seqs, ks, cds, alt=o["alt"], soft=o["soft"], width=o["width"])
+        got = b"".join(b">" + n.encode() + b"\n" + bodies[ordered.index(n)] for n in order)
+        assert got == open(d / "orc.fa", "rb").read(), t
+        ks.close()
+        cds.close()
+
+
+def _numpy_mask(g, off, members, k):  # members: [m x k] uint8
+    """Independent mask of ACGT-only sequences: 2-bit window codes, lookup in
+    the sorted member codes, cover by prefix sums."""
+    def codes_of(arr):
+        b = (arr.astype(np.uint64) >> np.uint64(1)) & np.uint64(3)
+        n = len(arr) - k + 1
+        c = np.zeros(n, np.uint64)
+        for j in range(k):
+            c = (c << np.uint64(2)) | b[j:j + n]
+        return c
+    mb = (members.astype(np.uint64) >> np.uint64(1)) & np.uint64(3)
+    mc = np.zeros(len(members), np.uint64)
+    for j in range(k):
+        mc = (mc << np.uint64(2)) | mb[:, j]
+    mc = np.unique(mc)
+    out = np.zeros(len(g), np.uint8)
+    for s in range(len(off) - 1):
+        a, e = int(off[s]), int(off[s + 1])
+        c = codes_of(g[a:e])
+        idx = np.minimum(np.searchsorted(mc, c), len(mc) - 1)
+        hit = mc[idx] == c
+        d = np.zeros(e - a + 1, np.int32)
+        h = np.nonzero(hit)[0]  # distinct starts
+        d[h] += 1
+        d[h + k] -= 1
+        out[a:e] = np.cumsum(d[:-1]) > 0
+    return out
+
+
+def test_scan_64mb_vs_numpy(gpu, tmp_path):
+    rng = np.random.default_rng(64)
+    k = 20
+    parts = [synth.repeat_genome(rng, n, families=60) for n in (40_000_000, 16_000_000, 8_000_000)]
+    g = np.concatenate(parts)
+    off = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.int64)
+    sample = g[:6_000_000]
+    km, cnt = synth.kmer_counts(sample, k, 2)
+    members = [r.tobytes() for r in km]
+    dump = _dump(tmp_path / "big.fa", members, "A" * k)
+    ks = dcns_amd.KmerSet(dump, 5)
+    print("64 Mb scan, members", ks.members, flush=True)
+    seqs = [p.tobytes() for p in parts]
+    masks, _ = gpu.mask_genome(seqs, ks, width=60)
+    got = np.concatenate(masks)
+    want = _numpy_mask(g, off, km, k)
+    assert np.array_equal(got, want), int(np.sum(got != want))
+    assert 0.05 < got.mean() < 0.95
+    masks2, _ = gpu.mask_run(width=60)  # the staged genome again: same result
+    assert np.array_equal(np.concatenate(masks2), got)
+    ks.close()
