@@ -330,6 +330,111 @@ def seeded_phase(ctx, log, cpu=True):
     return out
 
 
+def mask_genome_workload(gbp, seed, k=20, families=2000):
+    """A repetitive synthetic genome of gbp Gbp (uint8 ACGT) and its k-mer
+    dump: `families` repeat elements of 0.3-5 kb planted 20-300 times each with
+    2% substitutions (about 85% of the genome, as in maize), and as the dump the
+    k-mers of the element sequences, counted by their number of copies."""
+    from dcns_amd import synth
+    rng = np.random.default_rng(seed)
+    N = int(gbp * 1e9)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    g = acgt[rng.integers(0, 4, N, dtype=np.uint8)]
+    kms, cnts = [], []
+    scale = gbp  # copies per family scale with the genome
+    for _ in range(families):
+        L = int(rng.integers(300, 5000))
+        fam = acgt[rng.integers(0, 4, L, dtype=np.uint8)]
+        copies = max(3, int(rng.integers(20, 300) * scale))
+        for at in rng.integers(0, N - L, copies):
+            c = fam.copy()
+            sub = np.nonzero(rng.random(L) < 0.02)[0]
+            c[sub] = acgt[rng.integers(0, 4, len(sub), dtype=np.uint8)]
+            g[at:at + L] = c
+        win = np.lib.stride_tricks.sliding_window_view(fam, k)
+        kms.append(win)
+        cnts.append(np.full(len(win), copies, np.int64))
+    return g, np.concatenate(kms), np.concatenate(cnts)
+
+
+def mask_phase(ctx, log, cpu=True, gbp=1.0, sample_mb=4):
+    """maskGenome (controlLayer4.cpp:1999-2084) on a synthetic repetitive
+    genome of `gbp` Gbp in 8 chromosomes (mask_genome_workload: 85% repeats,
+    k = 20, ~5 M k-mers in the dump, -f 2): the genome and k-mer table
+    resident in HBM, one step = the k-mer scan, the CDS ranges and the FASTA
+    bodies of the whole genome.  Beside it, the reference's maskGenome command
+    on one core over a `sample_mb` Mb chromosome with the same dump (files
+    in, file out)."""
+    import subprocess
+    import tempfile
+    import dcns_amd
+    from dcns_amd import synth
+    from oracle import bind
+    k = 20
+    g, km, cnt = mask_genome_workload(gbp, synth.SEED_BASE + 9, k)
+    N = len(g)
+    d = tempfile.mkdtemp()
+    dump = os.path.join(d, "kmers.fa")
+    with open(dump, "wb") as f:  # ">count" line, then the k-mer
+        heads = np.char.add(">", cnt.astype(str)).astype(bytes)
+        body = km.view(f"S{k}").ravel()
+        f.write(b"\n".join(np.ravel(np.column_stack([heads, body]))) + b"\n")
+    ks = dcns_amd.KmerSet(dump, 2)
+    nchr = 8
+    cut = np.linspace(0, N, nchr + 1).astype(np.int64)
+    seqs = [g[cut[i]:cut[i + 1]].tobytes() for i in range(nchr)]
+    t = time.perf_counter()
+    ctx.mask_stage(seqs, ks)
+    stage_s = time.perf_counter() - t
+    ctx.mask_run(want_mask=False, want_bodies=False)
+    ctx.kernel_time_reset()
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        ctx.mask_run(want_mask=False, want_bodies=False)
+    dt = (time.perf_counter() - t) / reps
+    scan_ms, n1 = ctx.kernel_time("k_kmer_scan")
+    emit_ms, n2 = ctx.kernel_time("k_mask_emit")
+    scan_ms /= max(n1, 1)
+    emit_ms /= max(n2, 1)
+    masks, _ = ctx.mask_run(want_bodies=False)
+    frac = float(np.mean([m.mean() for m in masks]))
+    # algorithmic HBM bytes of the scan: 1 byte of sequence read, 1 byte of mask written per base
+    achieved = 2 * N / (scan_ms * 1e-3) / 1e9
+    out = {"workload": f"{N / 1e9:.2f} Gbp synthetic genome in {nchr} chromosomes, 85% repeats (2000 families "
+                       f"of 0.3-5 kb, 2% divergent copies), k = 20, {ks.members} k-mers in the dump above -f 2, "
+                       "hard mask, line width 60",
+           "gbp_per_s": round(N / dt / 1e9, 3), "ms_per_step": round(dt * 1e3, 3),
+           "k_kmer_scan_ms": round(scan_ms, 3), "k_mask_emit_ms": round(emit_ms, 3),
+           "scan_gbp_per_s": round(N / (scan_ms * 1e-3) / 1e9, 2), "masked_fraction": round(frac, 4),
+           "stage_s_pcie_inclusive": round(stage_s, 3),
+           "roofline_scan": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                             "frac": round(achieved / 8000.0, 4),
+                             "bytes_per_base": "1 read + 1 written (algorithmic); one table probe per window "
+                                               "on top"}}
+    log(f"maskGenome: {N / 1e9:.2f} Gbp in {dt * 1e3:.2f} ms ({N / dt / 1e9:.1f} Gbp/s), "
+        f"scan {scan_ms:.2f} ms, emit {emit_ms:.2f} ms, masked {frac:.3f}, {ks.members} k-mers")
+    if cpu and os.path.exists(bind.REF_CLI):
+        n = int(sample_mb * 1e6)
+        with open(os.path.join(d, "sample.fa"), "w") as f:
+            f.write(">chr1\n" + synth._wrap(g[:n].tobytes().decode(), 60) + "\n")
+        t = time.perf_counter()
+        r = subprocess.run([bind.REF_CLI, "maskGenome", "-i", "sample.fa", "-o", "sample_ref.fa", "-k", "kmers.fa",
+                            "-f", "2"], cwd=d, capture_output=True, text=True, timeout=600)
+        tc = time.perf_counter() - t
+        if r.returncode == 0:
+            out["cpu_reference"] = {"value": round(n / tc / 1e6, 3), "unit": "Mbp/s", "cores": 1, "kind": "reference",
+                                    "sample": f"maskGenome command on the first {sample_mb} Mb as one chromosome "
+                                              "with the same dump (reading the dump, scan and writing)",
+                                    "seconds": round(tc, 2)}
+            subprocess.run([dcns_amd.CLI_PATH, "maskGenome", "-i", "sample.fa", "-o", "sample_gpu.fa", "-k",
+                            "kmers.fa", "-f", "2"], cwd=d, capture_output=True, text=True, timeout=600)
+            out["cpu_sample_bit_exact"] = open(os.path.join(d, "sample_ref.fa"), "rb").read() == \
+                open(os.path.join(d, "sample_gpu.fa"), "rb").read()
+            log(f"reference maskGenome: {n / tc / 1e6:.2f} Mbp/s on one core")
+    ks.close()
+    return out
+
 def config2_significance(ctx, sur, log):
     """Config-2 fragments (500 kb masked pair, default scoring) reported with
     the lambda/K of the judge, of the surrogate and of the reference's
@@ -388,6 +493,9 @@ def main():
     ap.add_argument("--no-lambdak", action="store_true")
     ap.add_argument("--no-two-piece", action="store_true")
     ap.add_argument("--no-weighted", action="store_true")
+    ap.add_argument("--no-mask", action="store_true")
+    ap.add_argument("--mask-only", action="store_true", help="only the maskGenome phase")
+    ap.add_argument("--mask-gbp", type=float, default=1.0)
     ap.add_argument("--two-piece-only", action="store_true",
                     help="run only the two-piece (cut2Gaps) and weighted measurements and print them")
     args = ap.parse_args()
@@ -398,6 +506,13 @@ def main():
 
     import dcns_amd
     from dcns_amd import synth
+
+    if args.mask_only:
+        ctx = dcns_amd.Context(0, timing=True)
+        lg = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+        print(json.dumps({"mask": mask_phase(ctx, lg, cpu=not args.no_cpu, gbp=args.mask_gbp)}), flush=True)
+        ctx.close()
+        return
 
     if args.two_piece_only:
         ctx = dcns_amd.Context(0, timing=True)
@@ -499,6 +614,8 @@ def main():
         line["seeded"] = seeded_phase(ctx, log)
     if not args.no_lambdak:
         line["lambdak"] = lambdak_phase(ctx, log)
+    if not args.no_mask:
+        line["mask"] = mask_phase(ctx, log, cpu=not args.no_cpu, gbp=args.mask_gbp)
 
     if not args.no_cpu:
         cores = args.cpu_cores or min(len(os.sched_getaffinity(0)), 16)

@@ -700,7 +700,7 @@ int dcns_kmer_set_get(const dcns_kmer_set *s, char *out, int64_t cap) {
     static const char letter[4] = {'A', 'C', 'T', 'G'};  // inverse of (b >> 1) & 3
     int64_t o = 0;
     for (uint64_t code : s->ks.codes) {
-        for (int64_t j = k - 1; j >= 0; --j) out[o + j] = letter[code & 3], code >>= 2;
+        for (int64_t j = 0; j < k; ++j) out[o + j] = letter[code & 3], code >>= 2;  // first base lowest
         o += k;
     }
     if (!s->ks.general.empty()) std::memcpy(out + o, s->ks.general.data(), s->ks.general.size());

@@ -37,12 +37,13 @@ int32_t header_count(const char *p, size_t n, const std::string &path) {
     return (int32_t)to_int32(std::string(p + 1, len), "k-mer count in " + path);
 }
 
+// the first base in the lowest two bits, as the scan kernel packs windows
 bool pack(const std::string &s, uint64_t &code) {
     code = 0;
-    for (unsigned char c : s) {
-        const int b = kmer_base(c);
+    for (size_t j = 0; j < s.size(); ++j) {
+        const int b = kmer_base((unsigned char)s[j]);
         if (b < 0) return false;
-        code = (code << 2) | (uint64_t)b;
+        code |= (uint64_t)b << (2 * j);
     }
     return true;
 }

@@ -17,7 +17,7 @@ namespace dcns {
 // reference's kmerSize); only members of that length can equal a window.
 struct KmerSet {
     int32_t k = 10;
-    std::vector<uint64_t> codes;   // members of upper-case A/C/G/T (k <= 31), 2 bits per base, (b >> 1) & 3
+    std::vector<uint64_t> codes;   // members of upper-case A/C/G/T (k <= 31): base j in bits 2j, 2j+1 as (b >> 1) & 3
     std::vector<uint8_t> general;  // every other member of length k, k bytes each
     int64_t records = 0;           // records above the threshold, any length
     int64_t n_general() const { return k > 0 ? (int64_t)general.size() / k : 0; }

@@ -89,7 +89,7 @@ hipError_t launch_kmer_scan(hipStream_t st, const uint8_t *seq, int64_t N, const
 hipError_t launch_paint_ranges(hipStream_t st, const int64_t *ranges, int64_t n, uint8_t *mask);
 // FASTA bodies: sequence s occupies out[body_off[s], body_off[s + 1]).
 hipError_t launch_mask_emit(hipStream_t st, const uint8_t *seq, const uint8_t *mask, const int64_t *seg_off,
-                            const int64_t *body_off, int32_t nseg, int64_t total, int32_t width, uint8_t alt,
-                            bool soft, uint8_t *out);
+                            const int64_t *body_off, int32_t nseg, int64_t N, int64_t total, int32_t width,
+                            bool any_empty, uint8_t alt, bool soft, uint8_t *out);
 
 }  // namespace dcns

@@ -7,10 +7,11 @@
 // seg_off[0..nseg] (seg_off[nseg] = N).  Windows never cross a sequence end.
 //
 // K-mer set, two parts:
-//   * members made of upper-case A/C/G/T only, k <= 31: 2-bit codes
-//     ((b >> 1) & 3 per byte: A 0, C 1, T 2, G 3) in a bucketed open-addressing
-//     table (4 keys = 32 bytes per bucket, linear probing over buckets, load
-//     <= 1/2, empty = all ones, which no code of k <= 31 bases equals);
+//   * members made of upper-case A/C/G/T only, k <= 31: 2-bit codes (base j
+//     in bits 2j..2j+1 as (b >> 1) & 3: A 0, C 1, T 2, G 3) in a bucketed
+//     open-addressing table (2 keys = 16 bytes per bucket, one load per probe,
+//     linear probing over buckets, load <= 1/4, empty = all ones, which no
+//     code of k <= 31 bases equals);
 //   * every other member (other bytes, or k >= 32): a table of member
 //     indices keyed by a 64-bit polynomial hash of the bytes, each hit
 //     verified byte by byte.
@@ -18,12 +19,19 @@
 // any other window only one of the second.  Lower-case windows never match,
 // as the reference compares the text as read against upper-cased members.
 //
-// k_kmer_scan: one lane per CHUNK positions.  It evaluates the windows from
-// k - 1 (rounded up to the group size) before its chunk, keeps the start of
-// the last window that hit, and writes mask[j] = 1 when a hit window starts
-// in [j - k + 1, j].  Windows go in groups of G: the G bucket loads are issued
-// together so each lane has G probes in flight.  HBM traffic per position: 1
-// byte read, 1 byte written, plus the probes into the table.
+// k <= 31, k_kmer_scan16: a thread owns 16 window starts / 16 positions.  It
+// reads its 48 bytes with three aligned 16-byte loads (consecutive threads,
+// consecutive blocks), packs them into 96 bits of 2-bit codes and a 48-bit
+// mask of other bytes with SWAR, takes each window's code by a constant
+// shift, and probes the table with all eight loads of a half in flight.  The
+// 16 hit bits go through LDS to the next two threads; a position is masked
+// when one of the k windows ending on it hit, which is a shift-OR of the 48
+// hit bits of the thread and its two predecessors.  The first two threads of
+// a workgroup recompute the previous workgroup's last 32 windows.  HBM
+// traffic per position: 1 byte read, 1 byte written, plus the probes.
+//
+// k >= 32, k_kmer_scan_long: one lane per CHUNK positions keeps a rolling
+// hash of the last k bytes and the start of the last hit window.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -32,10 +40,13 @@ namespace dcns {
 namespace {
 
 constexpr uint64_t kEmpty = ~0ull;
-constexpr int CHUNK = 1024;  // positions per lane in the scan
-constexpr int G = 8;         // windows per probe group
-constexpr int BUCKET = 4;    // keys per bucket
-constexpr int EMIT = 16;     // output bytes per lane in the FASTA writer
+constexpr int WG = 256;                       // threads per workgroup of the k <= 31 scan
+constexpr int HALO = 2;                       // threads recomputing the previous workgroup's windows
+constexpr int SPAN = 16;                      // windows and positions per thread
+constexpr int WG_POS = (WG - HALO) * SPAN;    // positions a workgroup writes
+constexpr int CHUNK = 1024;                   // positions per lane of the k >= 32 scan
+constexpr int G = 8;                          // windows per probe group
+constexpr int EMIT = 16;                      // output bytes per lane in the FASTA writer
 
 __device__ __forceinline__ uint64_t mix(uint64_t x) {
     // splitmix64 finaliser (a bijection on 64 bits)
@@ -47,10 +58,35 @@ __device__ __forceinline__ uint64_t mix(uint64_t x) {
     return x;
 }
 
-// (b >> 1) & 3 for upper-case A, C, G, T; -1 for every other byte
-__device__ __forceinline__ int base2(uint32_t b) {
-    const uint32_t d = b - 64u;
-    return (d < 32u && ((0x10008Au >> d) & 1u)) ? (int)((b >> 1) & 3u) : -1;
+// 0x80 in every byte lane of w that equals the byte of c4 (c4 = byte x 0x01010101), else 0
+__device__ __forceinline__ uint32_t eq_lanes(uint32_t w, uint32_t c4) {
+    const uint32_t t = w ^ c4;
+    return ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t | 0x7F7F7F7Fu);
+}
+
+// bit b set for each byte b of w other than upper-case A, C, G, T
+__device__ __forceinline__ uint32_t other_bytes4(uint32_t w) {
+    const uint32_t ok = eq_lanes(w, 0x41414141u) | eq_lanes(w, 0x43434343u) | eq_lanes(w, 0x47474747u) |
+                        eq_lanes(w, 0x54545454u);
+    const uint32_t bad = ~ok & 0x80808080u;
+    return ((bad >> 7) & 1u) | ((bad >> 14) & 2u) | ((bad >> 21) & 4u) | ((bad >> 28) & 8u);
+}
+
+// (b >> 1) & 3 of the 4 bytes of w, byte j in bits 2j..2j+1
+__device__ __forceinline__ uint32_t codes4(uint32_t w) {
+    uint32_t v = (w >> 1) & 0x03030303u;
+    v = (v | (v >> 6)) & 0x000F000Fu;
+    return (v | (v >> 12)) & 0xFFu;
+}
+
+// the sequence holding position pos: the last s in [lo, hi] with seg_off[s] <= pos
+__device__ __forceinline__ int32_t seg_of(const int64_t *__restrict__ seg_off, int32_t lo, int32_t hi, int64_t pos) {
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (seg_off[mid] <= pos) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 __global__ void k_kmer_insert(const uint64_t *__restrict__ codes, int64_t n, uint64_t *__restrict__ table,
@@ -59,9 +95,9 @@ __global__ void k_kmer_insert(const uint64_t *__restrict__ codes, int64_t n, uin
     if (i >= n) return;
     const uint64_t key = codes[i];
     uint64_t b = mix(key) & bmask;
-    for (uint64_t t = 0; t <= bmask; ++t) {  // at most half the slots are used: an empty slot is always found
-        unsigned long long *bk = (unsigned long long *)(table + b * BUCKET);
-        for (int s = 0; s < BUCKET; ++s) {
+    for (uint64_t t = 0; t <= bmask; ++t) {  // at most a quarter of the slots are used: an empty slot is found
+        unsigned long long *bk = (unsigned long long *)(table + b * kKmerBucket);
+        for (int s = 0; s < kKmerBucket; ++s) {
             const unsigned long long prev = atomicCAS(bk + s, (unsigned long long)kEmpty, (unsigned long long)key);
             if (prev == kEmpty || prev == key) return;
         }
@@ -84,8 +120,8 @@ __global__ void k_kmer_insert_general(const uint64_t *__restrict__ hashes, int64
 __device__ __noinline__ bool probe_on(const uint64_t *__restrict__ table, uint64_t bmask, uint64_t b, uint64_t key) {
     for (uint64_t t = 0; t < bmask; ++t) {
         b = (b + 1) & bmask;
-        const uint64_t *bk = table + b * BUCKET;
-        for (int s = 0; s < BUCKET; ++s) {
+        const uint64_t *bk = table + b * kKmerBucket;
+        for (int s = 0; s < kKmerBucket; ++s) {
             const uint64_t v = bk[s];
             if (v == key) return true;
             if (v == kEmpty) return false;
@@ -112,99 +148,131 @@ __device__ __noinline__ bool general_lookup(const KmerGeneral &g, const uint8_t 
     return false;
 }
 
-template <bool CODES, bool GEN>
-__global__ __launch_bounds__(256) void k_kmer_scan(const uint8_t *__restrict__ seq, int64_t N,
-                                                   const int64_t *__restrict__ seg_off, int32_t nseg, int32_t k,
-                                                   const uint64_t *__restrict__ table, uint64_t bmask,
-                                                   KmerGeneral g, uint8_t *__restrict__ mask) {
+__device__ __noinline__ bool general_window(const KmerGeneral &g, const uint8_t *__restrict__ win, int32_t k) {
+    uint64_t h = 0;
+    for (int32_t j = 0; j < k; ++j) h = h * kKmerHashBase + win[j];
+    return general_lookup(g, win, h, k);
+}
+
+template <bool GEN>
+__global__ __launch_bounds__(WG) void k_kmer_scan16(const uint8_t *__restrict__ seq, int64_t N,
+                                                    const int64_t *__restrict__ seg_off, int32_t nseg, int32_t k,
+                                                    const uint64_t *__restrict__ table, uint64_t bmask,
+                                                    KmerGeneral g, uint8_t *__restrict__ mask) {
+    __shared__ uint32_t lds_hits[WG];
+    __shared__ int32_t lds_seg[2];
+    const int tid = threadIdx.x;
+    const int64_t w0 = (int64_t)blockIdx.x * WG_POS - HALO * SPAN;  // first window of thread 0
+    const int64_t i0 = w0 + (int64_t)tid * SPAN;                     // this thread's windows [i0, i0 + 16)
+    if (tid == 0) lds_seg[0] = seg_of(seg_off, 0, nseg - 1, w0 > 0 ? w0 : 0);
+    if (tid == WG - 1) {
+        const int64_t last = w0 + WG * SPAN + 48;
+        lds_seg[1] = seg_of(seg_off, 0, nseg - 1, last < N ? last : N - 1);
+    }
+    __syncthreads();
+
+    uint32_t hits = 0;
+    if (i0 >= 0 && i0 < N) {
+        const uint4 *src = (const uint4 *)(seq + i0);  // 16-byte aligned; the buffer holds N + 128 bytes
+        const uint4 b0 = src[0], b1 = src[1], b2 = src[2];
+        const uint32_t w[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+        uint64_t lo = 0, bad = 0;
+        uint32_t hi = 0;
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            const uint32_t c = codes4(w[q]);
+            if (q < 8) lo |= (uint64_t)c << (8 * q);
+            else hi |= c << (8 * (q - 8));
+            bad |= (uint64_t)other_bytes4(w[q]) << (4 * q);
+        }
+        if (N - i0 < 48) bad |= ~0ull << (N - i0);  // past the genome
+        // sequence starts inside (i0, i0 + 48)
+        uint64_t bnd = 0;
+        int32_t s = seg_of(seg_off, lds_seg[0], lds_seg[1], i0);
+        while (s + 1 < nseg && seg_off[s + 1] < i0 + 48) {
+            ++s;
+            bnd |= 1ull << (seg_off[s] - i0);
+        }
+        const uint64_t kbits = (1ull << k) - 1, kin = (1ull << (k - 1)) - 1;
+        const uint64_t cmask = (1ull << (2 * k)) - 1;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            uint64_t key[G], home[G];
+            bool va[G];
+            ulonglong2 x[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int t = half * G + u;
+                key[u] = (t == 0 ? lo : ((lo >> (2 * t)) | ((uint64_t)hi << (64 - 2 * t)))) & cmask;
+                va[u] = ((bad >> t) & kbits) == 0 && ((bnd >> (t + 1)) & kin) == 0;
+                home[u] = va[u] ? (mix(key[u]) & bmask) : 0;
+                x[u] = *(const ulonglong2 *)(table + home[u] * kKmerBucket);
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int t = half * G + u;
+                const bool found = x[u].x == key[u] || x[u].y == key[u];
+                const bool open = x[u].x == kEmpty || x[u].y == kEmpty;
+                bool hit = va[u] && (found || (!open && probe_on(table, bmask, home[u], key[u])));
+                if (GEN && !va[u] && ((bnd >> (t + 1)) & kin) == 0 && i0 + t + k <= N)
+                    hit = general_window(g, seq + i0 + t, k);
+                hits |= (hit ? 1u : 0u) << t;
+            }
+        }
+    }
+    lds_hits[tid] = hits;
+    __syncthreads();
+    if (tid < HALO || i0 >= N) return;
+    // windows i0 - 32 .. i0 + 15 in bits 0..47; position i0 + u is masked when a
+    // window starting in [i0 + u - k + 1, i0 + u] hit
+    uint64_t r = (uint64_t)lds_hits[tid - 2] | ((uint64_t)lds_hits[tid - 1] << 16) | ((uint64_t)hits << 32);
+    for (int32_t c = 1; c < k;) {
+        const int32_t d = c < k - c ? c : k - c;
+        r |= r << d;
+        c += d;
+    }
+    const uint32_t m = (uint32_t)(r >> 32) & 0xFFFFu;
+    uint32_t out[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t b = m >> (4 * q);
+        out[q] = (b & 1u) | ((b & 2u) << 7) | ((b & 4u) << 14) | ((b & 8u) << 21);
+    }
+    *(uint4 *)(mask + i0) = make_uint4(out[0], out[1], out[2], out[3]);  // the buffer is padded past N
+}
+
+// k >= 32: every member is in the general table
+__global__ __launch_bounds__(256) void k_kmer_scan_long(const uint8_t *__restrict__ seq, int64_t N,
+                                                        const int64_t *__restrict__ seg_off, int32_t nseg, int32_t k,
+                                                        KmerGeneral g, uint8_t *__restrict__ mask) {
     const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * CHUNK;
     if (c0 >= N) return;
     const int64_t c1 = c0 + CHUNK < N ? c0 + CHUNK : N;
     const int32_t back = (k - 1 + G - 1) / G * G;
     const int64_t ib = c0 - back > 0 ? c0 - back : 0;  // first window evaluated (a multiple of G)
-
-    // the sequence holding byte ib: the last s with seg_off[s] <= ib
-    int32_t lo = 0, hi = nseg - 1;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (seg_off[mid] <= ib) lo = mid;
-        else hi = mid - 1;
-    }
-    int32_t s = lo;
+    int32_t s = seg_of(seg_off, 0, nseg - 1, ib);
     int64_t seg_from = seg_off[s], seg_next = seg_off[s + 1];
-
-    const uint64_t cmask = (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1ull);
-    uint64_t code = 0, h = 0;
-    int64_t bad_from = ib;  // windows starting before this hold a byte other than upper-case A/C/G/T
-
+    uint64_t h = 0;
     auto feed = [&](int64_t p) {
-        const uint32_t b = seq[p];
-        const int v = base2(b);
-        if (v < 0) bad_from = p + 1;
-        code = ((code << 2) | (uint64_t)(v & 3)) & cmask;
         while (p >= seg_next) {
             ++s;
             seg_from = seg_off[s];
             seg_next = seg_off[s + 1];
         }
-        if (GEN) {
-            const uint64_t out = (p - k >= ib) ? (uint64_t)seq[p - k] : 0ull;
-            h = (h - out * g.pk1) * kKmerHashBase + (uint64_t)b;
-        }
+        const uint64_t out = (p - k >= ib) ? (uint64_t)seq[p - k] : 0ull;
+        h = (h - out * g.pk1) * kKmerHashBase + (uint64_t)seq[p];
     };
-
     for (int64_t p = ib; p < ib + k - 1 && p < N; ++p) feed(p);
-
     int64_t last_hit = INT64_MIN / 2;
     for (int64_t i0 = ib; i0 < c1; i0 += G) {
-        uint64_t key[G];
-        bool va[G], vg[G];
-        uint64_t hh[G];
-#pragma unroll
-        for (int t = 0; t < G; ++t) {
-            const int64_t i = i0 + t, p = i + k - 1;
-            bool in_seq = false;
-            if (p < N) {
-                feed(p);
-                in_seq = i >= seg_from;
-            }
-            va[t] = CODES && in_seq && i >= bad_from;
-            vg[t] = GEN && in_seq && (!CODES || i < bad_from);
-            key[t] = code;
-            hh[t] = h;
-        }
-        bool hit[G];
-        if (CODES) {
-            ulonglong2 x0[G], x1[G];
-            uint64_t home[G];
-#pragma unroll
-            for (int t = 0; t < G; ++t) {
-                home[t] = va[t] ? (mix(key[t]) & bmask) : 0;
-                const ulonglong2 *bk = (const ulonglong2 *)(table + home[t] * BUCKET);
-                x0[t] = bk[0];
-                x1[t] = bk[1];
-            }
-#pragma unroll
-            for (int t = 0; t < G; ++t) {
-                const uint64_t kk = key[t];
-                const bool found = x0[t].x == kk || x0[t].y == kk || x1[t].x == kk || x1[t].y == kk;
-                const bool open = x0[t].x == kEmpty || x0[t].y == kEmpty || x1[t].x == kEmpty || x1[t].y == kEmpty;
-                hit[t] = va[t] && (found || (!open && probe_on(table, bmask, home[t], kk)));
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < G; ++t) hit[t] = false;
-        }
-        if (GEN) {
-#pragma unroll
-            for (int t = 0; t < G; ++t)
-                if (vg[t]) hit[t] = general_lookup(g, seq + i0 + t, hh[t], k);
-        }
         uint32_t lo4 = 0, hi4 = 0;
 #pragma unroll
         for (int t = 0; t < G; ++t) {
-            const int64_t i = i0 + t;
-            if (hit[t]) last_hit = i;
+            const int64_t i = i0 + t, p = i + k - 1;
+            if (p < N) {
+                feed(p);
+                if (i >= seg_from && general_lookup(g, seq + i, h, k)) last_hit = i;
+            }
             const uint32_t m = (last_hit > i - k) ? 1u : 0u;
             if (t < 4) lo4 |= m << (8 * t);
             else hi4 |= m << (8 * (t - 4));
@@ -221,7 +289,100 @@ __global__ __launch_bounds__(256) void k_paint_ranges(const int64_t *__restrict_
     }
 }
 
-__global__ __launch_bounds__(256) void k_mask_emit(const uint8_t *__restrict__ seq, const uint8_t *__restrict__ mask,
+// FASTA bodies, one workgroup per EMIT_WG * EMIT input bases: every output
+// byte that is not a base is a newline, so the workgroup's output range is
+// filled with newlines in LDS, each thread scatters its 16 bases (read with
+// one 16-byte load of sequence and of mask) to their output offsets, and the
+// range is written with aligned 16-byte stores.  Needs no empty sequence.
+constexpr int EMIT_WG = 256;
+constexpr int EMIT_TILE = 3 * EMIT_WG * EMIT + 64;  // width 1: a base, its line end, a sequence end
+
+__global__ __launch_bounds__(EMIT_WG) void k_mask_emit(const uint8_t *__restrict__ seq,
+                                                       const uint8_t *__restrict__ mask,
+                                                       const int64_t *__restrict__ seg_off,
+                                                       const int64_t *__restrict__ body_off, int32_t nseg, int64_t N,
+                                                       int64_t total, int32_t width, uint32_t alt, int32_t soft,
+                                                       uint8_t *__restrict__ out) {
+    __shared__ uint4 tile4[EMIT_TILE / 16];
+    __shared__ int64_t lds_range[2];
+    __shared__ int32_t lds_seg[2];
+    uint8_t *tile = (uint8_t *)tile4;
+    const int tid = threadIdx.x;
+    const int64_t in0 = (int64_t)blockIdx.x * EMIT_WG * EMIT;
+    const int64_t in1 = in0 + EMIT_WG * EMIT < N ? in0 + EMIT_WG * EMIT : N;
+    auto out_pos = [&](int32_t s, int64_t i) {  // output offset of base i of sequence s
+        const int64_t r = i - seg_off[s];
+        return body_off[s] + r + (width > 0 ? r / width : 0);
+    };
+    if (tid == 0) {
+        const int32_t s0 = seg_of(seg_off, 0, nseg - 1, in0);
+        lds_seg[0] = s0;
+        lds_range[0] = out_pos(s0, in0);
+    } else if (tid == EMIT_WG - 1) {
+        const int32_t s1 = seg_of(seg_off, 0, nseg - 1, in1 - 1);
+        lds_seg[1] = s1;
+        lds_range[1] = in1 < N ? out_pos(seg_of(seg_off, 0, nseg - 1, in1), in1) : total;
+    }
+    for (int q = tid; q < EMIT_TILE / 16; q += EMIT_WG) tile4[q] = make_uint4(0x0A0A0A0Au, 0x0A0A0A0Au, 0x0A0A0A0Au,
+                                                                              0x0A0A0A0Au);
+    __syncthreads();
+    const int64_t o0 = lds_range[0], o1 = lds_range[1];
+    const int64_t a = in0 + (int64_t)tid * EMIT;
+    if (a < in1) {
+        const uint4 sv = *(const uint4 *)(seq + a), mv = *(const uint4 *)(mask + a);  // buffers padded to 16
+        const uint32_t sw[4] = {sv.x, sv.y, sv.z, sv.w}, mw[4] = {mv.x, mv.y, mv.z, mv.w};
+        int32_t s = seg_of(seg_off, lds_seg[0], lds_seg[1], a);
+        int64_t next = seg_off[s + 1];
+        int64_t r = a - seg_off[s];
+        int64_t o = body_off[s] + r;
+        int64_t col = 0;
+        if (width > 0) {
+            o += r / width;
+            col = r % width;
+        }
+#pragma unroll
+        for (int t = 0; t < EMIT; ++t) {
+            const int64_t i = a + t;
+            if (i < in1) {
+                if (i == next) {  // the first base of the next sequence (none is empty)
+                    ++s;
+                    next = seg_off[s + 1];
+                    o = body_off[s];
+                    col = 0;
+                }
+                uint32_t v = (sw[t >> 2] >> (8 * (t & 3))) & 0xFFu;
+                if (v - 'a' < 26u) v -= 32u;  // upper-cased for output
+                if ((mw[t >> 2] >> (8 * (t & 3))) & 0xFFu) v = soft ? (v - 'A' < 26u ? v + 32u : v) : alt;
+                tile[o - o0] = (uint8_t)v;
+                ++o;
+                if (width > 0 && ++col == width) {
+                    col = 0;
+                    ++o;  // the line end
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // [o0, o1): aligned 16-byte blocks inside, single bytes at the two ends
+    const int64_t a16 = (o0 + 15) & ~(int64_t)15, b16 = o1 & ~(int64_t)15;
+    if (a16 < b16) {
+        for (int64_t q = a16 + (int64_t)tid * 16; q < b16; q += EMIT_WG * 16) {
+            const int64_t l = q - o0;
+            uint32_t wv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                wv[u] = (uint32_t)tile[l + 4 * u] | ((uint32_t)tile[l + 4 * u + 1] << 8) |
+                        ((uint32_t)tile[l + 4 * u + 2] << 16) | ((uint32_t)tile[l + 4 * u + 3] << 24);
+            *(uint4 *)(out + q) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+        }
+        if (tid < a16 - o0) out[o0 + tid] = tile[tid];
+        if (tid < o1 - b16) out[b16 + tid] = tile[b16 - o0 + tid];
+    } else {
+        for (int64_t q = o0 + tid; q < o1; q += EMIT_WG) out[q] = tile[q - o0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mask_emit_bytes(const uint8_t *__restrict__ seq, const uint8_t *__restrict__ mask,
                                                    const int64_t *__restrict__ seg_off,
                                                    const int64_t *__restrict__ body_off, int32_t nseg, int64_t total,
                                                    int32_t width, uint32_t alt, int32_t soft,
@@ -274,8 +435,8 @@ __global__ __launch_bounds__(256) void k_mask_emit(const uint8_t *__restrict__ s
 
 int64_t kmer_table_buckets(int64_t nkeys) {
     int64_t slots = 1024;
-    while (slots < 2 * nkeys) slots *= 2;
-    return slots / BUCKET;
+    while (slots < 4 * nkeys) slots *= 2;
+    return slots / kKmerBucket;
 }
 
 int64_t kmer_general_slots(int64_t n) {
@@ -288,7 +449,7 @@ int64_t kmer_mask_padded(int64_t n) { return (n + CHUNK - 1) / CHUNK * CHUNK + C
 int64_t kmer_body_padded(int64_t n) { return (n + EMIT - 1) / EMIT * EMIT + EMIT; }
 
 hipError_t launch_kmer_table(hipStream_t st, const uint64_t *codes, int64_t n, uint64_t *table, int64_t buckets) {
-    hipError_t e = hipMemsetAsync(table, 0xff, (size_t)buckets * BUCKET * sizeof(uint64_t), st);
+    hipError_t e = hipMemsetAsync(table, 0xff, (size_t)buckets * kKmerBucket * sizeof(uint64_t), st);
     if (e != hipSuccess || n == 0) return e;
     hipLaunchKernelGGL(k_kmer_insert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, codes, n, table,
                        (uint64_t)(buckets - 1));
@@ -307,21 +468,23 @@ hipError_t launch_kmer_scan(hipStream_t st, const uint8_t *seq, int64_t N, const
                             int32_t k, bool codes, const uint64_t *table, int64_t buckets, const KmerGeneral &g,
                             uint8_t *mask) {
     if (N <= 0) return hipSuccess;
-    const int64_t lanes = (N + CHUNK - 1) / CHUNK;
-    const dim3 grid((unsigned)((lanes + 255) / 256)), block(256);
-    const uint64_t bmask = buckets > 0 ? (uint64_t)(buckets - 1) : 0;
     const bool gen = g.n > 0;
-    if (codes && gen)
-        hipLaunchKernelGGL((k_kmer_scan<true, true>), grid, block, 0, st, seq, N, seg_off, nseg, k, table, bmask, g,
-                           mask);
-    else if (codes)
-        hipLaunchKernelGGL((k_kmer_scan<true, false>), grid, block, 0, st, seq, N, seg_off, nseg, k, table, bmask, g,
-                           mask);
-    else if (gen)
-        hipLaunchKernelGGL((k_kmer_scan<false, true>), grid, block, 0, st, seq, N, seg_off, nseg, k, table, bmask, g,
-                           mask);
-    else
+    if (k >= 1 && k <= 31 && codes) {  // codes: the code table exists (it may hold no key)
+        const dim3 grid((unsigned)((N + WG_POS - 1) / WG_POS)), block(WG);
+        const uint64_t bmask = buckets > 0 ? (uint64_t)(buckets - 1) : 0;
+        if (gen)
+            hipLaunchKernelGGL(k_kmer_scan16<true>, grid, block, 0, st, seq, N, seg_off, nseg, k, table, bmask, g,
+                               mask);
+        else
+            hipLaunchKernelGGL(k_kmer_scan16<false>, grid, block, 0, st, seq, N, seg_off, nseg, k, table, bmask, g,
+                               mask);
+    } else if (k >= 32 && gen) {
+        const int64_t lanes = (N + CHUNK - 1) / CHUNK;
+        hipLaunchKernelGGL(k_kmer_scan_long, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, seq, N,
+                           seg_off, nseg, k, g, mask);
+    } else {
         return hipMemsetAsync(mask, 0, (size_t)N, st);
+    }
     return hipGetLastError();
 }
 
@@ -333,12 +496,18 @@ hipError_t launch_paint_ranges(hipStream_t st, const int64_t *ranges, int64_t n,
 }
 
 hipError_t launch_mask_emit(hipStream_t st, const uint8_t *seq, const uint8_t *mask, const int64_t *seg_off,
-                            const int64_t *body_off, int32_t nseg, int64_t total, int32_t width, uint8_t alt,
-                            bool soft, uint8_t *out) {
+                            const int64_t *body_off, int32_t nseg, int64_t N, int64_t total, int32_t width,
+                            bool any_empty, uint8_t alt, bool soft, uint8_t *out) {
     if (total <= 0) return hipSuccess;
-    const int64_t lanes = (total + EMIT - 1) / EMIT;
-    hipLaunchKernelGGL(k_mask_emit, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, seq, mask, seg_off,
-                       body_off, nseg, total, width, (uint32_t)alt, soft ? 1 : 0, out);
+    if (!any_empty && N > 0) {
+        const int64_t wgs = (N + EMIT_WG * EMIT - 1) / (EMIT_WG * EMIT);
+        hipLaunchKernelGGL(k_mask_emit, dim3((unsigned)wgs), dim3(EMIT_WG), 0, st, seq, mask, seg_off, body_off, nseg,
+                           N, total, width, (uint32_t)alt, soft ? 1 : 0, out);
+    } else {  // empty sequences (k = 0 only): one lane per 16 output bytes
+        const int64_t lanes = (total + EMIT - 1) / EMIT;
+        hipLaunchKernelGGL(k_mask_emit_bytes, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, seq, mask,
+                           seg_off, body_off, nseg, total, width, (uint32_t)alt, soft ? 1 : 0, out);
+    }
     return hipGetLastError();
 }
 

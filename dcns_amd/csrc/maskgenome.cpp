@@ -28,19 +28,22 @@ void mask_stage(dcns_ctx *c, const KmerSet &ks, const uint8_t *seq, const int64_
     ms->N = ms->off.back();
     ms->k = ks.k;
     hipStream_t st = c->stream;
-    c->mk_seq.ensure((size_t)ms->N + 64);
+    c->mk_seq.ensure((size_t)ms->N + 128);  // the scan reads 48 bytes from each 16-byte block start
     if (ms->N > 0) hip_check(hipMemcpyAsync(c->mk_seq.p, seq, (size_t)ms->N, hipMemcpyHostToDevice, st), "upload genome");
     c->mk_off.ensure(ms->off.size() * sizeof(int64_t));
     hip_check(hipMemcpyAsync(c->mk_off.p, ms->off.data(), ms->off.size() * sizeof(int64_t), hipMemcpyHostToDevice, st),
               "upload offsets");
-    if (ks.k >= 1 && ks.k <= 31 && !ks.codes.empty()) {
+    // k <= 31: the scan probes the code table for every ACGT window, so it is
+    // built (possibly empty) whenever the set has a member of length k
+    if (ks.k >= 1 && ks.k <= 31 && (!ks.codes.empty() || ks.n_general() > 0)) {
         const int64_t n = (int64_t)ks.codes.size();
         ms->buckets = kmer_table_buckets(n);
-        c->mk_codes.ensure((size_t)n * sizeof(uint64_t));
-        hip_check(hipMemcpyAsync(c->mk_codes.p, ks.codes.data(), (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice,
-                                 st),
-                  "upload k-mer codes");
-        c->mk_table.ensure((size_t)ms->buckets * 4 * sizeof(uint64_t));
+        c->mk_codes.ensure((size_t)n * sizeof(uint64_t) + 8);
+        if (n > 0)
+            hip_check(hipMemcpyAsync(c->mk_codes.p, ks.codes.data(), (size_t)n * sizeof(uint64_t),
+                                     hipMemcpyHostToDevice, st),
+                      "upload k-mer codes");
+        c->mk_table.ensure((size_t)ms->buckets * kKmerBucket * sizeof(uint64_t));
         c->launch("k_kmer_insert", [&] {
             return launch_kmer_table(st, c->mk_codes.as<uint64_t>(), n, c->mk_table.as<uint64_t>(), ms->buckets);
         });
@@ -116,9 +119,12 @@ void mask_run(dcns_ctx *c, const CdsMask *cds, uint8_t alt, bool soft, int32_t w
     hip_check(hipMemcpyAsync(c->mk_boff.p, boff.data(), boff.size() * sizeof(int64_t), hipMemcpyHostToDevice, st),
               "upload body offsets");
     c->mk_body.ensure((size_t)kmer_body_padded(total));
+    bool any_empty = false;
+    for (int32_t s = 0; s < ms.nseq; ++s) any_empty |= ms.off[(size_t)s + 1] == ms.off[(size_t)s];
     c->launch("k_mask_emit", [&] {
         return launch_mask_emit(st, c->mk_seq.as<uint8_t>(), c->mk_mask.as<uint8_t>(), c->mk_off.as<int64_t>(),
-                                c->mk_boff.as<int64_t>(), ms.nseq, total, width, alt, soft, c->mk_body.as<uint8_t>());
+                                c->mk_boff.as<int64_t>(), ms.nseq, N, total, width, any_empty, alt, soft,
+                                c->mk_body.as<uint8_t>());
     });
     if (mask_out && N > 0)
         hip_check(hipMemcpyAsync(mask_out, c->mk_mask.p, (size_t)N, hipMemcpyDeviceToHost, st), "download mask");
