@@ -133,7 +133,7 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_weighted2gaps_main", "dcns_find_similar_seeded_batch", "dcns_cut2gaps2_main", "dcns_kmer_set_read",
            "dcns_kmer_set_info", "dcns_kmer_set_get", "dcns_kmer_set_free", "dcns_cds_mask_create",
            "dcns_cds_mask_gff", "dcns_cds_mask_sam", "dcns_cds_mask_get", "dcns_cds_mask_free", "dcns_mask_stage",
-           "dcns_mask_body_size", "dcns_mask_run", "dcns_mask_genome_main"]
+           "dcns_mask_body_size", "dcns_mask_run", "dcns_mask_genome_main", "dcns_batch_main"]
 
 
 def default_params(**kw):

@@ -18,7 +18,8 @@ static void usage() {
               << "    weighted2Gaps       the same with the weighted two-piece gap alignment\n"
               << "    maskGenome          mask repetitive k-mers, CDS (GFF3) or aligned CDS (SAM) in a genome\n"
               << "    ranSco              smith-waterman scores of random sequence fragments\n"
-              << "                        for lambda and k fitting\n";
+              << "                        for lambda and k fitting\n"
+              << "    batch               run a file of dCNS command lines in one process (cut1Gap lines batched)\n";
 }
 
 int main(int argc, char **argv) {
@@ -40,6 +41,7 @@ int main(int argc, char **argv) {
     if (!std::strcmp(p, "weighted1Gap")) return dcns_weighted1gap_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "weighted2Gaps")) return dcns_weighted2gaps_main(argc - 1, argv + 1);
     if (!std::strcmp(p, "maskGenome")) return dcns_mask_genome_main(argc - 1, argv + 1);
+    if (!std::strcmp(p, "batch")) return dcns_batch_main(argc - 1, argv + 1);
     std::cerr << "dCNS: command " << p << " is not part of this build" << std::endl;
     usage();
     return 1;

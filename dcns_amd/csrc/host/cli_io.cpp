@@ -3,6 +3,7 @@
 #include "cli_io.h"
 
 #include <algorithm>
+#include <memory>
 #include <regex>
 #include <cctype>
 #include <cstdlib>
@@ -372,11 +373,46 @@ std::vector<uint32_t> unmask_cigar(const std::vector<uint32_t> &cigar, const std
 }
 }  // namespace
 
+// Positions of the non-'n' characters of a masked sequence: the c-th
+// (1-based) sits at pos[c - 1] and has pos[c - 1] + 1 - c n's before it, which
+// is what outputMasking's scans (controlLayer4.cpp:857-888) find by walking the
+// sequence for every fragment.
+struct NonN {
+    std::vector<int32_t> pos;
+    explicit NonN(const std::string &s) {
+        pos.reserve(s.size());
+        for (size_t j = 0; j < s.size(); ++j)
+            if (s[j] != 'n') pos.push_back((int32_t)j);
+    }
+    bool has(uint32_t c) const { return c >= 1 && c <= pos.size(); }
+    int32_t at(uint32_t c) const { return pos[c - 1]; }                  // position j where the count reaches c
+    int32_t ns(uint32_t c) const { return pos[c - 1] + 1 - (int32_t)c; }  // n's up to and including j
+};
+
+namespace {
+// sub_sequence(s, start, end[, 0]) without copying s
+std::string sub_text(const std::string &s, int start, int end, bool revcomp) {
+    size_t a = (size_t)start, b = (size_t)end;
+    if (a > b) std::swap(a, b);
+    if (a < 1) a = 1;
+    if (a > s.size()) {
+        a = s.size();
+        b = a;
+    } else if (b > s.size()) {
+        b = s.size();
+    }
+    const std::string t = s.substr(a - 1, b - a + 1);
+    return revcomp ? reverse_complement(t) : t;
+}
+}  // namespace
+
 // outputMasking, controlLayer4.cpp:835-1358.  Stripped (n-free) 1-based
 // fragment coordinates are mapped back onto the masked region by counting
 // n's; the four strand combinations differ in which end is anchored, where
 // the n count is applied, the SAM flag, the cigar direction and whether the
-// query text is reverse-complemented.
+// query text is reverse-complemented.  The counts come from NonN indexes
+// built once per call; a fragment whose coordinates fall outside the usual
+// 1 <= start <= end <= length takes the reference's own walk.
 void write_sam_and_o(const std::string &prefix, const std::vector<PairedSimilarFragment> &frags, int32_t refStrand,
                      int32_t queStrand, int32_t refStart, int32_t queStart, const std::string &refChr,
                      const std::string &queChr, const std::string &seq1_0, const std::string &seq2_0) {
@@ -384,49 +420,69 @@ void write_sam_and_o(const std::string &prefix, const std::vector<PairedSimilarF
     std::ofstream o(prefix + "o", std::ios_base::app);
     o << "#" << refChr << ":" << refStrand << ":" << refStart << "-" << queChr << ":" << queStrand << ":" << queStart
       << std::endl;
+    const bool strands_ok = (refStrand == 1 || refStrand == 0) && (queStrand == 1 || queStrand == 0);
+    std::unique_ptr<NonN> i1, i2;
+    if (strands_ok && !frags.empty()) {
+        i1.reset(new NonN(seq1_0));
+        i2.reset(new NonN(seq2_0));
+    }
     for (const PairedSimilarFragment &f : frags) {
         const bool qplus = queStrand == 1, rplus = refStrand == 1;
-        if ((refStrand == 1 || refStrand == 0) && (queStrand == 1 || queStrand == 0)) {
+        if (strands_ok) {
             int32_t quePos = qplus ? (int32_t)(queStart + f.getStart2() - 1)
                                    : (int32_t)(queStart + seq2_0.size() - 1 - f.getEnd2() + 1);
             int32_t refPos = rplus ? (int32_t)(refStart + f.getStart1() - 1)
                                    : (int32_t)(refStart + seq1_0.size() - 1 - f.getEnd1() + 1);
-            int32_t start = 1, end = 1, nchr = 0, nn = 0, started = 0;
-            for (int32_t j = 0; j < (int32_t)seq2_0.size(); ++j) {
-                if (seq2_0[(size_t)j] == 'n') ++nn;
-                else ++nchr;
-                if ((uint32_t)nchr == f.getStart2() && started == 0) {
-                    start += j;
-                    if (qplus) quePos += nn;  // plus strand: anchored at the start
-                    started = 1;
-                }
-                if ((uint32_t)nchr == f.getEnd2()) {
-                    end += j;
-                    if (!qplus) quePos -= nn;  // minus strand: anchored at the end
-                    break;
-                }
-            }
-            nchr = 0;
-            nn = 0;
-            int32_t ref_start = 1;
-            started = 0;
-            for (int32_t j = 0; j < (int32_t)seq1_0.size(); ++j) {
-                if (seq1_0[(size_t)j] != 'n') ++nchr;
-                else ++nn;
-                if (rplus) {
-                    if ((uint32_t)nchr == f.getStart1() && (started == 0 || !qplus)) {
-                        ref_start += j;
-                        refPos += nn;
-                        break;
-                    }
-                } else {
-                    if ((uint32_t)nchr == f.getStart1() && started == 0) {
-                        ref_start += j;
+            int32_t start = 1, end = 1;
+            if (i2->has(f.getStart2()) && i2->has(f.getEnd2()) && f.getStart2() <= f.getEnd2()) {
+                start += i2->at(f.getStart2());
+                if (qplus) quePos += i2->ns(f.getStart2());  // plus strand: anchored at the start
+                end += i2->at(f.getEnd2());
+                if (!qplus) quePos -= i2->ns(f.getEnd2());   // minus strand: anchored at the end
+            } else {
+                int32_t nchr = 0, nn = 0, started = 0;
+                for (int32_t j = 0; j < (int32_t)seq2_0.size(); ++j) {
+                    if (seq2_0[(size_t)j] == 'n') ++nn;
+                    else ++nchr;
+                    if ((uint32_t)nchr == f.getStart2() && started == 0) {
+                        start += j;
+                        if (qplus) quePos += nn;
                         started = 1;
                     }
-                    if ((uint32_t)nchr == f.getEnd1()) {
-                        refPos -= nn;
+                    if ((uint32_t)nchr == f.getEnd2()) {
+                        end += j;
+                        if (!qplus) quePos -= nn;
                         break;
+                    }
+                }
+            }
+            int32_t ref_start = 1;
+            if (rplus && i1->has(f.getStart1())) {
+                ref_start += i1->at(f.getStart1());
+                refPos += i1->ns(f.getStart1());
+            } else if (!rplus && i1->has(f.getStart1()) && i1->has(f.getEnd1()) && f.getStart1() <= f.getEnd1()) {
+                ref_start += i1->at(f.getStart1());
+                refPos -= i1->ns(f.getEnd1());
+            } else {
+                int32_t nchr = 0, nn = 0, started = 0;
+                for (int32_t j = 0; j < (int32_t)seq1_0.size(); ++j) {
+                    if (seq1_0[(size_t)j] != 'n') ++nchr;
+                    else ++nn;
+                    if (rplus) {
+                        if ((uint32_t)nchr == f.getStart1() && (started == 0 || !qplus)) {
+                            ref_start += j;
+                            refPos += nn;
+                            break;
+                        }
+                    } else {
+                        if ((uint32_t)nchr == f.getStart1() && started == 0) {
+                            ref_start += j;
+                            started = 1;
+                        }
+                        if ((uint32_t)nchr == f.getEnd1()) {
+                            refPos -= nn;
+                            break;
+                        }
                     }
                 }
             }
@@ -440,9 +496,7 @@ void write_sam_and_o(const std::string &prefix, const std::vector<PairedSimilarF
                 for (int32_t j = (int32_t)nc.size() - 1; j >= 0; --j)
                     sam << (nc[(size_t)j] >> 4) << "MIDDSHI=XB"[nc[(size_t)j] & 0xfu];
             }
-            std::string s2 = seq2_0;
-            const std::string text = rplus ? sub_sequence(s2, start, end) : sub_sequence(s2, start, end, 0);
-            sam << "\t*\t0\t0\t" << text << "\t*" << std::endl;
+            sam << "\t*\t0\t0\t" << sub_text(seq2_0, start, end, !rplus) << "\t*" << std::endl;
         }
         o << f.getStart1() << " " << f.getEnd1() << " " << f.getStart2() << " " << f.getEnd2() << " " << f.getScore()
           << " " << f.getPValue() << std::endl;

@@ -14,6 +14,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <fstream>
+#include <future>
 #include <iostream>
 #include <memory>
 #include <sstream>
@@ -120,9 +122,11 @@ dcns_params to_dcns_params(const Defaults &d) {
 
 }  // namespace
 
-int cut1gap_main(int argc, char **argv) {
-    Defaults d;
-    Args a(argc, argv);
+namespace {
+
+// The flags of cut1Gap (controlLayer4.cpp:1361-1400) into d; false when the
+// usage text is to be printed instead.
+bool cut1gap_flags(const Args &a, Defaults &d) {
     if (a.has("-A")) d.match = std::stoi(a.get("-A"));
     if (a.has("-B")) d.mismatch = std::stoi(a.get("-B"));
     if (a.has("-O1")) d.open1 = std::stoi(a.get("-O1"));
@@ -136,11 +140,119 @@ int cut1gap_main(int argc, char **argv) {
     if (a.has("-p")) d.pvalues = std::stod(a.get("-p"));
     if (a.has("-l")) d.lambda = std::stod(a.get("-l"));
     if (a.has("-k")) d.k = std::stod(a.get("-k"));
-    if (a.has("-h") || a.has("--help")) {
-        cut1gap_usage(d);
-        return 1;
+    if (a.has("-h") || a.has("--help")) return false;
+    return a.has("-i") && a.has("-r") && a.has("-o") && a.has("-ra") && a.has("-qa");
+}
+
+// One cut1Gap input file: the reference region and the query regions, read
+// from the genomes and stripped of their masked ('n') bases
+// (controlLayer4.cpp:1401-1410).
+struct Cut1Job {
+    std::string output, error;
+    Meta ref;
+    std::string seq1_0;
+    std::vector<int8_t> c1;
+    struct Q {
+        std::string name, seq2_0;
+        Meta meta;
+        std::vector<int8_t> codes;
+        bool run = false;
+    };
+    std::vector<Q> qs;
+    int64_t pairs() const {
+        int64_t n = 0;
+        for (const Q &q : qs) n += q.run;
+        return n;
     }
-    if (!(a.has("-i") && a.has("-r") && a.has("-o") && a.has("-ra") && a.has("-qa"))) {
+};
+
+Cut1Job cut1gap_job(const std::string &input, const std::string &reference, const std::string &output,
+                    IndexedFasta &refFa, IndexedFasta &queFa) {
+    Cut1Job j;
+    j.output = output;
+    if (!std::ifstream(input).good()) {
+        j.error = "error in opening fasta file " + input;
+        return j;
+    }
+    std::map<std::string, Meta> meta;
+    std::map<std::string, std::string> seqs;
+    std::vector<std::string> names;
+    read_meta_fasta(input, seqs, meta, names);
+    j.ref = meta[reference];
+    j.seq1_0 = refFa.fetch(j.ref.chr, j.ref.start, j.ref.end, (int8_t)j.ref.strand);
+    std::string seq1 = j.seq1_0;
+    seq1.erase(std::remove(seq1.begin(), seq1.end(), 'n'), seq1.end());
+    j.c1 = encode(seq1);
+    for (const std::string &q : names) {
+        if (q.compare(reference) == 0) continue;
+        Cut1Job::Q x;
+        x.name = q;
+        x.meta = meta[q];
+        x.run = seqs[q].size() > 0;
+        if (x.run) {
+            x.seq2_0 = queFa.fetch(x.meta.chr, x.meta.start, x.meta.end, (int8_t)x.meta.strand);
+            std::string s2 = x.seq2_0;
+            s2.erase(std::remove(s2.begin(), s2.end(), 'n'), s2.end());
+            x.codes = encode(s2);
+        }
+        j.qs.push_back(std::move(x));
+    }
+    return j;
+}
+
+// All pairs of the jobs in one batch through overload 1; per job the file
+// operations and stdout lines of controlLayer4.cpp:1411-1527, in job order.
+void cut1gap_run(dcns_ctx *ctx, const dcns_params &p, const std::vector<Cut1Job> &jobs) {
+    std::vector<int8_t> buf;
+    std::vector<int64_t> off1, off2;
+    std::vector<int32_t> len1, len2;
+    for (const Cut1Job &j : jobs) {
+        if (!j.error.empty() || j.pairs() == 0) continue;
+        const int64_t o1 = (int64_t)buf.size();
+        buf.insert(buf.end(), j.c1.begin(), j.c1.end());
+        for (const Cut1Job::Q &q : j.qs) {
+            if (!q.run) continue;
+            off1.push_back(o1);
+            len1.push_back((int32_t)j.c1.size());
+            off2.push_back((int64_t)buf.size());
+            len2.push_back((int32_t)q.codes.size());
+            buf.insert(buf.end(), q.codes.begin(), q.codes.end());
+        }
+    }
+    FragSet fs;
+    if (!off1.empty()) {
+        dcns_batch b{buf.data(), off1.data(), len1.data(), off2.data(), len2.data(), (int32_t)off1.size(), 0};
+        find_similar(ctx, b, p, fs);
+    }
+    size_t k = 0;
+    for (const Cut1Job &j : jobs) {
+        if (!j.error.empty()) {
+            std::cerr << j.error << std::endl;
+            continue;
+        }
+        for (const Cut1Job::Q &q : j.qs) {
+            const std::string prefix = j.output + q.meta.species;
+            { std::ofstream s(prefix + ".sam", std::ios_base::app); std::ofstream o(prefix + "o", std::ios_base::app); }
+            if (q.run) {
+                std::cout << "seeds size" << fs.seeds_size[k] << std::endl;
+                std::vector<PairedSimilarFragment> fr(fs.frags.begin() + fs.first[k],
+                                                      fs.frags.begin() + fs.first[k] + fs.count[k]);
+                write_sam_and_o(prefix, fr, j.ref.strand, q.meta.strand, j.ref.start, q.meta.start, j.ref.chr,
+                                q.meta.chr, j.seq1_0, q.seq2_0);
+                ++k;
+            }
+            std::ofstream o(prefix + "o", std::ios_base::app);
+            o << "#done" << std::endl;
+        }
+    }
+}
+
+}  // namespace
+
+int cut1gap_main(int argc, char **argv) {
+    Defaults d;
+    Args a(argc, argv);
+    if (!cut1gap_flags(a, d)) {
         cut1gap_usage(d);
         return 1;
     }
@@ -148,98 +260,141 @@ int cut1gap_main(int argc, char **argv) {
         std::cerr << "dCNS: syntenic filtering (-y) is outside this build's scope" << std::endl;
         return 1;
     }
-    const std::string input = a.get("-i"), reference = a.get("-r"), output = a.get("-o");
     IndexedFasta refFa(a.get("-ra")), queFa(a.get("-qa"));
-    std::map<std::string, Meta> meta;
-    std::map<std::string, std::string> seqs;
-    std::vector<std::string> names;
-    read_meta_fasta(input, seqs, meta, names);
-
-    const Meta refMeta = meta[reference];
-    const std::string seq1_0 = refFa.fetch(refMeta.chr, refMeta.start, refMeta.end, (int8_t)refMeta.strand);
-    std::string seq1 = seq1_0;
-    seq1.erase(std::remove(seq1.begin(), seq1.end(), 'n'), seq1.end());
-    const std::vector<int8_t> c1 = encode(seq1);
-
-    // every query region first (the alignment runs as one batch), then the
-    // file operations in the reference's order
-    struct Q {
-        std::string name, seq2_0;
-        std::vector<int8_t> codes;
-        bool run;
-    };
-    std::vector<Q> qs;
-    for (const std::string &q : names) {
-        if (q.compare(reference) == 0) continue;
-        Q x;
-        x.name = q;
-        x.run = seqs[q].size() > 0;
-        if (x.run) {
-            const Meta &m = meta[q];
-            x.seq2_0 = queFa.fetch(m.chr, m.start, m.end, (int8_t)m.strand);
-            std::string s2 = x.seq2_0;
-            s2.erase(std::remove(s2.begin(), s2.end(), 'n'), s2.end());
-            x.codes = encode(s2);
-        }
-        qs.push_back(std::move(x));
+    std::vector<Cut1Job> jobs;
+    jobs.push_back(cut1gap_job(a.get("-i"), a.get("-r"), a.get("-o"), refFa, queFa));
+    if (!jobs[0].error.empty()) {
+        std::cerr << jobs[0].error << std::endl;
+        return 1;
     }
-    std::vector<int8_t> buf(c1.begin(), c1.end());
-    std::vector<int64_t> off1, off2;
-    std::vector<int32_t> len1, len2;
-    std::vector<size_t> which;
-    for (size_t i = 0; i < qs.size(); ++i) {
-        if (!qs[i].run) continue;
-        off1.push_back(0);
-        len1.push_back((int32_t)c1.size());
-        off2.push_back((int64_t)buf.size());
-        len2.push_back((int32_t)qs[i].codes.size());
-        buf.insert(buf.end(), qs[i].codes.begin(), qs[i].codes.end());
-        which.push_back(i);
-    }
-    FragSet fs;
-    if (!which.empty()) {
-        dcns_params p;
-        dcns_default_params(&p);
-        p.match = d.match;
-        p.mismatch = d.mismatch;
-        p.open = d.open1;
-        p.ext = d.ext1;
-        p.window = d.window;
-        p.step = d.step;
-        p.min_score = d.min_score;
-        p.w = d.w;
-        p.xdrop = d.xdrop;
-        p.pvalue = d.pvalues;
-        p.lambda = d.lambda;
-        p.k = d.k;
-        dcns_batch b{buf.data(), off1.data(), len1.data(), off2.data(), len2.data(), (int32_t)which.size(), 0};
-        dcns_ctx *ctx = make_ctx();
-        try {
-            find_similar(ctx, b, p, fs);
-        } catch (const std::exception &e) {
-            std::cerr << "dCNS: " << e.what() << std::endl;
-            dcns_ctx_destroy(ctx);
-            return 1;
-        }
+    dcns_ctx *ctx = make_ctx();
+    try {
+        cut1gap_run(ctx, to_dcns_params(d), jobs);
+    } catch (const std::exception &e) {
+        std::cerr << "dCNS: " << e.what() << std::endl;
         dcns_ctx_destroy(ctx);
+        return 1;
     }
-    size_t k = 0;
-    for (size_t i = 0; i < qs.size(); ++i) {
-        const Meta &m = meta[qs[i].name];
-        const std::string prefix = output + m.species;
-        { std::ofstream s(prefix + ".sam", std::ios_base::app); std::ofstream o(prefix + "o", std::ios_base::app); }
-        if (qs[i].run) {
-            std::cout << "seeds size" << fs.seeds_size[k] << std::endl;
-            std::vector<PairedSimilarFragment> fr(fs.frags.begin() + fs.first[k],
-                                                  fs.frags.begin() + fs.first[k] + fs.count[k]);
-            write_sam_and_o(prefix, fr, refMeta.strand, m.strand, refMeta.start, m.start, refMeta.chr, m.chr, seq1_0,
-                            qs[i].seq2_0);
-            ++k;
-        }
-        std::ofstream o(prefix + "o", std::ios_base::app);
-        o << "#done" << std::endl;
-    }
+    dcns_ctx_destroy(ctx);
     return 0;
+}
+
+int dispatch(int argc, char **argv);
+
+// batch: the command file of a GNU-parallel fan-out (README.md:117-122: one
+// "dCNS cut1Gap ... -i <region> -o <prefix>" line per region) run in one
+// process.  Consecutive cut1Gap lines that differ only in -i / -o form a
+// group: its regions are read on a second thread, chunk by chunk (about
+// max_pairs pairs), while the device aligns the previous chunk, and every
+// chunk is one batched call.  Files and stdout are those of running the lines
+// one after another.  Lines of other commands run through their own entry
+// points, in order.
+int batch_main(int argc, char **argv) {
+    if (argc < 2) {
+        std::cerr << "Usage: dCNS batch commands.txt [-n pairs_per_chunk]\n"
+                  << "  commands.txt: one dCNS command line per line (e.g. dCNS cut1Gap ... -i region.fa -o out)\n";
+        return 1;
+    }
+    int64_t max_pairs = 4096;
+    for (int i = 2; i + 1 < argc; ++i)
+        if (!std::strcmp(argv[i], "-n")) max_pairs = std::max<int64_t>(1, std::atoll(argv[i + 1]));
+    std::ifstream in(argv[1]);
+    if (!in.good()) {
+        std::cerr << "error in opening command file " << argv[1] << std::endl;
+        return 1;
+    }
+    std::vector<std::vector<std::string>> lines;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        std::vector<std::string> t;
+        std::string w;
+        while (ss >> w) t.push_back(w);
+        if (t.empty() || t[0][0] == '#') continue;
+        if (t[0] == "dCNS" || (t[0].size() >= 5 && t[0].compare(t[0].size() - 5, 5, "/dCNS") == 0))
+            t.erase(t.begin());  // the program name
+        if (!t.empty()) lines.push_back(t);
+    }
+    // a cut1Gap line's group: every token but the -i / -o pairs
+    auto group_key = [](const std::vector<std::string> &t) {
+        std::string key;
+        for (size_t i = 0; i < t.size(); ++i) {
+            if ((t[i] == "-i" || t[i] == "-o") && i + 1 < t.size()) {
+                ++i;
+                continue;
+            }
+            key += t[i] + '\x1f';
+        }
+        return key;
+    };
+    int status = 0;
+    dcns_ctx *ctx = nullptr;
+    for (size_t li = 0; li < lines.size();) {
+        const std::vector<std::string> &t = lines[li];
+        std::vector<char *> av;
+        for (const std::string &w : t) av.push_back(const_cast<char *>(w.c_str()));
+        if (t[0] != "cut1Gap") {
+            std::cout.flush();
+            const int rc = dcns::cli::dispatch((int)av.size(), av.data());
+            status = status ? status : rc;
+            ++li;
+            continue;
+        }
+        size_t le = li + 1;
+        const std::string key = group_key(t);
+        while (le < lines.size() && lines[le][0] == "cut1Gap" && group_key(lines[le]) == key) ++le;
+        Defaults d;
+        Args a((int)av.size(), av.data());
+        bool ok = false;
+        try {
+            ok = cut1gap_flags(a, d) && !d.syntenic;
+        } catch (const std::exception &) {
+            std::cerr << "dCNS: invalid number in: " << key << std::endl;
+            status = 1;
+            li = le;
+            continue;
+        }
+        if (!ok) {
+            cut1gap_main((int)av.size(), av.data());  // its usage / refusal text, once per line
+            status = 1;
+            li = le;
+            continue;
+        }
+        if (!ctx) ctx = make_ctx();
+        const dcns_params p = to_dcns_params(d);
+        IndexedFasta refFa(a.get("-ra")), queFa(a.get("-qa"));
+        const std::string reference = a.get("-r");
+        size_t cursor = li;
+        auto next_chunk = [&]() {
+            std::vector<Cut1Job> jobs;
+            int64_t pairs = 0;
+            while (cursor < le && (jobs.empty() || pairs < max_pairs)) {
+                const std::vector<std::string> &u = lines[cursor++];
+                std::vector<char *> uv;
+                for (const std::string &w : u) uv.push_back(const_cast<char *>(w.c_str()));
+                Args ua((int)uv.size(), uv.data());
+                jobs.push_back(cut1gap_job(ua.get("-i"), reference, ua.get("-o"), refFa, queFa));
+                pairs += jobs.back().pairs();
+            }
+            return jobs;
+        };
+        std::future<std::vector<Cut1Job>> ahead = std::async(std::launch::async, next_chunk);
+        for (;;) {
+            std::vector<Cut1Job> jobs = ahead.get();
+            if (jobs.empty()) break;
+            ahead = std::async(std::launch::async, next_chunk);  // reads the next chunk while this one aligns
+            for (const Cut1Job &j : jobs) status = (status || !j.error.empty()) ? 1 : 0;
+            try {
+                cut1gap_run(ctx, p, jobs);
+            } catch (const std::exception &e) {
+                std::cerr << "dCNS: " << e.what() << std::endl;
+                status = 1;
+            }
+        }
+        li = le;
+    }
+    if (ctx) dcns_ctx_destroy(ctx);
+    return status;
 }
 
 // cut2Gaps (controlLayer4.cpp:1534-1713): whole genomes read without an
@@ -802,6 +957,22 @@ int ransco_main(int argc, char **argv) {
     return 0;
 }
 
+// One command line (argv[0] = the command) through its entry point.
+int dispatch(int argc, char **argv) {
+    const std::string c = argv[0];
+    if (c == "cut1Gap") return dcns_cut1gap_main(argc, argv);
+    if (c == "ranSco") return dcns_ransco_main(argc, argv);
+    if (c == "cut2Gaps") return dcns_cut2gaps_main(argc, argv);
+    if (c == "cut2Gaps2") return dcns_cut2gaps2_main(argc, argv);
+    if (c == "pairCns2Gaps") return dcns_paircns2gaps_main(argc, argv);
+    if (c == "pairCnsXExtend") return dcns_paircnsxextend_main(argc, argv);
+    if (c == "weighted1Gap") return dcns_weighted1gap_main(argc, argv);
+    if (c == "weighted2Gaps") return dcns_weighted2gaps_main(argc, argv);
+    if (c == "maskGenome") return dcns_mask_genome_main(argc, argv);
+    std::cerr << "dCNS: command " << c << " is not part of this build" << std::endl;
+    return 1;
+}
+
 }  // namespace cli
 }  // namespace dcns
 
@@ -813,3 +984,4 @@ extern "C" int dcns_paircnsxextend_main(int argc, char **argv) { return dcns::cl
 extern "C" int dcns_weighted1gap_main(int argc, char **argv) { return dcns::cli::weighted1gap_main(argc, argv); }
 extern "C" int dcns_weighted2gaps_main(int argc, char **argv) { return dcns::cli::weighted2gaps_main(argc, argv); }
 extern "C" int dcns_cut2gaps2_main(int argc, char **argv) { return dcns::cli::cut2gaps2_main(argc, argv); }
+extern "C" int dcns_batch_main(int argc, char **argv) { return dcns::cli::batch_main(argc, argv); }

@@ -354,6 +354,15 @@ int dcns_mask_run(dcns_ctx *ctx, const dcns_cds_mask *cds, char alt, int32_t sof
 /* int maskGenome(int argc, char** argv), controlLayer4.cpp:1999: the command with its flags,
  * stdout lines and output file. */
 int dcns_mask_genome_main(int argc, char **argv);
+/* No counterpart in the reference, whose regions fan out as one process per
+ * region under GNU parallel (README.md:117-122).  Runs such a command file
+ * (one "dCNS <command> <flags>" line per job) in one process: consecutive
+ * cut1Gap lines that differ only in -i / -o are aligned in chunked batches
+ * (about 4096 pairs, "-n" to change), the next chunk's regions read while the
+ * device aligns the current one; every other line runs through its command.
+ * argv: "batch" <file> [-n pairs].  Files and stdout are those of running the
+ * lines in order; the status is 1 when any line failed. */
+int dcns_batch_main(int argc, char **argv);
 
 /* lambda/K surrogate network.  No counterpart in the reference, which obtains
  * lambda and K only by permutation (ranSco, permutationLsqLambdaK.cpp:32-102,

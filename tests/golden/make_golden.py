@@ -14,7 +14,7 @@ overloads, GFF categories), overload 3 (seeded), and the text outputs of the
 cut1Gap, cut2Gaps, pairCnsXExtend, ranSco, weighted1Gap, weighted2Gaps and
 cut2Gaps2 commands, and the maskGenome command's files.  Only data is
 written.  ``--weighted`` / ``--seeded`` / ``--mask`` write those fixtures
-only.
+only; ``--batch`` the cut1Gap command-file fixture.
 """
 import json
 import os
@@ -518,9 +518,91 @@ def write_mask(here):
     print("mask cases:", len(cases) + 3)
 
 
+def write_cut1gap_batch(here):
+    """A GNU-parallel style command file of cut1Gap lines (README.md:117-122)
+    over 14 region files of two masked genomes, and the files and stdout of
+    the reference's cut1Gap run line by line in one folder: two option groups,
+    two lines appending to one output prefix, a missing input file."""
+    root = os.path.join(here, "cut1gap_batch")
+    if os.path.isdir(root):
+        shutil.rmtree(root)
+    os.makedirs(root)
+    rng = np.random.default_rng(20261205)
+
+    def masked(codes, nruns):
+        s = list(synth.to_text(codes))
+        for _ in range(nruns):
+            p = int(rng.integers(0, len(s) - 100))
+            s[p:p + int(rng.integers(5, 120))] = "n" * len(s[p:p + int(rng.integers(5, 120))])
+        for _ in range(nruns // 2):
+            p = int(rng.integers(0, len(s) - 50))
+            s[p:p + 25] = [c.lower() if c != "n" else c for c in s[p:p + 25]]
+        return "".join(s)
+
+    r_chr = [synth.random_seq(rng, 40000) for _ in range(2)]
+    q_chr = [synth.random_seq(rng, 40000) for _ in range(2)]
+    regions = []
+    for t in range(14):
+        rc = int(rng.integers(0, 2))
+        rs = int(rng.integers(1, 40000 - 6000))
+        rl = int(rng.integers(1500, 5000))
+        qs = []
+        for q in range(int(rng.integers(1, 5))):
+            qc = int(rng.integers(0, 2))
+            qs_ = int(rng.integers(1, 40000 - 6000))
+            ql = int(rng.integers(1200, 5000))
+            strand = int(rng.integers(0, 2))
+            for _ in range(int(rng.integers(2, 8))):  # conserved fragments
+                ln = int(rng.integers(40, 200))
+                if ln >= min(rl, ql) - 10:
+                    continue
+                p1 = rs - 1 + int(rng.integers(0, rl - ln))
+                p2 = qs_ - 1 + int(rng.integers(0, ql - ln))
+                m = synth.mutate(rng, r_chr[rc][p1:p1 + ln], 0.08, 0.02)[:ln]
+                if strand == 0:
+                    m = (4 - m[::-1]).astype(np.int8)
+                q_chr[qc][p2:p2 + len(m)] = m
+            qs.append((f"q{q}", f"sp{chr(66 + q)}", f"qc{qc}", strand, qs_, qs_ + ql - 1))
+        regions.append((("ref", "spA", f"rc{rc}", int(rng.integers(0, 2)), rs, rs + rl - 1), qs))
+    fasta_with_fai(os.path.join(root, "refgenome.fa"), [(f"rc{i}", masked(c, 20)) for i, c in enumerate(r_chr)])
+    fasta_with_fai(os.path.join(root, "querygenome.fa"), [(f"qc{i}", masked(c, 20)) for i, c in enumerate(q_chr)])
+    lines = []
+    for t, (ref, qs) in enumerate(regions):
+        with open(os.path.join(root, f"region{t:02d}.fa"), "w") as fh:
+            for name, sp, chrom, strand, st, en in [ref] + qs:
+                fh.write(f">{name}\tspecies:{sp}\tchr:{chrom}\tstrand:{strand}\tstart:{st}\tend:{en}\nACGT\n")
+        out = f"region{t:02d}.fa.5" if t != 5 else "region04.fa.5"  # line 5 appends to line 4's files
+        extra = ["-c", "30"] if t >= 11 else []  # a second group
+        lines.append(["dCNS", "cut1Gap", "-ra", "refgenome.fa", "-qa", "querygenome.fa", "-i", f"region{t:02d}.fa",
+                      "-r", "ref", "-o", out] + extra)
+    lines.insert(8, ["dCNS", "cut1Gap", "-ra", "refgenome.fa", "-qa", "querygenome.fa", "-i", "missing.fa", "-r",
+                     "ref", "-o", "missing.fa.5"])
+    with open(os.path.join(root, "command1"), "w") as fh:
+        fh.write("".join(" ".join(l) + "\n" for l in lines))
+    work = tempfile.mkdtemp()
+    for f in os.listdir(root):
+        shutil.copy(os.path.join(root, f), work)
+    stdout, status = "", []
+    for l in lines:
+        r = subprocess.run([bind.REF_CLI] + l[1:], cwd=work, capture_output=True, text=True, timeout=600)
+        stdout += r.stdout
+        status.append(r.returncode)
+    for f in sorted(os.listdir(work)):
+        if f.endswith(".sam") or f.endswith("o") and ".fa.5" in f:
+            shutil.copy(os.path.join(work, f), os.path.join(root, "expected_" + f))
+    with open(os.path.join(root, "expected_stdout.txt"), "w") as fh:
+        fh.write(stdout)
+    with open(os.path.join(root, "expected_status.json"), "w") as fh:
+        json.dump(status, fh)
+    print("cut1gap batch: lines", len(lines), "status", status, "outputs",
+          len([f for f in os.listdir(root) if f.startswith("expected_region")]))
+
+
 if __name__ == "__main__":
     if "--mask" in sys.argv[1:]:
         write_mask(HERE)
+    elif "--batch" in sys.argv[1:]:
+        write_cut1gap_batch(HERE)
     elif "--weighted" in sys.argv[1:] or "--seeded" in sys.argv[1:]:
         r = bind.reference()
         if r is None:
