@@ -435,6 +435,112 @@ def mask_phase(ctx, log, cpu=True, gbp=1.0, sample_mb=4):
     ks.close()
     return out
 
+def batch_phase(log, cpu=True, regions=2000, sample=6):
+    """The GNU-parallel fan-out of the reference's pipeline (README.md:117-122:
+    one dCNS cut1Gap process per region file) as one `dCNS batch` process:
+    `regions` region files, each a 2-12 kb reference region and 3 query
+    regions of 2-12 kb, over two 30 Mb genomes with masked runs, cut1Gap
+    defaults; files in, SAM / o files out.  Beside it: the same lines run one
+    process per line, on `sample` lines, through this build and through the
+    compiled reference (one core)."""
+    import shutil
+    import subprocess
+    import tempfile
+    import dcns_amd
+    from dcns_amd import synth
+    from oracle import bind
+    rng = np.random.default_rng(synth.SEED_BASE + 11)
+    d = tempfile.mkdtemp()
+    L = 30_000_000
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    gen = {}
+    for name in ("rg", "qg"):
+        g = acgt[rng.integers(0, 4, L, dtype=np.uint8)]
+        gen[name] = g
+    lines, pairs, bases = [], 0, 0
+    for t in range(regions):
+        rs, rl = int(rng.integers(1, L - 13000)), int(rng.integers(2000, 12000))
+        rows = [("ref", "spA", "c0", int(rng.integers(0, 2)), rs, rs + rl - 1)]
+        for k in range(3):
+            qs, ql = int(rng.integers(1, L - 13000)), int(rng.integers(2000, 12000))
+            for _ in range(int(rng.integers(0, 12))):  # conserved fragments, 8% substitutions
+                ln = int(rng.integers(40, 300))
+                if ln >= min(rl, ql):
+                    continue
+                a = rs - 1 + int(rng.integers(0, rl - ln))
+                b = qs - 1 + int(rng.integers(0, ql - ln))
+                m = gen["rg"][a:a + ln].copy()
+                sub = rng.random(ln) < 0.08
+                m[sub] = acgt[rng.integers(0, 4, int(sub.sum()), dtype=np.uint8)]
+                gen["qg"][b:b + ln] = m
+            rows.append((f"q{k}", f"sp{chr(66 + k)}", "c0", int(rng.integers(0, 2)), qs, qs + ql - 1))
+            pairs += 1
+            bases += rl * ql
+        with open(os.path.join(d, f"r{t:05d}.fa"), "w") as fh:
+            for name, sp, c, strand, st, en in rows:
+                fh.write(f">{name}\tspecies:{sp}\tchr:{c}\tstrand:{strand}\tstart:{st}\tend:{en}\nACGT\n")
+        lines.append(f"dCNS cut1Gap -ra rg.fa -qa qg.fa -i r{t:05d}.fa -r ref -o r{t:05d}.fa.5")
+    for name, g in gen.items():
+        g = g.copy()
+        for at in rng.integers(0, L - 500, 20000):  # masked runs
+            g[at:at + int(rng.integers(10, 400))] = ord("n")
+        with open(os.path.join(d, name + ".fa"), "wb") as fh:
+            fh.write(b">c0\n")
+            body = np.full((L + 59) // 60 * 61, ord("\n"), np.uint8)
+            full = L // 60
+            body[:full * 61].reshape(full, 61)[:, :60] = g[:full * 60].reshape(full, 60)
+            rest = L - full * 60
+            if rest:
+                body[full * 61:full * 61 + rest] = g[full * 60:]
+                body = body[:full * 61 + rest + 1]
+            fh.write(body.tobytes())
+        with open(os.path.join(d, name + ".fa.fai"), "w") as fh:
+            fh.write(f"c0\t{L}\t4\t60\t61\n")
+    with open(os.path.join(d, "command1"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    t = time.perf_counter()
+    r = subprocess.run([dcns_amd.CLI_PATH, "batch", "command1"], cwd=d, capture_output=True, text=True, timeout=900)
+    dt = time.perf_counter() - t
+    out = {"workload": f"{regions} cut1Gap region files (a 2-12 kb reference region and 3 query regions of 2-12 kb "
+                       "each) over two 30 Mb genomes with masked runs, cut1Gap defaults; files in, SAM / o files out",
+           "status": r.returncode, "regions_per_s": round(regions / dt, 1), "pairs_per_s": round(pairs / dt, 1),
+           "seconds": round(dt, 2), "pairs": pairs}
+    log(f"batch: {regions} regions ({pairs} pairs) in {dt:.2f} s ({regions / dt:.0f} regions/s)")
+    one = os.path.join(d, "one")
+    os.makedirs(one)
+    for f in ("rg.fa", "rg.fa.fai", "qg.fa", "qg.fa.fai"):
+        os.link(os.path.join(d, f), os.path.join(one, f))
+    for t in range(sample):
+        shutil.copy(os.path.join(d, f"r{t:05d}.fa"), one)
+    t = time.perf_counter()
+    for ln in lines[:sample]:
+        subprocess.run([dcns_amd.CLI_PATH] + ln.split()[1:], cwd=one, capture_output=True, timeout=300)
+    tg = time.perf_counter() - t
+    out["one_process_per_line_gpu"] = {"regions_per_s": round(sample / tg, 2), "sample": f"first {sample} lines"}
+    same = all(open(os.path.join(one, f), "rb").read() == open(os.path.join(d, f), "rb").read()
+               for f in os.listdir(one) if ".fa.5" in f)
+    out["per_line_files_identical"] = bool(same)
+    if cpu and os.path.exists(bind.REF_CLI):
+        ref = os.path.join(d, "ref")
+        os.makedirs(ref)
+        for f in os.listdir(one):
+            if ".fa.5" not in f:
+                os.link(os.path.join(one, f), os.path.join(ref, f))
+        t = time.perf_counter()
+        for ln in lines[:sample]:
+            subprocess.run([bind.REF_CLI] + ln.split()[1:], cwd=ref, capture_output=True, timeout=900)
+        tc = time.perf_counter() - t
+        out["cpu_reference"] = {"value": round(sample / tc, 3), "unit": "regions/s", "cores": 1, "kind": "reference",
+                                "sample": f"the first {sample} lines, one reference cut1Gap process each",
+                                "seconds": round(tc, 2)}
+        out["cpu_sample_bit_exact"] = all(
+            open(os.path.join(ref, f), "rb").read() == open(os.path.join(d, f), "rb").read()
+            for f in os.listdir(ref) if ".fa.5" in f)
+        log(f"reference cut1Gap: {sample / tc:.3f} regions/s on one core")
+    shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def config2_significance(ctx, sur, log):
     """Config-2 fragments (500 kb masked pair, default scoring) reported with
     the lambda/K of the judge, of the surrogate and of the reference's
@@ -496,6 +602,8 @@ def main():
     ap.add_argument("--no-mask", action="store_true")
     ap.add_argument("--mask-only", action="store_true", help="only the maskGenome phase")
     ap.add_argument("--mask-gbp", type=float, default=1.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true", help="only the batch-command phase")
     ap.add_argument("--two-piece-only", action="store_true",
                     help="run only the two-piece (cut2Gaps) and weighted measurements and print them")
     args = ap.parse_args()
@@ -506,6 +614,11 @@ def main():
 
     import dcns_amd
     from dcns_amd import synth
+
+    if args.batch_only:
+        lg = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+        print(json.dumps({"batch": batch_phase(lg, cpu=not args.no_cpu)}), flush=True)
+        return
 
     if args.mask_only:
         ctx = dcns_amd.Context(0, timing=True)
@@ -616,6 +729,8 @@ def main():
         line["lambdak"] = lambdak_phase(ctx, log)
     if not args.no_mask:
         line["mask"] = mask_phase(ctx, log, cpu=not args.no_cpu, gbp=args.mask_gbp)
+    if not args.no_batch:
+        line["batch"] = batch_phase(log, cpu=not args.no_cpu)
 
     if not args.no_cpu:
         cores = args.cpu_cores or min(len(os.sched_getaffinity(0)), 16)
