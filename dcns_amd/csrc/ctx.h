@@ -135,7 +135,7 @@ struct dcns_ctx {
     dcns::DBuf sort_scr, task_order;                     // scan tasks ordered by row count
     dcns::DBuf xorder;                                   // x-drop pass-2 order
     dcns::DBuf rnd_thr, rnd_len1, rnd_len2, rnd_seed, rnd_out;  // random fragment pairs generated on the device
-    dcns::DBuf mk_seq, mk_off, mk_codes, mk_table, mk_gbytes, mk_ghash, mk_gtab;  // maskGenome: genome, k-mer set
+    dcns::DBuf mk_seq, mk_off, mk_keys, mk_desc, mk_gbytes, mk_ghash, mk_gtab;  // maskGenome: genome, k-mer set
     dcns::DBuf mk_mask, mk_ranges, mk_boff, mk_body;                          // maskGenome: mask, CDS, FASTA bodies
     dcns::HBuf h_hits, h_roots, h_nodes, h_xout, h_cigar;  // pinned download buffers
 

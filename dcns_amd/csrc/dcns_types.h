@@ -150,6 +150,5 @@ struct KmerGeneral {
     uint64_t pk1;            // base^(k-1) mod 2^64
 };
 constexpr uint64_t kKmerHashBase = 0x100000001b3ull;
-constexpr int kKmerBucket = 2;  // 2-bit codes per bucket of the code table (one 16-byte load)
 
 }  // namespace dcns

@@ -75,16 +75,15 @@ hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *p
                             SgOut *out, uint32_t *cigar, const SgWeights &W);
 
 // mask.hip: maskGenome's k-mer scan, CDS ranges and FASTA bodies.
-int64_t kmer_table_buckets(int64_t nkeys);  // buckets (4 codes each) for nkeys codes
 int64_t kmer_general_slots(int64_t n);      // slots of the general members' table
 int64_t kmer_mask_padded(int64_t n);        // bytes to allocate for a mask of n positions
 int64_t kmer_body_padded(int64_t n);        // bytes to allocate for n bytes of FASTA bodies
-hipError_t launch_kmer_table(hipStream_t st, const uint64_t *codes, int64_t n, uint64_t *table, int64_t buckets);
 hipError_t launch_kmer_general(hipStream_t st, const uint64_t *hashes, int64_t n, uint32_t *gtab, int64_t slots);
 // mask[j] = 1 when a window [i, i + k) with i <= j < i + k, inside one sequence, is in the set.
+// slots / overflow: the minimizer buckets of the code members (kmer_hash.h), buckets a power of two.
 hipError_t launch_kmer_scan(hipStream_t st, const uint8_t *seq, int64_t N, const int64_t *seg_off, int32_t nseg,
-                            int32_t k, bool codes, const uint64_t *table, int64_t buckets, const KmerGeneral &g,
-                            uint8_t *mask);
+                            int32_t k, const uint64_t *slots, const uint64_t *overflow, int64_t buckets,
+                            const KmerGeneral &g, uint8_t *mask);
 // mask[x] = 1 for x in [ranges[2r], ranges[2r + 1]).
 hipError_t launch_paint_ranges(hipStream_t st, const int64_t *ranges, int64_t n, uint8_t *mask);
 // FASTA bodies: sequence s occupies out[body_off[s], body_off[s + 1]).

@@ -8,10 +8,9 @@
 //
 // K-mer set, two parts:
 //   * members made of upper-case A/C/G/T only, k <= 31: 2-bit codes (base j
-//     in bits 2j..2j+1 as (b >> 1) & 3: A 0, C 1, T 2, G 3) in a bucketed
-//     open-addressing table (2 keys = 16 bytes per bucket, one load per probe,
-//     linear probing over buckets, load <= 1/4, empty = all ones, which no
-//     code of k <= 31 bases equals);
+//     in bits 2j..2j+1 as (b >> 1) & 3: A 0, C 1, T 2, G 3) in minimizer
+//     buckets of one 128-byte slot each (kmer_hash.h), keys past the
+//     fifteenth of a bucket in an overflow array;
 //   * every other member (other bytes, or k >= 32): a table of member
 //     indices keyed by a 64-bit polynomial hash of the bytes, each hit
 //     verified byte by byte.
@@ -22,19 +21,23 @@
 // k <= 31, k_kmer_scan16: a thread owns 16 window starts / 16 positions.  It
 // reads its 48 bytes with three aligned 16-byte loads (consecutive threads,
 // consecutive blocks), packs them into 96 bits of 2-bit codes and a 48-bit
-// mask of other bytes with SWAR, takes each window's code by a constant
-// shift, and probes the table with all eight loads of a half in flight.  The
-// 16 hit bits go through LDS to the next two threads; a position is masked
-// when one of the k windows ending on it hit, which is a shift-OR of the 48
-// hit bits of the thread and its two predecessors.  The first two threads of
-// a workgroup recompute the previous workgroup's last 32 windows.  HBM
-// traffic per position: 1 byte read, 1 byte written, plus the probes.
+// mask of other bytes with SWAR, and takes each window's code and l-mers by
+// constant shifts.  A window whose bucket differs from the previous window's
+// loads the bucket's slot (eight 16-byte loads of one line, one round trip);
+// the windows of a genome share their minimizer over ~(w + 1) / 2
+// consecutive starts, so one line serves several windows.  The 16 hit bits go through LDS to the next
+// two threads; a position is masked when one of the k windows ending on it
+// hit, which is a shift-OR of the 48 hit bits of the thread and its two
+// predecessors.  The first two threads of a workgroup recompute the previous
+// workgroup's last 32 windows.  HBM traffic per position: 1 byte read, 1
+// byte written, plus the bucket reads.
 //
 // k >= 32, k_kmer_scan_long: one lane per CHUNK positions keeps a rolling
 // hash of the last k bytes and the start of the last hit window.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "kmer_hash.h"
 
 namespace dcns {
 namespace {
@@ -89,22 +92,6 @@ __device__ __forceinline__ int32_t seg_of(const int64_t *__restrict__ seg_off, i
     return lo;
 }
 
-__global__ void k_kmer_insert(const uint64_t *__restrict__ codes, int64_t n, uint64_t *__restrict__ table,
-                              uint64_t bmask) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t key = codes[i];
-    uint64_t b = mix(key) & bmask;
-    for (uint64_t t = 0; t <= bmask; ++t) {  // at most a quarter of the slots are used: an empty slot is found
-        unsigned long long *bk = (unsigned long long *)(table + b * kKmerBucket);
-        for (int s = 0; s < kKmerBucket; ++s) {
-            const unsigned long long prev = atomicCAS(bk + s, (unsigned long long)kEmpty, (unsigned long long)key);
-            if (prev == kEmpty || prev == key) return;
-        }
-        b = (b + 1) & bmask;
-    }
-}
-
 __global__ void k_kmer_insert_general(const uint64_t *__restrict__ hashes, int64_t n, uint32_t *__restrict__ gtab,
                                       uint64_t gmask) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -114,20 +101,6 @@ __global__ void k_kmer_insert_general(const uint64_t *__restrict__ hashes, int64
         if (atomicCAS(gtab + s, 0u, (uint32_t)(i + 1)) == 0u) return;
         s = (s + 1) & gmask;
     }
-}
-
-// Continues a lookup whose home bucket was full and did not hold the key.
-__device__ __noinline__ bool probe_on(const uint64_t *__restrict__ table, uint64_t bmask, uint64_t b, uint64_t key) {
-    for (uint64_t t = 0; t < bmask; ++t) {
-        b = (b + 1) & bmask;
-        const uint64_t *bk = table + b * kKmerBucket;
-        for (int s = 0; s < kKmerBucket; ++s) {
-            const uint64_t v = bk[s];
-            if (v == key) return true;
-            if (v == kEmpty) return false;
-        }
-    }
-    return false;
 }
 
 __device__ __noinline__ bool general_lookup(const KmerGeneral &g, const uint8_t *__restrict__ win, uint64_t h,
@@ -154,10 +127,24 @@ __device__ __noinline__ bool general_window(const KmerGeneral &g, const uint8_t 
     return general_lookup(g, win, h, k);
 }
 
+// keys of a bucket past its slot (sorted)
+__device__ __noinline__ bool overflow_lookup(const uint64_t *__restrict__ keys, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint64_t v = keys[mid];
+        if (v == key) return true;
+        if (v < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return false;
+}
+
 template <bool GEN>
 __global__ __launch_bounds__(WG) void k_kmer_scan16(const uint8_t *__restrict__ seq, int64_t N,
                                                     const int64_t *__restrict__ seg_off, int32_t nseg, int32_t k,
-                                                    const uint64_t *__restrict__ table, uint64_t bmask,
+                                                    const uint64_t *__restrict__ slots,
+                                                    const uint64_t *__restrict__ overflow, uint32_t nb_mask,
                                                     KmerGeneral g, uint8_t *__restrict__ mask) {
     __shared__ uint32_t lds_hits[WG];
     __shared__ int32_t lds_seg[2];
@@ -175,15 +162,15 @@ __global__ __launch_bounds__(WG) void k_kmer_scan16(const uint8_t *__restrict__ 
     if (i0 >= 0 && i0 < N) {
         const uint4 *src = (const uint4 *)(seq + i0);  // 16-byte aligned; the buffer holds N + 128 bytes
         const uint4 b0 = src[0], b1 = src[1], b2 = src[2];
-        const uint32_t w[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+        const uint32_t wd[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
         uint64_t lo = 0, bad = 0;
         uint32_t hi = 0;
 #pragma unroll
         for (int q = 0; q < 12; ++q) {
-            const uint32_t c = codes4(w[q]);
+            const uint32_t c = codes4(wd[q]);
             if (q < 8) lo |= (uint64_t)c << (8 * q);
             else hi |= c << (8 * (q - 8));
-            bad |= (uint64_t)other_bytes4(w[q]) << (4 * q);
+            bad |= (uint64_t)other_bytes4(wd[q]) << (4 * q);
         }
         if (N - i0 < 48) bad |= ~0ull << (N - i0);  // past the genome
         // sequence starts inside (i0, i0 + 48)
@@ -195,29 +182,55 @@ __global__ __launch_bounds__(WG) void k_kmer_scan16(const uint8_t *__restrict__ 
         }
         const uint64_t kbits = (1ull << k) - 1, kin = (1ull << (k - 1)) - 1;
         const uint64_t cmask = (1ull << (2 * k)) - 1;
+        const int w = minimizer_w(k), l = k - w + 1;
+        const uint64_t lmask = (1ull << (2 * l)) - 1;
+        // l-mer hashes at offsets 0 .. 15 + w - 1, then each window's bucket
+        uint32_t lh[SPAN + kMinimizerW - 1];
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-            uint64_t key[G], home[G];
-            bool va[G];
-            ulonglong2 x[G];
+        for (int p = 0; p < SPAN + kMinimizerW - 1; ++p)
+            lh[p] = lmer_hash((p == 0 ? lo : ((lo >> (2 * p)) | ((uint64_t)hi << (64 - 2 * p)))) & lmask);
+        uint64_t key[SPAN];
+        uint32_t bk[SPAN];
+        uint32_t valid = 0;
 #pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const int t = half * G + u;
-                key[u] = (t == 0 ? lo : ((lo >> (2 * t)) | ((uint64_t)hi << (64 - 2 * t)))) & cmask;
-                va[u] = ((bad >> t) & kbits) == 0 && ((bnd >> (t + 1)) & kin) == 0;
-                home[u] = va[u] ? (mix(key[u]) & bmask) : 0;
-                x[u] = *(const ulonglong2 *)(table + home[u] * kKmerBucket);
+        for (int t = 0; t < SPAN; ++t) {
+            key[t] = (t == 0 ? lo : ((lo >> (2 * t)) | ((uint64_t)hi << (64 - 2 * t)))) & cmask;
+            uint32_t mn = lh[t];
+#pragma unroll
+            for (int q = 1; q < kMinimizerW; ++q)
+                if (q < w) mn = min(mn, lh[t + q]);
+            bk[t] = minimizer_bucket(mn, nb_mask);
+            if (((bad >> t) & kbits) == 0 && ((bnd >> (t + 1)) & kin) == 0) valid |= 1u << t;
+        }
+        // a window whose bucket differs from the one held loads its slot
+        ulonglong2 sv[kSlotWords / 2];
+        uint32_t held = 0;
+        bool have = false;
+#pragma unroll
+        for (int t = 0; t < SPAN; ++t) {
+            if ((valid >> t) & 1u) {
+                if (!have || bk[t] != held) {
+                    const ulonglong2 *sl = (const ulonglong2 *)(slots + (uint64_t)bk[t] * kSlotWords);
+#pragma unroll
+                    for (int q = 0; q < kSlotWords / 2; ++q) sv[q] = sl[q];
+                    held = bk[t];
+                    have = true;
+                }
+                bool m = sv[0].y == key[t];
+#pragma unroll
+                for (int q = 1; q < kSlotWords / 2; ++q) m |= sv[q].x == key[t] || sv[q].y == key[t];
+                const uint32_t cnt = (uint32_t)sv[0].x;
+                if (!m && cnt > (uint32_t)kSlotKeys)
+                    m = overflow_lookup(overflow + (sv[0].x >> 32), cnt - kSlotKeys, key[t]);
+                hits |= (m ? 1u : 0u) << t;
             }
+        }
+        if (GEN) {
 #pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const int t = half * G + u;
-                const bool found = x[u].x == key[u] || x[u].y == key[u];
-                const bool open = x[u].x == kEmpty || x[u].y == kEmpty;
-                bool hit = va[u] && (found || (!open && probe_on(table, bmask, home[u], key[u])));
-                if (GEN && !va[u] && ((bnd >> (t + 1)) & kin) == 0 && i0 + t + k <= N)
-                    hit = general_window(g, seq + i0 + t, k);
-                hits |= (hit ? 1u : 0u) << t;
-            }
+            for (int t = 0; t < SPAN; ++t)
+                if (!((valid >> t) & 1u) && ((bnd >> (t + 1)) & kin) == 0 && i0 + t + k <= N &&
+                    general_window(g, seq + i0 + t, k))
+                    hits |= 1u << t;
         }
     }
     lds_hits[tid] = hits;
@@ -433,12 +446,6 @@ __global__ __launch_bounds__(256) void k_mask_emit_bytes(const uint8_t *__restri
 
 }  // namespace
 
-int64_t kmer_table_buckets(int64_t nkeys) {
-    int64_t slots = 1024;
-    while (slots < 4 * nkeys) slots *= 2;
-    return slots / kKmerBucket;
-}
-
 int64_t kmer_general_slots(int64_t n) {
     int64_t slots = 1024;
     while (slots < 2 * n) slots *= 2;
@@ -447,14 +454,6 @@ int64_t kmer_general_slots(int64_t n) {
 
 int64_t kmer_mask_padded(int64_t n) { return (n + CHUNK - 1) / CHUNK * CHUNK + CHUNK; }
 int64_t kmer_body_padded(int64_t n) { return (n + EMIT - 1) / EMIT * EMIT + EMIT; }
-
-hipError_t launch_kmer_table(hipStream_t st, const uint64_t *codes, int64_t n, uint64_t *table, int64_t buckets) {
-    hipError_t e = hipMemsetAsync(table, 0xff, (size_t)buckets * kKmerBucket * sizeof(uint64_t), st);
-    if (e != hipSuccess || n == 0) return e;
-    hipLaunchKernelGGL(k_kmer_insert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, codes, n, table,
-                       (uint64_t)(buckets - 1));
-    return hipGetLastError();
-}
 
 hipError_t launch_kmer_general(hipStream_t st, const uint64_t *hashes, int64_t n, uint32_t *gtab, int64_t slots) {
     hipError_t e = hipMemsetAsync(gtab, 0, (size_t)slots * sizeof(uint32_t), st);
@@ -465,19 +464,19 @@ hipError_t launch_kmer_general(hipStream_t st, const uint64_t *hashes, int64_t n
 }
 
 hipError_t launch_kmer_scan(hipStream_t st, const uint8_t *seq, int64_t N, const int64_t *seg_off, int32_t nseg,
-                            int32_t k, bool codes, const uint64_t *table, int64_t buckets, const KmerGeneral &g,
-                            uint8_t *mask) {
+                            int32_t k, const uint64_t *slots, const uint64_t *overflow, int64_t buckets,
+                            const KmerGeneral &g, uint8_t *mask) {
     if (N <= 0) return hipSuccess;
     const bool gen = g.n > 0;
-    if (k >= 1 && k <= 31 && codes) {  // codes: the code table exists (it may hold no key)
+    if (k >= 1 && k <= 31 && buckets > 0) {  // buckets: the bucket table exists (it may hold no key)
         const dim3 grid((unsigned)((N + WG_POS - 1) / WG_POS)), block(WG);
-        const uint64_t bmask = buckets > 0 ? (uint64_t)(buckets - 1) : 0;
+        const uint32_t nb_mask = (uint32_t)(buckets - 1);
         if (gen)
-            hipLaunchKernelGGL(k_kmer_scan16<true>, grid, block, 0, st, seq, N, seg_off, nseg, k, table, bmask, g,
-                               mask);
+            hipLaunchKernelGGL(k_kmer_scan16<true>, grid, block, 0, st, seq, N, seg_off, nseg, k, slots, overflow,
+                               nb_mask, g, mask);
         else
-            hipLaunchKernelGGL(k_kmer_scan16<false>, grid, block, 0, st, seq, N, seg_off, nseg, k, table, bmask, g,
-                               mask);
+            hipLaunchKernelGGL(k_kmer_scan16<false>, grid, block, 0, st, seq, N, seg_off, nseg, k, slots, overflow,
+                               nb_mask, g, mask);
     } else if (k >= 32 && gen) {
         const int64_t lanes = (N + CHUNK - 1) / CHUNK;
         hipLaunchKernelGGL(k_kmer_scan_long, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, seq, N,

@@ -1,14 +1,60 @@
 // maskgenome.cpp -- staging and launches of maskGenome's device pipeline.
 #include "maskgenome.h"
 
+#include <algorithm>
 #include <memory>
 #include <string>
 
 #include "kernels.h"
+#include "kmer_hash.h"
 
 namespace dcns {
 
 int64_t mask_body_size(int64_t len, int32_t width) { return len + (width > 0 ? len / width : 0) + 1; }
+
+// The code members grouped by minimizer bucket into 128-byte slots
+// (kmer_hash.h), distinct keys, about four per bucket on average; keys past
+// the fifteenth of a bucket go to the overflow array.
+void build_buckets(const std::vector<uint64_t> &codes, int32_t k, std::vector<uint64_t> &slots,
+                   std::vector<uint64_t> &overflow, uint32_t &nb_mask) {
+    const int64_t n = (int64_t)codes.size();
+    if (n >= (int64_t)1 << 32) throw Error(DCNS_ERR_CAPACITY, "more than 2^32 k-mers");
+    int64_t nb = 256;
+    while (nb < n / 4) nb *= 2;
+    nb_mask = (uint32_t)(nb - 1);
+    std::vector<uint32_t> b((size_t)n);
+    parallel_for(n, [&](int64_t i) { b[(size_t)i] = kmer_bucket(codes[(size_t)i], k, nb_mask); });
+    std::vector<int64_t> start((size_t)nb + 1, 0);
+    for (int64_t i = 0; i < n; ++i) ++start[(size_t)b[(size_t)i] + 1];
+    for (int64_t j = 0; j < nb; ++j) start[(size_t)j + 1] += start[(size_t)j];
+    std::vector<uint64_t> grouped((size_t)n);
+    {
+        std::vector<int64_t> at(start.begin(), start.end() - 1);
+        for (int64_t i = 0; i < n; ++i) grouped[(size_t)at[b[(size_t)i]]++] = codes[(size_t)i];
+    }
+    std::vector<uint32_t> cnt((size_t)nb);
+    const int64_t chunks = 256, per = (nb + chunks - 1) / chunks;
+    parallel_for(chunks, [&](int64_t ch) {
+        for (int64_t j = ch * per; j < std::min(nb, (ch + 1) * per); ++j) {
+            auto lo = grouped.begin() + start[(size_t)j], hi = grouped.begin() + start[(size_t)j + 1];
+            std::sort(lo, hi);
+            cnt[(size_t)j] = (uint32_t)(std::unique(lo, hi) - lo);
+        }
+    });
+    slots.assign((size_t)nb * kSlotWords, ~0ull);
+    overflow.clear();
+    for (int64_t j = 0; j < nb; ++j) {
+        const uint64_t *key = grouped.data() + start[(size_t)j];
+        const uint32_t c = cnt[(size_t)j];
+        uint64_t *slot = slots.data() + j * kSlotWords;
+        slot[0] = (uint64_t)c | ((uint64_t)overflow.size() << 32);
+        for (uint32_t q = 0; q < c; ++q) {
+            if (q < (uint32_t)kSlotKeys) slot[1 + q] = key[q];
+            else overflow.push_back(key[q]);
+        }
+    }
+    if (overflow.size() >= ((size_t)1 << 32)) throw Error(DCNS_ERR_CAPACITY, "k-mer overflow keys past 2^32");
+}
 
 void mask_stage(dcns_ctx *c, const KmerSet &ks, const uint8_t *seq, const int64_t *off, int32_t nseq) {
     if (nseq < 0 || (nseq > 0 && (!off || !seq))) throw Error(DCNS_ERR_ARG, "invalid genome arrays");
@@ -33,20 +79,23 @@ void mask_stage(dcns_ctx *c, const KmerSet &ks, const uint8_t *seq, const int64_
     c->mk_off.ensure(ms->off.size() * sizeof(int64_t));
     hip_check(hipMemcpyAsync(c->mk_off.p, ms->off.data(), ms->off.size() * sizeof(int64_t), hipMemcpyHostToDevice, st),
               "upload offsets");
-    // k <= 31: the scan probes the code table for every ACGT window, so it is
-    // built (possibly empty) whenever the set has a member of length k
+    // k <= 31: the scan looks every ACGT window up in the minimizer buckets,
+    // so they are built (possibly empty) whenever the set has a member of length k
     if (ks.k >= 1 && ks.k <= 31 && (!ks.codes.empty() || ks.n_general() > 0)) {
-        const int64_t n = (int64_t)ks.codes.size();
-        ms->buckets = kmer_table_buckets(n);
-        c->mk_codes.ensure((size_t)n * sizeof(uint64_t) + 8);
-        if (n > 0)
-            hip_check(hipMemcpyAsync(c->mk_codes.p, ks.codes.data(), (size_t)n * sizeof(uint64_t),
+        std::vector<uint64_t> slots, overflow;
+        uint32_t nb_mask = 0;
+        build_buckets(ks.codes, ks.k, slots, overflow, nb_mask);
+        ms->buckets = (int64_t)nb_mask + 1;
+        c->mk_desc.ensure(slots.size() * sizeof(uint64_t));
+        c->mk_keys.ensure(overflow.size() * sizeof(uint64_t) + 8);
+        hip_check(hipMemcpyAsync(c->mk_desc.p, slots.data(), slots.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                 st),
+                  "upload k-mer buckets");
+        if (!overflow.empty())
+            hip_check(hipMemcpyAsync(c->mk_keys.p, overflow.data(), overflow.size() * sizeof(uint64_t),
                                      hipMemcpyHostToDevice, st),
-                      "upload k-mer codes");
-        c->mk_table.ensure((size_t)ms->buckets * kKmerBucket * sizeof(uint64_t));
-        c->launch("k_kmer_insert", [&] {
-            return launch_kmer_table(st, c->mk_codes.as<uint64_t>(), n, c->mk_table.as<uint64_t>(), ms->buckets);
-        });
+                      "upload k-mer overflow keys");
+        c->sync();  // slots and overflow are local
     }
     const int64_t gn = ks.k >= 1 ? ks.n_general() : 0;
     if (gn > 0) {
@@ -85,7 +134,8 @@ void mask_run(dcns_ctx *c, const CdsMask *cds, uint8_t alt, bool soft, int32_t w
     g.bytes = c->mk_gbytes.as<uint8_t>();
     c->launch("k_kmer_scan", [&] {
         return launch_kmer_scan(st, c->mk_seq.as<uint8_t>(), N, c->mk_off.as<int64_t>(), ms.nseq, ms.k,
-                                ms.buckets > 0, c->mk_table.as<uint64_t>(), ms.buckets, g, c->mk_mask.as<uint8_t>());
+                                c->mk_desc.as<uint64_t>(), c->mk_keys.as<uint64_t>(), ms.buckets, g,
+                                c->mk_mask.as<uint8_t>());
     });
 
     std::vector<int64_t> ranges;

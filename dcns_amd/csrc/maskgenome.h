@@ -15,7 +15,7 @@ struct MaskStaged {
     int64_t N = 0;             // bases of all sequences
     std::vector<int64_t> off;  // nseq + 1 sequence offsets into the concatenation
     int32_t k = 10;
-    int64_t buckets = 0;       // code table buckets (0: no code members)
+    int64_t buckets = 0;       // minimizer buckets of the code members (0: no table)
     KmerGeneral general{};     // device pointers set at run time
     int64_t gslots = 0;
 };
