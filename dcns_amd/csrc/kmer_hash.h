@@ -4,10 +4,10 @@
 // A k-mer (2-bit codes, base j in bits 2j..2j+1) belongs to the bucket of its
 // minimizer: of its w = k - l + 1 l-mers, the one with the smallest
 // lmer_hash.  Consecutive windows of a genome mostly share their minimizer,
-// so a run of windows reads one bucket.  A bucket is one 128-byte slot, a
-// cache line: word 0 holds the key count (low 32 bits) and the start of the
-// keys past the fifteenth in an overflow array (high 32 bits); words 1..15
-// hold the first keys, the unused ones all ones (no code of k <= 31 bases).
+// so a run of windows reads one bucket.  A bucket is one 64-byte slot: word
+// 0 holds the key count (low 32 bits) and the start of the keys past the
+// seventh in an overflow array (high 32 bits); words 1..7 hold the first
+// keys, the unused ones all ones (no code of k <= 31 bases).
 // Which bucket a k-mer lands in depends on the k-mer alone, so the lookup is
 // exact whatever the hash's quality.
 #pragma once
@@ -19,7 +19,7 @@ namespace dcns {
 
 constexpr int kMinimizerL = 12;  // l-mer length for k >= 12
 constexpr int kMinimizerW = 9;   // l-mers per k-mer at most
-constexpr int kSlotWords = 16;   // 64-bit words per bucket slot
+constexpr int kSlotWords = 8;   // 64-bit words per bucket slot
 constexpr int kSlotKeys = kSlotWords - 1;
 
 // w for a given k: 1 (the k-mer is its own minimizer) below l + 1

@@ -9,8 +9,8 @@
 // K-mer set, two parts:
 //   * members made of upper-case A/C/G/T only, k <= 31: 2-bit codes (base j
 //     in bits 2j..2j+1 as (b >> 1) & 3: A 0, C 1, T 2, G 3) in minimizer
-//     buckets of one 128-byte slot each (kmer_hash.h), keys past the
-//     fifteenth of a bucket in an overflow array;
+//     buckets of one 64-byte slot each (kmer_hash.h), keys past the seventh
+//     of a bucket in an overflow array;
 //   * every other member (other bytes, or k >= 32): a table of member
 //     indices keyed by a 64-bit polynomial hash of the bytes, each hit
 //     verified byte by byte.
@@ -23,7 +23,8 @@
 // consecutive blocks), packs them into 96 bits of 2-bit codes and a 48-bit
 // mask of other bytes with SWAR, and takes each window's code and l-mers by
 // constant shifts.  A window whose bucket differs from the previous window's
-// loads the bucket's slot (eight 16-byte loads of one line, one round trip);
+// loads the bucket's slot (four 16-byte loads, one round trip), at 8 waves
+// per SIMD to keep many slot loads in flight;
 // the windows of a genome share their minimizer over ~(w + 1) / 2
 // consecutive starts, so one line serves several windows.  The 16 hit bits go through LDS to the next
 // two threads; a position is masked when one of the k windows ending on it
@@ -141,7 +142,7 @@ __device__ __noinline__ bool overflow_lookup(const uint64_t *__restrict__ keys, 
 }
 
 template <bool GEN>
-__global__ __launch_bounds__(WG) void k_kmer_scan16(const uint8_t *__restrict__ seq, int64_t N,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_kmer_scan16(const uint8_t *__restrict__ seq, int64_t N,
                                                     const int64_t *__restrict__ seg_off, int32_t nseg, int32_t k,
                                                     const uint64_t *__restrict__ slots,
                                                     const uint64_t *__restrict__ overflow, uint32_t nb_mask,
@@ -189,12 +190,14 @@ __global__ __launch_bounds__(WG) void k_kmer_scan16(const uint8_t *__restrict__ 
 #pragma unroll
         for (int p = 0; p < SPAN + kMinimizerW - 1; ++p)
             lh[p] = lmer_hash((p == 0 ? lo : ((lo >> (2 * p)) | ((uint64_t)hi << (64 - 2 * p)))) & lmask);
-        uint64_t key[SPAN];
+        // a window's code, taken again where it is compared (keeping all 16 would cost 32 VGPRs)
+        auto key_of = [&](int t) {
+            return (t == 0 ? lo : ((lo >> (2 * t)) | ((uint64_t)hi << (64 - 2 * t)))) & cmask;
+        };
         uint32_t bk[SPAN];
         uint32_t valid = 0;
 #pragma unroll
         for (int t = 0; t < SPAN; ++t) {
-            key[t] = (t == 0 ? lo : ((lo >> (2 * t)) | ((uint64_t)hi << (64 - 2 * t)))) & cmask;
             uint32_t mn = lh[t];
 #pragma unroll
             for (int q = 1; q < kMinimizerW; ++q)
@@ -216,12 +219,13 @@ __global__ __launch_bounds__(WG) void k_kmer_scan16(const uint8_t *__restrict__ 
                     held = bk[t];
                     have = true;
                 }
-                bool m = sv[0].y == key[t];
+                const uint64_t kt = key_of(t);
+                bool m = sv[0].y == kt;
 #pragma unroll
-                for (int q = 1; q < kSlotWords / 2; ++q) m |= sv[q].x == key[t] || sv[q].y == key[t];
+                for (int q = 1; q < kSlotWords / 2; ++q) m |= sv[q].x == kt || sv[q].y == kt;
                 const uint32_t cnt = (uint32_t)sv[0].x;
                 if (!m && cnt > (uint32_t)kSlotKeys)
-                    m = overflow_lookup(overflow + (sv[0].x >> 32), cnt - kSlotKeys, key[t]);
+                    m = overflow_lookup(overflow + (sv[0].x >> 32), cnt - kSlotKeys, kt);
                 hits |= (m ? 1u : 0u) << t;
             }
         }

@@ -12,15 +12,16 @@ namespace dcns {
 
 int64_t mask_body_size(int64_t len, int32_t width) { return len + (width > 0 ? len / width : 0) + 1; }
 
-// The code members grouped by minimizer bucket into 128-byte slots
-// (kmer_hash.h), distinct keys, about four per bucket on average; keys past
-// the fifteenth of a bucket go to the overflow array.
+// The code members grouped by minimizer bucket into 64-byte slots
+// (kmer_hash.h), distinct keys, about two per bucket on average (the k-mers
+// sharing a minimizer stay together); keys past the seventh of a bucket go to
+// the overflow array.
 void build_buckets(const std::vector<uint64_t> &codes, int32_t k, std::vector<uint64_t> &slots,
                    std::vector<uint64_t> &overflow, uint32_t &nb_mask) {
     const int64_t n = (int64_t)codes.size();
     if (n >= (int64_t)1 << 32) throw Error(DCNS_ERR_CAPACITY, "more than 2^32 k-mers");
     int64_t nb = 256;
-    while (nb < n / 4) nb *= 2;
+    while (nb < n / 2) nb *= 2;
     nb_mask = (uint32_t)(nb - 1);
     std::vector<uint32_t> b((size_t)n);
     parallel_for(n, [&](int64_t i) { b[(size_t)i] = kmer_bucket(codes[(size_t)i], k, nb_mask); });
