@@ -176,11 +176,18 @@ def lambdak_phase(ctx, log):
         r["settings_in_hull"] = [list(x) for x in sset if tuple(x) in set(hull)]
         ev[tag] = r
     out["surrogate"] = ev
+    ctx.kernel_time_reset()
     qps, dt = lk.inference_rate(sur)
+    kms, kn = ctx.kernel_time("mlp_infer")  # the fused five-layer kernel, HIP events
+    k_tflops = 65536 * 398336 / (kms / max(kn, 1) * 1e-3) / 1e12 if kn else None
     perm_s = ev["hull_settings"]["judge"]["seconds_per_query"]
     out["surrogate"]["inference"] = {"batch": 65536, "queries_per_s": round(qps, 1), "batch_ms": round(dt * 1e3, 3),
                                      "flop_per_query": 398336,
                                      "tflops": round(qps * 398336 / 1e12, 3),
+                                     "kernel_ms": round(kms / max(kn, 1), 4) if kn else None,
+                                     "kernel_roofline": {"bound": "mfma", "achieved": round(k_tflops, 1),
+                                                         "peak": 2500.0, "unit": "TFLOP/s",
+                                                         "frac": round(k_tflops / 2500.0, 4)} if kn else None,
                                      "permutation_seconds_per_query": perm_s,
                                      "speedup_vs_permutation": round(qps * perm_s, 1),
                                      "training_cells": sur.meta.get("training_cells"),

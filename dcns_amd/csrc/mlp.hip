@@ -100,6 +100,96 @@ __global__ __launch_bounds__(64) void k_gemm(const uint16_t *__restrict__ A, int
     }
 }
 
+// The weights of the layers in MFMA fragment order: layer l (K = in_pad,
+// N = out_pad), K step s (16 deep), column tile c (32 wide), lane q, element
+// t holds Wt(32c + q % 32, 16s + 8 (q / 32) + t) -- the B fragment k_gemm
+// builds in LDS, here one 16-byte read per lane.
+__global__ void k_mlp_frag(const uint16_t *__restrict__ wt, int K, int N, uint16_t *__restrict__ frag) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)K * N) return;
+    const int t = (int)(e & 7), q = (int)((e >> 3) & 63);
+    const int64_t rest = e >> 9;
+    const int nt = N / 32;
+    const int c = (int)(rest % nt), st = (int)(rest / nt);
+    frag[e] = wt[(int64_t)(32 * c + (q & 31)) * K + 16 * st + 8 * (q >> 5) + t];
+}
+
+// Inference, all five layers in one launch.  A wave owns 32 rows: their
+// activations stay in LDS (in place, layer after layer) and their 32 x 256
+// outputs in 8 accumulators; a workgroup's 8 waves share each 16-deep slice
+// of weight fragments through LDS (double-buffered, loaded while the
+// previous slice multiplies).  Per output tile the MFMA instruction, operand
+// fragments and K order are those of k_gemm, so the results are the same
+// bits as the layer-by-layer path.
+constexpr int kInferWaves = 8;
+constexpr int kActStride = kHid + 8;  // bf16 per LDS activation row (16-byte pad against bank conflicts)
+struct MlpOffsets {
+    int64_t frag[kLayers];
+    int64_t bias[kLayers];
+};
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const uint16_t *__restrict__ x0, int64_t M,
+                                                                const uint16_t *__restrict__ frag,
+                                                                const float *__restrict__ p, MlpOffsets off,
+                                                                float *__restrict__ out) {
+    __shared__ uint16_t act[kInferWaves][32 * kActStride];
+    __shared__ uint4 wbuf[2][8 * 64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t row0 = ((int64_t)blockIdx.x * kInferWaves + wave) * 32;
+    const bool active = row0 < M;  // M is a multiple of 32
+    uint16_t *A = act[wave];
+    const int r = lane & 31, kq = (lane >> 5) * 8;
+#pragma unroll 1
+    for (int l = 0; l < kLayers; ++l) {
+        const int S = (l == 0 ? kInPad : kHid) / 16;
+        const int NT = l == kLayers - 1 ? 1 : kHid / 32;
+        const uint4 *F = (const uint4 *)(frag + off.frag[l]);
+        const int slice = NT * 64;
+        f32x16 acc[kHid / 32];
+#pragma unroll
+        for (int c = 0; c < kHid / 32; ++c) acc[c] = f32x16{};
+        if (tid < slice) wbuf[0][tid] = F[tid];
+        __syncthreads();
+#pragma unroll 1
+        for (int st = 0; st < S; ++st) {
+            const bool more = st + 1 < S && tid < slice;
+            uint4 nxt;
+            if (more) nxt = F[(int64_t)(st + 1) * slice + tid];
+            if (active) {
+                uint4 av;
+                if (l == 0) av = *(const uint4 *)(x0 + (row0 + r) * kInPad + 16 * st + kq);
+                else av = *(const uint4 *)(A + r * kActStride + 16 * st + kq);
+                const bf16x8 fa = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+                for (int c = 0; c < kHid / 32; ++c)
+                    if (c < NT)
+                        acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            fa, __builtin_bit_cast(bf16x8, wbuf[st & 1][c * 64 + lane]), acc[c], 0, 0, 0);
+            }
+            if (more) wbuf[(st + 1) & 1][tid] = nxt;
+            __syncthreads();
+        }
+        if (!active) continue;
+        const float *bias = p + off.bias[l];
+        const int col_in = lane & 31;
+        if (l < kLayers - 1) {
+#pragma unroll
+            for (int c = 0; c < kHid / 32; ++c)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int row = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col = 32 * c + col_in;
+                    A[row * kActStride + col] = bf16_bits(gelu(acc[c][i] + bias[col]));
+                }
+        } else if (col_in < kOut) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int row = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                out[(row0 + row) * kOut + col_in] = acc[0][i] + bias[col_in];
+            }
+        }
+    }
+}
+
 // Standardised inputs, padded to 16 columns, as bf16.
 __global__ void k_prep_input(const float *__restrict__ X, const int32_t *__restrict__ idx, int64_t n, int64_t M,
                              const float *__restrict__ mean, const float *__restrict__ inv_std,
@@ -200,6 +290,7 @@ struct dcns_mlp {
     // activations for a batch of M rows
     int64_t cap_rows = 0;
     DBuf x0, act[4], pre[4], outf, dout, dpre_b[4], dpre_f[4], dact, idx, loss, Xd, Yd;
+    DBuf wfrag;  // bf16 weights in fragment order (k_mlp_infer)
 };
 
 namespace {
@@ -274,6 +365,35 @@ void forward(dcns_mlp *m, int64_t M) {
     }
     gemm<EPI_BIAS>(m, in, kHid, 1, pb + m->w_off[4], 1, kHid, (int)M, kOutPad, kHid, p + m->b_off[4], nullptr,
                    m->outf.as<float>(), nullptr, 0, kOut);
+}
+
+// Inference over M padded rows (x0 prepared): the weights are put in
+// fragment order (a few microseconds for 200 k parameters, so always current),
+// then one k_mlp_infer launch writes the output rows unpadded ([M x 2]).
+void infer(dcns_mlp *m, int64_t M) {
+    dcns_ctx *c = m->ctx;
+    MlpOffsets off{};
+    int64_t fo = 0;
+    for (int l = 0; l < kLayers; ++l) {
+        off.frag[l] = fo;
+        off.bias[l] = m->b_off[l];
+        fo += (int64_t)m->in_pad[l] * m->out_pad[l];
+    }
+    m->wfrag.ensure((size_t)fo * 2);
+    for (int l = 0; l < kLayers; ++l) {
+        const int64_t n = (int64_t)m->in_pad[l] * m->out_pad[l];
+        c->launch("mlp_frag", [&] {
+            hipLaunchKernelGGL(k_mlp_frag, dim3(blocks(n, 256)), dim3(256), 0, c->stream, m->pb.as<uint16_t>() + m->w_off[l],
+                               m->in_pad[l], m->out_pad[l], m->wfrag.as<uint16_t>() + off.frag[l]);
+            return hipGetLastError();
+        });
+    }
+    const unsigned grid = (unsigned)((M + 32 * kInferWaves - 1) / (32 * kInferWaves));
+    c->launch("mlp_infer", [&] {
+        hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(64 * kInferWaves), 0, c->stream, m->x0.as<uint16_t>(), M,
+                           m->wfrag.as<uint16_t>(), m->p.as<float>(), off, m->outf.as<float>());
+        return hipGetLastError();
+    });
 }
 
 // Backward pass: gradient of the mean squared error into m->g (fp32).
@@ -496,7 +616,7 @@ int dcns_mlp_predict(dcns_mlp *m, const float *X, int64_t n, float *Y) {
             m->Xd.ensure((size_t)k * kIn * 4);
             hip_check(hipMemcpyAsync(m->Xd.p, X + s * kIn, (size_t)k * kIn * 4, hipMemcpyHostToDevice, c->stream), "X");
             prep_input(m, m->Xd.as<float>(), nullptr, k, M);
-            forward(m, M);
+            infer(m, M);
             hip_check(hipMemcpyAsync(o.data(), m->outf.p, (size_t)k * kOut * 4, hipMemcpyDeviceToHost, c->stream), "Y");
             c->sync();
             for (int64_t r = 0; r < k; ++r)

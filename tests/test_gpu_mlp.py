@@ -102,3 +102,23 @@ def test_mlp_training_reduces_loss(gpu):
     after, _ = m.loss_grad(X, Y)
     assert np.all(np.isfinite(losses))
     assert after < 0.05 * before, (before, after)
+
+
+@pytest.mark.parametrize("n", [1, 333, 70000])
+def test_mlp_fused_inference_equals_layer_path(gpu, n):
+    """predict runs the fused inference kernel, loss_grad the layer-by-layer
+    GEMMs.  With identity scaling and targets set to predict's own outputs,
+    the loss is exactly 0 only if both give the same bits on every row
+    (70,000 rows span two predict chunks)."""
+    import dcns_amd
+    rng = np.random.default_rng(9 + n)
+    m = dcns_amd.Mlp(gpu, seed=9)
+    m.set_scaling(np.zeros(8, np.float32), np.ones(8, np.float32), np.zeros(2, np.float32), np.ones(2, np.float32))
+    X = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    Y = m.predict(X)
+    assert np.isfinite(Y).all() and Y.std() > 0 if n > 1 else np.isfinite(Y).all()
+    loss, _ = m.loss_grad(X, Y)
+    assert loss == 0.0, loss
+    Y2 = Y.copy()
+    Y2[n // 2, 1] += 1e-3  # one changed target: the loss sees it
+    assert m.loss_grad(X, Y2)[0] > 0
