@@ -360,6 +360,20 @@ int batch_main(int argc, char **argv) {
             li = le;
             continue;
         }
+        // an unreadable genome index fails this group's lines only (the index
+        // reader ends the process, as the reference's does for its one line)
+        bool genomes_ok = true;
+        for (const std::string g : {a.get("-ra"), a.get("-qa")})
+            if (!std::ifstream(g + ".fai").good() || !std::ifstream(g).good()) {
+                std::cerr << "error in opening fasta index file " << g << ".fai." << std::endl
+                          << " Please index your fasta file using samtools samtools faidx command." << std::endl;
+                genomes_ok = false;
+            }
+        if (!genomes_ok) {
+            status = 1;
+            li = le;
+            continue;
+        }
         if (!ctx) ctx = make_ctx();
         const dcns_params p = to_dcns_params(d);
         IndexedFasta refFa(a.get("-ra")), queFa(a.get("-qa"));
