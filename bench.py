@@ -421,6 +421,26 @@ def mask_phase(ctx, log, cpu=True, gbp=1.0, sample_mb=4):
                                                "on top"}}
     log(f"maskGenome: {N / 1e9:.2f} Gbp in {dt * 1e3:.2f} ms ({N / dt / 1e9:.1f} Gbp/s), "
         f"scan {scan_ms:.2f} ms, emit {emit_ms:.2f} ms, masked {frac:.3f}, {ks.members} k-mers")
+    # the whole command on the whole genome: FASTA and dump read, masked FASTA written
+    with open(os.path.join(d, "genome.fa"), "wb") as f:
+        for i, q in enumerate(seqs):
+            f.write(b">chr%d\n" % (i + 1))
+            body = np.frombuffer(q, np.uint8)
+            full = len(body) // 60
+            lines = np.full((full, 61), ord("\n"), np.uint8)
+            lines[:, :60] = body[:full * 60].reshape(full, 60)
+            f.write(lines.tobytes())
+            if len(body) > full * 60:
+                f.write(body[full * 60:].tobytes() + b"\n")
+    t = time.perf_counter()
+    r = subprocess.run([dcns_amd.CLI_PATH, "maskGenome", "-i", "genome.fa", "-o", "genome_masked.fa", "-k",
+                        "kmers.fa", "-f", "2"], cwd=d, capture_output=True, text=True, timeout=900)
+    out["command_seconds"] = round(time.perf_counter() - t, 2) if r.returncode == 0 else None
+    out["command_note"] = "dCNS maskGenome on the whole genome: files in, masked FASTA out (host I/O included)"
+    log(f"dCNS maskGenome command on {N / 1e9:.2f} Gbp: {out['command_seconds']} s")
+    for f in ("genome.fa", "genome_masked.fa"):
+        if os.path.exists(os.path.join(d, f)):
+            os.remove(os.path.join(d, f))
     if cpu and os.path.exists(bind.REF_CLI):
         n = int(sample_mb * 1e6)
         with open(os.path.join(d, "sample.fa"), "w") as f:
