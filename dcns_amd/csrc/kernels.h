@@ -12,6 +12,19 @@ namespace dcns {
 // seedscan.hip
 // The level-0 scan runs in packed f16 when every DP value is an exact f16 integer.
 inline bool scan_uses_f16(const Params &P) { return (int64_t)P.window * P.match <= 2048; }
+// The later levels' scan runs two tasks per lane in packed f16 when, besides,
+// the f16 patterns of match and mismatch have a zero low byte: a reference
+// base's scores against A, C, G, T are then the four high bytes of one word,
+// mis4 (mismatch in every byte) with diff (match ^ mismatch) xored into the
+// base's own byte.
+inline bool scan_general_packed(const Params &P, uint32_t &mis4, uint32_t &diff) {
+    if (!scan_uses_f16(P)) return false;
+    const uint32_t m = f16_bits_of_int(P.match), x = f16_bits_of_int(P.mismatch);
+    if ((m & 0xffu) || (x & 0xffu)) return false;
+    mis4 = (x >> 8) * 0x01010101u;
+    diff = (m ^ x) >> 8;
+    return true;
+}
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
                                int32_t nwt, const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount,
                                int32_t cap, int32_t *hitlist, Counters *ctr);

@@ -12,8 +12,13 @@
 //    v_perm_b32 of a uniform 64-bit score table by a per-lane selector.
 //    H, E and the selectors of all window columns stay in VGPRs for the whole
 //    scan, so the only memory traffic is the reference stream.
-//  * k_scan_general  -- later levels (right sub-ranges of the recursion).
-//    One lane per task, int32 lanes, per-lane reference rows.
+//  * k_scan_general_h -- later levels (right sub-ranges of the recursion).
+//    One lane per task, per-lane reference rows, the window split in two
+//    column strips held as packed-f16 halves (the second strip one row
+//    behind); tasks of more than 4096 rows one per wave instead (columns
+//    across the lanes on an anti-diagonal), so that a long rescan does not
+//    hold a whole level.  k_scan_general is the int32 version for scorings
+//    the packed one cannot represent.
 //  * k_resolve       -- walks the left spine of the recursion for every
 //    scanned node: the left child's scan is a prefix of its parent's, so its
 //    maximum is read from the parent's trajectory; the reverse SW (<= 76 x 38)
@@ -24,6 +29,8 @@
 // min_score - 1.  The first maximum of any row prefix [a, b') of the scan is
 // then the last entry with row < b'.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "dp_common.h"
 #include "kernels.h"
@@ -253,22 +260,114 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 // in every cell (a horizontal run followed by a vertical one can always be
 // swapped without changing score or end cell), and the shared H + open term
 // saves two operations per cell.
+// One long ScanTask across a wave: lane j owns window column j and works on
+// row a + s - j at step s (an anti-diagonal), so a cell's left and diagonal
+// neighbours were computed by lane j - 1 one and two steps earlier and reach
+// lane j by a DPP shift (wave_shr:1, lane 0 taking the row's inputs).  The
+// reference base travels the same way as its score selector: with symmetric
+// scoring the lane's table is the one of its own query base.  The row
+// maximum travels as a partial (maximum, first column) and is complete at the
+// window's last column, whose lane keeps the running best and records the
+// trajectory.  The N columns past the window that the lane kernel also
+// evaluates never hold a new maximum (each is bounded by the previous row's
+// last real cell), so the records are the same.  R rows take R + width - 1
+// steps instead of R x width lane steps.
+template <int WM>
+__device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const ScanTask &t,
+                               const Params &P, const ScoreTab &tab, TrajEntry *__restrict__ traj,
+                               int32_t *__restrict__ tcount, int32_t cap, Counters *__restrict__ ctr) {
+    const int lane = threadIdx.x;
+    const PairDesc pd = pairs[t.pair];
+    const int8_t *__restrict__ ref = seq + pd.off1;
+    const int8_t *__restrict__ qry = seq + pd.off2;
+    const int sp = t.win * P.step;
+    const int tw = min(P.window, pd.len2 - sp);
+    const int qc = lane < tw ? qry[sp + lane] : 2;
+    const int qi = (unsigned)qc < 8u ? qc : 2;
+    const uint32_t thi = tab.hi[qi], tlo = tab.lo[qi];
+    const int32_t O = P.open, X = P.ext;
+    const int rows = t.b - t.a, steps = rows + tw - 1, last = tw - 1;
+    int32_t E = 0, diag = 0, h_out = 0, f_out = 0, pm_out = -1, pj_out = 0;
+    uint32_t sel_out = 0x0C0C0C0Cu;
+    int32_t best = P.min_score - 1, cnt = 0;
+    // reference bases 64 rows at a time, one per lane (2 past the task's
+    // rows), the next 64 in flight while these are used
+    int32_t cur = lane < rows ? ref[t.a + lane] : 2;
+    int32_t nxt = 64 + lane < rows ? ref[t.a + 64 + lane] : 2;
+    for (int st = 0; st < steps; ++st) {
+        const int c0 = __builtin_amdgcn_readlane(cur, st & 63);  // row a + st
+        if ((st & 63) == 63) {
+            cur = nxt;
+            const int r = st + 65 + lane;
+            nxt = r < rows ? ref[t.a + r] : 2;
+        }
+        const uint32_t sel = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)sel32(c0), (int32_t)sel_out, 0x138, 0xf,
+                                                                   0xf, false);
+        const int32_t fin = __builtin_amdgcn_update_dpp(0, f_out, 0x138, 0xf, 0xf, false);
+        const int32_t hin = __builtin_amdgcn_update_dpp(0, h_out, 0x138, 0xf, 0xf, false);
+        const int32_t pmin = __builtin_amdgcn_update_dpp(-1, pm_out, 0x138, 0xf, 0xf, false);
+        const int32_t pjin = __builtin_amdgcn_update_dpp(0, pj_out, 0x138, 0xf, 0xf, false);
+        const int row = st - lane;
+        const bool valid = row >= 0 && row < rows && lane < tw;
+        int32_t h = 0, fo = 0, pm = pmin, pj = pjin;
+        if (valid) {
+            const int32_t tt = diag + (int32_t)perm(thi, tlo, sel);
+            h = max3i(tt, E, fin);  // E, fin >= 0: includes the 0 floor
+            const int32_t ho = h + O;
+            E = max3i(E + X, ho, 0);
+            fo = max3i(ho, fin + X, 0);
+            if (h > pmin) {
+                pm = h;
+                pj = lane;
+            }
+            if (lane == last && pm > best) {
+                record(traj, t.scan, cap, cnt, t.a + row, pm, pj);
+                best = pm;
+            }
+        }
+        diag = hin;  // H(row, lane - 1): the diagonal of this lane's next row
+        h_out = h;
+        f_out = fo;
+        pm_out = pm;
+        pj_out = pj;
+        sel_out = sel;
+    }
+    if (lane == last) {
+        tcount[t.scan] = cnt;
+        if (cnt > 0) atomicAdd(&ctr->entries, min(cnt, cap));
+        atomicAdd(&ctr->gen_cells, (unsigned long long)rows * (unsigned long long)max(tw, 0));
+    }
+}
+
+// Tasks of more than kWaveRows rows, the first wave_blocks of the order
+// (longest first), run one per wave (the first wave_blocks blocks); every
+// other task runs one per lane (the blocks after them).
+constexpr int kWaveRows = 4096;
+
 template <int WM>
 __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                      const ScanTask *__restrict__ tasks, const int32_t *__restrict__ order,
-                                                     const int32_t *__restrict__ ntasks_p,
+                                                     const int32_t *__restrict__ ntasks_p, int32_t wave_blocks,
                                                      Params P, ScoreTab tab, TrajEntry *__restrict__ traj,
                                                      int32_t *__restrict__ tcount, int32_t cap,
                                                      Counters *__restrict__ ctr) {
+    const int ntasks = *ntasks_p;
+    if ((int)blockIdx.x < wave_blocks) {
+        if ((int)blockIdx.x >= ntasks) return;
+        const ScanTask t = tasks[order ? order[blockIdx.x] : blockIdx.x];
+        if (t.b - t.a > kWaveRows) scan_task_wave<WM>(seq, pairs, t, P, tab, traj, tcount, cap, ctr);
+        return;
+    }
     __shared__ uint32_t shi[8], slo[8];
     if (threadIdx.x < 8) {
         shi[threadIdx.x] = tab.hi[threadIdx.x];
         slo[threadIdx.x] = tab.lo[threadIdx.x];
     }
     __syncthreads();
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= *ntasks_p) return;
+    const int gid = ((int)blockIdx.x - wave_blocks) * blockDim.x + threadIdx.x;
+    if (gid >= ntasks) return;
     const ScanTask t = tasks[order ? order[gid] : gid];
+    if (gid < wave_blocks && t.b - t.a > kWaveRows) return;  // a wave's task
     const PairDesc pd = pairs[t.pair];
     const int8_t *__restrict__ ref = seq + pd.off1;
     const int8_t *__restrict__ qry = seq + pd.off2;
@@ -317,6 +416,119 @@ __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ 
     tcount[t.scan] = cnt;
     if (cnt > 0) atomicAdd(&ctr->entries, min(cnt, cap));
     atomicAdd(&ctr->gen_cells, (unsigned long long)(t.b - t.a) * (unsigned long long)max(tw, 0));
+}
+
+// Packed-f16 k_scan_general: one task per lane, its window split in two
+// column strips, columns [0, C) in the low f16 half and [C, 2C) in the high
+// half, the high half one row behind: at step s the low half computes row s
+// and the high half row s - 1, whose left neighbours (F out of column C - 1,
+// and H(s - 2, C - 1) for the diagonal) the low half left at the previous
+// step.  A row thus costs C packed columns instead of 2C int32 ones, with as
+// many tasks in flight.  The two halves score different reference rows:
+// besides the f16 bound of the window scan this needs scores whose f16
+// patterns have a zero low byte (the small integers of the usual scorings,
+// e.g. 2 = 0x4000, -3 = 0xC200), so that the high bytes of s(c, A..T) of a
+// base c fit one word and one v_perm_b32 over the two rows' words yields
+// both scores.  A row's maximum is complete when its high half is done; the
+// first column of a low-half maximum is taken the step before, when that
+// half alone exceeds the running best.  Tasks the int32 wave path takes (the
+// first wave_blocks of the order, more than kWaveRows rows) run there as in
+// k_scan_general.
+template <int WM>
+__global__ __launch_bounds__(64) void k_scan_general_h(
+    const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const ScanTask *__restrict__ tasks,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ ntasks_p, int32_t wave_blocks, Params P,
+    ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount, int32_t cap,
+    Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h, uint32_t thr_h, uint32_t mis4, uint32_t diff) {
+    constexpr int C = WM / 2;
+    const int ntasks = *ntasks_p;
+    if ((int)blockIdx.x < wave_blocks) {
+        if ((int)blockIdx.x >= ntasks) return;
+        const ScanTask t = tasks[order ? order[blockIdx.x] : blockIdx.x];
+        if (t.b - t.a > kWaveRows) scan_task_wave<WM>(seq, pairs, t, P, tab, traj, tcount, cap, ctr);
+        return;
+    }
+    const int gid = ((int)blockIdx.x - wave_blocks) * 64 + (int)threadIdx.x;
+    if (gid >= ntasks) return;
+    const ScanTask t = tasks[order ? order[gid] : gid];
+    if (gid < wave_blocks && t.b - t.a > kWaveRows) return;  // a wave's task
+    const PairDesc pd = pairs[t.pair];
+    const int8_t *__restrict__ ref = seq + pd.off1 + t.a;
+    const int8_t *__restrict__ qry = seq + pd.off2;
+    const int sp = t.win * P.step;
+    const int tw = min(P.window, pd.len2 - sp);
+    const int rows = t.b - t.a;
+    // selector bytes: 1 = the low half's base class in the row word of
+    // bytes 0-3, 3 = the high half's in bytes 4-7; bytes 0, 2 (and N) zero
+    uint32_t sel[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const int ql = j < tw ? qry[sp + j] : 2;
+        const int qh = C + j < tw ? qry[sp + C + j] : 2;
+        const bool bl = (ql >= 0) & (ql <= 4) & (ql != 2), bh = (qh >= 0) & (qh <= 4) & (qh != 2);
+        const uint32_t kl = bl ? (uint32_t)(ql - (ql > 2)) : 0x0Cu;
+        const uint32_t kh = bh ? (uint32_t)(qh - (qh > 2)) + 4u : 0x0Cu;
+        sel[j] = 0x000C000Cu | (kl << 8) | (kh << 24);
+    }
+    f16x2 H[C], E[C];
+    const f16x2 zero = as_h2(0u);
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        H[j] = zero;
+        E[j] = zero;
+    }
+    const f16x2 O = as_h2(open_h * 0x10001u), X = as_h2(ext_h * 0x10001u);
+    uint32_t best = thr_h, lo_prev = 0, w_prev = 0, d0 = 0, f0 = 0;
+    int32_t cnt = 0, jj_prev = 0;
+    int c1 = rows > 0 ? ref[0] : 2, c2 = rows > 1 ? ref[1] : 2;  // two rows ahead
+    for (int s = 0; s <= rows; ++s) {
+        const int c = c1;
+        c1 = c2;
+        c2 = s + 2 < rows ? ref[s + 2] : 2;
+        const bool base = (c >= 0) & (c <= 4) & (c != 2);
+        // high bytes of s(c, A), s(c, C), s(c, G), s(c, T): mismatch but at c's own class
+        const uint32_t w = base ? mis4 ^ (diff << (8 * (c - (c > 2)))) : 0u;
+        f16x2 diag = as_h2(d0), fc = as_h2(f0), rmax = zero;
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            const f16x2 sc = as_h2(perm(w_prev, w, sel[j]));
+            const f16x2 tt = diag + sc;
+            diag = H[j];
+            const f16x2 hv = hmax3(tt, E[j], fc);  // E, fc >= 0: includes the 0 floor
+            const f16x2 ho = hv + O;
+            E[j] = hmax3(E[j] + X, ho, zero);
+            fc = hmax3(ho, fc + X, zero);
+            H[j] = hv;
+            if (j & 1) rmax = hmax3(rmax, H[j - 1], hv);
+            else if (j == C - 1) rmax = __builtin_elementwise_maximum(rmax, hv);
+        }
+        const uint32_t rb = bits(rmax), lo = rb & 0xffffu, hi = rb >> 16;
+        // row s - 1 is complete: its low-half maximum lo_prev, high-half hi
+        if (s > 0 && max(lo_prev, hi) > best) {
+            int jj = jj_prev;
+            if (hi > lo_prev) {
+                jj = 0;
+#pragma unroll
+                for (int j = C - 1; j >= 0; --j)
+                    if ((bits(H[j]) >> 16) == hi) jj = C + j;
+            }
+            best = max(lo_prev, hi);
+            record(traj, t.scan, cap, cnt, t.a + s - 1, (int32_t)(float)__builtin_bit_cast(_Float16, (uint16_t)best), jj);
+        }
+        if (lo > best && s < rows) {  // first column of row s's low-half maximum
+            jj_prev = 0;
+#pragma unroll
+            for (int j = C - 1; j >= 0; --j)
+                if ((bits(H[j]) & 0xffffu) == lo) jj_prev = j;
+        }
+        lo_prev = lo;
+        d0 = bits(diag) << 16;  // H(s - 1, C - 1) of the low half
+        f0 = bits(fc) << 16;    // F out of column C - 1, row s
+        w_prev = w;
+    }
+    tcount[t.scan] = cnt;
+    if (cnt > 0) atomicAdd(&ctr->entries, min(cnt, cap));
+    atomicAdd(&ctr->gen_cells, (unsigned long long)rows * (unsigned long long)max(tw, 0));
 }
 
 // Reverse SW of getSeeds (:152-159): reverse complements of seq1[e1-r1+1..e1]
@@ -521,13 +733,27 @@ hipError_t launch_scan_general(hipStream_t st, int wm, const int8_t *seq, const 
                                int32_t ntasks_max, const Params &P, const ScoreTab &tab, TrajEntry *traj,
                                int32_t *tcount, int32_t cap, Counters *ctr) {
     if (ntasks_max <= 0) return hipSuccess;
-    dim3 g((ntasks_max + 63) / 64), b(64);
+    // the long tasks sort first: up to 2048 of them get a wave each
+    const int32_t wave_blocks = std::min<int32_t>(ntasks_max, 2048);
+    uint32_t mis4, diff;
+    if (scan_general_packed(P, mis4, diff)) {
+        const uint32_t oh = f16_bits_of_int(P.open), xh = f16_bits_of_int(P.ext), thr = f16_bits_of_int(P.min_score - 1);
+        dim3 g(wave_blocks + (ntasks_max + 63) / 64), b(64);
+        if (wm <= 38)
+            hipLaunchKernelGGL(k_scan_general_h<38>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, P,
+                               tab, traj, tcount, cap, ctr, oh, xh, thr, mis4, diff);
+        else
+            hipLaunchKernelGGL(k_scan_general_h<64>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, P,
+                               tab, traj, tcount, cap, ctr, oh, xh, thr, mis4, diff);
+        return hipGetLastError();
+    }
+    dim3 g(wave_blocks + (ntasks_max + 63) / 64), b(64);
     if (wm <= 38)
-        hipLaunchKernelGGL(k_scan_general<38>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, P, tab, traj, tcount,
-                           cap, ctr);
+        hipLaunchKernelGGL(k_scan_general<38>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, P, tab,
+                           traj, tcount, cap, ctr);
     else
-        hipLaunchKernelGGL(k_scan_general<64>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, P, tab, traj, tcount,
-                           cap, ctr);
+        hipLaunchKernelGGL(k_scan_general<64>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, P, tab,
+                           traj, tcount, cap, ctr);
     return hipGetLastError();
 }
 
