@@ -695,7 +695,8 @@ def main():
     achieved = l0 * VALU_OPS_PER_CELL / (avg_ms * 1e-3) / 1e12
     peak = dev["cu"] * 4 * 16 * 2 * dev["clock_mhz"] * 1e6 / 1e12  # packed int16 VALU lane-ops
     kernels = {}
-    for k in ("scan_uniform", "resolve", "scan_general", "xdrop_pass1", "xdrop_pass2", "gather"):
+    for k in ("scan_uniform", "resolve", "scan_general", "xdrop_pass1", "xdrop_pass1_wave", "xdrop_pass2",
+              "xdrop_pass2_wave", "gather"):
         kms, kn = ctx.kernel_time(k)
         if kn:
             kernels[k] = {"ms_per_step": round(kms / args.steps, 3), "launches_per_step": kn / args.steps}

@@ -80,7 +80,9 @@ struct HBuf {
     }
 };
 
-// Runs fn(i) for i in [0, n) on up to `threads` host threads (dynamic schedule).
+// Runs fn(i) for i in [0, n) on up to `threads` host threads (dynamic
+// schedule over chunks of consecutive indices, about 64 per thread, so that
+// cheap bodies do not serialise on the shared counter).
 template <class F>
 void parallel_for(int64_t n, F &&fn, int threads = 0);
 
@@ -97,9 +99,13 @@ void parallel_for(int64_t n, F &&fn, int threads) {
         for (int64_t i = 0; i < n; ++i) fn(i);
         return;
     }
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n / ((int64_t)threads * 64), 4096));
     std::atomic<int64_t> next(0);
     auto work = [&]() {
-        for (int64_t i = next.fetch_add(1); i < n; i = next.fetch_add(1)) fn(i);
+        for (int64_t a = next.fetch_add(chunk); a < n; a = next.fetch_add(chunk)) {
+            const int64_t b = std::min(n, a + chunk);
+            for (int64_t i = a; i < b; ++i) fn(i);
+        }
     };
     std::vector<std::thread> pool;
     for (int t = 1; t < threads; ++t) pool.emplace_back(work);
@@ -131,9 +137,11 @@ struct dcns_ctx {
     dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound;
     dcns::DBuf sg_tasks, sg_out, sg_scr, sg_tb, sg_cig, sg_strips, sg_sync;  // two-piece semi-global tasks
     dcns::DBuf sg_wcat, sg_wcat_off, sg_wtab;  // weighted DP: per-position profile indices and the profiles
-    dcns::DBuf xw_scr, xw_off, xw_wmax;                  // x-drop tasks with bands past 161 cells
+    dcns::DBuf xw_scr, xw_off, xw_wmax;                  // x-drop tasks with bands past the wave kernel's
     dcns::DBuf sort_scr, task_order;                     // scan tasks ordered by row count
     dcns::DBuf xorder;                                   // x-drop pass-2 order
+    dcns::DBuf xlist;                                    // x-drop tasks handed from the lane to the wave kernel
+    dcns::DBuf cig_scan, cig_packed;                     // cigar packing: counts / offsets / scan storage, runs
     dcns::DBuf rnd_thr, rnd_len1, rnd_len2, rnd_seed, rnd_out;  // random fragment pairs generated on the device
     dcns::DBuf mk_seq, mk_off, mk_keys, mk_desc, mk_gbytes, mk_ghash, mk_gtab;  // maskGenome: genome, k-mer set
     dcns::DBuf mk_mask, mk_ranges, mk_boff, mk_body;                          // maskGenome: mask, CDS, FASTA bodies

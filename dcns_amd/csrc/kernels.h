@@ -46,15 +46,36 @@ hipError_t launch_gather_roots(hipStream_t st, const Node *nodes, const int32_t 
                                int32_t n_max, Node *out);
 
 // xdrop.hip
-// Pass 1: forward score-only extension + reverse score-only extension.
-hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
-                              int32_t n, const Params &P, XOut *out, int32_t *pending);
-// Pass 2: reverse extension with traceback into a band-compressed store, then the cigar.
-hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+// Bands of up to kXdropLaneBand cells run one task per lane, up to
+// kXdropWaveBand one task per wave, wider ones with the band in scratch.
+constexpr int kXdropLaneBand = 21;
+constexpr int kXdropWaveBand = 192;
+// Pass 1: forward score-only extension + reverse score-only extension, one
+// lane per task; tasks whose band outgrows the lane's are counted in
+// *pending and listed in list[0, *pending).
+hipError_t launch_xdrop_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                              int32_t n, const Params &P, XOut *out, int32_t *pending, int32_t *list);
+// Pass 1 one wave per task for list[0, n) (every task when list is null);
+// tasks too wide for the wave are counted in *pending.
+hipError_t launch_xdrop_wave_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                                   const int32_t *list, int32_t n, const Params &P, XOut *out, int32_t *pending);
+// Pass 2: reverse extension with traceback into a band-compressed store, then
+// the cigar; the lane kernel takes order[0, n)'s tasks of the lane band, the
+// wave kernel one task of the wave band per order entry.
+hipError_t launch_xdrop_pass2(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
                               const int32_t *order, int32_t n, const Params &P, XOut *out, const int64_t *tb_off,
                               uint8_t *tb, uint32_t *cigar);
+hipError_t launch_xdrop_wave_pass2(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                                   const int32_t *order, int32_t n, const Params &P, XOut *out,
+                                   const int64_t *tb_off, uint8_t *tb, uint32_t *cigar);
 
-// Wide bands (2w+1 > 161): band rows in per-task global scratch (scr_off[t] < 0: task not handled).
+// Packs the cigar runs of out[0, n) (each task's arena holds o.ncig of them
+// at o.cig_off) into packed[], task after task, and points cig_off there.
+size_t pack_cigars_scratch_bytes(int32_t n);
+hipError_t launch_pack_cigars(hipStream_t st, XOut *out, int32_t n, void *scratch, size_t scratch_bytes,
+                              const uint32_t *cig, uint32_t *packed);
+
+// Wide bands (2w+1 > kXdropWaveBand): band rows in per-task global scratch (scr_off[t] < 0: task not handled).
 hipError_t launch_xdrop_wide_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
                                    int32_t n, const Params &P, XOut *out, int32_t *scr, const int64_t *scr_off,
                                    const int32_t *wmax);

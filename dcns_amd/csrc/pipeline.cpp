@@ -471,21 +471,35 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
     c->xout.ensure((size_t)n * sizeof(XOut));
     c->ctr.ensure(sizeof(Counters) + 64);
     hip_check(hipMemsetAsync(c->xout.p, 0, (size_t)n * sizeof(XOut), c->stream), "hipMemsetAsync");
-    static const int bands[] = {21, 41, 81, 161};
-    int first = 0;
-    while (first < 4 && bands[first] < 2 * P.w + 1) ++first;
     int32_t *pending_dev = &c->ctr.as<Counters>()->pending;
-    int32_t pending = 1;
-    // pass 1 with the narrowest register band; tasks whose band outgrows it
-    // count themselves in `pending` and are resumed by the next instantiation
-    for (int bi = first; bi < 4 && pending; ++bi) {
-        hip_check(hipMemsetAsync(pending_dev, 0, sizeof(int32_t), c->stream), "hipMemsetAsync");
-        c->launch("xdrop_pass1", [&] {
-            return launch_xdrop_pass1(c->stream, bands[bi], c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
-                                      c->xtasks.as<XTask>(), n, P, c->xout.as<XOut>(), pending_dev);
-        });
+    int32_t pending = 0;
+    auto read_pending = [&]() {
         hip_check(hipMemcpyAsync(&pending, pending_dev, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream), "pending");
         c->sync();
+    };
+    // pass 1: one lane per task while the band fits the lane kernel; the
+    // tasks that outgrow it (listed in xlist) and, when the initial band is
+    // already wider, every task go on one wave each
+    const bool lane_first = 2 * P.w + 1 <= kXdropLaneBand;
+    int32_t nwave = n;
+    if (lane_first) {
+        c->xlist.ensure((size_t)n * sizeof(int32_t));
+        hip_check(hipMemsetAsync(pending_dev, 0, sizeof(int32_t), c->stream), "hipMemsetAsync");
+        c->launch("xdrop_pass1", [&] {
+            return launch_xdrop_pass1(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->xtasks.as<XTask>(),
+                                      n, P, c->xout.as<XOut>(), pending_dev, c->xlist.as<int32_t>());
+        });
+        read_pending();
+        nwave = pending;
+    }
+    if (nwave > 0) {
+        hip_check(hipMemsetAsync(pending_dev, 0, sizeof(int32_t), c->stream), "hipMemsetAsync");
+        c->launch("xdrop_pass1_wave", [&] {
+            return launch_xdrop_wave_pass1(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                           c->xtasks.as<XTask>(), lane_first ? c->xlist.as<int32_t>() : nullptr,
+                                           nwave, P, c->xout.as<XOut>(), pending_dev);
+        });
+        read_pending();
     }
     c->h_xout.ensure((size_t)n * sizeof(XOut));
     XOut *xo = c->h_xout.as<XOut>();
@@ -544,9 +558,11 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
         });
         return;
     }
+    // the band class of a task's pass 2: 0 lane kernel, 1 wave kernel, 2 scratch-band kernel
+    auto band_class = [](int32_t B) { return B <= kXdropLaneBand ? 0 : (B <= kXdropWaveBand ? 1 : 2); };
     std::vector<int64_t> tboff((size_t)n, 0);
     int64_t tbtotal = 0, cigtotal = 0;
-    bool any_band[4] = {false, false, false, false};
+    int32_t in_class[3] = {0, 0, 0};
     for (int32_t i = 0; i < n; ++i) {
         XOut &o = xo[i];
         cells += o.cells;
@@ -556,32 +572,24 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
         tboff[(size_t)i] = tbtotal;
         tbtotal += (int64_t)(o.rrows + 1) * BP;
         cigtotal += std::max(0, o.re1) + std::max(0, o.re2) + 2;
-        for (int k = 0; k < 4; ++k)
-            if (B <= bands[k]) {
-                any_band[k] = true;
-                break;
-            }
+        ++in_class[band_class(B)];
     }
     hip_check(hipMemcpyAsync(c->xout.p, xo, (size_t)n * sizeof(XOut), hipMemcpyHostToDevice, c->stream), "xout up");
     upload(c, c->tb_off, tboff);
-    // pass 2 order: by band class (each instantiation's tasks contiguous),
-    // then by traced cells, largest first, on a quarter-octave scale (a
-    // counting sort; the order only balances the launches)
+    // pass 2 order: by band class (each kernel's tasks contiguous, the lane
+    // class first, then the wave class), then by traced cells, largest
+    // first, on a quarter-octave scale (a counting sort; the order only
+    // balances the launches)
     {
         constexpr int NB = 4 * 48;
         std::vector<int32_t> bucket((size_t)n);
-        std::vector<int64_t> start(5 * NB + 2, 0);
+        std::vector<int64_t> start(3 * NB + 2, 0);
         for (int32_t i = 0; i < n; ++i) {
             const XOut &o = xo[i];
-            int32_t b = 5 * NB;  // no traceback: last
+            int32_t b = 3 * NB;  // no traceback: last
             if (o.fe1 >= 0 && o.fe2 >= 0) {
                 const int32_t B = 2 * o.rw + 1;
-                int cls = 4;
-                for (int k = 0; k < 4; ++k)
-                    if (B <= bands[k]) {
-                        cls = k;
-                        break;
-                    }
+                const int cls = band_class(B);
                 const uint64_t cl = (uint64_t)(o.rrows + 1) * (uint64_t)B;
                 const int lg = 63 - __builtin_clzll(cl | 1);
                 const int q = lg >= 2 ? (int)((cl >> (lg - 2)) & 3) : 0;
@@ -605,21 +613,39 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
                                            c->xw_off.as<int64_t>());
         });
     }
-    for (int k = 0; k < 4; ++k) {
-        if (!any_band[k]) continue;
+    if (in_class[0] > 0)
         c->launch("xdrop_pass2", [&] {
-            return launch_xdrop_pass2(c->stream, bands[k], c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
-                                      c->xtasks.as<XTask>(), c->xorder.as<int32_t>(), n, P, c->xout.as<XOut>(),
+            return launch_xdrop_pass2(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->xtasks.as<XTask>(),
+                                      c->xorder.as<int32_t>(), in_class[0], P, c->xout.as<XOut>(),
                                       c->tb_off.as<int64_t>(), c->tb.as<uint8_t>(), c->cigar.as<uint32_t>());
         });
-    }
-    c->h_cigar.ensure((size_t)std::max<int64_t>(cigtotal, 16) * sizeof(uint32_t));
-    const uint32_t *cig = c->h_cigar.as<uint32_t>();
+    if (in_class[1] > 0)
+        c->launch("xdrop_pass2_wave", [&] {
+            return launch_xdrop_wave_pass2(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                           c->xtasks.as<XTask>(), c->xorder.as<int32_t>() + in_class[0], in_class[1],
+                                           P, c->xout.as<XOut>(), c->tb_off.as<int64_t>(), c->tb.as<uint8_t>(),
+                                           c->cigar.as<uint32_t>());
+        });
+    // the arena holds re1 + re2 + 2 entries per task, the walk writes a few
+    // runs: pack them before the download
+    const size_t scan_bytes = pack_cigars_scratch_bytes(n);
+    c->cig_scan.ensure(scan_bytes);
+    c->cig_packed.ensure((size_t)std::max<int64_t>(cigtotal, 16) * sizeof(uint32_t));
+    hip_check(launch_pack_cigars(c->stream, c->xout.as<XOut>(), n, c->cig_scan.p, scan_bytes,
+                                 c->cigar.as<uint32_t>(), c->cig_packed.as<uint32_t>()),
+              "pack cigars");
     hip_check(hipMemcpyAsync(xo, c->xout.p, (size_t)n * sizeof(XOut), hipMemcpyDeviceToHost, c->stream), "xout");
-    if (cigtotal)
-        hip_check(hipMemcpyAsync(c->h_cigar.p, c->cigar.p, (size_t)cigtotal * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                 c->stream), "cigar");
     c->sync();
+    int64_t packed = 0;
+    for (int32_t i = 0; i < n; ++i)
+        if (xo[i].fe1 >= 0 && xo[i].fe2 >= 0) packed += xo[i].ncig;
+    c->h_cigar.ensure((size_t)std::max<int64_t>(packed, 16) * sizeof(uint32_t));
+    const uint32_t *cig = c->h_cigar.as<uint32_t>();
+    if (packed) {
+        hip_check(hipMemcpyAsync(c->h_cigar.p, c->cig_packed.p, (size_t)packed * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->stream), "cigar");
+        c->sync();
+    }
     parallel_for(n, [&](int64_t i) {
         const XOut &o = xo[i];
         const XTask &t = tasks[(size_t)i];

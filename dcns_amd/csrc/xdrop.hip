@@ -6,20 +6,25 @@
 // traceback (:1318-1451).  A row's band is j in [max(1,i-w), min(i+w,L2)];
 // a pass restarts with 2w when the row maximum sits on a band edge.
 //
-// One lane per seed.  The band is held band-relative in VGPRs: in band
-// coordinates k = j - (i - w) the diagonal predecessor of (i, k) is (i-1, k)
-// and the vertical one is (i-1, k+1), so rows advance without moving data.
-// Query bases ride in a byte window packed four per VGPR that shifts by one
-// byte per row.  Two passes:
+// One lane per seed while the band is at most 21 cells.  The band is held
+// band-relative in VGPRs: in band coordinates k = j - (i - w) the diagonal
+// predecessor of (i, k) is (i-1, k) and the vertical one is (i-1, k+1), so
+// rows advance without moving data.  Query bases ride in a byte window packed
+// four per VGPR that shifts by one byte per row.  Two passes:
 //   pass 1: forward + reverse, score only; reports the final band of each
 //           direction and the rows the final reverse pass evaluated;
 //   pass 2: the final reverse pass again, storing one traceback byte per
 //           band cell (rows x band, default 255 like Matrix::reset), then the
 //           traceback state machine writes the cigar.
-// A task whose band outgrows the kernel's register band is handed to the
-// next wider instantiation by the host, and past 161 cells to a kernel that
-// keeps the band in global scratch.
+// A task whose band outgrows 21 cells (a row maximum on the band edge, so
+// the pass restarts at twice the width) is listed for the wave kernels: one
+// wave per task, the band's cells across the lanes, 192 cells at most, which
+// keeps a long extension at a wide band from holding a launch for the
+// length of one lane's walk across every cell.  Past 192 cells a kernel keeps
+// the band in global scratch.
 #include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
 
 #include "dp_common.h"
 #include "kernels.h"
@@ -210,6 +215,190 @@ __device__ PassOut xrun(const Dir &D, int32_t &w, const Params &P, uint8_t *tb, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave passes: one task per wave, for the bands past the lane kernel's 21
+// cells.  A row's band slots lie across the wave, S consecutive slots per
+// lane (slot k in lane k / S).  The diagonal predecessor of slot k is slot k
+// of the previous row (same register); the vertical one, slot k + 1, comes
+// from the next lane by DPP wave_shl:1 for a lane's last slot.  The
+// horizontal gap is a max-plus recurrence along the row,
+//   e(k + 1) = max(e(k) + D, m'(k) + open),  D = max(ext, open),
+// where m'(k) is the cell's value before the horizontal term (a cell's
+// horizontal output max(e + ext, max(m', e) + open) equals that, as
+// max(e + ext, e + open) = e + D), so the wave evaluates it as a prefix scan:
+// each lane folds its slots, DPP row_shr 1/2/4/8 and row_bcast 15/31 combine
+// the lanes, and each lane re-runs its slots from its incoming value.  The
+// row's band starts with e = kOutOfBand at its first valid slot, entered as
+// a term at the slot before it.  Values, traceback bits, the row maximum
+// (first column on ties) and every stop rule are those of xpass.
+// ---------------------------------------------------------------------------
+constexpr int kWaveSlots = 3;  // slots per lane of the widest wave band
+static_assert(64 * kWaveSlots == kXdropWaveBand, "wave band");
+constexpr int32_t kScanNeg = -(1 << 30);  // below every value of a row, decays included
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int32_t dpp(int32_t old, int32_t src) {
+    return __builtin_amdgcn_update_dpp(old, src, CTRL, ROWS, 0xf, false);
+}
+
+// y(l) = max over l' <= l of x(l') + (l - l') * step, across the wave
+__device__ __forceinline__ int32_t wave_maxplus_scan(int32_t x, int32_t step, int lane) {
+    int32_t y = x;
+    y = max(y, dpp<0x111>(kScanNeg, y) + step);      // row_shr:1
+    y = max(y, dpp<0x112>(kScanNeg, y) + 2 * step);  // row_shr:2
+    y = max(y, dpp<0x114>(kScanNeg, y) + 4 * step);  // row_shr:4
+    y = max(y, dpp<0x118>(kScanNeg, y) + 8 * step);  // row_shr:8
+    y = max(y, dpp<0x142, 0xa>(kScanNeg, y) + ((lane & 15) + 1) * step);  // row_bcast:15 into rows 1, 3
+    y = max(y, dpp<0x143, 0xc>(kScanNeg, y) + (lane - 31) * step);        // row_bcast:31 into rows 2, 3
+    return y;
+}
+
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+    v = max(v, dpp<0x111>(INT32_MIN, v));
+    v = max(v, dpp<0x112>(INT32_MIN, v));
+    v = max(v, dpp<0x114>(INT32_MIN, v));
+    v = max(v, dpp<0x118>(INT32_MIN, v));
+    v = max(v, dpp<0x142, 0xa>(INT32_MIN, v));
+    v = max(v, dpp<0x143, 0xc>(INT32_MIN, v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+template <int S, bool TB>
+__device__ PassOut wave_xpass(const Dir &D, int32_t w, const Params &P, uint8_t *__restrict__ tb, int32_t BP) {
+    const int lane = threadIdx.x;
+    const int32_t O = P.open, X = P.ext, DK = max(P.ext, P.open);
+    int32_t Mp[S], Fp[S], qk[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+        Mp[u] = kOutOfBand;
+        Fp[u] = kOutOfBand;
+        const int32_t j = 1 - w + lane * S + u;  // column of slot k in row 1
+        qk[u] = (j >= 1 && j <= D.L2) ? D.c2(j - 1) : 2;
+    }
+    // 64 rows at a time, one per lane: the row's reference base, and the
+    // query base entering the band's last slot for the next row (column
+    // i - w + 64 S); the next 64 in flight while these are used
+    auto chunk = [&](int32_t i0, int32_t &cb, int32_t &qb) {
+        const int32_t ri = i0 + lane;
+        cb = ri <= D.L1 ? D.c1(ri - 1) : 2;
+        const int32_t jn = ri - w + 64 * S;
+        qb = (jn >= 1 && jn <= D.L2) ? D.c2(jn - 1) : 2;
+    };
+    int32_t ccur, qcur, cnext, qnext;
+    chunk(1, ccur, qcur);
+    chunk(65, cnext, qnext);
+    PassOut r{0, 0, 0, 0, 0, false};
+    for (int32_t i = 1; i <= D.L1; ++i) {
+        const int t = (i - 1) & 63;
+        if (t == 0 && i > 1) {
+            ccur = cnext;
+            qcur = qnext;
+            chunk(i + 64, cnext, qnext);
+        }
+        const int c = __builtin_amdgcn_readlane(ccur, t);
+        const int32_t st = max(i - w, 1), en = min(i + w, D.L2), lo = i - w;
+        const int32_t kst = st - lo;  // first valid slot
+        if (st <= en) r.cells += en - st + 1;
+        const int32_t fnext = dpp<0x130>(kOutOfBand, Fp[0]);  // wave_shl:1: the next lane's first slot
+        int32_t A[S], d0[S], fv[S];
+        int32_t L = kScanNeg;  // the lane's slots folded from an empty input
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int32_t k = lane * S + u, j = lo + k;
+            const bool valid = (j >= st) && (j <= en);
+            const int32_t sc = score(c, qk[u], P.match, P.mismatch);
+            const int32_t diag = (j == 1) ? (i == 1 ? 0 : kOutOfBand) : Mp[u];
+            const int32_t fr = (u + 1 < S) ? Fp[u + 1 < S ? u + 1 : u] : fnext;
+            fv[u] = (j != en) ? fr : kOutOfBand;
+            int32_t m = sc + diag;
+            int32_t d = 0;
+            if (!(m > fv[u])) {
+                m = fv[u];
+                d = 1;
+            }
+            A[u] = m;
+            d0[u] = d;
+            L = max(L + DK, valid ? m + O : (k == kst - 1 ? kOutOfBand : kScanNeg));
+        }
+        const int32_t y = wave_maxplus_scan(L, S * DK, lane);
+        int32_t e = max(dpp<0x138>(kScanNeg, y), (kst == 0 ? kOutOfBand : kScanNeg) + lane * S * DK);  // wave_shr:1
+        int32_t lm = 0, lj = -1;
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int32_t k = lane * S + u, j = lo + k;
+            const bool valid = (j >= st) && (j <= en);
+            int32_t m = A[u];
+            uint32_t d = (uint32_t)d0[u];
+            if (!(m > e)) {
+                m = e;
+                d = 2;
+            }
+            const int32_t h = m + O;
+            int32_t f = fv[u] + X;
+            if (f >= h) d |= 8;
+            else f = h;
+            if (e + X >= h) d |= 16;
+            e = max(e + DK, valid ? A[u] + O : (k == kst - 1 ? kOutOfBand : kScanNeg));
+            if (valid) {
+                if (m > lm) {
+                    lm = m;
+                    lj = j;
+                }
+                Mp[u] = m;
+                Fp[u] = f;
+            }
+            if (TB && k < BP) tb[(int64_t)i * BP + k] = valid ? (uint8_t)d : (uint8_t)255;
+        }
+        const int32_t rowMax = wave_max(lm);
+        int32_t rowJ = -1;
+        if (rowMax > 0) {
+            const uint64_t bal = __ballot(lm == rowMax);
+            rowJ = __builtin_amdgcn_readlane(lj, (int)__builtin_ctzll(bal));
+        }
+        r.rows = i;
+        if (rowJ != D.L2 && rowJ != 1 && (rowJ == st || rowJ == en)) {
+            r.restart = true;
+            return r;
+        }
+        if (rowMax > r.max) {
+            r.max = rowMax;
+            r.e1 = i;
+            r.e2 = rowJ;
+        }
+        if (r.max - rowMax > P.xdrop) break;
+        if (rowJ <= 0) break;
+        // row i + 1: slot k holds column j + 1, the column of slot k + 1 now
+        const int32_t q0 = dpp<0x130>(__builtin_amdgcn_readlane(qcur, t), qk[0]);  // wave_shl:1
+#pragma unroll
+        for (int u = 0; u + 1 < S; ++u) qk[u] = qk[u + 1];
+        qk[S - 1] = q0;
+    }
+    return r;
+}
+
+// The narrowest slot count that holds the band (fewer slots: a shorter row)
+template <bool TB>
+__device__ PassOut wave_xpass_fit(const Dir &D, int32_t w, const Params &P, uint8_t *__restrict__ tb, int32_t BP) {
+    const int32_t B = 2 * w + 1;
+    if (B <= 64) return wave_xpass<1, TB>(D, w, P, tb, BP);
+    if (B <= 128) return wave_xpass<2, TB>(D, w, P, tb, BP);
+    return wave_xpass<kWaveSlots, TB>(D, w, P, tb, BP);
+}
+
+__device__ PassOut wave_xrun(const Dir &D, int32_t &w, const Params &P, int64_t &cells, bool &too_wide) {
+    too_wide = false;
+    while (true) {
+        if (2 * w + 1 > kXdropWaveBand) {
+            too_wide = true;
+            return PassOut{0, 0, 0, 0, 0, false};
+        }
+        PassOut r = wave_xpass_fit<false>(D, w, P, nullptr, 0);
+        cells += r.cells;
+        if (!r.restart) return r;
+        w *= 2;
+    }
+}
+
 __device__ Dir forward_dir(const int8_t *seq, const PairDesc &pd, const XTask &t) {
     Dir D;
     D.s1 = seq + pd.off1 + t.start1;
@@ -235,7 +424,8 @@ __device__ Dir reverse_dir(const int8_t *seq, const PairDesc &pd, const XTask &t
 template <int BMAX>
 __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                     const XTask *__restrict__ tasks, int32_t n, Params P,
-                                                    XOut *__restrict__ out, int32_t *__restrict__ pending) {
+                                                    XOut *__restrict__ out, int32_t *__restrict__ pending,
+                                                    int32_t *__restrict__ list) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
     const XTask t = tasks[gid];
@@ -251,7 +441,7 @@ __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ s
             o.status = 1;
             o.fw = w;
             out[gid] = o;
-            atomicAdd(pending, 1);
+            list[atomicAdd(pending, 1)] = gid;
             return;
         }
         o.fmax = f.max;
@@ -275,7 +465,7 @@ __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ s
         o.status = 2;
         o.rw = w;
         out[gid] = o;
-        atomicAdd(pending, 1);
+        list[atomicAdd(pending, 1)] = gid;
         return;
     }
     o.status = 0;
@@ -285,6 +475,71 @@ __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ s
     o.rw = w;
     o.rrows = r.rows;
     out[gid] = o;
+}
+
+// Pass 1, one wave per task, for the tasks of list[0, n) (all tasks when
+// list is null): those the lane kernel handed back (status 1: forward pass
+// pending at band fw, 2: reverse pending at rw), or every task when the
+// initial band is wider than the lane kernel's.  Tasks whose band outgrows
+// the wave's are counted in `pending` for the scratch-band kernel.
+__global__ __launch_bounds__(64) void k_xdrop_wave_pass1(const int8_t *__restrict__ seq,
+                                                         const PairDesc *__restrict__ pairs,
+                                                         const XTask *__restrict__ tasks,
+                                                         const int32_t *__restrict__ list, Params P,
+                                                         XOut *__restrict__ out, int32_t *__restrict__ pending) {
+    const int gid = list ? list[blockIdx.x] : (int)blockIdx.x;
+    const bool lead = threadIdx.x == 0;
+    const XTask t = tasks[gid];
+    XOut o = out[gid];
+    if (o.status == 0 && o.fw != 0) return;  // already finished
+    const PairDesc pd = pairs[t.pair];
+    bool wide;
+    if (o.status != 2) {
+        int32_t w = o.fw > 0 ? o.fw : P.w;
+        const Dir F = forward_dir(seq, pd, t);
+        PassOut f = wave_xrun(F, w, P, o.cells, wide);
+        if (wide) {
+            o.status = 1;
+            o.fw = w;
+            if (lead) {
+                out[gid] = o;
+                atomicAdd(pending, 1);
+            }
+            return;
+        }
+        o.fmax = f.max;
+        o.fe1 = f.e1 - 1;
+        o.fe2 = f.e2 - 1;
+        o.fw = w;
+        o.rw = P.w;
+        if (o.fe1 < 0 || o.fe2 < 0) {
+            o.status = 0;
+            o.rmax = 0;
+            o.re1 = o.re2 = -1;
+            o.rrows = 0;
+            if (lead) out[gid] = o;
+            return;
+        }
+    }
+    int32_t w = o.rw > 0 ? o.rw : P.w;
+    const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
+    PassOut r = wave_xrun(R, w, P, o.cells, wide);
+    if (wide) {
+        o.status = 2;
+        o.rw = w;
+        if (lead) {
+            out[gid] = o;
+            atomicAdd(pending, 1);
+        }
+        return;
+    }
+    o.status = 0;
+    o.rmax = r.max;
+    o.re1 = r.e1 - 1;
+    o.re2 = r.e2 - 1;
+    o.rw = w;
+    o.rrows = r.rows;
+    if (lead) out[gid] = o;
 }
 
 // Pass 1 for the tasks the register kernels handed back (status 1: forward
@@ -389,8 +644,7 @@ __global__ __launch_bounds__(64) void k_xdrop_pass2(const int8_t *__restrict__ s
     XOut o = out[gid];
     if (o.fe1 < 0 || o.fe2 < 0) return;
     const int32_t w = o.rw;
-    // each task is traced by the narrowest instantiation that holds its band
-    if (2 * w + 1 > BMAX || (BMAX > 21 && 2 * w + 1 <= (BMAX + 1) / 2)) return;
+    if (2 * w + 1 > BMAX) return;  // a wider kernel's task
     const PairDesc pd = pairs[t.pair];
     const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
     const int32_t B = 2 * w + 1;
@@ -402,7 +656,35 @@ __global__ __launch_bounds__(64) void k_xdrop_pass2(const int8_t *__restrict__ s
     out[gid] = o;
 }
 
-// Pass 2 for the tasks whose final reverse band is wider than 161 cells.
+// Pass 2, one wave per task, for the tasks order[0, n) whose final reverse
+// band is wider than the lane kernel's and fits the wave; the first lane
+// walks the traceback once the wave's bytes are stored.
+__global__ __launch_bounds__(64) void k_xdrop_wave_pass2(const int8_t *__restrict__ seq,
+                                                         const PairDesc *__restrict__ pairs,
+                                                         const XTask *__restrict__ tasks,
+                                                         const int32_t *__restrict__ order, Params P,
+                                                         XOut *__restrict__ out, const int64_t *__restrict__ tb_off,
+                                                         uint8_t *__restrict__ tb, uint32_t *__restrict__ cigar) {
+    const int gid = order[blockIdx.x];
+    const XTask t = tasks[gid];
+    XOut o = out[gid];
+    if (o.fe1 < 0 || o.fe2 < 0) return;
+    const int32_t w = o.rw;
+    if (2 * w + 1 <= kXdropLaneBand || 2 * w + 1 > kXdropWaveBand) return;  // another kernel's class
+    const PairDesc pd = pairs[t.pair];
+    const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
+    const int32_t B = 2 * w + 1;
+    const int32_t BP = (B + 3) & ~3;
+    uint8_t *T = tb + tb_off[gid];
+    wave_xpass_fit<true>(R, w, P, T, BP);
+    __syncthreads();  // the wave's traceback bytes are stored before the walk reads them
+    if (threadIdx.x == 0) {
+        trace_cigar(o, w, T, BP, cigar);
+        out[gid] = o;
+    }
+}
+
+// Pass 2 for the tasks whose final reverse band is wider than the wave's.
 __global__ void k_xdrop_wide_pass2(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                    const XTask *__restrict__ tasks, int32_t n, Params P, XOut *__restrict__ out,
                                    const int64_t *__restrict__ tb_off, uint8_t *__restrict__ tb,
@@ -414,7 +696,7 @@ __global__ void k_xdrop_wide_pass2(const int8_t *__restrict__ seq, const PairDes
     XOut o = out[gid];
     if (o.fe1 < 0 || o.fe2 < 0) return;
     const int32_t w = o.rw;
-    if (2 * w + 1 <= 161 || scr_off[gid] < 0) return;
+    if (2 * w + 1 <= kXdropWaveBand || scr_off[gid] < 0) return;
     const PairDesc pd = pairs[t.pair];
     const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
     const int32_t B = 2 * w + 1;
@@ -427,19 +709,64 @@ __global__ void k_xdrop_wide_pass2(const int8_t *__restrict__ seq, const PairDes
     out[gid] = o;
 }
 
+// Cigar runs of each traced task (0 for the others), then the runs moved to
+// their packed offsets; cig_off is rewritten to the packed offset.
+__global__ void k_cig_counts(const XOut *__restrict__ out, int32_t n, int64_t *__restrict__ cnt) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const XOut &o = out[i];
+    cnt[i] = (o.fe1 >= 0 && o.fe2 >= 0) ? o.ncig : 0;
+}
+
+__global__ void k_cig_pack(XOut *__restrict__ out, int32_t n, const int64_t *__restrict__ off,
+                           const uint32_t *__restrict__ cig, uint32_t *__restrict__ packed) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    XOut &o = out[i];
+    if (o.fe1 < 0 || o.fe2 < 0) return;
+    const uint32_t *src = cig + o.cig_off;
+    uint32_t *dst = packed + off[i];
+    for (int32_t q = 0; q < o.ncig; ++q) dst[q] = src[q];
+    o.cig_off = off[i];
+}
+
 }  // namespace
 
-hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
-                              int32_t n, const Params &P, XOut *out, int32_t *pending) {
+size_t pack_cigars_scratch_bytes(int32_t n) {
+    size_t bytes = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int64_t *)nullptr, (int64_t *)nullptr, std::max(n, 1));
+    return bytes + 2 * (size_t)std::max(n, 1) * sizeof(int64_t) + 256;
+}
+
+hipError_t launch_pack_cigars(hipStream_t st, XOut *out, int32_t n, void *scratch, size_t scratch_bytes,
+                              const uint32_t *cig, uint32_t *packed) {
     if (n <= 0) return hipSuccess;
-    dim3 g((n + 63) / 64), b(64);
-    switch (band) {
-        case 21: hipLaunchKernelGGL(k_xdrop_pass1<21>, g, b, 0, st, seq, pairs, tasks, n, P, out, pending); break;
-        case 41: hipLaunchKernelGGL(k_xdrop_pass1<41>, g, b, 0, st, seq, pairs, tasks, n, P, out, pending); break;
-        case 81: hipLaunchKernelGGL(k_xdrop_pass1<81>, g, b, 0, st, seq, pairs, tasks, n, P, out, pending); break;
-        case 161: hipLaunchKernelGGL(k_xdrop_pass1<161>, g, b, 0, st, seq, pairs, tasks, n, P, out, pending); break;
-        default: return hipErrorInvalidValue;
-    }
+    int64_t *cnt = (int64_t *)scratch;
+    int64_t *off = cnt + n;
+    void *tmp = off + n;
+    size_t tmp_bytes = scratch_bytes - 2 * (size_t)n * sizeof(int64_t);
+    const dim3 g((n + 255) / 256), b(256);
+    hipLaunchKernelGGL(k_cig_counts, g, b, 0, st, out, n, cnt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, off, n, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cig_pack, g, b, 0, st, out, n, off, cig, packed);
+    return hipGetLastError();
+}
+
+hipError_t launch_xdrop_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                              int32_t n, const Params &P, XOut *out, int32_t *pending, int32_t *list) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xdrop_pass1<kXdropLaneBand>, dim3((n + 63) / 64), dim3(64), 0, st, seq, pairs, tasks, n, P,
+                       out, pending, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_xdrop_wave_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                                   const int32_t *list, int32_t n, const Params &P, XOut *out, int32_t *pending) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xdrop_wave_pass1, dim3(n), dim3(64), 0, st, seq, pairs, tasks, list, P, out, pending);
     return hipGetLastError();
 }
 
@@ -461,18 +788,21 @@ hipError_t launch_xdrop_wide_pass2(hipStream_t st, const int8_t *seq, const Pair
     return hipGetLastError();
 }
 
-hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+hipError_t launch_xdrop_pass2(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
                               const int32_t *order, int32_t n, const Params &P, XOut *out, const int64_t *tb_off,
                               uint8_t *tb, uint32_t *cigar) {
     if (n <= 0) return hipSuccess;
-    dim3 g((n + 63) / 64), b(64);
-    switch (band) {
-        case 21: hipLaunchKernelGGL(k_xdrop_pass2<21>, g, b, 0, st, seq, pairs, tasks, order, n, P, out, tb_off, tb, cigar); break;
-        case 41: hipLaunchKernelGGL(k_xdrop_pass2<41>, g, b, 0, st, seq, pairs, tasks, order, n, P, out, tb_off, tb, cigar); break;
-        case 81: hipLaunchKernelGGL(k_xdrop_pass2<81>, g, b, 0, st, seq, pairs, tasks, order, n, P, out, tb_off, tb, cigar); break;
-        case 161: hipLaunchKernelGGL(k_xdrop_pass2<161>, g, b, 0, st, seq, pairs, tasks, order, n, P, out, tb_off, tb, cigar); break;
-        default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(k_xdrop_pass2<kXdropLaneBand>, dim3((n + 63) / 64), dim3(64), 0, st, seq, pairs, tasks, order,
+                       n, P, out, tb_off, tb, cigar);
+    return hipGetLastError();
+}
+
+hipError_t launch_xdrop_wave_pass2(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                                   const int32_t *order, int32_t n, const Params &P, XOut *out,
+                                   const int64_t *tb_off, uint8_t *tb, uint32_t *cigar) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xdrop_wave_pass2, dim3(n), dim3(64), 0, st, seq, pairs, tasks, order, P, out, tb_off, tb,
+                       cigar);
     return hipGetLastError();
 }
 
