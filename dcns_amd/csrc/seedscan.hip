@@ -339,9 +339,9 @@ __device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *_
     }
 }
 
-// Tasks of more than kWaveRows rows, the first wave_blocks of the order
-// (longest first), run one per wave (the first wave_blocks blocks); every
-// other task runs one per lane (the blocks after them).
+// Tasks of more than kWaveRows rows among the first wave_blocks of the
+// order (longest first) run one per wave (the first wave_blocks blocks);
+// every other task runs one per lane (the blocks after them).
 constexpr int kWaveRows = 4096;
 
 template <int WM>
@@ -733,8 +733,10 @@ hipError_t launch_scan_general(hipStream_t st, int wm, const int8_t *seq, const 
                                int32_t ntasks_max, const Params &P, const ScoreTab &tab, TrajEntry *traj,
                                int32_t *tcount, int32_t cap, Counters *ctr) {
     if (ntasks_max <= 0) return hipSuccess;
-    // the long tasks sort first: up to 2048 of them get a wave each
-    const int32_t wave_blocks = std::min<int32_t>(ntasks_max, 2048);
+    // the long tasks sort first: up to 1024 of them get a wave each (a wave
+    // step costs ~11x the SIMD time per cell of the lane path; 1024 measured
+    // best on the 4096-pair batch, with 256-2048 within 3 ms per step)
+    const int32_t wave_blocks = std::min<int32_t>(ntasks_max, 1024);
     uint32_t mis4, diff;
     if (scan_general_packed(P, mis4, diff)) {
         const uint32_t oh = f16_bits_of_int(P.open), xh = f16_bits_of_int(P.ext), thr = f16_bits_of_int(P.min_score - 1);
