@@ -505,7 +505,7 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
     XOut *xo = c->h_xout.as<XOut>();
     bool wide = false;
     if (pending) {
-        // bands past 161 cells: per-task scratch rows in global memory
+        // bands past the wave kernel's 192 cells: per-task scratch rows in global memory
         hip_check(hipMemcpyAsync(xo, c->xout.p, (size_t)n * sizeof(XOut), hipMemcpyDeviceToHost, c->stream), "xout");
         c->sync();
         std::vector<int64_t> off((size_t)n, -1);

@@ -97,7 +97,8 @@ int dcns_last_stats(const dcns_ctx *ctx, dcns_stats *out);
 
 /* Per-kernel timing (flags bit 0): total milliseconds and launch count of the
  * kernel family `name` ("scan_uniform", "scan_general", "resolve",
- * "xdrop_pass1", "xdrop_pass2", "sw_batch") since the last reset. */
+ * "xdrop_pass1", "xdrop_pass1_wave", "xdrop_pass2", "xdrop_pass2_wave",
+ * "sw_batch") since the last reset. */
 int dcns_kernel_time(const dcns_ctx *ctx, const char *name, double *ms, int64_t *launches);
 int dcns_kernel_time_reset(dcns_ctx *ctx);
 
