@@ -84,6 +84,40 @@ def test_xdrop_extend_matches_oracle(gpu, oracle_lib):
         assert g[6] == want[6], k
 
 
+@pytest.mark.parametrize("sc,w", [((2, -3, -1, -3), 10), ((2, -3, -4, -2), 20), ((1, -1, -1, -2), 40),
+                                  ((2, -3, -3, -1), 80)])
+def test_xdrop_extend_wave_bands_vs_oracle(gpu, oracle_lib, sc, w):
+    """Initial bands of 21-161 cells (past 21 every pass runs one wave per
+    seed) and cheap gaps, open above extend as well as below, so that bands
+    restart and double inside the wave kernels and the traceback pass runs
+    there too."""
+    pairs = _small_pairs(77, 4, 800, 3000, (6, 14))
+    rng = np.random.default_rng(w)
+    pos, starts = [], []
+    for i, (a, b) in enumerate(pairs):
+        for s in oracle_lib.seed_set(a, b, linked=True)[:30]:
+            pos.append(i)
+            starts.append((s[0] + 1, s[2] + 1))
+        for _ in range(10):
+            pos.append(i)
+            starts.append((int(rng.integers(0, len(a) + 1)), int(rng.integers(0, len(b) + 1))))
+    p = dcns_amd.default_params(match=sc[0], mismatch=sc[1], open=sc[2], ext=sc[3], w=w)
+    print("xdrop wave bands", sc, w, len(pos), flush=True)
+    got = gpu.xdrop_extend(pairs, pos, starts, params=p)
+    traced = 0
+    for k, (i, (s1, s2)) in enumerate(zip(pos, starts)):
+        a, b = pairs[i]
+        want = _x_extend_oracle(oracle_lib, a, b, s1, s2, sc=sc, w=w)
+        g = got[k]
+        if want[5] == 0:
+            assert g[5] == 0, (k, g[:6], want[:6])
+            continue
+        assert g[:6] == want[:6], (k, g[:6], want[:6])
+        assert g[6] == want[6], k
+        traced += 1
+    assert traced > 20
+
+
 def _cmp_fragments(got, want, where):
     assert len(got) == len(want), (where, len(got), len(want))
     for g, w in zip(got, want):
