@@ -80,6 +80,7 @@ def load_library():
     lib.dcns_semiglobal_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _P, _I, _I, _P,
                                           _P, _P, _L]
     lib.dcns_read_score_folder.argtypes = [ctypes.c_char_p, _P, _I, ctypes.POINTER(_I)]
+    lib.dcns_link_seeds.argtypes = [_P, ctypes.c_int64, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     lib.dcns_gff_categories.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _L, _L, _L, _P]
     lib.dcns_find_similar_weighted_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _I, _P,
                                                      _P, _I, ctypes.POINTER(_P)]
@@ -121,7 +122,7 @@ def load_library():
 # Entry points declared in include/dcns.h (the library must export all of them).
 EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_destroy", "dcns_last_error",
            "dcns_device_info", "dcns_last_stats", "dcns_kernel_time", "dcns_kernel_time_reset", "dcns_sw_batch",
-           "dcns_xdrop_extend", "dcns_seed_sets", "dcns_find_similar_batch", "dcns_stage",
+           "dcns_xdrop_extend", "dcns_seed_sets", "dcns_link_seeds", "dcns_find_similar_batch", "dcns_stage",
            "dcns_find_similar_staged", "dcns_frag_set_total",
            "dcns_frag_set_pair", "dcns_frag_set_get", "dcns_frag_set_free", "dcns_cut1gap_main",
            "dcns_ransco_main", "dcns_fit_lambda_k", "dcns_mlp_create", "dcns_mlp_destroy", "dcns_mlp_num_params",
@@ -181,6 +182,20 @@ def fit_lambda_k_batch(scores, space, k0=0.3, lambda0=0.2, maxiter=550):
     if rc != 0:
         raise DcnsError(f"dcns_fit_lambda_k_batch failed ({rc})")
     return out
+
+
+def link_seeds(seeds):
+    """link_seeds of a seed list (start1, end1, start2, end2 each), inserted into
+    the set in the order given; the linked set in set order
+    (dcns_link_seeds; no device needed)."""
+    lib = load_library()
+    a = np.ascontiguousarray(np.asarray(seeds, np.int32).reshape(-1, 4))
+    out = np.zeros((max(len(a), 1), 4), np.int32)
+    n = ctypes.c_int64()
+    rc = lib.dcns_link_seeds(a.ctypes.data, len(a), out.ctypes.data, len(out), ctypes.byref(n))
+    if rc != 0:
+        raise DcnsError(f"dcns_link_seeds failed ({rc}): {lib.dcns_last_error(None).decode()}")
+    return [tuple(int(v) for v in r) for r in out[:n.value]]
 
 
 # -- weighted (ZDP) inputs ----------------------------------------------------

@@ -4,6 +4,8 @@
 // dedup (findSimilarFragmentsForPairedSequence4.cpp:433-524).
 #include "pipeline.h"
 
+#include <immintrin.h>
+
 #include <algorithm>
 #include <unordered_map>
 #include <chrono>
@@ -330,41 +332,134 @@ std::set<Seed> inner_set(const Forest &F, int32_t id, int32_t sp, int32_t tw, in
 
 }  // namespace
 
+namespace {
+
+// One pass of link_seeds (findSimilarFragmentsForPairedSequence4.cpp:298-334)
+// as written: for each seed s1 still unused, in start1 order, the seeds
+// after it are taken in order until s1 merges or the scan stops; a seed
+// inside s1 is marked used on the way.  Returns whether any seed merged.
+bool link_pass_scalar(const std::vector<Seed> &cur, std::vector<Seed> &next) {
+    const size_t n = cur.size();
+    std::vector<char> used(n, 0);
+    bool changed = false;
+    for (size_t i = 0; i < n; ++i) {
+        if (used[i]) continue;
+        const Seed s1 = cur[i];
+        bool merged = false;
+        for (size_t j = i + 1; j < n; ++j) {
+            if (used[j]) continue;
+            const Seed s2 = cur[j];
+            if (s1.getStart2() <= s2.getStart2() && s1.getEnd1() >= s2.getEnd1() && s1.getEnd2() >= s2.getEnd2()) {
+                used[j] = 1;  // s2 inside s1
+            } else if (s1.getStart2() <= s2.getStart2() && s1.getEnd1() <= s2.getEnd1() &&
+                       s1.getEnd2() <= s2.getEnd2()) {
+                const int32_t d = s2.getStart1() - s1.getStart1();
+                if (d == s2.getStart2() - s1.getStart2() && d == s2.getEnd1() - s1.getEnd1() &&
+                    d == s2.getEnd2() - s1.getEnd2()) {  // same diagonal: merge
+                    next.push_back(Seed(s1.getStart1(), s2.getEnd1(), s1.getStart2(), s2.getEnd2()));
+                    used[i] = 1;
+                    used[j] = 1;
+                    merged = true;
+                    break;
+                }
+            } else if (s2.getStart1() > s1.getEnd1() || s2.getStart2() > s1.getEnd2()) {
+                break;
+            }
+        }
+        if (merged) changed = true;
+        else next.push_back(s1);
+    }
+    return changed;
+}
+
+// The same pass, 8 candidates at a time.  A candidate's fate depends only on
+// s1, its own coordinates and whether it is still unused (marking one seed
+// used changes no other's), so a block yields, per candidate, "inside s1",
+// "merges" and "stops the scan" masks; the first unused merge-or-stop ends
+// the scan, and the unused seeds inside s1 before it are marked used, which
+// is what the one-at-a-time loop does.  Coordinates are copied to separate
+// arrays, padded by 8 seeds that are marked used.
+__attribute__((target("avx2"))) inline uint32_t mask(__m256i v) {
+    return (uint32_t)_mm256_movemask_ps(_mm256_castsi256_ps(v));
+}
+
+__attribute__((target("avx2"))) bool link_pass_avx2(const std::vector<Seed> &cur, std::vector<Seed> &next) {
+    const int64_t n = (int64_t)cur.size();
+    std::vector<int32_t> S1((size_t)n + 8, 0), E1((size_t)n + 8, 0), S2((size_t)n + 8, 0), E2((size_t)n + 8, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        S1[(size_t)i] = cur[(size_t)i].getStart1();
+        E1[(size_t)i] = cur[(size_t)i].getEnd1();
+        S2[(size_t)i] = cur[(size_t)i].getStart2();
+        E2[(size_t)i] = cur[(size_t)i].getEnd2();
+    }
+    std::vector<uint8_t> used((size_t)n + 8, 1);
+    std::fill(used.begin(), used.begin() + n, 0);
+    const __m256i ones = _mm256_set1_epi32(-1);
+    bool changed = false;
+    for (int64_t i = 0; i < n; ++i) {
+        if (used[(size_t)i]) continue;
+        const int32_t a1 = S1[(size_t)i], ae1 = E1[(size_t)i], a2 = S2[(size_t)i], ae2 = E2[(size_t)i];
+        const __m256i vs1 = _mm256_set1_epi32(a1), ve1 = _mm256_set1_epi32(ae1), vs2 = _mm256_set1_epi32(a2),
+                      ve2 = _mm256_set1_epi32(ae2);
+        bool merged = false;
+        for (int64_t j = i + 1; j < n; j += 8) {
+            const __m256i b1 = _mm256_loadu_si256((const __m256i *)(S1.data() + j));
+            const __m256i be1 = _mm256_loadu_si256((const __m256i *)(E1.data() + j));
+            const __m256i b2 = _mm256_loadu_si256((const __m256i *)(S2.data() + j));
+            const __m256i be2 = _mm256_loadu_si256((const __m256i *)(E2.data() + j));
+            const __m256i lo2 = _mm256_cmpgt_epi32(vs2, b2);  // s2.start2 < s1.start2
+            const uint32_t inside = mask(_mm256_andnot_si256(
+                _mm256_or_si256(lo2, _mm256_or_si256(_mm256_cmpgt_epi32(be1, ve1), _mm256_cmpgt_epi32(be2, ve2))), ones));
+            const uint32_t after = mask(_mm256_andnot_si256(
+                _mm256_or_si256(lo2, _mm256_or_si256(_mm256_cmpgt_epi32(ve1, be1), _mm256_cmpgt_epi32(ve2, be2))), ones));
+            const __m256i d = _mm256_sub_epi32(b1, vs1);
+            const uint32_t diag = mask(_mm256_and_si256(
+                _mm256_cmpeq_epi32(d, _mm256_sub_epi32(b2, vs2)),
+                _mm256_and_si256(_mm256_cmpeq_epi32(d, _mm256_sub_epi32(be1, ve1)),
+                                 _mm256_cmpeq_epi32(d, _mm256_sub_epi32(be2, ve2)))));
+            const uint32_t beyond = mask(_mm256_or_si256(_mm256_cmpgt_epi32(b1, ve1), _mm256_cmpgt_epi32(b2, ve2)));
+            const uint32_t live = (uint32_t)_mm_movemask_epi8(
+                                      _mm_cmpeq_epi8(_mm_loadl_epi64((const __m128i *)(used.data() + j)), _mm_setzero_si128())) &
+                                  0xffu;
+            const uint32_t merge = ~inside & after & diag;
+            const uint32_t stop = live & (merge | (~inside & ~after & beyond)) & 0xffu;
+            uint32_t absorb = live & inside & (stop ? (stop & (0u - stop)) - 1u : 0xffu);
+            while (absorb) {
+                used[(size_t)(j + __builtin_ctz(absorb))] = 1;
+                absorb &= absorb - 1;
+            }
+            if (stop) {
+                const int k = __builtin_ctz(stop);
+                if ((merge >> k) & 1u) {
+                    const int64_t q = j + k;
+                    next.push_back(Seed(a1, E1[(size_t)q], a2, E2[(size_t)q]));
+                    used[(size_t)i] = 1;
+                    used[(size_t)q] = 1;
+                    merged = true;
+                }
+                break;
+            }
+        }
+        if (merged) changed = true;
+        else next.push_back(Seed(a1, ae1, a2, ae2));
+    }
+    return changed;
+}
+
+}  // namespace
+
+// link_seeds (findSimilarFragmentsForPairedSequence4.cpp:287-344): seeds in
+// start1 order (std::sort, as the reference), merge passes until none
+// merges, then back into the set.
 void link_seeds(std::set<Seed> &seeds) {
     std::vector<Seed> cur(seeds.begin(), seeds.end());
     std::sort(cur.begin(), cur.end(), [](Seed a, Seed b) { return a.getStart1() < b.getStart1(); });
+    static const bool avx2 = __builtin_cpu_supports("avx2");
     bool changed = true;
     while (changed) {
-        changed = false;
         std::vector<Seed> next;
-        std::vector<char> used(cur.size(), 0);
-        for (size_t i = 0; i < cur.size(); ++i) {
-            if (used[i]) continue;
-            const Seed s1 = cur[i];
-            bool merged = false;
-            for (size_t j = i + 1; j < cur.size(); ++j) {
-                if (used[j]) continue;
-                const Seed s2 = cur[j];
-                if (s1.getStart2() <= s2.getStart2() && s1.getEnd1() >= s2.getEnd1() && s1.getEnd2() >= s2.getEnd2()) {
-                    used[j] = 1;  // s2 inside s1
-                } else if (s1.getStart2() <= s2.getStart2() && s1.getEnd1() <= s2.getEnd1() &&
-                           s1.getEnd2() <= s2.getEnd2()) {
-                    const int32_t d = s2.getStart1() - s1.getStart1();
-                    if (d == s2.getStart2() - s1.getStart2() && d == s2.getEnd1() - s1.getEnd1() &&
-                        d == s2.getEnd2() - s1.getEnd2()) {  // same diagonal: merge
-                        next.push_back(Seed(s1.getStart1(), s2.getEnd1(), s1.getStart2(), s2.getEnd2()));
-                        used[i] = 1;
-                        used[j] = 1;
-                        merged = true;
-                        break;
-                    }
-                } else if (s2.getStart1() > s1.getEnd1() || s2.getStart2() > s1.getEnd2()) {
-                    break;
-                }
-            }
-            if (merged) changed = true;
-            else next.push_back(s1);
-        }
+        next.reserve(cur.size());
+        changed = avx2 ? link_pass_avx2(cur, next) : link_pass_scalar(cur, next);
         cur.swap(next);
     }
     seeds.clear();

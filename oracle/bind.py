@@ -103,6 +103,17 @@ class Checker:
         assert n >= 0
         return [tuple(int(v) for v in out[4 * i:4 * i + 4]) for i in range(n)]
 
+    def link_seeds(self, seeds):
+        """link_seeds of a seed list (inserted into the set in the order given);
+        the linked set in set order."""
+        a = np.ascontiguousarray(np.asarray(seeds, np.int32).reshape(-1, 4))
+        out = np.zeros((max(len(a), 1), 4), np.int32)
+        fn = self._f("link_seeds")
+        fn.argtypes = [_P, _I, _P, _I]
+        n = fn(a.ctypes.data, len(a), out.ctypes.data, len(out))
+        assert n >= 0
+        return [tuple(int(v) for v in r) for r in out[:n]]
+
     def find_similar(self, s1, s2, sc=(2, -3, -4, -2), window=38, step=8, min_score=40, w=10, xdrop=20,
                      pvalues=0.1, lam=0.382291, k=0.006662, stats=None):
         """Returns a list of (start1, end1, start2, end2, score, pvalue, evalue, cigar)."""

@@ -516,6 +516,25 @@ struct Fragment {
 
 extern "C" {
 
+// link_seeds on a given list: inserted into the set in the order given,
+// linked, written in set order; returns the count (or -1 past cap).
+int32_t orc_link_seeds(const int32_t *seeds, int32_t n, int32_t *out, int32_t cap) {
+    std::set<Seed> set;
+    for (int32_t k = 0; k < n; ++k)
+        set.insert(Seed{seeds[4 * k], seeds[4 * k + 1], seeds[4 * k + 2], seeds[4 * k + 3]});
+    link_seeds(set);
+    if ((int64_t)set.size() > cap) return -1;
+    int32_t o = 0;
+    for (const Seed &s : set) {
+        out[4 * o] = s.start1;
+        out[4 * o + 1] = s.end1;
+        out[4 * o + 2] = s.start2;
+        out[4 * o + 3] = s.end2;
+        ++o;
+    }
+    return o;
+}
+
 int orc_sw_uses_simd_variant(int32_t refLen, int32_t readLen, int32_t match, int32_t open, int32_t ext) {
     return simd_bound(refLen, readLen, match, open, ext, 32767) ? 1 : 0;
 }

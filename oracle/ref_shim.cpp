@@ -12,6 +12,7 @@
 //   ref_xextend_tb    -> SemiGlobal_xextend (+ traceback)      impl/sequenceAlignment3.cpp:1318
 //   ref_semiglobal    -> SemiGlobal (two-piece affine)         impl/sequenceAlignment3.cpp:932
 //   ref_seed_set      -> getSeeds window loop + link_seeds     impl/findSimilarFragmentsForPairedSequence4.cpp:131,288,461-483
+//   ref_link_seeds    -> link_seeds on a given list             impl/findSimilarFragmentsForPairedSequence4.cpp:288-344
 //   ref_find_similar  -> findSimilarFragmentsForPairedSequence impl/findSimilarFragmentsForPairedSequence4.cpp:433
 //   ref_find_similar_2gaps -> findSimilarFragmentsForPairedSequence (overload 4) :752
 //   ref_semiglobal_weighted -> SemiGlobal(..., weights, Score&) / SemiGlobal_single_gap_penalty
@@ -154,6 +155,25 @@ int ref_seed_set(const int8_t *seq1, int32_t L1, const int8_t *seq2, int32_t L2,
         ++n;
     }
     return n <= cap ? n : -n;
+}
+
+// link_seeds on a given list: the seeds inserted into a std::set<Seed> in the
+// order given, then the reference's link_seeds; out in set order.
+int ref_link_seeds(const int32_t *seeds, int32_t n, int32_t *out, int32_t cap) {
+    std::set<Seed> set;
+    for (int32_t k = 0; k < n; ++k) {
+        int32_t s1 = seeds[4 * k], e1 = seeds[4 * k + 1], s2 = seeds[4 * k + 2], e2 = seeds[4 * k + 3];
+        set.insert(Seed(s1, e1, s2, e2));
+    }
+    link_seeds(set);
+    if ((int64_t)set.size() > cap) return -1;
+    int32_t o = 0;
+    for (const Seed &s : set) {
+        out[4 * o + 0] = s.getStart1(); out[4 * o + 1] = s.getEnd1();
+        out[4 * o + 2] = s.getStart2(); out[4 * o + 3] = s.getEnd2();
+        ++o;
+    }
+    return o;
 }
 
 // findSimilarFragmentsForPairedSequence overload 1.  ints = 5 per fragment
