@@ -80,7 +80,7 @@ def load_library():
     lib.dcns_semiglobal_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _P, _I, _I, _P,
                                           _P, _P, _L]
     lib.dcns_read_score_folder.argtypes = [ctypes.c_char_p, _P, _I, ctypes.POINTER(_I)]
-    lib.dcns_link_seeds.argtypes = [_P, ctypes.c_int64, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+    lib.dcns_link_seeds.argtypes = [_P, ctypes.c_int64, _I, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     lib.dcns_gff_categories.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _L, _L, _L, _P]
     lib.dcns_find_similar_weighted_batch.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _I, _P,
                                                      _P, _I, ctypes.POINTER(_P)]
@@ -184,15 +184,16 @@ def fit_lambda_k_batch(scores, space, k0=0.3, lambda0=0.2, maxiter=550):
     return out
 
 
-def link_seeds(seeds):
+def link_seeds(seeds, insertion_order=False):
     """link_seeds of a seed list (start1, end1, start2, end2 each), inserted into
-    the set in the order given; the linked set in set order
+    the set in the order given: the linked set in set order, or with
+    insertion_order the merged list in the order link_seeds inserts it back
     (dcns_link_seeds; no device needed)."""
     lib = load_library()
     a = np.ascontiguousarray(np.asarray(seeds, np.int32).reshape(-1, 4))
     out = np.zeros((max(len(a), 1), 4), np.int32)
     n = ctypes.c_int64()
-    rc = lib.dcns_link_seeds(a.ctypes.data, len(a), out.ctypes.data, len(out), ctypes.byref(n))
+    rc = lib.dcns_link_seeds(a.ctypes.data, len(a), int(insertion_order), out.ctypes.data, len(out), ctypes.byref(n))
     if rc != 0:
         raise DcnsError(f"dcns_link_seeds failed ({rc}): {lib.dcns_last_error(None).decode()}")
     return [tuple(int(v) for v in r) for r in out[:n.value]]

@@ -586,23 +586,27 @@ int dcns_find_similar_seeded_batch(dcns_ctx *c, const dcns_batch *b, const dcns_
     return DCNS_OK;
 }
 
-int dcns_link_seeds(const int32_t *seeds, int64_t n, int32_t *out, int64_t cap, int64_t *n_out) {
+int dcns_link_seeds(const int32_t *seeds, int64_t n, int32_t insertion_order, int32_t *out, int64_t cap,
+                    int64_t *n_out) {
     return guarded(nullptr, [&] {
         if (n < 0 || (n > 0 && !seeds) || !n_out || cap < 0 || (cap > 0 && !out))
             throw Error(DCNS_ERR_ARG, "invalid arguments");
         std::set<Seed> set;
         for (int64_t k = 0; k < n; ++k)
             set.insert(Seed(seeds[4 * k], seeds[4 * k + 1], seeds[4 * k + 2], seeds[4 * k + 3]));
-        link_seeds(set);
-        *n_out = (int64_t)set.size();
+        std::vector<Seed> list = link_seed_list(set);
+        if (!insertion_order) {
+            set.clear();
+            for (const Seed &s : list) set.insert(s);
+            list.assign(set.begin(), set.end());
+        }
+        *n_out = (int64_t)list.size();
         if (*n_out > cap) throw Error(DCNS_ERR_CAPACITY, "more linked seeds than cap");
-        int64_t o = 0;
-        for (const Seed &s : set) {
-            out[4 * o] = s.getStart1();
-            out[4 * o + 1] = s.getEnd1();
-            out[4 * o + 2] = s.getStart2();
-            out[4 * o + 3] = s.getEnd2();
-            ++o;
+        for (size_t o = 0; o < list.size(); ++o) {
+            out[4 * o] = list[o].getStart1();
+            out[4 * o + 1] = list[o].getEnd1();
+            out[4 * o + 2] = list[o].getStart2();
+            out[4 * o + 3] = list[o].getEnd2();
         }
     });
 }

@@ -448,10 +448,11 @@ __attribute__((target("avx2"))) bool link_pass_avx2(const std::vector<Seed> &cur
 
 }  // namespace
 
-// link_seeds (findSimilarFragmentsForPairedSequence4.cpp:287-344): seeds in
-// start1 order (std::sort, as the reference), merge passes until none
-// merges, then back into the set.
-void link_seeds(std::set<Seed> &seeds) {
+// link_seeds (findSimilarFragmentsForPairedSequence4.cpp:287-344) up to its
+// last loop: seeds in start1 order (std::sort, as the reference), merge
+// passes until none merges; the list in the order the reference inserts it
+// back into the set.
+std::vector<Seed> link_seed_list(const std::set<Seed> &seeds) {
     std::vector<Seed> cur(seeds.begin(), seeds.end());
     std::sort(cur.begin(), cur.end(), [](Seed a, Seed b) { return a.getStart1() < b.getStart1(); });
     static const bool avx2 = __builtin_cpu_supports("avx2");
@@ -462,6 +463,11 @@ void link_seeds(std::set<Seed> &seeds) {
         changed = avx2 ? link_pass_avx2(cur, next) : link_pass_scalar(cur, next);
         cur.swap(next);
     }
+    return cur;
+}
+
+void link_seeds(std::set<Seed> &seeds) {
+    const std::vector<Seed> cur = link_seed_list(seeds);
     seeds.clear();
     for (const Seed &s : cur) seeds.insert(s);
 }

@@ -40,6 +40,8 @@ void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked
 
 // link_seeds, findSimilarFragmentsForPairedSequence4.cpp:288-344.
 void link_seeds(std::set<Seed> &seeds);
+// link_seeds before its final insertion: the list in insertion order.
+std::vector<Seed> link_seed_list(const std::set<Seed> &seeds);
 
 // x_extend_seed with traceback for a list of tasks (seed start + 1 already applied).
 struct XResult {

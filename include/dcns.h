@@ -142,10 +142,14 @@ int dcns_seed_sets(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, int
 /* link_seeds (impl/findSimilarFragmentsForPairedSequence4.cpp:287-344) on the
  * host: the n seeds (4 int32 each: start1, end1, start2, end2) are inserted
  * into a std::set<Seed> in the order given, as the reference's window loop
- * inserts them, then merged; out receives the linked set in set order and
- * *n_out its size.  DCNS_ERR_CAPACITY when more than cap seeds result.
- * Needs no device or context. */
-int dcns_link_seeds(const int32_t *seeds, int64_t n, int32_t *out, int64_t cap, int64_t *n_out);
+ * inserts them, then merged.  insertion_order = 0: out receives the linked
+ * set in set order; 1: the merged list in the order link_seeds inserts it
+ * back into its set (inserting it so reproduces that set: Seed's comparator
+ * is not a strict weak order, so the order matters).  *n_out = the count;
+ * DCNS_ERR_CAPACITY when it exceeds cap (never above n).  Needs no device or
+ * context. */
+int dcns_link_seeds(const int32_t *seeds, int64_t n, int32_t insertion_order, int32_t *out, int64_t cap,
+                    int64_t *n_out);
 
 /* findSimilarFragmentsForPairedSequence (overload 1) for every pair. */
 int dcns_find_similar_batch(dcns_ctx *ctx, const dcns_batch *b, const dcns_params *p, dcns_frag_set **out);

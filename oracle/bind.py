@@ -103,14 +103,20 @@ class Checker:
         assert n >= 0
         return [tuple(int(v) for v in out[4 * i:4 * i + 4]) for i in range(n)]
 
-    def link_seeds(self, seeds):
-        """link_seeds of a seed list (inserted into the set in the order given);
-        the linked set in set order."""
+    def link_seeds(self, seeds, insertion_order=False):
+        """link_seeds of a seed list (inserted into the set in the order given):
+        the linked set in set order, or (oracle only) with insertion_order the
+        merged list in the order link_seeds inserts it back."""
         a = np.ascontiguousarray(np.asarray(seeds, np.int32).reshape(-1, 4))
         out = np.zeros((max(len(a), 1), 4), np.int32)
         fn = self._f("link_seeds")
-        fn.argtypes = [_P, _I, _P, _I]
-        n = fn(a.ctypes.data, len(a), out.ctypes.data, len(out))
+        if self.prefix == "orc":
+            fn.argtypes = [_P, _I, _I, _P, _I]
+            n = fn(a.ctypes.data, len(a), int(insertion_order), out.ctypes.data, len(out))
+        else:
+            assert not insertion_order
+            fn.argtypes = [_P, _I, _P, _I]
+            n = fn(a.ctypes.data, len(a), out.ctypes.data, len(out))
         assert n >= 0
         return [tuple(int(v) for v in r) for r in out[:n]]
 

@@ -467,7 +467,7 @@ void window_loop(SeedCtx &c, int32_t step, std::set<Seed> &seeds) {
 
 // link_seeds, :288-344.  Sorting is std::sort on start1 (not stable), so the
 // result depends on the library's sort; the product uses the same toolchain.
-void link_seeds(std::set<Seed> &seeds) {
+std::vector<Seed> link_seed_list(const std::set<Seed> &seeds) {
     std::vector<Seed> cur(seeds.begin(), seeds.end());
     std::sort(cur.begin(), cur.end(), [](Seed x, Seed y) { return x.start1 < y.start1; });
     bool changed = true;
@@ -502,6 +502,11 @@ void link_seeds(std::set<Seed> &seeds) {
         }
         cur.swap(next);
     }
+    return cur;
+}
+
+void link_seeds(std::set<Seed> &seeds) {
+    const std::vector<Seed> cur = link_seed_list(seeds);
     seeds.clear();
     for (const Seed &s : cur) seeds.insert(s);
 }
@@ -517,22 +522,26 @@ struct Fragment {
 extern "C" {
 
 // link_seeds on a given list: inserted into the set in the order given,
-// linked, written in set order; returns the count (or -1 past cap).
-int32_t orc_link_seeds(const int32_t *seeds, int32_t n, int32_t *out, int32_t cap) {
+// linked, written in set order (insertion_order 0) or as the merged list in
+// the order link_seeds inserts it back (1); returns the count (or -1 past cap).
+int32_t orc_link_seeds(const int32_t *seeds, int32_t n, int32_t insertion_order, int32_t *out, int32_t cap) {
     std::set<Seed> set;
     for (int32_t k = 0; k < n; ++k)
         set.insert(Seed{seeds[4 * k], seeds[4 * k + 1], seeds[4 * k + 2], seeds[4 * k + 3]});
-    link_seeds(set);
-    if ((int64_t)set.size() > cap) return -1;
-    int32_t o = 0;
-    for (const Seed &s : set) {
-        out[4 * o] = s.start1;
-        out[4 * o + 1] = s.end1;
-        out[4 * o + 2] = s.start2;
-        out[4 * o + 3] = s.end2;
-        ++o;
+    std::vector<Seed> list = link_seed_list(set);
+    if (!insertion_order) {
+        set.clear();
+        for (const Seed &s : list) set.insert(s);
+        list.assign(set.begin(), set.end());
     }
-    return o;
+    if ((int64_t)list.size() > cap) return -1;
+    for (size_t o = 0; o < list.size(); ++o) {
+        out[4 * o] = list[o].start1;
+        out[4 * o + 1] = list[o].end1;
+        out[4 * o + 2] = list[o].start2;
+        out[4 * o + 3] = list[o].end2;
+    }
+    return (int32_t)list.size();
 }
 
 int orc_sw_uses_simd_variant(int32_t refLen, int32_t readLen, int32_t match, int32_t open, int32_t ext) {

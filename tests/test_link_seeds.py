@@ -48,6 +48,7 @@ def test_link_seeds_vs_oracle_and_reference(oracle_lib, seed, n):
     got = dcns_amd.link_seeds(seeds)
     want = oracle_lib.link_seeds(seeds)
     assert got == want
+    assert dcns_amd.link_seeds(seeds, insertion_order=True) == oracle_lib.link_seeds(seeds, insertion_order=True)
     assert len(got) < len(set(seeds))  # merges and absorptions happened
     ref = bind.reference()
     if ref is not None:
