@@ -74,6 +74,19 @@ def measured_valu_instructions(path=os.path.join(ROOT, "profiles", "pmc_traffic_
     return None
 
 
+def measured_mask_traffic(path=os.path.join(ROOT, "profiles", "pmc_traffic_mask_genome.md")):
+    """HBM bytes per launch of the k-mer scan kernel (fetched + written) from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes, or None."""
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 5 and cells[1].startswith("`k_kmer_scan16"):
+                return int((float(cells[3]) + float(cells[5])) * 1e9)
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def _cpu_worker_init():
     # the reference prints a "seeds size" line per pair; keep stdout for the JSON line
     devnull = os.open(os.devnull, os.O_WRONLY)
@@ -419,6 +432,12 @@ def mask_phase(ctx, log, cpu=True, gbp=1.0, sample_mb=4):
                              "frac": round(achieved / 8000.0, 4),
                              "bytes_per_base": "1 read + 1 written (algorithmic); one table probe per window "
                                                "on top"}}
+    traffic = measured_mask_traffic()
+    if traffic is not None:  # the bucket reads the algorithmic bytes leave out, from the committed PMC passes
+        rs = out["roofline_scan"]
+        rs["traffic"] = traffic
+        rs["traffic_gb_per_s"] = round(traffic / (scan_ms * 1e-3) / 1e9, 1)
+        rs["traffic_frac"] = round(rs["traffic_gb_per_s"] / 8000.0, 4)
     log(f"maskGenome: {N / 1e9:.2f} Gbp in {dt * 1e3:.2f} ms ({N / dt / 1e9:.1f} Gbp/s), "
         f"scan {scan_ms:.2f} ms, emit {emit_ms:.2f} ms, masked {frac:.3f}, {ks.members} k-mers")
     # the whole command on the whole genome: FASTA and dump read, masked FASTA written
