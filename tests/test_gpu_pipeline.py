@@ -152,10 +152,14 @@ def test_find_similar_one_pair_10kb(gpu, oracle_lib):
 
 
 @pytest.mark.parametrize("sc,min_score", [((3, -4, -6, -3), 50), ((1, -1, -2, -1), 20), ((2, -1, -2, -1), 40),
-                                          ((60, -80, -100, -60), 600), ((2, -3, -1, -3), 40)])
+                                          ((60, -80, -100, -60), 600), ((2, -3, -1, -3), 40),
+                                          ((9, -11, -14, -5), 180)])
 def test_seed_sets_other_scorings(gpu, oracle_lib, sc, min_score):
     """Scorings that take the packed-f16 window scan and (window * match >
-    2048) the int16 one, including gap open cheaper than gap extend."""
+    2048) the int16 one, including gap open cheaper than gap extend.  The
+    later levels run packed too, except where the f16 patterns of match or
+    mismatch have a nonzero low byte (9 = 0x4880) or the f16 bound fails:
+    the int32 kernel."""
     pairs = _small_pairs(51, 10, 300, 2500, (2, 12))
     p = dcns_amd.default_params(match=sc[0], mismatch=sc[1], open=sc[2], ext=sc[3], min_score=min_score)
     print("seed_sets", sc, min_score, flush=True)
