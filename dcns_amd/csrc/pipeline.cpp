@@ -817,6 +817,11 @@ void find_similar_staged(dcns_ctx *c, Staged &S, const dcns_params &p, FragSet &
     seed_sets(c, S, P, false, sr);
     const double t1 = now_ms();
     std::vector<XTask> tasks;
+    {
+        size_t nl = 0;
+        for (const auto &v : sr.linked) nl += v.size();
+        tasks.reserve(nl);
+    }
     std::vector<int64_t> first_task((size_t)npairs + 1, 0);
     for (int32_t i = 0; i < npairs; ++i) {
         first_task[(size_t)i] = (int64_t)tasks.size();
