@@ -151,6 +151,34 @@ def test_find_similar_one_pair_10kb(gpu, oracle_lib):
     assert s["xdrop_cells"] == st["xdrop_cells"]
 
 
+def test_find_similar_long_rescans_with_n_vs_oracle(gpu, oracle_lib):
+    """Rescans of more than 4,096 rows (one wave per task, columns across the
+    lanes) on a reference with N runs left in place, a query whose last
+    window is partial (the padded columns of the two-strip lane kernel), and
+    planted fragments repeated along the reference so the recursion goes
+    several levels deep; fragments, seed counts and cell counts equal the
+    oracle's."""
+    rng = np.random.default_rng(4242)
+    a, b = synth.one_pair(seed=4243, L1=14000, L2=6013, nfrag=16)[:2]
+    for _ in range(25):  # N runs in the reference
+        p = int(rng.integers(0, len(a) - 60))
+        a[p:p + int(rng.integers(1, 50))] = 2
+    for k in range(6):  # copies of a query stretch along the reference: deep recursion
+        src = int(rng.integers(0, len(b) - 300))
+        dst = 1000 + k * 2100
+        a[dst:dst + 300] = synth.mutate(rng, b[src:src + 300], 0.05, 0.0)[:300]
+    assert (len(b) - 38) % 8 != 0
+    print("long rescans with N", len(a), len(b), flush=True)
+    got = gpu.find_similar([(a, b)])[0]
+    st = gpu.stats()
+    o = {}
+    want = oracle_lib.find_similar(a, b, stats=o)
+    assert got[0] == o["seeds"] and st["linked_seeds"] == o["linked"]
+    assert st["scan_cells"] == o["scan_cells"] and st["xdrop_cells"] == o["xdrop_cells"]
+    assert st["levels"] >= 3
+    _cmp_fragments(got[1], want, "long rescans")
+
+
 @pytest.mark.parametrize("sc,min_score", [((3, -4, -6, -3), 50), ((1, -1, -2, -1), 20), ((2, -1, -2, -1), 40),
                                           ((60, -80, -100, -60), 600), ((2, -3, -1, -3), 40),
                                           ((9, -11, -14, -5), 180)])
