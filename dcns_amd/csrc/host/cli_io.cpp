@@ -413,13 +413,12 @@ std::string sub_text(const std::string &s, int start, int end, bool revcomp) {
 // query text is reverse-complemented.  The counts come from NonN indexes
 // built once per call; a fragment whose coordinates fall outside the usual
 // 1 <= start <= end <= length takes the reference's own walk.
-void write_sam_and_o(const std::string &prefix, const std::vector<PairedSimilarFragment> &frags, int32_t refStrand,
-                     int32_t queStrand, int32_t refStart, int32_t queStart, const std::string &refChr,
-                     const std::string &queChr, const std::string &seq1_0, const std::string &seq2_0) {
-    std::ofstream sam(prefix + ".sam", std::ios_base::app);
-    std::ofstream o(prefix + "o", std::ios_base::app);
+void write_sam_and_o(std::ostream &sam, std::ostream &o, const std::vector<PairedSimilarFragment> &frags,
+                     int32_t refStrand, int32_t queStrand, int32_t refStart, int32_t queStart,
+                     const std::string &refChr, const std::string &queChr, const std::string &seq1_0,
+                     const std::string &seq2_0) {
     o << "#" << refChr << ":" << refStrand << ":" << refStart << "-" << queChr << ":" << queStrand << ":" << queStart
-      << std::endl;
+      << '\n';
     const bool strands_ok = (refStrand == 1 || refStrand == 0) && (queStrand == 1 || queStrand == 0);
     std::unique_ptr<NonN> i1, i2;
     if (strands_ok && !frags.empty()) {
@@ -496,11 +495,19 @@ void write_sam_and_o(const std::string &prefix, const std::vector<PairedSimilarF
                 for (int32_t j = (int32_t)nc.size() - 1; j >= 0; --j)
                     sam << (nc[(size_t)j] >> 4) << "MIDDSHI=XB"[nc[(size_t)j] & 0xfu];
             }
-            sam << "\t*\t0\t0\t" << sub_text(seq2_0, start, end, !rplus) << "\t*" << std::endl;
+            sam << "\t*\t0\t0\t" << sub_text(seq2_0, start, end, !rplus) << "\t*" << '\n';
         }
         o << f.getStart1() << " " << f.getEnd1() << " " << f.getStart2() << " " << f.getEnd2() << " " << f.getScore()
-          << " " << f.getPValue() << std::endl;
+          << " " << f.getPValue() << '\n';
     }
+}
+
+void write_sam_and_o(const std::string &prefix, const std::vector<PairedSimilarFragment> &frags, int32_t refStrand,
+                     int32_t queStrand, int32_t refStart, int32_t queStart, const std::string &refChr,
+                     const std::string &queChr, const std::string &seq1_0, const std::string &seq2_0) {
+    std::ofstream sam(prefix + ".sam", std::ios_base::app);
+    std::ofstream o(prefix + "o", std::ios_base::app);
+    write_sam_and_o(sam, o, frags, refStrand, queStrand, refStart, queStart, refChr, queChr, seq1_0, seq2_0);
 }
 
 void write_sam_and_o_unmasked(const std::string &prefix, const std::vector<PairedSimilarFragment> &frags,

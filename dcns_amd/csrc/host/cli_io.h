@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <fstream>
 #include <map>
+#include <ostream>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,11 @@ std::vector<int8_t> encode(const std::string &s);
 void write_sam_and_o(const std::string &prefix, const std::vector<PairedSimilarFragment> &frags, int32_t refStrand,
                      int32_t queStrand, int32_t refStart, int32_t queStart, const std::string &refChr,
                      const std::string &queChr, const std::string &seq1_0, const std::string &seq2_0);
+// The same into streams the caller has opened (in append mode, .sam first).
+void write_sam_and_o(std::ostream &sam, std::ostream &o, const std::vector<PairedSimilarFragment> &frags,
+                     int32_t refStrand, int32_t queStrand, int32_t refStart, int32_t queStart,
+                     const std::string &refChr, const std::string &queChr, const std::string &seq1_0,
+                     const std::string &seq2_0);
 
 // outputNoMasking, controlLayer4.cpp:56-117: coordinates without masked
 // runs, SAM cigar ops "MID", header line first and "#done" last in the o file.

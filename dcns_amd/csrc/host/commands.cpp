@@ -231,18 +231,20 @@ void cut1gap_run(dcns_ctx *ctx, const dcns_params &p, const std::vector<Cut1Job>
             continue;
         }
         for (const Cut1Job::Q &q : j.qs) {
+            // the reference opens the files three times per query (created,
+            // appended, "#done" appended); one opening each gives the same bytes
             const std::string prefix = j.output + q.meta.species;
-            { std::ofstream s(prefix + ".sam", std::ios_base::app); std::ofstream o(prefix + "o", std::ios_base::app); }
+            std::ofstream sam(prefix + ".sam", std::ios_base::app);
+            std::ofstream o(prefix + "o", std::ios_base::app);
             if (q.run) {
                 std::cout << "seeds size" << fs.seeds_size[k] << std::endl;
                 std::vector<PairedSimilarFragment> fr(fs.frags.begin() + fs.first[k],
                                                       fs.frags.begin() + fs.first[k] + fs.count[k]);
-                write_sam_and_o(prefix, fr, j.ref.strand, q.meta.strand, j.ref.start, q.meta.start, j.ref.chr,
+                write_sam_and_o(sam, o, fr, j.ref.strand, q.meta.strand, j.ref.start, q.meta.start, j.ref.chr,
                                 q.meta.chr, j.seq1_0, q.seq2_0);
                 ++k;
             }
-            std::ofstream o(prefix + "o", std::ios_base::app);
-            o << "#done" << std::endl;
+            o << "#done\n";
         }
     }
 }
