@@ -200,9 +200,8 @@ Cut1Job cut1gap_job(const std::string &input, const std::string &reference, cons
     return j;
 }
 
-// All pairs of the jobs in one batch through overload 1; per job the file
-// operations and stdout lines of controlLayer4.cpp:1411-1527, in job order.
-void cut1gap_run(dcns_ctx *ctx, const dcns_params &p, const std::vector<Cut1Job> &jobs) {
+// All pairs of the jobs in one batch through overload 1.
+FragSet cut1gap_align(dcns_ctx *ctx, const dcns_params &p, const std::vector<Cut1Job> &jobs) {
     std::vector<int8_t> buf;
     std::vector<int64_t> off1, off2;
     std::vector<int32_t> len1, len2;
@@ -224,6 +223,12 @@ void cut1gap_run(dcns_ctx *ctx, const dcns_params &p, const std::vector<Cut1Job>
         dcns_batch b{buf.data(), off1.data(), len1.data(), off2.data(), len2.data(), (int32_t)off1.size(), 0};
         find_similar(ctx, b, p, fs);
     }
+    return fs;
+}
+
+// Per job the file operations and stdout lines of controlLayer4.cpp:1411-1527,
+// in job order.
+void cut1gap_write(const std::vector<Cut1Job> &jobs, const FragSet &fs) {
     size_t k = 0;
     for (const Cut1Job &j : jobs) {
         if (!j.error.empty()) {
@@ -271,7 +276,7 @@ int cut1gap_main(int argc, char **argv) {
     }
     dcns_ctx *ctx = make_ctx();
     try {
-        cut1gap_run(ctx, to_dcns_params(d), jobs);
+        cut1gap_write(jobs, cut1gap_align(ctx, to_dcns_params(d), jobs));
     } catch (const std::exception &e) {
         std::cerr << "dCNS: " << e.what() << std::endl;
         dcns_ctx_destroy(ctx);
@@ -394,19 +399,41 @@ int batch_main(int argc, char **argv) {
             }
             return jobs;
         };
+        // three stages at once: the next chunk is read while this one aligns
+        // and the previous one's files are written; writes stay in chunk order
         std::future<std::vector<Cut1Job>> ahead = std::async(std::launch::async, next_chunk);
-        for (;;) {
-            std::vector<Cut1Job> jobs = ahead.get();
-            if (jobs.empty()) break;
-            ahead = std::async(std::launch::async, next_chunk);  // reads the next chunk while this one aligns
-            for (const Cut1Job &j : jobs) status = (status || !j.error.empty()) ? 1 : 0;
+        std::future<void> writing;
+        auto finish_writing = [&]() {
+            if (!writing.valid()) return;
             try {
-                cut1gap_run(ctx, p, jobs);
+                writing.get();
             } catch (const std::exception &e) {
                 std::cerr << "dCNS: " << e.what() << std::endl;
                 status = 1;
             }
+        };
+        for (;;) {
+            std::vector<Cut1Job> jobs = ahead.get();
+            if (jobs.empty()) break;
+            ahead = std::async(std::launch::async, next_chunk);
+            for (const Cut1Job &j : jobs) status = (status || !j.error.empty()) ? 1 : 0;
+            FragSet fs;
+            bool aligned = false;
+            try {
+                fs = cut1gap_align(ctx, p, jobs);
+                aligned = true;
+            } catch (const std::exception &e) {
+                finish_writing();
+                std::cerr << "dCNS: " << e.what() << std::endl;
+                status = 1;
+            }
+            finish_writing();
+            if (aligned)
+                writing = std::async(std::launch::async, [jobs = std::move(jobs), fs = std::move(fs)]() {
+                    cut1gap_write(jobs, fs);
+                });
         }
+        finish_writing();
         li = le;
     }
     if (ctx) dcns_ctx_destroy(ctx);
