@@ -28,7 +28,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS & pairs/s, batched vs plain GPU SW; λ/K surrogate speedup at ≤2% err"
-VALU_OPS_PER_CELL = 10  # add (substitution), 3 max (H), 2 add + 1 max (E), 1 add + 1 max (F), 1 max (running max)
+# Algorithmic VALU operations per DP cell of the seed scan, counted in the
+# instructions the peak counts (one packed-16-bit VALU operation per half,
+# a three-input maximum being one operation, as v_pk_maximum3_f16 is):
+#   t = diag + s                1 add (the score s comes from a query profile)
+#   H = max3(t, E, F)           1 (E, F >= 0 keep the local-alignment floor)
+#   Ho = H + open               1 (shared by both gap kinds)
+#   E = max3(E + ext, Ho, 0)    1 add + 1 max3
+#   F = max3(F + ext, Ho, 0)    1 add + 1 max3
+#   running row maximum         0.5 (one max3 folds two cells)
+VALU_OPS_PER_CELL = 7.5
 
 
 def window_cells(len1, len2, W=38, step=8):
@@ -87,6 +96,16 @@ def measured_mask_traffic(path=os.path.join(ROOT, "profiles", "pmc_traffic_mask_
     return None
 
 
+def host_cores():
+    """The host cores this process may use: the thread count the machine sets
+    for its jobs (OMP_NUM_THREADS) where it sets one, else the affinity mask."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return min(n, len(os.sched_getaffinity(0))) if n > 0 else len(os.sched_getaffinity(0))
+
+
 def _cpu_worker_init():
     # the reference prints a "seeds size" line per pair; keep stdout for the JSON line
     devnull = os.open(os.devnull, os.O_WRONLY)
@@ -101,10 +120,12 @@ def _cpu_worker(args):
     return chk.find_similar(a, b)
 
 
-def cpu_baseline(ctx, pairs, budget_s, cores):
+def cpu_baseline(ctx, pairs, budget_s, cores, timed=None):
     """The compiled reference (oracle/_ref) on a bounded sample of the batch,
     one single-threaded pair per worker over `cores` workers, as the reference
-    is deployed (one process per region).  Returns (json dict, parity flag)."""
+    is deployed (one process per region).  Returns (json dict, parity flag of
+    a separate call on the sample, parity flag of the sample's pairs in
+    `timed`, the fragments of one staged run of the whole batch)."""
     import multiprocessing as mp
     from oracle import bind
     kind = "reference" if bind.reference() is not None else "port"
@@ -137,10 +158,13 @@ def cpu_baseline(ctx, pairs, budget_s, cores):
     st = ctx.stats()
     cells = st["scan_cells"] + st["xdrop_cells"]
     identical = all(g[1] == c for g, c in zip(gpu_out, cpu_out))
+    timed_same = None
+    if timed is not None:
+        timed_same = all(timed[i][1] == c for i, c in zip(sample, cpu_out))
     return {"value": round(cells / wall / 1e9, 4), "unit": "GCUPS", "cores": cores, "kind": kind,
             "pairs_per_s": round(len(sub) / wall, 3),
             "sample": f"{len(sub)} of {len(pairs)} pairs (fixed random order), {cells:.3e} cells, "
-                      f"{wall:.1f} s wall, one single-threaded pair per worker"}, identical
+                      f"{wall:.1f} s wall, one single-threaded pair per worker"}, identical, timed_same
 
 
 def lambdak_phase(ctx, log):
@@ -642,6 +666,7 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="take the launch shapes from the tuning table")
     ap.add_argument("--no-lambdak", action="store_true")
     ap.add_argument("--no-two-piece", action="store_true")
     ap.add_argument("--no-weighted", action="store_true")
@@ -697,6 +722,17 @@ def main():
     pcie_s = time.perf_counter() - t
     for _ in range(max(args.warmup - 1, 0)):
         ctx.find_similar_staged(collect=False)
+    # launch shapes for this batch: tuned live (the default), or the table's entry
+    from dcns_amd import autotune
+    if args.no_autotune:
+        chosen = autotune.lookup(len1, len2) or dict(autotune.DEFAULT)
+        ctx.set_tuning(**chosen)
+        tuning = {"source": "table" if autotune.lookup(len1, len2) else "defaults", "chosen": chosen}
+    else:
+        best, rows = autotune.tune(ctx, reps=2, log=lambda m: print(m, file=sys.stderr, flush=True))
+        tuning = {"source": "autotuned on this batch (coordinate descent, best of 2 steps per candidate)",
+                  "chosen": best, "measured": rows, "shape": autotune.shape_key(len1, len2)}
+        ctx.find_similar_staged(collect=False)
     ctx.kernel_time_reset()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -712,23 +748,29 @@ def main():
     avg_ms = ms / max(launches, 1)
     l0 = window_cells(len1, len2)
     achieved = l0 * VALU_OPS_PER_CELL / (avg_ms * 1e-3) / 1e12
-    peak = dev["cu"] * 4 * 16 * 2 * dev["clock_mhz"] * 1e6 / 1e12  # packed int16 VALU lane-ops
+    peak = dev["cu"] * 4 * 16 * 2 * dev["clock_mhz"] * 1e6 / 1e12  # packed 16-bit VALU ops (2 per lane-instruction)
     kernels = {}
-    for k in ("scan_uniform", "resolve", "scan_general", "xdrop_pass1", "xdrop_pass1_wave", "xdrop_pass2",
-              "xdrop_pass2_wave", "gather"):
+    for k in ("seed_init", "scan_uniform", "seed_levels", "level_begin", "order_tasks", "scan_general", "resolve",
+              "gather", "xdrop_pass1", "xdrop_pass1_wave", "xdrop_pass2", "xdrop_pass2_wave"):
         kms, kn = ctx.kernel_time(k)
         if kn:
             kernels[k] = {"ms_per_step": round(kms / args.steps, 3), "launches_per_step": kn / args.steps}
+    # one more staged run of the same batch, its fragments kept: the run the
+    # compiled reference's sample is checked against (same path as the timed steps)
+    timed_frags = ctx.find_similar_staged(collect=True)
+    timed_nfrag = sum(len(f) for _, f in timed_frags)
 
     line = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f16x2 exact integers (seed scan), int32 (extension)", "data": "synthetic",
         "config": {"workload": f"batched {args.pairs} pairs, log-uniform 1-50 kb, GC 0.40-0.60, 0-30 planted "
                                "fragments, cut1Gap defaults (window 38, step 8, min seed 40, x-drop 20)",
                    "pairs": args.pairs, "reference_cells_per_step": cells, "fragments": int(nfrag),
+                   "fragments_checked_run": int(timed_nfrag),
                    "seeds": st["seeds"], "linked_seeds": st["linked_seeds"], "levels": st["levels"]},
         "pairs_per_s": round(args.pairs / dt, 1),
+        "tuning": tuning,
         "phases_ms": {"seed": round(st["ms_seed"], 1), "xdrop": round(st["ms_xdrop"], 1),
                       "dedup": round(st["ms_replay"], 1)},
         "kernels": kernels,
@@ -736,7 +778,8 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_scan_uniform", "achieved": round(achieved, 3),
                      "peak": round(peak, 2), "unit": "TOP/s", "frac": round(achieved / peak, 4),
                      "traffic": measured_traffic(),
-                     "work_per_launch": f"{l0:.4e} cells x {VALU_OPS_PER_CELL} int16 ops",
+                     "work_per_launch": f"{l0:.4e} level-0 cells x {VALU_OPS_PER_CELL} packed-f16 VALU ops "
+                                        "(a 3-input max = 1 op)",
                      "avg_launch_ms": round(avg_ms, 3)},
     }
 
@@ -780,10 +823,11 @@ def main():
         line["batch"] = batch_phase(log, cpu=not args.no_cpu)
 
     if not args.no_cpu:
-        cores = args.cpu_cores or min(len(os.sched_getaffinity(0)), 16)
-        cb, same = cpu_baseline(ctx, pairs, args.cpu_seconds, cores)
+        cores = args.cpu_cores or host_cores()
+        cb, same, timed_same = cpu_baseline(ctx, pairs, args.cpu_seconds, cores, timed_frags)
         line["cpu_baseline"] = cb
         line["cpu_sample_bit_exact"] = bool(same)
+        line["timed_run_bit_exact"] = bool(timed_same)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -43,6 +43,12 @@ class Stats(ctypes.Structure):
                 ("ms_xdrop", _D), ("ms_total", _D), ("semiglobal_cells", _L), ("ms_semiglobal", _D)]
 
 
+class Tuning(ctypes.Structure):
+    """dcns_tuning: launch shapes of the seed phase (0 = default)."""
+    _fields_ = [("level0_mode", _I), ("level0_waves", _I), ("wave_tasks", _I), ("wave_rows", _I),
+                ("graph_levels", _I), ("pool_min", _I), ("pad0", _I), ("pad1", _I)]
+
+
 _lib = None
 
 
@@ -64,6 +70,8 @@ def load_library():
     lib.dcns_last_stats.argtypes = [_P, ctypes.POINTER(Stats)]
     lib.dcns_kernel_time.argtypes = [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_L)]
     lib.dcns_kernel_time_reset.argtypes = [_P]
+    lib.dcns_get_tuning.argtypes = [_P, ctypes.POINTER(Tuning)]
+    lib.dcns_set_tuning.argtypes = [_P, ctypes.POINTER(Tuning)]
     lib.dcns_sw_batch.argtypes = [_P, ctypes.POINTER(Batch), _I, _I, _I, _I, _P]
     lib.dcns_xdrop_extend.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _P, _P, _I, _P, _P, _P, _L]
     lib.dcns_seed_sets.argtypes = [_P, ctypes.POINTER(Batch), ctypes.POINTER(Params), _I, _P, _P, _L]
@@ -134,7 +142,8 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_weighted2gaps_main", "dcns_find_similar_seeded_batch", "dcns_cut2gaps2_main", "dcns_kmer_set_read",
            "dcns_kmer_set_info", "dcns_kmer_set_get", "dcns_kmer_set_free", "dcns_cds_mask_create",
            "dcns_cds_mask_gff", "dcns_cds_mask_sam", "dcns_cds_mask_get", "dcns_cds_mask_free", "dcns_mask_stage",
-           "dcns_mask_body_size", "dcns_mask_run", "dcns_mask_genome_main", "dcns_batch_main"]
+           "dcns_mask_body_size", "dcns_mask_run", "dcns_mask_genome_main", "dcns_batch_main", "dcns_get_tuning",
+           "dcns_set_tuning"]
 
 
 def default_params(**kw):
@@ -433,6 +442,19 @@ class Context:
 
     def kernel_time_reset(self):
         self._check(self.lib.dcns_kernel_time_reset(self.h), "kernel_time_reset")
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self.lib.dcns_get_tuning(self.h, ctypes.byref(t)), "dcns_get_tuning")
+        return {f: getattr(t, f) for f, _ in Tuning._fields_ if not f.startswith("pad")}
+
+    def set_tuning(self, **kw):
+        """Sets the given dcns_tuning fields (others keep their values);
+        returns the previous settings."""
+        old = self.get_tuning()
+        t = Tuning(**{**old, **kw})
+        self._check(self.lib.dcns_set_tuning(self.h, ctypes.byref(t)), "dcns_set_tuning")
+        return old
 
     # -- K1 -----------------------------------------------------------------
     def sw_batch(self, pairs=None, sc=(2, -3, -4, -2), flat=None):

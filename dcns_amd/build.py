@@ -28,7 +28,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
 
 LIB_SOURCES = ["seedscan.hip", "xdrop.hip", "swbatch.hip", "pipeline.cpp", "capi.cpp", "host/cli_io.cpp",
                "host/commands.cpp", "host/lambdak.cpp", "mlp.hip",
-               "semiglobal.hip", "twogaps.cpp", "tasksort.hip", "host/weights.cpp", "mask.hip", "maskgenome.cpp",
+               "semiglobal.hip", "twogaps.cpp", "host/weights.cpp", "mask.hip", "maskgenome.cpp",
                "host/mask_io.cpp", "host/mask_cmd.cpp"]
 CLI_SOURCES = ["cli/main.cpp"]
 

@@ -141,6 +141,24 @@ int dcns_last_stats(const dcns_ctx *c, dcns_stats *out) {
     return DCNS_OK;
 }
 
+int dcns_get_tuning(const dcns_ctx *c, dcns_tuning *out) {
+    if (!c || !out) return DCNS_ERR_ARG;
+    *out = c->tune;
+    return DCNS_OK;
+}
+
+int dcns_set_tuning(dcns_ctx *c, const dcns_tuning *in) {
+    if (!c || !in) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        const dcns_tuning &t = *in;
+        if (t.level0_mode < 0 || t.level0_mode > 2) throw Error(DCNS_ERR_ARG, "level0_mode must be 0, 1 or 2");
+        if (t.level0_waves < 0 || t.level0_waves > 32) throw Error(DCNS_ERR_ARG, "level0_waves must be in [0, 32]");
+        if (t.wave_tasks < 0 || t.wave_rows < 0 || t.pool_min < 0) throw Error(DCNS_ERR_ARG, "negative tuning value");
+        if (t.graph_levels < -1 || t.graph_levels > 64) throw Error(DCNS_ERR_ARG, "graph_levels must be in [-1, 64]");
+        c->tune = t;
+    });
+}
+
 int dcns_kernel_time(const dcns_ctx *c, const char *name, double *ms, int64_t *launches) {
     if (!c || !name) return DCNS_ERR_ARG;
     auto it = c->timers.find(name);

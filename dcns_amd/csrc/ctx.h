@@ -118,16 +118,19 @@ void parallel_for(int64_t n, F &&fn, int threads) {
 namespace dcns {
 struct Staged;
 struct MaskStaged;
+struct SeedPhase;
 }
 
 struct dcns_ctx {
     int device = 0;
     std::shared_ptr<dcns::Staged> staged;  // sequences kept resident by dcns_stage
     std::shared_ptr<dcns::MaskStaged> mstaged;  // genome and k-mer tables kept resident by dcns_mask_stage
+    std::shared_ptr<dcns::SeedPhase> seedphase;  // recursion pool sizes and the captured level graph
     hipStream_t stream = nullptr;
     bool timing = false;
     std::string err;
     dcns_stats stats{};
+    dcns_tuning tune{};  // launch shapes (0 = default), see dcns_set_tuning
     std::map<std::string, dcns::KernelTimer> timers;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;  // recorded, not yet read
     std::vector<hipEvent_t> event_pool;
@@ -138,7 +141,7 @@ struct dcns_ctx {
     dcns::DBuf sg_tasks, sg_out, sg_scr, sg_tb, sg_cig, sg_strips, sg_sync;  // two-piece semi-global tasks
     dcns::DBuf sg_wcat, sg_wcat_off, sg_wtab;  // weighted DP: per-position profile indices and the profiles
     dcns::DBuf xw_scr, xw_off, xw_wmax;                  // x-drop tasks with bands past the wave kernel's
-    dcns::DBuf sort_scr, task_order;                     // scan tasks ordered by row count
+    dcns::DBuf order_hist, task_order;                   // scan tasks ordered by row count
     dcns::DBuf xorder;                                   // x-drop pass-2 order
     dcns::DBuf xlist;                                    // x-drop tasks handed from the lane to the wave kernel
     dcns::DBuf cig_scan, cig_packed;                     // cigar packing: counts / offsets / scan storage, runs
@@ -146,6 +149,16 @@ struct dcns_ctx {
     dcns::DBuf mk_seq, mk_off, mk_keys, mk_desc, mk_gbytes, mk_ghash, mk_gtab;  // maskGenome: genome, k-mer set
     dcns::DBuf mk_mask, mk_ranges, mk_boff, mk_body;                          // maskGenome: mask, CDS, FASTA bodies
     dcns::HBuf h_hits, h_roots, h_nodes, h_xout, h_cigar;  // pinned download buffers
+
+    int32_t cus = 0;  // compute units of the device (cached)
+    int32_t cu_count() {
+        if (!cus) {
+            hipDeviceProp_t prop;
+            dcns::hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+            cus = prop.multiProcessorCount;
+        }
+        return cus;
+    }
 
     hipEvent_t take_event() {
         if (!event_pool.empty()) {

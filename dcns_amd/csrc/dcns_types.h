@@ -64,6 +64,8 @@ struct Counters {
     int32_t level_tasks;  // scan tasks of the current level (read by the level's kernels)
     int32_t pending;      // x-drop tasks that need a wider band kernel
     unsigned long long gen_cells;  // DP cells evaluated by the later-level scans
+    int32_t queue;        // next level-0 task of the persistent scan's queue
+    int32_t levels;       // recursion levels after level 0 that had tasks
 };
 
 // x-drop extension task for one seed (x_extend_seed,

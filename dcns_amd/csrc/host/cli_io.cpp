@@ -184,7 +184,7 @@ std::string IndexedFasta::fetch(const std::string &chr, int32_t start1, int32_t 
         const int nl_end = (start + length - 1) / e.line_blen;
         const int seqlen = length + (nl_end - nl_before);
         in_.seekg(e.offset + nl_before + start, std::ios::beg);
-        char c;
+        char c = 'N';  // a get() that fails keeps the previous character (none before the first)
         for (int32_t i = 0; i < seqlen && !in_.eof(); ++i) {
             in_.get(c);
             sub += c;

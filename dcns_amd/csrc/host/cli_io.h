@@ -50,6 +50,8 @@ class IndexedFasta {
   public:
     explicit IndexedFasta(const std::string &fasta);
     std::string fetch(const std::string &chr, int32_t start1, int32_t end1, int8_t strand);
+    // clears the stream's error state (a fetch past the end of the file sets it)
+    void clear() { in_.clear(); }
     struct Entry {
         int32_t length = 0;
         int64_t offset = 0;

@@ -170,6 +170,11 @@ Cut1Job cut1gap_job(const std::string &input, const std::string &reference, cons
                     IndexedFasta &refFa, IndexedFasta &queFa) {
     Cut1Job j;
     j.output = output;
+    // each line starts with clean genome streams: a fetch that ran past the
+    // end of a genome file (a truncated FASTA, a stale index) affects only
+    // the rest of its own line, as it does in the reference's one process per line
+    refFa.clear();
+    queFa.clear();
     if (!std::ifstream(input).good()) {
         j.error = "error in opening fasta file " + input;
         return j;
@@ -394,7 +399,16 @@ int batch_main(int argc, char **argv) {
                 std::vector<char *> uv;
                 for (const std::string &w : u) uv.push_back(const_cast<char *>(w.c_str()));
                 Args ua((int)uv.size(), uv.data());
-                jobs.push_back(cut1gap_job(ua.get("-i"), reference, ua.get("-o"), refFa, queFa));
+                // a line whose input cannot be read (a malformed region header)
+                // fails alone, as its own process would
+                try {
+                    jobs.push_back(cut1gap_job(ua.get("-i"), reference, ua.get("-o"), refFa, queFa));
+                } catch (const std::exception &e) {
+                    Cut1Job bad;
+                    bad.output = ua.get("-o");
+                    bad.error = "dCNS: " + ua.get("-i") + ": " + e.what();
+                    jobs.push_back(std::move(bad));
+                }
                 pairs += jobs.back().pairs();
             }
             return jobs;
@@ -1019,12 +1033,38 @@ int dispatch(int argc, char **argv) {
 }  // namespace cli
 }  // namespace dcns
 
-extern "C" int dcns_cut1gap_main(int argc, char **argv) { return dcns::cli::cut1gap_main(argc, argv); }
-extern "C" int dcns_ransco_main(int argc, char **argv) { return dcns::cli::ransco_main(argc, argv); }
-extern "C" int dcns_cut2gaps_main(int argc, char **argv) { return dcns::cli::cut2gaps_main(argc, argv); }
-extern "C" int dcns_paircns2gaps_main(int argc, char **argv) { return dcns::cli::paircns2gaps_main(argc, argv); }
-extern "C" int dcns_paircnsxextend_main(int argc, char **argv) { return dcns::cli::paircnsxextend_main(argc, argv); }
-extern "C" int dcns_weighted1gap_main(int argc, char **argv) { return dcns::cli::weighted1gap_main(argc, argv); }
-extern "C" int dcns_weighted2gaps_main(int argc, char **argv) { return dcns::cli::weighted2gaps_main(argc, argv); }
-extern "C" int dcns_cut2gaps2_main(int argc, char **argv) { return dcns::cli::cut2gaps2_main(argc, argv); }
-extern "C" int dcns_batch_main(int argc, char **argv) { return dcns::cli::batch_main(argc, argv); }
+namespace {
+// No exception crosses the C boundary: a command that throws (a malformed
+// number in a header or a flag, a failed allocation) reports it and fails.
+template <class F>
+int cli_guard(F &&f, int argc, char **argv) {
+    try {
+        return f(argc, argv);
+    } catch (const std::exception &e) {
+        std::cout.flush();
+        std::cerr << "dCNS: " << (argc > 0 ? argv[0] : "") << ": " << e.what() << std::endl;
+    } catch (...) {
+        std::cout.flush();
+        std::cerr << "dCNS: " << (argc > 0 ? argv[0] : "") << ": unknown error" << std::endl;
+    }
+    return 1;
+}
+}  // namespace
+
+extern "C" int dcns_cut1gap_main(int argc, char **argv) { return cli_guard(dcns::cli::cut1gap_main, argc, argv); }
+extern "C" int dcns_ransco_main(int argc, char **argv) { return cli_guard(dcns::cli::ransco_main, argc, argv); }
+extern "C" int dcns_cut2gaps_main(int argc, char **argv) { return cli_guard(dcns::cli::cut2gaps_main, argc, argv); }
+extern "C" int dcns_paircns2gaps_main(int argc, char **argv) {
+    return cli_guard(dcns::cli::paircns2gaps_main, argc, argv);
+}
+extern "C" int dcns_paircnsxextend_main(int argc, char **argv) {
+    return cli_guard(dcns::cli::paircnsxextend_main, argc, argv);
+}
+extern "C" int dcns_weighted1gap_main(int argc, char **argv) {
+    return cli_guard(dcns::cli::weighted1gap_main, argc, argv);
+}
+extern "C" int dcns_weighted2gaps_main(int argc, char **argv) {
+    return cli_guard(dcns::cli::weighted2gaps_main, argc, argv);
+}
+extern "C" int dcns_cut2gaps2_main(int argc, char **argv) { return cli_guard(dcns::cli::cut2gaps2_main, argc, argv); }
+extern "C" int dcns_batch_main(int argc, char **argv) { return cli_guard(dcns::cli::batch_main, argc, argv); }

@@ -25,22 +25,36 @@ inline bool scan_general_packed(const Params &P, uint32_t &mis4, uint32_t &diff)
     diff = (m ^ x) >> 8;
     return true;
 }
+// Level-0 window scan: one workgroup per 128-window task, or, with
+// persistent_waves > 0, that many waves popping the task list from ctr->queue.
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
-                               int32_t nwt, const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount,
-                               int32_t cap, int32_t *hitlist, Counters *ctr);
+                               int32_t nwt, int32_t persistent_waves, const Params &P, const ScoreTab &tab,
+                               TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr);
+// Later-level rescans: *ntasks_dev tasks (at most ntasks_max) in order[];
+// up to wave_tasks of the first ones with more than wave_rows rows run one
+// per wave, the rest one per lane on at most lane_blocks_max waves (0: one
+// lane per task).
 hipError_t launch_scan_general(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
                                const ScanTask *tasks, const int32_t *order, const int32_t *ntasks_dev,
-                               int32_t ntasks_max, const Params &P,
-                               const ScoreTab &tab, TrajEntry *traj, int32_t *tcount, int32_t cap, Counters *ctr);
+                               int32_t ntasks_max, int32_t wave_tasks, int32_t wave_rows, int32_t lane_blocks_max,
+                               const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount, int32_t cap,
+                               Counters *ctr);
 hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
                           const int32_t *pair_of_window, const Params &P, const ScoreTab &tab, int level0,
                           const int32_t *hitlist, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
-                          Node *nodes, int32_t node_cap, const TrajEntry *traj, const int32_t *tcount, int32_t cap,
-                          int32_t scan_cap, ScanTask *next, int32_t next_cap, Counters *ctr);
-// tasksort.hip: index of the scan tasks by row count, longest first.
-size_t sort_tasks_scratch_bytes(int32_t n_max);
-hipError_t launch_sort_tasks(hipStream_t st, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
-                             void *scratch, size_t scratch_bytes, int32_t *order);
+                          int32_t blocks_max, Node *nodes, int32_t node_cap, const TrajEntry *traj,
+                          const int32_t *tcount, int32_t cap, int32_t scan_cap, ScanTask *next, int32_t next_cap,
+                          Counters *ctr);
+// Counter block at the start of a seed phase (nodes = scans = nw, the rest 0).
+hipError_t launch_seed_init(hipStream_t st, Counters *ctr, int32_t nw);
+// Start of a recursion level: level_tasks = next_tasks (0 after an
+// overflow), next_tasks = 0, the order histogram (2 * kOrderBuckets ints) cleared.
+constexpr int kOrderBuckets = 512;
+hipError_t launch_level_begin(hipStream_t st, Counters *ctr, int32_t *hist);
+// order[0, *n_dev): the level's tasks by row count, longest first (16 buckets
+// per doubling); hist: 2 * kOrderBuckets ints cleared by launch_level_begin.
+hipError_t launch_order_tasks(hipStream_t st, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
+                              int32_t *hist, int32_t *order);
 hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v);
 hipError_t launch_gather_roots(hipStream_t st, const Node *nodes, const int32_t *hitlist, const int32_t *n_dev,
                                int32_t n_max, Node *out);

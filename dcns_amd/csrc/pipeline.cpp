@@ -182,6 +182,48 @@ struct Forest {
     const Node &child(int32_t id) const { return dyn[(size_t)(id - NW)]; }
 };
 
+// Launch shapes of the seed phase with the defaults filled in.
+struct SeedTuning {
+    int32_t level0_waves;  // > 0: persistent level-0 waves (total); 0: one workgroup per task
+    int32_t wave_tasks, wave_rows, graph_levels, pool_min;
+};
+
+SeedTuning seed_tuning(dcns_ctx *c) {
+    const dcns_tuning &t = c->tune;
+    SeedTuning T;
+    const int32_t mode = t.level0_mode ? t.level0_mode : 1;
+    T.level0_waves = mode == 2 ? (t.level0_waves ? t.level0_waves : 8) * c->cu_count() : 0;
+    T.wave_tasks = t.wave_tasks ? t.wave_tasks : 1024;
+    T.wave_rows = t.wave_rows ? t.wave_rows : 4096;
+    T.graph_levels = t.graph_levels ? std::max(t.graph_levels, 0) : 4;
+    T.pool_min = t.pool_min;
+    return T;
+}
+
+}  // namespace
+
+// Seed-phase state of a context: the recursion pools' capacities (grown when
+// a level overflows them, kept for the next batch with the same window
+// count) and the captured graph of the recursion levels.
+struct SeedPhase {
+    int64_t nw = -1;
+    int32_t pool_min = -1;
+    int64_t scan_cap = 0, node_cap = 0, next_cap = 0;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int64_t> key;   // every value the captured launches bake in
+    std::vector<int64_t> seen;  // the last layout run without a graph
+    ~SeedPhase() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+namespace {
+
+// Lane and resolve launches of the captured levels stride over the task list
+// with at most this many waves (the task count is not known when they are
+// enqueued).
+constexpr int32_t kGraphLaneWaves = 4096;
+
 void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_t &gpu_cells, int32_t &levels) {
     const int32_t NW = (int32_t)S.nwin;
     F = Forest{};
@@ -193,92 +235,182 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
     const int wm = P.window <= 38 ? 38 : 64;
     const int32_t cap = P.window * P.match - P.min_score + 1;
     const ScoreTab tab = make_score_tab(P.match, P.mismatch);
-
-    int64_t scan_cap = (int64_t)NW + std::max<int64_t>(NW / 4, 4096);
-    int64_t node_cap = (int64_t)NW + std::max<int64_t>(NW / 2, 8192);
-    int64_t next_cap = std::max<int64_t>(NW / 4, 4096);
-    c->traj.ensure((size_t)scan_cap * cap * sizeof(TrajEntry));
-    c->tcount.ensure((size_t)scan_cap * sizeof(int32_t));
+    const SeedTuning T = seed_tuning(c);
+    if (!c->seedphase) c->seedphase = std::make_shared<SeedPhase>();
+    SeedPhase &SP = *c->seedphase;
+    if (SP.nw != NW || SP.pool_min != T.pool_min) {
+        if (T.pool_min > 0) {  // pools that must grow (tests of the growth path)
+            SP.scan_cap = (int64_t)NW + T.pool_min;
+            SP.node_cap = (int64_t)NW + T.pool_min;
+            SP.next_cap = T.pool_min;
+        } else {
+            SP.scan_cap = (int64_t)NW + std::max<int64_t>(NW / 4, 4096);
+            SP.node_cap = (int64_t)NW + std::max<int64_t>(NW / 2, 8192);
+            SP.next_cap = std::max<int64_t>(NW / 4, 4096);
+        }
+        SP.nw = NW;
+        SP.pool_min = T.pool_min;
+    }
+    auto ensure_pools = [&]() {
+        c->traj.ensure((size_t)SP.scan_cap * cap * sizeof(TrajEntry));
+        c->tcount.ensure((size_t)SP.scan_cap * sizeof(int32_t));
+        c->nodes.ensure((size_t)SP.node_cap * sizeof(Node));
+        c->tasks_a.ensure((size_t)SP.next_cap * sizeof(ScanTask));
+        c->tasks_b.ensure((size_t)SP.next_cap * sizeof(ScanTask));
+        c->task_order.ensure((size_t)SP.next_cap * sizeof(int32_t) + 64);
+    };
+    ensure_pools();
     c->hitlist.ensure((size_t)NW * sizeof(int32_t) + 64);
-    c->nodes.ensure((size_t)node_cap * sizeof(Node));
-    c->tasks_a.ensure((size_t)next_cap * sizeof(ScanTask));
-    c->tasks_b.ensure((size_t)next_cap * sizeof(ScanTask));
     c->ctr.ensure(sizeof(Counters) + 64);
+    c->order_hist.ensure(2 * kOrderBuckets * sizeof(int32_t));
+    Counters *ctr = c->ctr.as<Counters>();
+    const int32_t *hits_dev = &ctr->hits, *lt_dev = &ctr->level_tasks;
+    // level L >= 1 scans the tasks in buffer (L - 1) & 1 and queues the next
+    // level's into buffer L & 1; level 0's resolve queues into buffer 0
+    auto buf = [&](int L) { return (L & 1) ? c->tasks_b.as<ScanTask>() : c->tasks_a.as<ScanTask>(); };
+    auto resolve_launch = [&](int L, int32_t n_max, int32_t blocks_max) {
+        return launch_resolve(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->pow.as<int32_t>(), P, tab,
+                              L == 0, c->hitlist.as<int32_t>(), L == 0 ? nullptr : buf(L - 1),
+                              L == 0 ? hits_dev : lt_dev, n_max, blocks_max, c->nodes.as<Node>(),
+                              (int32_t)SP.node_cap, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
+                              (int32_t)SP.scan_cap, buf(L), (int32_t)SP.next_cap, ctr);
+    };
+    auto level_launches = [&](int L, int32_t n_max, int32_t lane_blocks_max, bool timed) {
+        auto run = [&](const char *name, auto f) {
+            if (timed) c->launch(name, f);
+            else hip_check(f(), name);
+        };
+        run("level_begin", [&] { return launch_level_begin(c->stream, ctr, c->order_hist.as<int32_t>()); });
+        run("order_tasks", [&] {
+            return launch_order_tasks(c->stream, buf(L - 1), lt_dev, n_max, c->order_hist.as<int32_t>(),
+                                      c->task_order.as<int32_t>());
+        });
+        run("scan_general", [&] {
+            return launch_scan_general(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), buf(L - 1),
+                                       c->task_order.as<int32_t>(), lt_dev, n_max, T.wave_tasks, T.wave_rows,
+                                       lane_blocks_max, P, tab, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
+                                       ctr);
+        });
+    };
 
-    Counters h{};
-    h.nodes = NW;
-    h.scans = NW;
-    write_ctr(c, h);
+    c->launch("seed_init", [&] { return launch_seed_init(c->stream, ctr, NW); });
     c->launch("scan_uniform", [&] {
         return launch_scan_uniform(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->wtasks.as<int2>(),
-                                   (int32_t)S.wt.size(), P, tab, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
-                                   c->hitlist.as<int32_t>(), c->ctr.as<Counters>());
+                                   (int32_t)S.wt.size(), T.level0_waves, P, tab, c->traj.as<TrajEntry>(),
+                                   c->tcount.as<int32_t>(), cap, c->hitlist.as<int32_t>(), ctr);
     });
-    h = read_ctr(c);
-    const int32_t hits = h.hits;
 
-    // resolve with capacity retry: counters keep counting past a full pool,
-    // so one retry with the requested sizes always fits
-    auto resolve = [&](int level0, const ScanTask *tasks, int32_t n, const int32_t *n_dev) {
+    Counters h{};
+    int L = 1;  // the next level to run
+    bool resolved0 = false;
+    bool use_graph = T.graph_levels > 0;
+    if (use_graph) {
+        // level 0's resolve and graph_levels levels, enqueued with no host
+        // read between them: every launch reads its task count from the
+        // counter block; captured once per batch layout, pools and settings
+        const PairDesc *pd = c->pairs.as<PairDesc>();
+        std::vector<int64_t> key = {(int64_t)c->seq.p, (int64_t)pd, (int64_t)c->pow.p, (int64_t)c->hitlist.p,
+                                    (int64_t)c->nodes.p, (int64_t)c->traj.p, (int64_t)c->tcount.p,
+                                    (int64_t)c->tasks_a.p, (int64_t)c->tasks_b.p, (int64_t)c->task_order.p,
+                                    (int64_t)c->order_hist.p, (int64_t)ctr, NW, SP.scan_cap, SP.node_cap,
+                                    SP.next_cap, T.wave_tasks, T.wave_rows, T.graph_levels, wm, cap,
+                                    P.match, P.mismatch, P.open, P.ext, P.window, P.step, P.min_score, P.min_ref};
+        // a layout seen for the first time runs host-driven; one that recurs
+        // (a staged batch stepped again) is captured and replayed from then on
+        if ((!SP.exec || SP.key != key) && SP.seen != key) {
+            SP.seen = key;
+            use_graph = false;
+        } else if (!SP.exec || SP.key != key) {
+            if (SP.exec) {
+                hip_check(hipGraphExecDestroy(SP.exec), "hipGraphExecDestroy");
+                SP.exec = nullptr;
+            }
+            hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            hipGraph_t g = nullptr;
+            try {
+                hip_check(resolve_launch(0, NW, kGraphLaneWaves), "resolve");
+                for (int l = 1; l <= T.graph_levels; ++l) {
+                    level_launches(l, (int32_t)SP.next_cap, kGraphLaneWaves, false);
+                    hip_check(resolve_launch(l, (int32_t)SP.next_cap, kGraphLaneWaves), "resolve");
+                }
+            } catch (...) {
+                (void)hipStreamEndCapture(c->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
+            const hipError_t e = hipGraphInstantiate(&SP.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hip_check(e, "hipGraphInstantiate");
+            SP.key = key;
+        }
+    }
+    if (use_graph) {
+        c->launch("seed_levels", [&] { return hipGraphLaunch(SP.exec, c->stream); });
+        h = read_ctr(c);
+        if (!h.overflow) {
+            resolved0 = true;
+            levels = 1 + h.levels;
+            L = T.graph_levels + 1;
+        } else {
+            // a pool overflowed: the recursion runs again from level 0, host-driven,
+            // growing the pools (the level-0 scan's trajectories are intact)
+            Counters r{};
+            r.nodes = NW;
+            r.scans = NW;
+            r.hits = h.hits;
+            r.entries = h.entries;
+            h = r;
+            write_ctr(c, h);
+        }
+    } else {
+        h = read_ctr(c);
+    }
+
+    // resolve with capacity retry: a walk stops at its first allocation that
+    // does not fit, so a retry can need more than the last one requested;
+    // pools at least double per attempt
+    auto resolve = [&](int level) {
         Counters before = h;
         before.next_tasks = 0;
         before.overflow = 0;
-        for (int attempt = 0; attempt < 4; ++attempt) {
-            size_t used_nodes = (size_t)before.nodes * sizeof(Node);
-            size_t used_scans = (size_t)before.scans;
-            c->nodes.ensure((size_t)node_cap * sizeof(Node), c->stream, true, used_nodes);
-            c->traj.ensure((size_t)scan_cap * cap * sizeof(TrajEntry), c->stream, true, used_scans * cap * sizeof(TrajEntry));
-            c->tcount.ensure((size_t)scan_cap * sizeof(int32_t), c->stream, true, used_scans * sizeof(int32_t));
-            c->tasks_b.ensure((size_t)next_cap * sizeof(ScanTask));
+        const int32_t n = level == 0 ? h.hits : h.level_tasks;
+        for (int attempt = 0; attempt < 32; ++attempt) {
+            c->nodes.ensure((size_t)SP.node_cap * sizeof(Node), c->stream, true, (size_t)before.nodes * sizeof(Node));
+            c->traj.ensure((size_t)SP.scan_cap * cap * sizeof(TrajEntry), c->stream, true,
+                           (size_t)before.scans * cap * sizeof(TrajEntry));
+            c->tcount.ensure((size_t)SP.scan_cap * sizeof(int32_t), c->stream, true,
+                             (size_t)before.scans * sizeof(int32_t));
+            // the output task buffer (its contents are rewritten); the input one is untouched
+            ((level & 1) ? c->tasks_b : c->tasks_a).ensure((size_t)SP.next_cap * sizeof(ScanTask));
             write_ctr(c, before);
-            c->launch("resolve", [&] {
-                return launch_resolve(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->pow.as<int32_t>(), P,
-                                      tab, level0, c->hitlist.as<int32_t>(), tasks, n_dev, n, c->nodes.as<Node>(),
-                                      (int32_t)node_cap, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
-                                      (int32_t)scan_cap, c->tasks_b.as<ScanTask>(), (int32_t)next_cap,
-                                      c->ctr.as<Counters>());
-            });
+            c->launch("resolve", [&] { return resolve_launch(level, n, 0); });
             h = read_ctr(c);
             if (!h.overflow) return;
-            node_cap = std::max<int64_t>(node_cap, (int64_t)h.nodes + h.nodes / 4 + 1024);
-            scan_cap = std::max<int64_t>(scan_cap, (int64_t)h.scans + h.scans / 4 + 1024);
-            next_cap = std::max<int64_t>(next_cap, (int64_t)h.next_tasks + h.next_tasks / 4 + 1024);
+            SP.node_cap = std::max<int64_t>(2 * SP.node_cap, (int64_t)h.nodes + h.nodes / 4 + 1024);
+            SP.scan_cap = std::max<int64_t>(2 * SP.scan_cap, (int64_t)h.scans + h.scans / 4 + 1024);
+            SP.next_cap = std::max<int64_t>(2 * SP.next_cap, (int64_t)h.next_tasks + h.next_tasks / 4 + 1024);
         }
         throw Error(DCNS_ERR_INTERNAL, "seed recursion pools could not be sized");
     };
-
-    // the hit count lives in the counter block; resolve reads it from there
-    const int32_t *hits_dev = &c->ctr.as<Counters>()->hits;
-    resolve(1, nullptr, hits, hits_dev);
-    levels = 1;
+    if (!resolved0) {
+        resolve(0);
+        levels = 1;
+    }
+    // levels past the captured ones (or all of them): host-driven, exact grids
     while (h.next_tasks > 0) {
         const int32_t n = h.next_tasks;
-        std::swap(c->tasks_a.p, c->tasks_b.p);
-        std::swap(c->tasks_a.cap, c->tasks_b.cap);
-        h.level_tasks = n;  // next_tasks is reset by resolve; the level's kernels read this copy
-        h.next_tasks = 0;
-        write_ctr(c, h);
-        const int32_t *n_dev = &c->ctr.as<Counters>()->level_tasks;
-        // longest tasks first: the lanes of a wave then run similar row counts
-        const size_t sort_bytes = sort_tasks_scratch_bytes(n);
-        c->sort_scr.ensure(sort_bytes);
         c->task_order.ensure((size_t)n * sizeof(int32_t) + 64);
-        c->launch("sort_tasks", [&] {
-            return launch_sort_tasks(c->stream, c->tasks_a.as<ScanTask>(), n_dev, n, c->sort_scr.p, sort_bytes,
-                                     c->task_order.as<int32_t>());
-        });
-        c->launch("scan_general", [&] {
-            return launch_scan_general(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
-                                       c->tasks_a.as<ScanTask>(), c->task_order.as<int32_t>(), n_dev, n, P, tab,
-                                       c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap, c->ctr.as<Counters>());
-        });
+        level_launches(L, n, 0, true);
         h = read_ctr(c);
-        resolve(0, c->tasks_a.as<ScanTask>(), n, n_dev);
+        resolve(L);
         ++levels;
+        ++L;
     }
     gpu_cells += (int64_t)h.gen_cells;
 
     // bring the forest home (pinned buffers): hit windows, their roots, the dynamic nodes
+    const int32_t hits = h.hits;
     const size_t ndyn = (size_t)(h.nodes - NW);
     c->roots.ensure((size_t)std::max(hits, 1) * sizeof(Node));
     c->launch("gather", [&] {

@@ -59,14 +59,21 @@ __device__ __forceinline__ void record(TrajEntry *__restrict__ traj, int64_t slo
     ++cnt;
 }
 
+// The next level-0 task of a persistent wave: lane 0 pops the queue head,
+// the wave reads the index from it (uniform).
+__device__ __forceinline__ int32_t pop_task(int32_t *queue) {
+    int32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(queue, 1);
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
 template <int WM>
-__global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
-                                                     const int2 *__restrict__ wtasks, Params P, ScoreTab tab,
-                                                     TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
-                                                     int32_t cap, int32_t *__restrict__ hitlist,
-                                                     Counters *__restrict__ ctr) {
+__device__ __forceinline__ void scan_uniform_task(const int2 task, const int8_t *__restrict__ seq,
+                                                  const PairDesc *__restrict__ pairs, const Params &P,
+                                                  const ScoreTab &tab, TrajEntry *__restrict__ traj,
+                                                  int32_t *__restrict__ tcount, int32_t cap,
+                                                  int32_t *__restrict__ hitlist, Counters *__restrict__ ctr) {
     const int lane = threadIdx.x;
-    const int2 task = wtasks[blockIdx.x];
     const PairDesc pd = pairs[task.x];
     const int8_t *__restrict__ ref = seq + pd.off1;
     const int8_t *__restrict__ qry = seq + pd.off2;
@@ -152,20 +159,37 @@ __global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ 
     }
 }
 
+// Level-0 window scan, one wave per 128-window task: task blockIdx.x of the
+// length-sorted list or, with a queue (persistent waves, a grid that fills
+// the chip once), the tasks popped from its head in list order.
+template <int WM, bool PERSIST>
+__global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                     const int2 *__restrict__ wtasks, int32_t nwt,
+                                                     int32_t *__restrict__ queue, Params P, ScoreTab tab,
+                                                     TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
+                                                     int32_t cap, int32_t *__restrict__ hitlist,
+                                                     Counters *__restrict__ ctr) {
+    if (!PERSIST) {
+        scan_uniform_task<WM>(wtasks[blockIdx.x], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr);
+        return;
+    }
+    for (int32_t t = pop_task(queue); t < nwt; t = pop_task(queue))
+        scan_uniform_task<WM>(wtasks[t], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr);
+}
+
 // Packed-f16 version of k_scan_uniform (same wave layout).  Valid when
 // window * match <= 2048: every H, E, F and score is then an exact f16
 // integer, and v_pk_maximum3_f16 merges the three-way maxima.  Non-negative
 // f16 values order like their bit patterns, so the improvement test and the
 // first-column search compare bits.
 template <int WM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_uniform_h(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
-                                                       const int2 *__restrict__ wtasks, Params P, ScoreTab tab,
-                                                       TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
-                                                       int32_t cap, int32_t *__restrict__ hitlist,
-                                                       Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
-                                                       uint32_t thr_h) {
+__device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_t *__restrict__ seq,
+                                                    const PairDesc *__restrict__ pairs, const Params &P,
+                                                    const ScoreTab &tab, TrajEntry *__restrict__ traj,
+                                                    int32_t *__restrict__ tcount, int32_t cap,
+                                                    int32_t *__restrict__ hitlist, Counters *__restrict__ ctr,
+                                                    uint32_t open_h, uint32_t ext_h, uint32_t thr_h) {
     const int lane = threadIdx.x;
-    const int2 task = wtasks[blockIdx.x];
     const PairDesc pd = pairs[task.x];
     const int8_t *__restrict__ ref = seq + pd.off1;
     const int8_t *__restrict__ qry = seq + pd.off2;
@@ -251,6 +275,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             atomicAdd(&ctr->entries, min(cnth, cap));
         }
     }
+}
+
+// One wave per task: task blockIdx.x of the length-sorted list (3 waves
+// per SIMD fit its 168 registers).
+template <int WM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_uniform_h(
+    const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const int2 *__restrict__ wtasks, int32_t nwt,
+    int32_t *__restrict__ queue, Params P, ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
+    int32_t cap, int32_t *__restrict__ hitlist, Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
+    uint32_t thr_h) {
+    scan_uniform_task_h<WM>(wtasks[blockIdx.x], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h,
+                            thr_h);
+}
+
+// Persistent waves: each pops tasks from the queue head in list order until
+// the list is exhausted.  The loop's own state does not fit beside the task's
+// registers at 3 waves per SIMD without spilling, so this one runs 2.
+template <int WM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_scan_uniform_hp(
+    const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const int2 *__restrict__ wtasks, int32_t nwt,
+    int32_t *__restrict__ queue, Params P, ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
+    int32_t cap, int32_t *__restrict__ hitlist, Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
+    uint32_t thr_h) {
+    for (int32_t t = pop_task(queue); t < nwt; t = pop_task(queue))
+        scan_uniform_task_h<WM>(wtasks[t], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h, thr_h);
 }
 
 // One lane per ScanTask: rows [a, b) of seq1 against one window, int32.
@@ -339,23 +388,23 @@ __device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *_
     }
 }
 
-// Tasks of more than kWaveRows rows among the first wave_blocks of the
+// Tasks of more than wave_rows rows among the first wave_blocks of the
 // order (longest first) run one per wave (the first wave_blocks blocks);
-// every other task runs one per lane (the blocks after them).
-constexpr int kWaveRows = 4096;
-
+// every other task runs one per lane (the blocks after them, which stride
+// over the order when they are fewer than its tasks).  The task count is
+// read from the device, so the launch can be enqueued before it is known.
 template <int WM>
 __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                      const ScanTask *__restrict__ tasks, const int32_t *__restrict__ order,
                                                      const int32_t *__restrict__ ntasks_p, int32_t wave_blocks,
-                                                     Params P, ScoreTab tab, TrajEntry *__restrict__ traj,
-                                                     int32_t *__restrict__ tcount, int32_t cap,
-                                                     Counters *__restrict__ ctr) {
+                                                     int32_t wave_rows, Params P, ScoreTab tab,
+                                                     TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
+                                                     int32_t cap, Counters *__restrict__ ctr) {
     const int ntasks = *ntasks_p;
     if ((int)blockIdx.x < wave_blocks) {
         if ((int)blockIdx.x >= ntasks) return;
         const ScanTask t = tasks[order ? order[blockIdx.x] : blockIdx.x];
-        if (t.b - t.a > kWaveRows) scan_task_wave<WM>(seq, pairs, t, P, tab, traj, tcount, cap, ctr);
+        if (t.b - t.a > wave_rows) scan_task_wave<WM>(seq, pairs, t, P, tab, traj, tcount, cap, ctr);
         return;
     }
     __shared__ uint32_t shi[8], slo[8];
@@ -364,10 +413,10 @@ __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ 
         slo[threadIdx.x] = tab.lo[threadIdx.x];
     }
     __syncthreads();
-    const int gid = ((int)blockIdx.x - wave_blocks) * blockDim.x + threadIdx.x;
-    if (gid >= ntasks) return;
+    const int stride = ((int)gridDim.x - wave_blocks) * (int)blockDim.x;
+    for (int gid = ((int)blockIdx.x - wave_blocks) * blockDim.x + threadIdx.x; gid < ntasks; gid += stride) {
     const ScanTask t = tasks[order ? order[gid] : gid];
-    if (gid < wave_blocks && t.b - t.a > kWaveRows) return;  // a wave's task
+    if (gid < wave_blocks && t.b - t.a > wave_rows) continue;  // a wave's task
     const PairDesc pd = pairs[t.pair];
     const int8_t *__restrict__ ref = seq + pd.off1;
     const int8_t *__restrict__ qry = seq + pd.off2;
@@ -416,6 +465,7 @@ __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ 
     tcount[t.scan] = cnt;
     if (cnt > 0) atomicAdd(&ctr->entries, min(cnt, cap));
     atomicAdd(&ctr->gen_cells, (unsigned long long)(t.b - t.a) * (unsigned long long)max(tw, 0));
+    }
 }
 
 // Packed-f16 k_scan_general: one task per lane, its window split in two
@@ -432,26 +482,26 @@ __global__ __launch_bounds__(64) void k_scan_general(const int8_t *__restrict__ 
 // both scores.  A row's maximum is complete when its high half is done; the
 // first column of a low-half maximum is taken the step before, when that
 // half alone exceeds the running best.  Tasks the int32 wave path takes (the
-// first wave_blocks of the order, more than kWaveRows rows) run there as in
+// first wave_blocks of the order, more than wave_rows rows) run there as in
 // k_scan_general.
 template <int WM>
 __global__ __launch_bounds__(64) void k_scan_general_h(
     const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const ScanTask *__restrict__ tasks,
-    const int32_t *__restrict__ order, const int32_t *__restrict__ ntasks_p, int32_t wave_blocks, Params P,
-    ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount, int32_t cap,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ ntasks_p, int32_t wave_blocks, int32_t wave_rows,
+    Params P, ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount, int32_t cap,
     Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h, uint32_t thr_h, uint32_t mis4, uint32_t diff) {
     constexpr int C = WM / 2;
     const int ntasks = *ntasks_p;
     if ((int)blockIdx.x < wave_blocks) {
         if ((int)blockIdx.x >= ntasks) return;
         const ScanTask t = tasks[order ? order[blockIdx.x] : blockIdx.x];
-        if (t.b - t.a > kWaveRows) scan_task_wave<WM>(seq, pairs, t, P, tab, traj, tcount, cap, ctr);
+        if (t.b - t.a > wave_rows) scan_task_wave<WM>(seq, pairs, t, P, tab, traj, tcount, cap, ctr);
         return;
     }
-    const int gid = ((int)blockIdx.x - wave_blocks) * 64 + (int)threadIdx.x;
-    if (gid >= ntasks) return;
+    const int stride = ((int)gridDim.x - wave_blocks) * 64;
+    for (int gid = ((int)blockIdx.x - wave_blocks) * 64 + (int)threadIdx.x; gid < ntasks; gid += stride) {
     const ScanTask t = tasks[order ? order[gid] : gid];
-    if (gid < wave_blocks && t.b - t.a > kWaveRows) return;  // a wave's task
+    if (gid < wave_blocks && t.b - t.a > wave_rows) continue;  // a wave's task
     const PairDesc pd = pairs[t.pair];
     const int8_t *__restrict__ ref = seq + pd.off1 + t.a;
     const int8_t *__restrict__ qry = seq + pd.off2;
@@ -529,6 +579,7 @@ __global__ __launch_bounds__(64) void k_scan_general_h(
     tcount[t.scan] = cnt;
     if (cnt > 0) atomicAdd(&ctr->entries, min(cnt, cap));
     atomicAdd(&ctr->gen_cells, (unsigned long long)rows * (unsigned long long)max(tw, 0));
+    }
 }
 
 // Reverse SW of getSeeds (:152-159): reverse complements of seq1[e1-r1+1..e1]
@@ -596,8 +647,8 @@ __global__ __launch_bounds__(64) void k_resolve(const int8_t *__restrict__ seq, 
         slo[threadIdx.x] = tab.lo[threadIdx.x];
     }
     __syncthreads();
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= *n_p) return;
+    const int n_in = *n_p;
+    for (int gid = blockIdx.x * blockDim.x + threadIdx.x; gid < n_in; gid += gridDim.x * blockDim.x) {
     int32_t node, pair, win;
     if (level0) {
         node = hitlist[gid];
@@ -687,6 +738,93 @@ __global__ __launch_bounds__(64) void k_resolve(const int8_t *__restrict__ seq, 
         if (lchild < 0) break;
         node = lchild;
     }
+    }
+}
+
+// Counters at the start of a seed phase: the window nodes and trajectory
+// slots allocated, everything else zero.
+__global__ void k_seed_init(Counters *__restrict__ ctr, int32_t nw) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        Counters c{};
+        c.nodes = nw;
+        c.scans = nw;
+        *ctr = c;
+    }
+}
+
+// Start of a recursion level: the tasks the previous resolve queued become
+// the level's (none once a pool has overflowed, so that every later launch
+// of an enqueued sequence is empty), and the order histogram is cleared.
+__global__ void k_level_begin(Counters *__restrict__ ctr, int32_t *__restrict__ hist) {
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) hist[b] = 0;
+    if (threadIdx.x == 0) {
+        const int32_t n = ctr->overflow ? 0 : ctr->next_tasks;
+        ctr->level_tasks = n;
+        ctr->next_tasks = 0;
+        if (n > 0) ctr->levels += 1;
+    }
+}
+
+// Bucket of a scan task in the level order: 16 buckets per doubling of the
+// row count, longest first (bucket 0 holds the longest).
+__device__ __forceinline__ int order_bucket(int32_t rows) {
+    if (rows <= 0) return kOrderBuckets - 1;
+    const int lg = 31 - __clz(rows);
+    const int frac = lg >= 4 ? (rows >> (lg - 4)) & 15 : (rows << (4 - lg)) & 15;
+    return kOrderBuckets - 1 - min(lg * 16 + frac, kOrderBuckets - 1);
+}
+
+// The level order: a counting sort of the tasks by bucket (histogram,
+// exclusive scan, scatter).  Within a bucket the order is that of the
+// scatter's atomics; only the load balance depends on it.
+__global__ __launch_bounds__(256) void k_order_count(const ScanTask *__restrict__ tasks, const int32_t *__restrict__ n_p,
+                                                     int32_t *__restrict__ hist) {
+    __shared__ int32_t h[kOrderBuckets];
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const int n = *n_p;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicAdd(&h[order_bucket(tasks[i].b - tasks[i].a)], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
+        if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+__global__ __launch_bounds__(kOrderBuckets) void k_order_scan(int32_t *__restrict__ hist) {
+    __shared__ int32_t v[kOrderBuckets];
+    const int b = threadIdx.x;
+    v[b] = hist[b];
+    __syncthreads();
+    for (int d = 1; d < kOrderBuckets; d <<= 1) {
+        const int32_t x = b >= d ? v[b - d] : 0;
+        __syncthreads();
+        v[b] += x;
+        __syncthreads();
+    }
+    hist[kOrderBuckets + b] = v[b] - hist[b];  // exclusive start of bucket b
+}
+
+__global__ __launch_bounds__(256) void k_order_scatter(const ScanTask *__restrict__ tasks,
+                                                       const int32_t *__restrict__ n_p, int32_t *__restrict__ hist,
+                                                       int32_t *__restrict__ order) {
+    __shared__ int32_t h[kOrderBuckets], base[kOrderBuckets];
+    const int n = *n_p;
+    for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) h[b] = 0;
+        __syncthreads();
+        const int i = i0 + threadIdx.x;
+        int bk = 0, rank = 0;
+        if (i < n) {
+            bk = order_bucket(tasks[i].b - tasks[i].a);
+            rank = atomicAdd(&h[bk], 1);
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
+            if (h[b]) base[b] = atomicAdd(&hist[kOrderBuckets + b], h[b]);
+        __syncthreads();
+        if (i < n) order[base[bk] + rank] = i;
+        __syncthreads();
+    }
 }
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -696,8 +834,9 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
 
 __global__ void k_gather_roots(const Node *__restrict__ nodes, const int32_t *__restrict__ hitlist,
                                const int32_t *__restrict__ n_p, Node *__restrict__ out) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid < *n_p) out[gid] = nodes[hitlist[gid]];
+    const int n = *n_p;
+    for (int gid = blockIdx.x * blockDim.x + threadIdx.x; gid < n; gid += gridDim.x * blockDim.x)
+        out[gid] = nodes[hitlist[gid]];
 }
 
 }  // namespace
@@ -706,72 +845,101 @@ __global__ void k_gather_roots(const Node *__restrict__ nodes, const int32_t *__
 // Launchers (host side).
 // ---------------------------------------------------------------------------
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
-                               int32_t nwt, const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount,
-                               int32_t cap, int32_t *hitlist, Counters *ctr) {
+                               int32_t nwt, int32_t persistent_waves, const Params &P, const ScoreTab &tab,
+                               TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr) {
     if (nwt <= 0) return hipSuccess;
+    // persistent waves pop the task list from the counter block's queue head
+    // (zeroed by k_seed_init); otherwise one workgroup per task
+    const bool persist = persistent_waves > 0;
+    int32_t *queue = &ctr->queue;
+    const dim3 g(persist ? (unsigned)std::min(persistent_waves, nwt) : (unsigned)nwt), b(64);
     if (scan_uses_f16(P)) {
         const ScoreTab th = make_score_tab_f16(P.match, P.mismatch);
         const uint32_t oh = f16_bits_of_int(P.open), xh = f16_bits_of_int(P.ext), thr = f16_bits_of_int(P.min_score - 1);
-        if (wm <= 38)
-            hipLaunchKernelGGL(k_scan_uniform_h<38>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, th, traj, tcount,
-                               cap, hitlist, ctr, oh, xh, thr);
-        else
-            hipLaunchKernelGGL(k_scan_uniform_h<64>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, th, traj, tcount,
-                               cap, hitlist, ctr, oh, xh, thr);
-    } else if (wm <= 38) {
-        hipLaunchKernelGGL(k_scan_uniform<38>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, tab, traj, tcount,
-                           cap, hitlist, ctr);
+        auto k = wm <= 38 ? (persist ? k_scan_uniform_hp<38> : k_scan_uniform_h<38>)
+                          : (persist ? k_scan_uniform_hp<64> : k_scan_uniform_h<64>);
+        hipLaunchKernelGGL(k, g, b, 0, st, seq, pairs, wtasks, nwt, queue, P, th, traj, tcount, cap, hitlist, ctr, oh,
+                           xh, thr);
     } else {
-        hipLaunchKernelGGL(k_scan_uniform<64>, dim3(nwt), dim3(64), 0, st, seq, pairs, wtasks, P, tab, traj, tcount,
-                           cap, hitlist, ctr);
+        auto k = wm <= 38 ? (persist ? k_scan_uniform<38, true> : k_scan_uniform<38, false>)
+                          : (persist ? k_scan_uniform<64, true> : k_scan_uniform<64, false>);
+        hipLaunchKernelGGL(k, g, b, 0, st, seq, pairs, wtasks, nwt, queue, P, tab, traj, tcount, cap, hitlist, ctr);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_scan_general(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
                                const ScanTask *tasks, const int32_t *order, const int32_t *ntasks_dev,
-                               int32_t ntasks_max, const Params &P, const ScoreTab &tab, TrajEntry *traj,
-                               int32_t *tcount, int32_t cap, Counters *ctr) {
+                               int32_t ntasks_max, int32_t wave_tasks, int32_t wave_rows, int32_t lane_blocks_max,
+                               const Params &P, const ScoreTab &tab, TrajEntry *traj, int32_t *tcount, int32_t cap,
+                               Counters *ctr) {
     if (ntasks_max <= 0) return hipSuccess;
-    // the long tasks sort first: up to 1024 of them get a wave each (a wave
-    // step costs ~11x the SIMD time per cell of the lane path; 1024 measured
-    // best on the 4096-pair batch, with 256-2048 within 3 ms per step)
-    const int32_t wave_blocks = std::min<int32_t>(ntasks_max, 1024);
+    // the long tasks sort first: up to wave_tasks of them get a wave each (a
+    // wave step costs ~11x the SIMD time per cell of the lane path)
+    const int32_t wave_blocks = std::min<int32_t>(ntasks_max, wave_tasks);
+    int32_t lane_blocks = (ntasks_max + 63) / 64;
+    if (lane_blocks_max > 0) lane_blocks = std::min(lane_blocks, lane_blocks_max);
+    const dim3 g(wave_blocks + lane_blocks), b(64);
     uint32_t mis4, diff;
     if (scan_general_packed(P, mis4, diff)) {
         const uint32_t oh = f16_bits_of_int(P.open), xh = f16_bits_of_int(P.ext), thr = f16_bits_of_int(P.min_score - 1);
-        dim3 g(wave_blocks + (ntasks_max + 63) / 64), b(64);
         if (wm <= 38)
-            hipLaunchKernelGGL(k_scan_general_h<38>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, P,
-                               tab, traj, tcount, cap, ctr, oh, xh, thr, mis4, diff);
+            hipLaunchKernelGGL(k_scan_general_h<38>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks,
+                               wave_rows, P, tab, traj, tcount, cap, ctr, oh, xh, thr, mis4, diff);
         else
-            hipLaunchKernelGGL(k_scan_general_h<64>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, P,
-                               tab, traj, tcount, cap, ctr, oh, xh, thr, mis4, diff);
+            hipLaunchKernelGGL(k_scan_general_h<64>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks,
+                               wave_rows, P, tab, traj, tcount, cap, ctr, oh, xh, thr, mis4, diff);
         return hipGetLastError();
     }
-    dim3 g(wave_blocks + (ntasks_max + 63) / 64), b(64);
     if (wm <= 38)
-        hipLaunchKernelGGL(k_scan_general<38>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, P, tab,
-                           traj, tcount, cap, ctr);
+        hipLaunchKernelGGL(k_scan_general<38>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, wave_rows,
+                           P, tab, traj, tcount, cap, ctr);
     else
-        hipLaunchKernelGGL(k_scan_general<64>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, P, tab,
-                           traj, tcount, cap, ctr);
+        hipLaunchKernelGGL(k_scan_general<64>, g, b, 0, st, seq, pairs, tasks, order, ntasks_dev, wave_blocks, wave_rows,
+                           P, tab, traj, tcount, cap, ctr);
     return hipGetLastError();
 }
 
 hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs,
                           const int32_t *pair_of_window, const Params &P, const ScoreTab &tab, int level0,
                           const int32_t *hitlist, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
-                          Node *nodes, int32_t node_cap, const TrajEntry *traj, const int32_t *tcount, int32_t cap,
-                          int32_t scan_cap, ScanTask *next, int32_t next_cap, Counters *ctr) {
+                          int32_t blocks_max, Node *nodes, int32_t node_cap, const TrajEntry *traj,
+                          const int32_t *tcount, int32_t cap, int32_t scan_cap, ScanTask *next, int32_t next_cap,
+                          Counters *ctr) {
     if (n_max <= 0) return hipSuccess;
-    dim3 g((n_max + 63) / 64), b(64);
+    int32_t blocks = (n_max + 63) / 64;
+    if (blocks_max > 0) blocks = std::min(blocks, blocks_max);
+    dim3 g(blocks), b(64);
     if (wm <= 38)
         hipLaunchKernelGGL(k_resolve<38>, g, b, 0, st, seq, pairs, pair_of_window, P, tab, level0, hitlist, tasks, n_dev,
                            nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr);
     else
         hipLaunchKernelGGL(k_resolve<64>, g, b, 0, st, seq, pairs, pair_of_window, P, tab, level0, hitlist, tasks, n_dev,
                            nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_init(hipStream_t st, Counters *ctr, int32_t nw) {
+    hipLaunchKernelGGL(k_seed_init, dim3(1), dim3(64), 0, st, ctr, nw);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_begin(hipStream_t st, Counters *ctr, int32_t *hist) {
+    hipLaunchKernelGGL(k_level_begin, dim3(1), dim3(256), 0, st, ctr, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_order_tasks(hipStream_t st, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
+                              int32_t *hist, int32_t *order) {
+    if (n_max <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<int32_t>((n_max + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_order_count, dim3(blocks), dim3(256), 0, st, tasks, n_dev, hist);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kOrderBuckets), 0, st, hist);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_order_scatter, dim3(blocks), dim3(256), 0, st, tasks, n_dev, hist, order);
     return hipGetLastError();
 }
 
@@ -784,7 +952,8 @@ hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v) {
 hipError_t launch_gather_roots(hipStream_t st, const Node *nodes, const int32_t *hitlist, const int32_t *n_dev,
                                int32_t n_max, Node *out) {
     if (n_max <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_roots, dim3((n_max + 255) / 256), dim3(256), 0, st, nodes, hitlist, n_dev, out);
+    const unsigned blocks = (unsigned)std::min<int32_t>((n_max + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_gather_roots, dim3(blocks), dim3(256), 0, st, nodes, hitlist, n_dev, out);
     return hipGetLastError();
 }
 

@@ -90,10 +90,11 @@ def log_uniform_int(rng, lo, hi, size=None):
     return np.exp(rng.uniform(np.log(lo), np.log(hi), size=size)).astype(np.int64)
 
 
-def batch(seed=SEED_BASE + 1, npairs=4096, lo=1000, hi=50000, max_frag=30):
-    """List of (s1, s2) pairs."""
+def batch(seed=SEED_BASE + 1, npairs=4096, lo=1000, hi=50000, max_frag=30, meta=False):
+    """List of (s1, s2) pairs (with meta: and the number of fragments
+    planted in each pair)."""
     rng = np.random.default_rng(seed)
-    pairs = []
+    pairs, planted = [], []
     for _ in range(npairs):
         L1 = int(log_uniform_int(rng, lo, hi))
         L2 = int(log_uniform_int(rng, lo, hi))
@@ -103,7 +104,8 @@ def batch(seed=SEED_BASE + 1, npairs=4096, lo=1000, hi=50000, max_frag=30):
         nfrag = int(rng.integers(0, max_frag + 1))
         plant(rng, s1, s2, nfrag, 20, 300, float(rng.uniform(0.10, 0.25)), float(rng.uniform(0.01, 0.03)), gc)
         pairs.append((s1, s2))
-    return pairs
+        planted.append(nfrag)
+    return (pairs, planted) if meta else pairs
 
 
 def long_masked(seed=SEED_BASE + 2, L=500000, gc=0.45, masked_frac=0.30, te_len=3000, te_copies=200,

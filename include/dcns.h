@@ -95,6 +95,37 @@ const char *dcns_last_error(const dcns_ctx *ctx);
 int dcns_device_info(dcns_ctx *ctx, int32_t *cu_count, int32_t *clock_mhz, char *name, int32_t name_cap);
 int dcns_last_stats(const dcns_ctx *ctx, dcns_stats *out);
 
+/* Launch shapes of the seed phase (the batching of the reference's
+ * per-window, per-process calls; no counterpart in the reference, which runs
+ * one pair per process).  Fields at 0 take the defaults; results never
+ * depend on them, only the time does.
+ *   level0_mode   1 (default): the level-0 window scan runs one workgroup
+ *                 per 128-window task of the length-sorted task list; 2:
+ *                 persistent waves (level0_waves per CU, default 8) pop the
+ *                 list from a queue head in device memory.
+ *   wave_tasks    later levels: at most this many of the longest rescans run
+ *                 one per wave (default 1024) ...
+ *   wave_rows     ... those of more than wave_rows rows (default 4096).
+ *   graph_levels  level 0's resolve and this many recursion levels are
+ *                 enqueued with no host read between them and replayed as
+ *                 one captured HIP graph (default 4; -1: no graph, the host
+ *                 reads the level counters after every launch).  Levels past
+ *                 them run host-driven; when a pool overflows inside the
+ *                 graph, the recursion runs again host-driven with grown
+ *                 pools (kept for later calls).
+ *   pool_min      > 0: the recursion's node, trajectory and task pools start
+ *                 this many entries above the window count instead of a
+ *                 quarter of it (makes the pools grow; for tests). */
+typedef struct dcns_tuning {
+    int32_t level0_mode, level0_waves;
+    int32_t wave_tasks, wave_rows;
+    int32_t graph_levels;
+    int32_t pool_min;
+    int32_t pad[2];
+} dcns_tuning;
+int dcns_get_tuning(const dcns_ctx *ctx, dcns_tuning *out);
+int dcns_set_tuning(dcns_ctx *ctx, const dcns_tuning *in);
+
 /* Per-kernel timing (flags bit 0): total milliseconds and launch count of the
  * kernel family `name` ("scan_uniform", "scan_general", "resolve",
  * "xdrop_pass1", "xdrop_pass1_wave", "xdrop_pass2", "xdrop_pass2_wave",

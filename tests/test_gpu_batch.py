@@ -102,3 +102,46 @@ def test_batch_chunks_equal_line_by_line(tmp_path):
         assert ta == open(tmp_path / "b" / f).read(), f
         nonempty += f.endswith(".sam") and len(ta) > 0
     assert nonempty > 5
+
+
+def test_batch_bad_region_and_truncated_genome_fail_alone(tmp_path):
+    """One line's region file has a malformed header (its own process would
+    die on it) and one line's region runs past the end of a truncated genome
+    file (the index still gives the full length): the malformed line fails
+    alone, every other line's files and stdout equal this build's cut1Gap run
+    line by line, one process each (fresh genome streams per line)."""
+    rng = np.random.default_rng(23)
+    (tmp_path / "a").mkdir()
+    lines = _region_set(str(tmp_path / "a"), rng, 10)
+    a = tmp_path / "a"
+    with open(a / "r003.fa", "w") as fh:  # malformed start coordinate
+        fh.write(">ref\tspecies:spA\tchr:c0\tstrand:1\tstart:x12\tend:3000\nACGT\n"
+                 ">q0\tspecies:spB\tchr:c0\tstrand:1\tstart:1000\tend:4000\nACGT\n")
+    # the query genome loses its last 9 kb of text, its index does not
+    qg = (a / "qg.fa").read_text()
+    (a / "qg.fa").write_text(qg[:len(qg) - 9000])
+    with open(a / "r006.fa", "w") as fh:  # a query region inside the lost text
+        fh.write(">ref\tspecies:spA\tchr:c0\tstrand:1\tstart:5000\tend:9000\nACGT\n"
+                 ">q0\tspecies:spB\tchr:c0\tstrand:0\tstart:195000\tend:199000\nACGT\n"
+                 ">q1\tspecies:spC\tchr:c0\tstrand:1\tstart:2000\tend:6000\nACGT\n")
+    shutil.copytree(a, tmp_path / "b")
+    with open(a / "cmds", "w") as fh:
+        fh.write("".join("dCNS " + " ".join(l) + "\n" for l in lines))
+    print("dCNS batch with a malformed region and a truncated genome", flush=True)
+    ra = subprocess.run([dcns_amd.CLI_PATH, "batch", "cmds", "-n", "6"], cwd=a, capture_output=True, text=True,
+                        timeout=300)
+    assert ra.returncode == 1, ra.stderr
+    assert "r003.fa" in ra.stderr and "stoi" in ra.stderr
+    out_b, failed = "", []
+    for l in lines:
+        rb = subprocess.run([dcns_amd.CLI_PATH] + l, cwd=tmp_path / "b", capture_output=True, text=True, timeout=120)
+        if rb.returncode != 0:
+            failed.append(l[6])
+        out_b += rb.stdout
+    assert failed == ["r003.fa"]
+    assert ra.stdout == out_b
+    fa, fb = _outputs(a), _outputs(tmp_path / "b")
+    assert fa == fb and not [f for f in fa if f.startswith("r003")]
+    assert [f for f in fa if f.startswith("r006")] and [f for f in fa if f.startswith("r009")]
+    for f in fa:
+        assert open(a / f).read() == open(tmp_path / "b" / f).read(), f

@@ -108,3 +108,42 @@ def bind_sw_max(seqs, o1, l1, o2, l2, sc):
     from oracle import bind
     return bind.sw_max_batch(seqs, np.array(o1, np.int64), np.array(l1, np.int32), np.array(o2, np.int64),
                              np.array(l2, np.int32), sc, nthreads=8)
+
+
+@pytest.mark.parametrize("sc", synth.scoring_settings())
+def test_sw_random_config3_scorings_match_oracle(gpu, sc):
+    """The six scoring settings of the permutation benchmark at its shapes:
+    one pair per point (every lane its own lengths and composition), lengths
+    log-uniform on 38-2000 and GC uniform on 0.30-0.70, plus a few points of
+    64 pairs; scores equal the oracle's SmithWaterman on the same fragments
+    regenerated on the host."""
+    rng = np.random.default_rng(1000 + sum(abs(x) * 7 ** k for k, x in enumerate(sc)))
+    npts = 384
+    l1 = synth.log_uniform_int(rng, 38, 2000, npts).astype(np.int32)
+    l2 = synth.log_uniform_int(rng, 38, 2000, npts).astype(np.int32)
+    gc = rng.uniform(0.30, 0.70, npts)
+    seeds = rng.integers(0, 2**62, npts).astype(np.uint64)
+    print("sw random config-3 shapes", sc, npts, flush=True)
+    got = gpu.sw_random_max(sc, gc, l1, l2, seeds, 1).ravel()
+    got64 = gpu.sw_random_max(sc, gc[:4], l1[:4], l2[:4], seeds[:4], 64).ravel()
+    with np.errstate(over="ignore"):
+        seqs, o1, ln1, o2, ln2 = [], [], [], [], []
+        off = 0
+
+        def add(a, b):
+            nonlocal off
+            seqs.extend([a, b])
+            o1.append(off); ln1.append(len(a)); off += len(a)
+            o2.append(off); ln2.append(len(b)); off += len(b)
+
+        for k in range(npts):
+            add(_random_fragment(_mix64(seeds[k]), int(l1[k]), gc[k]),
+                _random_fragment(_mix64(seeds[k] + np.uint64(1)), int(l2[k]), gc[k]))
+        for k in range(4):
+            for r in range(64):
+                add(_random_fragment(_mix64(seeds[k] + np.uint64(2 * r)), int(l1[k]), gc[k]),
+                    _random_fragment(_mix64(seeds[k] + np.uint64(2 * r + 1)), int(l2[k]), gc[k]))
+    want = bind_sw_max(np.concatenate(seqs), o1, ln1, o2, ln2, sc)
+    assert np.array_equal(got, want[:npts])
+    assert np.array_equal(got64, want[npts:])
+    assert got.max() > 0

@@ -217,3 +217,105 @@ def test_long_masked_repeat_pair_vs_oracle(gpu, oracle_lib):
     assert len(got[1]) == len(want) > 50
     for g, w in zip(got[1], want):
         assert list(g[:5]) == list(w[:5]) and g[5] == w[5] and g[6] == w[6] and list(g[7]) == list(w[7])
+
+
+@pytest.mark.parametrize("pool_min", [1, 64])
+def test_recursion_pools_grow_vs_oracle(gpu, oracle_lib, pool_min):
+    """Recursion pools that start nearly empty: every resolve launch of the
+    first levels overflows its node, trajectory or task pool and is run again
+    with the pools grown, on pairs whose recursion goes deep (repeats along
+    the reference, a tandem repeat); seed sets, fragments and cell counts
+    equal the oracle's, and the same context then runs with default pools."""
+    rng = np.random.default_rng(606)
+    a, b = synth.one_pair(seed=607, L1=9000, L2=4000, nfrag=12)[:2]
+    for k in range(5):  # copies of a query stretch along the reference
+        src = int(rng.integers(0, len(b) - 300))
+        a[500 + k * 1700:800 + k * 1700] = synth.mutate(rng, b[src:src + 300], 0.04, 0.0)[:300]
+    unit = synth.random_seq(rng, 70)
+    rep = np.concatenate([synth.mutate(rng, unit, 0.05, 0.0)[:70] for _ in range(50)])
+    pairs = [(a, b), (rep, synth.mutate(rng, unit, 0.02, 0.0)), (rep, rep[:900].copy())] + _small_pairs(61, 6)
+    old = gpu.set_tuning(pool_min=pool_min)
+    try:
+        print("recursion pools from", pool_min, flush=True)
+        got = gpu.find_similar(pairs)
+        st = gpu.stats()
+        sets = gpu.seed_sets(pairs, linked=False)
+    finally:
+        gpu.set_tuning(**old)
+    cells = xcells = 0
+    for i, (x, y) in enumerate(pairs):
+        o = {}
+        want = oracle_lib.find_similar(x, y, stats=o)
+        assert got[i][0] == o["seeds"], i
+        _cmp_fragments(got[i][1], want, i)
+        assert sets[i] == oracle_lib.seed_set(x, y, linked=False), i
+        cells += o["scan_cells"]
+        xcells += o["xdrop_cells"]
+    assert st["scan_cells"] == cells and st["xdrop_cells"] == xcells
+    assert st["levels"] >= 4
+    again = gpu.find_similar(pairs)
+    assert again == got
+
+
+def _repeat_pair(rng, L1, L2):
+    """A reference of tandem copies of a 70 bp unit and a query holding a few
+    of them: every query window over a copy recurses through all the
+    reference's copies."""
+    unit = synth.random_seq(rng, 70)
+    ref = np.concatenate([synth.mutate(rng, unit, 0.04, 0.0)[:70] for _ in range(L1 // 70 + 1)])[:L1]
+    qry = synth.random_seq(rng, L2)
+    for p in range(100, L2 - 300, 700):
+        qry[p:p + 210] = np.concatenate([unit, unit, unit])
+    return ref.astype(np.int8), qry.astype(np.int8)
+
+
+def test_level_graph_replays_vs_host_driven(gpu, oracle_lib):
+    """The recursion levels replayed from a captured HIP graph (after one
+    host-driven run of a new layout), with the level-0 scan per task and as
+    persistent waves, with fewer captured levels than the recursion has,
+    against the host-driven run: identical fragments, seeds and cell counts
+    on every replay, with other device work between the replays.  Then a
+    batch of the same layout whose recursion outgrows the pools the graph
+    was captured with: the overflow inside the graph falls back to the
+    host-driven levels, and the results equal the oracle's."""
+    rng = np.random.default_rng(515)
+    a_pairs = _small_pairs(81, 10, 600, 5000, (1, 4))
+    a_pairs.append(synth.one_pair(seed=82, L1=9000, L2=3000, nfrag=6))
+    gpu.stage(a_pairs)
+    old = gpu.set_tuning(graph_levels=-1)
+    try:
+        want = gpu.find_similar_staged()
+        st0 = gpu.stats()
+        for gl, mode, waves in [(4, 1, 0), (1, 2, 4), (8, 2, 8), (2, 1, 0)]:
+            print("level graph", gl, "level-0 mode", mode, flush=True)
+            for rep in range(3):
+                gpu.set_tuning(graph_levels=gl, level0_mode=mode, level0_waves=waves)
+                got = gpu.find_similar_staged()
+                st = gpu.stats()
+                assert got == want, (gl, mode, rep)
+                for k in ("scan_cells", "gpu_scan_cells", "xdrop_cells", "seeds", "linked_seeds", "levels"):
+                    assert st[k] == st0[k], (gl, mode, rep, k)
+                gpu.set_tuning(graph_levels=-1)  # eager work between the replays
+                assert gpu.find_similar_staged() == want
+        # pools sized for a shallow recursion, then a deep one of the same layout
+        gpu.set_tuning(graph_levels=4, level0_mode=1, pool_min=64)
+        for rep in range(3):
+            assert gpu.find_similar_staged() == want
+        b_pairs = [_repeat_pair(rng, len(x), len(y)) if k % 3 == 0 else (x, y)
+                   for k, (x, y) in enumerate(a_pairs)]
+        gpu.stage(b_pairs)
+        print("deep recursion on the captured layout", flush=True)
+        got_b = gpu.find_similar_staged()
+        st_b = gpu.stats()
+        again = gpu.find_similar_staged()
+    finally:
+        gpu.set_tuning(**old)
+    assert again == got_b
+    cells = 0
+    for i, (x, y) in enumerate(b_pairs):
+        o = {}
+        w = oracle_lib.find_similar(x, y, stats=o)
+        assert got_b[i][0] == o["seeds"], i
+        _cmp_fragments(got_b[i][1], w, i)
+        cells += o["scan_cells"]
+    assert st_b["scan_cells"] == cells and st_b["levels"] >= 5
