@@ -185,6 +185,7 @@ def lambdak_phase(ctx, log):
     l1, l2, gc = l1[order], l2[order], gc[order]
     cells = float(np.sum(l1.astype(np.float64) * l2)) * len(settings)
     fits, t_scores, t_fit = [], 0.0, 0.0
+    ctx.kernel_time_reset()
     for k, sc in enumerate(settings):
         t = time.perf_counter()
         scores = lk.permutation_scores(ctx, sc, gc, l1, l2, 1, seed=300 + k).ravel()
@@ -194,9 +195,12 @@ def lambdak_phase(ctx, log):
         t_fit += time.perf_counter() - t
         fits.append({"setting": list(sc), "lambda": lam, "K": K, "rss": rss, "iterations": it,
                      "mean_score": float(scores.mean())})
+    kms, kn = ctx.kernel_time("sw_random")  # HIP events around the scoring kernel's launches
     out["config3"] = {"pairs_per_setting": n, "settings": len(settings), "cells": cells,
                       "scores_per_s": round(n * len(settings) / t_scores, 1),
-                      "gcups": round(cells / t_scores / 1e9, 2), "fit_seconds": round(t_fit, 3), "fits": fits}
+                      "gcups": round(cells / t_scores / 1e9, 2), "fit_seconds": round(t_fit, 3),
+                      "k_sw_random_ms": round(kms, 1) if kn else None,
+                      "k_sw_random_gcups": round(cells / (kms * 1e-3) / 1e9, 1) if kn else None, "fits": fits}
     log(f"config 3: {n * len(settings)} scores in {t_scores:.2f} s ({cells / t_scores / 1e9:.1f} GCUPS)")
     # the surrogate against the judge (4e4-pair permutation fits) at 40 random
     # points of each config-3 setting, and of 6 more settings from the hull
@@ -281,6 +285,62 @@ def two_piece_phase(ctx, log, cpu=True):
                       "k_sg_strip_gcups": round(st["semiglobal_cells"] / dp_ms / 1e6, 1)}
     log(f"two-piece batch of 64: {cells:.3e} cells in {dt * 1e3:.1f} ms; K4 strips "
         f"{st['semiglobal_cells'] / dp_ms / 1e6:.1f} GCUPS")
+    return out
+
+
+def long_gap_runs(frags, min_len=22):
+    """Cigar gap runs long enough that the second gap piece (-45, 0) scores
+    them above the first (-4 - 2 (L - 1) < -45 from L = 22 on)."""
+    return sum(1 for f in frags for c in f[7] if (c & 15) in (1, 2) and (c >> 4) >= min_len)
+
+
+def two_piece_config2(ctx, log, cpu=True, crop=10000, gpu_crop=120000):
+    """Overload 4 (cut2Gaps) on the long masked pair: 500 kb x 500 kb at GC
+    0.45 with 30% lowercase-n runs stripped (as cut2Gaps strips them), 200
+    copies of a 3 kb repeat at 10% divergence, 300 planted fragments 25-2,000
+    bp apart; cut2Gaps defaults.  With the second gap piece (-45, 0) the
+    z-drop never ends a SemiGlobal early, so every x-seed's reverse DP covers
+    the whole band back to the sequence start: the work grows with the cube
+    of the length (1.6e13 band cells at 160 kb).  The GPU runs the first
+    `gpu_crop` bases of both stripped sequences; the compiled reference's
+    overload 4 (one core) runs the first `crop` bases, compared with this
+    build's result on the same crop."""
+    from dcns_amd import synth
+    from oracle import bind
+    t1, t2 = synth.long_masked()
+    a = synth.encode(t1.replace("n", ""))[:gpu_crop]
+    b = synth.encode(t2.replace("n", ""))[:gpu_crop]
+    t = time.perf_counter()
+    res = ctx.find_similar_2gaps([(a, b)])[0][1]
+    dt = time.perf_counter() - t
+    st = ctx.stats()
+    cells = st["scan_cells"] + st["xdrop_cells"] + st["semiglobal_cells"]
+    out = {"workload": f"long masked pair (500 kb each, 30% n runs stripped, 200 copies of a 3 kb repeat, 300 "
+                       f"fragments 25-2,000 bp apart), the first {len(a)} x {len(b)} bases of the stripped "
+                       "sequences, cut2Gaps defaults (open2 -45, ext2 0, z-drop 50, band 20000)",
+           "reference_cells": cells, "semiglobal_cells": st["semiglobal_cells"], "seconds": round(dt, 3),
+           "gcups": round(cells / dt / 1e9, 2), "ms_seed": round(st["ms_seed"], 1),
+           "ms_semiglobal": round(st["ms_semiglobal"], 1), "fragments": len(res),
+           "long_gap_runs": long_gap_runs(res)}
+    log(f"two-piece config 2: {cells:.3e} cells in {dt:.2f} s ({cells / dt / 1e9:.1f} GCUPS), {len(res)} fragments, "
+        f"{out['long_gap_runs']} gap runs of >= 22 bp")
+    ref = bind.reference() if cpu else None
+    if ref is not None:
+        ca, cb = a[:crop], b[:crop]
+        got = ctx.find_similar_2gaps([(ca, cb)])[0][1]
+        stc = ctx.stats()
+        ccells = stc["scan_cells"] + stc["xdrop_cells"] + stc["semiglobal_cells"]
+        t = time.perf_counter()
+        want = ref.find_similar_2gaps(ca, cb)
+        tc = time.perf_counter() - t
+        out["cpu_reference"] = {"sample": f"the first {crop} bases of both stripped sequences", "seconds": round(tc, 2),
+                                "reference_cells": ccells, "gcups": round(ccells / tc / 1e9, 4), "cores": 1,
+                                "kind": "reference"}
+        out["crop_fragments"] = len(got)
+        out["crop_long_gap_runs"] = long_gap_runs(got)
+        out["bit_exact"] = [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in got] == \
+                           [tuple(f[:5]) + (f[5], f[6], list(f[7])) for f in want]
+        log(f"two-piece config 2 crop: reference {tc:.1f} s, bit_exact {out['bit_exact']}")
     return out
 
 
@@ -702,6 +762,7 @@ def main():
         ctx = dcns_amd.Context(0, timing=True)
         lg = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
         out = {"two_piece": two_piece_phase(ctx, lg, cpu=not args.no_cpu)}
+        out["two_piece"]["config2"] = two_piece_config2(ctx, lg, cpu=not args.no_cpu)
         if not args.no_weighted:
             out["weighted"] = weighted_phase(ctx, lg, cpu=not args.no_cpu)
         out["seeded"] = seeded_phase(ctx, lg, cpu=not args.no_cpu)
@@ -811,6 +872,7 @@ def main():
 
     if not args.no_two_piece:
         line["two_piece"] = two_piece_phase(ctx, log)
+        line["two_piece"]["config2"] = two_piece_config2(ctx, log, cpu=not args.no_cpu)
     if not args.no_weighted:
         line["weighted"] = weighted_phase(ctx, log)
     if not args.no_two_piece:

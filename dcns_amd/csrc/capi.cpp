@@ -213,14 +213,9 @@ int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t misma
             std::memcpy(&host[(size_t)o], b->seqs + b->off2[i], (size_t)l2[(size_t)i]);
             o += (l2[(size_t)i] + 47) / 16 * 16;  // query stripes read 32 bytes from a multiple of 32
         }
-        // cost order: pairs of similar stripe counts and lengths share waves
-        std::vector<int32_t> order((size_t)n);
-        for (int32_t i = 0; i < n; ++i) order[(size_t)i] = i;
-        std::sort(order.begin(), order.end(), [&](int32_t a, int32_t bb) {
-            const int64_t ca = (int64_t)((l2[(size_t)a] + 31) / 32) * l1[(size_t)a];
-            const int64_t cb = (int64_t)((l2[(size_t)bb] + 31) / 32) * l1[(size_t)bb];
-            return ca != cb ? ca > cb : a < bb;
-        });
+        // lane plan: waves of one loop shape, longest first
+        SwPlan plan;
+        sw_plan(l1.data(), l2.data(), n, 1, plan);
         auto up = [&](DBuf &d, const void *src, size_t bytes) {
             d.ensure(std::max<size_t>(bytes, 64));
             hip_check(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
@@ -231,29 +226,23 @@ int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t misma
         up(c->sw_len1, l1.data(), l1.size() * 4);
         up(c->sw_off2, o2.data(), o2.size() * 8);
         up(c->sw_len2, l2.data(), l2.size() * 4);
-        up(c->sw_order, order.data(), order.size() * 4);
+        up(c->sw_order, plan.order.data(), plan.order.size() * 4);
         c->sw_out.ensure((size_t)n * 12 + 64);
-        // boundary scratch: chunks of the ordered list with a per-chunk stride
-        const int64_t budget = (int64_t)1 << 30;
-        int32_t s = 0;
-        while (s < n) {
-            int32_t maxl1 = 1, e = s;
-            while (e < n) {
-                const int32_t l = std::max(l1[(size_t)order[(size_t)e]], 1);
-                const int32_t m = std::max(maxl1, l);
-                if ((int64_t)(e - s + 1) * (m + 16) * 8 > budget && e > s) break;
-                maxl1 = m;
-                ++e;
-            }
-            const int64_t stride = (int64_t)(maxl1 + 15) / 16 * 16 * 2;  // int2 per row, rows rounded to 16
-            c->sw_bound.ensure((size_t)(e - s) * (size_t)stride * 4 + 64);
+        // boundary rows: launches of whole waves within 1 GiB
+        const int64_t budget = ((int64_t)1 << 30) / 4;
+        std::vector<int64_t> woff;
+        for (int64_t w0 = 0; w0 < (int64_t)plan.waves.size();) {
+            const int64_t w1 = sw_plan_chunk(plan, w0, budget, woff);
+            c->sw_bound.ensure((size_t)woff.back() * 4 + 64);
+            up(c->sw_woff, woff.data(), woff.size() * 8);
+            const int32_t s = (int32_t)(64 * w0), m = (int32_t)std::min<int64_t>(64 * w1, n) - s;
             c->launch("sw_batch", [&] {
                 return launch_sw_batch(c->stream, c->seq.as<int8_t>(), c->sw_off1.as<int64_t>(), c->sw_len1.as<int32_t>(),
                                        c->sw_off2.as<int64_t>(), c->sw_len2.as<int32_t>(),
-                                       c->sw_order.as<int32_t>() + s, e - s, P, tab, c->sw_out.as<int32_t>(),
-                                       c->sw_bound.as<int32_t>(), stride);
+                                       c->sw_order.as<int32_t>() + s, m, P, tab, c->sw_out.as<int32_t>(),
+                                       c->sw_bound.as<int32_t>(), c->sw_woff.as<int64_t>());
             });
-            s = e;
+            w0 = w1;
         }
         hip_check(hipMemcpyAsync(out3, c->sw_out.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream), "download");
         c->sync();
@@ -301,18 +290,28 @@ int dcns_sw_random_max(dcns_ctx *c, int32_t match, int32_t mismatch, int32_t ope
         up(c->rnd_len1, len1, (size_t)npoints * 4);
         up(c->rnd_len2, len2, (size_t)npoints * 4);
         up(c->rnd_seed, seed, (size_t)npoints * 8);
+        // lane plan: waves of one loop shape, longest first; boundary rows
+        // in launches of whole waves within 4 GiB
+        SwPlan plan;
+        sw_plan(len1, len2, npoints, pairs_per_point, plan);
+        up(c->rnd_order, plan.order.data(), plan.order.size() * 4);
         const int64_t total = (int64_t)npoints * pairs_per_point;
         c->rnd_out.ensure((size_t)total * 4 + 64);
-        const int64_t stride = (int64_t)(maxl1 + 15) / 16 * 16 * 2;
-        const int64_t chunk = std::max<int64_t>(64, (((int64_t)1 << 31) / (stride * 4)) / 64 * 64);
-        c->sw_bound.ensure((size_t)std::min(chunk, total) * (size_t)stride * 4 + 64);
-        for (int64_t first = 0; first < total; first += chunk) {
-            const int64_t n = std::min(chunk, total - first);
+        const int64_t budget = ((int64_t)4 << 30) / 4;
+        std::vector<int64_t> woff;
+        for (int64_t w0 = 0; w0 < (int64_t)plan.waves.size();) {
+            const int64_t w1 = sw_plan_chunk(plan, w0, budget, woff);
+            c->sw_bound.ensure((size_t)woff.back() * 4 + 64);
+            up(c->sw_woff, woff.data(), woff.size() * 8);
+            const int64_t first = 64 * w0, n = std::min<int64_t>(64 * w1, total) - first;
             c->launch("sw_random", [&] {
                 return launch_sw_random(c->stream, c->rnd_thr.as<uint32_t>(), c->rnd_len1.as<int32_t>(),
-                                        c->rnd_len2.as<int32_t>(), c->rnd_seed.as<uint64_t>(), pairs_per_point, first,
-                                        n, P, tab, c->rnd_out.as<int32_t>(), c->sw_bound.as<int32_t>(), stride);
+                                        c->rnd_len2.as<int32_t>(), c->rnd_seed.as<uint64_t>(),
+                                        c->rnd_order.as<int32_t>(), plan.slot_order, pairs_per_point, first, n, P,
+                                        tab, c->rnd_out.as<int32_t>(), c->sw_bound.as<int32_t>(),
+                                        c->sw_woff.as<int64_t>());
             });
+            w0 = w1;
         }
         hip_check(hipMemcpyAsync(out, c->rnd_out.p, (size_t)total * 4, hipMemcpyDeviceToHost, c->stream), "download");
         c->sync();

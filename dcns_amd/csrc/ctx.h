@@ -137,7 +137,7 @@ struct dcns_ctx {
     // reusable device buffers
     dcns::DBuf seq, pairs, wtasks, pow, traj, tcount, hitlist, ctr, nodes, roots, tasks_a, tasks_b;
     dcns::DBuf xtasks, xout, tb_off, tb, cigar, cig_host_dummy;
-    dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound;
+    dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound, sw_woff;
     dcns::DBuf sg_tasks, sg_out, sg_scr, sg_tb, sg_cig, sg_strips, sg_sync;  // two-piece semi-global tasks
     dcns::DBuf sg_wcat, sg_wcat_off, sg_wtab;  // weighted DP: per-position profile indices and the profiles
     dcns::DBuf xw_scr, xw_off, xw_wmax;                  // x-drop tasks with bands past the wave kernel's
@@ -145,7 +145,7 @@ struct dcns_ctx {
     dcns::DBuf xorder;                                   // x-drop pass-2 order
     dcns::DBuf xlist;                                    // x-drop tasks handed from the lane to the wave kernel
     dcns::DBuf cig_scan, cig_packed;                     // cigar packing: counts / offsets / scan storage, runs
-    dcns::DBuf rnd_thr, rnd_len1, rnd_len2, rnd_seed, rnd_out;  // random fragment pairs generated on the device
+    dcns::DBuf rnd_thr, rnd_len1, rnd_len2, rnd_seed, rnd_out, rnd_order;  // random fragment pairs generated on the device
     dcns::DBuf mk_seq, mk_off, mk_keys, mk_desc, mk_gbytes, mk_ghash, mk_gtab;  // maskGenome: genome, k-mer set
     dcns::DBuf mk_mask, mk_ranges, mk_boff, mk_body;                          // maskGenome: mask, CDS, FASTA bodies
     dcns::HBuf h_hits, h_roots, h_nodes, h_xout, h_cigar;  // pinned download buffers

@@ -98,14 +98,33 @@ hipError_t launch_xdrop_wide_pass2(hipStream_t st, const int8_t *seq, const Pair
                                    uint32_t *cigar, int32_t *scr, const int64_t *scr_off);
 
 // swbatch.hip: SmithWaterman max score and first position for arbitrary pairs.
+// Lane plan of a K1 launch sequence over npoints points of per_point pairs
+// (pair = point * per_point + r): the waves of 64 lane slots share their loop
+// shape and run longest first.  slot_order: order[slot] is the pair; else
+// order lists the points (pairs of a point adjacent).  waves[w]: boundary
+// rows per lane of wave w.
+struct SwPlan {
+    std::vector<int32_t> order;
+    std::vector<int32_t> waves;
+    bool slot_order = false;
+    int32_t per_point = 1;
+    int64_t total = 0;
+};
+void sw_plan(const int32_t *len1, const int32_t *len2, int64_t npoints, int32_t per_point, SwPlan &plan);
+// The waves [w0, return) that fit `budget` int32 of boundary rows (at least
+// one); wave_off gets their offsets relative to the first (one more entry).
+int64_t sw_plan_chunk(const SwPlan &plan, int64_t w0, int64_t budget, std::vector<int64_t> &wave_off);
+// order[0, n) are the pairs of the launch's slots; wave_off as sw_plan_chunk.
 hipError_t launch_sw_batch(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
                            const int64_t *off2, const int32_t *len2, const int32_t *order, int32_t n,
-                           const Params &P, const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride);
-// Max scores of random fragment pairs generated on the device (pairs [first, first + n)).
+                           const Params &P, const ScoreTab &tab, int32_t *out, int32_t *bound,
+                           const int64_t *wave_off);
+// Max scores of random fragment pairs generated on the device: slots
+// [first, first + n) of the plan (order, slot_order) with the launch's wave_off.
 hipError_t launch_sw_random(hipStream_t st, const uint32_t *thr, const int32_t *len1, const int32_t *len2,
-                            const uint64_t *seed, int32_t per_point, int64_t first, int64_t n, const Params &P,
-                            const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride);
-
+                            const uint64_t *seed, const int32_t *order, bool slot_order, int32_t per_point,
+                            int64_t first, int64_t n, const Params &P, const ScoreTab &tab, int32_t *out,
+                            int32_t *bound, const int64_t *wave_off);
 // semiglobal.hip: two-piece semi-global DP, one wave per 256-column strip.
 int64_t sg_trace_bytes(int32_t R, int32_t C, int32_t w);   // traceback store of one task
 int64_t sg_scratch_ints(int32_t R, int32_t C, int32_t w);  // row results + strip boundaries of one task

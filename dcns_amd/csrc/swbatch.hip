@@ -18,6 +18,8 @@
 // that lanes of one wave run similar loop counts.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dp_common.h"
 #include "kernels.h"
 
@@ -32,15 +34,27 @@ constexpr int RB = 16;  // reference rows per block: one 16-byte load of codes a
 // past their end; the boundary array holds one int2 (H, clamped horizontal
 // gap) per reference row, 16-byte aligned, rows rounded up to RB.
 // Codes from memory: reference rows 16 at a time, query stripes 32 at a time.
+// Rows at or past the lane's own length are not read (the wave runs its
+// widest lane's rectangle; those rows and columns are N).
 struct MemCodes {
     const int8_t *ref, *qry;
+    int32_t L1, L2;
     __device__ __forceinline__ void rows16(int32_t ib, uint32_t w[4]) const {
-        const uint4 v = *(const uint4 *)(ref + ib);
-        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        if (ib < L1) {
+            const uint4 v = *(const uint4 *)(ref + ib);
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+            w[0] = w[1] = w[2] = w[3] = 0x02020202u;
+        }
     }
     __device__ __forceinline__ void cols32(int32_t j0, uint32_t w[8]) const {
-        const uint4 a = *(const uint4 *)(qry + j0), b = *(const uint4 *)(qry + j0 + 16);
-        w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+        if (j0 < L2) {
+            const uint4 a = *(const uint4 *)(qry + j0), b = *(const uint4 *)(qry + j0 + 16);
+            w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) w[q] = 0x02020202u;
+        }
     }
 };
 
@@ -74,6 +88,19 @@ struct RandCodes {
     __device__ __forceinline__ void cols32(int32_t j0, uint32_t w[8]) const { fill(kqry, j0, w, 8); }
 };
 
+// Largest value of v over the wave's active lanes, wave-uniform.
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) v = max(v, __shfl_xor(v, d, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// The lanes of a wave run the same loops: the widest lane's stripes and rows
+// (scalar loop control); a lane's rows and columns past its own lengths are
+// N, which scores 0 against everything, so no cell there exceeds the best of
+// the cells before it and the maximum and its first position are unchanged.
+// The boundary slot of every lane must hold the wave's rows; a lane without
+// a pair (bnd null, lengths 0) takes part in the loops but touches no memory.
 template <class Codes>
 __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &P, const uint32_t *shi,
                         const uint32_t *slo, int2 *__restrict__ bnd, int32_t &best, int32_t &bi, int32_t &bj) {
@@ -81,7 +108,8 @@ __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &
     bi = -1;
     bj = -1;
     const int32_t O = P.open, X = P.ext;
-    const int nstripes = (L2 + S - 1) / S;
+    const int32_t L1w = wave_max(L1);
+    const int nstripes = wave_max((L2 + S - 1) / S);
     for (int s = 0; s < nstripes; ++s) {
         const int j0 = s * S;
         uint32_t sel[S];
@@ -97,14 +125,14 @@ __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &
                 E[j] = 0;
             }
         }
-        const bool has_left = s > 0, has_right = j0 + S < L2;
+        const bool has_left = s > 0, has_right = s + 1 < nstripes;
         int32_t sbest = 0, si = -1, sj = -1;
         int32_t hb_prev = 0;  // boundary H of the previous row (diagonal of column j0)
-        for (int32_t ib = 0; ib < L1; ib += RB) {
+        for (int32_t ib = 0; ib < L1w; ib += RB) {
             uint32_t rw[4];
             src.rows16(ib, rw);
             int2 b[RB];
-            if (has_left) {
+            if (has_left && bnd) {
 #pragma unroll
                 for (int q = 0; q < RB / 2; ++q) {
                     const int4 v = *(const int4 *)(bnd + ib + 2 * q);
@@ -115,11 +143,11 @@ __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &
 #pragma unroll
                 for (int r = 0; r < RB; ++r) b[r] = make_int2(0, 0);
             }
-            const int rows = min(RB, L1 - ib);
+            const int rows = min(RB, L1w - ib);  // wave-uniform
 #pragma unroll
             for (int r = 0; r < RB; ++r) {
                 if (r < rows) {
-                    const int c = (int)(int8_t)((rw[r >> 2] >> (8 * (r & 3))) & 0xffu);
+                    const int c = ib + r < L1 ? (int)(int8_t)((rw[r >> 2] >> (8 * (r & 3))) & 0xffu) : 2;
                     const int ci = (unsigned)c < 8u ? c : 2;
                     const uint32_t thi = shi[ci], tlo = slo[ci];
                     int32_t left = b[r].x, fc = b[r].y;
@@ -152,7 +180,7 @@ __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &
                     }
                 }
             }
-            if (has_right) {
+            if (has_right && bnd) {
 #pragma unroll
                 for (int q = 0; q < RB / 2; ++q)
                     *(int4 *)(bnd + ib + 2 * q) = make_int4(b[2 * q].x, b[2 * q].y, b[2 * q + 1].x, b[2 * q + 1].y);
@@ -166,24 +194,51 @@ __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &
     }
 }
 
+// Pairs order[gid] (or first + gid without an order) for gid < n.  With
+// max_rows > 0, a pair of more reference rows than its boundary slot holds
+// is skipped and flagged in *err (bit 0), its outputs left as they are.
+// A lane's boundary rows: with per-wave offsets (wave_off[w] .. wave_off[w+1]
+// int32 of wave w = blockIdx.x, relative to the launch, split evenly among its
+// 64 lanes), else gid * stride.
+__device__ __forceinline__ int2 *boundary_slot(int32_t *bound, int64_t stride, const int64_t *wave_off, int64_t gid) {
+    if (!wave_off) return (int2 *)(bound + gid * stride);
+    const int64_t a = wave_off[blockIdx.x], per_lane = (wave_off[blockIdx.x + 1] - a) / 64;
+    return (int2 *)(bound + a + (int64_t)threadIdx.x * per_lane);
+}
+
 __global__ __launch_bounds__(64) void k_sw_batch(const int8_t *__restrict__ seq, const int64_t *__restrict__ off1,
                                                  const int32_t *__restrict__ len1, const int64_t *__restrict__ off2,
                                                  const int32_t *__restrict__ len2, const int32_t *__restrict__ order,
-                                                 int32_t n, Params P, ScoreTab tab, int32_t *__restrict__ out,
-                                                 int32_t *__restrict__ bound, int64_t bound_stride) {
+                                                 int32_t first, int32_t n, Params P, ScoreTab tab,
+                                                 int32_t *__restrict__ out, int32_t *__restrict__ bound,
+                                                 int64_t bound_stride, const int64_t *__restrict__ wave_off,
+                                                 int32_t max_rows, int32_t *__restrict__ err) {
     __shared__ uint32_t shi[8], slo[8];
     if (threadIdx.x < 8) {
         shi[threadIdx.x] = tab.hi[threadIdx.x];
         slo[threadIdx.x] = tab.lo[threadIdx.x];
     }
     __syncthreads();
+    // every lane of the wave stays for the wave-uniform loops of sw_pair
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n) return;
-    const int p = order ? order[gid] : gid;
-    const int32_t L1 = len1[p], L2 = len2[p];
-    int2 *bnd = (int2 *)(bound + (int64_t)gid * bound_stride);
+    bool ok = gid < n;
+    int p = 0;
+    int32_t L1 = 0, L2 = 0;
+    if (ok) {
+        p = order ? order[gid] : first + gid;
+        L1 = len1[p];
+        L2 = len2[p];
+        if (max_rows > 0 && (L1 > max_rows || L1 < 0 || L2 < 0)) {
+            atomicOr(err, 1);
+            ok = false;
+            L1 = L2 = 0;
+        }
+    }
+    int2 *bnd = ok ? boundary_slot(bound, bound_stride, wave_off, gid) : nullptr;
     int32_t best, bi, bj;
-    sw_pair(MemCodes{seq + off1[p], seq + off2[p]}, L1, L2, P, shi, slo, bnd, best, bi, bj);
+    sw_pair(MemCodes{ok ? seq + off1[p] : seq, ok ? seq + off2[p] : seq, L1, L2}, L1, L2, P, shi, slo, bnd, best, bi,
+            bj);
+    if (!ok) return;
     out[3 * p + 0] = best;
     out[3 * p + 1] = bi;
     out[3 * p + 2] = bj;
@@ -193,52 +248,156 @@ __global__ __launch_bounds__(64) void k_sw_batch(const int8_t *__restrict__ seq,
 // point k has lengths len1[k] x len2[k] and base thresholds thr[3k..3k+2]
 // (C, G, A cumulative, out of 2^32); its keys are mix64(seed[k] + 2r)
 // (reference) and mix64(seed[k] + 2r + 1) (query).
-// Lanes of a wave share a point, so their loops are uniform.
+// The lane slots take the pairs in the host's plan: order[slot] is the pair
+// (slot_order) or order[slot / per_point] the point (pairs of a point
+// adjacent), so that the lanes of a wave share their loop shape and the
+// launch's waves come longest first.
 __global__ __launch_bounds__(64) void k_sw_random(const uint32_t *__restrict__ thr, const int32_t *__restrict__ len1,
                                                   const int32_t *__restrict__ len2, const uint64_t *__restrict__ seed,
+                                                  const int32_t *__restrict__ order, int32_t slot_order,
                                                   int32_t per_point, int64_t first, int64_t n, Params P, ScoreTab tab,
                                                   int32_t *__restrict__ out, int32_t *__restrict__ bound,
-                                                  int64_t bound_stride) {
+                                                  int64_t bound_stride, const int64_t *__restrict__ wave_off) {
     __shared__ uint32_t shi[8], slo[8];
     if (threadIdx.x < 8) {
         shi[threadIdx.x] = tab.hi[threadIdx.x];
         slo[threadIdx.x] = tab.lo[threadIdx.x];
     }
     __syncthreads();
+    // every lane of the wave stays for the wave-uniform loops of sw_pair
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n) return;
-    const int64_t pair = first + gid;
-    const int64_t k = pair / per_point, r = pair % per_point;
+    const bool ok = gid < n;
+    const int64_t slot = first + (ok ? gid : 0);
+    int64_t k, r;
+    if (order && slot_order) {
+        const int64_t q = order[slot];
+        k = q / per_point;
+        r = q % per_point;
+    } else {
+        k = order ? (int64_t)order[slot / per_point] : slot / per_point;
+        r = slot % per_point;
+    }
+    const int64_t pair = k * per_point + r;
     RandCodes src;
     src.kref = mix64(seed[k] + 2 * (uint64_t)r);
     src.kqry = mix64(seed[k] + 2 * (uint64_t)r + 1);
     src.tC = thr[3 * k];
     src.tG = thr[3 * k + 1];
     src.tA = thr[3 * k + 2];
-    int2 *bnd = (int2 *)(bound + gid * bound_stride);
+    int2 *bnd = ok ? boundary_slot(bound, bound_stride, wave_off, gid) : nullptr;
     int32_t best, bi, bj;
-    sw_pair(src, len1[k], len2[k], P, shi, slo, bnd, best, bi, bj);
-    out[pair] = best;
+    sw_pair(src, ok ? len1[k] : 0, ok ? len2[k] : 0, P, shi, slo, bnd, best, bi, bj);
+    if (ok) out[pair] = best;
 }
 
 }  // namespace
 
 hipError_t launch_sw_batch(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
                            const int64_t *off2, const int32_t *len2, const int32_t *order, int32_t n, const Params &P,
-                           const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride) {
+                           const ScoreTab &tab, int32_t *out, int32_t *bound, const int64_t *wave_off) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sw_batch, dim3((n + 63) / 64), dim3(64), 0, st, seq, off1, len1, off2, len2, order, n, P, tab,
-                       out, bound, bound_stride);
+    hipLaunchKernelGGL(k_sw_batch, dim3((n + 63) / 64), dim3(64), 0, st, seq, off1, len1, off2, len2, order, 0, n, P,
+                       tab, out, bound, (int64_t)0, wave_off, 0, nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_sw_random(hipStream_t st, const uint32_t *thr, const int32_t *len1, const int32_t *len2,
-                            const uint64_t *seed, int32_t per_point, int64_t first, int64_t n, const Params &P,
-                            const ScoreTab &tab, int32_t *out, int32_t *bound, int64_t bound_stride) {
+                            const uint64_t *seed, const int32_t *order, bool slot_order, int32_t per_point,
+                            int64_t first, int64_t n, const Params &P, const ScoreTab &tab, int32_t *out,
+                            int32_t *bound, const int64_t *wave_off) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sw_random, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, thr, len1, len2, seed, per_point,
-                       first, n, P, tab, out, bound, bound_stride);
+    hipLaunchKernelGGL(k_sw_random, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, thr, len1, len2, seed, order,
+                       (int32_t)slot_order, per_point, first, n, P, tab, out, bound, (int64_t)0, wave_off);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Lane plans of the K1 launches.
+// ---------------------------------------------------------------------------
+void sw_plan(const int32_t *len1, const int32_t *len2, int64_t npoints, int32_t per_point, SwPlan &plan) {
+    plan.order.clear();
+    plan.waves.clear();
+    plan.per_point = per_point;
+    plan.total = npoints * per_point;
+    auto blk = [&](int64_t k) { return (std::max(len1[k], 1) + 15) / 16; };
+    auto str = [&](int64_t k) { return (std::max(len2[k], 1) + 31) / 32; };
+    plan.slot_order = per_point < 64;
+    if (plan.slot_order) {
+        // pairs by loop shape (stripes, then 16-row blocks, both descending):
+        // two stable counting sorts, then the pairs of each point in turn
+        int32_t max_blk = 1, max_str = 1;
+        for (int64_t k = 0; k < npoints; ++k) {
+            max_blk = std::max(max_blk, blk(k) + 1);
+            max_str = std::max(max_str, str(k) + 1);
+        }
+        auto counting = [&](const std::vector<int32_t> &in, std::vector<int32_t> &out, auto key, int32_t range) {
+            std::vector<int64_t> start((size_t)range + 1, 0);
+            for (int32_t k : in) ++start[(size_t)(range - 1 - key(k)) + 1];
+            for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+            out.resize(in.size());
+            for (int32_t k : in) out[(size_t)start[(size_t)(range - 1 - key(k))]++] = k;
+        };
+        std::vector<int32_t> ident((size_t)npoints), by_blk, by_shape;
+        for (int64_t k = 0; k < npoints; ++k) ident[(size_t)k] = (int32_t)k;
+        counting(ident, by_blk, blk, max_blk);
+        counting(by_blk, by_shape, str, max_str);
+        std::vector<int32_t> slots;
+        slots.reserve((size_t)plan.total);
+        for (int32_t k : by_shape)
+            for (int32_t r = 0; r < per_point; ++r) slots.push_back(k * per_point + r);
+        // waves of 64 slots, their loops the widest lane's; the waves longest first
+        const int64_t nw = (plan.total + 63) / 64;
+        std::vector<int64_t> cost((size_t)nw);
+        std::vector<int32_t> rows((size_t)nw), widx((size_t)nw);
+        for (int64_t w = 0; w < nw; ++w) {
+            int32_t b = 1, t = 1;
+            for (int64_t s = 64 * w; s < std::min(plan.total, 64 * w + 64); ++s) {
+                b = std::max(b, blk(slots[(size_t)s] / per_point));
+                t = std::max(t, str(slots[(size_t)s] / per_point));
+            }
+            cost[(size_t)w] = (int64_t)b * t;
+            rows[(size_t)w] = 16 * b;
+            widx[(size_t)w] = (int32_t)w;
+        }
+        // a partial last wave stays last, so that every wave before it is full
+        if (plan.total % 64) cost[(size_t)(nw - 1)] = -1;
+        std::stable_sort(widx.begin(), widx.end(), [&](int32_t a, int32_t b) { return cost[(size_t)a] > cost[(size_t)b]; });
+        plan.order.reserve((size_t)plan.total);
+        for (int32_t w : widx) {
+            const int64_t s0 = 64 * (int64_t)w, s1 = std::min(plan.total, s0 + 64);
+            for (int64_t s = s0; s < s1; ++s) plan.order.push_back(slots[(size_t)s]);
+            plan.waves.push_back(rows[(size_t)w]);
+        }
+    } else {
+        // points longest first; a wave holds at most two points' pairs
+        plan.order.resize((size_t)npoints);
+        for (int64_t k = 0; k < npoints; ++k) plan.order[(size_t)k] = (int32_t)k;
+        std::stable_sort(plan.order.begin(), plan.order.end(), [&](int32_t a, int32_t b) {
+            return (int64_t)blk(a) * str(a) > (int64_t)blk(b) * str(b);
+        });
+        const int64_t nw = (plan.total + 63) / 64;
+        plan.waves.resize((size_t)nw);
+        for (int64_t w = 0; w < nw; ++w) {
+            const int64_t a = plan.order[(size_t)(64 * w / per_point)];
+            const int64_t b = plan.order[(size_t)(std::min(plan.total - 1, 64 * w + 63) / per_point)];
+            plan.waves[(size_t)w] = 16 * std::max(blk(a), blk(b));
+        }
+    }
+}
+
+// Launches [w0, w1) of the plan's waves fit `budget` int32 of boundary rows
+// (at least one wave).
+int64_t sw_plan_chunk(const SwPlan &plan, int64_t w0, int64_t budget, std::vector<int64_t> &wave_off) {
+    wave_off.assign(1, 0);
+    int64_t w = w0;
+    const int64_t nw = (int64_t)plan.waves.size();
+    while (w < nw) {
+        const int64_t ints = (int64_t)plan.waves[(size_t)w] * 2 * 64;
+        if (w > w0 && wave_off.back() + ints > budget) break;
+        wave_off.push_back(wave_off.back() + ints);
+        ++w;
+    }
+    return w;
 }
 
 }  // namespace dcns

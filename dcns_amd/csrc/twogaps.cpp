@@ -15,6 +15,8 @@
 // fragment replace/append pass (host, in x-seed order).  The host passes keep
 // the reference's sequential semantics; only independent DPs are batched.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <map>
@@ -43,6 +45,13 @@ void validate_two_piece(const dcns_params &p) {
 
 }  // namespace
 
+// DCNS_PROGRESS in the environment: the two-piece path reports its task
+// lists and every SemiGlobal launch chunk on stderr as they finish.
+bool progress() {
+    static const bool on = std::getenv("DCNS_PROGRESS") != nullptr;
+    return on;
+}
+
 void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &tasks, bool trace,
                       std::vector<SgResult> &out, int64_t &cells, const SgWeights *Wp) {
     const SgWeights W = Wp ? *Wp : SgWeights{nullptr, nullptr, nullptr, 0, 0};
@@ -61,18 +70,31 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     const int64_t budget = std::max<int64_t>(std::min<int64_t>((int64_t)(free_b / 2), (int64_t)96 << 30),
                                              (int64_t)1 << 30);
+    // a chunk also stops at this many band cells (a few seconds of DP), so
+    // that very large task lists finish chunk by chunk
+    const int64_t cell_budget = (int64_t)2 << 40;
+    int64_t band_total = 0;
+    for (int32_t i : idx) band_total += sg_trace_bytes(tasks[(size_t)i].R, tasks[(size_t)i].C, P.bandwidth);
+    if (progress())
+        std::fprintf(stderr, "SemiGlobal%s: %zu tasks, %.3e band cells\n", trace ? " with traceback" : "", idx.size(),
+                     (double)band_total);
+    int64_t band_done = 0;
     size_t pos = 0;
     while (pos < idx.size()) {
         // one chunk: tasks whose traceback stores and scratch fit the budget
         std::vector<SgTask> ct;
         std::vector<int32_t> which;
         std::vector<SgOut> so;
-        int64_t tb_total = 0, scr_total = 0, cig_total = 0;
+        int64_t tb_total = 0, scr_total = 0, cig_total = 0, band_chunk = 0;
+        const double tc0 = ms_now();
         while (pos < idx.size()) {
             SgTask t = tasks[(size_t)idx[pos]];
-            const int64_t tbb = trace ? (sg_trace_bytes(t.R, t.C, P.bandwidth) + 255) / 256 * 256 : 0;
+            const int64_t band = sg_trace_bytes(t.R, t.C, P.bandwidth);
+            const int64_t tbb = trace ? (band + 255) / 256 * 256 : 0;
             const int64_t scr = sg_scratch_ints(t.R, t.C, P.bandwidth);
-            if (!ct.empty() && tb_total + tbb + 4 * (scr_total + scr) > budget) break;
+            if (!ct.empty() && (tb_total + tbb + 4 * (scr_total + scr) > budget || band_chunk + band > cell_budget))
+                break;
+            band_chunk += band;
             if (ct.empty() && tbb + 4 * scr > budget)
                 throw Error(DCNS_ERR_CAPACITY, "a SemiGlobal task's traceback store exceeds device memory");
             t.tb = trace ? tb_total : -1;
@@ -139,6 +161,10 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
         }
         c->sync();
         if (stalled) throw Error(DCNS_ERR_INTERNAL, "SemiGlobal strip pipeline: a strip waited too long for its predecessor");
+        band_done += band_chunk;
+        if (progress())
+            std::fprintf(stderr, "  chunk of %d tasks: %.3e band cells in %.1f ms (%.3e of %.3e done)\n", m,
+                         (double)band_chunk, ms_now() - tc0, (double)band_done, (double)band_total);
         for (int32_t k = 0; k < m; ++k) {
             const SgOut &o = so[(size_t)k];
             SgResult &r = out[(size_t)which[(size_t)k]];
@@ -219,6 +245,12 @@ void two_gaps_flow(dcns_ctx *c, Staged &S, const dcns_params &p, const SgWeights
         }
     });
     const double t2 = ms_now();
+    if (progress()) {
+        size_t nx = 0;
+        for (const auto &v : xs) nx += v.size();
+        std::fprintf(stderr, "two-piece: %zu linked seeds, %zu x-seeds after the p-value filter and merge (%.0f ms)\n",
+                     xt.size(), nx, t2 - t0);
+    }
     // overload 2: reverse SemiGlobal (no traceback) from every x-seed's end
     std::vector<SgTask> rt;
     std::vector<int64_t> xfirst((size_t)npairs + 1, 0);

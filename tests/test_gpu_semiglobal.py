@@ -167,3 +167,33 @@ def test_xdrop_wide_bands_vs_oracle(gpu, oracle_lib):
         for (a, b), r in zip(pairs, res):
             want = oracle_lib.find_similar(a, b, sc=sc, min_score=30)
             _cmp_frags(r[1], want)
+
+
+def test_two_piece_long_masked_vs_fixture(gpu):
+    """Overload 4 on a long-masked pair of the benchmark's shape (3 kb repeat
+    copies at 10% divergence, fragments 25-2,000 bp apart, n runs stripped)
+    at 13-14 kb: fragments and reference cell counts equal the oracle's
+    (tests/golden/two_piece_masked.json, checked against the compiled
+    reference when written), and the alignments take the second gap piece:
+    gap runs of 22 bp or more, which -4 / -2 alone scores below -45."""
+    import importlib.util
+    import json
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(here, "make_two_piece_masked.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    fx = json.load(open(os.path.join(here, "two_piece_masked.json")))
+    a, b = mk.pair()
+    assert (len(a), len(b)) == (fx["len1"], fx["len2"])
+    print("two-piece long masked pair", len(a), len(b), flush=True)
+    got = gpu.find_similar_2gaps([(a, b)])[0][1]
+    st = gpu.stats()
+    want = fx["fragments"]
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        assert list(g[:5]) == w[:5] and g[5] == w[5] and g[6] == w[6] and list(g[7]) == w[7]
+    for k in ("scan_cells", "xdrop_cells", "semiglobal_cells", "seeds"):
+        assert st[k] == fx["stats"][k], k
+    long_runs = sum(1 for f in got for c in f[7] if (c & 15) in (1, 2) and (c >> 4) >= 22)
+    assert long_runs >= 10
