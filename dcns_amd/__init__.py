@@ -70,6 +70,7 @@ def load_library():
     lib.dcns_last_stats.argtypes = [_P, ctypes.POINTER(Stats)]
     lib.dcns_kernel_time.argtypes = [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_L)]
     lib.dcns_kernel_time_reset.argtypes = [_P]
+    lib.dcns_sw_batch_noalloc.argtypes = [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P]
     lib.dcns_get_tuning.argtypes = [_P, ctypes.POINTER(Tuning)]
     lib.dcns_set_tuning.argtypes = [_P, ctypes.POINTER(Tuning)]
     lib.dcns_sw_batch.argtypes = [_P, ctypes.POINTER(Batch), _I, _I, _I, _I, _P]
@@ -143,7 +144,7 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_kmer_set_info", "dcns_kmer_set_get", "dcns_kmer_set_free", "dcns_cds_mask_create",
            "dcns_cds_mask_gff", "dcns_cds_mask_sam", "dcns_cds_mask_get", "dcns_cds_mask_free", "dcns_mask_stage",
            "dcns_mask_body_size", "dcns_mask_run", "dcns_mask_genome_main", "dcns_batch_main", "dcns_get_tuning",
-           "dcns_set_tuning"]
+           "dcns_set_tuning", "dcns_sw_batch_noalloc"]
 
 
 def default_params(**kw):

@@ -96,6 +96,7 @@ int dcns_ctx_create(int device, uint32_t flags, dcns_ctx **out) {
         c->device = device;
         c->timing = (flags & 1u) != 0;
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(sw_noalloc_init(device), "allocation-free scratch address");
     });
     if (rc != DCNS_OK) {
         delete c;
@@ -246,6 +247,39 @@ int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t misma
         }
         hip_check(hipMemcpyAsync(out3, c->sw_out.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream), "download");
         c->sync();
+    });
+}
+
+int dcns_sw_batch_noalloc(dcns_ctx *c, const int8_t *d_seqs, const int64_t *d_off1, const int32_t *d_len1,
+                          const int64_t *d_off2, const int32_t *d_len2, int32_t n, int32_t max_len1, int32_t match,
+                          int32_t mismatch, int32_t open, int32_t ext, int32_t *d_out3) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        if (n < 0) throw Error(DCNS_ERR_ARG, "n must be >= 0");
+        if (n == 0) return;
+        if (!d_seqs || !d_off1 || !d_len1 || !d_off2 || !d_len2 || !d_out3)
+            throw Error(DCNS_ERR_ARG, "device pointers must be non-null");
+        if (((uintptr_t)d_seqs & 15u) != 0) throw Error(DCNS_ERR_ARG, "d_seqs must be 16-byte aligned");
+        if (max_len1 < 0 || (int64_t)max_len1 > sw_noalloc_max_rows())
+            throw Error(DCNS_ERR_ARG, "max_len1 must be in [0, " + std::to_string(sw_noalloc_max_rows()) +
+                                          "] (the library's boundary array holds one reference of that many rows)");
+        if (match < 1 || match > 127 || mismatch > 0 || mismatch < -128 || open > 0 || open < -128 || ext > 0 ||
+            ext < -128)
+            throw Error(DCNS_ERR_ARG, "scores must fit int8 with match > 0 and penalties <= 0");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        dcns_params dp;
+        dcns_default_params(&dp);
+        dp.match = match;
+        dp.mismatch = mismatch;
+        dp.open = open;
+        dp.ext = ext;
+        const Params P = to_params(dp);
+        const ScoreTab tab = make_score_tab(match, mismatch);
+        int32_t err = 0;
+        hip_check(sw_noalloc_run(c->device, c->stream, d_seqs, d_off1, d_len1, d_off2, d_len2, n, max_len1, P, tab,
+                                 d_out3, &err),
+                  "dcns_sw_batch_noalloc");
+        if (err) throw Error(DCNS_ERR_ARG, "a pair's reference is longer than max_len1 (or a length is negative)");
     });
 }
 

@@ -125,6 +125,20 @@ hipError_t launch_sw_random(hipStream_t st, const uint32_t *thr, const int32_t *
                             const uint64_t *seed, const int32_t *order, bool slot_order, int32_t per_point,
                             int64_t first, int64_t n, const Params &P, const ScoreTab &tab, int32_t *out,
                             int32_t *bound, const int64_t *wave_off);
+// Allocation-free K1 (dcns_sw_batch_noalloc): boundary rows in a static device
+// array.  sw_noalloc_init takes its address on the current device (once per
+// device); sw_noalloc_run launches over pairs [0, n) in order, in as many
+// launches as the array needs, and returns after the stream has finished,
+// with *err_host bit 0 set when a pair had more than max_len1 rows.
+hipError_t sw_noalloc_init(int device);
+size_t sw_noalloc_scratch_bytes();
+int64_t sw_noalloc_max_rows();
+hipError_t sw_noalloc_run(int device, hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
+                          const int64_t *off2, const int32_t *len2, int32_t n, int32_t max_len1, const Params &P,
+                          const ScoreTab &tab, int32_t *out3, int32_t *err_host);
+// Max scores of random fragment pairs generated on the device (pairs [first, first + n)).
+
+
 // semiglobal.hip: two-piece semi-global DP, one wave per 256-column strip.
 int64_t sg_trace_bytes(int32_t R, int32_t C, int32_t w);   // traceback store of one task
 int64_t sg_scratch_ints(int32_t R, int32_t C, int32_t w);  // row results + strip boundaries of one task

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "dp_common.h"
 #include "kernels.h"
@@ -398,6 +399,71 @@ int64_t sw_plan_chunk(const SwPlan &plan, int64_t w0, int64_t budget, std::vecto
         ++w;
     }
     return w;
+}
+
+// ---------------------------------------------------------------------------
+// dcns_sw_batch_noalloc: dcns_sw_batch on device-resident pairs with the
+// stripe boundaries in a library-owned device array, for callers that cannot
+// allocate device memory.  The array is one per device (its address taken
+// once per device at context creation); every call on a device shares it.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;  // 64 MiB per device
+constexpr size_t kNoAllocHeader = 256;                     // error word, then the boundary rows
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void k_clear_word(int32_t *w) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *w = 0;
+}
+}  // namespace
+
+hipError_t sw_noalloc_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));  // the device is current
+    if (e == hipSuccess) g_scratch_ptr[device] = ptr;
+    return e;
+}
+
+size_t sw_noalloc_scratch_bytes() { return kNoAllocScratchBytes; }
+
+// Rows of the longest reference a call can take: one pair's boundary slot
+// must fit the array.
+int64_t sw_noalloc_max_rows() { return (int64_t)((kNoAllocScratchBytes - kNoAllocHeader) / 8) - 16; }
+
+hipError_t sw_noalloc_run(int device, hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
+                          const int64_t *off2, const int32_t *len2, int32_t n, int32_t max_len1, const Params &P,
+                          const ScoreTab &tab, int32_t *out3, int32_t *err_host) {
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (device >= 0 && device < kMaxDevices) base = g_scratch_ptr[device];
+    }
+    if (!base) return hipErrorNotInitialized;
+    int32_t *err = (int32_t *)base;
+    int32_t *bound = (int32_t *)((unsigned char *)base + kNoAllocHeader);
+    const int64_t rows = ((int64_t)std::max(max_len1, 1) + 15) / 16 * 16;  // rows rounded to the 16-row blocks
+    const int64_t stride = rows * 2;                                        // int2 per row, in int32 units
+    const int64_t per_launch = (int64_t)((kNoAllocScratchBytes - kNoAllocHeader) / 4) / stride;
+    hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(64), 0, st, err);
+    hipError_t e = hipGetLastError();
+    // launches in turn reuse the boundary array; stream order keeps them apart
+    for (int32_t s = 0; e == hipSuccess && s < n; s += (int32_t)std::min<int64_t>(per_launch, n - s)) {
+        const int32_t m = (int32_t)std::min<int64_t>(per_launch, n - s);
+        hipLaunchKernelGGL(k_sw_batch, dim3((m + 63) / 64), dim3(64), 0, st, seq, off1, len1, off2, len2, nullptr, s,
+                           m, P, tab, out3, bound, stride, (const int64_t *)nullptr, max_len1, err);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(err_host, err, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
 }
 
 }  // namespace dcns

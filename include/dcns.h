@@ -139,6 +139,25 @@ int dcns_kernel_time_reset(dcns_ctx *ctx);
 int dcns_sw_batch(dcns_ctx *ctx, const dcns_batch *b, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
                   int32_t *out3);
 
+/* dcns_sw_batch for pairs already in device memory, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  The stripe
+ * boundaries live in a 64 MiB device array the library holds on each device
+ * from its first context there to the end of the process; every call on a
+ * device shares it, so calls that overlap on one device race.  All pointers
+ * are device pointers on the context's device: pair i's reference is
+ * d_seqs[d_off1[i], + d_len1[i]) and its query d_seqs[d_off2[i], + d_len2[i])
+ * (codes as for dcns_sw_batch), offsets multiples of 16, d_seqs 16-byte
+ * aligned, and round_up(len, 16) bytes of a reference and round_up(len, 32)
+ * of a query readable (the bytes past the length are ignored).  d_out3
+ * receives 3 int32 per pair as dcns_sw_batch's out3.  max_len1 bounds every
+ * d_len1[i]; it may not exceed (64 MiB - 256) / 8 - 16 rows (DCNS_ERR_ARG).
+ * A pair longer than max_len1 is not computed and the call returns
+ * DCNS_ERR_ARG.  Runs on the context's stream and returns when it is done. */
+int dcns_sw_batch_noalloc(dcns_ctx *ctx, const int8_t *d_seqs, const int64_t *d_off1, const int32_t *d_len1,
+                          const int64_t *d_off2, const int32_t *d_len2, int32_t n, int32_t max_len1, int32_t match,
+                          int32_t mismatch, int32_t open, int32_t ext, int32_t *d_out3);
+
 /* Max local scores of random fragment pairs generated on the device: the
  * sampling loop of permutationLsqLambdaK (impl/permutationLsqLambdaK.cpp:32-102),
  * which scores random fragment pairs with SmithWaterman, with i.i.d. fragments
