@@ -83,6 +83,19 @@ def measured_valu_instructions(path=os.path.join(ROOT, "profiles", "pmc_traffic_
     return None
 
 
+def measured_valu_busy(path=os.path.join(ROOT, "profiles", "pmc_valu_busy_batch4096.md")):
+    """VALU busy of the window-scan kernel (SQ_ACTIVE_INST_VALU x 4 over the
+    SIMDs' active cycles) from the committed rocprofv3 pass, or None."""
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 8 and cells[1].startswith("`k_scan_uniform"):
+                return float(cells[7])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def measured_mask_traffic(path=os.path.join(ROOT, "profiles", "pmc_traffic_mask_genome.md")):
     """HBM bytes per launch of the k-mer scan kernel (fetched + written) from
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes, or None."""
@@ -850,6 +863,9 @@ def main():
     if vi:
         ceiling = dev["cu"] * 4 * dev["clock_mhz"] * 1e6 / 4
         line["roofline"]["valu_issue_frac"] = round(vi / (avg_ms * 1e-3) / ceiling, 4)
+    busy = measured_valu_busy()
+    if busy is not None:
+        line["roofline"]["valu_busy"] = busy  # SQ_ACTIVE_INST_VALU / active cycles, committed counter pass
 
     if not args.no_plain:  # the plain path: one pair per call, host buffers in, fragments out
         rng = np.random.default_rng(11)

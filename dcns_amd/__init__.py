@@ -70,6 +70,7 @@ def load_library():
     lib.dcns_last_stats.argtypes = [_P, ctypes.POINTER(Stats)]
     lib.dcns_kernel_time.argtypes = [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_L)]
     lib.dcns_kernel_time_reset.argtypes = [_P]
+    lib.dcns_sw_bas_batch.argtypes = [_P, ctypes.POINTER(Batch), _I, _I, _I, _P, _P, _P, _L, _P]
     lib.dcns_sw_batch_noalloc.argtypes = [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P]
     lib.dcns_get_tuning.argtypes = [_P, ctypes.POINTER(Tuning)]
     lib.dcns_set_tuning.argtypes = [_P, ctypes.POINTER(Tuning)]
@@ -144,7 +145,7 @@ EXPORTS = ["dcns_version", "dcns_default_params", "dcns_ctx_create", "dcns_ctx_d
            "dcns_kmer_set_info", "dcns_kmer_set_get", "dcns_kmer_set_free", "dcns_cds_mask_create",
            "dcns_cds_mask_gff", "dcns_cds_mask_sam", "dcns_cds_mask_get", "dcns_cds_mask_free", "dcns_mask_stage",
            "dcns_mask_body_size", "dcns_mask_run", "dcns_mask_genome_main", "dcns_batch_main", "dcns_get_tuning",
-           "dcns_set_tuning", "dcns_sw_batch_noalloc"]
+           "dcns_set_tuning", "dcns_sw_batch_noalloc", "dcns_sw_bas_batch"]
 
 
 def default_params(**kw):
@@ -466,6 +467,24 @@ class Context:
         out = np.zeros(3 * max(n, 1), np.int32)
         self._check(self.lib.dcns_sw_batch(self.h, ctypes.byref(ba.b), *sc, out.ctypes.data), "dcns_sw_batch")
         return out[:3 * n].reshape(n, 3)
+
+    def sw_bas(self, pairs, open=-2, ext=-1, reverse=False):
+        """SmithWaterman_bas per pair: (maxScore, end1, end2) as uint32 and,
+        with reverse, (cigar list, status) per pair."""
+        ba = _BatchArrays(pairs)
+        n = ba.b.n
+        out = np.zeros(3 * max(n, 1), np.uint32)
+        cap = sum(2 * (len(a) + len(b)) + 4 for a, b in pairs) + 16
+        cig = np.zeros(cap, np.uint32)
+        off = np.zeros(n + 1, np.int64)
+        st = np.zeros(max(n, 1), np.int32)
+        self._check(self.lib.dcns_sw_bas_batch(self.h, ctypes.byref(ba.b), int(open), int(ext), int(bool(reverse)),
+                                               out.ctypes.data, cig.ctypes.data, off.ctypes.data, cap,
+                                               st.ctypes.data), "dcns_sw_bas_batch")
+        res = [tuple(int(v) for v in out[3 * i:3 * i + 3]) for i in range(n)]
+        if not reverse:
+            return res
+        return [(r, [int(c) for c in cig[off[i]:off[i + 1]]], int(st[i])) for i, r in enumerate(res)]
 
     def sw_random_max(self, sc, gc, len1, len2, seed, pairs_per_point):
         """Max local scores of random pairs generated on the device

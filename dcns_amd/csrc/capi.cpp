@@ -176,6 +176,99 @@ int dcns_kernel_time_reset(dcns_ctx *c) {
     });
 }
 
+namespace {
+// SmithWaterman max and first end for the pairs of b (dcns_sw_batch); nn:
+// N against N scores -1.  Scores already validated.
+void sw_batch_run(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
+                  bool nn, int32_t *out3) {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const int32_t n = b->n;
+    if (n == 0) return;
+    dcns_params dp;
+    dcns_default_params(&dp);
+    dp.match = match;
+    dp.mismatch = mismatch;
+    dp.open = open;
+    dp.ext = ext;
+    Params P = to_params(dp);
+    ScoreTab tab = make_score_tab(match, mismatch);
+    // stage sequences
+    std::vector<int64_t> o1((size_t)n), o2((size_t)n);
+    std::vector<int32_t> l1(b->len1, b->len1 + n), l2(b->len2, b->len2 + n);
+    int64_t total = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (l1[(size_t)i] < 0 || l2[(size_t)i] < 0) throw Error(DCNS_ERR_ARG, "negative length");
+        total += (l1[(size_t)i] + 31) / 16 * 16 + (l2[(size_t)i] + 47) / 16 * 16;
+    }
+    std::vector<int8_t> host((size_t)total + 64, 2);
+    int64_t o = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        o1[(size_t)i] = o;
+        std::memcpy(&host[(size_t)o], b->seqs + b->off1[i], (size_t)l1[(size_t)i]);
+        o += (l1[(size_t)i] + 31) / 16 * 16;  // 16-byte aligned, >= 16 bytes of padding (the kernel reads 16 at a time)
+        o2[(size_t)i] = o;
+        std::memcpy(&host[(size_t)o], b->seqs + b->off2[i], (size_t)l2[(size_t)i]);
+        o += (l2[(size_t)i] + 47) / 16 * 16;  // query stripes read 32 bytes from a multiple of 32
+    }
+    // lane plan: waves of one loop shape, longest first
+    SwPlan plan;
+    sw_plan(l1.data(), l2.data(), n, 1, plan);
+    auto up = [&](DBuf &d, const void *src, size_t bytes) {
+        d.ensure(std::max<size_t>(bytes, 64));
+        hip_check(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+    };
+    c->staged.reset();
+    up(c->seq, host.data(), host.size());
+    up(c->sw_off1, o1.data(), o1.size() * 8);
+    up(c->sw_len1, l1.data(), l1.size() * 4);
+    up(c->sw_off2, o2.data(), o2.size() * 8);
+    up(c->sw_len2, l2.data(), l2.size() * 4);
+    up(c->sw_order, plan.order.data(), plan.order.size() * 4);
+    c->sw_out.ensure((size_t)n * 12 + 64);
+    // boundary rows: launches of whole waves within 1 GiB
+    const int64_t budget = ((int64_t)1 << 30) / 4;
+    std::vector<int64_t> woff;
+    for (int64_t w0 = 0; w0 < (int64_t)plan.waves.size();) {
+        const int64_t w1 = sw_plan_chunk(plan, w0, budget, woff);
+        c->sw_bound.ensure((size_t)woff.back() * 4 + 64);
+        up(c->sw_woff, woff.data(), woff.size() * 8);
+        const int32_t s = (int32_t)(64 * w0), m = (int32_t)std::min<int64_t>(64 * w1, n) - s;
+        c->launch("sw_batch", [&] {
+            return launch_sw_batch(c->stream, c->seq.as<int8_t>(), c->sw_off1.as<int64_t>(), c->sw_len1.as<int32_t>(),
+                                   c->sw_off2.as<int64_t>(), c->sw_len2.as<int32_t>(),
+                                   c->sw_order.as<int32_t>() + s, m, P, tab, c->sw_out.as<int32_t>(),
+                                   c->sw_bound.as<int32_t>(), c->sw_woff.as<int64_t>(), nn);
+        });
+        w0 = w1;
+    }
+    hip_check(hipMemcpyAsync(out3, c->sw_out.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream), "download");
+    c->sync();
+}
+
+// Sequences of a batch staged for one-thread-per-pair kernels: 16-byte
+// aligned offsets with slack past every sequence.
+void stage_pairs(dcns_ctx *c, const dcns_batch *b, std::vector<int64_t> &o1, std::vector<int64_t> &o2) {
+    const int32_t n = b->n;
+    o1.resize((size_t)n);
+    o2.resize((size_t)n);
+    int64_t total = 0;
+    for (int32_t i = 0; i < n; ++i) total += (b->len1[i] + 31) / 16 * 16 + (b->len2[i] + 47) / 16 * 16;
+    std::vector<int8_t> host((size_t)total + 64, 2);
+    int64_t o = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        o1[(size_t)i] = o;
+        std::memcpy(&host[(size_t)o], b->seqs + b->off1[i], (size_t)b->len1[i]);
+        o += (b->len1[i] + 31) / 16 * 16;
+        o2[(size_t)i] = o;
+        std::memcpy(&host[(size_t)o], b->seqs + b->off2[i], (size_t)b->len2[i]);
+        o += (b->len2[i] + 47) / 16 * 16;
+    }
+    c->staged.reset();
+    c->seq.ensure(std::max<size_t>(host.size(), 64));
+    hip_check(hipMemcpyAsync(c->seq.p, host.data(), host.size(), hipMemcpyHostToDevice, c->stream), "upload");
+}
+}  // namespace
+
 int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
                   int32_t *out3) {
     if (!c) return DCNS_ERR_ARG;
@@ -185,68 +278,91 @@ int dcns_sw_batch(dcns_ctx *c, const dcns_batch *b, int32_t match, int32_t misma
         if (match < 1 || match > 127 || mismatch > 0 || mismatch < -128 || open > 0 || open < -128 || ext > 0 ||
             ext < -128)
             throw Error(DCNS_ERR_ARG, "scores must fit int8 with match > 0 and penalties <= 0");
-        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        sw_batch_run(c, b, match, mismatch, open, ext, false, out3);
+    });
+}
+
+int dcns_sw_bas_batch(dcns_ctx *c, const dcns_batch *b, int32_t open, int32_t ext, int32_t reverse, uint32_t *out3,
+                      uint32_t *cigar, int64_t *cig_off, int64_t cig_cap, int32_t *status) {
+    if (!c) return DCNS_ERR_ARG;
+    return guarded(c, [&] {
+        check_batch(b);
         const int32_t n = b->n;
-        if (n == 0) return;
-        dcns_params dp;
-        dcns_default_params(&dp);
-        dp.match = match;
-        dp.mismatch = mismatch;
-        dp.open = open;
-        dp.ext = ext;
-        Params P = to_params(dp);
-        ScoreTab tab = make_score_tab(match, mismatch);
-        // stage sequences
-        std::vector<int64_t> o1((size_t)n), o2((size_t)n);
-        std::vector<int32_t> l1(b->len1, b->len1 + n), l2(b->len2, b->len2 + n);
-        int64_t total = 0;
-        for (int32_t i = 0; i < n; ++i) {
-            if (l1[(size_t)i] < 0 || l2[(size_t)i] < 0) throw Error(DCNS_ERR_ARG, "negative length");
-            total += (l1[(size_t)i] + 31) / 16 * 16 + (l2[(size_t)i] + 47) / 16 * 16;
+        if (n > 0 && !out3) throw Error(DCNS_ERR_ARG, "out3 must be non-null");
+        if (open > 0 || ext > 0 || open + ext < -128)
+            throw Error(DCNS_ERR_ARG, "gap penalties must be <= 0 with open + ext >= -128");
+        for (int32_t i = 0; i < n; ++i)
+            if (b->len1[i] < 0 || b->len2[i] < 0) throw Error(DCNS_ERR_ARG, "negative length");
+        if (!reverse) {
+            // max and ends: the batched K1 kernel with the file's scores (match
+            // 2, mismatch -1, N against N -1) and gap open = open + ext
+            std::vector<int32_t> o3((size_t)std::max(n, 1) * 3);
+            if (n > 0) sw_batch_run(c, b, 2, -1, open + ext, ext, true, o3.data());
+            for (int64_t k = 0; k < 3 * (int64_t)n; ++k) out3[k] = (uint32_t)o3[(size_t)k];
+            if (status)
+                for (int32_t i = 0; i < n; ++i) status[i] = 0;
+            if (cig_off)
+                for (int32_t i = 0; i <= n; ++i) cig_off[i] = 0;
+            return;
         }
-        std::vector<int8_t> host((size_t)total + 64, 2);
-        int64_t o = 0;
-        for (int32_t i = 0; i < n; ++i) {
-            o1[(size_t)i] = o;
-            std::memcpy(&host[(size_t)o], b->seqs + b->off1[i], (size_t)l1[(size_t)i]);
-            o += (l1[(size_t)i] + 31) / 16 * 16;  // 16-byte aligned, >= 16 bytes of padding (the kernel reads 16 at a time)
-            o2[(size_t)i] = o;
-            std::memcpy(&host[(size_t)o], b->seqs + b->off2[i], (size_t)l2[(size_t)i]);
-            o += (l2[(size_t)i] + 47) / 16 * 16;  // query stripes read 32 bytes from a multiple of 32
+        if (n > 0 && (!cig_off || !status || (cig_cap > 0 && !cigar)))
+            throw Error(DCNS_ERR_ARG, "cigar offsets and status must be non-null with reverse");
+        if (n == 0) {
+            if (cig_off) cig_off[0] = 0;
+            return;
         }
-        // lane plan: waves of one loop shape, longest first
-        SwPlan plan;
-        sw_plan(l1.data(), l2.data(), n, 1, plan);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        // the full matrices (12 bytes per cell) and a walk of at most 2 (L1 + L2) + 4 entries per pair
+        std::vector<int64_t> wso((size_t)n), co((size_t)n + 1, 0);
+        int64_t ws = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            wso[(size_t)i] = ws;
+            ws += 3 * ((int64_t)b->len1[i] + 1) * ((int64_t)b->len2[i] + 1);
+            co[(size_t)i + 1] = co[(size_t)i] + 2 * ((int64_t)b->len1[i] + b->len2[i]) + 4;
+        }
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        if (ws * 4 > (int64_t)(free_b / 2))
+            throw Error(DCNS_ERR_CAPACITY, "SmithWaterman_bas full matrices exceed half the free device memory");
+        std::vector<int64_t> o1, o2;
+        stage_pairs(c, b, o1, o2);
         auto up = [&](DBuf &d, const void *src, size_t bytes) {
             d.ensure(std::max<size_t>(bytes, 64));
             hip_check(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
         };
-        c->staged.reset();
-        up(c->seq, host.data(), host.size());
         up(c->sw_off1, o1.data(), o1.size() * 8);
-        up(c->sw_len1, l1.data(), l1.size() * 4);
+        up(c->sw_len1, b->len1, (size_t)n * 4);
         up(c->sw_off2, o2.data(), o2.size() * 8);
-        up(c->sw_len2, l2.data(), l2.size() * 4);
-        up(c->sw_order, plan.order.data(), plan.order.size() * 4);
+        up(c->sw_len2, b->len2, (size_t)n * 4);
+        up(c->sw_order, wso.data(), wso.size() * 8);  // workspace offsets
+        up(c->sw_woff, co.data(), co.size() * 8);     // walk capacity offsets
+        c->sw_bound.ensure((size_t)std::max<int64_t>(ws, 16) * 4);
         c->sw_out.ensure((size_t)n * 12 + 64);
-        // boundary rows: launches of whole waves within 1 GiB
-        const int64_t budget = ((int64_t)1 << 30) / 4;
-        std::vector<int64_t> woff;
-        for (int64_t w0 = 0; w0 < (int64_t)plan.waves.size();) {
-            const int64_t w1 = sw_plan_chunk(plan, w0, budget, woff);
-            c->sw_bound.ensure((size_t)woff.back() * 4 + 64);
-            up(c->sw_woff, woff.data(), woff.size() * 8);
-            const int32_t s = (int32_t)(64 * w0), m = (int32_t)std::min<int64_t>(64 * w1, n) - s;
-            c->launch("sw_batch", [&] {
-                return launch_sw_batch(c->stream, c->seq.as<int8_t>(), c->sw_off1.as<int64_t>(), c->sw_len1.as<int32_t>(),
-                                       c->sw_off2.as<int64_t>(), c->sw_len2.as<int32_t>(),
-                                       c->sw_order.as<int32_t>() + s, m, P, tab, c->sw_out.as<int32_t>(),
-                                       c->sw_bound.as<int32_t>(), c->sw_woff.as<int64_t>());
-            });
-            w0 = w1;
-        }
+        c->bas_cig.ensure((size_t)std::max<int64_t>(co[(size_t)n], 16) * 4);
+        c->bas_st.ensure((size_t)n * 8 + 64);
+        c->launch("sw_bas", [&] {
+            return launch_sw_bas_full(c->stream, c->seq.as<int8_t>(), c->sw_off1.as<int64_t>(),
+                                      c->sw_len1.as<int32_t>(), c->sw_off2.as<int64_t>(), c->sw_len2.as<int32_t>(), n,
+                                      open, ext, true, c->sw_bound.as<int32_t>(), c->sw_order.as<int64_t>(),
+                                      c->sw_out.as<uint32_t>(), c->bas_cig.as<uint32_t>(), c->sw_woff.as<int64_t>(),
+                                      c->bas_st.as<int32_t>(), c->bas_st.as<int32_t>() + n);
+        });
+        std::vector<uint32_t> cg((size_t)co[(size_t)n]);
+        std::vector<int32_t> nst((size_t)2 * n);
         hip_check(hipMemcpyAsync(out3, c->sw_out.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream), "download");
+        hip_check(hipMemcpyAsync(nst.data(), c->bas_st.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream), "status");
+        hip_check(hipMemcpyAsync(cg.data(), c->bas_cig.p, cg.size() * 4, hipMemcpyDeviceToHost, c->stream), "cigar");
         c->sync();
+        int64_t used = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            cig_off[i] = used;
+            status[i] = nst[(size_t)n + i];
+            const int32_t k = nst[(size_t)i];
+            if (used + k > cig_cap) throw Error(DCNS_ERR_CAPACITY, "cigar buffer too small");
+            std::memcpy(cigar + used, cg.data() + co[(size_t)i], (size_t)k * 4);
+            used += k;
+        }
+        cig_off[n] = used;
     });
 }
 

@@ -114,11 +114,20 @@ void sw_plan(const int32_t *len1, const int32_t *len2, int64_t npoints, int32_t 
 // The waves [w0, return) that fit `budget` int32 of boundary rows (at least
 // one); wave_off gets their offsets relative to the first (one more entry).
 int64_t sw_plan_chunk(const SwPlan &plan, int64_t w0, int64_t budget, std::vector<int64_t> &wave_off);
-// order[0, n) are the pairs of the launch's slots; wave_off as sw_plan_chunk.
+// order[0, n) are the pairs of the launch's slots; wave_off as sw_plan_chunk;
+// nn: N against N scores -1 (SmithWaterman_bas's matrix) instead of 0.
 hipError_t launch_sw_batch(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
                            const int64_t *off2, const int32_t *len2, const int32_t *order, int32_t n,
                            const Params &P, const ScoreTab &tab, int32_t *out, int32_t *bound,
-                           const int64_t *wave_off);
+                           const int64_t *wave_off, bool nn = false);
+// SmithWaterman_bas as written, one pair per thread with its full matrices in
+// ws (3 x (L1+1) x (L2+1) int32 at ws_off[t]), and with reverse the
+// reference's traceback walk (cigar at cig_off[t] .. cig_off[t+1], ncig,
+// status 0 / 1 / 2 = ended / stopped at a zero cell / never ends).
+hipError_t launch_sw_bas_full(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
+                              const int64_t *off2, const int32_t *len2, int32_t n, int32_t open, int32_t ext,
+                              bool reverse, int32_t *ws, const int64_t *ws_off, uint32_t *out3, uint32_t *cig,
+                              const int64_t *cig_off, int32_t *ncig, int32_t *status);
 // Max scores of random fragment pairs generated on the device: slots
 // [first, first + n) of the plan (order, slot_order) with the launch's wave_off.
 hipError_t launch_sw_random(hipStream_t st, const uint32_t *thr, const int32_t *len1, const int32_t *len2,

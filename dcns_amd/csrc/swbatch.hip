@@ -102,7 +102,10 @@ __device__ __forceinline__ int32_t wave_max(int32_t v) {
 // the cells before it and the maximum and its first position are unchanged.
 // The boundary slot of every lane must hold the wave's rows; a lane without
 // a pair (bnd null, lengths 0) takes part in the loops but touches no memory.
-template <class Codes>
+// NN: an N row against an N column scores -1 instead of 0 (the score matrix
+// of SmithWaterman_bas, impl/SmithWaterman_bas.cpp:13-20); padding rows and
+// columns past the sequences keep 0.
+template <class Codes, bool NN = false>
 __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &P, const uint32_t *shi,
                         const uint32_t *slo, int2 *__restrict__ bnd, int32_t &best, int32_t &bi, int32_t &bj) {
     best = 0;
@@ -115,6 +118,7 @@ __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &
         const int j0 = s * S;
         uint32_t sel[S];
         int32_t H[S], E[S];
+        uint32_t qn = 0;  // NN: bit j = column j0 + j is a real N
         {
             uint32_t qw[8];
             src.cols32(j0, qw);
@@ -122,6 +126,7 @@ __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &
             for (int j = 0; j < S; ++j) {
                 const int code = (int)(int8_t)((qw[j >> 2] >> (8 * (j & 3))) & 0xffu);
                 sel[j] = sel32(j0 + j < L2 ? code : 2);
+                if (NN && j0 + j < L2 && code == 2) qn |= 1u << j;
                 H[j] = 0;
                 E[j] = 0;
             }
@@ -151,13 +156,15 @@ __device__ void sw_pair(const Codes &src, int32_t L1, int32_t L2, const Params &
                     const int c = ib + r < L1 ? (int)(int8_t)((rw[r >> 2] >> (8 * (r & 3))) & 0xffu) : 2;
                     const int ci = (unsigned)c < 8u ? c : 2;
                     const uint32_t thi = shi[ci], tlo = slo[ci];
+                    const uint32_t rn = (NN && ib + r < L1 && c == 2) ? qn : 0u;  // N row: its N columns score -1
                     int32_t left = b[r].x, fc = b[r].y;
                     int32_t diag = hb_prev;
                     hb_prev = left;
                     int32_t rmax = 0;
 #pragma unroll
                     for (int j = 0; j < S; ++j) {
-                        const int32_t sc = (int32_t)perm(thi, tlo, sel[j]);
+                        int32_t sc = (int32_t)perm(thi, tlo, sel[j]);
+                        if (NN) sc -= (int32_t)((rn >> j) & 1u);
                         const int32_t t = diag + sc;
                         diag = H[j];
                         // fc here is the clamped horizontal gap entering column j
@@ -207,6 +214,7 @@ __device__ __forceinline__ int2 *boundary_slot(int32_t *bound, int64_t stride, c
     return (int2 *)(bound + a + (int64_t)threadIdx.x * per_lane);
 }
 
+template <bool NN>
 __global__ __launch_bounds__(64) void k_sw_batch(const int8_t *__restrict__ seq, const int64_t *__restrict__ off1,
                                                  const int32_t *__restrict__ len1, const int64_t *__restrict__ off2,
                                                  const int32_t *__restrict__ len2, const int32_t *__restrict__ order,
@@ -237,8 +245,8 @@ __global__ __launch_bounds__(64) void k_sw_batch(const int8_t *__restrict__ seq,
     }
     int2 *bnd = ok ? boundary_slot(bound, bound_stride, wave_off, gid) : nullptr;
     int32_t best, bi, bj;
-    sw_pair(MemCodes{ok ? seq + off1[p] : seq, ok ? seq + off2[p] : seq, L1, L2}, L1, L2, P, shi, slo, bnd, best, bi,
-            bj);
+    sw_pair<MemCodes, NN>(MemCodes{ok ? seq + off1[p] : seq, ok ? seq + off2[p] : seq, L1, L2}, L1, L2, P, shi, slo,
+                          bnd, best, bi, bj);
     if (!ok) return;
     out[3 * p + 0] = best;
     out[3 * p + 1] = bi;
@@ -291,14 +299,154 @@ __global__ __launch_bounds__(64) void k_sw_random(const uint32_t *__restrict__ t
     if (ok) out[pair] = best;
 }
 
+// SmithWaterman_bas (impl/SmithWaterman_bas.cpp:59-201) as written, for one
+// task per thread: the full M, E, F matrices (in ws, 3 x (L1+1) x (L2+1)
+// int32 from ws_off[t]) under the reference's initialisation (reverse:
+// E = open + ext on row and column 0, F(0, j) = open + ext + ext * j), its
+// three-way recurrences and comparisons, the first strict maximum, then its
+// traceback walk.  The walk as written keeps going without moving as soon as
+// its state variable (assigned the int8 of a score) is neither 0 nor, after a
+// move, taken back to 0; status 2 reports that the reference's loop would
+// not end, status 1 the walk's stop at a zero cell (the reference prints
+// "... line 110" to stderr), status 0 the walk reaching row or column 0.
+// Cigar entries as the walk appends them (len << 4 | op).
+__global__ __launch_bounds__(64) void k_sw_bas_full(const int8_t *__restrict__ seq, const int64_t *__restrict__ off1,
+                                                    const int32_t *__restrict__ len1,
+                                                    const int64_t *__restrict__ off2,
+                                                    const int32_t *__restrict__ len2, int32_t n, int32_t open,
+                                                    int32_t ext, int32_t reverse, int32_t *__restrict__ ws,
+                                                    const int64_t *__restrict__ ws_off, uint32_t *__restrict__ out3,
+                                                    uint32_t *__restrict__ cig, const int64_t *__restrict__ cig_off,
+                                                    int32_t *__restrict__ ncig, int32_t *__restrict__ status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int8_t *s1 = seq + off1[t], *s2 = seq + off2[t];
+    const int32_t L1 = len1[t], L2 = len2[t], W = L2 + 1;
+    int32_t *M = ws + ws_off[t];
+    int32_t *E = M + (int64_t)(L1 + 1) * W;
+    int32_t *F = E + (int64_t)(L1 + 1) * W;
+    const int32_t q = open + ext, e = ext;
+    auto sc = [&](int a, int b) -> int32_t {  // the file's score matrix
+        if (a == 2 || b == 2) return (a == 2 && b == 2) ? -1 : 0;
+        return a == b ? 2 : -1;
+    };
+    for (int32_t i = 0; i <= L1; ++i)
+        for (int32_t j = 0; j <= L2; ++j) {
+            const int64_t x = (int64_t)i * W + j;
+            M[x] = 0;
+            E[x] = 0;
+            F[x] = 0;
+        }
+    if (reverse) {
+        for (int32_t j = 0; j <= L2; ++j) {
+            E[j] = q;
+            F[j] = q + e * j;
+        }
+        for (int32_t i = 0; i <= L1; ++i) {
+            E[(int64_t)i * W] = q;
+            F[(int64_t)i * W] = q;
+        }
+    }
+    uint32_t maxScore = 0, end1 = 0, end2 = 0;
+    for (int32_t i = 1; i <= L1; ++i) {
+        const int a = s1[i - 1];
+        for (int32_t j = 1; j <= L2; ++j) {
+            const int64_t x = (int64_t)i * W + j, xl = x - 1, xu = x - W, xd = x - W - 1;
+            int32_t ev = (q + M[xl]) >= (e + E[xl]) ? q + M[xl] : e + E[xl];
+            if (ev < q + F[xl]) ev = q + F[xl];
+            int32_t fv = (q + M[xu]) >= (q + E[xu]) ? q + M[xu] : q + E[xu];
+            if (fv < e + F[xu]) fv = e + F[xu];
+            const int32_t dg = sc(a, s2[j - 1]) + M[xd];
+            int32_t mv = dg >= ev ? dg : ev;
+            if (mv < fv) mv = fv;
+            if (mv < 0) mv = 0;
+            E[x] = ev;
+            F[x] = fv;
+            M[x] = mv;
+            if ((uint32_t)mv > maxScore) {
+                maxScore = (uint32_t)mv;
+                end1 = (uint32_t)i;
+                end2 = (uint32_t)j;
+            }
+        }
+    }
+    out3[3 * t] = maxScore;
+    out3[3 * t + 1] = end1 - 1u;  // the file's final decrement (0-based; all ones when nothing scored)
+    out3[3 * t + 2] = end2 - 1u;
+    int32_t nc = 0, st = 0;
+    if (reverse) {
+        uint32_t *cg = cig + cig_off[t];
+        const int64_t cap = cig_off[t + 1] - cig_off[t];
+        int32_t ii = (int32_t)end1, jj = (int32_t)end2;
+        int8_t which = 0;
+        uint32_t op = 0;
+        while (ii > 0 && jj > 0) {
+            bool moved = false;
+            if (which == 0) {
+                op = 0;
+                const int64_t x = (int64_t)ii * W + jj, xd = x - W - 1;
+                which = (int8_t)M[xd];
+                if (M[x] == M[xd] + sc(s1[ii - 1], s2[jj - 1])) {
+                    --ii;
+                    --jj;
+                    moved = true;
+                } else if (M[x] == E[x]) {
+                    --jj;
+                    op = 1;
+                    moved = true;
+                } else if (M[x] == F[x]) {
+                    --ii;
+                    op = 2;
+                    moved = true;
+                } else if (M[x] == 0) {
+                    st = 1;
+                    break;
+                }
+            } else if (which == 1) {
+                op = 1;
+                which = (int8_t)M[(int64_t)(ii - 1) * W + (jj - 1)];
+            }
+            if (nc == 0 || op != (cg[nc - 1] & 0xfu)) {
+                if (nc >= cap) {
+                    st = 2;
+                    break;
+                }
+                cg[nc++] = (1u << 4) | op;
+            } else {
+                cg[nc - 1] += 1u << 4;
+            }
+            // without a move the state can only change from 1 to the diagonal
+            // score; once it is neither 0 nor 1 after such a step, or the
+            // step repeats, nothing changes again: the reference never ends
+            if (!moved && !(which == 0 && op == 1)) {
+                st = 2;
+                break;
+            }
+        }
+    }
+    ncig[t] = nc;
+    status[t] = st;
+}
+
 }  // namespace
 
 hipError_t launch_sw_batch(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
                            const int64_t *off2, const int32_t *len2, const int32_t *order, int32_t n, const Params &P,
-                           const ScoreTab &tab, int32_t *out, int32_t *bound, const int64_t *wave_off) {
+                           const ScoreTab &tab, int32_t *out, int32_t *bound, const int64_t *wave_off, bool nn) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sw_batch, dim3((n + 63) / 64), dim3(64), 0, st, seq, off1, len1, off2, len2, order, 0, n, P,
-                       tab, out, bound, (int64_t)0, wave_off, 0, nullptr);
+    auto k = nn ? k_sw_batch<true> : k_sw_batch<false>;
+    hipLaunchKernelGGL(k, dim3((n + 63) / 64), dim3(64), 0, st, seq, off1, len1, off2, len2, order, 0, n, P, tab, out,
+                       bound, (int64_t)0, wave_off, 0, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_bas_full(hipStream_t st, const int8_t *seq, const int64_t *off1, const int32_t *len1,
+                              const int64_t *off2, const int32_t *len2, int32_t n, int32_t open, int32_t ext,
+                              bool reverse, int32_t *ws, const int64_t *ws_off, uint32_t *out3, uint32_t *cig,
+                              const int64_t *cig_off, int32_t *ncig, int32_t *status) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sw_bas_full, dim3((n + 63) / 64), dim3(64), 0, st, seq, off1, len1, off2, len2, n, open, ext,
+                       (int32_t)reverse, ws, ws_off, out3, cig, cig_off, ncig, status);
     return hipGetLastError();
 }
 
@@ -456,7 +604,7 @@ hipError_t sw_noalloc_run(int device, hipStream_t st, const int8_t *seq, const i
     // launches in turn reuse the boundary array; stream order keeps them apart
     for (int32_t s = 0; e == hipSuccess && s < n; s += (int32_t)std::min<int64_t>(per_launch, n - s)) {
         const int32_t m = (int32_t)std::min<int64_t>(per_launch, n - s);
-        hipLaunchKernelGGL(k_sw_batch, dim3((m + 63) / 64), dim3(64), 0, st, seq, off1, len1, off2, len2, nullptr, s,
+        hipLaunchKernelGGL(k_sw_batch<false>, dim3((m + 63) / 64), dim3(64), 0, st, seq, off1, len1, off2, len2, nullptr, s,
                            m, P, tab, out3, bound, stride, (const int64_t *)nullptr, max_len1, err);
         e = hipGetLastError();
     }

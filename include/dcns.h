@@ -11,6 +11,8 @@
  *
  * Reference interfaces each entry point replaces (paths in the reference
  * tree):
+ *   dcns_sw_bas_batch        SmithWaterman(...) of impl/SmithWaterman_bas.h:15-17
+ *                            (body impl/SmithWaterman_bas.cpp:59-201)
  *   dcns_sw_batch            SmithWaterman(..., bool& positions)
  *                            impl/sequenceAlignment3.cpp:220 (decl.
  *                            impl/sequenceAlignment.h:26-28); caller
@@ -138,6 +140,26 @@ int dcns_kernel_time_reset(dcns_ctx *ctx);
  * the max is 0). */
 int dcns_sw_batch(dcns_ctx *ctx, const dcns_batch *b, int32_t match, int32_t mismatch, int32_t open, int32_t ext,
                   int32_t *out3);
+
+/* SmithWaterman_bas for n pairs: the body in impl/SmithWaterman_bas.cpp:59-201
+ * (declared at impl/SmithWaterman_bas.h:15-17 with a returnCigar argument no
+ * body takes; the reference does not compile the file).  Fixed scores: match
+ * 2, mismatch -1, N against a base 0, N against N -1; a gap of k bases costs
+ * open + k * ext (open, ext <= 0, open + ext >= -128).  out3 (uint32, 3 per
+ * pair): maxScore and the end positions as the function leaves them, 0-based
+ * after its final decrement (0xFFFFFFFF when no cell scores above 0).
+ * reverse != 0: the reverse initialisation and the traceback walk, on the
+ * full matrices (12 bytes per cell of device memory): pair i's cigar entries
+ * (len << 4 | op, in the walk's order) at cigar[cig_off[i] .. cig_off[i+1]),
+ * status[i] 0 when the walk reached row or column 0, 1 when it stopped at a
+ * zero cell (the reference then prints "there is something wrong with
+ * smith-waterman algorithm in line 110" to stderr), 2 when the reference's
+ * walk never ends (its state stops changing; the function does not return).
+ * With reverse == 0, cigar may be NULL; cig_off and status, when given, get
+ * zeros.  The reference also prints "maxScore <n>" to stdout on every call;
+ * this entry point prints nothing. */
+int dcns_sw_bas_batch(dcns_ctx *ctx, const dcns_batch *b, int32_t open, int32_t ext, int32_t reverse, uint32_t *out3,
+                      uint32_t *cigar, int64_t *cig_off, int64_t cig_cap, int32_t *status);
 
 /* dcns_sw_batch for pairs already in device memory, allocating nothing.  No
  * counterpart in the reference: it is for callers that cannot allocate device

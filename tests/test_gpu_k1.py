@@ -147,3 +147,38 @@ def test_sw_random_config3_scorings_match_oracle(gpu, sc):
     assert np.array_equal(got, want[:npts])
     assert np.array_equal(got64, want[npts:])
     assert got.max() > 0
+
+
+def test_sw_bas_matches_reference_body():
+    """dcns_sw_bas_batch against the reference's SmithWaterman_bas body
+    compiled as its own translation unit (tests/golden/sw_bas.json): maxScore
+    and end positions for every case (N against N scores -1, gap open + k *
+    extend), and with reverseAlignment the reference's traceback walk: the
+    cigar where the walk ends, the zero-cell stop ("line 110"), and the cases
+    where it never ends."""
+    import json
+    import os
+    import dcns_amd
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sw_bas.json")))
+    ctx = dcns_amd.Context(0)
+    try:
+        for rev in (0, 1):
+            for key in sorted({(c["open"], c["ext"]) for c in gold if c["reverse"] == rev}):
+                cs = [c for c in gold if c["reverse"] == rev and (c["open"], c["ext"]) == key]
+                pairs = [(synth.encode(c["seq1"]), synth.encode(c["seq2"])) for c in cs]
+                print("sw_bas", rev, key, len(cs), flush=True)
+                got = ctx.sw_bas(pairs, open=key[0], ext=key[1], reverse=bool(rev))
+                for c, g in zip(cs, got):
+                    res, cig, st = (g, None, 0) if not rev else g
+                    if c["never_ends"]:
+                        assert st == 2, (c, g)
+                        if c["max"] is not None:
+                            assert res[0] == c["max"], (c, g)
+                        continue
+                    assert res == (c["max"], c["end1"], c["end2"]), (c, g)
+                    if rev:
+                        assert cig == c["cigar"], (c, g)
+                        assert st == (1 if c["line110"] else 0), (c, g)
+    finally:
+        ctx.close()
+    assert sum(c["never_ends"] for c in gold) > 20 and sum(1 for c in gold if c["reverse"] and not c["never_ends"]) > 10
