@@ -139,18 +139,25 @@ class Surrogate:
 
     # -- training ---------------------------------------------------------
     def fit(self, X, Y, steps=20000, batch=4096, lr=2e-3, weight_decay=1e-4, seed=0, val_frac=0.1, chunk=500,
-            log=None):
+            lambda_weight=1.0, log=None):
         """AdamW on the device with a cosine learning-rate decay.  The targets
         are noisy permutation fits, so a held-out fraction of the points
         (val_frac) is scored after every `chunk` steps and the parameters with
         the lowest held-out loss are kept: past that point the network starts
-        to fit the noise of individual targets."""
+        to fit the noise of individual targets.
+
+        lambda_weight scales the ln(lambda) term of the squared-error loss
+        against ln(K)'s: its output is standardised by std / sqrt(weight).
+        A fit's ln(K) scatters ~10x more than its ln(lambda) in standardised
+        units, so at weight 1 the loss (and the held-out choice) is almost
+        all ln(K) noise."""
         X = np.asarray(X, np.float32)
         Y = np.asarray(Y, np.float32)
         sd = X.std(0)
         sd[sd == 0] = 1
         ys = Y.std(0)
         ys[ys == 0] = 1
+        ys[0] /= math.sqrt(lambda_weight)
         self.scaling = (X.mean(0), sd, Y.mean(0), ys)
         self.net.set_scaling(*self.scaling)
         perm = np.random.default_rng(seed).permutation(len(X))
@@ -250,6 +257,7 @@ def train_main(argv=None):
     ap.add_argument("--steps", type=int, default=30000)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lambda-weight", type=float, default=1.0, help="loss weight of ln(lambda) against ln(K)")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--data", default="", help="optional .npz to save / reuse the training set")
@@ -275,10 +283,11 @@ def train_main(argv=None):
     t1 = time.perf_counter()
     log(f"{len(X)} training points, {cells:.3e} DP cells, {t1 - t0:.1f} s")
     sur = Surrogate(ctx, settings, seed=a.seed)
-    losses = sur.fit(X, Y, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+    losses = sur.fit(X, Y, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, lambda_weight=a.lambda_weight, log=log)
     t2 = time.perf_counter()
     sur.save(a.out, meta={"points": int(len(X)), "pairs_per_point": a.pairs, "training_cells": cells,
                           "steps": a.steps, "batch": a.batch, "lr": a.lr, "seed": a.seed,
+                          "lambda_weight": a.lambda_weight,
                           "data_seconds": round(t1 - t0, 1), "train_seconds": round(t2 - t1, 1),
                           "final_loss": float(np.mean(losses[-200:])), **sur.fit_info})
     log(f"trained in {t2 - t1:.1f} s, weights -> {a.out}")
@@ -290,10 +299,13 @@ if __name__ == "__main__":
 
 
 def evaluate(ctx, sur, settings, points_per_setting=100, judge_pairs=10000, cheap_pairs=1024, seed=31,
-             log=None):
+             matched_pairs=0, log=None):
     """The surrogate against the permutation fit ("judge", judge_pairs random
     pairs per point) at random (composition, lengths) points of each setting,
-    beside a permutation fit at the training points' budget (cheap_pairs).
+    beside a permutation fit at the training points' budget (cheap_pairs)
+    and, when matched_pairs > 0, one at that many pairs (a fit of about the
+    surrogate's accuracy, whose cost per query prices the surrogate's
+    speedup).  All three fits draw independent pairs.
 
     Returns a dict of per-point arrays and summary figures: relative errors
     of lambda and K, the error of the E-value at the score where the judge's
@@ -301,9 +313,11 @@ def evaluate(ctx, sur, settings, points_per_setting=100, judge_pairs=10000, chea
     method, and the wall-clock cost per query."""
     rng = np.random.default_rng(seed)
     rows = []
-    t_judge = 0.0
+    t_judge = t_matched = 0.0
     cells_judge = 0.0
     cells_cheap = 0.0
+    cells_matched = 0.0
+    lm_all, km_all = [], []
     for si, s in enumerate(settings):
         n = points_per_setting
         gc = rng.uniform(*GC_RANGE, n)
@@ -322,6 +336,13 @@ def evaluate(ctx, sur, settings, points_per_setting=100, judge_pairs=10000, chea
         f1 = fit_lambda_k_batch(np.ascontiguousarray(sj[:, :h]), N)
         f2 = fit_lambda_k_batch(np.ascontiguousarray(sj[:, h:2 * h]), N)
         lc, kc, cc = permutation_fit(ctx, s, gc, l1, l2, cheap_pairs, seed=seed * 104729 + si)
+        if matched_pairs:
+            t = time.perf_counter()
+            lm, km, cm = permutation_fit(ctx, s, gc, l1, l2, matched_pairs, seed=seed * 15485863 + si)
+            t_matched += time.perf_counter() - t
+            cells_matched += cm
+            lm_all.append(lm)
+            km_all.append(km)
         cells_judge += cj
         cells_cheap += cc
         X = features(s, open2, gc, l1, l2)
@@ -357,6 +378,7 @@ def evaluate(ctx, sur, settings, points_per_setting=100, judge_pairs=10000, chea
         "surrogate": {"lambda_rel_err": summ(np.abs(ls / lj - 1)[inside]), "K_rel_err": summ(np.abs(ks / kj - 1)[inside]),
                       "evalue_rel_err_at_p0.1": summ(e_err(ls, ks)[inside]),
                       "within_2pct_lambda": float(np.mean(np.abs(ls / lj - 1)[inside] <= 0.02)) if inside.any() else None,
+                      "within_2pct_K": float(np.mean(np.abs(ks / kj - 1)[inside] <= 0.02)) if inside.any() else None,
                       "cells_per_query": 0},
         "permutation_cheap": {"pairs": cheap_pairs, "lambda_rel_err": summ(np.abs(lc / lj - 1)),
                               "K_rel_err": summ(np.abs(kc / kj - 1)), "evalue_rel_err_at_p0.1": summ(e_err(lc, kc)),
@@ -366,6 +388,13 @@ def evaluate(ctx, sur, settings, points_per_setting=100, judge_pairs=10000, chea
                   "own_K_rel_err": summ(judge_noise_k)},
         "rows": r, "inside": inside,
     }
+    if matched_pairs:
+        lm, km = np.concatenate(lm_all), np.concatenate(km_all)
+        res["permutation_matched"] = {"pairs": matched_pairs, "lambda_rel_err": summ(np.abs(lm / lj - 1)),
+                                      "K_rel_err": summ(np.abs(km / kj - 1)),
+                                      "evalue_rel_err_at_p0.1": summ(e_err(lm, km)),
+                                      "cells_per_query": cells_matched / len(r),
+                                      "seconds_per_query": t_matched / len(r)}
     return res
 
 

@@ -20,6 +20,7 @@ def test_surrogate_beats_permutation_at_training_budget(gpu, sur):
     from dcns_amd import lambdak as lk
     rng = np.random.default_rng(11)
     settings = [(2, -3, -4, -2), (1, -2, -3, -1)]
+    budget = int(sur.meta["pairs_per_point"])  # random pairs behind each training target
     errs_s, errs_c, errs_d, errs_ks, errs_kc = [], [], [], [], []
     for k, s in enumerate(settings):
         assert s in set(sur.settings)
@@ -27,8 +28,8 @@ def test_surrogate_beats_permutation_at_training_budget(gpu, sur):
         gc = rng.uniform(0.3, 0.7, n)
         l1, l2 = lk.log_uniform(rng, 100, 2000, n), lk.log_uniform(rng, 100, 2000, n)
         print("judge", s, flush=True)
-        lj, kj, _ = lk.permutation_fit(gpu, s, gc, l1, l2, pairs=20000, seed=500 + k)
-        lc, kc, _ = lk.permutation_fit(gpu, s, gc, l1, l2, pairs=2048, seed=600 + k)
+        lj, kj, _ = lk.permutation_fit(gpu, s, gc, l1, l2, pairs=100000, seed=500 + k)
+        lc, kc, _ = lk.permutation_fit(gpu, s, gc, l1, l2, pairs=budget, seed=600 + k)
         X = lk.features(s, -45, gc, l1, l2)
         ls, ks, inside = sur.lambda_k(X)
         assert inside.all()
@@ -38,7 +39,7 @@ def test_surrogate_beats_permutation_at_training_budget(gpu, sur):
         errs_ks += list(np.abs(ks / kj - 1))
         errs_kc += list(np.abs(kc / kj - 1))
     ms, mc, md = np.median(errs_s), np.median(errs_c), np.median(errs_d)
-    print("lambda median rel err: surrogate", ms, "permutation(2048)", mc, "default constant", md)
+    print("lambda median rel err: surrogate", ms, f"permutation({budget})", mc, "default constant", md)
     assert ms < 0.02 and ms < mc and ms < md
     assert np.median(errs_ks) < np.median(errs_kc)
 
@@ -63,4 +64,4 @@ def test_hull_bounds(sur):
                         lk.features((2, -3, -4, -2), -45, [0.5], [37], [500]),
                         lk.features((2.5, -3, -4, -2), -45, [0.5], [500], [500])])
     assert list(sur.in_hull(X)) == [True, True, False, False, False]
-    assert math.isclose(sur.meta["pairs_per_point"], 2048)
+    assert sur.meta["pairs_per_point"] == 8192 and sur.meta["lambda_weight"] == 10.0
