@@ -18,10 +18,11 @@
 // step t (an anti-diagonal skew), so the values a row carries from left to
 // right (M, the two horizontal gap states, the running row maximum, the
 // reference base) move one lane per step by a DPP wave shift.  The strip's
-// last lane writes each row's carried values to the strip's boundary array;
-// every 64 steps the strip publishes how many boundary rows are complete (a
-// progress word), and the next strip waits for the rows its first lane needs
-// in its next 64 steps.  Every row in
+// last lane puts each row's carried values in LDS, and every 64 steps the
+// wave stores that block of records to the strip's boundary array (one store
+// per field) and publishes how many boundary rows are complete (a progress
+// word); the next strip waits for the rows its first lane needs in its next
+// 64 steps.  Every row in
 // the band is computed (the reference's termination test usually never
 // fires with the default long-gap piece, whose column state keeps every later
 // row within open2 of the best score); the finish kernel cuts the rows at the
@@ -32,6 +33,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 #include <vector>
 
 #include "dp_common.h"
@@ -261,6 +263,7 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
                                                  uint8_t *__restrict__ tb, SgOut *__restrict__ out, SgWeights W,
                                                  int32_t *__restrict__ sync) {
     __shared__ int32_t wtab[kMaxProfiles * 8];
+    __shared__ int32_t bstage[64 * BND];  // lane 63's boundary records of one block of 64 steps
     const int lane = threadIdx.x;
     if (MODE != 0) {
         for (int i = lane; i < W.ncat * 8; i += 64) wtab[i] = W.tab[i];
@@ -302,6 +305,7 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
 
     const int32_t nsteps = S.rhi - S.rlo + 1 + 63;
     const int32_t jfirst = S.s * SW + 1 + lane * K;
+    const int32_t hi_c = C - jfirst;  // the lane's band end in right-edge blocks
     uint32_t tlo[K], thi[K];
     int32_t qc[K], F1[K], F2[K], Mp[K];
 #pragma unroll
@@ -329,7 +333,17 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
         int32_t cm, ce1, ce2, crm, crj, crc;  // rows rlo + t0 + lane
         in.load(S.rlo + t0 + lane, cm, ce1, ce2, crm, crj, crc);
         const int32_t t1 = min(t0 + 64, nsteps);
-        for (int32_t t = t0; t < t1; ++t) {
+        // One step of the strip.  KIND 0: any block.  KIND 1 (full): every
+        // lane holds a valid row whose band covers the lane's K columns and
+        // the next one (no edge masks), the row's result lies in a later strip
+        // and no stale diagonal is recorded, so the step needs no row, band or
+        // lane tests.  KIND 2 (right edge): as full, except that the band of
+        // every row ends at column C inside this strip, so each lane's band
+        // end hi = C - jfirst is the same on every row and every row's result
+        // is written here.
+        auto step = [&](auto kind_c, int32_t t) {
+            constexpr int KIND = decltype(kind_c)::value;
+            constexpr bool FULL = KIND != 0;
             const int32_t q = t - t0;
             // lanes 1..63 take the carried values of lane l-1; lane 0 takes row
             // rlo + t's input (the DPP shift leaves the old operand in lane 0)
@@ -345,16 +359,17 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
             const int32_t irc = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(crc, q), orc, 0x138, 0xf, 0xf,
                                                             false);
             const int32_t r = S.rlo + t - lane;
-            const bool valid = lane <= t && r <= S.rhi;
+            const bool valid = FULL || (lane <= t && r <= S.rhi);
             const int32_t diag = prev_im;
             prev_im = im;
             uint32_t tword = 0xffffffffu;
             if (valid) {
-                const int32_t st = max(r - w, 1), en = min(r + w, C);
-                const int32_t lo = st - jfirst, hi = en - jfirst;  // in-band columns: lo <= c < hi; c == hi: last
+                const int32_t st = FULL ? 0 : max(r - w, 1), en = FULL ? 0 : min(r + w, C);
+                // in-band columns: lo <= c < hi; c == hi: last
+                const int32_t lo = KIND == 2 ? 0 : st - jfirst, hi = KIND == 2 ? hi_c : en - jfirst;
                 int32_t e1 = ie1, e2 = ie2, rm = irm, rj = irj;
                 // every lane's columns inside the band (the usual case): no edge masks
-                const bool interior = __ballot(lo > 0 || hi < K) == 0;
+                const bool interior = KIND == 1 || (KIND == 0 && __ballot(lo > 0 || hi < K) == 0);
                 if (MODE == 0) {
                     const bool hsel = irc >= 4;
                     const uint32_t sh = (uint32_t)(irc & 3) * 8u;
@@ -383,7 +398,8 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
                 oe2 = e2;
                 orm = rm;
                 orj = rj;
-                if (MODE == 0 && S.s == stale_strip && lane == stale_lane && (r == C + w - 2 || r == C + w - 1))
+                if (!FULL && MODE == 0 && S.s == stale_strip && lane == stale_lane &&
+                    (r == C + w - 2 || r == C + w - 1))
                     out[S.task].stale[r - (C + w - 2)] = Mp[stale_c];
             } else {
                 om = NEG;
@@ -393,18 +409,56 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
             orc = irc;
             if (TB) *(uint32_t *)(tb + T.tb + S.tb + (int64_t)t * SW + lane * K) = tword;
             if (lane == 63 && valid) {
-                int32_t *b = bout + (int64_t)(r - S.rlo) * BND;
-                ast(b, om);
-                ast(b + 1, oe1);
-                ast(b + 2, oe2);
-                ast(b + 3, orm);
-                ast(b + 4, orj);
-                if ((min(r + w, C) - 1) / SW == S.s) {  // the row's last strip holds its result
+                // the row's boundary record, staged for the block's store
+                int32_t *b = bstage + q * BND;
+                b[0] = om;
+                b[1] = oe1;
+                b[2] = oe2;
+                b[3] = orm;
+                b[4] = orj;
+                if (KIND == 2 || (KIND == 0 && (min(r + w, C) - 1) / SW == S.s)) {  // the row's last strip holds its result
                     ast(rowres + 2 * (int64_t)r, orm);
                     ast(rowres + 2 * (int64_t)r + 1, orj);
                 }
             }
+        };
+        // The block's rows run from rlo + t0 - 63 (lane 63 at t0) to rlo + t0 + 63
+        // (lane 0 at t0 + 63).  It is FULL when all its 64 steps hold a row in
+        // every lane, the band of its last row starts at or before the strip's
+        // first column, the band of its first row reaches past the strip's last
+        // column (so lane 63's last column is not the band's last), and no row
+        // of the block is one whose stale diagonal is recorded.  Then for every
+        // row of the block max(r - w, 1) <= jfirst and min(r + w, C) >=
+        // jfirst + K in every lane, which is the interior case, and the row's
+        // last strip lies to the right, so no row result is written here.
+        // A right-edge block has the same rows and left edge, but column C lies
+        // in this strip and the band of the block's first row already reaches
+        // it, so every row's band ends at C.
+        const int32_t jb = S.s * SW + 1, rmin_b = S.rlo + t0 - 63, rmax_b = S.rlo + t0 + 63;
+        const bool rows_left = t1 - t0 == 64 && t0 >= 63 && rmax_b <= S.rhi && rmax_b - w <= jb &&
+                               !(MODE == 0 && S.s == stale_strip && rmax_b >= C + w - 2);
+        if (rows_left && min(rmin_b + w, C) >= jb + SW) {
+            for (int32_t t = t0; t < t1; ++t) step(std::integral_constant<int, 1>{}, t);
+        } else if (rows_left && C < jb + SW && rmin_b + w >= C) {
+            for (int32_t t = t0; t < t1; ++t) step(std::integral_constant<int, 2>{}, t);
+        } else {
+            for (int32_t t = t0; t < t1; ++t) step(std::integral_constant<int, 0>{}, t);
         }
+        // The block's boundary records leave LDS in one store per field: lane
+        // l stores the record lane 63 wrote at step t0 + l (row rlo + t0 + l -
+        // 63), where that step held a row.  A single-wave workgroup, so the
+        // barriers only order the LDS accesses around the copy.
+        __syncthreads();
+        {
+            const int32_t rl = S.rlo + t0 + lane - 63;
+            if (lane < t1 - t0 && t0 + lane >= 63 && rl <= S.rhi) {
+                int32_t *b = bout + (int64_t)(rl - S.rlo) * BND;
+                const int32_t *src = bstage + lane * BND;
+#pragma unroll
+                for (int f = 0; f < BND; ++f) ast(b + f, src[f]);
+            }
+        }
+        __syncthreads();
         // boundary rows up to rlo + t1 - 64 are written (the last lane trails by 63 steps)
         publish_rows(prog + k, min(S.rlo + t1 - 64, S.rhi));
     }
