@@ -139,7 +139,7 @@ struct dcns_ctx {
     dcns::DBuf xtasks, xout, tb_off, tb, cigar, cig_host_dummy;
     dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound, sw_woff;
     dcns::DBuf bas_cig, bas_st;  // SmithWaterman_bas walks: cigar entries, counts and status
-    dcns::DBuf sg_tasks, sg_out, sg_scr, sg_tb, sg_cig, sg_strips, sg_sync;  // two-piece semi-global tasks
+    dcns::DBuf sg_tasks, sg_out, sg_scr, sg_tb, sg_cig, sg_strips, sg_order, sg_sync;  // two-piece semi-global tasks
     dcns::DBuf sg_wcat, sg_wcat_off, sg_wtab;  // weighted DP: per-position profile indices and the profiles
     dcns::DBuf xw_scr, xw_off, xw_wmax;                  // x-drop tasks with bands past the wave kernel's
     dcns::DBuf order_hist, task_order;                   // scan tasks ordered by row count

@@ -158,8 +158,8 @@ int32_t sg_task_strips(int32_t R, int32_t C, int32_t w, int32_t task, int64_t bn
 // W.mode 0: SemiGlobal_no_avx with P's scores; 1 / 2: weighted two-piece /
 // single gap with W's profiles (P's scores unused).
 hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
-                        const SgStrip *strips, int32_t nstrips, const Params &P, int32_t *scr, uint8_t *tb,
-                        SgOut *out, const SgWeights &W, int32_t *sync);
+                        const SgStrip *strips, const int32_t *order, int32_t nstrips, const Params &P, int32_t *scr,
+                        uint8_t *tb, SgOut *out, const SgWeights &W, int32_t *sync);
 hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
                             const SgStrip *strips, int32_t n, const Params &P, const int32_t *scr, const uint8_t *tb,
                             SgOut *out, uint32_t *cigar, const SgWeights &W);

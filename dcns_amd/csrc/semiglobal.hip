@@ -249,17 +249,21 @@ __device__ __forceinline__ void columns_w(const int32_t (&qc)[K], int32_t (&F1)[
     }
 }
 
-// One wave per strip.  Strips take their list index from a launch-wide
-// counter in the order they start running, so a strip's predecessor (the
-// entry before it) has always started: every wait ends, whatever number of
-// waves the device holds at once.  sync[0] = the counter, sync[1] = the
-// error word, sync[2 + k] = rows of strip k whose boundary records are
-// complete.  MODE 0: SemiGlobal_no_avx; 1: weighted two-piece; 2: weighted
-// single gap (profiles in W).
+// One wave per strip.  The strip list holds every task's strips in order
+// (strip k's predecessor is entry k - 1 when it is not the task's first).
+// Strips start in launch order: each wave takes the next position from a
+// launch-wide counter in the order the waves start running and runs strip
+// order[position].  The host puts every strip after its predecessor in that
+// order, so a strip's predecessor has always started: every wait ends,
+// whatever number of waves the device holds at once.  sync[0] = the counter,
+// sync[1] = the error word, sync[2 + k] = rows of strip k whose boundary
+// records are complete.  MODE 0: SemiGlobal_no_avx; 1: weighted two-piece;
+// 2: weighted single gap (profiles in W).
 template <bool TB, int MODE>
 __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                  const SgTask *__restrict__ tasks, const SgStrip *__restrict__ strips,
-                                                 int32_t total, Params P, int32_t *__restrict__ scr,
+                                                 const int32_t *__restrict__ order, int32_t total, Params P,
+                                                 int32_t *__restrict__ scr,
                                                  uint8_t *__restrict__ tb, SgOut *__restrict__ out, SgWeights W,
                                                  int32_t *__restrict__ sync) {
     __shared__ int32_t wtab[kMaxProfiles * 8];
@@ -273,6 +277,7 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
     if (lane == 0) k = atomicAdd(sync, 1);
     k = __builtin_amdgcn_readfirstlane(k);
     if (k >= total) return;
+    k = order[k];
     int32_t *err = sync + 1, *prog = sync + 2;
     const SgStrip S = strips[k];
     const SgTask T = tasks[S.task];
@@ -715,12 +720,14 @@ int32_t sg_task_strips(int32_t R, int32_t C, int32_t w, int32_t task, int64_t bn
 }
 
 hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
-                        const SgStrip *strips, int32_t nstrips, const Params &P, int32_t *scr, uint8_t *tb,
+                        const SgStrip *strips, const int32_t *order, int32_t nstrips, const Params &P, int32_t *scr,
+                        uint8_t *tb,
                         SgOut *out, const SgWeights &W, int32_t *sync) {
     if (nstrips <= 0) return hipSuccess;
     if (W.mode != 0 && (W.ncat < 1 || W.ncat > kMaxProfiles)) return hipErrorInvalidValue;
 #define DCNS_SG_LAUNCH(TBV, MV)                                                                                      \
-    hipLaunchKernelGGL((k_sg_strip<TBV, MV>), dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, nstrips, P,  \
+    hipLaunchKernelGGL((k_sg_strip<TBV, MV>), dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, order,       \
+                       nstrips, P,                                                                                  \
                        scr, tb, out, W, sync)
     switch (W.mode * 2 + (trace ? 1 : 0)) {
         case 0: DCNS_SG_LAUNCH(false, 0); break;

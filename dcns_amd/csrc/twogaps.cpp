@@ -120,9 +120,32 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
             t.nstrips = sg_task_strips(t.R, t.C, P.bandwidth, k, t.scr + 2 * ((int64_t)t.R + 2), strips);
         }
         const int32_t ns = (int32_t)strips.size();
+        // Launch order: longest remaining chain first.  A strip's chain is the
+        // steps from its start to its task's end: its own rows, or the next
+        // strip's chain one hand-over (two 64-step blocks) later, whichever
+        // ends last.  The chain shrinks along a task, so every strip comes
+        // after its predecessor; the long tasks' strips start first and the
+        // short tasks fill the device around them.
+        std::vector<int64_t> chain((size_t)ns);
+        for (int32_t k = m - 1; k >= 0; --k) {
+            const SgTask &t = ct[(size_t)k];
+            int64_t next = 0;
+            for (int64_t i = t.sched + t.nstrips - 1; i >= t.sched; --i) {
+                const SgStrip &g = strips[(size_t)i];
+                next = std::max<int64_t>((int64_t)g.rhi - g.rlo + 64, next + 128);
+                chain[(size_t)i] = next;
+            }
+        }
+        std::vector<int32_t> order((size_t)ns);
+        for (int32_t i = 0; i < ns; ++i) order[(size_t)i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t a, int32_t b) { return chain[(size_t)a] > chain[(size_t)b]; });
         c->sg_strips.ensure(std::max<size_t>(strips.size(), 1) * sizeof(SgStrip));
         hip_check(hipMemcpyAsync(c->sg_strips.p, strips.data(), strips.size() * sizeof(SgStrip), hipMemcpyHostToDevice,
                                  c->stream), "sg strips");
+        c->sg_order.ensure(std::max<size_t>(order.size(), 1) * sizeof(int32_t));
+        hip_check(hipMemcpyAsync(c->sg_order.p, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                 c->stream), "sg order");
         c->sg_sync.ensure((size_t)(ns + 2) * 4);
         hip_check(hipMemsetAsync(c->sg_sync.p, 0, (size_t)(ns + 2) * 4, c->stream), "sg sync");
         c->sg_tasks.ensure((size_t)m * sizeof(SgTask));
@@ -138,7 +161,7 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
                   "sg out");
         c->launch("semiglobal", [&] {
             return launch_sg_dp(c->stream, trace, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
-                                c->sg_strips.as<SgStrip>(), ns, P, c->sg_scr.as<int32_t>(),
+                                c->sg_strips.as<SgStrip>(), c->sg_order.as<int32_t>(), ns, P, c->sg_scr.as<int32_t>(),
                                 trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>(), W,
                                 c->sg_sync.as<int32_t>());
         });
