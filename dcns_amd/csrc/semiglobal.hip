@@ -259,7 +259,7 @@ __device__ __forceinline__ void columns_w(const int32_t (&qc)[K], int32_t (&F1)[
 // sync[1] = the error word, sync[2 + k] = rows of strip k whose boundary
 // records are complete.  MODE 0: SemiGlobal_no_avx; 1: weighted two-piece;
 // 2: weighted single gap (profiles in W).
-template <bool TB, int MODE>
+template <bool TB, int MODE, bool X2Z>
 __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                  const SgTask *__restrict__ tasks, const SgStrip *__restrict__ strips,
                                                  const int32_t *__restrict__ order, int32_t total, Params P,
@@ -268,6 +268,7 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
                                                  int32_t *__restrict__ sync) {
     __shared__ int32_t wtab[kMaxProfiles * 8];
     __shared__ int32_t bstage[64 * BND];  // lane 63's boundary records of one block of 64 steps
+    __shared__ __align__(16) int32_t cstage[64 * 8];  // lane 0's inputs of one block of 64 steps
     const int lane = threadIdx.x;
     if (MODE != 0) {
         for (int i = lane; i < W.ncat * 8; i += 64) wtab[i] = W.tab[i];
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
     const PairDesc pd = pairs[T.pair];
     const int8_t *s1 = seq + pd.off1, *s2 = seq + pd.off2;
     const int32_t R = T.R, C = T.C, w = P.bandwidth;
-    const int32_t o1 = P.open, x1 = P.ext, o2 = P.open2, x2 = P.ext2;
+    const int32_t o1 = P.open, x1 = P.ext, o2 = P.open2, x2 = X2Z ? 0 : P.ext2;
     int32_t *rowres = scr + T.scr;
     int32_t *bout = scr + S.bnd;
     // the lane and column that hold column C-1 (for the stale diagonals)
@@ -335,8 +336,25 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
             in.load(S.rlo - 1, prev_im, a, b, c2, d, f);
             if (lane != 0) prev_im = NEG;
         }
-        int32_t cm, ce1, ce2, crm, crj, crc;  // rows rlo + t0 + lane
-        in.load(S.rlo + t0 + lane, cm, ce1, ce2, crm, crj, crc);
+        {
+            // lane 0's inputs of the block's steps (rows rlo + t0 + q), one
+            // 8-int record per step in LDS, read back by every lane at once
+            int32_t cm, ce1, ce2, crm, crj, crc;
+            in.load(S.rlo + t0 + lane, cm, ce1, ce2, crm, crj, crc);
+            int32_t *cs = cstage + lane * 8;
+            cs[0] = cm;
+            cs[1] = ce1;
+            cs[2] = ce2;
+            cs[3] = crm;
+            cs[4] = crj;
+            cs[5] = crc;
+        }
+        __syncthreads();
+        auto load_in = [&](int32_t q, int4 &a, int2 &b) {
+            const int32_t *p = cstage + min(q, 63) * 8;
+            a = *(const int4 *)p;
+            b = *(const int2 *)(p + 4);
+        };
         const int32_t t1 = min(t0 + 64, nsteps);
         // One step of the strip.  KIND 0: any block.  KIND 1 (full): every
         // lane holds a valid row whose band covers the lane's K columns and
@@ -346,23 +364,19 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
         // every row ends at column C inside this strip, so each lane's band
         // end hi = C - jfirst is the same on every row and every row's result
         // is written here.
-        auto step = [&](auto kind_c, int32_t t) {
+        auto step = [&](auto kind_c, int32_t t, const int4 &a, const int2 &b) {
             constexpr int KIND = decltype(kind_c)::value;
             constexpr bool FULL = KIND != 0;
             const int32_t q = t - t0;
             // lanes 1..63 take the carried values of lane l-1; lane 0 takes row
-            // rlo + t's input (the DPP shift leaves the old operand in lane 0)
-            const int32_t im = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(cm, q), om, 0x138, 0xf, 0xf, false);
-            const int32_t ie1 = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(ce1, q), oe1, 0x138, 0xf, 0xf,
-                                                            false);
-            const int32_t ie2 = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(ce2, q), oe2, 0x138, 0xf, 0xf,
-                                                            false);
-            const int32_t irm = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(crm, q), orm, 0x138, 0xf, 0xf,
-                                                            false);
-            const int32_t irj = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(crj, q), orj, 0x138, 0xf, 0xf,
-                                                            false);
-            const int32_t irc = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(crc, q), orc, 0x138, 0xf, 0xf,
-                                                            false);
+            // rlo + t's input (a, b: the step's LDS record, which the DPP shift
+            // leaves in lane 0 as its old operand)
+            const int32_t im = __builtin_amdgcn_update_dpp(a.x, om, 0x138, 0xf, 0xf, false);
+            const int32_t ie1 = __builtin_amdgcn_update_dpp(a.y, oe1, 0x138, 0xf, 0xf, false);
+            const int32_t ie2 = __builtin_amdgcn_update_dpp(a.z, oe2, 0x138, 0xf, 0xf, false);
+            const int32_t irm = __builtin_amdgcn_update_dpp(a.w, orm, 0x138, 0xf, 0xf, false);
+            const int32_t irj = __builtin_amdgcn_update_dpp(b.x, orj, 0x138, 0xf, 0xf, false);
+            const int32_t irc = __builtin_amdgcn_update_dpp(b.y, orc, 0x138, 0xf, 0xf, false);
             const int32_t r = S.rlo + t - lane;
             const bool valid = FULL || (lane <= t && r <= S.rhi);
             const int32_t diag = prev_im;
@@ -442,13 +456,28 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
         const int32_t jb = S.s * SW + 1, rmin_b = S.rlo + t0 - 63, rmax_b = S.rlo + t0 + 63;
         const bool rows_left = t1 - t0 == 64 && t0 >= 63 && rmax_b <= S.rhi && rmax_b - w <= jb &&
                                !(MODE == 0 && S.s == stale_strip && rmax_b >= C + w - 2);
-        if (rows_left && min(rmin_b + w, C) >= jb + SW) {
-            for (int32_t t = t0; t < t1; ++t) step(std::integral_constant<int, 1>{}, t);
-        } else if (rows_left && C < jb + SW && rmin_b + w >= C) {
-            for (int32_t t = t0; t < t1; ++t) step(std::integral_constant<int, 2>{}, t);
-        } else {
-            for (int32_t t = t0; t < t1; ++t) step(std::integral_constant<int, 0>{}, t);
-        }
+        // two steps per iteration, each step's LDS record read one step ahead
+        auto run = [&](auto kind_c) {
+            int4 a;
+            int2 b;
+            load_in(0, a, b);
+            for (int32_t t = t0; t < t1; t += 2) {
+                int4 a2;
+                int2 b2;
+                load_in(t + 1 - t0, a2, b2);
+                step(kind_c, t, a, b);
+                if (t + 1 < t1) {
+                    load_in(t + 2 - t0, a, b);
+                    step(kind_c, t + 1, a2, b2);
+                }
+            }
+        };
+        if (rows_left && min(rmin_b + w, C) >= jb + SW)
+            run(std::integral_constant<int, 1>{});
+        else if (rows_left && C < jb + SW && rmin_b + w >= C)
+            run(std::integral_constant<int, 2>{});
+        else
+            run(std::integral_constant<int, 0>{});
         // The block's boundary records leave LDS in one store per field: lane
         // l stores the record lane 63 wrote at step t0 + l (row rlo + t0 + l -
         // 63), where that step held a row.  A single-wave workgroup, so the
@@ -725,17 +754,29 @@ hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const Pai
                         SgOut *out, const SgWeights &W, int32_t *sync) {
     if (nstrips <= 0) return hipSuccess;
     if (W.mode != 0 && (W.ncat < 1 || W.ncat > kMaxProfiles)) return hipErrorInvalidValue;
-#define DCNS_SG_LAUNCH(TBV, MV)                                                                                      \
-    hipLaunchKernelGGL((k_sg_strip<TBV, MV>), dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, order,       \
+#define DCNS_SG_LAUNCH(TBV, MV, X2V)                                                                                 \
+    hipLaunchKernelGGL((k_sg_strip<TBV, MV, X2V>), dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, order,  \
                        nstrips, P,                                                                                  \
                        scr, tb, out, W, sync)
     switch (W.mode * 2 + (trace ? 1 : 0)) {
-        case 0: DCNS_SG_LAUNCH(false, 0); break;
-        case 1: DCNS_SG_LAUNCH(true, 0); break;
-        case 2: DCNS_SG_LAUNCH(false, 1); break;
-        case 3: DCNS_SG_LAUNCH(true, 1); break;
-        case 4: DCNS_SG_LAUNCH(false, 2); break;
-        case 5: DCNS_SG_LAUNCH(true, 2); break;
+        // the unweighted kernel with a flat second gap piece (ext2 0, the
+        // cut2Gaps default) drops the piece's two extension adds per cell
+        case 0:
+            if (P.ext2 == 0)
+                DCNS_SG_LAUNCH(false, 0, true);
+            else
+                DCNS_SG_LAUNCH(false, 0, false);
+            break;
+        case 1:
+            if (P.ext2 == 0)
+                DCNS_SG_LAUNCH(true, 0, true);
+            else
+                DCNS_SG_LAUNCH(true, 0, false);
+            break;
+        case 2: DCNS_SG_LAUNCH(false, 1, false); break;
+        case 3: DCNS_SG_LAUNCH(true, 1, false); break;
+        case 4: DCNS_SG_LAUNCH(false, 2, false); break;
+        case 5: DCNS_SG_LAUNCH(true, 2, false); break;
         default: return hipErrorInvalidValue;
     }
 #undef DCNS_SG_LAUNCH
