@@ -135,7 +135,7 @@ struct dcns_ctx {
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;  // recorded, not yet read
     std::vector<hipEvent_t> event_pool;
     // reusable device buffers
-    dcns::DBuf seq, pairs, wtasks, pow, traj, tcount, hitlist, ctr, nodes, roots, tasks_a, tasks_b;
+    dcns::DBuf seq, pairs, wtasks, l0wins, pow, traj, tcount, hitlist, ctr, nodes, roots, tasks_a, tasks_b;
     dcns::DBuf xtasks, xout, tb_off, tb, cigar, cig_host_dummy;
     dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound, sw_woff;
     dcns::DBuf bas_cig, bas_st;  // SmithWaterman_bas walks: cigar entries, counts and status

@@ -27,9 +27,12 @@ inline bool scan_general_packed(const Params &P, uint32_t &mis4, uint32_t &diff)
 }
 // Level-0 window scan: one workgroup per 128-window task, or, with
 // persistent_waves > 0, that many waves popping the task list from ctr->queue.
+// wins[0, nwins): level-0 windows (pair, window) scanned one per wave on the
+// anti-diagonal path, in the launch's first workgroups.
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
                                int32_t nwt, int32_t persistent_waves, const Params &P, const ScoreTab &tab,
-                               TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr);
+                               TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr,
+                               const int2 *wins, int32_t nwins);
 // Later-level rescans: *ntasks_dev tasks (at most ntasks_max) in order[];
 // up to wave_tasks of the first ones with more than wave_rows rows run one
 // per wave, the rest one per lane on at most lane_blocks_max waves (0: one

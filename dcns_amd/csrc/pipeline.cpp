@@ -129,9 +129,22 @@ Staged stage_batch(dcns_ctx *c, const dcns_batch &b, const Params &P) {
     return S;
 }
 
+// Issue slots (wave instructions) of the two level-0 paths, measured on the
+// kernels: a row of the 128-window lane kernel (~337 VALU, ~115 s_nop on its
+// critical path) and a step of the one-window wave kernel.
+constexpr double kLaneRowIssue = 340.0, kLaneRowLatency = 450.0;
+constexpr double kWaveStepIssue = 35.0, kWaveStepLatency = 40.0;
+
 // Level-0 wave tasks (128 windows of one pair per wave, longest reference
 // first), window -> pair map and per-pair window-scan cells; uploaded once
 // per staged batch and window layout.
+//
+// A lane task's rows are serial, so a batch with few pairs is held to its
+// longest reference's rows while most of the chip idles.  The longest tasks
+// move to the one-window-per-wave path (R + width - 1 steps per window, ~10x
+// the issue slots) for as long as that shortens the estimated launch: the
+// larger of the longest remaining chain and the total issue slots over the
+// chip's SIMDs.  A full batch keeps every task on the lane kernel.
 void upload_layout(dcns_ctx *c, Staged &S) {
     S.wt.clear();
     S.pair_of_window.assign((size_t)S.nwin, 0);
@@ -149,8 +162,34 @@ void upload_layout(dcns_ctx *c, Staged &S) {
     std::stable_sort(S.wt.begin(), S.wt.end(), [&](const int2 &a, const int2 &b) {
         return S.pd[(size_t)a.x].len1 > S.pd[(size_t)b.x].len1;
     });
+    const size_t n = S.wt.size();
+    auto task_windows = [&](size_t i) {
+        return std::min(128, S.pd[(size_t)S.wt[i].x].nwin - S.wt[i].y);
+    };
+    const double simds = 4.0 * c->cu_count();
+    double lane_total = 0;
+    for (size_t i = 0; i < n; ++i) lane_total += (double)S.pd[(size_t)S.wt[i].x].len1 * kLaneRowIssue;
+    auto lane_chain = [&](size_t i) { return i < n ? (double)S.pd[(size_t)S.wt[i].x].len1 * kLaneRowLatency : 0.0; };
+    double best = std::max(lane_chain(0), lane_total / simds), wave_total = 0, wave_chain = 0;
+    size_t moved = 0;
+    for (size_t k = 1; k <= n; ++k) {
+        const double rows = (double)S.pd[(size_t)S.wt[k - 1].x].len1 + S.window - 1;
+        lane_total -= (double)S.pd[(size_t)S.wt[k - 1].x].len1 * kLaneRowIssue;
+        wave_total += task_windows(k - 1) * rows * kWaveStepIssue;
+        wave_chain = std::max(wave_chain, rows * kWaveStepLatency);
+        const double t = std::max(std::max(lane_chain(k), wave_chain), (lane_total + wave_total) / simds);
+        if (t < best) {
+            best = t;
+            moved = k;
+        }
+    }
+    S.ww.clear();
+    for (size_t i = 0; i < moved; ++i)
+        for (int32_t w = 0; w < task_windows(i); ++w) S.ww.push_back(make_int2(S.wt[i].x, S.wt[i].y + w));
+    S.wt.erase(S.wt.begin(), S.wt.begin() + (std::ptrdiff_t)moved);
     upload(c, c->pairs, S.pd);
     upload(c, c->wtasks, S.wt);
+    upload(c, c->l0wins, S.ww);
     upload(c, c->pow, S.pair_of_window);
 }
 
@@ -297,7 +336,8 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
     c->launch("scan_uniform", [&] {
         return launch_scan_uniform(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->wtasks.as<int2>(),
                                    (int32_t)S.wt.size(), T.level0_waves, P, tab, c->traj.as<TrajEntry>(),
-                                   c->tcount.as<int32_t>(), cap, c->hitlist.as<int32_t>(), ctr);
+                                   c->tcount.as<int32_t>(), cap, c->hitlist.as<int32_t>(), ctr,
+                                   c->l0wins.as<int2>(), (int32_t)S.ww.size());
     });
 
     Counters h{};

@@ -20,6 +20,7 @@ struct Staged {
     int64_t nwin = 0;  // windows over all pairs
     int32_t window = 0, step = 0;  // schedule the window layout was made for
     std::vector<int2> wt;                 // level-0 wave tasks (pair, first window)
+    std::vector<int2> ww;                 // level-0 windows (pair, window) scanned one per wave
     std::vector<int32_t> pair_of_window;  // global window -> pair
     std::vector<int64_t> level0_cells;    // per pair: L1 * sum of window widths
 };

@@ -59,6 +59,114 @@ __device__ __forceinline__ void record(TrajEntry *__restrict__ traj, int64_t slo
     ++cnt;
 }
 
+// One long ScanTask across a wave: lane j owns window column j and works on
+// row a + s - j at step s (an anti-diagonal), so a cell's left and diagonal
+// neighbours were computed by lane j - 1 one and two steps earlier and reach
+// lane j by a DPP shift (wave_shr:1, lane 0 taking the row's inputs).  The
+// reference base travels the same way as its score selector: with symmetric
+// scoring the lane's table is the one of its own query base.  The row
+// maximum travels as a partial (maximum, first column) and is complete at the
+// window's last column, whose lane keeps the running best and records the
+// trajectory.  The N columns past the window that the lane kernel also
+// evaluates never hold a new maximum (each is bounded by the previous row's
+// last real cell), so the records are the same.  R rows take R + width - 1
+// steps instead of R x width lane steps.
+// L0: a level-0 window (rows [0, L1) of its pair): a window with a record
+// joins the hit list, and its cells are counted on the host instead.
+template <int WM, bool L0 = false>
+__device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const ScanTask &t,
+                               const Params &P, const ScoreTab &tab, TrajEntry *__restrict__ traj,
+                               int32_t *__restrict__ tcount, int32_t cap, Counters *__restrict__ ctr,
+                               int32_t *__restrict__ hitlist = nullptr) {
+    const int lane = threadIdx.x;
+    const PairDesc pd = pairs[t.pair];
+    const int8_t *__restrict__ ref = seq + pd.off1;
+    const int8_t *__restrict__ qry = seq + pd.off2;
+    const int sp = t.win * P.step;
+    const int tw = min(P.window, pd.len2 - sp);
+    const int qc = lane < tw ? qry[sp + lane] : 2;
+    const int qi = (unsigned)qc < 8u ? qc : 2;
+    const uint32_t thi = tab.hi[qi], tlo = tab.lo[qi];
+    const int32_t O = P.open, X = P.ext;
+    const int rows = t.b - t.a, steps = rows + tw - 1, last = tw - 1;
+    int32_t E = 0, diag = 0, h_out = 0, f_out = 0, pm_out = -1, pj_out = 0;
+    uint32_t sel_out = 0x0C0C0C0Cu;
+    int32_t best = P.min_score - 1, cnt = 0;
+    // reference bases 64 rows at a time, one per lane (2 past the task's
+    // rows), the next 64 in flight while these are used
+    int32_t cur = lane < rows ? ref[t.a + lane] : 2;
+    int32_t nxt = 64 + lane < rows ? ref[t.a + 64 + lane] : 2;
+    for (int st = 0; st < steps; ++st) {
+        const int c0 = __builtin_amdgcn_readlane(cur, st & 63);  // row a + st
+        if ((st & 63) == 63) {
+            cur = nxt;
+            const int r = st + 65 + lane;
+            nxt = r < rows ? ref[t.a + r] : 2;
+        }
+        const uint32_t sel = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)sel32(c0), (int32_t)sel_out, 0x138, 0xf,
+                                                                   0xf, false);
+        const int32_t fin = __builtin_amdgcn_update_dpp(0, f_out, 0x138, 0xf, 0xf, false);
+        const int32_t hin = __builtin_amdgcn_update_dpp(0, h_out, 0x138, 0xf, 0xf, false);
+        const int32_t pmin = __builtin_amdgcn_update_dpp(-1, pm_out, 0x138, 0xf, 0xf, false);
+        const int32_t pjin = __builtin_amdgcn_update_dpp(0, pj_out, 0x138, 0xf, 0xf, false);
+        const int row = st - lane;
+        const bool valid = row >= 0 && row < rows && lane < tw;
+        int32_t h = 0, fo = 0, pm = pmin, pj = pjin;
+        if (valid) {
+            const int32_t tt = diag + (int32_t)perm(thi, tlo, sel);
+            h = max3i(tt, E, fin);  // E, fin >= 0: includes the 0 floor
+            const int32_t ho = h + O;
+            E = max3i(E + X, ho, 0);
+            fo = max3i(ho, fin + X, 0);
+            if (h > pmin) {
+                pm = h;
+                pj = lane;
+            }
+            if (lane == last && pm > best) {
+                record(traj, t.scan, cap, cnt, t.a + row, pm, pj);
+                best = pm;
+            }
+        }
+        diag = hin;  // H(row, lane - 1): the diagonal of this lane's next row
+        h_out = h;
+        f_out = fo;
+        pm_out = pm;
+        pj_out = pj;
+        sel_out = sel;
+    }
+    if (lane == last) {
+        tcount[t.scan] = cnt;
+        if (cnt > 0) {
+            if (L0) hitlist[atomicAdd(&ctr->hits, 1)] = t.scan;
+            atomicAdd(&ctr->entries, min(cnt, cap));
+        }
+        if (!L0) atomicAdd(&ctr->gen_cells, (unsigned long long)rows * (unsigned long long)max(tw, 0));
+    }
+}
+
+// A level-0 window of one of the longest references, one wave on the
+// anti-diagonal of scan_task_wave: a reference of R rows takes R + width - 1
+// steps instead of R rows of the 128-window lane kernel, at ~10x the VALU per
+// cell.  The level-0 kernels run these windows in their first blocks (wins,
+// nwins; tab32: the int32 score table); the host picks the windows whose
+// lane-kernel rows would otherwise set the launch's duration.
+template <int WM>
+__device__ __forceinline__ void l0_wave_window(const int2 w, const int8_t *__restrict__ seq,
+                                               const PairDesc *__restrict__ pairs, const Params &P,
+                                               const ScoreTab &tab32, TrajEntry *__restrict__ traj,
+                                               int32_t *__restrict__ tcount, int32_t cap,
+                                               int32_t *__restrict__ hitlist, Counters *__restrict__ ctr) {
+    const PairDesc pd = pairs[w.x];
+    ScanTask t;
+    t.pair = w.x;
+    t.win = w.y;
+    t.a = 0;
+    t.b = pd.len1;
+    t.node = -1;
+    t.scan = pd.win_base + w.y;
+    scan_task_wave<WM, true>(seq, pairs, t, P, tab32, traj, tcount, cap, ctr, hitlist);
+}
+
 // The next level-0 task of a persistent wave: lane 0 pops the queue head,
 // the wave reads the index from it (uniform).
 __device__ __forceinline__ int32_t pop_task(int32_t *queue) {
@@ -162,13 +270,19 @@ __device__ __forceinline__ void scan_uniform_task(const int2 task, const int8_t 
 // Level-0 window scan, one wave per 128-window task: task blockIdx.x of the
 // length-sorted list or, with a queue (persistent waves, a grid that fills
 // the chip once), the tasks popped from its head in list order.
-template <int WM, bool PERSIST>
+template <int WM, bool PERSIST, bool WAVES>
 __global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                      const int2 *__restrict__ wtasks, int32_t nwt,
                                                      int32_t *__restrict__ queue, Params P, ScoreTab tab,
                                                      TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
                                                      int32_t cap, int32_t *__restrict__ hitlist,
-                                                     Counters *__restrict__ ctr) {
+                                                     Counters *__restrict__ ctr, const int2 *__restrict__ wins,
+                                                     int32_t nwins) {
+    const int32_t lane_blocks = (int32_t)gridDim.x - nwins;
+    if (WAVES && (int32_t)blockIdx.x >= lane_blocks) {
+        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr);
+        return;
+    }
     if (!PERSIST) {
         scan_uniform_task<WM>(wtasks[blockIdx.x], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr);
         return;
@@ -279,12 +393,17 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
 
 // One wave per task: task blockIdx.x of the length-sorted list (3 waves
 // per SIMD fit its 168 registers).
-template <int WM>
+template <int WM, bool WAVES>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_uniform_h(
     const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const int2 *__restrict__ wtasks, int32_t nwt,
     int32_t *__restrict__ queue, Params P, ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
     int32_t cap, int32_t *__restrict__ hitlist, Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
-    uint32_t thr_h) {
+    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins) {
+    const int32_t lane_blocks = (int32_t)gridDim.x - nwins;
+    if (WAVES && (int32_t)blockIdx.x >= lane_blocks) {
+        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist, ctr);
+        return;
+    }
     scan_uniform_task_h<WM>(wtasks[blockIdx.x], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h,
                             thr_h);
 }
@@ -292,12 +411,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 // Persistent waves: each pops tasks from the queue head in list order until
 // the list is exhausted.  The loop's own state does not fit beside the task's
 // registers at 3 waves per SIMD without spilling, so this one runs 2.
-template <int WM>
+template <int WM, bool WAVES>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_scan_uniform_hp(
     const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const int2 *__restrict__ wtasks, int32_t nwt,
     int32_t *__restrict__ queue, Params P, ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
     int32_t cap, int32_t *__restrict__ hitlist, Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
-    uint32_t thr_h) {
+    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins) {
+    const int32_t lane_blocks = (int32_t)gridDim.x - nwins;
+    if (WAVES && (int32_t)blockIdx.x >= lane_blocks) {
+        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist, ctr);
+        return;
+    }
     for (int32_t t = pop_task(queue); t < nwt; t = pop_task(queue))
         scan_uniform_task_h<WM>(wtasks[t], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h, thr_h);
 }
@@ -309,84 +433,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 // in every cell (a horizontal run followed by a vertical one can always be
 // swapped without changing score or end cell), and the shared H + open term
 // saves two operations per cell.
-// One long ScanTask across a wave: lane j owns window column j and works on
-// row a + s - j at step s (an anti-diagonal), so a cell's left and diagonal
-// neighbours were computed by lane j - 1 one and two steps earlier and reach
-// lane j by a DPP shift (wave_shr:1, lane 0 taking the row's inputs).  The
-// reference base travels the same way as its score selector: with symmetric
-// scoring the lane's table is the one of its own query base.  The row
-// maximum travels as a partial (maximum, first column) and is complete at the
-// window's last column, whose lane keeps the running best and records the
-// trajectory.  The N columns past the window that the lane kernel also
-// evaluates never hold a new maximum (each is bounded by the previous row's
-// last real cell), so the records are the same.  R rows take R + width - 1
-// steps instead of R x width lane steps.
-template <int WM>
-__device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const ScanTask &t,
-                               const Params &P, const ScoreTab &tab, TrajEntry *__restrict__ traj,
-                               int32_t *__restrict__ tcount, int32_t cap, Counters *__restrict__ ctr) {
-    const int lane = threadIdx.x;
-    const PairDesc pd = pairs[t.pair];
-    const int8_t *__restrict__ ref = seq + pd.off1;
-    const int8_t *__restrict__ qry = seq + pd.off2;
-    const int sp = t.win * P.step;
-    const int tw = min(P.window, pd.len2 - sp);
-    const int qc = lane < tw ? qry[sp + lane] : 2;
-    const int qi = (unsigned)qc < 8u ? qc : 2;
-    const uint32_t thi = tab.hi[qi], tlo = tab.lo[qi];
-    const int32_t O = P.open, X = P.ext;
-    const int rows = t.b - t.a, steps = rows + tw - 1, last = tw - 1;
-    int32_t E = 0, diag = 0, h_out = 0, f_out = 0, pm_out = -1, pj_out = 0;
-    uint32_t sel_out = 0x0C0C0C0Cu;
-    int32_t best = P.min_score - 1, cnt = 0;
-    // reference bases 64 rows at a time, one per lane (2 past the task's
-    // rows), the next 64 in flight while these are used
-    int32_t cur = lane < rows ? ref[t.a + lane] : 2;
-    int32_t nxt = 64 + lane < rows ? ref[t.a + 64 + lane] : 2;
-    for (int st = 0; st < steps; ++st) {
-        const int c0 = __builtin_amdgcn_readlane(cur, st & 63);  // row a + st
-        if ((st & 63) == 63) {
-            cur = nxt;
-            const int r = st + 65 + lane;
-            nxt = r < rows ? ref[t.a + r] : 2;
-        }
-        const uint32_t sel = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)sel32(c0), (int32_t)sel_out, 0x138, 0xf,
-                                                                   0xf, false);
-        const int32_t fin = __builtin_amdgcn_update_dpp(0, f_out, 0x138, 0xf, 0xf, false);
-        const int32_t hin = __builtin_amdgcn_update_dpp(0, h_out, 0x138, 0xf, 0xf, false);
-        const int32_t pmin = __builtin_amdgcn_update_dpp(-1, pm_out, 0x138, 0xf, 0xf, false);
-        const int32_t pjin = __builtin_amdgcn_update_dpp(0, pj_out, 0x138, 0xf, 0xf, false);
-        const int row = st - lane;
-        const bool valid = row >= 0 && row < rows && lane < tw;
-        int32_t h = 0, fo = 0, pm = pmin, pj = pjin;
-        if (valid) {
-            const int32_t tt = diag + (int32_t)perm(thi, tlo, sel);
-            h = max3i(tt, E, fin);  // E, fin >= 0: includes the 0 floor
-            const int32_t ho = h + O;
-            E = max3i(E + X, ho, 0);
-            fo = max3i(ho, fin + X, 0);
-            if (h > pmin) {
-                pm = h;
-                pj = lane;
-            }
-            if (lane == last && pm > best) {
-                record(traj, t.scan, cap, cnt, t.a + row, pm, pj);
-                best = pm;
-            }
-        }
-        diag = hin;  // H(row, lane - 1): the diagonal of this lane's next row
-        h_out = h;
-        f_out = fo;
-        pm_out = pm;
-        pj_out = pj;
-        sel_out = sel;
-    }
-    if (lane == last) {
-        tcount[t.scan] = cnt;
-        if (cnt > 0) atomicAdd(&ctr->entries, min(cnt, cap));
-        atomicAdd(&ctr->gen_cells, (unsigned long long)rows * (unsigned long long)max(tw, 0));
-    }
-}
 
 // Tasks of more than wave_rows rows among the first wave_blocks of the
 // order (longest first) run one per wave (the first wave_blocks blocks);
@@ -846,24 +892,41 @@ __global__ void k_gather_roots(const Node *__restrict__ nodes, const int32_t *__
 // ---------------------------------------------------------------------------
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
                                int32_t nwt, int32_t persistent_waves, const Params &P, const ScoreTab &tab,
-                               TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr) {
-    if (nwt <= 0) return hipSuccess;
-    // persistent waves pop the task list from the counter block's queue head
-    // (zeroed by k_seed_init); otherwise one workgroup per task
+                               TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr,
+                               const int2 *wins, int32_t nwins) {
+    if (nwt <= 0 && nwins <= 0) return hipSuccess;
+    // lane tasks first (persistent waves popping the task list from the
+    // counter block's queue head, zeroed by k_seed_init, or one workgroup per
+    // task), then the one-per-wave windows in the last nwins workgroups; the
+    // kernels without the wave path (WAVES false) keep the lane kernels' own
+    // register allocation
     const bool persist = persistent_waves > 0;
     int32_t *queue = &ctr->queue;
-    const dim3 g(persist ? (unsigned)std::min(persistent_waves, nwt) : (unsigned)nwt), b(64);
+    const dim3 g((unsigned)nwins + (persist ? (unsigned)std::min(persistent_waves, nwt) : (unsigned)std::max(nwt, 0))),
+        b(64);
+    const bool waves = nwins > 0;
     if (scan_uses_f16(P)) {
         const ScoreTab th = make_score_tab_f16(P.match, P.mismatch);
         const uint32_t oh = f16_bits_of_int(P.open), xh = f16_bits_of_int(P.ext), thr = f16_bits_of_int(P.min_score - 1);
-        auto k = wm <= 38 ? (persist ? k_scan_uniform_hp<38> : k_scan_uniform_h<38>)
-                          : (persist ? k_scan_uniform_hp<64> : k_scan_uniform_h<64>);
+        decltype(&k_scan_uniform_h<38, false>) k;
+        if (wm <= 38)
+            k = persist ? (waves ? k_scan_uniform_hp<38, true> : k_scan_uniform_hp<38, false>)
+                        : (waves ? k_scan_uniform_h<38, true> : k_scan_uniform_h<38, false>);
+        else
+            k = persist ? (waves ? k_scan_uniform_hp<64, true> : k_scan_uniform_hp<64, false>)
+                        : (waves ? k_scan_uniform_h<64, true> : k_scan_uniform_h<64, false>);
         hipLaunchKernelGGL(k, g, b, 0, st, seq, pairs, wtasks, nwt, queue, P, th, traj, tcount, cap, hitlist, ctr, oh,
-                           xh, thr);
+                           xh, thr, tab, wins, nwins);
     } else {
-        auto k = wm <= 38 ? (persist ? k_scan_uniform<38, true> : k_scan_uniform<38, false>)
-                          : (persist ? k_scan_uniform<64, true> : k_scan_uniform<64, false>);
-        hipLaunchKernelGGL(k, g, b, 0, st, seq, pairs, wtasks, nwt, queue, P, tab, traj, tcount, cap, hitlist, ctr);
+        decltype(&k_scan_uniform<38, false, false>) k;
+        if (wm <= 38)
+            k = persist ? (waves ? k_scan_uniform<38, true, true> : k_scan_uniform<38, true, false>)
+                        : (waves ? k_scan_uniform<38, false, true> : k_scan_uniform<38, false, false>);
+        else
+            k = persist ? (waves ? k_scan_uniform<64, true, true> : k_scan_uniform<64, true, false>)
+                        : (waves ? k_scan_uniform<64, false, true> : k_scan_uniform<64, false, false>);
+        hipLaunchKernelGGL(k, g, b, 0, st, seq, pairs, wtasks, nwt, queue, P, tab, traj, tcount, cap, hitlist, ctr,
+                           wins, nwins);
     }
     return hipGetLastError();
 }
