@@ -1,0 +1,170 @@
+"""Size-independent properties of cut1Gap fragments, checked on the full-size
+workloads the oracle cannot finish: the 4,096-pair batch (config 1) and the
+500 kb x 500 kb long masked pair (config 2).
+
+Every fragment findSimilarFragmentsForPairedSequence reports
+(impl/findSimilarFragmentsForPairedSequence4.cpp:433-524) carries the x-drop
+extension's cigar from its start to its end cell, so:
+
+  * the cigar spans exactly the fragment: M and D runs cover end1 - start1 + 1
+    reference bases, M and I runs end2 - start2 + 1 query bases (1-based,
+    inclusive coordinates);
+  * the cigar scores the fragment's score under the extension's scoring:
+    match / mismatch on M columns, 0 when either base is N (code 2), and a gap
+    run of length L costs open + (L - 1) * ext;
+  * its E-value and p-value are the reference's formula on that score
+    (x_extend_seed, findSimilarFragmentsForPairedSequence4.cpp:282-283).
+
+The CPU tests pin these properties on the compiled reference's own fragments
+(the golden fixtures); the GPU tests then hold this build's full-size
+outputs to them."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MATCH, MISMATCH, OPEN, EXT = 2, -3, -4, -2
+
+
+def fragment_problems(a, b, frags, lam=0.382291, k=0.006662, ext=EXT):
+    """Problems of fragments (start1, end1, start2, end2, score, p, e, cigar)
+    of the code sequences a, b; [] when every property holds."""
+    a = np.asarray(a, np.int8)
+    b = np.asarray(b, np.int8)
+    bad = []
+    for f in frags:
+        s1, e1, s2, e2, score, p, e, cig = f
+        if not (1 <= s1 <= e1 <= len(a) and 1 <= s2 <= e2 <= len(b)):
+            bad.append(("bounds", f[:5]))
+            continue
+        i, j, tot = s1 - 1, s2 - 1, 0
+        for c in cig:
+            n, op = int(c) >> 4, int(c) & 15
+            if op == 0:
+                x, y = a[i:i + n], b[j:j + n]
+                if len(x) != n or len(y) != n:
+                    bad.append(("cigar past the sequence", f[:5]))
+                    break
+                real = (x != 2) & (y != 2)
+                tot += int(np.sum(np.where(real, np.where(x == y, MATCH, MISMATCH), 0)))
+                i += n
+                j += n
+            elif op in (1, 2):
+                tot += OPEN + (n - 1) * ext
+                if op == 1:
+                    j += n
+                else:
+                    i += n
+            else:
+                bad.append(("cigar op", f[:5]))
+                break
+        else:
+            if (i - (s1 - 1), j - (s2 - 1)) != (e1 - s1 + 1, e2 - s2 + 1):
+                bad.append(("span", f[:5], i - (s1 - 1), j - (s2 - 1)))
+            elif tot != score:
+                bad.append(("score", f[:5], tot))
+            else:
+                ev = k * len(a) * len(b) * math.exp(-lam * score)
+                pv = 1.0 - math.exp(-ev)
+                if not (math.isclose(e, ev, rel_tol=1e-12, abs_tol=1e-300) and
+                        math.isclose(p, pv, rel_tol=1e-12, abs_tol=1e-15)):
+                    bad.append(("p-value", f[:5], p, pv))
+    return bad
+
+
+def _codes(s):
+    """A0 C1 N2 G3 T4; every other letter is N."""
+    table = np.full(256, 2, np.int8)
+    for code, base in enumerate("ACNGT"):
+        table[ord(base)] = code
+    return table[np.frombuffer(s.upper().encode(), np.uint8)]
+
+
+def test_properties_hold_on_reference_fragments():
+    """The reference's own fragments (pipeline fixtures, written by the
+    compiled reference) satisfy every property."""
+    cases = json.load(open(os.path.join(GOLDEN, "pipeline_fragments.json")))
+    n = 0
+    for c in cases:
+        frags = [tuple(f) for f in c.get("frags") or c.get("fragments")]
+        a, b = _codes(c["s1"]), _codes(c["s2"])
+        assert fragment_problems(a, b, frags) == []
+        n += len(frags)
+    assert n > 20
+
+
+def test_properties_hold_on_reference_batch_slice():
+    """The compiled reference's fragments of 96 pairs of the 4,096-pair batch
+    (the batch-slice fixture) satisfy every property."""
+    from dcns_amd import synth
+    d = json.load(open(os.path.join(GOLDEN, "batch_slice.json")))
+    pairs = synth.batch()
+    n = 0
+    for p in d["pairs"]:
+        a, b = pairs[p["index"]]
+        frags = [tuple(f) for f in p["fragments"]]
+        assert fragment_problems(a, b, frags) == []
+        n += len(frags)
+    assert n > 500
+
+
+def test_properties_catch_planted_mistakes():
+    """Each property fails for the mistake it is there to catch: a score off
+    by one, a cigar run one base longer, another gap price, and a p-value from
+    another lambda."""
+    cases = json.load(open(os.path.join(GOLDEN, "pipeline_fragments.json")))
+    c = cases[0]
+    a, b = _codes(c["s1"]), _codes(c["s2"])
+    f = tuple(c.get("frags") or c.get("fragments"))[0]
+    assert fragment_problems(a, b, [f]) == []
+    s1, e1, s2, e2, score, p, e, cig = f
+    assert fragment_problems(a, b, [(s1, e1, s2, e2, score + 1, p, e, cig)])[0][0] == "score"
+    longer = list(cig)
+    longer[-1] += 16
+    assert fragment_problems(a, b, [(s1, e1, s2, e2, score, p, e, longer)])[0][0] in ("span", "score")
+    assert [int(x) for x in cig if int(x) & 15], "the first fixture fragment has a gap"
+    # a gap priced open + L * ext instead of open + (L - 1) * ext
+    assert fragment_problems(a, b, [f], ext=2 * EXT)[0][0] == "score"
+    assert fragment_problems(a, b, [f], lam=0.39)[0][0] == "p-value"
+
+
+@pytest.mark.gpu
+def test_config1_full_batch_properties(gpu):
+    """Every fragment of the 4,096-pair batch (config 1), from a staged run
+    as the bench times it."""
+    from dcns_amd import synth
+    pairs = synth.batch()
+    print("config 1: staging 4096 pairs", flush=True)
+    gpu.stage(pairs)
+    res = gpu.find_similar_staged()
+    total, bad = 0, []
+    for (a, b), (_, frags) in zip(pairs, res):
+        total += len(frags)
+        bad += fragment_problems(a, b, frags)
+    print(f"config 1: {total} fragments checked", flush=True)
+    assert total > 30000
+    assert bad == [], bad[:5]
+
+
+@pytest.mark.gpu
+def test_config2_long_masked_pair_properties(gpu):
+    """Every fragment of the 500 kb x 500 kb long masked pair (config 2)
+    through overload 1, and the same properties for every candidate when the
+    p-value filter lets all of them through."""
+    import dcns_amd
+    from dcns_amd import synth
+    t1, t2 = synth.long_masked()
+    a = synth.encode(t1.replace("n", ""))
+    b = synth.encode(t2.replace("n", ""))
+    print(f"config 2: {len(a)} x {len(b)}", flush=True)
+    frags = gpu.find_similar([(a, b)])[0][1]
+    assert len(frags) > 1000
+    assert fragment_problems(a, b, frags) == []
+    cand = gpu.find_similar([(a, b)], params=dcns_amd.default_params(pvalue=1.0))[0][1]
+    print(f"config 2: {len(frags)} fragments, {len(cand)} candidates", flush=True)
+    assert len(cand) > len(frags)
+    bad = fragment_problems(a, b, cand)
+    assert bad == [], bad[:5]
