@@ -1,6 +1,7 @@
-"""Size-independent properties of cut1Gap fragments, checked on the full-size
+"""Size-independent properties of fragments, checked on the full-size
 workloads the oracle cannot finish: the 4,096-pair batch (config 1) and the
-500 kb x 500 kb long masked pair (config 2).
+500 kb x 500 kb long masked pair (config 2) through cut1Gap, and the two-piece
+path (cut2Gaps) on config 2's 120 kb crop and 64 pairs of the batch.
 
 Every fragment findSimilarFragmentsForPairedSequence reports
 (impl/findSimilarFragmentsForPairedSequence4.cpp:433-524) carries the x-drop
@@ -11,9 +12,20 @@ extension's cigar from its start to its end cell, so:
     inclusive coordinates);
   * the cigar scores the fragment's score under the extension's scoring:
     match / mismatch on M columns, 0 when either base is N (code 2), and a gap
-    run of length L costs open + (L - 1) * ext;
+    run of length L costs open + (L - 1) * ext (cut1Gap's x-drop), or the
+    cheaper of the two pieces, max(open + (L - 1) * ext, open2 + (L - 1) *
+    ext2), for cut2Gaps' SemiGlobal (impl/sequenceAlignment3.cpp:681-928);
   * its E-value and p-value are the reference's formula on that score
     (x_extend_seed, findSimilarFragmentsForPairedSequence4.cpp:282-283).
+
+One exception, the reference's own: SemiGlobal fills each row's last band
+column (min(row + band, C)) by a maximum alone, with no gap update and no
+trace byte, so a cut2Gaps fragment whose end cell is that column (it ends at
+the query's last base, or its end lies band columns right of its start's
+diagonal) can carry a score its cigar does not reach: the traceback's first
+step there reads the store's fill value.  The compiled reference reports such
+fragments too (test_two_piece_last_column_quirk_in_reference); they are held
+to every property but the score.
 
 The CPU tests pin these properties on the compiled reference's own fragments
 (the golden fixtures); the GPU tests then hold this build's full-size
@@ -29,9 +41,14 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MATCH, MISMATCH, OPEN, EXT = 2, -3, -4, -2
 
 
-def fragment_problems(a, b, frags, lam=0.382291, k=0.006662, ext=EXT):
+def fragment_problems(a, b, frags, lam=0.382291, k=0.006662, ext=EXT, second_piece=None,
+                      scoring=(MATCH, MISMATCH, OPEN), band=None):
     """Problems of fragments (start1, end1, start2, end2, score, p, e, cigar)
-    of the code sequences a, b; [] when every property holds."""
+    of the code sequences a, b; [] when every property holds.  scoring:
+    (match, mismatch, open); second_piece: (open2, ext2) of a two-piece gap
+    cost.  band: SemiGlobal's band width; a fragment whose end cell is its
+    band's last column (see below) is not held to the score check."""
+    match, mismatch, open_ = scoring
     a = np.asarray(a, np.int8)
     b = np.asarray(b, np.int8)
     bad = []
@@ -49,11 +66,14 @@ def fragment_problems(a, b, frags, lam=0.382291, k=0.006662, ext=EXT):
                     bad.append(("cigar past the sequence", f[:5]))
                     break
                 real = (x != 2) & (y != 2)
-                tot += int(np.sum(np.where(real, np.where(x == y, MATCH, MISMATCH), 0)))
+                tot += int(np.sum(np.where(real, np.where(x == y, match, mismatch), 0)))
                 i += n
                 j += n
             elif op in (1, 2):
-                tot += OPEN + (n - 1) * ext
+                cost = open_ + (n - 1) * ext
+                if second_piece:
+                    cost = max(cost, second_piece[0] + (n - 1) * second_piece[1])
+                tot += cost
                 if op == 1:
                     j += n
                 else:
@@ -64,7 +84,7 @@ def fragment_problems(a, b, frags, lam=0.382291, k=0.006662, ext=EXT):
         else:
             if (i - (s1 - 1), j - (s2 - 1)) != (e1 - s1 + 1, e2 - s2 + 1):
                 bad.append(("span", f[:5], i - (s1 - 1), j - (s2 - 1)))
-            elif tot != score:
+            elif tot != score and not (band is not None and (e2 == len(b) or (e2 - s2) - (e1 - s1) == band)):
                 bad.append(("score", f[:5], tot))
             else:
                 ev = k * len(a) * len(b) * math.exp(-lam * score)
@@ -129,6 +149,86 @@ def test_properties_catch_planted_mistakes():
     # a gap priced open + L * ext instead of open + (L - 1) * ext
     assert fragment_problems(a, b, [f], ext=2 * EXT)[0][0] == "score"
     assert fragment_problems(a, b, [f], lam=0.39)[0][0] == "p-value"
+
+
+def test_two_piece_properties_hold_on_reference_fragments():
+    """The compiled reference's overload-4 fragments (two-piece fixtures,
+    incl. a long-masked crop) satisfy every property with the two-piece gap
+    cost, and fail it with the one-piece cost."""
+    from dcns_amd import synth
+    n = one_piece_fails = 0
+    for c in json.load(open(os.path.join(GOLDEN, "two_piece_fragments.json"))):
+        m, mm, o1, x1, o2, x2 = c["params"]["sc"]
+        frags = [tuple(f) for f in c["fragments"]]
+        a, b = _codes(c["s1"]), _codes(c["s2"])
+        assert fragment_problems(a, b, frags, ext=x1, second_piece=(o2, x2), scoring=(m, mm, o1)) == []
+        one_piece_fails += len(fragment_problems(a, b, frags, ext=x1, scoring=(m, mm, o1)))
+        n += len(frags)
+    d = json.load(open(os.path.join(GOLDEN, "two_piece_masked.json")))
+    t1, t2 = synth.long_masked(seed=88, L=20000, te_copies=8, te_len=3000, nfrag=20)
+    a, b = synth.encode(t1.replace("n", "")), synth.encode(t2.replace("n", ""))
+    assert (len(a), len(b)) == (d["len1"], d["len2"])
+    frags = [tuple(f) for f in d["fragments"]]
+    assert fragment_problems(a, b, frags, second_piece=(-45, 0)) == []
+    one_piece_fails += len(fragment_problems(a, b, frags))
+    assert n + len(frags) > 20 and one_piece_fails > 0
+
+
+def test_two_piece_last_column_quirk_in_reference(ref_lib):
+    """The compiled reference's overload 4 where a fragment ends in the last
+    band column: on the first 10 kb of config 2's pair (a fragment ending at
+    the query's last base), and on pairs of two shared 400 bp fragments with
+    a query insertion about as long as a narrow band (fragments whose end is
+    band columns right of their start's diagonal).  Those fragments score
+    more than their cigars; every other property holds, and no fragment
+    elsewhere breaks one."""
+    from dcns_amd import synth
+    t1, t2 = synth.long_masked()
+    a = synth.encode(t1.replace("n", ""))[:10000]
+    b = synth.encode(t2.replace("n", ""))[:10000]
+    cases = [(a, b, 20000)]
+    rng = np.random.default_rng(7)
+
+    def rs(n):
+        return rng.choice([0, 1, 3, 4], n).astype(np.int8)
+    for _ in range(12):
+        w = int(rng.integers(60, 160))
+        f1, f2, ins = rs(400), rs(400), rs(w + int(rng.integers(-2, 3)))
+        cases.append((np.concatenate([rs(300), f1, f2, rs(300)]), np.concatenate([rs(300), f1, ins, f2, rs(300)]), w))
+    quirks = 0
+    for x, y, w in cases:
+        frags = [tuple(f) for f in ref_lib.find_similar_2gaps(x, y, bandwidth=w)]
+        for f in frags:
+            probs = fragment_problems(x, y, [f], second_piece=(-45, 0))
+            on_edge = f[3] == len(y) or (f[3] - f[2]) - (f[1] - f[0]) == w
+            if probs:
+                assert on_edge and probs[0][0] == "score" and probs[0][2] < f[4], (w, f[:5], probs)
+                quirks += 1
+        assert fragment_problems(x, y, frags, second_piece=(-45, 0), band=w) == []
+    assert quirks >= 2
+
+
+@pytest.mark.gpu
+def test_config2_two_piece_crop_properties(gpu):
+    """Every cut2Gaps fragment of config 2's long masked pair cropped to its
+    first 120 kb (the crop the bench times), and of 64 pairs of the batch."""
+    from dcns_amd import synth
+    t1, t2 = synth.long_masked()
+    a = synth.encode(t1.replace("n", ""))[:120000]
+    b = synth.encode(t2.replace("n", ""))[:120000]
+    print("config 2 crop: two-piece", flush=True)
+    frags = gpu.find_similar_2gaps([(a, b)])[0][1]
+    assert len(frags) > 10
+    bad = fragment_problems(a, b, frags, second_piece=(-45, 0), band=20000)
+    assert bad == [], bad[:5]
+    pairs = synth.batch(npairs=64, seed=123)
+    print("64-pair batch: two-piece", flush=True)
+    total = 0
+    for (x, y), (_, fr) in zip(pairs, gpu.find_similar_2gaps(pairs)):
+        total += len(fr)
+        bad = fragment_problems(x, y, fr, second_piece=(-45, 0), band=20000)
+        assert bad == [], bad[:5]
+    assert total > 64
 
 
 @pytest.mark.gpu
