@@ -335,8 +335,10 @@ def two_piece_config2(ctx, log, cpu=True, crop=10000, gpu_crop=120000):
     from dcns_amd import synth
     from oracle import bind
     t1, t2 = synth.long_masked()
-    a = synth.encode(t1.replace("n", ""))[:gpu_crop]
-    b = synth.encode(t2.replace("n", ""))[:gpu_crop]
+    a = synth.encode(t1.replace("n", ""))
+    b = synth.encode(t2.replace("n", ""))
+    if gpu_crop > 0:
+        a, b = a[:gpu_crop], b[:gpu_crop]
     t = time.perf_counter()
     res = ctx.find_similar_2gaps([(a, b)])[0][1]
     dt = time.perf_counter() - t
@@ -762,6 +764,9 @@ def main():
     ap.add_argument("--mask-gbp", type=float, default=1.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true", help="only the batch-command phase")
+    ap.add_argument("--config2-crop", type=int, default=120000,
+                    help="bases of config 2's stripped pair the two-piece leg runs on the GPU (0: the whole pair, "
+                         "~2 minutes)")
     ap.add_argument("--two-piece-only", action="store_true",
                     help="run only the two-piece (cut2Gaps) and weighted measurements and print them")
     args = ap.parse_args()
@@ -789,7 +794,7 @@ def main():
         ctx = dcns_amd.Context(0, timing=True)
         lg = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
         out = {"two_piece": two_piece_phase(ctx, lg, cpu=not args.no_cpu)}
-        out["two_piece"]["config2"] = two_piece_config2(ctx, lg, cpu=not args.no_cpu)
+        out["two_piece"]["config2"] = two_piece_config2(ctx, lg, cpu=not args.no_cpu, gpu_crop=args.config2_crop)
         if not args.no_weighted:
             out["weighted"] = weighted_phase(ctx, lg, cpu=not args.no_cpu)
         out["seeded"] = seeded_phase(ctx, lg, cpu=not args.no_cpu)
@@ -902,7 +907,7 @@ def main():
 
     if not args.no_two_piece:
         line["two_piece"] = two_piece_phase(ctx, log)
-        line["two_piece"]["config2"] = two_piece_config2(ctx, log, cpu=not args.no_cpu)
+        line["two_piece"]["config2"] = two_piece_config2(ctx, log, cpu=not args.no_cpu, gpu_crop=args.config2_crop)
     if not args.no_weighted:
         line["weighted"] = weighted_phase(ctx, log)
     if not args.no_two_piece:
