@@ -104,14 +104,32 @@ __device__ bool wait_rows(const int32_t *prog, int32_t need, const int32_t *err)
     return ok;
 }
 
-// Scores of one column for reference codes 0..3 (bytes of lo) and 4 (hi).
+// Scores of one column as v_perm_b32 sources: the score against reference
+// code 0 and 1 in bytes 1 and 3 of lo, against 3 and 4 in bytes 1 and 3 of hi
+// (N scores 0 and has no byte).  row_selector(rc) then takes the row's byte
+// into byte 0 and replicates its sign into bytes 1-3 (selectors 8-11 are the
+// signs of bytes 1, 3, 5, 7), or yields 0 for N (selector 12): one v_perm
+// gives the sign-extended score.
 __device__ __forceinline__ void column_table(int32_t qc, int32_t match, int32_t mismatch, uint32_t &lo, uint32_t &hi) {
-    lo = 0;
-    for (int rc = 0; rc < 4; ++rc) {
-        const int32_t s = (rc == 2 || qc == 2) ? 0 : (rc == qc ? match : mismatch);
-        lo |= ((uint32_t)s & 0xffu) << (8 * rc);
-    }
-    hi = (uint32_t)((qc == 2) ? 0 : (qc == 4 ? match : mismatch)) & 0xffu;
+    auto s = [&](int32_t rc) { return (uint32_t)((qc == 2) ? 0 : (rc == qc ? match : mismatch)) & 0xffu; };
+    lo = (s(0) << 8) | (s(1) << 24);
+    hi = (s(3) << 8) | (s(4) << 24);
+}
+__device__ __forceinline__ uint32_t row_selector(int32_t rc) {
+    return rc == 0 ? 0x08080801u : rc == 1 ? 0x09090903u : rc == 3 ? 0x0A0A0A05u : rc == 4 ? 0x0B0B0B07u : 0x0C0C0C0Cu;
+}
+
+// Keyed row maximum: (maximum, first column) as one int, maximum << 9 | a
+// tie rank, so that one v_max keeps the first strict maximum.  Inside a strip
+// a local column lc (0..255) ranks 510 - lc; the value carried in from the
+// strips to the left ranks 511 and keeps its column beside the key.  The host
+// uses it only where every in-band score fits 22 bits.
+constexpr int kKeyShift = 9;
+constexpr int32_t kKeyIn = 511;
+__device__ __forceinline__ int32_t key_max(int32_t key) { return key >> kKeyShift; }
+__device__ __forceinline__ int32_t key_col(int32_t key, int32_t col_in, int32_t jb) {
+    const int32_t rank = key & kKeyIn;
+    return rank == kKeyIn ? col_in : jb + (kKeyIn - 1) - rank;
 }
 
 // Lane-0 input of row r of a strip: the previous strip's boundary record,
@@ -157,23 +175,31 @@ struct StripIn {
 // hold anything; columns right of the band end keep the NEG vertical state
 // they start with until they become the band's last column, which takes the
 // maximum only.  EDGE = false: every column is inside the band.
-template <bool TB, bool EDGE>
+// KEYED: rm is the keyed row maximum (rank base kc for column 0 of the lane)
+// and rj is unused.
+template <bool TB, bool EDGE, bool KEYED>
 __device__ __forceinline__ void columns(const uint32_t (&tlo)[K], const uint32_t (&thi)[K], int32_t (&F1)[K],
-                                        int32_t (&F2)[K], int32_t (&Mp)[K], bool hsel, uint32_t sh, int32_t lo,
-                                        int32_t hi, int32_t jfirst, int32_t o1, int32_t x1, int32_t o2, int32_t x2,
+                                        int32_t (&F2)[K], int32_t (&Mp)[K], uint32_t sel, int32_t lo, int32_t hi,
+                                        int32_t jfirst, int32_t kc, int32_t o1, int32_t x1, int32_t o2, int32_t x2,
                                         int32_t dg, int32_t &e1, int32_t &e2, int32_t &rm, int32_t &rj,
                                         uint32_t &tword) {
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-        const int32_t sc = __builtin_amdgcn_sbfe((int32_t)(hsel ? thi[c] : tlo[c]), sh, 8);
+        const int32_t sc = (int32_t)perm(thi[c], tlo[c], sel);
         const int32_t m0 = sc + dg;
         dg = Mp[c];
         const int32_t f1 = F1[c], f2o = F2[c];
         const int32_t m = max(max(max(m0, f1), max(e1, f2o)), e2);
         const bool left = EDGE && c < lo, inner = !EDGE || c < hi;
-        const bool up = (EDGE ? (!left & (c <= hi)) : true) & (m > rm);
-        rm = up ? m : rm;
-        rj = up ? jfirst + c : rj;
+        const bool inb = EDGE ? (!left & (c <= hi)) : true;
+        if (KEYED) {
+            const int32_t key = (int32_t)(((uint32_t)m << kKeyShift) | (uint32_t)(kc - c));
+            rm = inb ? max(rm, key) : rm;
+        } else {
+            const bool up = inb & (m > rm);
+            rm = up ? m : rm;
+            rj = up ? jfirst + c : rj;
+        }
         const int32_t h = m + o1, f = f1 + x1, ne1 = e1 + x1;
         const int32_t h2 = m + o2, f2 = f2o + x2, ne2 = e2 + x2;
         if (TB) {
@@ -259,7 +285,7 @@ __device__ __forceinline__ void columns_w(const int32_t (&qc)[K], int32_t (&F1)[
 // sync[1] = the error word, sync[2 + k] = rows of strip k whose boundary
 // records are complete.  MODE 0: SemiGlobal_no_avx; 1: weighted two-piece;
 // 2: weighted single gap (profiles in W).
-template <bool TB, int MODE, bool X2Z>
+template <bool TB, int MODE, bool X2Z, bool KEYED>
 __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                  const SgTask *__restrict__ tasks, const SgStrip *__restrict__ strips,
                                                  const int32_t *__restrict__ order, int32_t total, Params P,
@@ -312,6 +338,8 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
     const int32_t nsteps = S.rhi - S.rlo + 1 + 63;
     const int32_t jfirst = S.s * SW + 1 + lane * K;
     const int32_t hi_c = C - jfirst;  // the lane's band end in right-edge blocks
+    const int32_t jb = S.s * SW + 1;  // the strip's first column
+    const int32_t kc = (kKeyIn - 1) - lane * K;  // keyed row maximum: rank of the lane's first column
     uint32_t tlo[K], thi[K];
     int32_t qc[K], F1[K], F2[K], Mp[K];
 #pragma unroll
@@ -321,7 +349,10 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
         if (MODE == 0) column_table(qc[c], P.match, P.mismatch, tlo[c], thi[c]);
         F1[c] = F2[c] = Mp[c] = NEG;
     }
-    int32_t prev_im = NEG, om = NEG, oe1 = NEG, oe2 = NEG, orm = 0, orj = 0, orc = 2;
+    // carried row maximum: (max, column), or keyed (key, column carried in)
+    constexpr int32_t kRm0 = KEYED ? kKeyIn : 0;
+    int32_t prev_im = NEG, om = NEG, oe1 = NEG, oe2 = NEG, orm = kRm0, orj = 0,
+            orc = MODE == 0 ? (int32_t)row_selector(2) : 2;
     bool ok = true;
     for (int32_t t0 = 0; t0 < nsteps; t0 += 64) {
         if (!in.first) {
@@ -341,6 +372,15 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
             // 8-int record per step in LDS, read back by every lane at once
             int32_t cm, ce1, ce2, crm, crj, crc;
             in.load(S.rlo + t0 + lane, cm, ce1, ce2, crm, crj, crc);
+            if (MODE == 0) crc = (int32_t)row_selector(crc);  // the row's score selector travels instead of its code
+            if (KEYED) {
+                // the previous strip's record holds its keyed maximum and the
+                // column carried into it: decode, then rank it above every
+                // column of this strip
+                if (!in.first && crm != 0) crj = key_col(crm, crj, jb - SW);
+                crm = ((in.first ? 0 : key_max(crm)) << kKeyShift) | kKeyIn;
+                if (in.first) crj = 0;
+            }
             int32_t *cs = cstage + lane * 8;
             cs[0] = cm;
             cs[1] = ce1;
@@ -390,14 +430,13 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
                 // every lane's columns inside the band (the usual case): no edge masks
                 const bool interior = KIND == 1 || (KIND == 0 && __ballot(lo > 0 || hi < K) == 0);
                 if (MODE == 0) {
-                    const bool hsel = irc >= 4;
-                    const uint32_t sh = (uint32_t)(irc & 3) * 8u;
+                    const uint32_t sel = (uint32_t)irc;
                     if (interior)
-                        columns<TB, false>(tlo, thi, F1, F2, Mp, hsel, sh, lo, hi, jfirst, o1, x1, o2, x2, diag, e1,
-                                           e2, rm, rj, tword);
+                        columns<TB, false, KEYED>(tlo, thi, F1, F2, Mp, sel, lo, hi, jfirst, kc, o1, x1, o2, x2, diag,
+                                                  e1, e2, rm, rj, tword);
                     else
-                        columns<TB, true>(tlo, thi, F1, F2, Mp, hsel, sh, lo, hi, jfirst, o1, x1, o2, x2, diag, e1,
-                                          e2, rm, rj, tword);
+                        columns<TB, true, KEYED>(tlo, thi, F1, F2, Mp, sel, lo, hi, jfirst, kc, o1, x1, o2, x2, diag,
+                                                 e1, e2, rm, rj, tword);
                 } else {
                     // this row's profile and the next row's (vertical gaps)
                     const int32_t rc = irc & 15;
@@ -423,7 +462,8 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
             } else {
                 om = NEG;
                 oe1 = oe2 = NEG;
-                orm = orj = 0;
+                orm = kRm0;
+                orj = 0;
             }
             orc = irc;
             if (TB) *(uint32_t *)(tb + T.tb + S.tb + (int64_t)t * SW + lane * K) = tword;
@@ -436,8 +476,8 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
                 b[3] = orm;
                 b[4] = orj;
                 if (KIND == 2 || (KIND == 0 && (min(r + w, C) - 1) / SW == S.s)) {  // the row's last strip holds its result
-                    ast(rowres + 2 * (int64_t)r, orm);
-                    ast(rowres + 2 * (int64_t)r + 1, orj);
+                    ast(rowres + 2 * (int64_t)r, KEYED ? key_max(orm) : orm);
+                    ast(rowres + 2 * (int64_t)r + 1, KEYED ? key_col(orm, orj, jb) : orj);
                 }
             }
         };
@@ -453,7 +493,7 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
         // A right-edge block has the same rows and left edge, but column C lies
         // in this strip and the band of the block's first row already reaches
         // it, so every row's band ends at C.
-        const int32_t jb = S.s * SW + 1, rmin_b = S.rlo + t0 - 63, rmax_b = S.rlo + t0 + 63;
+        const int32_t rmin_b = S.rlo + t0 - 63, rmax_b = S.rlo + t0 + 63;
         const bool rows_left = t1 - t0 == 64 && t0 >= 63 && rmax_b <= S.rhi && rmax_b - w <= jb &&
                                !(MODE == 0 && S.s == stale_strip && rmax_b >= C + w - 2);
         // two steps per iteration, each step's LDS record read one step ahead
@@ -748,35 +788,65 @@ int32_t sg_task_strips(int32_t R, int32_t C, int32_t w, int32_t task, int64_t bn
     return n;
 }
 
-hipError_t launch_sg_dp(hipStream_t st, bool trace, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
-                        const SgStrip *strips, const int32_t *order, int32_t nstrips, const Params &P, int32_t *scr,
-                        uint8_t *tb,
+bool sg_keyed_ok(const Params &P, int64_t max_rows, int64_t max_cols) {
+    // Every in-band M(i, j) lies between the score of a diagonal of
+    // mismatches to (min, min) followed by one gap run (which stays in the
+    // band), and match * min(rows, cols); the key holds scores below 2^22 in
+    // magnitude (here 2^21, with margin).
+    const int64_t L = std::max<int64_t>(max_cols, 1);
+    const int64_t gap = std::max((int64_t)P.open + (L - 1) * P.ext, (int64_t)P.open2 + (L - 1) * P.ext2);
+    const int64_t low = std::min<int64_t>(P.mismatch, 0) * std::max<int64_t>(max_rows, 0) + std::min<int64_t>(gap, 0);
+    const int64_t high = (int64_t)std::max(P.match, 0) * std::min(max_rows, max_cols);
+    const int64_t lim = (int64_t)1 << 21;
+    return -low < lim && high < lim;
+}
+
+hipError_t launch_sg_dp(hipStream_t st, bool trace, bool keyed, const int8_t *seq, const PairDesc *pairs,
+                        const SgTask *tasks, const SgStrip *strips, const int32_t *order, int32_t nstrips,
+                        const Params &P, int32_t *scr, uint8_t *tb,
                         SgOut *out, const SgWeights &W, int32_t *sync) {
     if (nstrips <= 0) return hipSuccess;
     if (W.mode != 0 && (W.ncat < 1 || W.ncat > kMaxProfiles)) return hipErrorInvalidValue;
-#define DCNS_SG_LAUNCH(TBV, MV, X2V)                                                                                 \
-    hipLaunchKernelGGL((k_sg_strip<TBV, MV, X2V>), dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips, order,  \
+#define DCNS_SG_LAUNCH(TBV, MV, X2V, KV)                                                                             \
+    hipLaunchKernelGGL((k_sg_strip<TBV, MV, X2V, KV>), dim3(nstrips), dim3(64), 0, st, seq, pairs, tasks, strips,     \
+                       order,                                                                                       \
                        nstrips, P,                                                                                  \
                        scr, tb, out, W, sync)
     switch (W.mode * 2 + (trace ? 1 : 0)) {
         // the unweighted kernel with a flat second gap piece (ext2 0, the
         // cut2Gaps default) drops the piece's two extension adds per cell
+        // keyed: the row maximum and its column as one int (see key_max),
+        // where every in-band score fits 22 bits
         case 0:
-            if (P.ext2 == 0)
-                DCNS_SG_LAUNCH(false, 0, true);
-            else
-                DCNS_SG_LAUNCH(false, 0, false);
+            if (P.ext2 == 0) {
+                if (keyed)
+                    DCNS_SG_LAUNCH(false, 0, true, true);
+                else
+                    DCNS_SG_LAUNCH(false, 0, true, false);
+            } else {
+                if (keyed)
+                    DCNS_SG_LAUNCH(false, 0, false, true);
+                else
+                    DCNS_SG_LAUNCH(false, 0, false, false);
+            }
             break;
         case 1:
-            if (P.ext2 == 0)
-                DCNS_SG_LAUNCH(true, 0, true);
-            else
-                DCNS_SG_LAUNCH(true, 0, false);
+            if (P.ext2 == 0) {
+                if (keyed)
+                    DCNS_SG_LAUNCH(true, 0, true, true);
+                else
+                    DCNS_SG_LAUNCH(true, 0, true, false);
+            } else {
+                if (keyed)
+                    DCNS_SG_LAUNCH(true, 0, false, true);
+                else
+                    DCNS_SG_LAUNCH(true, 0, false, false);
+            }
             break;
-        case 2: DCNS_SG_LAUNCH(false, 1, false); break;
-        case 3: DCNS_SG_LAUNCH(true, 1, false); break;
-        case 4: DCNS_SG_LAUNCH(false, 2, false); break;
-        case 5: DCNS_SG_LAUNCH(true, 2, false); break;
+        case 2: DCNS_SG_LAUNCH(false, 1, false, false); break;
+        case 3: DCNS_SG_LAUNCH(true, 1, false, false); break;
+        case 4: DCNS_SG_LAUNCH(false, 2, false, false); break;
+        case 5: DCNS_SG_LAUNCH(true, 2, false, false); break;
         default: return hipErrorInvalidValue;
     }
 #undef DCNS_SG_LAUNCH

@@ -120,6 +120,12 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
             t.nstrips = sg_task_strips(t.R, t.C, P.bandwidth, k, t.scr + 2 * ((int64_t)t.R + 2), strips);
         }
         const int32_t ns = (int32_t)strips.size();
+        int64_t max_rows = 0, max_cols = 0;
+        for (const SgTask &t : ct) {
+            max_rows = std::max<int64_t>(max_rows, t.R);
+            max_cols = std::max<int64_t>(max_cols, t.C);
+        }
+        const bool keyed = sg_keyed_ok(P, max_rows, max_cols);
         // Launch order: longest remaining chain first.  A strip's chain is the
         // steps from its start to its task's end: its own rows, or the next
         // strip's chain one hand-over (two 64-step blocks) later, whichever
@@ -160,7 +166,7 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
         hip_check(hipMemcpyAsync(c->sg_out.p, so.data(), (size_t)m * sizeof(SgOut), hipMemcpyHostToDevice, c->stream),
                   "sg out");
         c->launch("semiglobal", [&] {
-            return launch_sg_dp(c->stream, trace, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
+            return launch_sg_dp(c->stream, trace, keyed, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
                                 c->sg_strips.as<SgStrip>(), c->sg_order.as<int32_t>(), ns, P, c->sg_scr.as<int32_t>(),
                                 trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>(), W,
                                 c->sg_sync.as<int32_t>());
