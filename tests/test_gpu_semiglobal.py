@@ -75,6 +75,37 @@ def test_semiglobal_tasks_vs_oracle(gpu, oracle_lib, bandwidth, zdrop, sc2):
         assert cig[i] == wc, (i, k, a1, a2, r)
 
 
+@pytest.mark.parametrize("sc2", [(2, -3, -4, -2, -45, 0), (2, -3, -128, -128, -1000000, -10000)])
+def test_semiglobal_wide_query_vs_oracle(gpu, oracle_lib, sc2):
+    """A 2 kb reference against a 17 kb query holding a mutated copy of it,
+    forward and reverse tasks from anchors along the copy.  With the default
+    scoring every in-band score fits the kernel's keyed row maximum (score
+    and column in one int).  With gap costs of -128 per base and a second
+    piece that never helps, a gap across the query costs ~2.2e6, past the
+    22-bit bound, so the launch takes the kernel that carries the maximum
+    and its column apart.  Both are compared with the oracle, cigars
+    included."""
+    rng = np.random.default_rng(17)
+    a = synth.random_seq(rng, 2000)
+    b = np.concatenate([synth.random_seq(rng, 7000), synth.mutate(rng, a, 0.1, 0.0), synth.random_seq(rng, 8000)])
+    # anchors on matching bases of the copy (a SemiGlobal whose first row has
+    # no positive cell ends there), plus the two sequence ends
+    on = [x for x in range(0, 2000, 37) if a[x] == b[7000 + x]]
+    anchors = [(x, 7000 + x) for x in (on[0], on[len(on) // 3], on[2 * len(on) // 3], on[-1])]
+    anchors += [(0, 0), (1999, len(b) - 1)]
+    tasks = [0] * (2 * len(anchors))
+    anc = [x for x in anchors for _ in (0, 1)]
+    rev = [r for _ in anchors for r in (0, 1)]
+    p = _params(sc2, 50, 20000)
+    print("wide query", sc2, flush=True)
+    out, cig = gpu.semiglobal([(a, b)], tasks, anc, rev, params=p)
+    for i, ((a1, a2), r) in enumerate(zip(anc, rev)):
+        s1, s2 = (_rc(a[:a1 + 1]), _rc(b[:a2 + 1])) if r else (a[a1:], b[a2:])
+        want, wc = oracle_lib.semiglobal(s1, s2, sc2, 50, 20000)
+        assert tuple(int(v) for v in out[i]) == want, (i, a1, a2, r)
+        assert cig[i] == wc, (i, a1, a2, r)
+
+
 def _cmp_frags(got, want):
     assert len(got) == len(want)
     for g, w in zip(got, want):
