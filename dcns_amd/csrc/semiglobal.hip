@@ -710,14 +710,37 @@ __global__ __launch_bounds__(64) void k_sg_finish(const int8_t *__restrict__ seq
             for (int g = 0; g < 16; ++g) tile[lane * 16 + g] = wv[g];
         }
         __syncthreads();
+        // The walk one run at a time.  A cell's step is: in state 0 take the
+        // byte's source (tmp & 7); in a gap state keep it while the byte's
+        // extension bit (state + 2) is set, else fall to 0 and take the
+        // source; then move diagonally (0), up (1, 3) or left.  So from
+        // state 0 the walk goes diagonally over the bytes whose source is 0,
+        // and from a gap state straight on over the bytes with its extension
+        // bit; the first other byte sets the state to its source and takes
+        // one move.  Lane k reads the k-th cell of the run's line at once;
+        // a ballot finds where the run stops (that byte, the tile's edge or
+        // the matrix's).
         while (ii > 0 && jj > 0 && ii >= r0 && jj >= c0) {
-            const uint32_t wd = tile[(ii - r0) * 16 + ((jj - c0) >> 2)];
-            const int32_t tmp = (int32_t)((wd >> (8 * ((jj - c0) & 3))) & 0xffu);
-            if (state == 0)
-                state = tmp & 7;
-            else if (!((tmp >> (state + 2)) & 1))
-                state = 0;
-            if (state == 0) state = tmp & 7;
+            const int32_t st = state;
+            const bool vert = st == 1 || st == 3;
+            const int32_t dy = (st == 0 || vert) ? 1 : 0, dx = vert ? 0 : 1;
+            const int32_t ci = ii - lane * dy, cj = jj - lane * dx;
+            const bool in = ci > 0 && cj > 0 && ci >= r0 && cj >= c0;
+            int32_t tmp = 0;
+            if (in) {
+                const uint32_t wd = tile[(ci - r0) * 16 + ((cj - c0) >> 2)];
+                tmp = (int32_t)((wd >> (8 * ((cj - c0) & 3))) & 0xffu);
+            }
+            const bool cont = in && (st == 0 ? (tmp & 7) == 0 : ((tmp >> (st + 2)) & 1) != 0);
+            const uint64_t stop = __ballot(!cont);
+            const int32_t n = stop ? __ffsll((unsigned long long)stop) - 1 : 64;
+            if (n > 0) {
+                push(st == 0 ? 0 : (vert ? 2 : 1), n);
+                ii -= n * dy;
+                jj -= n * dx;
+            }
+            if (n == 64 || !__builtin_amdgcn_readlane((int32_t)in, n)) continue;  // the tile's or matrix's edge
+            state = __builtin_amdgcn_readlane(tmp, n) & 7;
             if (state == 0) {
                 push(0, 1);
                 --ii;
