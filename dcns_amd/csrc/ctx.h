@@ -136,6 +136,7 @@ struct dcns_ctx {
     std::vector<hipEvent_t> event_pool;
     // reusable device buffers
     dcns::DBuf seq, pairs, wtasks, l0wins, pow, traj, tcount, hitlist, ctr, nodes, roots, tasks_a, tasks_b;
+    dcns::DBuf lastrow;  // per window: the last level-0 row whose maximum reaches min_score
     dcns::DBuf xtasks, xout, tb_off, tb, cigar, cig_host_dummy;
     dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound, sw_woff;
     dcns::DBuf bas_cig, bas_st;  // SmithWaterman_bas walks: cigar entries, counts and status

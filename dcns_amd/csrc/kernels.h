@@ -32,7 +32,7 @@ inline bool scan_general_packed(const Params &P, uint32_t &mis4, uint32_t &diff)
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
                                int32_t nwt, int32_t persistent_waves, const Params &P, const ScoreTab &tab,
                                TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr,
-                               const int2 *wins, int32_t nwins);
+                               const int2 *wins, int32_t nwins, int32_t *lastrow);
 // Later-level rescans: *ntasks_dev tasks (at most ntasks_max) in order[];
 // up to wave_tasks of the first ones with more than wave_rows rows run one
 // per wave, the rest one per lane on at most lane_blocks_max waves (0: one
@@ -47,7 +47,7 @@ hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairD
                           const int32_t *hitlist, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
                           int32_t blocks_max, Node *nodes, int32_t node_cap, const TrajEntry *traj,
                           const int32_t *tcount, int32_t cap, int32_t scan_cap, ScanTask *next, int32_t next_cap,
-                          Counters *ctr);
+                          Counters *ctr, const int32_t *lastrow);
 // Counter block at the start of a seed phase (nodes = scans = nw, the rest 0).
 hipError_t launch_seed_init(hipStream_t st, Counters *ctr, int32_t nw);
 // Start of a recursion level: level_tasks = next_tasks (0 after an

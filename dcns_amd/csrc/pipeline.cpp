@@ -300,6 +300,7 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
     };
     ensure_pools();
     c->hitlist.ensure((size_t)NW * sizeof(int32_t) + 64);
+    c->lastrow.ensure((size_t)NW * sizeof(int32_t) + 64);
     c->ctr.ensure(sizeof(Counters) + 64);
     c->order_hist.ensure(2 * kOrderBuckets * sizeof(int32_t));
     Counters *ctr = c->ctr.as<Counters>();
@@ -312,7 +313,7 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
                               L == 0, c->hitlist.as<int32_t>(), L == 0 ? nullptr : buf(L - 1),
                               L == 0 ? hits_dev : lt_dev, n_max, blocks_max, c->nodes.as<Node>(),
                               (int32_t)SP.node_cap, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
-                              (int32_t)SP.scan_cap, buf(L), (int32_t)SP.next_cap, ctr);
+                              (int32_t)SP.scan_cap, buf(L), (int32_t)SP.next_cap, ctr, c->lastrow.as<int32_t>());
     };
     auto level_launches = [&](int L, int32_t n_max, int32_t lane_blocks_max, bool timed) {
         auto run = [&](const char *name, auto f) {
@@ -337,7 +338,7 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
         return launch_scan_uniform(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->wtasks.as<int2>(),
                                    (int32_t)S.wt.size(), T.level0_waves, P, tab, c->traj.as<TrajEntry>(),
                                    c->tcount.as<int32_t>(), cap, c->hitlist.as<int32_t>(), ctr,
-                                   c->l0wins.as<int2>(), (int32_t)S.ww.size());
+                                   c->l0wins.as<int2>(), (int32_t)S.ww.size(), c->lastrow.as<int32_t>());
     });
 
     Counters h{};
@@ -352,7 +353,8 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
         std::vector<int64_t> key = {(int64_t)c->seq.p, (int64_t)pd, (int64_t)c->pow.p, (int64_t)c->hitlist.p,
                                     (int64_t)c->nodes.p, (int64_t)c->traj.p, (int64_t)c->tcount.p,
                                     (int64_t)c->tasks_a.p, (int64_t)c->tasks_b.p, (int64_t)c->task_order.p,
-                                    (int64_t)c->order_hist.p, (int64_t)ctr, NW, SP.scan_cap, SP.node_cap,
+                                    (int64_t)c->order_hist.p, (int64_t)c->lastrow.p, (int64_t)ctr, NW,
+                                    SP.scan_cap, SP.node_cap,
                                     SP.next_cap, T.wave_tasks, T.wave_rows, T.graph_levels, wm, cap,
                                     P.match, P.mismatch, P.open, P.ext, P.window, P.step, P.min_score, P.min_ref};
         // a layout seen for the first time runs host-driven; one that recurs

@@ -28,6 +28,15 @@
 // running maximum rose strictly above both every earlier cell and
 // min_score - 1.  The first maximum of any row prefix [a, b') of the scan is
 // then the last entry with row < b'.
+//
+// Level 0 also records, per window, the last row whose maximum reaches
+// min_score (lastrow).  A rescan is a fresh SmithWaterman over rows [a, b)
+// of the same window: every value is >= 0 and every step of the recurrence
+// is monotone, so starting from zeros at row a instead of from the level-0
+// values there never raises a cell.  A rescan's trajectory entry (a cell of
+// at least min_score) can therefore only lie on a row where level 0 reached
+// min_score: k_resolve stops every rescan after the window's lastrow, and
+// resolves a right child that starts past it as empty without scanning it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,6 +68,16 @@ __device__ __forceinline__ void record(TrajEntry *__restrict__ traj, int64_t slo
     ++cnt;
 }
 
+// lastrow of a 128-window task's windows (base + lane: the low half, base +
+// 64 + lane: the high half) whose halves of hit are nonzero, set to row.
+// The lane index comes from mbcnt here rather than from a register kept
+// through the row loop, which has none to spare.
+__device__ __forceinline__ void mark_lastrow(int32_t *__restrict__ lastrow, uint32_t base, uint32_t hit, int32_t row) {
+    const uint32_t ln = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    if (hit & 0xffffu) lastrow[base + ln] = row;
+    if (hit >> 16) lastrow[base + 64u + ln] = row;
+}
+
 // One long ScanTask across a wave: lane j owns window column j and works on
 // row a + s - j at step s (an anti-diagonal), so a cell's left and diagonal
 // neighbours were computed by lane j - 1 one and two steps earlier and reach
@@ -77,7 +96,7 @@ template <int WM, bool L0 = false>
 __device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const ScanTask &t,
                                const Params &P, const ScoreTab &tab, TrajEntry *__restrict__ traj,
                                int32_t *__restrict__ tcount, int32_t cap, Counters *__restrict__ ctr,
-                               int32_t *__restrict__ hitlist = nullptr) {
+                               int32_t *__restrict__ hitlist = nullptr, int32_t *__restrict__ lastrow = nullptr) {
     const int lane = threadIdx.x;
     const PairDesc pd = pairs[t.pair];
     const int8_t *__restrict__ ref = seq + pd.off1;
@@ -122,9 +141,12 @@ __device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *_
                 pm = h;
                 pj = lane;
             }
-            if (lane == last && pm > best) {
-                record(traj, t.scan, cap, cnt, t.a + row, pm, pj);
-                best = pm;
+            if (lane == last && pm >= P.min_score) {
+                if (L0) lastrow[t.scan] = t.a + row;  // rows rise: the last store is the last such row
+                if (pm > best) {
+                    record(traj, t.scan, cap, cnt, t.a + row, pm, pj);
+                    best = pm;
+                }
             }
         }
         diag = hin;  // H(row, lane - 1): the diagonal of this lane's next row
@@ -155,7 +177,8 @@ __device__ __forceinline__ void l0_wave_window(const int2 w, const int8_t *__res
                                                const PairDesc *__restrict__ pairs, const Params &P,
                                                const ScoreTab &tab32, TrajEntry *__restrict__ traj,
                                                int32_t *__restrict__ tcount, int32_t cap,
-                                               int32_t *__restrict__ hitlist, Counters *__restrict__ ctr) {
+                                               int32_t *__restrict__ hitlist, Counters *__restrict__ ctr,
+                                               int32_t *__restrict__ lastrow) {
     const PairDesc pd = pairs[w.x];
     ScanTask t;
     t.pair = w.x;
@@ -164,7 +187,7 @@ __device__ __forceinline__ void l0_wave_window(const int2 w, const int8_t *__res
     t.b = pd.len1;
     t.node = -1;
     t.scan = pd.win_base + w.y;
-    scan_task_wave<WM, true>(seq, pairs, t, P, tab32, traj, tcount, cap, ctr, hitlist);
+    scan_task_wave<WM, true>(seq, pairs, t, P, tab32, traj, tcount, cap, ctr, hitlist, lastrow);
 }
 
 // The next level-0 task of a persistent wave: lane 0 pops the queue head,
@@ -180,7 +203,8 @@ __device__ __forceinline__ void scan_uniform_task(const int2 task, const int8_t 
                                                   const PairDesc *__restrict__ pairs, const Params &P,
                                                   const ScoreTab &tab, TrajEntry *__restrict__ traj,
                                                   int32_t *__restrict__ tcount, int32_t cap,
-                                                  int32_t *__restrict__ hitlist, Counters *__restrict__ ctr) {
+                                                  int32_t *__restrict__ hitlist, Counters *__restrict__ ctr,
+                                                  int32_t *__restrict__ lastrow) {
     const int lane = threadIdx.x;
     const PairDesc pd = pairs[task.x];
     const int8_t *__restrict__ ref = seq + pd.off1;
@@ -204,7 +228,8 @@ __device__ __forceinline__ void scan_uniform_task(const int2 task, const int8_t 
     const uint32_t OPEN = (uint32_t)(-P.open) * 0x10001u;
     const uint32_t EXT = (uint32_t)(-P.ext) * 0x10001u;
     const uint32_t thr = (uint32_t)(P.min_score - 1);
-    uint32_t best = thr | (thr << 16);
+    const uint32_t thr2 = thr | (thr << 16);
+    uint32_t best = thr2;
     int32_t cntl = 0, cnth = 0;
 
     uint32_t word = L1 > 0 ? *(const uint32_t *)ref : 0u;
@@ -229,8 +254,10 @@ __device__ __forceinline__ void scan_uniform_task(const int2 task, const int8_t 
             H[j] = h;
             rmax = pk_max_s(rmax, h);
         }
-        const uint32_t imp = pk_subsat(rmax, best);
-        if (__builtin_expect(imp != 0, 0)) {
+        const uint32_t hit = pk_subsat(rmax, thr2);  // halves whose row reaches min_score
+        if (__builtin_expect(hit != 0, 0)) {
+            mark_lastrow(lastrow, (uint32_t)(pd.win_base + task.y), hit, i);
+            const uint32_t imp = pk_subsat(rmax, best);
             if (imp & 0xffffu) {
                 const uint32_t v = rmax & 0xffffu;
                 int jj = 0;
@@ -277,18 +304,19 @@ __global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ 
                                                      TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
                                                      int32_t cap, int32_t *__restrict__ hitlist,
                                                      Counters *__restrict__ ctr, const int2 *__restrict__ wins,
-                                                     int32_t nwins) {
+                                                     int32_t nwins, int32_t *__restrict__ lastrow) {
     const int32_t lane_blocks = (int32_t)gridDim.x - nwins;
     if (WAVES && (int32_t)blockIdx.x >= lane_blocks) {
-        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr);
+        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr,
+                           lastrow);
         return;
     }
     if (!PERSIST) {
-        scan_uniform_task<WM>(wtasks[blockIdx.x], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr);
+        scan_uniform_task<WM>(wtasks[blockIdx.x], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, lastrow);
         return;
     }
     for (int32_t t = pop_task(queue); t < nwt; t = pop_task(queue))
-        scan_uniform_task<WM>(wtasks[t], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr);
+        scan_uniform_task<WM>(wtasks[t], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, lastrow);
 }
 
 // Packed-f16 version of k_scan_uniform (same wave layout).  Valid when
@@ -302,7 +330,8 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
                                                     const ScoreTab &tab, TrajEntry *__restrict__ traj,
                                                     int32_t *__restrict__ tcount, int32_t cap,
                                                     int32_t *__restrict__ hitlist, Counters *__restrict__ ctr,
-                                                    uint32_t open_h, uint32_t ext_h, uint32_t thr_h) {
+                                                    uint32_t open_h, uint32_t ext_h, uint32_t thr_h,
+                                                    int32_t *__restrict__ lastrow) {
     const int lane = threadIdx.x;
     const PairDesc pd = pairs[task.x];
     const int8_t *__restrict__ ref = seq + pd.off1;
@@ -326,7 +355,8 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
         E[j] = zero;
     }
     const f16x2 O = as_h2(open_h * 0x10001u), X = as_h2(ext_h * 0x10001u);
-    uint32_t best = thr_h * 0x10001u;
+    const uint32_t thr2 = thr_h * 0x10001u;
+    uint32_t best = thr2;
     int32_t cntl = 0, cnth = 0;
 
     uint32_t word = L1 > 0 ? *(const uint32_t *)ref : 0u;
@@ -352,9 +382,11 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
             if (j & 1) rmax = hmax3(rmax, H[j - 1], h);
             else if (j == WM - 1) rmax = __builtin_elementwise_maximum(rmax, h);
         }
-        const uint32_t imp = pk_subsat(bits(rmax), best);
-        if (__builtin_expect(imp != 0, 0)) {
+        const uint32_t hit = pk_subsat(bits(rmax), thr2);  // halves whose row reaches min_score
+        if (__builtin_expect(hit != 0, 0)) {
+            mark_lastrow(lastrow, (uint32_t)(pd.win_base + task.y), hit, i);
             const uint32_t rb = bits(rmax);
+            const uint32_t imp = pk_subsat(rb, best);
             if (imp & 0xffffu) {
                 const uint32_t v = rb & 0xffffu;
                 int jj = 0;
@@ -398,14 +430,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const int2 *__restrict__ wtasks, int32_t nwt,
     int32_t *__restrict__ queue, Params P, ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
     int32_t cap, int32_t *__restrict__ hitlist, Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
-    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins) {
+    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins, int32_t *__restrict__ lastrow) {
     const int32_t lane_blocks = (int32_t)gridDim.x - nwins;
     if (WAVES && (int32_t)blockIdx.x >= lane_blocks) {
-        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist, ctr);
+        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist, ctr,
+                           lastrow);
         return;
     }
     scan_uniform_task_h<WM>(wtasks[blockIdx.x], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h,
-                            thr_h);
+                            thr_h, lastrow);
 }
 
 // Persistent waves: each pops tasks from the queue head in list order until
@@ -416,14 +449,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const int2 *__restrict__ wtasks, int32_t nwt,
     int32_t *__restrict__ queue, Params P, ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
     int32_t cap, int32_t *__restrict__ hitlist, Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
-    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins) {
+    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins, int32_t *__restrict__ lastrow) {
     const int32_t lane_blocks = (int32_t)gridDim.x - nwins;
     if (WAVES && (int32_t)blockIdx.x >= lane_blocks) {
-        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist, ctr);
+        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist, ctr,
+                           lastrow);
         return;
     }
     for (int32_t t = pop_task(queue); t < nwt; t = pop_task(queue))
-        scan_uniform_task_h<WM>(wtasks[t], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h, thr_h);
+        scan_uniform_task_h<WM>(wtasks[t], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h, thr_h,
+                                lastrow);
 }
 
 // One lane per ScanTask: rows [a, b) of seq1 against one window, int32.
@@ -686,7 +721,8 @@ __global__ __launch_bounds__(64) void k_resolve(const int8_t *__restrict__ seq, 
                                                 Node *__restrict__ nodes, int32_t node_cap,
                                                 const TrajEntry *__restrict__ traj, const int32_t *__restrict__ tcount,
                                                 int32_t cap, int32_t scan_cap, ScanTask *__restrict__ next,
-                                                int32_t next_cap, Counters *__restrict__ ctr) {
+                                                int32_t next_cap, Counters *__restrict__ ctr,
+                                                const int32_t *__restrict__ lastrow) {
     __shared__ uint32_t shi[8], slo[8];
     if (threadIdx.x < 8) {
         shi[threadIdx.x] = tab.hi[threadIdx.x];
@@ -714,6 +750,7 @@ __global__ __launch_bounds__(64) void k_resolve(const int8_t *__restrict__ seq, 
     const int8_t *__restrict__ s2 = seq + pd.off2;
     const int sp = win * P.step;
     const int tw = min(P.window, pd.len2 - sp);
+    const int32_t scan_end = lastrow[pd.win_base + win] + 1;  // no rescan entry at or past this row
     while (true) {
         Node nd = nodes[node];
         const int cnt = min(tcount[nd.scan], cap);
@@ -738,7 +775,22 @@ __global__ __launch_bounds__(64) void k_resolve(const int8_t *__restrict__ seq, 
         nd.s1 = e1 - q1;
         nd.s2 = sp + col - q2;
         nd.left = nd.right = -1;
-        if (nd.b - e1 - 1 >= P.min_ref) {
+        if (nd.b - e1 - 1 >= P.min_ref && e1 + 1 >= scan_end) {
+            // the right child's rows hold no cell of min_score: resolved empty, not scanned
+            const int32_t r = atomicAdd(&ctr->nodes, 1);
+            if (r < node_cap) {
+                Node rn;
+                rn.a = e1 + 1;
+                rn.b = nd.b;
+                rn.scan = -1;
+                rn.s1 = rn.e1 = rn.s2 = rn.e2 = -1;
+                rn.left = rn.right = -1;
+                nodes[r] = rn;
+                nd.right = r;
+            } else {
+                atomicOr(&ctr->overflow, 1);
+            }
+        } else if (nd.b - e1 - 1 >= P.min_ref) {
             const int32_t r = atomicAdd(&ctr->nodes, 1);
             const int32_t sc = atomicAdd(&ctr->scans, 1);
             const int32_t q = atomicAdd(&ctr->next_tasks, 1);
@@ -754,7 +806,7 @@ __global__ __launch_bounds__(64) void k_resolve(const int8_t *__restrict__ seq, 
                 st.pair = pair;
                 st.win = win;
                 st.a = e1 + 1;
-                st.b = nd.b;
+                st.b = min(nd.b, scan_end);
                 st.node = r;
                 st.scan = sc;
                 next[q] = st;
@@ -893,7 +945,7 @@ __global__ void k_gather_roots(const Node *__restrict__ nodes, const int32_t *__
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
                                int32_t nwt, int32_t persistent_waves, const Params &P, const ScoreTab &tab,
                                TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr,
-                               const int2 *wins, int32_t nwins) {
+                               const int2 *wins, int32_t nwins, int32_t *lastrow) {
     if (nwt <= 0 && nwins <= 0) return hipSuccess;
     // lane tasks first (persistent waves popping the task list from the
     // counter block's queue head, zeroed by k_seed_init, or one workgroup per
@@ -916,7 +968,7 @@ hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const 
             k = persist ? (waves ? k_scan_uniform_hp<64, true> : k_scan_uniform_hp<64, false>)
                         : (waves ? k_scan_uniform_h<64, true> : k_scan_uniform_h<64, false>);
         hipLaunchKernelGGL(k, g, b, 0, st, seq, pairs, wtasks, nwt, queue, P, th, traj, tcount, cap, hitlist, ctr, oh,
-                           xh, thr, tab, wins, nwins);
+                           xh, thr, tab, wins, nwins, lastrow);
     } else {
         decltype(&k_scan_uniform<38, false, false>) k;
         if (wm <= 38)
@@ -926,7 +978,7 @@ hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const 
             k = persist ? (waves ? k_scan_uniform<64, true, true> : k_scan_uniform<64, true, false>)
                         : (waves ? k_scan_uniform<64, false, true> : k_scan_uniform<64, false, false>);
         hipLaunchKernelGGL(k, g, b, 0, st, seq, pairs, wtasks, nwt, queue, P, tab, traj, tcount, cap, hitlist, ctr,
-                           wins, nwins);
+                           wins, nwins, lastrow);
     }
     return hipGetLastError();
 }
@@ -968,17 +1020,17 @@ hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairD
                           const int32_t *hitlist, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
                           int32_t blocks_max, Node *nodes, int32_t node_cap, const TrajEntry *traj,
                           const int32_t *tcount, int32_t cap, int32_t scan_cap, ScanTask *next, int32_t next_cap,
-                          Counters *ctr) {
+                          Counters *ctr, const int32_t *lastrow) {
     if (n_max <= 0) return hipSuccess;
     int32_t blocks = (n_max + 63) / 64;
     if (blocks_max > 0) blocks = std::min(blocks, blocks_max);
     dim3 g(blocks), b(64);
     if (wm <= 38)
         hipLaunchKernelGGL(k_resolve<38>, g, b, 0, st, seq, pairs, pair_of_window, P, tab, level0, hitlist, tasks, n_dev,
-                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr);
+                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr, lastrow);
     else
         hipLaunchKernelGGL(k_resolve<64>, g, b, 0, st, seq, pairs, pair_of_window, P, tab, level0, hitlist, tasks, n_dev,
-                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr);
+                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr, lastrow);
     return hipGetLastError();
 }
 
