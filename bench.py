@@ -844,7 +844,8 @@ def main():
     peak = dev["cu"] * 4 * 16 * 2 * dev["clock_mhz"] * 1e6 / 1e12  # packed 16-bit VALU ops (2 per lane-instruction)
     kernels = {}
     for k in ("seed_init", "scan_uniform", "seed_levels", "level_begin", "order_tasks", "scan_general", "resolve",
-              "gather", "xdrop_pass1", "xdrop_pass1_wave", "xdrop_pass2", "xdrop_pass2_wave"):
+              "gather", "xdrop_pass1", "xdrop_pass1_41", "xdrop_pass1_wave", "xdrop_pass2", "xdrop_pass2_41",
+              "xdrop_pass2_wave"):
         kms, kn = ctx.kernel_time(k)
         if kn:
             kernels[k] = {"ms_per_step": round(kms / args.steps, 3), "launches_per_step": kn / args.steps}

@@ -10,7 +10,9 @@ it is a choice whose best value depends on the batch:
   * wave_tasks / wave_rows: how many of the longest later-level rescans run
     one per wave (64 lanes on one task) instead of one per lane;
   * graph_levels: how many recursion levels are replayed as one captured HIP
-    graph without host reads between them.
+    graph without host reads between them;
+  * xdrop_lane_rows: how many rows an x-drop pass may run on the
+    one-seed-per-lane kernels before its seed moves to one wave.
 
 Results never depend on these settings, only the time does.  `tune` measures
 the whole findSimilarFragmentsForPairedSequence step on the staged batch,
@@ -31,9 +33,10 @@ import numpy as np
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "seed_tuning.json")
 
 DEFAULT = {"level0_mode": 1, "level0_waves": 8, "wave_tasks": 1024, "wave_rows": 4096, "graph_levels": 4,
-           "pool_min": 0}
+           "pool_min": 0, "xdrop_lane_rows": 512}
 SPACE = [("level0_mode", [1, 2]), ("level0_waves", [4, 8]), ("wave_tasks", [256, 1024, 2048]),
-         ("wave_rows", [2048, 4096, 8192]), ("graph_levels", [-1, 4, 8])]
+         ("wave_rows", [2048, 4096, 8192]), ("graph_levels", [-1, 4, 8]),
+         ("xdrop_lane_rows", [256, 512, 1024, -1])]
 
 
 def shape_key(len1, len2):
@@ -94,7 +97,7 @@ def load_table(path=TABLE):
 def lookup(len1, len2, path=TABLE):
     """The table's setting for this batch shape, or None."""
     e = load_table(path)["entries"].get(shape_key(len1, len2))
-    return {k: e["tuning"][k] for k in DEFAULT} if e else None
+    return {k: e["tuning"].get(k, DEFAULT[k]) for k in DEFAULT} if e else None
 
 
 def main():

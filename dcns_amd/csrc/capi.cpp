@@ -156,6 +156,7 @@ int dcns_set_tuning(dcns_ctx *c, const dcns_tuning *in) {
         if (t.level0_waves < 0 || t.level0_waves > 32) throw Error(DCNS_ERR_ARG, "level0_waves must be in [0, 32]");
         if (t.wave_tasks < 0 || t.wave_rows < 0 || t.pool_min < 0) throw Error(DCNS_ERR_ARG, "negative tuning value");
         if (t.graph_levels < -1 || t.graph_levels > 64) throw Error(DCNS_ERR_ARG, "graph_levels must be in [-1, 64]");
+        if (t.xdrop_lane_rows < -1) throw Error(DCNS_ERR_ARG, "xdrop_lane_rows must be >= -1");
         c->tune = t;
     });
 }

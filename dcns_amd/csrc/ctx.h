@@ -145,7 +145,7 @@ struct dcns_ctx {
     dcns::DBuf xw_scr, xw_off, xw_wmax;                  // x-drop tasks with bands past the wave kernel's
     dcns::DBuf order_hist, task_order;                   // scan tasks ordered by row count
     dcns::DBuf xorder;                                   // x-drop pass-2 order
-    dcns::DBuf xlist;                                    // x-drop tasks handed from the lane to the wave kernel
+    dcns::DBuf xlist, xlist2;                            // x-drop tasks handed to the next wider kernel
     dcns::DBuf cig_scan, cig_packed;                     // cigar packing: counts / offsets / scan storage, runs
     dcns::DBuf rnd_thr, rnd_len1, rnd_len2, rnd_seed, rnd_out, rnd_order;  // random fragment pairs generated on the device
     dcns::DBuf mk_seq, mk_off, mk_keys, mk_desc, mk_gbytes, mk_ghash, mk_gtab;  // maskGenome: genome, k-mer set

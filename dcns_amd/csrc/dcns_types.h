@@ -84,8 +84,8 @@ struct XOut {
     int32_t rrows;                // rows evaluated by the final reverse pass
     int32_t iii, jjj;             // traceback leftovers
     int32_t ncig;                 // cigar length
-    int32_t status;               // 0 done, 1 needs a wider band kernel
-    int32_t pad;
+    int32_t status;               // 0 done, 1 forward / 2 reverse pass pending for a wider band kernel
+    int32_t lane_stop;            // a lane kernel stopped the pending pass at its row budget: the wave kernel takes it
     int64_t cig_off;              // offset into the cigar arena
     int64_t cells;                // DP cells evaluated (all passes)
 };

@@ -64,22 +64,30 @@ hipError_t launch_gather_roots(hipStream_t st, const Node *nodes, const int32_t 
 
 // xdrop.hip
 // Bands of up to kXdropLaneBand cells run one task per lane, up to
+// kXdropLane2Band one task per lane of the wider lane kernel, up to
 // kXdropWaveBand one task per wave, wider ones with the band in scratch.
 constexpr int kXdropLaneBand = 21;
+constexpr int kXdropLane2Band = 41;
 constexpr int kXdropWaveBand = 192;
+// Rows a lane kernel's pass may evaluate before its task moves to the wave
+// kernel (dcns_tuning.xdrop_lane_rows overrides it).
+constexpr int kXdropLaneRows = 512;
 // Pass 1: forward score-only extension + reverse score-only extension, one
-// lane per task; tasks whose band outgrows the lane's are counted in
+// lane per task of in_list[0, n) (all tasks when null) at bands of up to
+// `band` cells (kXdropLaneBand or kXdropLane2Band); tasks whose band
+// outgrows it, or whose pass runs past max_rows rows, are counted in
 // *pending and listed in list[0, *pending).
-hipError_t launch_xdrop_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
-                              int32_t n, const Params &P, XOut *out, int32_t *pending, int32_t *list);
+hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                              const int32_t *in_list, int32_t n, const Params &P, XOut *out, int32_t *pending,
+                              int32_t *list, int32_t max_rows);
 // Pass 1 one wave per task for list[0, n) (every task when list is null);
 // tasks too wide for the wave are counted in *pending.
 hipError_t launch_xdrop_wave_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
                                    const int32_t *list, int32_t n, const Params &P, XOut *out, int32_t *pending);
 // Pass 2: reverse extension with traceback into a band-compressed store, then
-// the cigar; the lane kernel takes order[0, n)'s tasks of the lane band, the
+// the cigar; the lane kernel of `band` cells takes order[0, n)'s tasks, the
 // wave kernel one task of the wave band per order entry.
-hipError_t launch_xdrop_pass2(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
                               const int32_t *order, int32_t n, const Params &P, XOut *out, const int64_t *tb_off,
                               uint8_t *tb, uint32_t *cigar);
 hipError_t launch_xdrop_wave_pass2(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,

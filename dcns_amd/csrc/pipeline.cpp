@@ -752,27 +752,37 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
         hip_check(hipMemcpyAsync(&pending, pending_dev, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream), "pending");
         c->sync();
     };
-    // pass 1: one lane per task while the band fits the lane kernel; the
-    // tasks that outgrow it (listed in xlist) and, when the initial band is
-    // already wider, every task go on one wave each
-    const bool lane_first = 2 * P.w + 1 <= kXdropLaneBand;
-    int32_t nwave = n;
-    if (lane_first) {
-        c->xlist.ensure((size_t)n * sizeof(int32_t));
+    // pass 1: one lane per task while the band fits the 21-cell lane
+    // kernel, then one lane per task at 41 cells for the tasks that outgrew
+    // it, then one wave per task for those that outgrew that; each kernel
+    // lists its tasks that need a wider one (xlist, then xlist2).  A kernel
+    // narrower than the initial band is skipped.
+    int32_t nnext = n;
+    const int32_t *in_list = nullptr;
+    const int32_t B0 = 2 * P.w + 1;
+    // rows a lane kernel's pass may run before the wave kernel takes the task
+    const int32_t lane_rows = c->tune.xdrop_lane_rows > 0 ? c->tune.xdrop_lane_rows
+                              : c->tune.xdrop_lane_rows < 0 ? INT32_MAX : kXdropLaneRows;
+    c->xlist.ensure((size_t)n * sizeof(int32_t));
+    c->xlist2.ensure((size_t)n * sizeof(int32_t));
+    for (const int band : {kXdropLaneBand, kXdropLane2Band}) {
+        if (B0 > band || nnext == 0) continue;
+        int32_t *list = band == kXdropLaneBand ? c->xlist.as<int32_t>() : c->xlist2.as<int32_t>();
         hip_check(hipMemsetAsync(pending_dev, 0, sizeof(int32_t), c->stream), "hipMemsetAsync");
-        c->launch("xdrop_pass1", [&] {
-            return launch_xdrop_pass1(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->xtasks.as<XTask>(),
-                                      n, P, c->xout.as<XOut>(), pending_dev, c->xlist.as<int32_t>());
+        c->launch(band == kXdropLaneBand ? "xdrop_pass1" : "xdrop_pass1_41", [&] {
+            return launch_xdrop_pass1(c->stream, band, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                      c->xtasks.as<XTask>(), in_list, nnext, P, c->xout.as<XOut>(), pending_dev, list,
+                                      lane_rows);
         });
         read_pending();
-        nwave = pending;
+        nnext = pending;
+        in_list = list;
     }
-    if (nwave > 0) {
+    if (nnext > 0) {
         hip_check(hipMemsetAsync(pending_dev, 0, sizeof(int32_t), c->stream), "hipMemsetAsync");
         c->launch("xdrop_pass1_wave", [&] {
             return launch_xdrop_wave_pass1(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
-                                           c->xtasks.as<XTask>(), lane_first ? c->xlist.as<int32_t>() : nullptr,
-                                           nwave, P, c->xout.as<XOut>(), pending_dev);
+                                           c->xtasks.as<XTask>(), in_list, nnext, P, c->xout.as<XOut>(), pending_dev);
         });
         read_pending();
     }
@@ -833,11 +843,18 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
         });
         return;
     }
-    // the band class of a task's pass 2: 0 lane kernel, 1 wave kernel, 2 scratch-band kernel
-    auto band_class = [](int32_t B) { return B <= kXdropLaneBand ? 0 : (B <= kXdropWaveBand ? 1 : 2); };
+    // the band class of a task's pass 2: 0 and 1 the lane kernels, 2 the wave
+    // kernel (also the lane bands' tasks of more than lane_rows rows), 3 the
+    // scratch-band kernel
+    auto band_class = [&](int32_t B, int32_t rows) {
+        if (B > kXdropWaveBand) return 3;
+        if (B > kXdropLane2Band || rows > lane_rows) return 2;
+        return B <= kXdropLaneBand ? 0 : 1;
+    };
+    constexpr int NCLS = 4;
     std::vector<int64_t> tboff((size_t)n, 0);
     int64_t tbtotal = 0, cigtotal = 0;
-    int32_t in_class[3] = {0, 0, 0};
+    int32_t in_class[NCLS] = {0, 0, 0, 0};
     for (int32_t i = 0; i < n; ++i) {
         XOut &o = xo[i];
         cells += o.cells;
@@ -847,24 +864,24 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
         tboff[(size_t)i] = tbtotal;
         tbtotal += (int64_t)(o.rrows + 1) * BP;
         cigtotal += std::max(0, o.re1) + std::max(0, o.re2) + 2;
-        ++in_class[band_class(B)];
+        ++in_class[band_class(B, o.rrows)];
     }
     hip_check(hipMemcpyAsync(c->xout.p, xo, (size_t)n * sizeof(XOut), hipMemcpyHostToDevice, c->stream), "xout up");
     upload(c, c->tb_off, tboff);
     // pass 2 order: by band class (each kernel's tasks contiguous, the lane
-    // class first, then the wave class), then by traced cells, largest
+    // classes first, then the wave class), then by traced cells, largest
     // first, on a quarter-octave scale (a counting sort; the order only
     // balances the launches)
     {
         constexpr int NB = 4 * 48;
         std::vector<int32_t> bucket((size_t)n);
-        std::vector<int64_t> start(3 * NB + 2, 0);
+        std::vector<int64_t> start(NCLS * NB + 2, 0);
         for (int32_t i = 0; i < n; ++i) {
             const XOut &o = xo[i];
-            int32_t b = 3 * NB;  // no traceback: last
+            int32_t b = NCLS * NB;  // no traceback: last
             if (o.fe1 >= 0 && o.fe2 >= 0) {
                 const int32_t B = 2 * o.rw + 1;
-                const int cls = band_class(B);
+                const int cls = band_class(B, o.rrows);
                 const uint64_t cl = (uint64_t)(o.rrows + 1) * (uint64_t)B;
                 const int lg = 63 - __builtin_clzll(cl | 1);
                 const int q = lg >= 2 ? (int)((cl >> (lg - 2)) & 3) : 0;
@@ -890,16 +907,24 @@ void xdrop_extend(dcns_ctx *c, const Staged &S, const Params &P, const std::vect
     }
     if (in_class[0] > 0)
         c->launch("xdrop_pass2", [&] {
-            return launch_xdrop_pass2(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->xtasks.as<XTask>(),
-                                      c->xorder.as<int32_t>(), in_class[0], P, c->xout.as<XOut>(),
-                                      c->tb_off.as<int64_t>(), c->tb.as<uint8_t>(), c->cigar.as<uint32_t>());
+            return launch_xdrop_pass2(c->stream, kXdropLaneBand, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                      c->xtasks.as<XTask>(), c->xorder.as<int32_t>(), in_class[0], P,
+                                      c->xout.as<XOut>(), c->tb_off.as<int64_t>(), c->tb.as<uint8_t>(),
+                                      c->cigar.as<uint32_t>());
         });
     if (in_class[1] > 0)
+        c->launch("xdrop_pass2_41", [&] {
+            return launch_xdrop_pass2(c->stream, kXdropLane2Band, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                      c->xtasks.as<XTask>(), c->xorder.as<int32_t>() + in_class[0], in_class[1], P,
+                                      c->xout.as<XOut>(), c->tb_off.as<int64_t>(), c->tb.as<uint8_t>(),
+                                      c->cigar.as<uint32_t>());
+        });
+    if (in_class[2] > 0)
         c->launch("xdrop_pass2_wave", [&] {
             return launch_xdrop_wave_pass2(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
-                                           c->xtasks.as<XTask>(), c->xorder.as<int32_t>() + in_class[0], in_class[1],
-                                           P, c->xout.as<XOut>(), c->tb_off.as<int64_t>(), c->tb.as<uint8_t>(),
-                                           c->cigar.as<uint32_t>());
+                                           c->xtasks.as<XTask>(), c->xorder.as<int32_t>() + in_class[0] + in_class[1],
+                                           in_class[2], P, c->xout.as<XOut>(), c->tb_off.as<int64_t>(),
+                                           c->tb.as<uint8_t>(), c->cigar.as<uint32_t>());
         });
     // the arena holds re1 + re2 + 2 entries per task, the walk writes a few
     // runs: pack them before the download

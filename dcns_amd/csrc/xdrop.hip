@@ -6,7 +6,8 @@
 // traceback (:1318-1451).  A row's band is j in [max(1,i-w), min(i+w,L2)];
 // a pass restarts with 2w when the row maximum sits on a band edge.
 //
-// One lane per seed while the band is at most 21 cells.  The band is held
+// One lane per seed while the band is at most 21 cells, then (the tasks
+// whose band doubled) one lane per seed at 41 cells.  The band is held
 // band-relative in VGPRs: in band coordinates k = j - (i - w) the diagonal
 // predecessor of (i, k) is (i-1, k) and the vertical one is (i-1, k+1), so
 // rows advance without moving data.  Query bases ride in a byte window packed
@@ -16,8 +17,9 @@
 //   pass 2: the final reverse pass again, storing one traceback byte per
 //           band cell (rows x band, default 255 like Matrix::reset), then the
 //           traceback state machine writes the cigar.
-// A task whose band outgrows 21 cells (a row maximum on the band edge, so
-// the pass restarts at twice the width) is listed for the wave kernels: one
+// A task whose band outgrows the lane kernel's (a row maximum on the band
+// edge, so the pass restarts at twice the width) is listed for the next
+// kernel: the 41-cell lane kernel, then the wave kernels: one
 // wave per task, the band's cells across the lanes, 192 cells at most, which
 // keeps a long extension at a wide band from holding a launch for the
 // length of one lane's walk across every cell.  Past 192 cells a kernel keeps
@@ -50,12 +52,15 @@ struct PassOut {
     int32_t rows;         // rows evaluated
     int64_t cells;
     bool restart;
+    bool stopped;         // the row budget ran out before the pass ended
 };
 
 // One pass at half-width w (2w+1 <= BMAX).  With TB, row i's traceback bytes
 // are written to tb[i * BP + k] (k in band coordinates, 255 outside the band).
+// A pass that would evaluate a row past max_rows stops there (stopped).
 template <int BMAX, bool TB>
-__device__ PassOut xpass(const Dir &D, int32_t w, const Params &P, uint8_t *__restrict__ tb, int32_t BP) {
+__device__ PassOut xpass(const Dir &D, int32_t w, const Params &P, uint8_t *__restrict__ tb, int32_t BP,
+                         int32_t max_rows = INT32_MAX) {
     constexpr int NQ = BMAX / 4 + 2;
     int32_t Mp[BMAX], Fp[BMAX];
     uint32_t qw[NQ];
@@ -76,9 +81,13 @@ __device__ PassOut xpass(const Dir &D, int32_t w, const Params &P, uint8_t *__re
         }
         qw[q] = v;
     }
-    PassOut r{0, 0, 0, 0, 0, false};
+    PassOut r{0, 0, 0, 0, 0, false, false};
     const int32_t O = P.open, X = P.ext;
     for (int32_t i = 1; i <= D.L1; ++i) {
+        if (i > max_rows) {
+            r.stopped = true;
+            return r;
+        }
         const int32_t st = max(i - w, 1);
         const int32_t en = min(i + w, D.L2);
         const int32_t lo = i - w;  // column of band slot 0
@@ -150,7 +159,7 @@ __device__ PassOut xpass_mem(const Dir &D, int32_t w, const Params &P, uint8_t *
         Mp[k] = kOutOfBand;
         Fp[k] = kOutOfBand;
     }
-    PassOut r{0, 0, 0, 0, 0, false};
+    PassOut r{0, 0, 0, 0, 0, false, false};
     const int32_t O = P.open, X = P.ext;
     for (int32_t i = 1; i <= D.L1; ++i) {
         const int32_t st = max(i - w, 1);
@@ -199,16 +208,25 @@ __device__ PassOut xpass_mem(const Dir &D, int32_t w, const Params &P, uint8_t *
     return r;
 }
 
+// Passes at w, 2w, ... until one ends without a restart.  too_wide: the next
+// band is wider than BMAX, or (stopped) a pass ran out of its max_rows
+// budget; the pending pass is then at band w, and a stopped pass's cells are
+// not counted (the kernel that takes the task over runs it again).
 template <int BMAX, bool TB>
 __device__ PassOut xrun(const Dir &D, int32_t &w, const Params &P, uint8_t *tb, int32_t BP, int64_t &cells,
-                        bool &too_wide) {
+                        bool &too_wide, bool &stopped, int32_t max_rows) {
     too_wide = false;
+    stopped = false;
     while (true) {
         if (2 * w + 1 > BMAX) {
             too_wide = true;
-            return PassOut{0, 0, 0, 0, 0, false};
+            return PassOut{0, 0, 0, 0, 0, false, false};
         }
-        PassOut r = xpass<BMAX, TB>(D, w, P, tb, BP);
+        PassOut r = xpass<BMAX, TB>(D, w, P, tb, BP, max_rows);
+        if (r.stopped) {
+            too_wide = stopped = true;
+            return r;
+        }
         cells += r.cells;
         if (!r.restart) return r;
         w *= 2;
@@ -287,7 +305,7 @@ __device__ PassOut wave_xpass(const Dir &D, int32_t w, const Params &P, uint8_t 
     int32_t ccur, qcur, cnext, qnext;
     chunk(1, ccur, qcur);
     chunk(65, cnext, qnext);
-    PassOut r{0, 0, 0, 0, 0, false};
+    PassOut r{0, 0, 0, 0, 0, false, false};
     for (int32_t i = 1; i <= D.L1; ++i) {
         const int t = (i - 1) & 63;
         if (t == 0 && i > 1) {
@@ -390,7 +408,7 @@ __device__ PassOut wave_xrun(const Dir &D, int32_t &w, const Params &P, int64_t 
     while (true) {
         if (2 * w + 1 > kXdropWaveBand) {
             too_wide = true;
-            return PassOut{0, 0, 0, 0, 0, false};
+            return PassOut{0, 0, 0, 0, 0, false, false};
         }
         PassOut r = wave_xpass_fit<false>(D, w, P, nullptr, 0);
         cells += r.cells;
@@ -421,25 +439,37 @@ __device__ Dir reverse_dir(const int8_t *seq, const PairDesc &pd, const XTask &t
     return D;
 }
 
+// Pass 1, one lane per task, for the tasks in_list[0, n) (all tasks when
+// in_list is null); tasks whose band outgrows BMAX, or whose pass runs past
+// max_rows rows (a lane walks a row per step, so one long extension would
+// hold the whole wave), are listed in list for the next kernel.  A task a
+// narrower lane kernel stopped for its length is listed on at once.
 template <int BMAX>
 __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
-                                                    const XTask *__restrict__ tasks, int32_t n, Params P,
+                                                    const XTask *__restrict__ tasks,
+                                                    const int32_t *__restrict__ in_list, int32_t n, Params P,
                                                     XOut *__restrict__ out, int32_t *__restrict__ pending,
-                                                    int32_t *__restrict__ list) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n) return;
+                                                    int32_t *__restrict__ list, int32_t max_rows) {
+    const int lane_gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane_gid >= n) return;
+    const int gid = in_list ? in_list[lane_gid] : lane_gid;
     const XTask t = tasks[gid];
     XOut o = out[gid];
     if (o.status == 0 && o.fw != 0) return;  // already finished by a narrower kernel
+    if (o.lane_stop) {
+        list[atomicAdd(pending, 1)] = gid;
+        return;
+    }
     const PairDesc pd = pairs[t.pair];
-    bool wide;
+    bool wide, stopped;
     if (o.status != 2) {  // forward pass still to do
         int32_t w = o.fw > 0 ? o.fw : P.w;
         const Dir F = forward_dir(seq, pd, t);
-        PassOut f = xrun<BMAX, false>(F, w, P, nullptr, 0, o.cells, wide);
+        PassOut f = xrun<BMAX, false>(F, w, P, nullptr, 0, o.cells, wide, stopped, max_rows);
         if (wide) {
             o.status = 1;
             o.fw = w;
+            o.lane_stop = stopped;
             out[gid] = o;
             list[atomicAdd(pending, 1)] = gid;
             return;
@@ -460,10 +490,11 @@ __global__ __launch_bounds__(64) void k_xdrop_pass1(const int8_t *__restrict__ s
     }
     int32_t w = o.rw > 0 ? o.rw : P.w;
     const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
-    PassOut r = xrun<BMAX, false>(R, w, P, nullptr, 0, o.cells, wide);
+    PassOut r = xrun<BMAX, false>(R, w, P, nullptr, 0, o.cells, wide, stopped, max_rows);
     if (wide) {
         o.status = 2;
         o.rw = w;
+        o.lane_stop = stopped;
         out[gid] = o;
         list[atomicAdd(pending, 1)] = gid;
         return;
@@ -644,7 +675,7 @@ __global__ __launch_bounds__(64) void k_xdrop_pass2(const int8_t *__restrict__ s
     XOut o = out[gid];
     if (o.fe1 < 0 || o.fe2 < 0) return;
     const int32_t w = o.rw;
-    if (2 * w + 1 > BMAX) return;  // a wider kernel's task
+    if (2 * w + 1 > BMAX) return;  // a wider kernel's task (the order keeps the narrower ones out)
     const PairDesc pd = pairs[t.pair];
     const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
     const int32_t B = 2 * w + 1;
@@ -656,9 +687,10 @@ __global__ __launch_bounds__(64) void k_xdrop_pass2(const int8_t *__restrict__ s
     out[gid] = o;
 }
 
-// Pass 2, one wave per task, for the tasks order[0, n) whose final reverse
-// band is wider than the lane kernel's and fits the wave; the first lane
-// walks the traceback once the wave's bytes are stored.
+// Pass 2, one wave per task, for the tasks order[0, n): those whose final
+// reverse band is wider than the lane kernels' and fits the wave, and the
+// long ones of the lane bands; the first lane walks the traceback once the
+// wave's bytes are stored.
 __global__ __launch_bounds__(64) void k_xdrop_wave_pass2(const int8_t *__restrict__ seq,
                                                          const PairDesc *__restrict__ pairs,
                                                          const XTask *__restrict__ tasks,
@@ -670,7 +702,7 @@ __global__ __launch_bounds__(64) void k_xdrop_wave_pass2(const int8_t *__restric
     XOut o = out[gid];
     if (o.fe1 < 0 || o.fe2 < 0) return;
     const int32_t w = o.rw;
-    if (2 * w + 1 <= kXdropLaneBand || 2 * w + 1 > kXdropWaveBand) return;  // another kernel's class
+    if (2 * w + 1 > kXdropWaveBand) return;  // the scratch-band kernel's
     const PairDesc pd = pairs[t.pair];
     const Dir R = reverse_dir(seq, pd, t, o.fe1, o.fe2);
     const int32_t B = 2 * w + 1;
@@ -755,11 +787,13 @@ hipError_t launch_pack_cigars(hipStream_t st, XOut *out, int32_t n, void *scratc
     return hipGetLastError();
 }
 
-hipError_t launch_xdrop_pass1(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
-                              int32_t n, const Params &P, XOut *out, int32_t *pending, int32_t *list) {
+hipError_t launch_xdrop_pass1(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+                              const int32_t *in_list, int32_t n, const Params &P, XOut *out, int32_t *pending,
+                              int32_t *list, int32_t max_rows) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_xdrop_pass1<kXdropLaneBand>, dim3((n + 63) / 64), dim3(64), 0, st, seq, pairs, tasks, n, P,
-                       out, pending, list);
+    hipLaunchKernelGGL(band <= kXdropLaneBand ? k_xdrop_pass1<kXdropLaneBand> : k_xdrop_pass1<kXdropLane2Band>,
+                       dim3((n + 63) / 64), dim3(64), 0, st, seq, pairs, tasks, in_list, n, P, out, pending, list,
+                       max_rows);
     return hipGetLastError();
 }
 
@@ -788,12 +822,12 @@ hipError_t launch_xdrop_wide_pass2(hipStream_t st, const int8_t *seq, const Pair
     return hipGetLastError();
 }
 
-hipError_t launch_xdrop_pass2(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
+hipError_t launch_xdrop_pass2(hipStream_t st, int band, const int8_t *seq, const PairDesc *pairs, const XTask *tasks,
                               const int32_t *order, int32_t n, const Params &P, XOut *out, const int64_t *tb_off,
                               uint8_t *tb, uint32_t *cigar) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_xdrop_pass2<kXdropLaneBand>, dim3((n + 63) / 64), dim3(64), 0, st, seq, pairs, tasks, order,
-                       n, P, out, tb_off, tb, cigar);
+    hipLaunchKernelGGL(band <= kXdropLaneBand ? k_xdrop_pass2<kXdropLaneBand> : k_xdrop_pass2<kXdropLane2Band>,
+                       dim3((n + 63) / 64), dim3(64), 0, st, seq, pairs, tasks, order, n, P, out, tb_off, tb, cigar);
     return hipGetLastError();
 }
 

@@ -117,21 +117,27 @@ int dcns_last_stats(const dcns_ctx *ctx, dcns_stats *out);
  *                 pools (kept for later calls).
  *   pool_min      > 0: the recursion's node, trajectory and task pools start
  *                 this many entries above the window count instead of a
- *                 quarter of it (makes the pools grow; for tests). */
+ *                 quarter of it (makes the pools grow; for tests).
+ *   xdrop_lane_rows  x-drop extension: a pass on the one-task-per-lane
+ *                 kernels that runs past this many rows moves its task to
+ *                 the one-task-per-wave kernel, and so does a traceback pass
+ *                 of more rows (default 512; -1: no limit). */
 typedef struct dcns_tuning {
     int32_t level0_mode, level0_waves;
     int32_t wave_tasks, wave_rows;
     int32_t graph_levels;
     int32_t pool_min;
-    int32_t pad[2];
+    int32_t xdrop_lane_rows;
+    int32_t pad;
 } dcns_tuning;
 int dcns_get_tuning(const dcns_ctx *ctx, dcns_tuning *out);
 int dcns_set_tuning(dcns_ctx *ctx, const dcns_tuning *in);
 
 /* Per-kernel timing (flags bit 0): total milliseconds and launch count of the
- * kernel family `name` ("scan_uniform", "scan_general", "resolve",
- * "xdrop_pass1", "xdrop_pass1_wave", "xdrop_pass2", "xdrop_pass2_wave",
- * "sw_batch") since the last reset. */
+ * kernel family `name` ("scan_uniform", "seed_levels", "scan_general",
+ * "resolve", "xdrop_pass1", "xdrop_pass1_41", "xdrop_pass1_wave",
+ * "xdrop_pass2", "xdrop_pass2_41", "xdrop_pass2_wave", "sw_batch") since the
+ * last reset. */
 int dcns_kernel_time(const dcns_ctx *ctx, const char *name, double *ms, int64_t *launches);
 int dcns_kernel_time_reset(dcns_ctx *ctx);
 

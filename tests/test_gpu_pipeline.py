@@ -118,6 +118,43 @@ def test_xdrop_extend_wave_bands_vs_oracle(gpu, oracle_lib, sc, w):
     assert traced > 20
 
 
+@pytest.mark.parametrize("w", [10, 15])
+def test_xdrop_lane_row_budget_vs_oracle(gpu, oracle_lib, w):
+    """The lane kernels' row budget (a pass past it moves to the wave kernel,
+    a traceback pass of more rows runs there): budgets of 1, 5 and 64 rows
+    and none give the oracle's results and the same reference cell count.
+    w = 15 starts at a 31-cell band, on the 41-cell lane kernel."""
+    pairs = _small_pairs(123, 4, 600, 2500, (6, 14))
+    rng = np.random.default_rng(w + 1)
+    pos, starts = [], []
+    for i, (a, b) in enumerate(pairs):
+        for s in oracle_lib.seed_set(a, b, linked=True)[:30]:
+            pos.append(i)
+            starts.append((s[0] + 1, s[2] + 1))
+        for _ in range(10):
+            pos.append(i)
+            starts.append((int(rng.integers(0, len(a) + 1)), int(rng.integers(0, len(b) + 1))))
+    p = dcns_amd.default_params(w=w)
+    want = [_x_extend_oracle(oracle_lib, pairs[i][0], pairs[i][1], s1, s2, w=w) for i, (s1, s2) in zip(pos, starts)]
+    old = gpu.get_tuning()
+    cells = set()
+    try:
+        for rows in (1, 5, 64, -1):
+            gpu.set_tuning(xdrop_lane_rows=rows)
+            print("xdrop lane row budget", w, rows, len(pos), flush=True)
+            got = gpu.xdrop_extend(pairs, pos, starts, params=p)
+            cells.add(gpu.stats()["xdrop_cells"])
+            for k, (g, wt) in enumerate(zip(got, want)):
+                if wt[5] == 0:
+                    assert g[5] == 0, (rows, k, g[:6], wt[:6])
+                    continue
+                assert g[:6] == wt[:6], (rows, k, g[:6], wt[:6])
+                assert g[6] == wt[6], (rows, k)
+    finally:
+        gpu.set_tuning(**old)
+    assert len(cells) == 1, cells
+
+
 def _cmp_fragments(got, want, where):
     assert len(got) == len(want), (where, len(got), len(want))
     for g, w in zip(got, want):
