@@ -361,6 +361,9 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
 
     uint32_t word = L1 > 0 ? *(const uint32_t *)ref : 0u;
     uint32_t next = 0;
+    // two rows per iteration: the register allocator then keeps H, E in
+    // place across rows instead of rotating them with moves at the loop end
+#pragma unroll 2
     for (int i = 0; i < L1; ++i) {
         const int sub = i & 3;
         if (sub == 0 && i + 4 < L1) next = *(const uint32_t *)(ref + i + 4);
@@ -368,13 +371,14 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
         if (sub == 3) word = next;
         uint32_t thi, tlo;
         uniform_table(tab, c, thi, tlo);
-        f16x2 diag = zero, fc = zero, rmax = zero;
+        // t: the diagonal term of column j, the previous row's H[j - 1] plus
+        // the score (0 before the first column), formed before H[j - 1] is
+        // overwritten
+        f16x2 t = as_h2(perm(thi, tlo, sel[0])), fc = zero, rmax = zero;
 #pragma unroll
         for (int j = 0; j < WM; ++j) {
-            const f16x2 s = as_h2(perm(thi, tlo, sel[j]));
-            const f16x2 t = diag + s;
-            diag = H[j];
             const f16x2 h = hmax3(t, E[j], fc);  // E, fc >= 0: includes the 0 floor
+            if (j + 1 < WM) t = H[j] + as_h2(perm(thi, tlo, sel[j + 1]));
             const f16x2 ho = h + O;              // both gap kinds open from the final H
             E[j] = hmax3(E[j] + X, ho, zero);
             fc = hmax3(ho, fc + X, zero);
