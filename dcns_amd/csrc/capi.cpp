@@ -157,6 +157,7 @@ int dcns_set_tuning(dcns_ctx *c, const dcns_tuning *in) {
         if (t.wave_tasks < 0 || t.wave_rows < 0 || t.pool_min < 0) throw Error(DCNS_ERR_ARG, "negative tuning value");
         if (t.graph_levels < -1 || t.graph_levels > 64) throw Error(DCNS_ERR_ARG, "graph_levels must be in [-1, 64]");
         if (t.xdrop_lane_rows < -1) throw Error(DCNS_ERR_ARG, "xdrop_lane_rows must be >= -1");
+        if (t.k1_lanes < 0 || t.k1_lanes > 3) throw Error(DCNS_ERR_ARG, "k1_lanes must be in [0, 3]");
         c->tune = t;
     });
 }
@@ -400,6 +401,19 @@ int dcns_sw_batch_noalloc(dcns_ctx *c, const int8_t *d_seqs, const int64_t *d_of
     });
 }
 
+namespace dcns {
+// The launch plan of dcns_sw_random_max for one set of point shapes: the
+// routing to the wave kernel, the lane plan (uploaded to rnd_order) and the
+// wave pairs (uploaded to rnd_wlist).  A call with the same lengths, pairs
+// per point and launch shape (the permutation method's settings in turn)
+// reuses it.
+struct RandPlan {
+    uint64_t key = 0;
+    SwPlan plan;
+    std::vector<int32_t> wids;
+};
+}  // namespace dcns
+
 int dcns_sw_random_max(dcns_ctx *c, int32_t match, int32_t mismatch, int32_t open, int32_t ext, const double *gc,
                        const int32_t *len1, const int32_t *len2, const uint64_t *seed, int32_t npoints,
                        int32_t pairs_per_point, int32_t *out) {
@@ -442,27 +456,125 @@ int dcns_sw_random_max(dcns_ctx *c, int32_t match, int32_t mismatch, int32_t ope
         up(c->rnd_len2, len2, (size_t)npoints * 4);
         up(c->rnd_seed, seed, (size_t)npoints * 8);
         // lane plan: waves of one loop shape, longest first; boundary rows
-        // in launches of whole waves within 4 GiB
-        SwPlan plan;
-        sw_plan(len1, len2, npoints, pairs_per_point, plan);
-        up(c->rnd_order, plan.order.data(), plan.order.size() * 4);
+        // in launches of whole waves within 4 GiB.  Two pairs per lane in
+        // packed f16 where the scoring allows (a pair whose maximum reaches
+        // 2048 runs again on the int32 kernel), else one pair per lane.
+        // The longest points' pairs go one per wave (stripes across the
+        // lanes) when that shortens the launch.
+        uint32_t mis4 = 0, diff = 0;
+        const int32_t mode = c->tune.k1_lanes;
+        const bool packed = mode != 1 && sw_random_packed(P, mis4, diff);
+        const int32_t WS = packed ? 128 : 64;
+        uint64_t key = 0xcbf29ce484222325ull;  // FNV-1a over the shapes and the launch shape
+        auto mixin = [&](uint64_t v) {
+            key ^= v;
+            key *= 0x100000001b3ull;
+        };
+        mixin((uint64_t)npoints);
+        mixin((uint64_t)pairs_per_point);
+        mixin((uint64_t)(mode * 2 + (packed ? 1 : 0)));
+        mixin((uint64_t)c->cu_count());
+        for (int32_t k = 0; k < npoints; ++k) mixin(((uint64_t)(uint32_t)len1[k] << 32) | (uint32_t)len2[k]);
+        if (!c->rnd_plan || c->rnd_plan->key != key) {
+            auto rp = std::make_shared<RandPlan>();
+            std::vector<uint8_t> wave_pts((size_t)npoints, 0);
+            if (mode == 0) wave_pts = sw_wave_points(len1, len2, npoints, pairs_per_point, packed, c->cu_count());
+            if (mode == 3)
+                for (int32_t k = 0; k < npoints; ++k) wave_pts[(size_t)k] = len2[k] <= 64 * 32;
+            sw_plan(len1, len2, npoints, pairs_per_point, rp->plan, WS, &wave_pts);
+            for (int32_t k = 0; k < npoints; ++k)  // the pairs scored one per wave
+                if (wave_pts[(size_t)k])
+                    for (int32_t r = 0; r < pairs_per_point; ++r) rp->wids.push_back(k * pairs_per_point + r);
+            c->rnd_plan.reset();
+            up(c->rnd_order, rp->plan.order.data(), rp->plan.order.size() * 4);
+            if (packed && !rp->wids.empty()) up(c->rnd_wlist, rp->wids.data(), rp->wids.size() * 4);
+            rp->key = key;
+            c->rnd_plan = rp;
+        }
+        const SwPlan &plan = c->rnd_plan->plan;
+        const std::vector<int32_t> &wids = c->rnd_plan->wids;
         const int64_t total = (int64_t)npoints * pairs_per_point;
         c->rnd_out.ensure((size_t)total * 4 + 64);
+        auto run_waves = [&](const std::vector<int32_t> &ids) {
+            if (ids.empty()) return;
+            up(c->rnd_wtmp, ids.data(), ids.size() * 4);
+            c->launch("sw_random_wave", [&] {
+                return launch_sw_random_wave(c->stream, c->rnd_thr.as<uint32_t>(), c->rnd_len1.as<int32_t>(),
+                                             c->rnd_len2.as<int32_t>(), c->rnd_seed.as<uint64_t>(),
+                                             c->rnd_wtmp.as<int32_t>(), pairs_per_point, (int64_t)ids.size(), P,
+                                             tab, c->rnd_out.as<int32_t>());
+            });
+        };
+        if (packed) {
+            c->rnd_ovf.ensure((size_t)total * 4 + 64);
+            c->ctr.ensure(sizeof(Counters) + 64);
+            hip_check(hipMemsetAsync(&c->ctr.as<Counters>()->pending, 0, sizeof(int32_t), c->stream), "memset");
+        } else {
+            run_waves(wids);
+        }
         const int64_t budget = ((int64_t)4 << 30) / 4;
         std::vector<int64_t> woff;
-        for (int64_t w0 = 0; w0 < (int64_t)plan.waves.size();) {
-            const int64_t w1 = sw_plan_chunk(plan, w0, budget, woff);
-            c->sw_bound.ensure((size_t)woff.back() * 4 + 64);
-            up(c->sw_woff, woff.data(), woff.size() * 8);
-            const int64_t first = 64 * w0, n = std::min<int64_t>(64 * w1, total) - first;
-            c->launch("sw_random", [&] {
-                return launch_sw_random(c->stream, c->rnd_thr.as<uint32_t>(), c->rnd_len1.as<int32_t>(),
-                                        c->rnd_len2.as<int32_t>(), c->rnd_seed.as<uint64_t>(),
-                                        c->rnd_order.as<int32_t>(), plan.slot_order, pairs_per_point, first, n, P,
-                                        tab, c->rnd_out.as<int32_t>(), c->sw_bound.as<int32_t>(),
-                                        c->sw_woff.as<int64_t>());
-            });
-            w0 = w1;
+        // with h (packed), the first launch also takes the nwave pairs of
+        // rnd_wlist in its first blocks
+        auto run_plan = [&](const SwPlan &pl, const int32_t *order, bool h, int32_t nwave) {
+            if (h && pl.waves.empty() && nwave > 0) {
+                c->launch("sw_random", [&] {
+                    return launch_sw_random_h(c->stream, c->rnd_thr.as<uint32_t>(), c->rnd_len1.as<int32_t>(),
+                                              c->rnd_len2.as<int32_t>(), c->rnd_seed.as<uint64_t>(), order,
+                                              pl.slot_order, pairs_per_point, 0, 0, P, tab, c->rnd_out.as<int32_t>(),
+                                              nullptr, nullptr, c->rnd_ovf.as<int32_t>(),
+                                              &c->ctr.as<Counters>()->pending, c->rnd_wlist.as<int32_t>(), nwave);
+                });
+                return;
+            }
+            for (int64_t w0 = 0; w0 < (int64_t)pl.waves.size();) {
+                const int64_t w1 = sw_plan_chunk(pl, w0, budget, woff);
+                c->sw_bound.ensure((size_t)woff.back() * 4 + 64);
+                up(c->sw_woff, woff.data(), woff.size() * 8);
+                const int64_t first = pl.wslots * w0, n = std::min<int64_t>(pl.wslots * w1, pl.total) - first;
+                const int32_t nw = w0 == 0 ? nwave : 0;
+                if (h)
+                    c->launch("sw_random", [&] {
+                        return launch_sw_random_h(c->stream, c->rnd_thr.as<uint32_t>(), c->rnd_len1.as<int32_t>(),
+                                                  c->rnd_len2.as<int32_t>(), c->rnd_seed.as<uint64_t>(), order,
+                                                  pl.slot_order, pairs_per_point, first, n, P, tab,
+                                                  c->rnd_out.as<int32_t>(), c->sw_bound.as<int32_t>(),
+                                                  c->sw_woff.as<int64_t>(), c->rnd_ovf.as<int32_t>(),
+                                                  &c->ctr.as<Counters>()->pending, c->rnd_wlist.as<int32_t>(), nw);
+                    });
+                else
+                    c->launch("sw_random", [&] {
+                        return launch_sw_random(c->stream, c->rnd_thr.as<uint32_t>(), c->rnd_len1.as<int32_t>(),
+                                                c->rnd_len2.as<int32_t>(), c->rnd_seed.as<uint64_t>(), order,
+                                                pl.slot_order, pairs_per_point, first, n, P, tab,
+                                                c->rnd_out.as<int32_t>(), c->sw_bound.as<int32_t>(),
+                                                c->sw_woff.as<int64_t>());
+                    });
+                w0 = w1;
+            }
+        };
+        run_plan(plan, c->rnd_order.as<int32_t>(), packed, packed ? (int32_t)wids.size() : 0);
+        if (packed) {
+            // pairs whose f16 maximum reached 2048: the int32 kernel, one pair per lane
+            int32_t novf = 0;
+            hip_check(hipMemcpyAsync(&novf, &c->ctr.as<Counters>()->pending, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                     c->stream), "overflow count");
+            c->sync();
+            if (novf > 0) {
+                std::vector<int32_t> ids((size_t)novf), wide, narrow;
+                hip_check(hipMemcpyAsync(ids.data(), c->rnd_ovf.p, (size_t)novf * 4, hipMemcpyDeviceToHost, c->stream),
+                          "overflow list");
+                c->sync();
+                // the wave kernel where the query fits its lanes, else int32 lanes
+                for (int32_t q : ids) (len2[q / pairs_per_point] <= 64 * 32 ? wide : narrow).push_back(q);
+                run_waves(wide);
+                if (!narrow.empty()) {
+                    SwPlan re;
+                    sw_plan_pairs(len1, len2, pairs_per_point, narrow, re);
+                    up(c->rnd_ovf, re.order.data(), re.order.size() * 4);
+                    run_plan(re, c->rnd_ovf.as<int32_t>(), false, 0);
+                }
+            }
         }
         hip_check(hipMemcpyAsync(out, c->rnd_out.p, (size_t)total * 4, hipMemcpyDeviceToHost, c->stream), "download");
         c->sync();

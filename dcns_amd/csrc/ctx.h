@@ -119,6 +119,7 @@ namespace dcns {
 struct Staged;
 struct MaskStaged;
 struct SeedPhase;
+struct RandPlan;
 }
 
 struct dcns_ctx {
@@ -126,6 +127,7 @@ struct dcns_ctx {
     std::shared_ptr<dcns::Staged> staged;  // sequences kept resident by dcns_stage
     std::shared_ptr<dcns::MaskStaged> mstaged;  // genome and k-mer tables kept resident by dcns_mask_stage
     std::shared_ptr<dcns::SeedPhase> seedphase;  // recursion pool sizes and the captured level graph
+    std::shared_ptr<dcns::RandPlan> rnd_plan;     // the last random-pair launch plan (reused for the same lengths)
     hipStream_t stream = nullptr;
     bool timing = false;
     std::string err;
@@ -148,6 +150,9 @@ struct dcns_ctx {
     dcns::DBuf xlist, xlist2;                            // x-drop tasks handed to the next wider kernel
     dcns::DBuf cig_scan, cig_packed;                     // cigar packing: counts / offsets / scan storage, runs
     dcns::DBuf rnd_thr, rnd_len1, rnd_len2, rnd_seed, rnd_out, rnd_order;  // random fragment pairs generated on the device
+    dcns::DBuf rnd_ovf;    // random pairs whose packed maximum left f16's exact integers
+    dcns::DBuf rnd_wlist;  // random pairs scored one per wave (the cached plan's)
+    dcns::DBuf rnd_wtmp;   // random pairs scored one per wave (int32 launches, re-runs)
     dcns::DBuf mk_seq, mk_off, mk_keys, mk_desc, mk_gbytes, mk_ghash, mk_gtab;  // maskGenome: genome, k-mer set
     dcns::DBuf mk_mask, mk_ranges, mk_boff, mk_body;                          // maskGenome: mask, CDS, FASTA bodies
     dcns::HBuf h_hits, h_roots, h_nodes, h_xout, h_cigar;  // pinned download buffers

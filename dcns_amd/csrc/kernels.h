@@ -110,18 +110,24 @@ hipError_t launch_xdrop_wide_pass2(hipStream_t st, const int8_t *seq, const Pair
 
 // swbatch.hip: SmithWaterman max score and first position for arbitrary pairs.
 // Lane plan of a K1 launch sequence over npoints points of per_point pairs
-// (pair = point * per_point + r): the waves of 64 lane slots share their loop
-// shape and run longest first.  slot_order: order[slot] is the pair; else
-// order lists the points (pairs of a point adjacent).  waves[w]: boundary
-// rows per lane of wave w.
+// (pair = point * per_point + r): the waves of wslots slots (64, or 128 for
+// the kernel with two pairs per lane) share their loop shape and run longest
+// first.  slot_order: order[slot] is the pair; else order lists the points
+// (pairs of a point adjacent; per_point >= wslots, so a wave spans at most
+// two points).  waves[w]: boundary rows per lane of wave w.
 struct SwPlan {
     std::vector<int32_t> order;
     std::vector<int32_t> waves;
     bool slot_order = false;
     int32_t per_point = 1;
+    int32_t wslots = 64;
     int64_t total = 0;
 };
-void sw_plan(const int32_t *len1, const int32_t *len2, int64_t npoints, int32_t per_point, SwPlan &plan);
+void sw_plan(const int32_t *len1, const int32_t *len2, int64_t npoints, int32_t per_point, SwPlan &plan,
+             int32_t wslots = 64, const std::vector<uint8_t> *skip = nullptr);
+// Plan over listed pairs, one per slot, waves of 64 slots of one loop shape.
+void sw_plan_pairs(const int32_t *len1, const int32_t *len2, int32_t per_point, const std::vector<int32_t> &pairs,
+                   SwPlan &plan);
 // The waves [w0, return) that fit `budget` int32 of boundary rows (at least
 // one); wave_off gets their offsets relative to the first (one more entry).
 int64_t sw_plan_chunk(const SwPlan &plan, int64_t w0, int64_t budget, std::vector<int64_t> &wave_off);
@@ -145,6 +151,29 @@ hipError_t launch_sw_random(hipStream_t st, const uint32_t *thr, const int32_t *
                             const uint64_t *seed, const int32_t *order, bool slot_order, int32_t per_point,
                             int64_t first, int64_t n, const Params &P, const ScoreTab &tab, int32_t *out,
                             int32_t *bound, const int64_t *wave_off);
+// The same with two pairs per lane in packed f16 (the plan made with wslots
+// 128), for scorings sw_random_packed accepts: match and mismatch whose f16
+// patterns have a zero low byte, gaps within f16's exact integers.  A pair
+// whose maximum reaches 2048 gets -1 in out and is appended to ovf (count in
+// *novf, zeroed by the caller) for the int32 kernel.
+bool sw_random_packed(const Params &P, uint32_t &mis4, uint32_t &diff);
+// One pair per wave, stripe j of the query in lane j (at most 64 stripes,
+// len2 <= 2048): the pairs list[0, n), int32, maxima only.
+hipError_t launch_sw_random_wave(hipStream_t st, const uint32_t *thr, const int32_t *len1, const int32_t *len2,
+                                 const uint64_t *seed, const int32_t *list, int32_t per_point, int64_t n,
+                                 const Params &P, const ScoreTab &tab, int32_t *out);
+// The points of dcns_sw_random_max whose pairs take the wave kernel: the
+// longest (rows x stripes) for as long as that shortens the launch in an
+// issue-slot model of the two kernels.
+std::vector<uint8_t> sw_wave_points(const int32_t *len1, const int32_t *len2, int64_t npoints, int32_t per_point,
+                                    bool packed, int32_t cus);
+// wlist[0, nwave): pairs scored one per wave (int32, as launch_sw_random_wave)
+// in the same launch's first blocks.
+hipError_t launch_sw_random_h(hipStream_t st, const uint32_t *thr, const int32_t *len1, const int32_t *len2,
+                              const uint64_t *seed, const int32_t *order, bool slot_order, int32_t per_point,
+                              int64_t first, int64_t n, const Params &P, const ScoreTab &tab, int32_t *out,
+                              int32_t *bound, const int64_t *wave_off, int32_t *ovf, int32_t *novf,
+                              const int32_t *wlist = nullptr, int32_t nwave = 0);
 // Allocation-free K1 (dcns_sw_batch_noalloc): boundary rows in a static device
 // array.  sw_noalloc_init takes its address on the current device (once per
 // device); sw_noalloc_run launches over pairs [0, n) in order, in as many

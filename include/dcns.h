@@ -121,14 +121,24 @@ int dcns_last_stats(const dcns_ctx *ctx, dcns_stats *out);
  *   xdrop_lane_rows  x-drop extension: a pass on the one-task-per-lane
  *                 kernels that runs past this many rows moves its task to
  *                 the one-task-per-wave kernel, and so does a traceback pass
- *                 of more rows (default 512; -1: no limit). */
+ *                 of more rows (default 512; -1: no limit).
+ *   k1_lanes      dcns_sw_random_max: 0 (default): the longest points' pairs
+ *                 one per wave (query stripes across the lanes, int32) where
+ *                 an issue-slot model shows that shortens the launch, the
+ *                 others two per lane in packed f16 when the scoring's f16
+ *                 match and mismatch have a zero low byte and the gap costs
+ *                 are within f16's exact integers (each pair whose maximum
+ *                 reaches 2048 scored again in int32), else one per lane in
+ *                 int32; 1: every pair one per lane, int32; 2: every pair on
+ *                 the lane kernels as in 0; 3: every pair whose query fits
+ *                 64 stripes (2048 bases) one per wave. */
 typedef struct dcns_tuning {
     int32_t level0_mode, level0_waves;
     int32_t wave_tasks, wave_rows;
     int32_t graph_levels;
     int32_t pool_min;
     int32_t xdrop_lane_rows;
-    int32_t pad;
+    int32_t k1_lanes;
 } dcns_tuning;
 int dcns_get_tuning(const dcns_ctx *ctx, dcns_tuning *out);
 int dcns_set_tuning(dcns_ctx *ctx, const dcns_tuning *in);

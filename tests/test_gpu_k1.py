@@ -182,3 +182,58 @@ def test_sw_bas_matches_reference_body():
     finally:
         ctx.close()
     assert sum(c["never_ends"] for c in gold) > 20 and sum(1 for c in gold if c["reverse"] and not c["never_ends"]) > 10
+
+
+def test_sw_random_packed_overflow_rerun_and_lanes(gpu):
+    """The K1 launch shapes (tuning k1_lanes): one int32 pair per lane, two
+    packed f16 pairs per lane, one pair per wave, and the routed mix give the
+    same maxima, on shapes where some maxima pass 2048 (GC 1.0 with a
+    linear-regime scoring, so packed pairs take the int32 re-run), plus a
+    scoring the packed kernel does not take (f16 match 9 has a nonzero low
+    byte); the overflowing ones also against the oracle."""
+    pts = [(1.0, 2000, 2000), (1.0, 1900, 1700), (0.98, 2000, 1999), (0.5, 300, 2000), (0.35, 38, 39)]
+    seeds = np.array([777 + 31 * k for k in range(len(pts))], np.uint64)
+    gc = [p[0] for p in pts]
+    l1 = [p[1] for p in pts]
+    l2 = [p[2] for p in pts]
+    old = gpu.get_tuning()
+    try:
+        for sc in ((3, -1, -2, -1), (9, -3, -4, -2), (2, -3, -4, -2)):
+            got = {}
+            for mode in (1, 2, 3, 0):  # int32 lanes, packed lanes, waves, the routed mix
+                print("sw random lanes", sc, mode, flush=True)
+                gpu.set_tuning(k1_lanes=mode)
+                got[mode] = gpu.sw_random_max(sc, gc, l1, l2, seeds, 70)
+            for mode in (2, 3, 0):
+                assert np.array_equal(got[mode], got[1]), (sc, mode)
+            if sc == (3, -1, -2, -1):
+                assert got[2].max() >= 2048 and got[2].min() >= 0
+            # the same shapes again with other seeds: the cached launch plan
+            # (after a call whose overflowing pairs re-ran) against int32 lanes
+            seeds2 = seeds + np.uint64(99991)
+            gpu.set_tuning(k1_lanes=0)
+            a = gpu.sw_random_max(sc, gc, l1, l2, seeds2, 70)
+            b = gpu.sw_random_max(sc, gc, l1, l2, seeds2, 70)
+            gpu.set_tuning(k1_lanes=1)
+            assert np.array_equal(a, gpu.sw_random_max(sc, gc, l1, l2, seeds2, 70)), sc
+            assert np.array_equal(a, b), sc
+    finally:
+        gpu.set_tuning(**old)
+    # the linear-regime pairs (maxima past 2048) against the oracle, packed lanes
+    gpu.set_tuning(k1_lanes=2)
+    try:
+        got = gpu.sw_random_max((3, -1, -2, -1), gc[:1], l1[:1], l2[:1], seeds[:1], 6).ravel()
+    finally:
+        gpu.set_tuning(**old)
+    with np.errstate(over="ignore"):
+        seqs, o1, ln1, o2, ln2 = [], [], [], [], []
+        off = 0
+        for r in range(6):
+            a = _random_fragment(_mix64(seeds[0] + np.uint64(2 * r)), l1[0], gc[0])
+            b = _random_fragment(_mix64(seeds[0] + np.uint64(2 * r + 1)), l2[0], gc[0])
+            seqs += [a, b]
+            o1.append(off); ln1.append(len(a)); off += len(a)
+            o2.append(off); ln2.append(len(b)); off += len(b)
+    want = bind_sw_max(np.concatenate(seqs), o1, ln1, o2, ln2, (3, -1, -2, -1))
+    assert np.array_equal(got, want)
+    assert got.max() >= 2048
