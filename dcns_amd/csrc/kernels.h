@@ -197,15 +197,17 @@ int32_t sg_task_strips(int32_t R, int32_t C, int32_t w, int32_t task, int64_t bn
 // sync: 2 + nstrips zeroed ints (start counter, error word, per-strip progress).
 // W.mode 0: SemiGlobal_no_avx with P's scores; 1 / 2: weighted two-piece /
 // single gap with W's profiles (P's scores unused).
-// keyed: every in-band score of the launch's tasks fits 22 bits (sg_keyed_ok),
-// so the unweighted kernel may carry the row maximum and its column as one int.
+// keyed: every in-band score of the launch's tasks fits 22 bits and every gap
+// penalty 20 bits (sg_keyed_ok), so the unweighted kernel may carry the row
+// maximum and its column as one int, and its traceback form uses tagged
+// values, whose trace bytes the finish kernel reads with tagfmt set.
 hipError_t launch_sg_dp(hipStream_t st, bool trace, bool keyed, const int8_t *seq, const PairDesc *pairs,
                         const SgTask *tasks, const SgStrip *strips, const int32_t *order, int32_t nstrips,
                         const Params &P, int32_t *scr, uint8_t *tb, SgOut *out, const SgWeights &W, int32_t *sync);
 bool sg_keyed_ok(const Params &P, int64_t max_rows, int64_t max_cols);
 hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
                             const SgStrip *strips, int32_t n, const Params &P, const int32_t *scr, const uint8_t *tb,
-                            SgOut *out, uint32_t *cigar, const SgWeights &W);
+                            SgOut *out, uint32_t *cigar, const SgWeights &W, bool tagfmt);
 
 // mask.hip: maskGenome's k-mer scan, CDS ranges and FASTA bodies.
 int64_t kmer_general_slots(int64_t n);      // slots of the general members' table

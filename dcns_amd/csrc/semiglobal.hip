@@ -118,6 +118,32 @@ __device__ __forceinline__ void column_table(int32_t qc, int32_t match, int32_t 
 __device__ __forceinline__ uint32_t row_selector(int32_t rc) {
     return rc == 0 ? 0x08080801u : rc == 1 ? 0x09090903u : rc == 3 ? 0x0A0A0A05u : rc == 4 ? 0x0B0B0B07u : 0x0C0C0C0Cu;
 }
+// The same lookup one byte up: byte 0 is zero (selector 12), so the v_perm
+// yields the score times 256, the scale of the tagged values below.
+__device__ __forceinline__ uint32_t row_selector_tag(int32_t rc) {
+    return rc == 0 ? 0x0808010Cu : rc == 1 ? 0x0909030Cu : rc == 3 ? 0x0A0A050Cu : rc == 4 ? 0x0B0B070Cu : 0x0C0C0C0Cu;
+}
+
+// Tagged values (the traceback kernel where the keyed row maximum applies).
+// A DP value v is held as v * 256 plus a low byte: bits 4-6 name the state
+// that holds it (0 the diagonal term, 1 F1, 2 e1, 3 F2, 4 e2, the reference's
+// order of the five-way maximum) and bits 0-3 are the extension flags of the
+// gap states (bit 0 F1, 1 e1, 2 F2, 3 e2), each set only in its own state.
+// So the maximum of the five tagged values is the reference's maximum with
+// its tie order (the later source wins on equal scores), and its bits 4-6 are
+// the source the trace byte records.  A new gap state is the maximum of its
+// extension, which carries its flag, and its opening, which does not: on
+// equal scores the extension wins, as the reference's test "extension >=
+// opening" does, and the state's flag is that test's outcome.  The trace
+// byte is then source << 4 | the four new states' flags, with no compare
+// and select per bit.  Out-of-band values are NEGK (-2^22 * 256) with their
+// state's tag: in-band scores stay below 2^21 in magnitude wherever the keyed
+// row maximum is used, every in-band cell has an in-band predecessor, and
+// the out-of-band values that in-band cells read are NEGK plus one step's
+// penalties, so no maximum or test that an in-band cell takes differs from
+// the reference's with its NEG.
+constexpr int32_t NEGK = -(1 << 30);
+constexpr int32_t kTagF1 = 0x10, kTagE1 = 0x20, kTagF2 = 0x30, kTagE2 = 0x40;
 
 // Keyed row maximum: (maximum, first column) as one int, maximum << 9 | a
 // tie rank, so that one v_max keeps the first strict maximum.  Inside a strip
@@ -138,12 +164,14 @@ __device__ __forceinline__ int32_t key_col(int32_t key, int32_t col_in, int32_t 
 struct StripIn {
     const int32_t *bin;  // previous strip's records, row plo first
     int32_t first, plo, phi, R, a1, rev;
+    bool tag;            // tagged values (see NEGK)
     const int8_t *s1;
     const uint8_t *wc;   // weighted: profile indices of reference positions -1 .. len1 (wc[x + 1])
     __device__ __forceinline__ void load(int32_t r, int32_t &m, int32_t &e1, int32_t &e2, int32_t &rm, int32_t &rj,
                                          int32_t &rc) const {
-        m = NEG;
-        e1 = e2 = NEG;
+        m = tag ? NEGK : NEG;
+        e1 = tag ? NEGK | kTagE1 : NEG;
+        e2 = tag ? NEGK | kTagE2 : NEG;
         rm = rj = 0;
         rc = 2;
         if (first) {
@@ -189,7 +217,7 @@ __device__ __forceinline__ void columns(const uint32_t (&tlo)[K], const uint32_t
         const int32_t m0 = sc + dg;
         dg = Mp[c];
         const int32_t f1 = F1[c], f2o = F2[c];
-        const int32_t m = max(max(max(m0, f1), max(e1, f2o)), e2);
+        const int32_t m = max(max(max(max(m0, f1), e1), f2o), e2);
         const bool left = EDGE && c < lo, inner = !EDGE || c < hi;
         const bool inb = EDGE ? (!left & (c <= hi)) : true;
         if (KEYED) {
@@ -219,6 +247,46 @@ __device__ __forceinline__ void columns(const uint32_t (&tlo)[K], const uint32_t
         e2 = left ? NEG : max(ne2, h2);
         Mp[c] = m;
     }
+}
+
+// columns() with traceback on tagged values (see NEGK): the same cells and
+// band rules; o1f/o1e/o2f/o2e are the openings times 256 with the tag of the
+// state they open (F1, e1, F2, e2), x1s/x2s the extensions times 256, and kc
+// the keyed row maximum's rank of the lane's column 0.  Stored diagonals (Mp)
+// and the carried M are untagged (low byte 0).
+template <bool EDGE, bool X2Z>
+__device__ __forceinline__ void columns_tag(const uint32_t (&tlo)[K], const uint32_t (&thi)[K], int32_t (&F1)[K],
+                                            int32_t (&F2)[K], int32_t (&Mp)[K], uint32_t sel, int32_t lo, int32_t hi,
+                                            int32_t kc, int32_t o1f, int32_t o1e, int32_t x1s, int32_t o2f,
+                                            int32_t o2e, int32_t x2s, int32_t dg, int32_t &e1, int32_t &e2,
+                                            int32_t &rm, uint32_t &tword) {
+    uint32_t bc[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const int32_t m0 = (int32_t)perm(thi[c], tlo[c], sel) + dg;
+        dg = Mp[c];
+        const int32_t f1 = F1[c], f2o = F2[c];
+        const int32_t mk = max(max(max(max(m0, f1), e1), f2o), e2);
+        const int32_t mc = mk & ~0xff;
+        const bool left = EDGE && c < lo, inner = !EDGE || c < hi;
+        const bool inb = EDGE ? (!left & (c <= hi)) : true;
+        // keyed row maximum: score << 9 | rank = mc * 2 + rank
+        const int32_t key = (int32_t)(((uint32_t)mc << 1) + (uint32_t)(kc - c));
+        rm = inb ? max(rm, key) : rm;
+        const int32_t f = (f1 | 1) + x1s, ne1 = (e1 | 2) + x1s;
+        const int32_t f2 = X2Z ? (f2o | 4) : (f2o | 4) + x2s, ne2 = X2Z ? (e2 | 8) : (e2 | 8) + x2s;
+        const int32_t nf1 = max(f, mc + o1f), ne1n = max(ne1, mc + o1e);
+        const int32_t nf2 = max(f2, mc + o2f), ne2n = max(ne2, mc + o2e);
+        const uint32_t b = ((uint32_t)(nf1 | ne1n | nf2 | ne2n) & 0x0fu) | ((uint32_t)mk & ~0x0fu);
+        bc[c] = (!left & inner) ? b : 0xffu;
+        F1[c] = inner ? nf1 : f1;
+        F2[c] = inner ? nf2 : f2o;
+        e1 = left ? (NEGK | kTagE1) : ne1n;
+        e2 = left ? (NEGK | kTagE2) : ne2n;
+        Mp[c] = mc;
+    }
+    // byte 0 of each column's value, columns 0..3 in bytes 0..3
+    tword = perm(bc[1], bc[0], 0x0C0C0400u) | perm(bc[3], bc[2], 0x04000C0Cu);
 }
 
 // The weighted counterpart of columns() (SemiGlobal with a Score profile per
@@ -312,6 +380,11 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
     const int8_t *s1 = seq + pd.off1, *s2 = seq + pd.off2;
     const int32_t R = T.R, C = T.C, w = P.bandwidth;
     const int32_t o1 = P.open, x1 = P.ext, o2 = P.open2, x2 = X2Z ? 0 : P.ext2;
+    // traceback with the keyed row maximum: tagged values (see NEGK)
+    constexpr bool TAG = TB && KEYED && MODE == 0;
+    const int32_t o1f = o1 * 256 + kTagF1, o1e = o1 * 256 + kTagE1, o2f = o2 * 256 + kTagF2, o2e = o2 * 256 + kTagE2,
+                  x1s = x1 * 256, x2s = x2 * 256;
+    constexpr int32_t NM = TAG ? NEGK : NEG, NE1 = TAG ? NEGK | kTagE1 : NEG, NE2 = TAG ? NEGK | kTagE2 : NEG;
     int32_t *rowres = scr + T.scr;
     int32_t *bout = scr + S.bnd;
     // the lane and column that hold column C-1 (for the stale diagonals)
@@ -334,6 +407,7 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
     in.rev = T.rev;
     in.s1 = s1;
     in.wc = MODE != 0 ? W.cat + W.cat_off[T.pair] : nullptr;
+    in.tag = TAG;
 
     const int32_t nsteps = S.rhi - S.rlo + 1 + 63;
     const int32_t jfirst = S.s * SW + 1 + lane * K;
@@ -347,11 +421,13 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
         const int32_t j = jfirst + c;
         qc[c] = j <= C ? code_at(s2, T.a2, j - 1, T.rev) : 2;
         if (MODE == 0) column_table(qc[c], P.match, P.mismatch, tlo[c], thi[c]);
-        F1[c] = F2[c] = Mp[c] = NEG;
+        F1[c] = TAG ? NEGK | kTagF1 : NEG;
+        F2[c] = TAG ? NEGK | kTagF2 : NEG;
+        Mp[c] = NM;
     }
     // carried row maximum: (max, column), or keyed (key, column carried in)
     constexpr int32_t kRm0 = KEYED ? kKeyIn : 0;
-    int32_t prev_im = NEG, om = NEG, oe1 = NEG, oe2 = NEG, orm = kRm0, orj = 0,
+    int32_t prev_im = NM, om = NM, oe1 = NE1, oe2 = NE2, orm = kRm0, orj = 0,
             orc = MODE == 0 ? (int32_t)row_selector(2) : 2;
     bool ok = true;
     for (int32_t t0 = 0; t0 < nsteps; t0 += 64) {
@@ -365,14 +441,15 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
         if (t0 == 0) {  // the diagonal of the strip's first row
             int32_t a, b, c2, d, f;
             in.load(S.rlo - 1, prev_im, a, b, c2, d, f);
-            if (lane != 0) prev_im = NEG;
+            if (lane != 0) prev_im = NM;
         }
         {
             // lane 0's inputs of the block's steps (rows rlo + t0 + q), one
             // 8-int record per step in LDS, read back by every lane at once
             int32_t cm, ce1, ce2, crm, crj, crc;
             in.load(S.rlo + t0 + lane, cm, ce1, ce2, crm, crj, crc);
-            if (MODE == 0) crc = (int32_t)row_selector(crc);  // the row's score selector travels instead of its code
+            // the row's score selector travels instead of its code
+            if (MODE == 0) crc = (int32_t)(TAG ? row_selector_tag(crc) : row_selector(crc));
             if (KEYED) {
                 // the previous strip's record holds its keyed maximum and the
                 // column carried into it: decode, then rank it above every
@@ -429,7 +506,15 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
                 int32_t e1 = ie1, e2 = ie2, rm = irm, rj = irj;
                 // every lane's columns inside the band (the usual case): no edge masks
                 const bool interior = KIND == 1 || (KIND == 0 && __ballot(lo > 0 || hi < K) == 0);
-                if (MODE == 0) {
+                if (TAG) {
+                    const uint32_t sel = (uint32_t)irc;
+                    if (interior)
+                        columns_tag<false, X2Z>(tlo, thi, F1, F2, Mp, sel, lo, hi, kc, o1f, o1e, x1s, o2f, o2e, x2s,
+                                                diag, e1, e2, rm, tword);
+                    else
+                        columns_tag<true, X2Z>(tlo, thi, F1, F2, Mp, sel, lo, hi, kc, o1f, o1e, x1s, o2f, o2e, x2s,
+                                               diag, e1, e2, rm, tword);
+                } else if (MODE == 0) {
                     const uint32_t sel = (uint32_t)irc;
                     if (interior)
                         columns<TB, false, KEYED>(tlo, thi, F1, F2, Mp, sel, lo, hi, jfirst, kc, o1, x1, o2, x2, diag,
@@ -458,10 +543,11 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
                 orj = rj;
                 if (!FULL && MODE == 0 && S.s == stale_strip && lane == stale_lane &&
                     (r == C + w - 2 || r == C + w - 1))
-                    out[S.task].stale[r - (C + w - 2)] = Mp[stale_c];
+                    out[S.task].stale[r - (C + w - 2)] = TAG ? Mp[stale_c] >> 8 : Mp[stale_c];
             } else {
-                om = NEG;
-                oe1 = oe2 = NEG;
+                om = NM;
+                oe1 = NE1;
+                oe2 = NE2;
                 orm = kRm0;
                 orj = 0;
             }
@@ -570,13 +656,14 @@ __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
 // One wave per task: the reference's row loop termination over the row
 // results (z-drop, empty row), 64 rows at a time, the maximum, and the
 // traceback (trace tasks) through 64 x 64 tiles of trace bytes loaded by the
-// whole wave at once.
+// whole wave at once.  tagfmt: the trace bytes are source << 4 | flags (the
+// tagged kernel); they are turned into source | flags << 3 as they load.
 __global__ __launch_bounds__(64) void k_sg_finish(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
                                                   const SgTask *__restrict__ tasks,
                                                   const SgStrip *__restrict__ strips, int32_t n, Params P,
                                                   const int32_t *__restrict__ scr, const uint8_t *__restrict__ tb,
                                                   SgOut *__restrict__ out, uint32_t *__restrict__ cigar,
-                                                  SgWeights W) {
+                                                  SgWeights W, int32_t tagfmt) {
     __shared__ uint32_t tile[64 * 16];
     const int32_t k = blockIdx.x;
     if (k >= n) return;
@@ -704,6 +791,7 @@ __global__ __launch_bounds__(64) void k_sg_finish(const int8_t *__restrict__ seq
                 const bool ok = j >= 1 && r >= 1 && r >= rlo && r <= rhi;
                 const int64_t off = ok ? T.tb + sto + (int64_t)(r - rlo + l) * SW + l * K : T.tb;
                 wv[g] = *(const uint32_t *)(tb + off);
+                if (tagfmt) wv[g] = ((wv[g] >> 4) & 0x07070707u) | ((wv[g] & 0x0F0F0F0Fu) << 3);
                 wv[g] = ok ? wv[g] : 0xffffffffu;
             }
 #pragma unroll
@@ -815,13 +903,18 @@ bool sg_keyed_ok(const Params &P, int64_t max_rows, int64_t max_cols) {
     // Every in-band M(i, j) lies between the score of a diagonal of
     // mismatches to (min, min) followed by one gap run (which stays in the
     // band), and match * min(rows, cols); the key holds scores below 2^22 in
-    // magnitude (here 2^21, with margin).
+    // magnitude (here 2^21, with margin), the tagged values of the traceback
+    // kernel scores and penalties times 256.
     const int64_t L = std::max<int64_t>(max_cols, 1);
     const int64_t gap = std::max((int64_t)P.open + (L - 1) * P.ext, (int64_t)P.open2 + (L - 1) * P.ext2);
     const int64_t low = std::min<int64_t>(P.mismatch, 0) * std::max<int64_t>(max_rows, 0) + std::min<int64_t>(gap, 0);
     const int64_t high = (int64_t)std::max(P.match, 0) * std::min(max_rows, max_cols);
     const int64_t lim = (int64_t)1 << 21;
-    return -low < lim && high < lim;
+    // the tagged traceback values hold every penalty times 256
+    const int64_t pen = (int64_t)1 << 20;
+    const bool pen_ok = -(int64_t)P.open < pen && -(int64_t)P.ext < pen && -(int64_t)P.open2 < pen &&
+                        -(int64_t)P.ext2 < pen && P.open <= 0 && P.ext <= 0 && P.open2 <= 0 && P.ext2 <= 0;
+    return -low < lim && high < lim && pen_ok;
 }
 
 hipError_t launch_sg_dp(hipStream_t st, bool trace, bool keyed, const int8_t *seq, const PairDesc *pairs,
@@ -878,10 +971,10 @@ hipError_t launch_sg_dp(hipStream_t st, bool trace, bool keyed, const int8_t *se
 
 hipError_t launch_sg_finish(hipStream_t st, const int8_t *seq, const PairDesc *pairs, const SgTask *tasks,
                             const SgStrip *strips, int32_t n, const Params &P, const int32_t *scr, const uint8_t *tb,
-                            SgOut *out, uint32_t *cigar, const SgWeights &W) {
+                            SgOut *out, uint32_t *cigar, const SgWeights &W, bool tagfmt) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sg_finish, dim3(n), dim3(64), 0, st, seq, pairs, tasks, strips, n, P, scr, tb, out, cigar,
-                       W);
+                       W, (int32_t)tagfmt);
     return hipGetLastError();
 }
 

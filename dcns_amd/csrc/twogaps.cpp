@@ -175,7 +175,7 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
             return launch_sg_finish(c->stream, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->sg_tasks.as<SgTask>(),
                                     c->sg_strips.as<SgStrip>(), m, P, c->sg_scr.as<int32_t>(),
                                     trace ? c->sg_tb.as<uint8_t>() : nullptr, c->sg_out.as<SgOut>(),
-                                    trace ? c->sg_cig.as<uint32_t>() : nullptr, W);
+                                    trace ? c->sg_cig.as<uint32_t>() : nullptr, W, trace && keyed && W.mode == 0);
         });
         int32_t stalled = 0;
         hip_check(hipMemcpyAsync(&stalled, c->sg_sync.as<int32_t>() + 1, 4, hipMemcpyDeviceToHost, c->stream),

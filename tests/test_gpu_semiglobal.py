@@ -44,6 +44,11 @@ def test_semiglobal_golden(gpu):
     (150, 50, (2, -3, -4, -2, -45, 0)),
     (40, 25, (2, -3, -4, -2, -20, -1)),
     (3, 10, (1, -2, -3, -1, -15, 0)),
+    # many equal candidates: the tie order of the five-way maximum and of
+    # extension against opening decides the trace
+    (60, 20, (1, -1, -2, -1, -3, 0)),
+    # penalties far from the defaults, scaled by 256 in the tagged traceback
+    (40, 30, (5, -4, -120, -7, -5000, -1)),
 ])
 def test_semiglobal_tasks_vs_oracle(gpu, oracle_lib, bandwidth, zdrop, sc2):
     """Forward and reverse tasks from anchors inside batched pairs, some with
