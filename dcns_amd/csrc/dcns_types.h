@@ -66,6 +66,7 @@ struct Counters {
     unsigned long long gen_cells;  // DP cells evaluated by the later-level scans
     int32_t queue;        // next level-0 task of the persistent scan's queue
     int32_t levels;       // recursion levels after level 0 that had tasks
+    int32_t strong_chunks;  // level-0 strong-row chunks handed out
 };
 
 // x-drop extension task for one seed (x_extend_seed,

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "dcns_types.h"
@@ -25,6 +26,68 @@ inline bool scan_general_packed(const Params &P, uint32_t &mis4, uint32_t &diff)
     diff = (m ^ x) >> 8;
     return true;
 }
+// Level-0 tails of the rescans.  A rescan of rows [a, b) of a window starts
+// from zeros at row a.  The scan kernels evaluate wm columns (the window,
+// padded with N columns, which score 0).  A path of cells that starts above
+// row a and ends on row r enters at most wm of its rows diagonally, gains at
+// most match on each of at most `window` real columns, and pays at least
+// g = -max(open, ext) for each row it enters by a vertical gap step, so its
+// score is at most window * match - g * (r - a + 1 - wm).  From row
+// a + level0_tail_rows(P, wm) on that is <= 0, so no such path reaches a
+// cell (every cell and gap state is floored at 0): the rescan's values
+// there equal level 0's.  Such a rescan runs only its first
+// level0_tail_rows rows and takes the rest of its trajectory from level 0's
+// strong rows (rows whose window maximum reaches min_score).  0: the bound
+// does not hold (g <= 0).
+inline int32_t level0_tail_rows(const Params &P, int32_t wm) {
+    const int32_t g = -std::max(P.open, P.ext);
+    if (g <= 0 || P.match <= 0) return 0;
+    return wm + (P.window * P.match + g - 1) / g;
+}
+// The column of a strong-row entry is not searched in level 0's lane
+// kernel: kStrongNoCol there, found again by the tail from a short rescan.
+constexpr int32_t kStrongNoCol = 0xff;
+// Level 0 appends every strong row of every window to the sink: chunks of
+// kStrongChunk entries handed out one at a time from *nchunks to each lane;
+// an entry is the global window id and row << 32 | maximum << 8 | first
+// column of the maximum.  cap_chunks 0: no recording.  When more chunks are
+// asked for than the buffers hold, the later entries are dropped and
+// *nchunks tells the host so.
+constexpr int32_t kStrongChunk = 64;
+struct StrongSink {
+    uint32_t *win;
+    uint64_t *rv;
+    int32_t *chunk_fill;  // entries used per chunk
+    int32_t *nchunks;
+    int32_t cap_chunks;
+};
+// The strong rows sorted by window (row order within a window): seg[w] =
+// [begin, end) of global window w, ng[i] = the next index of the same window
+// whose maximum is strictly larger (end of the window's range when none).
+struct StrongIndex {
+    const int2 *seg;
+    const uint64_t *rv;
+    const int32_t *ng;
+    int32_t delta;  // level0_tail_rows
+};
+// Builds the index from the sink's first nchunks chunks: scratch_bytes of
+// scratch (sized by strong_index_scratch) at scratch; win2/rv2 hold n
+// entries (the total the host read back), seg nw windows.
+size_t strong_index_scratch(int64_t nchunks, int64_t n);
+hipError_t launch_strong_index(hipStream_t st, const StrongSink &S, int32_t nchunks, int64_t n, int32_t nw,
+                               int32_t win_bits, void *scratch, size_t scratch_bytes, uint32_t *win2, uint64_t *rv2,
+                               int64_t *chunk_off, int2 *seg, int32_t *ng);
+// Entries of the chunks: the exclusive scan of chunk_fill into chunk_off and
+// the total in *total_dev (one int64).
+hipError_t launch_strong_count(hipStream_t st, const StrongSink &S, int32_t nchunks, void *scratch,
+                               size_t scratch_bytes, int64_t *chunk_off, int64_t *total_dev);
+// After a level's rescans (of at most index.delta rows each): every task
+// whose rows reach past a + delta appends level 0's strong rows of
+// [a + delta, min(node b, lastrow + 1)) that raise its running maximum.
+hipError_t launch_rescan_tail(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const Params &P,
+                              const ScanTask *tasks, const int32_t *n_dev, int32_t n_max, int32_t blocks_max,
+                              const Node *nodes, const int32_t *lastrow, const StrongIndex &I, TrajEntry *traj,
+                              int32_t *tcount, int32_t cap, Counters *ctr);
 // Level-0 window scan: one workgroup per 128-window task, or, with
 // persistent_waves > 0, that many waves popping the task list from ctr->queue.
 // wins[0, nwins): level-0 windows (pair, window) scanned one per wave on the
@@ -32,7 +95,7 @@ inline bool scan_general_packed(const Params &P, uint32_t &mis4, uint32_t &diff)
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
                                int32_t nwt, int32_t persistent_waves, const Params &P, const ScoreTab &tab,
                                TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr,
-                               const int2 *wins, int32_t nwins, int32_t *lastrow);
+                               const int2 *wins, int32_t nwins, int32_t *lastrow, const StrongSink &sink);
 // Later-level rescans: *ntasks_dev tasks (at most ntasks_max) in order[];
 // up to wave_tasks of the first ones with more than wave_rows rows run one
 // per wave, the rest one per lane on at most lane_blocks_max waves (0: one
@@ -47,7 +110,7 @@ hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairD
                           const int32_t *hitlist, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
                           int32_t blocks_max, Node *nodes, int32_t node_cap, const TrajEntry *traj,
                           const int32_t *tcount, int32_t cap, int32_t scan_cap, ScanTask *next, int32_t next_cap,
-                          Counters *ctr, const int32_t *lastrow);
+                          Counters *ctr, const int32_t *lastrow, int32_t tail_rows);
 // Counter block at the start of a seed phase (nodes = scans = nw, the rest 0).
 hipError_t launch_seed_init(hipStream_t st, Counters *ctr, int32_t nw);
 // Start of a recursion level: level_tasks = next_tasks (0 after an

@@ -262,6 +262,8 @@ namespace {
 // with at most this many waves (the task count is not known when they are
 // enqueued).
 constexpr int32_t kGraphLaneWaves = 4096;
+// References from which the rescans' level-0 tails are on by default.
+constexpr int32_t kTailMinRef = 65536;
 
 void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_t &gpu_cells, int32_t &levels) {
     const int32_t NW = (int32_t)S.nwin;
@@ -275,6 +277,15 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
     const int32_t cap = P.window * P.match - P.min_score + 1;
     const ScoreTab tab = make_score_tab(P.match, P.mismatch);
     const SeedTuning T = seed_tuning(c);
+    // level-0 tails of the rescans (level0_tail_rows): on by request, or by
+    // default where a reference is long enough for rescans to dominate
+    int32_t delta = level0_tail_rows(P, wm);
+    {
+        int32_t longest = 0;
+        for (const PairDesc &d : S.pd) longest = std::max(longest, d.len1);
+        const int32_t want = c->tune.level0_tail;
+        if (want < 0 || (want == 0 && longest < kTailMinRef) || !scan_uses_f16(P)) delta = 0;
+    }
     if (!c->seedphase) c->seedphase = std::make_shared<SeedPhase>();
     SeedPhase &SP = *c->seedphase;
     if (SP.nw != NW || SP.pool_min != T.pool_min) {
@@ -301,6 +312,21 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
     ensure_pools();
     c->hitlist.ensure((size_t)NW * sizeof(int32_t) + 64);
     c->lastrow.ensure((size_t)NW * sizeof(int32_t) + 64);
+    c->ctr.ensure(sizeof(Counters) + 64);  // before the sink takes the address of its chunk counter
+    StrongSink sink{nullptr, nullptr, nullptr, nullptr, 0};
+    StrongIndex SI{nullptr, nullptr, nullptr, 0};
+    if (delta > 0) {
+        // chunk budget: a quarter of the free device memory, at most 4 GiB
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const int64_t bytes = std::min<int64_t>((int64_t)(free_b / 4), (int64_t)4 << 30);
+        const int64_t chunks = std::max<int64_t>(bytes / (kStrongChunk * 12 + 4), 1024);
+        c->strong_win.ensure((size_t)chunks * kStrongChunk * sizeof(uint32_t));
+        c->strong_rv.ensure((size_t)chunks * kStrongChunk * sizeof(uint64_t));
+        c->strong_fill.ensure((size_t)chunks * sizeof(int32_t));
+        sink = StrongSink{c->strong_win.as<uint32_t>(), c->strong_rv.as<uint64_t>(), c->strong_fill.as<int32_t>(),
+                          &c->ctr.as<Counters>()->strong_chunks, (int32_t)std::min<int64_t>(chunks, INT32_MAX / 2)};
+    }
     c->ctr.ensure(sizeof(Counters) + 64);
     c->order_hist.ensure(2 * kOrderBuckets * sizeof(int32_t));
     Counters *ctr = c->ctr.as<Counters>();
@@ -313,7 +339,7 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
                               L == 0, c->hitlist.as<int32_t>(), L == 0 ? nullptr : buf(L - 1),
                               L == 0 ? hits_dev : lt_dev, n_max, blocks_max, c->nodes.as<Node>(),
                               (int32_t)SP.node_cap, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
-                              (int32_t)SP.scan_cap, buf(L), (int32_t)SP.next_cap, ctr, c->lastrow.as<int32_t>());
+                              (int32_t)SP.scan_cap, buf(L), (int32_t)SP.next_cap, ctr, c->lastrow.as<int32_t>(), delta);
     };
     auto level_launches = [&](int L, int32_t n_max, int32_t lane_blocks_max, bool timed) {
         auto run = [&](const char *name, auto f) {
@@ -331,6 +357,13 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
                                        lane_blocks_max, P, tab, c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
                                        ctr);
         });
+        if (delta > 0)
+            run("rescan_tail", [&] {
+                return launch_rescan_tail(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), P, buf(L - 1),
+                                          lt_dev, n_max, lane_blocks_max, c->nodes.as<Node>(),
+                                          c->lastrow.as<int32_t>(), SI, c->traj.as<TrajEntry>(),
+                                          c->tcount.as<int32_t>(), cap, ctr);
+            });
     };
 
     c->launch("seed_init", [&] { return launch_seed_init(c->stream, ctr, NW); });
@@ -338,8 +371,41 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
         return launch_scan_uniform(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->wtasks.as<int2>(),
                                    (int32_t)S.wt.size(), T.level0_waves, P, tab, c->traj.as<TrajEntry>(),
                                    c->tcount.as<int32_t>(), cap, c->hitlist.as<int32_t>(), ctr,
-                                   c->l0wins.as<int2>(), (int32_t)S.ww.size(), c->lastrow.as<int32_t>());
+                                   c->l0wins.as<int2>(), (int32_t)S.ww.size(), c->lastrow.as<int32_t>(), sink);
     });
+    // the strong rows' index: chunk fills -> offsets and total, compaction,
+    // a stable sort by window, per-window ranges and next-larger links
+    if (delta > 0) {
+        const int32_t nchunks = read_ctr(c).strong_chunks;
+        if (nchunks > sink.cap_chunks) {
+            delta = 0;  // entries were dropped: full rescans for this batch
+        } else {
+            c->strong_off.ensure((size_t)std::max(nchunks, 1) * sizeof(int64_t));
+            c->strong_total.ensure(64);
+            const size_t tmp0 = strong_index_scratch(nchunks, 1);
+            c->strong_tmp.ensure(tmp0);
+            hip_check(launch_strong_count(c->stream, sink, nchunks, c->strong_tmp.p, c->strong_tmp.cap,
+                                          c->strong_off.as<int64_t>(), c->strong_total.as<int64_t>()),
+                      "strong_count");
+            int64_t n = 0;
+            hip_check(hipMemcpyAsync(&n, c->strong_total.p, sizeof(n), hipMemcpyDeviceToHost, c->stream), "strong total");
+            c->sync();
+            c->strong_win2.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(uint32_t));
+            c->strong_rv2.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(uint64_t));
+            c->strong_seg.ensure((size_t)NW * sizeof(int2) + 64);
+            c->strong_ng.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t));
+            c->strong_tmp.ensure(strong_index_scratch(nchunks, n));
+            int32_t win_bits = 1;
+            while (win_bits < 32 && ((int64_t)1 << win_bits) < (int64_t)NW) ++win_bits;
+            c->launch("strong_index", [&] {
+                return launch_strong_index(c->stream, sink, nchunks, n, NW, win_bits, c->strong_tmp.p, c->strong_tmp.cap,
+                                           c->strong_win2.as<uint32_t>(), c->strong_rv2.as<uint64_t>(),
+                                           c->strong_off.as<int64_t>(), c->strong_seg.as<int2>(),
+                                           c->strong_ng.as<int32_t>());
+            });
+            SI = StrongIndex{c->strong_seg.as<int2>(), c->strong_rv.as<uint64_t>(), c->strong_ng.as<int32_t>(), delta};
+        }
+    }
 
     Counters h{};
     int L = 1;  // the next level to run
@@ -356,7 +422,8 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
                                     (int64_t)c->order_hist.p, (int64_t)c->lastrow.p, (int64_t)ctr, NW,
                                     SP.scan_cap, SP.node_cap,
                                     SP.next_cap, T.wave_tasks, T.wave_rows, T.graph_levels, wm, cap,
-                                    P.match, P.mismatch, P.open, P.ext, P.window, P.step, P.min_score, P.min_ref};
+                                    P.match, P.mismatch, P.open, P.ext, P.window, P.step, P.min_score, P.min_ref,
+                                    delta, (int64_t)SI.seg, (int64_t)SI.rv, (int64_t)SI.ng};
         // a layout seen for the first time runs host-driven; one that recurs
         // (a staged batch stepped again) is captured and replayed from then on
         if ((!SP.exec || SP.key != key) && SP.seen != key) {

@@ -38,6 +38,7 @@
 // min_score: k_resolve stops every rescan after the window's lastrow, and
 // resolves a right child that starts past it as empty without scanning it.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -78,6 +79,35 @@ __device__ __forceinline__ void mark_lastrow(int32_t *__restrict__ lastrow, uint
     if (hit >> 16) lastrow[base + 64u + ln] = row;
 }
 
+// Appends one strong row (window w, entry rv) to the calling lane's chunk of
+// the sink.  sk: the lane's two LDS words, its chunk (-1 before the first)
+// and fill.  Lanes append independently (no cross-lane operation, so the
+// row loops around the calls keep their unrolling).
+__device__ __forceinline__ void strong_push(const StrongSink &S, volatile int32_t *sk, uint32_t w, uint64_t rv) {
+    int32_t base = sk[0], fill = sk[1];
+    if (base < 0 || fill == kStrongChunk) {
+        base = atomicAdd(S.nchunks, 1);
+        fill = 0;
+        sk[0] = base;
+    }
+    if (base < S.cap_chunks) {
+        const int64_t at = (int64_t)base * kStrongChunk + fill;
+        S.win[at] = w;
+        S.rv[at] = rv;
+        S.chunk_fill[base] = fill + 1;
+    }
+    sk[1] = fill + 1;
+}
+
+// The lane's two words of a kernel's strong-row state (2 per lane).
+__device__ __forceinline__ volatile int32_t *lane_sk(int32_t *sk) {
+    return sk + 2 * (int32_t)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+__device__ __forceinline__ uint64_t strong_entry(int32_t row, int32_t v, int32_t col) {
+    return ((uint64_t)(uint32_t)row << 32) | (uint64_t)(uint32_t)((v << 8) | col);
+}
+
 // One long ScanTask across a wave: lane j owns window column j and works on
 // row a + s - j at step s (an anti-diagonal), so a cell's left and diagonal
 // neighbours were computed by lane j - 1 one and two steps earlier and reach
@@ -92,11 +122,12 @@ __device__ __forceinline__ void mark_lastrow(int32_t *__restrict__ lastrow, uint
 // steps instead of R x width lane steps.
 // L0: a level-0 window (rows [0, L1) of its pair): a window with a record
 // joins the hit list, and its cells are counted on the host instead.
-template <int WM, bool L0 = false>
+template <int WM, bool L0 = false, bool REC = false>
 __device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const ScanTask &t,
                                const Params &P, const ScoreTab &tab, TrajEntry *__restrict__ traj,
                                int32_t *__restrict__ tcount, int32_t cap, Counters *__restrict__ ctr,
-                               int32_t *__restrict__ hitlist = nullptr, int32_t *__restrict__ lastrow = nullptr) {
+                               int32_t *__restrict__ hitlist = nullptr, int32_t *__restrict__ lastrow = nullptr,
+                               const StrongSink *sink = nullptr, int32_t *sk = nullptr) {
     const int lane = threadIdx.x;
     const PairDesc pd = pairs[t.pair];
     const int8_t *__restrict__ ref = seq + pd.off1;
@@ -143,6 +174,7 @@ __device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *_
             }
             if (lane == last && pm >= P.min_score) {
                 if (L0) lastrow[t.scan] = t.a + row;  // rows rise: the last store is the last such row
+                if (L0 && REC) strong_push(*sink, lane_sk(sk), (uint32_t)t.scan, strong_entry(t.a + row, pm, pj));
                 if (pm > best) {
                     record(traj, t.scan, cap, cnt, t.a + row, pm, pj);
                     best = pm;
@@ -172,13 +204,14 @@ __device__ void scan_task_wave(const int8_t *__restrict__ seq, const PairDesc *_
 // cell.  The level-0 kernels run these windows in their first blocks (wins,
 // nwins; tab32: the int32 score table); the host picks the windows whose
 // lane-kernel rows would otherwise set the launch's duration.
-template <int WM>
+template <int WM, bool REC = false>
 __device__ __forceinline__ void l0_wave_window(const int2 w, const int8_t *__restrict__ seq,
                                                const PairDesc *__restrict__ pairs, const Params &P,
                                                const ScoreTab &tab32, TrajEntry *__restrict__ traj,
                                                int32_t *__restrict__ tcount, int32_t cap,
                                                int32_t *__restrict__ hitlist, Counters *__restrict__ ctr,
-                                               int32_t *__restrict__ lastrow) {
+                                               int32_t *__restrict__ lastrow, const StrongSink &sink,
+                                               int32_t *sk) {
     const PairDesc pd = pairs[w.x];
     ScanTask t;
     t.pair = w.x;
@@ -187,7 +220,7 @@ __device__ __forceinline__ void l0_wave_window(const int2 w, const int8_t *__res
     t.b = pd.len1;
     t.node = -1;
     t.scan = pd.win_base + w.y;
-    scan_task_wave<WM, true>(seq, pairs, t, P, tab32, traj, tcount, cap, ctr, hitlist, lastrow);
+    scan_task_wave<WM, true, REC>(seq, pairs, t, P, tab32, traj, tcount, cap, ctr, hitlist, lastrow, &sink, sk);
 }
 
 // The next level-0 task of a persistent wave: lane 0 pops the queue head,
@@ -307,8 +340,9 @@ __global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ 
                                                      int32_t nwins, int32_t *__restrict__ lastrow) {
     const int32_t lane_blocks = (int32_t)gridDim.x - nwins;
     if (WAVES && (int32_t)blockIdx.x >= lane_blocks) {
+        // this kernel's scorings never record strong rows
         l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr,
-                           lastrow);
+                           lastrow, StrongSink{nullptr, nullptr, nullptr, nullptr, 0}, nullptr);
         return;
     }
     if (!PERSIST) {
@@ -324,14 +358,15 @@ __global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ 
 // integer, and v_pk_maximum3_f16 merges the three-way maxima.  Non-negative
 // f16 values order like their bit patterns, so the improvement test and the
 // first-column search compare bits.
-template <int WM>
+template <int WM, bool REC>
 __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_t *__restrict__ seq,
                                                     const PairDesc *__restrict__ pairs, const Params &P,
                                                     const ScoreTab &tab, TrajEntry *__restrict__ traj,
                                                     int32_t *__restrict__ tcount, int32_t cap,
                                                     int32_t *__restrict__ hitlist, Counters *__restrict__ ctr,
                                                     uint32_t open_h, uint32_t ext_h, uint32_t thr_h,
-                                                    int32_t *__restrict__ lastrow) {
+                                                    int32_t *__restrict__ lastrow, const StrongSink &sink,
+                                                    int32_t *sk) {
     const int lane = threadIdx.x;
     const PairDesc pd = pairs[task.x];
     const int8_t *__restrict__ ref = seq + pd.off1;
@@ -391,23 +426,37 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
             mark_lastrow(lastrow, (uint32_t)(pd.win_base + task.y), hit, i);
             const uint32_t rb = bits(rmax);
             const uint32_t imp = pk_subsat(rb, best);
-            if (imp & 0xffffu) {
+            // first column of each half's row maximum, where a record needs
+            // it (a strong-row entry leaves it to the tail)
+            const uint32_t need = imp;
+            int jl = 0, jh = 0;
+            if (need & 0xffffu) {
                 const uint32_t v = rb & 0xffffu;
-                int jj = 0;
 #pragma unroll
                 for (int j = WM - 1; j >= 0; --j)
-                    if ((bits(H[j]) & 0xffffu) == v) jj = j;
-                record(traj, pd.win_base + wl, cap, cntl, i, (int32_t)(float)rmax[0], jj);
-                best = (best & 0xffff0000u) | v;
+                    if ((bits(H[j]) & 0xffffu) == v) jl = j;
+            }
+            if (need >> 16) {
+                const uint32_t v = rb >> 16;
+#pragma unroll
+                for (int j = WM - 1; j >= 0; --j)
+                    if ((bits(H[j]) >> 16) == v) jh = j;
+            }
+            const int32_t vl = (int32_t)(float)rmax[0], vh = (int32_t)(float)rmax[1];
+            if (imp & 0xffffu) {
+                record(traj, pd.win_base + wl, cap, cntl, i, vl, jl);
+                best = (best & 0xffff0000u) | (rb & 0xffffu);
             }
             if (imp >> 16) {
-                const uint32_t v = rb >> 16;
-                int jj = 0;
-#pragma unroll
-                for (int j = WM - 1; j >= 0; --j)
-                    if ((bits(H[j]) >> 16) == v) jj = j;
-                record(traj, pd.win_base + wh, cap, cnth, i, (int32_t)(float)rmax[1], jj);
-                best = (best & 0xffffu) | (v << 16);
+                record(traj, pd.win_base + wh, cap, cnth, i, vh, jh);
+                best = (best & 0xffffu) | (rb & 0xffff0000u);
+            }
+            if (REC) {
+                volatile int32_t *my = lane_sk(sk);
+                if (hit & 0xffffu)
+                    strong_push(sink, my, (uint32_t)(pd.win_base + wl), strong_entry(i, vl, kStrongNoCol));
+                if (hit >> 16)
+                    strong_push(sink, my, (uint32_t)(pd.win_base + wh), strong_entry(i, vh, kStrongNoCol));
             }
         }
     }
@@ -429,40 +478,50 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
 
 // One wave per task: task blockIdx.x of the length-sorted list (3 waves
 // per SIMD fit its 168 registers).
-template <int WM, bool WAVES>
+template <int WM, bool WAVES, bool REC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_uniform_h(
     const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const int2 *__restrict__ wtasks, int32_t nwt,
     int32_t *__restrict__ queue, Params P, ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
     int32_t cap, int32_t *__restrict__ hitlist, Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
-    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins, int32_t *__restrict__ lastrow) {
+    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins, int32_t *__restrict__ lastrow, StrongSink sink) {
+    __shared__ int32_t sk[REC ? 128 : 1];  // each lane's strong-row chunk and fill
+    if (REC) {
+        sk[2 * threadIdx.x] = -1;
+        sk[2 * threadIdx.x + 1] = 0;
+    }
     const int32_t lane_blocks = (int32_t)gridDim.x - nwins;
     if (WAVES && (int32_t)blockIdx.x >= lane_blocks) {
-        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist, ctr,
-                           lastrow);
+        l0_wave_window<WM, REC>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist,
+                                ctr, lastrow, sink, sk);
         return;
     }
-    scan_uniform_task_h<WM>(wtasks[blockIdx.x], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h,
-                            thr_h, lastrow);
+    scan_uniform_task_h<WM, REC>(wtasks[blockIdx.x], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h,
+                                 ext_h, thr_h, lastrow, sink, sk);
 }
 
 // Persistent waves: each pops tasks from the queue head in list order until
 // the list is exhausted.  The loop's own state does not fit beside the task's
 // registers at 3 waves per SIMD without spilling, so this one runs 2.
-template <int WM, bool WAVES>
+template <int WM, bool WAVES, bool REC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_scan_uniform_hp(
     const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, const int2 *__restrict__ wtasks, int32_t nwt,
     int32_t *__restrict__ queue, Params P, ScoreTab tab, TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
     int32_t cap, int32_t *__restrict__ hitlist, Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h,
-    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins, int32_t *__restrict__ lastrow) {
+    uint32_t thr_h, ScoreTab tab32, const int2 *__restrict__ wins, int32_t nwins, int32_t *__restrict__ lastrow, StrongSink sink) {
+    __shared__ int32_t sk[REC ? 128 : 1];  // each lane's strong-row chunk and fill
+    if (REC) {
+        sk[2 * threadIdx.x] = -1;
+        sk[2 * threadIdx.x + 1] = 0;
+    }
     const int32_t lane_blocks = (int32_t)gridDim.x - nwins;
     if (WAVES && (int32_t)blockIdx.x >= lane_blocks) {
-        l0_wave_window<WM>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist, ctr,
-                           lastrow);
+        l0_wave_window<WM, REC>(wins[blockIdx.x - lane_blocks], seq, pairs, P, tab32, traj, tcount, cap, hitlist,
+                                ctr, lastrow, sink, sk);
         return;
     }
     for (int32_t t = pop_task(queue); t < nwt; t = pop_task(queue))
-        scan_uniform_task_h<WM>(wtasks[t], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h, thr_h,
-                                lastrow);
+        scan_uniform_task_h<WM, REC>(wtasks[t], seq, pairs, P, tab, traj, tcount, cap, hitlist, ctr, open_h, ext_h,
+                                     thr_h, lastrow, sink, sk);
 }
 
 // One lane per ScanTask: rows [a, b) of seq1 against one window, int32.
@@ -726,7 +785,7 @@ __global__ __launch_bounds__(64) void k_resolve(const int8_t *__restrict__ seq, 
                                                 const TrajEntry *__restrict__ traj, const int32_t *__restrict__ tcount,
                                                 int32_t cap, int32_t scan_cap, ScanTask *__restrict__ next,
                                                 int32_t next_cap, Counters *__restrict__ ctr,
-                                                const int32_t *__restrict__ lastrow) {
+                                                const int32_t *__restrict__ lastrow, int32_t tail_rows) {
     __shared__ uint32_t shi[8], slo[8];
     if (threadIdx.x < 8) {
         shi[threadIdx.x] = tab.hi[threadIdx.x];
@@ -811,6 +870,9 @@ __global__ __launch_bounds__(64) void k_resolve(const int8_t *__restrict__ seq, 
                 st.win = win;
                 st.a = e1 + 1;
                 st.b = min(nd.b, scan_end);
+                // with level-0 tails the scan runs the rows whose values can
+                // differ from level 0's; k_rescan_tail adds the rest
+                if (tail_rows > 0) st.b = min(st.b, st.a + tail_rows);
                 st.node = r;
                 st.scan = sc;
                 next[q] = st;
@@ -929,6 +991,11 @@ __global__ __launch_bounds__(256) void k_order_scatter(const ScanTask *__restric
     }
 }
 
+__global__ void k_widen_i32(const int32_t *__restrict__ a, int32_t n, int64_t *__restrict__ b) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -941,6 +1008,160 @@ __global__ void k_gather_roots(const Node *__restrict__ nodes, const int32_t *__
         out[gid] = nodes[hitlist[gid]];
 }
 
+// Strong-row index.  Chunk c's entries move to chunk_off[c] onward (one
+// workgroup per chunk).
+__global__ __launch_bounds__(64) void k_strong_compact(const uint32_t *__restrict__ win,
+                                                        const uint64_t *__restrict__ rv,
+                                                        const int32_t *__restrict__ fill,
+                                                        const int64_t *__restrict__ off, uint32_t *__restrict__ win2,
+                                                        uint64_t *__restrict__ rv2) {
+    const int64_t c = blockIdx.x;
+    const int32_t n = fill[c];
+    for (int32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        win2[off[c] + i] = win[c * kStrongChunk + i];
+        rv2[off[c] + i] = rv[c * kStrongChunk + i];
+    }
+}
+
+// Total entries: the last chunk's offset plus its fill.
+__global__ void k_strong_total(const int32_t *__restrict__ fill, const int64_t *__restrict__ off, int32_t nchunks,
+                               int64_t *__restrict__ total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = nchunks > 0 ? off[nchunks - 1] + fill[nchunks - 1] : 0;
+}
+
+// seg[w] = [first, last + 1) of window w's entries in the sorted list (seg
+// cleared to empty beforehand).
+__global__ void k_strong_seg(const uint32_t *__restrict__ win, int64_t n, int2 *__restrict__ seg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t w = win[i];
+    if (i == 0 || win[i - 1] != w) seg[w].x = (int32_t)i;
+    if (i == n - 1 || win[i + 1] != w) seg[w].y = (int32_t)(i + 1);
+}
+
+__device__ __forceinline__ int32_t strong_val(uint64_t e) { return (int32_t)((uint32_t)e >> 8); }
+__device__ __forceinline__ int32_t strong_row(uint64_t e) { return (int32_t)(e >> 32); }
+
+// Next strictly larger maximum of each entry within its window, one lane per
+// window from the back: the candidates after i are i + 1 and then the
+// entries reachable through ng, which skip only values that are not larger.
+__global__ void k_strong_ng(const int2 *__restrict__ seg, int32_t nw, const uint64_t *__restrict__ rv,
+                            int32_t *__restrict__ ng) {
+    const int32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const int2 g = seg[w];
+    for (int32_t i = g.y - 1; i >= g.x; --i) {
+        const int32_t v = strong_val(rv[i]);
+        int32_t j = i + 1;
+        while (j < g.y && strong_val(rv[j]) <= v) j = ng[j];
+        ng[i] = j;
+    }
+}
+
+// First column of level 0's maximum v on row r of window (pd, win): the
+// window's DP restarted from zeros at row max(0, r - delta) has level 0's
+// values on row r (the tail bound), so r - r0 + 1 rows of it give the row.
+// The same recurrence as every scan kernel, int32, the window's tw real
+// columns (a tail entry's maximum is never only in the N padding: a padded
+// cell is bounded by an earlier real cell of the same rescan, which the
+// running maximum already holds).
+template <int WM>
+__device__ int32_t l0_row_col(const int8_t *__restrict__ s1, const int8_t *__restrict__ win, int32_t tw, int32_t r,
+                              int32_t delta, const Params &P, int32_t v) {
+    int32_t H[WM], E[WM], q[WM];
+#pragma unroll
+    for (int j = 0; j < WM; ++j) {
+        const int32_t c = j < tw ? (int32_t)win[j] : 2;
+        q[j] = (c >= 0 && c <= 4 && c != 2) ? c : -1;
+        H[j] = 0;
+        E[j] = 0;
+    }
+    for (int32_t i = max(0, r - delta); i <= r; ++i) {
+        const int32_t c0 = s1[i];
+        const int32_t c = (c0 >= 0 && c0 <= 4 && c0 != 2) ? c0 : -2;
+        int32_t diag = 0, f = 0;
+#pragma unroll
+        for (int j = 0; j < WM; ++j) {
+            const int32_t sc = (q[j] < 0 || c < 0) ? 0 : (q[j] == c ? P.match : P.mismatch);
+            const int32_t h = max(max(diag + sc, E[j]), max(f, 0));
+            const int32_t ho = h + P.open;
+            E[j] = max(max(E[j] + P.ext, ho), 0);
+            f = max(max(ho, f + P.ext), 0);
+            diag = H[j];
+            H[j] = h;
+        }
+    }
+    int32_t col = kStrongNoCol;
+#pragma unroll
+    for (int j = WM - 1; j >= 0; --j)
+        if (j < tw && H[j] == v) col = j;
+    return col;
+}
+
+// A rescan task's trajectory continued with level 0's strong rows: rows at or
+// past a + delta, before the rescan's end (its node's b, cut at the window's
+// lastrow + 1 as the scan was), that exceed the running maximum; the first
+// such entry is found by following ng from the first entry on or past
+// a + delta, and so is each next one.  Every step raises the maximum, so a
+// task takes at most about 2 * cap steps.  An entry without its column gets
+// it from l0_row_col.
+template <int WM>
+__global__ __launch_bounds__(64) void k_rescan_tail(const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs,
+                                                    Params P, const ScanTask *__restrict__ tasks,
+                                                    const int32_t *__restrict__ n_p, const Node *__restrict__ nodes,
+                                                    const int32_t *__restrict__ lastrow, StrongIndex I,
+                                                    TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
+                                                    int32_t cap, Counters *__restrict__ ctr) {
+    const int n = *n_p;
+    for (int gid = blockIdx.x * blockDim.x + threadIdx.x; gid < n; gid += gridDim.x * blockDim.x) {
+        const ScanTask t = tasks[gid];
+        const PairDesc pd = pairs[t.pair];
+        const int32_t gw = pd.win_base + t.win;
+        const int32_t end = min(nodes[t.node].b, lastrow[gw] + 1);
+        const int32_t from = t.a + I.delta;
+        if (end <= from) continue;
+        const int2 g = I.seg[gw];
+        // first entry with row >= from
+        int32_t lo = g.x, hi = g.y;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (strong_row(I.rv[mid]) < from) lo = mid + 1;
+            else hi = mid;
+        }
+        int32_t cnt = tcount[t.scan];
+        TrajEntry *tr = traj + (int64_t)t.scan * cap;
+        int32_t best = cnt > 0 ? (tr[min(cnt, cap) - 1].score_col >> 8) : P.min_score - 1;
+        const int32_t c0 = cnt;
+        const int sp = t.win * P.step;
+        const int tw = min(P.window, pd.len2 - sp);
+        int32_t p = lo;
+        while (p < g.y) {
+            const uint64_t e = I.rv[p];
+            const int32_t row = strong_row(e);
+            if (row >= end) break;
+            const int32_t v = strong_val(e);
+            if (v > best) {
+                if (cnt < cap) {
+                    int32_t col = (int32_t)(e & 0xffu);
+                    if (col == kStrongNoCol)
+                        col = l0_row_col<WM>(seq + pd.off1, seq + pd.off2 + sp, tw, row, I.delta, P, v);
+                    TrajEntry x;
+                    x.row = row;
+                    x.score_col = (v << 8) | col;
+                    tr[cnt] = x;
+                }
+                ++cnt;
+                best = v;
+            }
+            p = I.ng[p];
+        }
+        if (cnt != c0) {
+            tcount[t.scan] = cnt;
+            atomicAdd(&ctr->entries, min(cnt, cap) - min(c0, cap));
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -949,7 +1170,7 @@ __global__ void k_gather_roots(const Node *__restrict__ nodes, const int32_t *__
 hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const int2 *wtasks,
                                int32_t nwt, int32_t persistent_waves, const Params &P, const ScoreTab &tab,
                                TrajEntry *traj, int32_t *tcount, int32_t cap, int32_t *hitlist, Counters *ctr,
-                               const int2 *wins, int32_t nwins, int32_t *lastrow) {
+                               const int2 *wins, int32_t nwins, int32_t *lastrow, const StrongSink &sink) {
     if (nwt <= 0 && nwins <= 0) return hipSuccess;
     // lane tasks first (persistent waves popping the task list from the
     // counter block's queue head, zeroed by k_seed_init, or one workgroup per
@@ -964,15 +1185,20 @@ hipError_t launch_scan_uniform(hipStream_t st, int wm, const int8_t *seq, const 
     if (scan_uses_f16(P)) {
         const ScoreTab th = make_score_tab_f16(P.match, P.mismatch);
         const uint32_t oh = f16_bits_of_int(P.open), xh = f16_bits_of_int(P.ext), thr = f16_bits_of_int(P.min_score - 1);
-        decltype(&k_scan_uniform_h<38, false>) k;
+        // REC: the strong rows go to the sink (level-0 tails on); its own
+        // instantiation, so that the others carry no trace of it
+        const bool rec = sink.cap_chunks > 0;
+        decltype(&k_scan_uniform_h<38, false, false>) k;
+#define DCNS_L0_PICK(W, R)                                                                                          \
+    (persist ? (waves ? k_scan_uniform_hp<W, true, R> : k_scan_uniform_hp<W, false, R>)                             \
+             : (waves ? k_scan_uniform_h<W, true, R> : k_scan_uniform_h<W, false, R>))
         if (wm <= 38)
-            k = persist ? (waves ? k_scan_uniform_hp<38, true> : k_scan_uniform_hp<38, false>)
-                        : (waves ? k_scan_uniform_h<38, true> : k_scan_uniform_h<38, false>);
+            k = rec ? DCNS_L0_PICK(38, true) : DCNS_L0_PICK(38, false);
         else
-            k = persist ? (waves ? k_scan_uniform_hp<64, true> : k_scan_uniform_hp<64, false>)
-                        : (waves ? k_scan_uniform_h<64, true> : k_scan_uniform_h<64, false>);
+            k = rec ? DCNS_L0_PICK(64, true) : DCNS_L0_PICK(64, false);
+#undef DCNS_L0_PICK
         hipLaunchKernelGGL(k, g, b, 0, st, seq, pairs, wtasks, nwt, queue, P, th, traj, tcount, cap, hitlist, ctr, oh,
-                           xh, thr, tab, wins, nwins, lastrow);
+                           xh, thr, tab, wins, nwins, lastrow, sink);
     } else {
         decltype(&k_scan_uniform<38, false, false>) k;
         if (wm <= 38)
@@ -1024,17 +1250,17 @@ hipError_t launch_resolve(hipStream_t st, int wm, const int8_t *seq, const PairD
                           const int32_t *hitlist, const ScanTask *tasks, const int32_t *n_dev, int32_t n_max,
                           int32_t blocks_max, Node *nodes, int32_t node_cap, const TrajEntry *traj,
                           const int32_t *tcount, int32_t cap, int32_t scan_cap, ScanTask *next, int32_t next_cap,
-                          Counters *ctr, const int32_t *lastrow) {
+                          Counters *ctr, const int32_t *lastrow, int32_t tail_rows) {
     if (n_max <= 0) return hipSuccess;
     int32_t blocks = (n_max + 63) / 64;
     if (blocks_max > 0) blocks = std::min(blocks, blocks_max);
     dim3 g(blocks), b(64);
     if (wm <= 38)
         hipLaunchKernelGGL(k_resolve<38>, g, b, 0, st, seq, pairs, pair_of_window, P, tab, level0, hitlist, tasks, n_dev,
-                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr, lastrow);
+                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr, lastrow, tail_rows);
     else
         hipLaunchKernelGGL(k_resolve<64>, g, b, 0, st, seq, pairs, pair_of_window, P, tab, level0, hitlist, tasks, n_dev,
-                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr, lastrow);
+                           nodes, node_cap, traj, tcount, cap, scan_cap, next, next_cap, ctr, lastrow, tail_rows);
     return hipGetLastError();
 }
 
@@ -1065,6 +1291,71 @@ hipError_t launch_order_tasks(hipStream_t st, const ScanTask *tasks, const int32
 hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+size_t strong_index_scratch(int64_t nchunks, int64_t n) {
+    size_t a = 0, b = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, a, (const int64_t *)nullptr, (int64_t *)nullptr,
+                                     (int)std::max<int64_t>(nchunks, 1));
+    hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                       (const uint64_t *)nullptr, (uint64_t *)nullptr, (int)std::max<int64_t>(n, 1));
+    return std::max(a, b) + 256;
+}
+
+hipError_t launch_strong_count(hipStream_t st, const StrongSink &S, int32_t nchunks, void *scratch,
+                               size_t scratch_bytes, int64_t *chunk_off, int64_t *total_dev) {
+    if (nchunks <= 0) {
+        hipLaunchKernelGGL(k_strong_total, dim3(1), dim3(64), 0, st, S.chunk_fill, chunk_off, 0, total_dev);
+        return hipGetLastError();
+    }
+    // chunk fills widen to int64 in chunk_off, then scan in place
+    hipLaunchKernelGGL(k_widen_i32, dim3((unsigned)((nchunks + 255) / 256)), dim3(256), 0, st, S.chunk_fill, nchunks,
+                       chunk_off);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t bytes = scratch_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(scratch, bytes, chunk_off, chunk_off, nchunks, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_strong_total, dim3(1), dim3(64), 0, st, S.chunk_fill, chunk_off, nchunks, total_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_strong_index(hipStream_t st, const StrongSink &S, int32_t nchunks, int64_t n, int32_t nw,
+                               int32_t win_bits, void *scratch, size_t scratch_bytes, uint32_t *win2, uint64_t *rv2,
+                               int64_t *chunk_off, int2 *seg, int32_t *ng) {
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((2 * (int64_t)nw + 255) / 256)), dim3(256), 0, st, (int32_t *)seg,
+                       2 * (int64_t)nw, 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n <= 0) return e;
+    // the chunks' entries, compacted into (win2, rv2), sort back into the
+    // sink's own arrays, which then hold the index's entries
+    hipLaunchKernelGGL(k_strong_compact, dim3((unsigned)nchunks), dim3(64), 0, st, S.win, S.rv, S.chunk_fill,
+                       chunk_off, win2, rv2);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    size_t bytes = scratch_bytes;
+    // sort by window, stable, so each window's entries keep their row order
+    e = hipcub::DeviceRadixSort::SortPairs(scratch, bytes, win2, S.win, rv2, S.rv, (int)n, 0, win_bits, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_strong_seg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, S.win, n, seg);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_strong_ng, dim3((unsigned)((nw + 63) / 64)), dim3(64), 0, st, seg, nw, S.rv, ng);
+    return hipGetLastError();
+}
+
+hipError_t launch_rescan_tail(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const Params &P,
+                              const ScanTask *tasks, const int32_t *n_dev, int32_t n_max, int32_t blocks_max,
+                              const Node *nodes, const int32_t *lastrow, const StrongIndex &I, TrajEntry *traj,
+                              int32_t *tcount, int32_t cap, Counters *ctr) {
+    if (n_max <= 0) return hipSuccess;
+    int32_t blocks = (n_max + 63) / 64;
+    if (blocks_max > 0) blocks = std::min(blocks, blocks_max);
+    if (wm <= 38)
+        hipLaunchKernelGGL(k_rescan_tail<38>, dim3(blocks), dim3(64), 0, st, seq, pairs, P, tasks, n_dev, nodes, lastrow,
+                           I, traj, tcount, cap, ctr);
+    else
+        hipLaunchKernelGGL(k_rescan_tail<64>, dim3(blocks), dim3(64), 0, st, seq, pairs, P, tasks, n_dev, nodes, lastrow,
+                           I, traj, tcount, cap, ctr);
     return hipGetLastError();
 }
 

@@ -347,9 +347,10 @@ __device__ __forceinline__ void columns_w(const int32_t (&qc)[K], int32_t (&F1)[
 // (strip k's predecessor is entry k - 1 when it is not the task's first).
 // Strips start in launch order: each wave takes the next position from a
 // launch-wide counter in the order the waves start running and runs strip
-// order[position].  The host puts every strip after its predecessor in that
-// order, so a strip's predecessor has always started: every wait ends,
-// whatever number of waves the device holds at once.  sync[0] = the counter,
+// order[position] (bits 0-29; bit 30 raises the wave's issue priority).  The
+// host puts every strip after its predecessor in that order, so a strip's
+// predecessor has always started: every wait ends, whatever number of waves
+// the device holds at once.  sync[0] = the counter,
 // sync[1] = the error word, sync[2 + k] = rows of strip k whose boundary
 // records are complete.  MODE 0: SemiGlobal_no_avx; 1: weighted two-piece;
 // 2: weighted single gap (profiles in W).
@@ -372,7 +373,13 @@ __global__ __launch_bounds__(64) void k_sg_strip(const int8_t *__restrict__ seq,
     if (lane == 0) k = atomicAdd(sync, 1);
     k = __builtin_amdgcn_readfirstlane(k);
     if (k >= total) return;
-    k = order[k];
+    {
+        // bit 30: a strip on or near the launch's longest chain of strips
+        // issues ahead of the other waves of its SIMD
+        const int32_t ok = order[k];
+        k = ok & 0x3fffffff;
+        if (ok & 0x40000000) __builtin_amdgcn_s_setprio(3);
+    }
     int32_t *err = sync + 1, *prog = sync + 2;
     const SgStrip S = strips[k];
     const SgTask T = tasks[S.task];

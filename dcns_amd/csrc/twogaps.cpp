@@ -146,6 +146,13 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
         for (int32_t i = 0; i < ns; ++i) order[(size_t)i] = i;
         std::stable_sort(order.begin(), order.end(),
                          [&](int32_t a, int32_t b) { return chain[(size_t)a] > chain[(size_t)b]; });
+        // The strips whose chain is within a quarter of the longest set the
+        // launch's end: they issue ahead of the waves that share their SIMD.
+        if (ns > 0) {
+            const int64_t crit = chain[(size_t)order[0]] - chain[(size_t)order[0]] / 4;
+            for (int32_t &o : order)
+                if (chain[(size_t)o] >= crit) o |= 0x40000000;
+        }
         c->sg_strips.ensure(std::max<size_t>(strips.size(), 1) * sizeof(SgStrip));
         hip_check(hipMemcpyAsync(c->sg_strips.p, strips.data(), strips.size() * sizeof(SgStrip), hipMemcpyHostToDevice,
                                  c->stream), "sg strips");
@@ -192,8 +199,11 @@ void semiglobal_batch(dcns_ctx *c, const Params &P, const std::vector<SgTask> &t
         if (stalled) throw Error(DCNS_ERR_INTERNAL, "SemiGlobal strip pipeline: a strip waited too long for its predecessor");
         band_done += band_chunk;
         if (progress())
-            std::fprintf(stderr, "  chunk of %d tasks: %.3e band cells in %.1f ms (%.3e of %.3e done)\n", m,
-                         (double)band_chunk, ms_now() - tc0, (double)band_done, (double)band_total);
+            std::fprintf(stderr,
+                         "  chunk of %d tasks, %d strips (longest chain %lld steps): %.3e band cells in %.1f ms "
+                         "(%.3e of %.3e done)\n",
+                         m, ns, (long long)(ns ? chain[(size_t)(order[0] & 0x3fffffff)] : 0), (double)band_chunk,
+                         ms_now() - tc0, (double)band_done, (double)band_total);
         for (int32_t k = 0; k < m; ++k) {
             const SgOut &o = so[(size_t)k];
             SgResult &r = out[(size_t)which[(size_t)k]];

@@ -131,7 +131,16 @@ int dcns_last_stats(const dcns_ctx *ctx, dcns_stats *out);
  *                 reaches 2048 scored again in int32), else one per lane in
  *                 int32; 1: every pair one per lane, int32; 2: every pair on
  *                 the lane kernels as in 0; 3: every pair whose query fits
- *                 64 stripes (2048 bases) one per wave. */
+ *                 64 stripes (2048 bases) one per wave.
+ *   level0_tail   cut1Gap seed recursion: 0 (default): on when a pair's
+ *                 reference reaches 65,536 bases; 1: on; -1: off.  When on,
+ *                 level 0 also records every row where a window's maximum
+ *                 reaches min_score, and a rescan of rows [a, b) runs only
+ *                 rows [a, a + D) (D = window + ceil(window * match / g),
+ *                 g = -max(open, ext)); from row a + D on its values are
+ *                 level 0's, whose recorded rows complete its trajectory.
+ *                 Off when g <= 0 or the level-0 scan is not packed f16, and
+ *                 for a batch whose recorded rows overflow their buffers. */
 typedef struct dcns_tuning {
     int32_t level0_mode, level0_waves;
     int32_t wave_tasks, wave_rows;
@@ -139,6 +148,7 @@ typedef struct dcns_tuning {
     int32_t pool_min;
     int32_t xdrop_lane_rows;
     int32_t k1_lanes;
+    int32_t level0_tail;
 } dcns_tuning;
 int dcns_get_tuning(const dcns_ctx *ctx, dcns_tuning *out);
 int dcns_set_tuning(dcns_ctx *ctx, const dcns_tuning *in);
