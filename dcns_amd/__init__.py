@@ -47,7 +47,7 @@ class Tuning(ctypes.Structure):
     """dcns_tuning: launch shapes of the seed phase and the x-drop lane kernels (0 = default)."""
     _fields_ = [("level0_mode", _I), ("level0_waves", _I), ("wave_tasks", _I), ("wave_rows", _I),
                 ("graph_levels", _I), ("pool_min", _I), ("xdrop_lane_rows", _I), ("k1_lanes", _I),
-                ("level0_tail", _I)]
+                ("level0_tail", _I), ("level0_split", _I)]
 
 
 _lib = None

@@ -159,6 +159,7 @@ int dcns_set_tuning(dcns_ctx *c, const dcns_tuning *in) {
         if (t.xdrop_lane_rows < -1) throw Error(DCNS_ERR_ARG, "xdrop_lane_rows must be >= -1");
         if (t.k1_lanes < 0 || t.k1_lanes > 3) throw Error(DCNS_ERR_ARG, "k1_lanes must be in [0, 3]");
         if (t.level0_tail < -1 || t.level0_tail > 1) throw Error(DCNS_ERR_ARG, "level0_tail must be in [-1, 1]");
+        if (t.level0_split < -1 || t.level0_split > 1) throw Error(DCNS_ERR_ARG, "level0_split must be in [-1, 1]");
         c->tune = t;
     });
 }

@@ -140,8 +140,9 @@ struct dcns_ctx {
     dcns::DBuf seq, pairs, wtasks, l0wins, pow, traj, tcount, hitlist, ctr, nodes, roots, tasks_a, tasks_b;
     dcns::DBuf lastrow;  // per window: the last level-0 row whose maximum reaches min_score
     // level-0 strong rows (sink chunks, then the sorted index) for the rescans' level-0 tails
-    dcns::DBuf strong_win, strong_rv, strong_fill, strong_win2, strong_rv2, strong_off, strong_seg, strong_ng,
-        strong_tmp, strong_total;
+    dcns::DBuf strong_win, strong_rv, strong_fill, strong_rv2, strong_off, strong_seg, strong_ng,
+        strong_tmp, strong_total, strong_key, strong_key2, strong_fill64;
+    dcns::DBuf seg_tasks, seg_groups, seg_traj, seg_tcount;  // level 0 in row segments
     dcns::DBuf xtasks, xout, tb_off, tb, cigar, cig_host_dummy;
     dcns::DBuf sw_off1, sw_len1, sw_off2, sw_len2, sw_order, sw_out, sw_bound, sw_woff;
     dcns::DBuf bas_cig, bas_st;  // SmithWaterman_bas walks: cigar entries, counts and status

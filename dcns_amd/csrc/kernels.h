@@ -61,6 +61,24 @@ struct StrongSink {
     int32_t *nchunks;
     int32_t cap_chunks;
 };
+// Level 0 in row segments (a batch whose longest references would hold the
+// launch): task (pair, first window, r0, r1) scans rows [r0, r1) of its 128
+// windows after `warm` rows from zeros (level0_tail_rows: rows from r0 on are
+// then level 0's values), into 128 trajectory slots of its own; a group
+// (pair, first window, first task, segments) lists its consecutive tasks in
+// row order, and k_l0_merge turns them into the windows' trajectories.
+struct SegL0 {
+    const int4 *tasks = nullptr;
+    const int4 *groups = nullptr;
+    TrajEntry *traj = nullptr;
+    int32_t *tcount = nullptr;
+    int32_t warm = 0;
+};
+hipError_t launch_scan_uniform_seg(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const SegL0 &sg,
+                                   int32_t ntasks, const Params &P, int32_t cap, Counters *ctr, int32_t *lastrow,
+                                   const StrongSink &sink);
+hipError_t launch_l0_merge(hipStream_t st, const PairDesc *pairs, const SegL0 &sg, int32_t ngroups, TrajEntry *traj,
+                           int32_t *tcount, int32_t cap, int32_t min_score, int32_t *hitlist, Counters *ctr);
 // The strong rows sorted by window (row order within a window): seg[w] =
 // [begin, end) of global window w, ng[i] = the next index of the same window
 // whose maximum is strictly larger (end of the window's range when none).
@@ -71,16 +89,16 @@ struct StrongIndex {
     int32_t delta;  // level0_tail_rows
 };
 // Builds the index from the sink's first nchunks chunks: scratch_bytes of
-// scratch (sized by strong_index_scratch) at scratch; win2/rv2 hold n
-// entries (the total the host read back), seg nw windows.
+// scratch (sized by strong_index_scratch) at scratch; key2, rv2 and key hold
+// n entries (the total the host read back), seg nw windows.
 size_t strong_index_scratch(int64_t nchunks, int64_t n);
 hipError_t launch_strong_index(hipStream_t st, const StrongSink &S, int32_t nchunks, int64_t n, int32_t nw,
-                               int32_t win_bits, void *scratch, size_t scratch_bytes, uint32_t *win2, uint64_t *rv2,
-                               int64_t *chunk_off, int2 *seg, int32_t *ng);
-// Entries of the chunks: the exclusive scan of chunk_fill into chunk_off and
-// the total in *total_dev (one int64).
+                               int32_t win_bits, void *scratch, size_t scratch_bytes, uint64_t *key2, uint64_t *rv2,
+                               uint64_t *key, int64_t *chunk_off, int2 *seg, int32_t *ng);
+// Entries of the chunks: the exclusive scan of chunk_fill (widened into
+// fill64) into chunk_off and the total in *total_dev (one int64).
 hipError_t launch_strong_count(hipStream_t st, const StrongSink &S, int32_t nchunks, void *scratch,
-                               size_t scratch_bytes, int64_t *chunk_off, int64_t *total_dev);
+                               size_t scratch_bytes, int64_t *fill64, int64_t *chunk_off, int64_t *total_dev);
 // After a level's rescans (of at most index.delta rows each): every task
 // whose rows reach past a + delta appends level 0's strong rows of
 // [a + delta, min(node b, lastrow + 1)) that raise its running maximum.

@@ -367,12 +367,78 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
     };
 
     c->launch("seed_init", [&] { return launch_seed_init(c->stream, ctr, NW); });
-    c->launch("scan_uniform", [&] {
-        return launch_scan_uniform(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), c->wtasks.as<int2>(),
-                                   (int32_t)S.wt.size(), T.level0_waves, P, tab, c->traj.as<TrajEntry>(),
-                                   c->tcount.as<int32_t>(), cap, c->hitlist.as<int32_t>(), ctr,
-                                   c->l0wins.as<int2>(), (int32_t)S.ww.size(), c->lastrow.as<int32_t>(), sink);
-    });
+    // Level 0 in row segments when the batch's longest references would hold
+    // the launch (its lane rows are serial): each 128-window group's rows in
+    // segments of about seg rows, each started warm rows early from zeros,
+    // as many waves as segments; the segments' trajectories then merge.
+    SegL0 sg;
+    std::vector<int4> seg_tasks, seg_groups;
+    {
+        const int32_t warm = level0_tail_rows(P, wm);
+        const int32_t want = c->tune.level0_split;
+        if (warm > 0 && scan_uses_f16(P) && want >= 0) {
+            double total = 0;
+            int32_t longest = 0;
+            for (const PairDesc &d : S.pd) {
+                total += (double)((d.nwin + 127) / 128) * d.len1;
+                longest = std::max(longest, d.len1);
+            }
+            const double thr_time = total * kLaneRowIssue / (4.0 * c->cu_count());
+            int32_t seg = 0;
+            if (want == 1)
+                seg = 4 * warm;
+            else if (longest * kLaneRowLatency > 1.5 * thr_time)
+                seg = std::max({8 * warm, 1024, (int32_t)(thr_time / (2.0 * kLaneRowLatency))});
+            if (seg > 0) {
+                std::vector<int32_t> byl((size_t)S.pd.size());
+                for (size_t p = 0; p < byl.size(); ++p) byl[p] = (int32_t)p;
+                std::stable_sort(byl.begin(), byl.end(),
+                                 [&](int32_t a, int32_t b) { return S.pd[(size_t)a].len1 > S.pd[(size_t)b].len1; });
+                for (int32_t p : byl) {
+                    const PairDesc &d = S.pd[(size_t)p];
+                    if (d.len1 <= 0) continue;
+                    const int32_t nseg = d.len1 > seg + seg / 2 ? (d.len1 + seg - 1) / seg : 1;
+                    for (int32_t g = 0; g < d.nwin; g += 128) {
+                        seg_groups.push_back(make_int4(p, g, (int32_t)seg_tasks.size(), nseg));
+                        for (int32_t k = 0; k < nseg; ++k) {
+                            const int32_t r0 = (int32_t)((int64_t)d.len1 * k / nseg);
+                            const int32_t r1 = (int32_t)((int64_t)d.len1 * (k + 1) / nseg);
+                            seg_tasks.push_back(make_int4(p, g, r0, r1));
+                        }
+                    }
+                }
+                upload(c, c->seg_tasks, seg_tasks);
+                upload(c, c->seg_groups, seg_groups);
+                c->seg_traj.ensure(std::max<size_t>(seg_tasks.size(), 1) * 128 * (size_t)cap * sizeof(TrajEntry));
+                c->seg_tcount.ensure(std::max<size_t>(seg_tasks.size(), 1) * 128 * sizeof(int32_t));
+                sg.tasks = c->seg_tasks.as<int4>();
+                sg.groups = c->seg_groups.as<int4>();
+                sg.traj = c->seg_traj.as<TrajEntry>();
+                sg.tcount = c->seg_tcount.as<int32_t>();
+                sg.warm = warm;
+            }
+        }
+    }
+    if (!seg_tasks.empty()) {
+        hip_check(launch_fill_i32(c->stream, c->lastrow.as<int32_t>(), NW, -1), "lastrow");
+        c->launch("scan_uniform", [&] {
+            return launch_scan_uniform_seg(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(), sg,
+                                           (int32_t)seg_tasks.size(), P, cap, ctr, c->lastrow.as<int32_t>(), sink);
+        });
+        c->launch("l0_merge", [&] {
+            return launch_l0_merge(c->stream, c->pairs.as<PairDesc>(), sg, (int32_t)seg_groups.size(),
+                                   c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap, P.min_score,
+                                   c->hitlist.as<int32_t>(), ctr);
+        });
+    } else {
+        c->launch("scan_uniform", [&] {
+            return launch_scan_uniform(c->stream, wm, c->seq.as<int8_t>(), c->pairs.as<PairDesc>(),
+                                       c->wtasks.as<int2>(), (int32_t)S.wt.size(), T.level0_waves, P, tab,
+                                       c->traj.as<TrajEntry>(), c->tcount.as<int32_t>(), cap,
+                                       c->hitlist.as<int32_t>(), ctr, c->l0wins.as<int2>(), (int32_t)S.ww.size(),
+                                       c->lastrow.as<int32_t>(), sink);
+        });
+    }
     // the strong rows' index: chunk fills -> offsets and total, compaction,
     // a stable sort by window, per-window ranges and next-larger links
     if (delta > 0) {
@@ -381,16 +447,19 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
             delta = 0;  // entries were dropped: full rescans for this batch
         } else {
             c->strong_off.ensure((size_t)std::max(nchunks, 1) * sizeof(int64_t));
+            c->strong_fill64.ensure((size_t)std::max(nchunks, 1) * sizeof(int64_t));
             c->strong_total.ensure(64);
             const size_t tmp0 = strong_index_scratch(nchunks, 1);
             c->strong_tmp.ensure(tmp0);
             hip_check(launch_strong_count(c->stream, sink, nchunks, c->strong_tmp.p, c->strong_tmp.cap,
-                                          c->strong_off.as<int64_t>(), c->strong_total.as<int64_t>()),
+                                          c->strong_fill64.as<int64_t>(), c->strong_off.as<int64_t>(),
+                                          c->strong_total.as<int64_t>()),
                       "strong_count");
             int64_t n = 0;
             hip_check(hipMemcpyAsync(&n, c->strong_total.p, sizeof(n), hipMemcpyDeviceToHost, c->stream), "strong total");
             c->sync();
-            c->strong_win2.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(uint32_t));
+            c->strong_key2.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(uint64_t));
+            c->strong_key.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(uint64_t));
             c->strong_rv2.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(uint64_t));
             c->strong_seg.ensure((size_t)NW * sizeof(int2) + 64);
             c->strong_ng.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t));
@@ -399,8 +468,9 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
             while (win_bits < 32 && ((int64_t)1 << win_bits) < (int64_t)NW) ++win_bits;
             c->launch("strong_index", [&] {
                 return launch_strong_index(c->stream, sink, nchunks, n, NW, win_bits, c->strong_tmp.p, c->strong_tmp.cap,
-                                           c->strong_win2.as<uint32_t>(), c->strong_rv2.as<uint64_t>(),
-                                           c->strong_off.as<int64_t>(), c->strong_seg.as<int2>(),
+                                           c->strong_key2.as<uint64_t>(), c->strong_rv2.as<uint64_t>(),
+                                           c->strong_key.as<uint64_t>(), c->strong_off.as<int64_t>(),
+                                           c->strong_seg.as<int2>(),
                                            c->strong_ng.as<int32_t>());
             });
             SI = StrongIndex{c->strong_seg.as<int2>(), c->strong_rv.as<uint64_t>(), c->strong_ng.as<int32_t>(), delta};
@@ -507,7 +577,12 @@ void run_levels(dcns_ctx *c, const Staged &S, const Params &P, Forest &F, int64_
         levels = 1;
     }
     // levels past the captured ones (or all of them): host-driven, exact grids
+    // every right child starts below its parent's first row, so a window's
+    // recursion is at most its reference's length deep
+    int32_t max_len1 = 0;
+    for (const PairDesc &d : S.pd) max_len1 = std::max(max_len1, d.len1);
     while (h.next_tasks > 0) {
+        if (levels > max_len1 + 2) throw Error(DCNS_ERR_INTERNAL, "seed recursion deeper than its reference");
         const int32_t n = h.next_tasks;
         c->task_order.ensure((size_t)n * sizeof(int32_t) + 64);
         level_launches(L, n, 0, true);

@@ -79,6 +79,14 @@ __device__ __forceinline__ void mark_lastrow(int32_t *__restrict__ lastrow, uint
     if (hit >> 16) lastrow[base + 64u + ln] = row;
 }
 
+// mark_lastrow for segments that run at once: the largest row wins.
+__device__ __forceinline__ void mark_lastrow_max(int32_t *__restrict__ lastrow, uint32_t base, uint32_t hit,
+                                                 int32_t row) {
+    const uint32_t ln = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    if (hit & 0xffffu) atomicMax(&lastrow[base + ln], row);
+    if (hit >> 16) atomicMax(&lastrow[base + 64u + ln], row);
+}
+
 // Appends one strong row (window w, entry rv) to the calling lane's chunk of
 // the sink.  sk: the lane's two LDS words, its chunk (-1 before the first)
 // and fill.  Lanes append independently (no cross-lane operation, so the
@@ -358,7 +366,7 @@ __global__ __launch_bounds__(64) void k_scan_uniform(const int8_t *__restrict__ 
 // integer, and v_pk_maximum3_f16 merges the three-way maxima.  Non-negative
 // f16 values order like their bit patterns, so the improvement test and the
 // first-column search compare bits.
-template <int WM, bool REC>
+template <int WM, bool REC, bool SEG = false>
 __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_t *__restrict__ seq,
                                                     const PairDesc *__restrict__ pairs, const Params &P,
                                                     const ScoreTab &tab, TrajEntry *__restrict__ traj,
@@ -366,7 +374,8 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
                                                     int32_t *__restrict__ hitlist, Counters *__restrict__ ctr,
                                                     uint32_t open_h, uint32_t ext_h, uint32_t thr_h,
                                                     int32_t *__restrict__ lastrow, const StrongSink &sink,
-                                                    int32_t *sk) {
+                                                    int32_t *sk, const SegL0 &sg = SegL0{}, int32_t r0 = 0,
+                                                    int32_t r1 = 0, int32_t tid = 0) {
     const int lane = threadIdx.x;
     const PairDesc pd = pairs[task.x];
     const int8_t *__restrict__ ref = seq + pd.off1;
@@ -394,12 +403,16 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
     uint32_t best = thr2;
     int32_t cntl = 0, cnth = 0;
 
-    uint32_t word = L1 > 0 ? *(const uint32_t *)ref : 0u;
+    // SEG: rows [r0, r1) after at least sg.warm rows from zeros (the level-0
+    // tail bound makes rows from r0 on level 0's values), the first row a
+    // multiple of 4 like level 0's own; records only from r0
+    const int i0 = SEG ? max(0, r0 - sg.warm) & ~3 : 0, i1 = SEG ? r1 : L1;
+    uint32_t word = i0 < L1 ? *(const uint32_t *)(ref + i0) : 0u;
     uint32_t next = 0;
     // two rows per iteration: the register allocator then keeps H, E in
     // place across rows instead of rotating them with moves at the loop end
 #pragma unroll 2
-    for (int i = 0; i < L1; ++i) {
+    for (int i = i0; i < i1; ++i) {
         const int sub = i & 3;
         if (sub == 0 && i + 4 < L1) next = *(const uint32_t *)(ref + i + 4);
         const int c = (int)(int8_t)(word >> (8 * sub));
@@ -422,8 +435,9 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
             else if (j == WM - 1) rmax = __builtin_elementwise_maximum(rmax, h);
         }
         const uint32_t hit = pk_subsat(bits(rmax), thr2);  // halves whose row reaches min_score
-        if (__builtin_expect(hit != 0, 0)) {
-            mark_lastrow(lastrow, (uint32_t)(pd.win_base + task.y), hit, i);
+        if (__builtin_expect(hit != 0 && (!SEG || i >= r0), 0)) {
+            if (SEG) mark_lastrow_max(lastrow, (uint32_t)(pd.win_base + task.y), hit, i);
+            else mark_lastrow(lastrow, (uint32_t)(pd.win_base + task.y), hit, i);
             const uint32_t rb = bits(rmax);
             const uint32_t imp = pk_subsat(rb, best);
             // first column of each half's row maximum, where a record needs
@@ -444,11 +458,13 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
             }
             const int32_t vl = (int32_t)(float)rmax[0], vh = (int32_t)(float)rmax[1];
             if (imp & 0xffffu) {
-                record(traj, pd.win_base + wl, cap, cntl, i, vl, jl);
+                if (SEG) record(sg.traj, (int64_t)tid * 128 + lane, cap, cntl, i, vl, jl);
+                else record(traj, pd.win_base + wl, cap, cntl, i, vl, jl);
                 best = (best & 0xffff0000u) | (rb & 0xffffu);
             }
             if (imp >> 16) {
-                record(traj, pd.win_base + wh, cap, cnth, i, vh, jh);
+                if (SEG) record(sg.traj, (int64_t)tid * 128 + 64 + lane, cap, cnth, i, vh, jh);
+                else record(traj, pd.win_base + wh, cap, cnth, i, vh, jh);
                 best = (best & 0xffffu) | (rb & 0xffff0000u);
             }
             if (REC) {
@@ -459,6 +475,12 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
                     strong_push(sink, my, (uint32_t)(pd.win_base + wh), strong_entry(i, vh, kStrongNoCol));
             }
         }
+    }
+    if (SEG) {
+        // the segments' trajectories merge in k_l0_merge
+        sg.tcount[(int64_t)tid * 128 + lane] = vl ? cntl : 0;
+        sg.tcount[(int64_t)tid * 128 + 64 + lane] = vh ? cnth : 0;
+        return;
     }
     if (vl) {
         tcount[pd.win_base + wl] = cntl;
@@ -473,6 +495,61 @@ __device__ __forceinline__ void scan_uniform_task_h(const int2 task, const int8_
             hitlist[atomicAdd(&ctr->hits, 1)] = pd.win_base + wh;
             atomicAdd(&ctr->entries, min(cnth, cap));
         }
+    }
+}
+
+// Level 0 in row segments: one wave per (128-window group, segment) task,
+// sg.tasks[blockIdx.x] = (pair, first window, r0, r1).
+template <int WM, bool REC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_uniform_seg(
+    const int8_t *__restrict__ seq, const PairDesc *__restrict__ pairs, SegL0 sg, Params P, ScoreTab tab,
+    int32_t cap, Counters *__restrict__ ctr, uint32_t open_h, uint32_t ext_h, uint32_t thr_h,
+    int32_t *__restrict__ lastrow, StrongSink sink) {
+    __shared__ int32_t sk[REC ? 128 : 1];  // each lane's strong-row chunk and fill
+    if (REC) {
+        sk[2 * threadIdx.x] = -1;
+        sk[2 * threadIdx.x + 1] = 0;
+    }
+    const int4 t = sg.tasks[blockIdx.x];
+    scan_uniform_task_h<WM, REC, true>(make_int2(t.x, t.y), seq, pairs, P, tab, nullptr, nullptr, cap, nullptr, ctr,
+                                       open_h, ext_h, thr_h, lastrow, sink, sk, sg, t.z, t.w, (int32_t)blockIdx.x);
+}
+
+// A segmented group's windows: the segments' trajectories in row order,
+// each entry kept when it beats every earlier one (a segment's own running
+// maximum starts at min_score - 1, so its records are a superset of the
+// window's past its first row), into the window's trajectory slot; windows
+// with an entry join the hit list.  sg.groups[g] = (pair, first window,
+// first task, segments).
+__global__ __launch_bounds__(128) void k_l0_merge(const PairDesc *__restrict__ pairs, SegL0 sg, int32_t ngroups,
+                                                  TrajEntry *__restrict__ traj, int32_t *__restrict__ tcount,
+                                                  int32_t cap, int32_t min_score, int32_t *__restrict__ hitlist,
+                                                  Counters *__restrict__ ctr) {
+    const int32_t g = blockIdx.x, k = threadIdx.x;
+    if (g >= ngroups) return;
+    const int4 G = sg.groups[g];
+    const PairDesc pd = pairs[G.x];
+    const int32_t w = G.y + k;
+    if (w >= pd.nwin) return;
+    const int64_t gw = pd.win_base + w;
+    int32_t best = min_score - 1, cnt = 0;
+    for (int32_t s = 0; s < G.w; ++s) {
+        const int64_t slot = (int64_t)(G.z + s) * 128 + k;
+        const int32_t n = min(sg.tcount[slot], cap);
+        const TrajEntry *e = sg.traj + slot * cap;
+        for (int32_t q = 0; q < n; ++q) {
+            const TrajEntry x = e[q];
+            if ((x.score_col >> 8) > best) {
+                if (cnt < cap) traj[gw * cap + cnt] = x;
+                ++cnt;
+                best = x.score_col >> 8;
+            }
+        }
+    }
+    tcount[gw] = cnt;
+    if (cnt > 0) {
+        hitlist[atomicAdd(&ctr->hits, 1)] = (int32_t)gw;
+        atomicAdd(&ctr->entries, min(cnt, cap));
     }
 }
 
@@ -1010,16 +1087,19 @@ __global__ void k_gather_roots(const Node *__restrict__ nodes, const int32_t *__
 
 // Strong-row index.  Chunk c's entries move to chunk_off[c] onward (one
 // workgroup per chunk).
+// The sort key is window << 32 | row (segments of one window run at once, so
+// a window's entries reach the sink out of row order).
 __global__ __launch_bounds__(64) void k_strong_compact(const uint32_t *__restrict__ win,
                                                         const uint64_t *__restrict__ rv,
                                                         const int32_t *__restrict__ fill,
-                                                        const int64_t *__restrict__ off, uint32_t *__restrict__ win2,
+                                                        const int64_t *__restrict__ off, uint64_t *__restrict__ key2,
                                                         uint64_t *__restrict__ rv2) {
     const int64_t c = blockIdx.x;
     const int32_t n = fill[c];
     for (int32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        win2[off[c] + i] = win[c * kStrongChunk + i];
-        rv2[off[c] + i] = rv[c * kStrongChunk + i];
+        const uint64_t e = rv[c * kStrongChunk + i];
+        key2[off[c] + i] = ((uint64_t)win[c * kStrongChunk + i] << 32) | (e >> 32);
+        rv2[off[c] + i] = e;
     }
 }
 
@@ -1031,12 +1111,12 @@ __global__ void k_strong_total(const int32_t *__restrict__ fill, const int64_t *
 
 // seg[w] = [first, last + 1) of window w's entries in the sorted list (seg
 // cleared to empty beforehand).
-__global__ void k_strong_seg(const uint32_t *__restrict__ win, int64_t n, int2 *__restrict__ seg) {
+__global__ void k_strong_seg(const uint64_t *__restrict__ key, int64_t n, int2 *__restrict__ seg) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t w = win[i];
-    if (i == 0 || win[i - 1] != w) seg[w].x = (int32_t)i;
-    if (i == n - 1 || win[i + 1] != w) seg[w].y = (int32_t)(i + 1);
+    const uint32_t w = (uint32_t)(key[i] >> 32);
+    if (i == 0 || (uint32_t)(key[i - 1] >> 32) != w) seg[w].x = (int32_t)i;
+    if (i == n - 1 || (uint32_t)(key[i + 1] >> 32) != w) seg[w].y = (int32_t)(i + 1);
 }
 
 __device__ __forceinline__ int32_t strong_val(uint64_t e) { return (int32_t)((uint32_t)e >> 8); }
@@ -1294,50 +1374,73 @@ hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v) {
     return hipGetLastError();
 }
 
+hipError_t launch_scan_uniform_seg(hipStream_t st, int wm, const int8_t *seq, const PairDesc *pairs, const SegL0 &sg,
+                                   int32_t ntasks, const Params &P, int32_t cap, Counters *ctr, int32_t *lastrow,
+                                   const StrongSink &sink) {
+    if (ntasks <= 0) return hipSuccess;
+    if (!scan_uses_f16(P)) return hipErrorInvalidValue;
+    const ScoreTab th = make_score_tab_f16(P.match, P.mismatch);
+    const uint32_t oh = f16_bits_of_int(P.open), xh = f16_bits_of_int(P.ext), thr = f16_bits_of_int(P.min_score - 1);
+    const bool rec = sink.cap_chunks > 0;
+    decltype(&k_scan_uniform_seg<38, false>) k =
+        wm <= 38 ? (rec ? k_scan_uniform_seg<38, true> : k_scan_uniform_seg<38, false>)
+                 : (rec ? k_scan_uniform_seg<64, true> : k_scan_uniform_seg<64, false>);
+    hipLaunchKernelGGL(k, dim3((unsigned)ntasks), dim3(64), 0, st, seq, pairs, sg, P, th, cap, ctr, oh, xh, thr, lastrow,
+                       sink);
+    return hipGetLastError();
+}
+
+hipError_t launch_l0_merge(hipStream_t st, const PairDesc *pairs, const SegL0 &sg, int32_t ngroups, TrajEntry *traj,
+                           int32_t *tcount, int32_t cap, int32_t min_score, int32_t *hitlist, Counters *ctr) {
+    if (ngroups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_l0_merge, dim3((unsigned)ngroups), dim3(128), 0, st, pairs, sg, ngroups, traj, tcount, cap,
+                       min_score, hitlist, ctr);
+    return hipGetLastError();
+}
+
 size_t strong_index_scratch(int64_t nchunks, int64_t n) {
     size_t a = 0, b = 0;
     hipcub::DeviceScan::ExclusiveSum(nullptr, a, (const int64_t *)nullptr, (int64_t *)nullptr,
                                      (int)std::max<int64_t>(nchunks, 1));
-    hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+    hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const uint64_t *)nullptr, (uint64_t *)nullptr,
                                        (const uint64_t *)nullptr, (uint64_t *)nullptr, (int)std::max<int64_t>(n, 1));
     return std::max(a, b) + 256;
 }
 
 hipError_t launch_strong_count(hipStream_t st, const StrongSink &S, int32_t nchunks, void *scratch,
-                               size_t scratch_bytes, int64_t *chunk_off, int64_t *total_dev) {
+                               size_t scratch_bytes, int64_t *fill64, int64_t *chunk_off, int64_t *total_dev) {
     if (nchunks <= 0) {
         hipLaunchKernelGGL(k_strong_total, dim3(1), dim3(64), 0, st, S.chunk_fill, chunk_off, 0, total_dev);
         return hipGetLastError();
     }
-    // chunk fills widen to int64 in chunk_off, then scan in place
+    // chunk fills widen to int64, then their exclusive scan into chunk_off
     hipLaunchKernelGGL(k_widen_i32, dim3((unsigned)((nchunks + 255) / 256)), dim3(256), 0, st, S.chunk_fill, nchunks,
-                       chunk_off);
+                       fill64);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     size_t bytes = scratch_bytes;
-    e = hipcub::DeviceScan::ExclusiveSum(scratch, bytes, chunk_off, chunk_off, nchunks, st);
+    e = hipcub::DeviceScan::ExclusiveSum(scratch, bytes, fill64, chunk_off, nchunks, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_strong_total, dim3(1), dim3(64), 0, st, S.chunk_fill, chunk_off, nchunks, total_dev);
     return hipGetLastError();
 }
 
 hipError_t launch_strong_index(hipStream_t st, const StrongSink &S, int32_t nchunks, int64_t n, int32_t nw,
-                               int32_t win_bits, void *scratch, size_t scratch_bytes, uint32_t *win2, uint64_t *rv2,
-                               int64_t *chunk_off, int2 *seg, int32_t *ng) {
+                               int32_t win_bits, void *scratch, size_t scratch_bytes, uint64_t *key2, uint64_t *rv2,
+                               uint64_t *key, int64_t *chunk_off, int2 *seg, int32_t *ng) {
     hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((2 * (int64_t)nw + 255) / 256)), dim3(256), 0, st, (int32_t *)seg,
                        2 * (int64_t)nw, 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n <= 0) return e;
-    // the chunks' entries, compacted into (win2, rv2), sort back into the
-    // sink's own arrays, which then hold the index's entries
+    // the chunks' entries, compacted into (key2, rv2), sort by (window, row)
+    // into (key, the sink's rv array), which then holds the index's entries
     hipLaunchKernelGGL(k_strong_compact, dim3((unsigned)nchunks), dim3(64), 0, st, S.win, S.rv, S.chunk_fill,
-                       chunk_off, win2, rv2);
+                       chunk_off, key2, rv2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     size_t bytes = scratch_bytes;
-    // sort by window, stable, so each window's entries keep their row order
-    e = hipcub::DeviceRadixSort::SortPairs(scratch, bytes, win2, S.win, rv2, S.rv, (int)n, 0, win_bits, st);
+    e = hipcub::DeviceRadixSort::SortPairs(scratch, bytes, key2, key, rv2, S.rv, (int)n, 0, 32 + win_bits, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_strong_seg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, S.win, n, seg);
+    hipLaunchKernelGGL(k_strong_seg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, key, n, seg);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_strong_ng, dim3((unsigned)((nw + 63) / 64)), dim3(64), 0, st, seg, nw, S.rv, ng);
     return hipGetLastError();

@@ -140,7 +140,14 @@ int dcns_last_stats(const dcns_ctx *ctx, dcns_stats *out);
  *                 g = -max(open, ext)); from row a + D on its values are
  *                 level 0's, whose recorded rows complete its trajectory.
  *                 Off when g <= 0 or the level-0 scan is not packed f16, and
- *                 for a batch whose recorded rows overflow their buffers. */
+ *                 for a batch whose recorded rows overflow their buffers.
+ *   level0_split  cut1Gap level-0 scan: 0 (default): when the batch's
+ *                 longest references would hold the launch, every 128-window
+ *                 group runs its rows in segments (each started D rows early
+ *                 from zeros, D as above, so its rows equal level 0's), as
+ *                 many waves as segments, and the segments' trajectories
+ *                 merge; 1: always, in segments of 4 D rows; -1: never.
+ *                 Needs the conditions of level0_tail's D. */
 typedef struct dcns_tuning {
     int32_t level0_mode, level0_waves;
     int32_t wave_tasks, wave_rows;
@@ -149,6 +156,7 @@ typedef struct dcns_tuning {
     int32_t xdrop_lane_rows;
     int32_t k1_lanes;
     int32_t level0_tail;
+    int32_t level0_split;
 } dcns_tuning;
 int dcns_get_tuning(const dcns_ctx *ctx, dcns_tuning *out);
 int dcns_set_tuning(dcns_ctx *ctx, const dcns_tuning *in);

@@ -1,9 +1,11 @@
 """The seed recursion with level-0 tails (level0_tail = 1): level 0 records
 every row where a window's maximum reaches min_score, and a rescan runs only
-its first D rows (D = window + ceil(window * match / g)), taking the rest
-of its trajectory from those rows.  Seed sets, fragments and reference cell
-counts must equal the oracle's exactly, as without the tails; the GPU's own
-cell count must drop where rescans are long."""
+its first D rows (D = wm + ceil(window * match / g)), taking the rest of its
+trajectory from those rows; and level 0 in row segments (level0_split = 1),
+each segment started D rows early from zeros, the segments' trajectories
+merged.  Seed sets, fragments and reference cell counts must equal the
+oracle's exactly, as without them; the GPU's own cell count must drop where
+rescans are long."""
 import numpy as np
 import pytest
 
@@ -15,6 +17,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _with_tail(gpu, fn, **tuning):
+    tuning.setdefault("level0_split", -1)
     old = gpu.set_tuning(level0_tail=1, **tuning)
     try:
         return fn()
@@ -22,12 +25,13 @@ def _with_tail(gpu, fn, **tuning):
         gpu.set_tuning(**old)
 
 
-def test_tail_masked_repeat_pair_vs_oracle(gpu, oracle_lib):
+@pytest.mark.parametrize("split", [-1, 1])
+def test_tail_masked_repeat_pair_vs_oracle(gpu, oracle_lib, split):
     t1, t2 = synth.long_masked(seed=77, L=20000, te_copies=20, te_len=1500, nfrag=30)
     a = synth.encode(t1.replace("n", ""))
     b = synth.encode(t2.replace("n", ""))
-    print("tail: masked repeat pair", len(a), len(b), flush=True)
-    got = _with_tail(gpu, lambda: gpu.find_similar([(a, b)])[0])
+    print("tail: masked repeat pair", len(a), len(b), "split", split, flush=True)
+    got = _with_tail(gpu, lambda: gpu.find_similar([(a, b)])[0], level0_split=split)
     st = gpu.stats()
     o = {}
     want = oracle_lib.find_similar(a, b, stats=o)
@@ -44,7 +48,8 @@ def test_tail_masked_repeat_pair_vs_oracle(gpu, oracle_lib):
     assert st["gpu_scan_cells"] < full
 
 
-def test_tail_long_rescans_with_n_vs_oracle(gpu, oracle_lib):
+@pytest.mark.parametrize("split", [-1, 1])
+def test_tail_long_rescans_with_n_vs_oracle(gpu, oracle_lib, split):
     rng = np.random.default_rng(4242)
     a, b = synth.one_pair(seed=4243, L1=14000, L2=6013, nfrag=16)[:2]
     for _ in range(25):
@@ -54,8 +59,8 @@ def test_tail_long_rescans_with_n_vs_oracle(gpu, oracle_lib):
         src = int(rng.integers(0, len(b) - 300))
         dst = 1000 + k * 2100
         a[dst:dst + 300] = synth.mutate(rng, b[src:src + 300], 0.05, 0.0)[:300]
-    print("tail: long rescans with N", flush=True)
-    got = _with_tail(gpu, lambda: gpu.find_similar([(a, b)])[0])
+    print("tail: long rescans with N, split", split, flush=True)
+    got = _with_tail(gpu, lambda: gpu.find_similar([(a, b)])[0], level0_split=split)
     st = gpu.stats()
     o = {}
     want = oracle_lib.find_similar(a, b, stats=o)
@@ -77,10 +82,10 @@ def test_tail_seed_sets_vs_oracle(gpu, oracle_lib, sc, min_score):
     pairs += [(rep, synth.mutate(rng, unit, 0.02, 0.0)), (rep, rep[:500].copy())]
     p = dcns_amd.default_params(match=sc[0], mismatch=sc[1], open=sc[2], ext=sc[3], min_score=min_score)
     print("tail: seed sets", sc, flush=True)
-    for linked in (False, True):
-        got = _with_tail(gpu, lambda: gpu.seed_sets(pairs, params=p, linked=linked))
+    for linked, split in ((False, -1), (True, -1), (False, 1)):
+        got = _with_tail(gpu, lambda: gpu.seed_sets(pairs, params=p, linked=linked), level0_split=split)
         for i, (a, b) in enumerate(pairs):
-            assert got[i] == oracle_lib.seed_set(a, b, sc, min_score=min_score, linked=linked), (sc, i, linked)
+            assert got[i] == oracle_lib.seed_set(a, b, sc, min_score=min_score, linked=linked), (sc, i, linked, split)
 
 
 @pytest.mark.parametrize("window,step,min_score", [(24, 6, 30), (50, 8, 60)])
@@ -91,9 +96,10 @@ def test_tail_other_windows_vs_oracle(gpu, oracle_lib, window, step, min_score):
     pairs = _small_pairs(93, 6, 400, 2500, (2, 10)) + [_repeat_pair(rng, 2500, 800)]
     p = dcns_amd.default_params(window=window, step=step, min_score=min_score)
     print("tail: window", window, flush=True)
-    got = _with_tail(gpu, lambda: gpu.seed_sets(pairs, params=p, linked=False))
-    for i, (a, b) in enumerate(pairs):
-        assert got[i] == oracle_lib.seed_set(a, b, window=window, step=step, min_score=min_score), (window, i)
+    for split in (-1, 1):
+        got = _with_tail(gpu, lambda: gpu.seed_sets(pairs, params=p, linked=False), level0_split=split)
+        for i, (a, b) in enumerate(pairs):
+            assert got[i] == oracle_lib.seed_set(a, b, window=window, step=step, min_score=min_score), (window, i)
 
 
 @pytest.mark.parametrize("pool_min", [1, 64])
@@ -107,7 +113,7 @@ def test_tail_pools_grow_vs_oracle(gpu, oracle_lib, pool_min):
         a[500 + k * 1700:800 + k * 1700] = synth.mutate(rng, b[src:src + 300], 0.04, 0.0)[:300]
     pairs = [(a, b), _repeat_pair(rng, 4000, 1200)] + _small_pairs(71, 4)
     print("tail: pools from", pool_min, flush=True)
-    got = _with_tail(gpu, lambda: gpu.find_similar(pairs), pool_min=pool_min)
+    got = _with_tail(gpu, lambda: gpu.find_similar(pairs), pool_min=pool_min, level0_split=1)
     st = gpu.stats()
     cells = 0
     for i, (x, y) in enumerate(pairs):
@@ -126,7 +132,7 @@ def test_tail_level_graph_replays(gpu, oracle_lib):
     rng = np.random.default_rng(919)
     pairs = _small_pairs(83, 6, 600, 5000, (1, 4)) + [_repeat_pair(rng, 6000, 1500)]
     gpu.stage(pairs)
-    old = gpu.set_tuning(level0_tail=1, graph_levels=-1)
+    old = gpu.set_tuning(level0_tail=1, level0_split=1, graph_levels=-1)
     try:
         want = gpu.find_similar_staged()
         st0 = gpu.stats()
@@ -148,3 +154,24 @@ def test_tail_level_graph_replays(gpu, oracle_lib):
         w = oracle_lib.find_similar(x, y, stats=o)
         assert want[i][0] == o["seeds"], i
         _cmp_fragments(want[i][1], w, i)
+
+
+def test_split_level0_only_vs_oracle(gpu, oracle_lib):
+    """Level 0 in row segments without the tails (rescans run in full): the
+    merged level-0 trajectories and lastrow feed the same recursion."""
+    pairs = _small_pairs(97, 8, 1500, 9000, (2, 12))
+    old = gpu.set_tuning(level0_tail=-1, level0_split=1)
+    try:
+        print("split only", flush=True)
+        got = gpu.find_similar(pairs)
+        st = gpu.stats()
+    finally:
+        gpu.set_tuning(**old)
+    cells = 0
+    for i, (x, y) in enumerate(pairs):
+        o = {}
+        want = oracle_lib.find_similar(x, y, stats=o)
+        assert got[i][0] == o["seeds"], i
+        _cmp_fragments(got[i][1], want, i)
+        cells += o["scan_cells"]
+    assert st["scan_cells"] == cells
