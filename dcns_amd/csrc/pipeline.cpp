@@ -762,6 +762,178 @@ __attribute__((target("avx2"))) bool link_pass_avx2(const std::vector<Seed> &cur
     return changed;
 }
 
+// One seed against one candidate, in the order of the reference's tests:
+// inside (absorbed), after on the same diagonal (merge, stop), beyond (stop).
+enum { kLinkNone = 0, kLinkInside = 1, kLinkMerge = 2, kLinkStop = 3 };
+inline int link_judge(int32_t a1, int32_t ae1, int32_t a2, int32_t ae2, int32_t b1, int32_t be1, int32_t b2,
+                      int32_t be2) {
+    if (a2 <= b2 && ae1 >= be1 && ae2 >= be2) return kLinkInside;
+    if (a2 <= b2 && ae1 <= be1 && ae2 <= be2) {
+        const int32_t d = b1 - a1;
+        return (d == b2 - a2 && d == be1 - ae1 && d == be2 - ae2) ? kLinkMerge : kLinkNone;
+    }
+    if (b1 > ae1 || b2 > ae2) return kLinkStop;
+    return kLinkNone;
+}
+
+// Link passes over fixed positions (a consumed seed dies in place, a merged
+// seed keeps its first seed's position), each pass cut into chunks that run
+// on all host threads.  A chunk starts where no earlier live seed's end1
+// reaches its first seed's start1, so no earlier seed can absorb a seed of it
+// (only a merge can cross).  A chunk's seeds read positions past the chunk
+// from the pass's starting state, and the seeds they consume there are
+// listed instead of written.  Then, in order, a chunk into which an earlier
+// chunk consumed runs again from the pass's starting state with those seeds
+// dead: every seed sees what the one-at-a-time pass shows it.
+struct LinkArrays {
+    std::vector<int32_t> S1, E1, S2, E2;
+    std::vector<uint8_t> dead;
+};
+
+__attribute__((target("avx2"))) bool link_chunk(LinkArrays &L, const LinkArrays &P, int64_t b, int64_t e,
+                                                int64_t n, std::vector<int64_t> &kills) {
+    kills.clear();
+    bool merged = false;
+    const __m256i ones = _mm256_set1_epi32(-1);
+    auto killed_past = [&](int64_t q) { return std::find(kills.begin(), kills.end(), q) != kills.end(); };
+    for (int64_t i = b; i < e; ++i) {
+        if (L.dead[(size_t)i]) continue;
+        const int32_t a1 = L.S1[(size_t)i], ae1 = L.E1[(size_t)i], a2 = L.S2[(size_t)i], ae2 = L.E2[(size_t)i];
+        const __m256i vs1 = _mm256_set1_epi32(a1), ve1 = _mm256_set1_epi32(ae1), vs2 = _mm256_set1_epi32(a2),
+                      ve2 = _mm256_set1_epi32(ae2);
+        int64_t mj = -1;
+        bool done = false;
+        int64_t j = i + 1;
+        // the chunk's own positions 8 at a time (as link_pass_avx2)
+        for (; j + 8 <= e; j += 8) {
+            const __m256i b1 = _mm256_loadu_si256((const __m256i *)(L.S1.data() + j));
+            const __m256i be1 = _mm256_loadu_si256((const __m256i *)(L.E1.data() + j));
+            const __m256i b2 = _mm256_loadu_si256((const __m256i *)(L.S2.data() + j));
+            const __m256i be2 = _mm256_loadu_si256((const __m256i *)(L.E2.data() + j));
+            const __m256i lo2 = _mm256_cmpgt_epi32(vs2, b2);
+            const uint32_t inside = mask(_mm256_andnot_si256(
+                _mm256_or_si256(lo2, _mm256_or_si256(_mm256_cmpgt_epi32(be1, ve1), _mm256_cmpgt_epi32(be2, ve2))), ones));
+            const uint32_t after = mask(_mm256_andnot_si256(
+                _mm256_or_si256(lo2, _mm256_or_si256(_mm256_cmpgt_epi32(ve1, be1), _mm256_cmpgt_epi32(ve2, be2))), ones));
+            const __m256i d = _mm256_sub_epi32(b1, vs1);
+            const uint32_t diag = mask(_mm256_and_si256(
+                _mm256_cmpeq_epi32(d, _mm256_sub_epi32(b2, vs2)),
+                _mm256_and_si256(_mm256_cmpeq_epi32(d, _mm256_sub_epi32(be1, ve1)),
+                                 _mm256_cmpeq_epi32(d, _mm256_sub_epi32(be2, ve2)))));
+            const uint32_t beyond = mask(_mm256_or_si256(_mm256_cmpgt_epi32(b1, ve1), _mm256_cmpgt_epi32(b2, ve2)));
+            const uint32_t live = (uint32_t)_mm_movemask_epi8(
+                                      _mm_cmpeq_epi8(_mm_loadl_epi64((const __m128i *)(L.dead.data() + j)), _mm_setzero_si128())) &
+                                  0xffu;
+            const uint32_t merge = ~inside & after & diag;
+            const uint32_t stop = live & (merge | (~inside & ~after & beyond)) & 0xffu;
+            uint32_t absorb = live & inside & (stop ? (stop & (0u - stop)) - 1u : 0xffu);
+            while (absorb) {
+                L.dead[(size_t)(j + __builtin_ctz(absorb))] = 1;
+                absorb &= absorb - 1;
+            }
+            if (stop) {
+                const int k = __builtin_ctz(stop);
+                if ((merge >> k) & 1u) mj = j + k;
+                done = true;
+                break;
+            }
+        }
+        // the rest of the chunk, then past it from the starting state
+        for (; !done && j < n; ++j) {
+            const bool own = j < e;
+            if (own ? L.dead[(size_t)j] != 0 : (P.dead[(size_t)j] != 0 || killed_past(j))) continue;
+            const LinkArrays &A = own ? L : P;
+            const int r = link_judge(a1, ae1, a2, ae2, A.S1[(size_t)j], A.E1[(size_t)j], A.S2[(size_t)j], A.E2[(size_t)j]);
+            if (r == kLinkInside) {
+                if (own) L.dead[(size_t)j] = 1;
+                else kills.push_back(j);
+            } else if (r == kLinkMerge) {
+                mj = j;
+                break;
+            } else if (r == kLinkStop) {
+                break;
+            }
+        }
+        if (mj >= 0) {
+            const bool own = mj < e;
+            L.E1[(size_t)i] = own ? L.E1[(size_t)mj] : P.E1[(size_t)mj];
+            L.E2[(size_t)i] = own ? L.E2[(size_t)mj] : P.E2[(size_t)mj];
+            if (own) L.dead[(size_t)mj] = 1;
+            else kills.push_back(mj);
+            merged = true;
+        }
+    }
+    return merged;
+}
+
+std::vector<Seed> link_parallel(const std::vector<Seed> &cur) {
+    const int64_t n = (int64_t)cur.size();
+    LinkArrays L, P;
+    for (std::vector<int32_t> *v : {&L.S1, &L.E1, &L.S2, &L.E2}) v->assign((size_t)n + 8, 0);
+    L.dead.assign((size_t)n + 8, 1);
+    for (int64_t i = 0; i < n; ++i) {
+        L.S1[(size_t)i] = cur[(size_t)i].getStart1();
+        L.E1[(size_t)i] = cur[(size_t)i].getEnd1();
+        L.S2[(size_t)i] = cur[(size_t)i].getStart2();
+        L.E2[(size_t)i] = cur[(size_t)i].getEnd2();
+        L.dead[(size_t)i] = 0;
+    }
+    P.S1 = L.S1;  // start coordinates never change
+    P.S2 = L.S2;
+    const int threads = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+    bool changed = true;
+    while (changed) {
+        P.E1 = L.E1;
+        P.E2 = L.E2;
+        P.dead = L.dead;
+        const int64_t want = std::max<int64_t>(n / ((int64_t)threads * 8), 4096);
+        std::vector<int64_t> starts = {0};
+        int32_t reach = INT32_MIN;
+        for (int64_t p = 0; p < n; ++p) {
+            if (L.dead[(size_t)p]) continue;
+            if (p - starts.back() >= want && reach < L.S1[(size_t)p]) starts.push_back(p);
+            reach = std::max(reach, L.E1[(size_t)p]);
+        }
+        starts.push_back(n);
+        const int64_t nc = (int64_t)starts.size() - 1;
+        std::vector<std::vector<int64_t>> kills((size_t)nc);
+        std::vector<uint8_t> merged((size_t)nc, 0);
+        parallel_for(nc, [&](int64_t c) {
+            merged[(size_t)c] = link_chunk(L, P, starts[(size_t)c], starts[(size_t)c + 1], n, kills[(size_t)c]);
+        }, threads);
+        std::vector<int64_t> incoming, keep;
+        for (int64_t c = 0; c < nc; ++c) {
+            const int64_t b = starts[(size_t)c], e = starts[(size_t)c + 1];
+            bool hit = false;
+            for (int64_t q : incoming) hit |= q >= b && q < e;
+            if (hit) {
+                for (int64_t p = b; p < e; ++p) {
+                    L.E1[(size_t)p] = P.E1[(size_t)p];
+                    L.E2[(size_t)p] = P.E2[(size_t)p];
+                    L.dead[(size_t)p] = P.dead[(size_t)p];
+                }
+                for (int64_t q : incoming)
+                    if (q >= b && q < e) L.dead[(size_t)q] = 1;
+                merged[(size_t)c] = link_chunk(L, P, b, e, n, kills[(size_t)c]);
+            }
+            keep.clear();
+            for (int64_t q : incoming)
+                if (q >= e) keep.push_back(q);
+            for (int64_t q : kills[(size_t)c]) keep.push_back(q);
+            incoming.swap(keep);
+        }
+        changed = false;
+        for (uint8_t m : merged) changed |= m != 0;
+    }
+    std::vector<Seed> out;
+    for (int64_t i = 0; i < n; ++i)
+        if (!L.dead[(size_t)i]) out.push_back(Seed(L.S1[(size_t)i], L.E1[(size_t)i], L.S2[(size_t)i], L.E2[(size_t)i]));
+    return out;
+}
+
+// Lists from this size on link on all host threads.
+constexpr size_t kLinkParallelMin = 65536;
+
 }  // namespace
 
 // link_seeds (findSimilarFragmentsForPairedSequence4.cpp:287-344) up to its
@@ -772,6 +944,7 @@ std::vector<Seed> link_seed_list(const std::set<Seed> &seeds) {
     std::vector<Seed> cur(seeds.begin(), seeds.end());
     std::sort(cur.begin(), cur.end(), [](Seed a, Seed b) { return a.getStart1() < b.getStart1(); });
     static const bool avx2 = __builtin_cpu_supports("avx2");
+    if (avx2 && cur.size() >= kLinkParallelMin) return link_parallel(cur);
     bool changed = true;
     while (changed) {
         std::vector<Seed> next;

@@ -71,3 +71,21 @@ def test_link_seeds_of_unlinked_sets(oracle_lib):
         a, b = synth.one_pair(seed=7100 + k, L1=6000, L2=5000, nfrag=12)[:2]
         unlinked = oracle_lib.seed_set(a, b, linked=False)
         assert dcns_amd.link_seeds(unlinked) == oracle_lib.seed_set(a, b, linked=True), k
+
+
+@pytest.mark.parametrize("seed,n,span", [(11, 100000, 100000), (12, 100000, 30000), (14, 80000, 4000)])
+def test_link_seeds_large_lists(oracle_lib, seed, n, span):
+    """Lists of 65,536 seeds and more link on all host threads, in chunks cut
+    where no earlier seed reaches the next one, with consumed seeds across a
+    chunk's end replayed in order: the result equals the one-seed-at-a-time
+    restatement's, sparse (many chunks), medium and dense (few, long)."""
+    from oracle import bind
+    rng = np.random.default_rng(seed)
+    seeds = _diagonal_seeds(rng, n, span=span, diagonals=40)
+    assert len(set(seeds)) >= 65536
+    want = oracle_lib.link_seeds(seeds)
+    assert dcns_amd.link_seeds(seeds) == want
+    assert dcns_amd.link_seeds(seeds, insertion_order=True) == oracle_lib.link_seeds(seeds, insertion_order=True)
+    ref = bind.reference()
+    if ref is not None:
+        assert ref.link_seeds(seeds) == want
