@@ -879,7 +879,7 @@ int dcns_link_seeds(const int32_t *seeds, int64_t n, int32_t insertion_order, in
         std::vector<Seed> list = link_seed_list(set);
         if (!insertion_order) {
             set.clear();
-            for (const Seed &s : list) set.insert(s);
+            insert_in_start1_order(set, list);
             list.assign(set.begin(), set.end());
         }
         *n_out = (int64_t)list.size();

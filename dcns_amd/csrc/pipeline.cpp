@@ -955,10 +955,33 @@ std::vector<Seed> link_seed_list(const std::set<Seed> &seeds) {
     return cur;
 }
 
+// Inserting k into a std::set descends from the root, going left at every
+// node x with k < x.  Under Seed's comparator, k < x needs x.start1 >
+// k.start1, or equal start1 and x.start2 > k.start2.  When k's start1
+// exceeds every earlier seed's, or equals the largest and k's start2 exceeds
+// every earlier start2 of that start1, no node passes that test: the descent
+// ends at the rightmost node, which is exactly where the end() hint puts k
+// (and where both reject it when the rightmost node is equivalent to k).  So
+// the hint changes no result, only the cost, for lists in start1 order.
+void insert_in_start1_order(std::set<Seed> &seeds, const std::vector<Seed> &list) {
+    int32_t max1 = INT32_MIN, max2 = INT32_MIN;
+    for (const Seed &s : list) {
+        const int32_t a1 = s.getStart1(), a2 = s.getStart2();
+        if (a1 > max1 || (a1 == max1 && a2 > max2)) seeds.insert(seeds.end(), s);
+        else seeds.insert(s);
+        if (a1 > max1) {
+            max1 = a1;
+            max2 = a2;
+        } else if (a1 == max1) {
+            max2 = std::max(max2, a2);
+        }
+    }
+}
+
 void link_seeds(std::set<Seed> &seeds) {
     const std::vector<Seed> cur = link_seed_list(seeds);
     seeds.clear();
-    for (const Seed &s : cur) seeds.insert(s);
+    insert_in_start1_order(seeds, cur);
 }
 
 void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked, SeedResult &out) {

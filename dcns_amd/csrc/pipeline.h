@@ -43,6 +43,9 @@ void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked
 void link_seeds(std::set<Seed> &seeds);
 // link_seeds before its final insertion: the list in insertion order.
 std::vector<Seed> link_seed_list(const std::set<Seed> &seeds);
+// Inserts a list into a set one seed at a time, in list order, with the
+// same result as plain inserts; fast when the list is in start1 order.
+void insert_in_start1_order(std::set<Seed> &seeds, const std::vector<Seed> &list);
 
 // x_extend_seed with traceback for a list of tasks (seed start + 1 already applied).
 struct XResult {
