@@ -36,7 +36,7 @@ DEFAULT = {"level0_mode": 1, "level0_waves": 8, "wave_tasks": 1024, "wave_rows":
            "pool_min": 0, "xdrop_lane_rows": 512}
 SPACE = [("level0_mode", [1, 2]), ("level0_waves", [4, 8]), ("wave_tasks", [256, 1024, 2048]),
          ("wave_rows", [2048, 4096, 8192]), ("graph_levels", [-1, 4, 8]),
-         ("xdrop_lane_rows", [256, 512, 1024, -1])]
+         ("xdrop_lane_rows", [256, 512, 1024, 4096, -1])]
 
 
 def shape_key(len1, len2):

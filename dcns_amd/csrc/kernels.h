@@ -152,7 +152,7 @@ constexpr int kXdropLane2Band = 41;
 constexpr int kXdropWaveBand = 192;
 // Rows a lane kernel's pass may evaluate before its task moves to the wave
 // kernel (dcns_tuning.xdrop_lane_rows overrides it).
-constexpr int kXdropLaneRows = 512;
+constexpr int kXdropLaneRows = 4096;
 // Pass 1: forward score-only extension + reverse score-only extension, one
 // lane per task of in_list[0, n) (all tasks when null) at bands of up to
 // `band` cells (kXdropLaneBand or kXdropLane2Band); tasks whose band

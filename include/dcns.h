@@ -121,7 +121,7 @@ int dcns_last_stats(const dcns_ctx *ctx, dcns_stats *out);
  *   xdrop_lane_rows  x-drop extension: a pass on the one-task-per-lane
  *                 kernels that runs past this many rows moves its task to
  *                 the one-task-per-wave kernel, and so does a traceback pass
- *                 of more rows (default 512; -1: no limit).
+ *                 of more rows (default 4096; -1: no limit).
  *   k1_lanes      dcns_sw_random_max: 0 (default): the longest points' pairs
  *                 one per wave (query stripes across the lanes, int32) where
  *                 an issue-slot model shows that shortens the launch, the
