@@ -876,7 +876,7 @@ int dcns_link_seeds(const int32_t *seeds, int64_t n, int32_t insertion_order, in
         std::set<Seed> set;
         for (int64_t k = 0; k < n; ++k)
             set.insert(Seed(seeds[4 * k], seeds[4 * k + 1], seeds[4 * k + 2], seeds[4 * k + 3]));
-        std::vector<Seed> list = link_seed_list(set);
+        std::vector<Seed> list = link_seed_list(std::vector<Seed>(set.begin(), set.end()));
         if (!insertion_order) {
             set.clear();
             insert_in_start1_order(set, list);

@@ -940,8 +940,7 @@ constexpr size_t kLinkParallelMin = 65536;
 // last loop: seeds in start1 order (std::sort, as the reference), merge
 // passes until none merges; the list in the order the reference inserts it
 // back into the set.
-std::vector<Seed> link_seed_list(const std::set<Seed> &seeds) {
-    std::vector<Seed> cur(seeds.begin(), seeds.end());
+std::vector<Seed> link_seed_list(std::vector<Seed> cur) {
     std::sort(cur.begin(), cur.end(), [](Seed a, Seed b) { return a.getStart1() < b.getStart1(); });
     static const bool avx2 = __builtin_cpu_supports("avx2");
     if (avx2 && cur.size() >= kLinkParallelMin) return link_parallel(cur);
@@ -963,7 +962,8 @@ std::vector<Seed> link_seed_list(const std::set<Seed> &seeds) {
 // ends at the rightmost node, which is exactly where the end() hint puts k
 // (and where both reject it when the rightmost node is equivalent to k).  So
 // the hint changes no result, only the cost, for lists in start1 order.
-void insert_in_start1_order(std::set<Seed> &seeds, const std::vector<Seed> &list) {
+template <class Set>
+void insert_in_start1_order(Set &seeds, const std::vector<Seed> &list) {
     int32_t max1 = INT32_MIN, max2 = INT32_MIN;
     for (const Seed &s : list) {
         const int32_t a1 = s.getStart1(), a2 = s.getStart2();
@@ -978,10 +978,14 @@ void insert_in_start1_order(std::set<Seed> &seeds, const std::vector<Seed> &list
     }
 }
 
-void link_seeds(std::set<Seed> &seeds) {
-    const std::vector<Seed> cur = link_seed_list(seeds);
-    seeds.clear();
-    insert_in_start1_order(seeds, cur);
+template void insert_in_start1_order(std::set<Seed> &, const std::vector<Seed> &);
+template void insert_in_start1_order(SeedSet &, const std::vector<Seed> &);
+
+void link_seeds(SeedSet &seeds) {
+    const std::vector<Seed> cur = link_seed_list(std::vector<Seed>(seeds.begin(), seeds.end()));
+    SeedSet fresh;  // the old nodes go with the old set's blocks
+    insert_in_start1_order(fresh, cur);
+    seeds.swap(fresh);
 }
 
 void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked, SeedResult &out) {
@@ -1036,7 +1040,7 @@ void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked
                 }
             });
         }
-        std::set<Seed> seeds;
+        SeedSet seeds;
         int64_t cl = S.level0_cells[(size_t)p];
         for (int64_t q = 0; q < nh; ++q) {
             const Node &r = F.roots[hk[q]];

@@ -1,6 +1,12 @@
 // pipeline.h -- host orchestration of the device kernels (internal API).
 #pragma once
+#include <algorithm>
+#include <cstddef>
 #include <cstdint>
+#include <cstdlib>
+#include <memory>
+#include <new>
+#include <type_traits>
 #include <set>
 #include <vector>
 
@@ -39,13 +45,64 @@ struct SeedResult {
 };
 void seed_sets(dcns_ctx *c, const Staged &S, const Params &P, bool keep_unlinked, SeedResult &out);
 
+// Tree nodes of one seed set, carved from blocks that grow geometrically
+// (64 KiB up to 8 MiB) and are freed together with the set: a set of a
+// million seeds allocates ~20 blocks instead of a million nodes.  Only the
+// allocator differs from std::set<Seed>; the tree, its balancing and every
+// comparison are the same, so the contents and order are too.
+template <class T>
+struct NodeArena {
+    using value_type = T;
+    // a set takes its blocks along when swapped or moved
+    using propagate_on_container_swap = std::true_type;
+    using propagate_on_container_move_assignment = std::true_type;
+    struct Pool {
+        std::vector<char *> blocks;
+        size_t used = 0, cap = 0;
+        ~Pool() {
+            for (char *b : blocks) std::free(b);
+        }
+    };
+    std::shared_ptr<Pool> pool;
+    NodeArena() : pool(std::make_shared<Pool>()) {}
+    template <class U>
+    NodeArena(const NodeArena<U> &o) : pool(std::reinterpret_pointer_cast<Pool>(o.pool)) {}
+    T *allocate(size_t n) {
+        const size_t bytes = (n * sizeof(T) + alignof(std::max_align_t) - 1) & ~(alignof(std::max_align_t) - 1);
+        if (pool->blocks.empty() || pool->used + bytes > pool->cap) {
+            const size_t next = std::min<size_t>(std::max<size_t>(pool->cap * 2, 64 << 10), 8 << 20);
+            const size_t cap = std::max(next, bytes);
+            char *b = (char *)std::malloc(cap);
+            if (!b) throw std::bad_alloc();
+            pool->blocks.push_back(b);
+            pool->used = 0;
+            pool->cap = cap;
+        }
+        T *p = (T *)(pool->blocks.back() + pool->used);
+        pool->used += bytes;
+        return p;
+    }
+    void deallocate(T *, size_t) {}
+    template <class U>
+    bool operator==(const NodeArena<U> &o) const {
+        return (const void *)pool.get() == (const void *)o.pool.get();
+    }
+    template <class U>
+    bool operator!=(const NodeArena<U> &o) const {
+        return !(*this == o);
+    }
+};
+using SeedSet = std::set<Seed, std::less<Seed>, NodeArena<Seed>>;
+
 // link_seeds, findSimilarFragmentsForPairedSequence4.cpp:288-344.
-void link_seeds(std::set<Seed> &seeds);
-// link_seeds before its final insertion: the list in insertion order.
-std::vector<Seed> link_seed_list(const std::set<Seed> &seeds);
+void link_seeds(SeedSet &seeds);
+// link_seeds before its final insertion: the list in insertion order, from
+// the set's contents in set order.
+std::vector<Seed> link_seed_list(std::vector<Seed> cur);
 // Inserts a list into a set one seed at a time, in list order, with the
 // same result as plain inserts; fast when the list is in start1 order.
-void insert_in_start1_order(std::set<Seed> &seeds, const std::vector<Seed> &list);
+template <class Set>
+void insert_in_start1_order(Set &seeds, const std::vector<Seed> &list);
 
 // x_extend_seed with traceback for a list of tasks (seed start + 1 already applied).
 struct XResult {
