@@ -765,7 +765,7 @@ __attribute__((target("avx2"))) bool link_pass_avx2(const std::vector<Seed> &cur
 // One seed against one candidate, in the order of the reference's tests:
 // inside (absorbed), after on the same diagonal (merge, stop), beyond (stop).
 enum { kLinkNone = 0, kLinkInside = 1, kLinkMerge = 2, kLinkStop = 3 };
-inline int link_judge(int32_t a1, int32_t ae1, int32_t a2, int32_t ae2, int32_t b1, int32_t be1, int32_t b2,
+inline int link_decide(int32_t a1, int32_t ae1, int32_t a2, int32_t ae2, int32_t b1, int32_t be1, int32_t b2,
                       int32_t be2) {
     if (a2 <= b2 && ae1 >= be1 && ae2 >= be2) return kLinkInside;
     if (a2 <= b2 && ae1 <= be1 && ae2 <= be2) {
@@ -843,7 +843,7 @@ __attribute__((target("avx2"))) bool link_chunk(LinkArrays &L, const LinkArrays 
             const bool own = j < e;
             if (own ? L.dead[(size_t)j] != 0 : (P.dead[(size_t)j] != 0 || killed_past(j))) continue;
             const LinkArrays &A = own ? L : P;
-            const int r = link_judge(a1, ae1, a2, ae2, A.S1[(size_t)j], A.E1[(size_t)j], A.S2[(size_t)j], A.E2[(size_t)j]);
+            const int r = link_decide(a1, ae1, a2, ae2, A.S1[(size_t)j], A.E1[(size_t)j], A.S2[(size_t)j], A.E2[(size_t)j]);
             if (r == kLinkInside) {
                 if (own) L.dead[(size_t)j] = 1;
                 else kills.push_back(j);
